@@ -1,0 +1,135 @@
+/* The C ABI of the MI355X semi-discrete optimal transport library ( libsdot_amd.so ).
+ *
+ * It replaces the FFI calls that the reference `sdot` makes on its GPU path for one damped-Newton
+ * solve and the diagrams it is made of. Each entry point names the reference call it stands for
+ * ( paths relative to the reference's repository root ). The calling convention is the same for all:
+ *
+ *   - `stream` is a `hipStream_t` ( null: the default stream ); work is queued on it, and the call
+ *     returns once its results are in place;
+ *   - arrays marked DEVICE are device pointers ( `hipMalloc`, or a torch tensor's `data_ptr()` ),
+ *     arrays marked HOST are host pointers;
+ *   - 2D positions are `[ n ][ 2 ]` float64, C order, in the USER's order ( the tree's permutation is
+ *     internal ); weights and masses are `[ n ]` float64 in the user's order;
+ *   - the return value is 0 on success, otherwise the error kind: 1 invalid argument / HIP error,
+ *     2 kernel failure ( `code` 1: a cell past the vertex limit, `value` its seed ), 3 capacity,
+ *     4 device memory; `sdot_amd_last_error` gives the message and the details of the calling thread's
+ *     last failure.
+ */
+#ifndef SDOT_AMD_H
+#define SDOT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDOT_AMD_ABI_VERSION 1
+
+/* the density: SDOT_AMD_CONSTANT ( `rho` on the domain box ) or SDOT_AMD_IMAGE ( piecewise constant on
+ * `nx * ny` pixels, `img[ j * nx + i ]` the value of pixel `( i, j )` covering
+ * `[ x0 + i hx, x0 + ( i + 1 ) hx ] x [ y0 + j hy, y0 + ( j + 1 ) hy ]` ) */
+#define SDOT_AMD_CONSTANT 0
+#define SDOT_AMD_IMAGE    1
+
+int         sdot_amd_abi_version( void );
+
+/* the message of the calling thread's last failure, and its kind / code / value */
+const char *sdot_amd_last_error( int *kind, int *code, long long *value );
+
+/* the bytes of device memory the library holds ( its grow-only arena ), and their release */
+long long   sdot_amd_device_bytes( void );
+void        sdot_amd_release( void );
+
+/* ONE DIAGRAM: the masses of the power cells of `( positions, weights )` in the box
+ * `box = { lo_x, lo_y, hi_x, hi_y }` ( HOST ) against a density, and optionally the facets
+ * `c_ij = int_facet rho / ( 2 |p_i - p_j| )` as triplets ( rows in user order, each row in its cell's
+ * edge order ).
+ * Replaces `power_diagram_measures_card_2d` -> `sdot::gpu2d::measures`
+ * ( src/sdot/include/sdot/gpu/Cell2D.cuh:1801-1811, called from src/sdot/PowerDiagram_Bsp.py:271-311 ).
+ *   positions    DEVICE [ n ][ 2 ]
+ *   weights      DEVICE [ n ]       ( null: all zero, a Voronoi diagram )
+ *   img          DEVICE [ ny * nx ] ( SDOT_AMD_IMAGE only ); img_geo HOST { x0, y0, hx, hy }
+ *   masses       DEVICE [ n ]       ( output )
+ *   facet_i/j/c  DEVICE [ facet_cap ] ( outputs; null: no facets )
+ *   nb_facets    HOST   [ 1 ]       ( output, may be null )
+ *   events       HOST   [ 5 ]       ( may be null: side tests, pruning vertex tests, new vertices, node visits, cuts )
+ */
+int sdot_amd_measures_2d( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                          int density_kind, double rho, const double *img, int nx, int ny, const double *img_geo,
+                          double *masses, long long *facet_i, long long *facet_j, double *facet_c, long long facet_cap,
+                          long long *nb_facets, unsigned long long *events );
+
+/* THE SOLVE: weights `w` such that every cell's mass matches its target `nu_i`, by the damped Newton of
+ * Kitagawa-Merigot-Thibert ( trials step ), FP64 cells, Jacobi-PCG directions.
+ * Replaces `sdotplan_solve_card_2d` / `_image` -> `sdot::gpu2d::solve`
+ * ( src/sdot/include/sdot/gpu/Newton2D.cuh:846-850, called from src/sdot/SdotPlanNd.py:864-890 ).
+ *   positions    DEVICE [ n ][ 2 ]
+ *   nu           DEVICE [ n ]       target masses ( rescaled to the mass of the domain )
+ *   w0           DEVICE [ n ]       starting weights ( null: zero, the Voronoi diagram )
+ *   dups         DEVICE [ 2 k ]     exact duplicates ( dup, rep ); accepted, not used by this version
+ *   options      HOST   [ 41 ]      the reference card's option vector, same order
+ *                                   ( src/sdot/SdotPlanNd.py:197-201 ); the image geometry is in
+ *                                   img_x0 .. img_ny as there
+ *   box          HOST   [ 4 ]       the domain { lo_x, lo_y, hi_x, hi_y }
+ *   img          DEVICE [ ny * nx ] ( SDOT_AMD_IMAGE only )
+ *   weights      DEVICE [ n ]       output, `weights[ 0 ] == 0` ( the gauge )
+ *   history      HOST   [ ( maxit + 1 ) * 8 ] output rows ( src/sdot/include/sdot/sdotplan/Report.h:25 )
+ *   nb_steps     HOST   [ 1 ]       output, rows written
+ *   stats        HOST   [ 31 ]      output ( src/sdot/include/sdot/sdotplan/Report.h:15-21 order )
+ *   cell_masses  DEVICE [ n ]       output
+ *   barycenters  DEVICE [ n ][ 2 ]  output ( the seed for an empty cell )
+ *   cost         HOST   [ 1 ]       output, sum_i int_{cell_i} |x - p_i|^2 rho
+ *   clusters     DEVICE [ n ] int64 output ( each seed its own representative; may be null )
+ */
+int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const double *nu, const double *w0,
+                       const long long *dups, long long nb_dup_values, const double *options, const double *box,
+                       int density_kind, double rho, const double *img,
+                       double *weights, double *history, long long *nb_steps, double *stats,
+                       double *cell_masses, double *barycenters, double *cost, long long *clusters );
+
+/* THE TREE: the BSP tree of a 2D cloud ( median cut on the longest axis, at most `leaf` seeds per leaf,
+ * perfect binary tree of depth ceil( log2( n / leaf ) ) + 1 in preorder ).
+ * Replaces `bsp_build_card` -> `sdot::gpu2d::build_tree`
+ * ( src/sdot/include/sdot/gpu/Bsp2D.cuh:367, called from src/sdot/AaBsp.py:733-765 ).
+ *   positions     DEVICE [ n ][ 2 ]
+ *   seed_indices  DEVICE [ n ] int64   rank -> user index
+ *   rank_of       DEVICE [ n ] int64   user index -> rank
+ *   node_begin / node_end / node_left / node_right  DEVICE [ N ] int64 ( -1: no child )
+ *   node_box      DEVICE [ N ][ 2 ][ 2 ] ( lo then hi )
+ * N = 2^depth - 1, see `sdot_amd_tree_nodes`.
+ */
+long long sdot_amd_tree_nodes( long long n, int leaf );
+int sdot_amd_build_tree_2d( void *stream, long long n, const double *positions, int leaf,
+                            long long *seed_indices, long long *rank_of, long long *node_begin, long long *node_end,
+                            long long *node_left, long long *node_right, double *node_box );
+
+/* THE MAJORANTS: for every node of a tree built as above, `( wa, wb )` with `w_k <= wa . p_k + wb` on
+ * its seeds ( affine least-squares fit kept when it tightens the spread, else constant ).
+ * Replaces `bsp_refresh_majorants_card_2d` -> `sdot::gpu2d::refresh_majorants`
+ * ( src/sdot/include/sdot/gpu/Majorant2D.cuh:276, called from src/sdot/AaBsp.py:356-372 ).
+ *   node_box / node_begin / node_end  DEVICE, as built
+ *   sorted_positions DEVICE [ n ][ 2 ], sorted_weights DEVICE [ n ] ( in rank order )
+ *   node_wa DEVICE [ N ][ 2 ], node_wb DEVICE [ N ] ( outputs )
+ */
+int sdot_amd_refresh_majorants_2d( void *stream, long long n, long long nb_nodes, const double *node_box,
+                                   const long long *node_begin, const long long *node_end,
+                                   const double *sorted_positions, const double *sorted_weights,
+                                   double *node_wa, double *node_wb );
+
+/* THE LINEAR SOLVER ALONE: `L d_j = rhs_j` for `k` right-hand sides of zero sum, `L` the Laplacian
+ * given as CSR of its off-diagonal entries ( `val > 0` are `c_ij`, the matrix holds `-c_ij` ) plus its
+ * diagonal; the solution in the gauge `d[ 0 ] = 0`.
+ * Replaces `sdotplan_card_linear_2d` -> `sdot::gpu2d::linear_solves`
+ * ( src/sdot/include/sdot/gpu/Newton2D.cuh:1916, called from src/sdot/SdotPlanNd.py:204-236 ).
+ *   row DEVICE [ n + 1 ] int64, col DEVICE [ nnz ] int64, val DEVICE [ nnz ], dia DEVICE [ n ]
+ *   rhs DEVICE [ k ][ n ], options HOST [ 10 ] ( n, k, method, tol, ... as the reference; tol used )
+ *   solution DEVICE [ k ][ n ] ( output ), iterations HOST [ k ] ( output, -1: failed )
+ */
+int sdot_amd_linear_solves_2d( void *stream, long long n, const long long *row, const long long *col, const double *val,
+                               const double *dia, const double *rhs, long long k, const double *options,
+                               double *solution, double *iterations );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,185 @@
+"""Python access to the oracle ( `oracle/sdot_oracle.cpp` ) -- TEST INFRASTRUCTURE.
+
+Only the tests, `__graft_entry__.smoke()` and the CPU-baseline leg of `bench.py` load this module;
+the product ( `sdot-ffi_amd/` ) never does. Every function takes and returns numpy arrays, in the
+reference's user order ( `positions[ n, 2 ]`, `weights[ n ]` ), and image values as
+`img[ ny, nx ]` ( row `j` = pixel row along y ).
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname( os.path.abspath( __file__ ) )
+_LIB_PATH = os.path.join( _HERE, "_build", "liboracle.so" )
+_lib = None
+
+_D = ctypes.POINTER( ctypes.c_double )
+_L = ctypes.POINTER( ctypes.c_longlong )
+
+
+def build():
+    """compiles the oracle ( `make -C oracle` )"""
+    subprocess.run( [ "make", "-s", "-C", _HERE ], check = True )
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists( _LIB_PATH ):
+            build()
+        _lib = ctypes.CDLL( _LIB_PATH )
+        _lib.oracle_build_tree.restype = ctypes.c_int
+        _lib.oracle_measures.restype = ctypes.c_longlong
+        _lib.oracle_cell.restype = ctypes.c_int
+        _lib.oracle_cg.restype = ctypes.c_int
+        _lib.oracle_solve.restype = ctypes.c_int
+        _lib.oracle_max_depth_for.restype = ctypes.c_int
+        _lib.oracle_get_threads.restype = ctypes.c_int
+    return _lib
+
+
+def _d( a ):
+    return None if a is None else a.ctypes.data_as( _D )
+
+
+def _l( a ):
+    return None if a is None else a.ctypes.data_as( _L )
+
+
+def _f64( a ):
+    return np.ascontiguousarray( np.asarray( a, dtype = np.float64 ) )
+
+
+def set_threads( nt ):
+    lib().oracle_set_threads( int( nt ) )
+
+
+def get_threads():
+    return int( lib().oracle_get_threads() )
+
+
+def max_depth_for( n, leaf = 10 ):
+    return int( lib().oracle_max_depth_for( ctypes.c_longlong( n ), ctypes.c_longlong( leaf ) ) )
+
+
+def build_tree( pos, w = None, leaf = 10 ):
+    """the tree: `dict( order, beg, end, box[ N, 2, 2 ], wa[ N, 2 ], wb[ N ], depth )`, preorder"""
+    pos = _f64( pos )
+    n = pos.shape[ 0 ]
+    depth = max_depth_for( n, leaf )
+    N = 2 ** depth - 1
+    order = np.zeros( n, np.int64 )
+    beg = np.zeros( N, np.int64 )
+    end = np.zeros( N, np.int64 )
+    box = np.zeros( ( N, 2, 2 ) )
+    wa = np.zeros( ( N, 2 ) )
+    wb = np.zeros( N )
+    ww = None if w is None else _f64( w )
+    lib().oracle_build_tree( ctypes.c_longlong( n ), _d( pos ), _d( ww ), ctypes.c_longlong( leaf ), _l( order ), _l( beg ), _l( end ),
+                             _d( box ), _d( wa ), _d( wb ) )
+    return dict( order = order, beg = beg, end = end, box = box, wa = wa, wb = wb, depth = depth )
+
+
+def weight_majorant( pos, w ):
+    pos = _f64( pos )
+    w = _f64( w )
+    wa = np.zeros( 2 )
+    wb = np.zeros( 1 )
+    lib().oracle_weight_majorant( ctypes.c_longlong( len( w ) ), _d( pos ), _d( w ), _d( wa ), _d( wb ) )
+    return wa, float( wb[ 0 ] )
+
+
+def _density_args( density ):
+    """`density`: `None` / a float ( constant ), or `dict( values = img[ ny, nx ], x0, y0, hx, hy )`"""
+    if density is None or np.isscalar( density ):
+        return 0, 1.0 if density is None else float( density ), 0, 0, None, None, None
+    img = _f64( density[ "values" ] )
+    geo = _f64( [ density[ "x0" ], density[ "y0" ], density[ "hx" ], density[ "hy" ] ] )
+    return 1, 0.0, img.shape[ 1 ], img.shape[ 0 ], geo, img, ( img, geo )
+
+
+def measures( pos, w = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = None, facets = False, brute_force = False, events = False ):
+    """masses `[ n ]` ( and if asked the facets `( fi, fj, fc )` in row order, and the event counts )"""
+    pos = _f64( pos )
+    n = pos.shape[ 0 ]
+    ww = None if w is None else _f64( w )
+    lo = _f64( lo )
+    hi = _f64( hi )
+    kind, value, nx, ny, geo, img, keep = _density_args( density )
+    masses = np.zeros( n )
+    cap = 8 * n + 64 if facets else 0
+    fi = np.zeros( cap, np.int64 ) if facets else None
+    fj = np.zeros( cap, np.int64 ) if facets else None
+    fc = np.zeros( cap ) if facets else None
+    ev = np.zeros( 5, np.int64 )
+    nf = lib().oracle_measures( ctypes.c_longlong( n ), _d( pos ), _d( ww ), _d( lo ), _d( hi ), ctypes.c_int( kind ), ctypes.c_double( value ),
+                                ctypes.c_int( nx ), ctypes.c_int( ny ), _d( geo ), _d( img ), _d( masses ), _l( fi ), _l( fj ), _d( fc ),
+                                ctypes.c_longlong( cap ), _l( ev ), ctypes.c_int( int( brute_force ) ) )
+    res = [ masses ]
+    if facets:
+        if nf < 0:
+            raise RuntimeError( "oracle: facet capacity exceeded" )
+        res.append( ( fi[ :nf ], fj[ :nf ], fc[ :nf ] ) )
+    if events:
+        res.append( dict( zip( ( "side_tests", "prune_vertex_tests", "new_vertices", "node_visits", "cuts" ), ev.tolist() ) ) )
+    return res[ 0 ] if len( res ) == 1 else tuple( res )
+
+
+def cell( pos, i, w = None, lo = ( 0, 0 ), hi = ( 1, 1 ), brute_force = False, cap = 4096 ):
+    """the vertices `[ nv, 2 ]` of the cell of seed `i` and the ids of its edges ( -1: the domain )"""
+    pos = _f64( pos )
+    n = pos.shape[ 0 ]
+    ww = None if w is None else _f64( w )
+    xy = np.zeros( ( cap, 2 ) )
+    ids = np.zeros( cap, np.int64 )
+    nv = lib().oracle_cell( ctypes.c_longlong( n ), _d( pos ), _d( ww ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_longlong( i ),
+                            _d( xy ), _l( ids ), ctypes.c_int( cap ), ctypes.c_int( int( brute_force ) ) )
+    if nv < 0:
+        raise RuntimeError( "oracle: cell capacity exceeded" )
+    return xy[ :nv ].copy(), ids[ :nv ].copy()
+
+
+def cg( n, fi, fj, fc, b, tol = 1e-12, maxit = 20000 ):
+    """Jacobi-PCG on the Laplacian of the facets; returns `( d, iterations )`"""
+    fi = np.ascontiguousarray( fi, dtype = np.int64 )
+    fj = np.ascontiguousarray( fj, dtype = np.int64 )
+    fc = _f64( fc )
+    b = _f64( b )
+    d = np.zeros( n )
+    its = lib().oracle_cg( ctypes.c_longlong( n ), ctypes.c_longlong( len( fi ) ), _l( fi ), _l( fj ), _d( fc ), _d( b ), ctypes.c_double( tol ),
+                           ctypes.c_int( maxit ), _d( d ) )
+    return d, its
+
+
+STATS = [ "status", "residual", "residual0", "nb_iter", "nb_diag", "nb_backtracks", "t_majorant", "t_diag", "t_asm", "t_lin", "t_lim", "eps",
+          "domain_mass", "nb_overflowed", "nb_cell_lim", "nb_limit_rounds", "lin_nb_hierarchies", "lin_nb_iter", "lin_worst", "start", "t_total",
+          "nb_continuation_steps", "min_start_mass", "it_switch", "it_double", "scratch_bytes",
+          "nb_clusters", "nb_aggregated", "nb_duplicates", "residual_full", "nb_polish" ]
+HISTORY = [ "step", "t", "residual_l2", "min_measure", "max_abs_residual", "nb_diag", "nb_evals", "s" ]
+
+
+def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = None, tol = 1e-9, tol_rel = 0.0, maxit = 100, max_backtracks = 60,
+           t_min = 1e-10, mult_ok = 4.0, residual = 1, power = 0.5, switch = 2.0, lin_tol = 1e-12, lin_maxit = 20000, keep_start = False,
+           start_box = None ):
+    """the damped-Newton solve ( trials ); returns a dict ( weights, masses, barycenters, cost, stats, history )"""
+    pos = _f64( pos )
+    n = pos.shape[ 0 ]
+    nu = np.full( n, 1.0 / n ) if nu is None else _f64( nu )
+    w0 = None if w0 is None else _f64( w0 )
+    kind, value, nx, ny, geo, img, keep = _density_args( density )
+    opts = _f64( [ tol, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol, lin_maxit, int( keep_start ) ] )
+    sb = _f64( [ 0, 0, 0, 0, 0 ] if start_box is None else [ 1, *start_box[ 0 ], *start_box[ 1 ] ] )
+    weights = np.zeros( n )
+    stats = np.zeros( len( STATS ) )
+    hist = np.zeros( ( maxit + 1, len( HISTORY ) ) )
+    masses = np.zeros( n )
+    bary = np.zeros( ( n, 2 ) )
+    cost = np.zeros( 1 )
+    nb = lib().oracle_solve( ctypes.c_longlong( n ), _d( pos ), _d( nu ), _d( w0 ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
+                             ctypes.c_double( value ), ctypes.c_int( nx ), ctypes.c_int( ny ), _d( geo ), _d( img ), _d( opts ), _d( sb ),
+                             _d( weights ), _d( stats ), _d( hist ), _d( masses ), _d( bary ), _d( cost ) )
+    return dict( weights = weights, masses = masses, barycenters = bary, cost = float( cost[ 0 ] ),
+                 stats = dict( zip( STATS, stats.tolist() ) ), history = hist[ :nb ].copy() )

@@ -1,0 +1,1232 @@
+// ============================================================================================
+// THE ORACLE -- test infrastructure only.
+//
+// A plain FP64 C++ restatement of the reference's CPU semi-discrete OT solver for the 2D path,
+// used as the checker of the HIP library: the tests, `__graft_entry__.smoke()` and the CPU
+// baseline leg of `bench.py` may load it, nothing in the product does.
+//
+// What it restates ( reference file:line, paths under `src/sdot/include/sdot/` unless noted ):
+//   * the BSP tree: median cut on the longest axis, leaf <= 10, perfect tree, heap -> preorder
+//       `bsp_build_level.h:43` ( quickselect ), `:230` ( one level ), `src/sdot/AaBsp.py:264`
+//       ( depth ), `src/sdot/AaBsp.py:545` ( levels, children slices ), `src/sdot/AaBsp.py:514`
+//       ( preorder numbering ).
+//   * the affine weight majorant of a node: `bsp_build_level.h:84-209`.
+//   * the walk: `cell/Providers.h:167` ( nearest child first, prune at pop time ), the pruning
+//       test `cell/Pruning.h:46` ( exact per-axis clamp ), the bisector `cell/Plane.h:39`.
+//   * the in-place convex polygon cut with the several-runs rule: `cell/Local2.h:206`, `:300`.
+//   * densities: constant, and `Image` cut into pixel pieces `Image.cxx:106-232`.
+//   * one cell's mass and facets `sdotplan/Sweep.h:64-128`, the Laplacian `sdotplan/Laplacian.h:40`.
+//   * the Jacobi-preconditioned CG `sdotplan/Linear.cpp:44-88`.
+//   * the damped Newton ( trials ), LOG -> LIN residual switch, HopelessDamping
+//       `sdotplan/Newton.h:243-420`, `sdotplan/Report.h:37-57`.
+//   * the start selection, target rescaling, moments and the stats / history layout
+//       `sdotplan/Solve.h:112-371`, `sdotplan/Report.h:15-31`.
+//
+// Geometry runs in the SEED'S FRAME ( a vertex is stored as `x - p0` ): the half-space of the
+// bisector of ( p0, w0 ) and ( q, wq ) is then `d . v <= ( |d|^2 + w0 - wq ) / 2`, `d = q - p0`,
+// which is the reference's `Plane.h:39` half-space translated by `p0` ( the reference card does
+// the same, `gpu/Cell2D.cuh:139-160` ). The HIP library uses the same frame, the same walk and the
+// same cut, so its cells agree with these to rounding.
+//
+// Parity pins ( the tests that keep this restatement honest, all in `tests/` ):
+//   tree           shape = f( n ) only, every seed once, boxes contain their subtree, majorant
+//                  majorates ( reference tests test_PowerDiagram.py:773,801,835,870 )
+//   cells          sum of masses = domain mass, one seed takes all, grid seeds give equal cells,
+//                  weight shift invariance ( test_PowerDiagram.py:46,58,217,304 ); neighbour sets
+//                  and areas against scipy's lifted convex hull / half-space intersection;
+//                  brute-force O( n^2 ) cells
+//   image          constant image == Lebesgue, one-pixel closed forms, exact polygon/pixel clipping
+//   facets         Hessian == finite differences of the masses, zero row sums
+//                  ( test_SdotPlanNd.py:184-243 )
+//   CG             residual and dense least squares ( test_SdotPlanNd.py:680-727 )
+//   Newton         max |m - nu| <= tol at the end, quadratic tail, warm start <= 6 steps
+//                  ( test_SdotPlanNd.py:39,245-276 ), similarity start
+// ============================================================================================
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+using I64 = long long;
+
+static inline size_t sz( I64 v ) { return size_t( v ); }
+
+static double now() {
+    using namespace std::chrono;
+    return duration<double>( steady_clock::now().time_since_epoch() ).count();
+}
+
+// ---- densities -------------------------------------------------------------------------------
+
+/// kind 0: constant `value` on the box; kind 1: an image, `img[ j * nx + i ]` the value of pixel
+/// `( i, j )`, pixel `( i, j )` covering `[ x0 + i hx, x0 + ( i + 1 ) hx ] x [ y0 + j hy, ... ]`
+struct Density {
+    int           kind  = 0;
+    double        value = 1;
+    int           nx = 0, ny = 0;
+    double        x0 = 0, y0 = 0, hx = 1, hy = 1;
+    const double *img = nullptr;
+};
+
+// ---- the tree --------------------------------------------------------------------------------
+
+struct Tree {
+    I64                 n = 0, nb_nodes = 0;
+    int                 depth = 0;
+    std::vector<I64>    order;           ///< rank -> user index ( `seed_indices` )
+    std::vector<I64>    rank_of;         ///< user index -> rank
+    std::vector<I64>    beg, end;        ///< per node, preorder
+    std::vector<double> box;             ///< [ N ][ lo/hi ][ 2 ]
+    std::vector<double> wa, wb;          ///< [ N ][ 2 ], [ N ]
+    std::vector<double> px, py;          ///< positions in rank order
+    std::vector<double> w;               ///< weights in rank order
+};
+
+/// `AaBsp.py:264`: `1 if n <= leaf else ceil( log2( n / leaf ) ) + 1`
+int max_depth_for( I64 n, I64 leaf ) {
+    if ( leaf < 1 ) leaf = 1;
+    if ( n <= leaf ) return 1;
+    return int( std::ceil( std::log2( double( n ) / double( leaf ) ) ) ) + 1;
+}
+
+/// Hoare quickselect with a median-of-three pivot that is a value of the slice
+/// ( `bsp_build_level.h:43-83` ): `perm[ b .. e )` rearranged so that rank `t - b` sits at `t`
+static void select_median( std::vector<I64> &perm, const std::vector<double> &key_of, I64 b, I64 e, I64 t ) {
+    auto key = [&]( I64 k ) { return key_of[ size_t( perm[ size_t( k ) ] ) ]; };
+    I64 lo = b, hi = e;
+    while ( hi - lo > 2 ) {
+        const I64 c = lo + ( hi - lo ) / 2;
+        const double k0 = key( lo ), k1 = key( c ), k2 = key( hi - 1 );
+        const double pivot = k0 < k1 ? ( k1 < k2 ? k1 : ( k0 < k2 ? k2 : k0 ) )
+                                     : ( k0 < k2 ? k0 : ( k1 < k2 ? k2 : k1 ) );
+        I64 i = lo - 1, j = hi;
+        for ( ;; ) {
+            do { ++i; } while ( key( i ) < pivot );
+            do { --j; } while ( key( j ) > pivot );
+            if ( i >= j ) break;
+            std::swap( perm[ size_t( i ) ], perm[ size_t( j ) ] );
+        }
+        if ( t <= j ) hi = j + 1;
+        else          lo = j + 1;
+    }
+    if ( hi - lo == 2 && key( lo ) > key( lo + 1 ) )
+        std::swap( perm[ size_t( lo ) ], perm[ size_t( lo + 1 ) ] );
+}
+
+/// the affine majorant `w <= a . y + b` of the slice `[ b, e )` of rank-ordered seeds
+/// ( `bsp_build_level.h:84-209` ): least-squares slope kept only when it clearly tightens the
+/// spread of the residuals and stays well behaved, `b` raised to majorize plus a relative margin
+static void weight_majorant( const double *px, const double *py, const double *w, I64 b, I64 e, double *wa, double &wb ) {
+    const I64 m = e - b;
+    double wmin = w[ b ], wmax = w[ b ], wsum = 0, psx = 0, psy = 0;
+    double plo[ 2 ] = { px[ b ], py[ b ] }, phi[ 2 ] = { px[ b ], py[ b ] };
+    for ( I64 k = b; k < e; ++k ) {
+        wmin = std::min( wmin, w[ k ] );
+        wmax = std::max( wmax, w[ k ] );
+        wsum += w[ k ];
+        psx += px[ k ];
+        psy += py[ k ];
+        plo[ 0 ] = std::min( plo[ 0 ], px[ k ] ); phi[ 0 ] = std::max( phi[ 0 ], px[ k ] );
+        plo[ 1 ] = std::min( plo[ 1 ], py[ k ] ); phi[ 1 ] = std::max( phi[ 1 ], py[ k ] );
+    }
+    const double spread = wmax - wmin;
+    double a[ 2 ] = { 0, 0 };
+    if ( m >= 2 * ( 2 + 1 ) && spread > 0 ) {
+        const double inv = 1.0 / double( m );
+        const double pm[ 2 ] = { psx * inv, psy * inv }, wm = wsum * inv;
+        double A[ 2 ][ 3 ] = { { 0, 0, 0 }, { 0, 0, 0 } };
+        for ( I64 k = b; k < e; ++k ) {
+            const double dw = w[ k ] - wm;
+            const double q[ 2 ] = { px[ k ] - pm[ 0 ], py[ k ] - pm[ 1 ] };
+            for ( int i = 0; i < 2; ++i ) {
+                for ( int j = 0; j < 2; ++j )
+                    A[ i ][ j ] += q[ i ] * q[ j ];
+                A[ i ][ 2 ] += q[ i ] * dw;
+            }
+        }
+        bool ok = true;
+        for ( int c = 0; c < 2 && ok; ++c ) {
+            int p = c;
+            for ( int i = c + 1; i < 2; ++i )
+                if ( std::fabs( A[ i ][ c ] ) > std::fabs( A[ p ][ c ] ) ) p = i;
+            if ( ! ( std::fabs( A[ p ][ c ] ) > 0 ) ) { ok = false; break; }
+            if ( p != c )
+                for ( int j = c; j <= 2; ++j ) std::swap( A[ c ][ j ], A[ p ][ j ] );
+            for ( int i = c + 1; i < 2; ++i ) {
+                const double f = A[ i ][ c ] / A[ c ][ c ];
+                for ( int j = c; j <= 2; ++j ) A[ i ][ j ] -= f * A[ c ][ j ];
+            }
+        }
+        if ( ok ) {
+            double fit[ 2 ];
+            for ( int i = 1; i >= 0; --i ) {
+                double s = A[ i ][ 2 ];
+                for ( int j = i + 1; j < 2; ++j ) s -= A[ i ][ j ] * fit[ j ];
+                fit[ i ] = s / A[ i ][ i ];
+            }
+            double rmin = 0, rmax = 0;
+            for ( I64 k = b; k < e; ++k ) {
+                const double r = w[ k ] - fit[ 0 ] * px[ k ] - fit[ 1 ] * py[ k ];
+                if ( k == b ) { rmin = r; rmax = r; }
+                else { rmin = std::min( rmin, r ); rmax = std::max( rmax, r ); }
+            }
+            const double u = 1.0 - 2.0 / double( m - 1 );
+            const double by_chance = std::sqrt( u > 0 ? u : 0.0 );
+            bool well_behaved = true;
+            for ( int d = 0; d < 2; ++d ) {
+                const double reach = std::max( std::fabs( plo[ d ] ), std::fabs( phi[ d ] ) );
+                if ( std::fabs( fit[ d ] ) * ( phi[ d ] - plo[ d ] ) > 8 * spread || std::fabs( fit[ d ] ) * reach > 100 * spread )
+                    well_behaved = false;
+            }
+            if ( well_behaved && rmax - rmin < 0.85 * by_chance * spread ) { a[ 0 ] = fit[ 0 ]; a[ 1 ] = fit[ 1 ]; }
+        }
+    }
+    double bb = 0, amax = 0;
+    for ( I64 k = b; k < e; ++k ) {
+        const double ay = a[ 0 ] * px[ k ] + a[ 1 ] * py[ k ];
+        const double v = w[ k ] - ay;
+        bb = k == b ? v : std::max( bb, v );
+        amax = std::max( amax, std::fabs( ay ) );
+    }
+    wa[ 0 ] = a[ 0 ];
+    wa[ 1 ] = a[ 1 ];
+    wb = bb + 1e-6 * ( std::fabs( bb ) + spread + amax );
+}
+
+/// `AaBsp.py:514-532`: where the node of heap rank `i` lands in preorder
+static std::vector<I64> preorder_of_heap( int depth ) {
+    const I64 nb = ( I64( 1 ) << depth ) - 1;
+    std::vector<I64> pre( sz( nb ), 0 );
+    for ( int level = 0; level + 1 < depth; ++level ) {
+        const int h = depth - level;
+        for ( I64 idx = ( I64( 1 ) << level ) - 1; idx < ( I64( 1 ) << ( level + 1 ) ) - 1; ++idx ) {
+            pre[ size_t( 2 * idx + 1 ) ] = pre[ size_t( idx ) ] + 1;
+            pre[ size_t( 2 * idx + 2 ) ] = pre[ size_t( idx ) ] + ( I64( 1 ) << ( h - 1 ) );
+        }
+    }
+    return pre;
+}
+
+/// the tree on `pos[ n ][ 2 ]`, level by level ( `AaBsp.py:545-680`, `bsp_build_level.h:230-318` )
+void build_tree( Tree &t, I64 n, const double *pos, I64 leaf ) {
+    const int depth = max_depth_for( n, leaf );
+    const I64 nb_nodes = ( I64( 1 ) << depth ) - 1;
+    t.n = n;
+    t.depth = depth;
+    t.nb_nodes = nb_nodes;
+    std::vector<double> X( sz( n ) ), Y( sz( n ) ), X2( sz( n ) ), Y2( sz( n ) );
+    std::vector<I64> ord( sz( n ) ), ord2( sz( n ) ), perm( sz( n ) );
+    for ( I64 i = 0; i < n; ++i ) { X[ size_t( i ) ] = pos[ 2 * i ]; Y[ size_t( i ) ] = pos[ 2 * i + 1 ]; ord[ size_t( i ) ] = i; }
+    std::vector<I64> hbeg( sz( nb_nodes ) ), hend( sz( nb_nodes ) );
+    std::vector<double> hbox( sz( nb_nodes ) * 4, 0.0 );
+    std::vector<I64> beg{ 0 }, end{ n };
+    for ( int level = 0; level < depth; ++level ) {
+        const I64 first = ( I64( 1 ) << level ) - 1;
+        std::vector<I64> mid( end );
+        for ( size_t j = 0; j < beg.size(); ++j ) {
+            const I64 b = beg[ j ], e = end[ j ], node = first + I64( j );
+            hbeg[ size_t( node ) ] = b;
+            hend[ size_t( node ) ] = e;
+            if ( e <= b ) continue;
+            double lo[ 2 ] = { X[ size_t( b ) ], Y[ size_t( b ) ] }, hi[ 2 ] = { lo[ 0 ], lo[ 1 ] };
+            for ( I64 k = b + 1; k < e; ++k ) {
+                lo[ 0 ] = std::min( lo[ 0 ], X[ size_t( k ) ] ); hi[ 0 ] = std::max( hi[ 0 ], X[ size_t( k ) ] );
+                lo[ 1 ] = std::min( lo[ 1 ], Y[ size_t( k ) ] ); hi[ 1 ] = std::max( hi[ 1 ], Y[ size_t( k ) ] );
+            }
+            double *bx = &hbox[ size_t( node ) * 4 ];
+            bx[ 0 ] = lo[ 0 ]; bx[ 1 ] = lo[ 1 ]; bx[ 2 ] = hi[ 0 ]; bx[ 3 ] = hi[ 1 ];
+            const int ax = ( hi[ 1 ] - lo[ 1 ] > hi[ 0 ] - lo[ 0 ] ) ? 1 : 0;
+            for ( I64 k = b; k < e; ++k ) perm[ size_t( k ) ] = k;
+            I64 m = e;
+            if ( e - b > leaf && hi[ ax ] > lo[ ax ] ) {
+                m = b + ( e - b ) / 2;
+                select_median( perm, ax ? Y : X, b, e, m );
+            }
+            mid[ j ] = m;
+            for ( I64 k = b; k < e; ++k ) {
+                const I64 s = perm[ size_t( k ) ];
+                X2[ size_t( k ) ] = X[ size_t( s ) ];
+                Y2[ size_t( k ) ] = Y[ size_t( s ) ];
+                ord2[ size_t( k ) ] = ord[ size_t( s ) ];
+            }
+        }
+        X.swap( X2 ); Y.swap( Y2 ); ord.swap( ord2 );
+        if ( level + 1 < depth ) {
+            std::vector<I64> nb_( 2 * beg.size() ), ne_( 2 * beg.size() );
+            for ( size_t j = 0; j < beg.size(); ++j ) {
+                nb_[ 2 * j ] = beg[ j ]; ne_[ 2 * j ] = mid[ j ];
+                nb_[ 2 * j + 1 ] = mid[ j ]; ne_[ 2 * j + 1 ] = end[ j ];
+            }
+            beg.swap( nb_ ); end.swap( ne_ );
+        }
+    }
+    // the seeds of each leaf in ascending user index: the in-leaf order of a quickselect is arbitrary
+    // ( `AaBsp.py:606-609` ), a canonical one lets two builds of the tree cut cells in the same order
+    {
+        const I64 first = ( I64( 1 ) << ( depth - 1 ) ) - 1;
+        std::vector<I64> idx;
+        for ( I64 h = first; h < nb_nodes; ++h ) {
+            const I64 b = hbeg[ size_t( h ) ], e = hend[ size_t( h ) ];
+            if ( e - b < 2 ) continue;
+            idx.resize( size_t( e - b ) );
+            for ( I64 k = b; k < e; ++k ) idx[ size_t( k - b ) ] = k;
+            std::sort( idx.begin(), idx.end(), [&]( I64 u, I64 v ) { return ord[ size_t( u ) ] < ord[ size_t( v ) ]; } );
+            std::vector<double> tx( idx.size() ), ty( idx.size() );
+            std::vector<I64> to( idx.size() );
+            for ( size_t q = 0; q < idx.size(); ++q ) { tx[ q ] = X[ size_t( idx[ q ] ) ]; ty[ q ] = Y[ size_t( idx[ q ] ) ]; to[ q ] = ord[ size_t( idx[ q ] ) ]; }
+            for ( size_t q = 0; q < idx.size(); ++q ) { X[ size_t( b ) + q ] = tx[ q ]; Y[ size_t( b ) + q ] = ty[ q ]; ord[ size_t( b ) + q ] = to[ q ]; }
+        }
+    }
+    const std::vector<I64> pre = preorder_of_heap( depth );
+    t.beg.assign( size_t( nb_nodes ), 0 );
+    t.end.assign( size_t( nb_nodes ), 0 );
+    t.box.assign( size_t( nb_nodes ) * 4, 0.0 );
+    for ( I64 h = 0; h < nb_nodes; ++h ) {
+        const I64 p = pre[ size_t( h ) ];
+        t.beg[ size_t( p ) ] = hbeg[ size_t( h ) ];
+        t.end[ size_t( p ) ] = hend[ size_t( h ) ];
+        for ( int c = 0; c < 4; ++c ) t.box[ size_t( p ) * 4 + c ] = hbox[ size_t( h ) * 4 + c ];
+    }
+    t.order = ord;
+    t.rank_of.assign( size_t( n ), 0 );
+    for ( I64 k = 0; k < n; ++k ) t.rank_of[ size_t( ord[ size_t( k ) ] ) ] = k;
+    t.px = X;
+    t.py = Y;
+    t.w.assign( size_t( n ), 0.0 );
+    t.wa.assign( size_t( nb_nodes ) * 2, 0.0 );
+    t.wb.assign( size_t( nb_nodes ), 0.0 );
+}
+
+// ---- a tiny thread pool: blocks of 256 cells dealt in turn ( `diagram/Ops.h:234-250` ) --------
+
+static int g_threads = 0;
+
+int nb_threads() {
+    if ( g_threads > 0 ) return g_threads;
+    const unsigned h = std::thread::hardware_concurrency();
+    return h ? int( h ) : 1;
+}
+
+void parallel_blocks( I64 n, I64 block, const std::function<void( int, I64, I64 )> &body ) {
+    const int nt = int( std::max<I64>( 1, std::min<I64>( nb_threads(), ( n + block - 1 ) / block ) ) );
+    auto run = [&]( int th ) {
+        for ( I64 b = I64( th ) * block; b < n; b += I64( nt ) * block )
+            body( th, b, std::min( n, b + block ) );
+    };
+    if ( nt == 1 ) { run( 0 ); return; }
+    std::vector<std::thread> ts;
+    for ( int th = 0; th < nt; ++th ) ts.emplace_back( run, th );
+    for ( auto &x : ts ) x.join();
+}
+
+/// weights ( user order ) into the tree, majorants refreshed node by node ( `sdotplan/Sweep.h:216` )
+void set_weights( Tree &t, const double *w_user ) {
+    for ( I64 k = 0; k < t.n; ++k ) t.w[ size_t( k ) ] = w_user[ t.order[ size_t( k ) ] ];
+    parallel_blocks( t.nb_nodes, 64, [&]( int, I64 b, I64 e ) {
+        for ( I64 m = b; m < e; ++m ) {
+            const I64 sb = t.beg[ size_t( m ) ], se = t.end[ size_t( m ) ];
+            if ( se <= sb ) { t.wa[ size_t( 2 * m ) ] = t.wa[ size_t( 2 * m + 1 ) ] = 0; t.wb[ size_t( m ) ] = 0; continue; }
+            weight_majorant( t.px.data(), t.py.data(), t.w.data(), sb, se, &t.wa[ size_t( 2 * m ) ], t.wb[ size_t( m ) ] );
+        }
+    } );
+}
+
+// ---- the polygon ------------------------------------------------------------------------------
+
+enum CutStatus { UNCHANGED = 0, CUT = 1, EMPTY = 2 };
+
+/// cut ids: >= 0 a seed rank, `DOMAIN_ID` a side of the domain, `PIECE_ID` a pixel plane
+constexpr int DOMAIN_ID = -1;
+constexpr int PIECE_ID  = -2;
+
+/// events of the cut loop ( side tests, pruning tests, new vertices ), for the roofline's count
+struct Events {
+    I64 side_tests = 0, prune_vertex_tests = 0, new_vertices = 0, node_visits = 0, cuts = 0;
+    void add( const Events &o ) {
+        side_tests += o.side_tests; prune_vertex_tests += o.prune_vertex_tests; new_vertices += o.new_vertices;
+        node_visits += o.node_visits; cuts += o.cuts;
+    }
+};
+
+/// a convex polygon in the seed's frame, vertex `i` being the corner of cuts `i - 1` and `i`;
+/// cut `i` carries the edge `[ v_i, v_i+1 ]` ( `cell/Local2.h:1-40` )
+struct Poly {
+    int                 nb = 0;
+    std::vector<double> x, y, s;
+    std::vector<int>    id;
+    void reserve( int c ) { if ( int( x.size() ) < c ) { x.resize( size_t( c ) ); y.resize( size_t( c ) ); s.resize( size_t( c ) ); id.resize( size_t( c ) ); } }
+};
+
+/// the box `[ lo, hi ]` seen from `p0`, counter-clockwise
+static void init_box( Poly &c, const double *lo, const double *hi, double p0x, double p0y ) {
+    c.reserve( 16 );
+    c.nb = 4;
+    c.x[ 0 ] = lo[ 0 ] - p0x; c.y[ 0 ] = lo[ 1 ] - p0y;
+    c.x[ 1 ] = hi[ 0 ] - p0x; c.y[ 1 ] = lo[ 1 ] - p0y;
+    c.x[ 2 ] = hi[ 0 ] - p0x; c.y[ 2 ] = hi[ 1 ] - p0y;
+    c.x[ 3 ] = lo[ 0 ] - p0x; c.y[ 3 ] = hi[ 1 ] - p0y;
+    for ( int i = 0; i < 4; ++i ) c.id[ size_t( i ) ] = DOMAIN_ID;
+}
+
+/// the outside run holding the largest `s` when the signs come in several runs
+/// ( concurrent bisectors; `cell/Local2.h:300-315` )
+static void main_run( const Poly &c, int &i1, int &nb_out ) {
+    const int nb = c.nb;
+    int a = 0;
+    for ( int i = 1; i < nb; ++i ) a = c.s[ size_t( i ) ] > c.s[ size_t( a ) ] ? i : a;
+    i1 = a;
+    for ( int q = a ? a - 1 : nb - 1; c.s[ size_t( q ) ] > 0; q = q ? q - 1 : nb - 1 ) i1 = q;
+    nb_out = 1;
+    for ( int q = a + 1 < nb ? a + 1 : 0; c.s[ size_t( q ) ] > 0; q = q + 1 < nb ? q + 1 : 0 ) ++nb_out;
+    nb_out += ( a - i1 + nb ) % nb;
+}
+
+/// the cut by `dx . v <= off`, in place ( `cell/Local2.h:206-298` )
+static int cut( Poly &c, double dx, double dy, double off, int pid, Events *ev ) {
+    const int nb = c.nb;
+    int nb_out = 0;
+    for ( int i = 0; i < nb; ++i ) {
+        c.s[ size_t( i ) ] = std::fma( dx, c.x[ size_t( i ) ], std::fma( dy, c.y[ size_t( i ) ], -off ) );
+        nb_out += c.s[ size_t( i ) ] > 0;
+    }
+    if ( ev ) ev->side_tests += nb;
+    if ( nb_out == 0 ) return UNCHANGED;
+    if ( nb_out == nb ) { c.nb = 0; return EMPTY; }
+    int i1 = 0;
+    for ( int i = 0; i < nb; ++i ) {
+        const int q = i ? i - 1 : nb - 1;
+        if ( c.s[ size_t( i ) ] > 0 && ! ( c.s[ size_t( q ) ] > 0 ) ) { i1 = i; break; }
+    }
+    for ( int o = 1, i = i1 + 1; o < nb_out; ++o, ++i ) {
+        if ( i == nb ) i = 0;
+        if ( ! ( c.s[ size_t( i ) ] > 0 ) ) { main_run( c, i1, nb_out ); break; }
+    }
+    const int nb_in = nb - nb_out, new_nb = nb_in + 2;
+    c.reserve( new_nb + 1 );
+    const int j0 = ( i1 + nb - 1 ) % nb, j2 = ( i1 + nb_out - 1 ) % nb, j3 = ( j2 + 1 ) % nb;
+    const double s0 = c.s[ size_t( j0 ) ], s1 = c.s[ size_t( i1 ) ], s2 = c.s[ size_t( j2 ) ], s3 = c.s[ size_t( j3 ) ];
+    const double ta = s0 / ( s0 - s1 );
+    const double pax = std::fma( c.x[ size_t( i1 ) ] - c.x[ size_t( j0 ) ], ta, c.x[ size_t( j0 ) ] );
+    const double pay = std::fma( c.y[ size_t( i1 ) ] - c.y[ size_t( j0 ) ], ta, c.y[ size_t( j0 ) ] );
+    const double tb = s3 / ( s3 - s2 );
+    const double pbx = std::fma( c.x[ size_t( j2 ) ] - c.x[ size_t( j3 ) ], tb, c.x[ size_t( j3 ) ] );
+    const double pby = std::fma( c.y[ size_t( j2 ) ] - c.y[ size_t( j3 ) ], tb, c.y[ size_t( j3 ) ] );
+    const int bid = c.id[ size_t( j2 ) ];
+    auto move = [&]( int d, int s ) { c.x[ size_t( d ) ] = c.x[ size_t( s ) ]; c.y[ size_t( d ) ] = c.y[ size_t( s ) ]; c.id[ size_t( d ) ] = c.id[ size_t( s ) ]; };
+    auto put = [&]( int d, double x, double y, int id ) { c.x[ size_t( d ) ] = x; c.y[ size_t( d ) ] = y; c.id[ size_t( d ) ] = id; };
+    if ( i1 <= j2 ) {
+        if ( nb_out == 1 ) {
+            for ( int i = nb; i > i1 + 1; --i ) move( i, i - 1 );
+        } else if ( nb_out > 2 ) {
+            const int gap = nb_out - 2;
+            for ( int i = j2 + 1; i < nb; ++i ) move( i - gap, i );
+        }
+        put( i1, pax, pay, pid );
+        put( i1 + 1, pbx, pby, bid );
+    } else {
+        if ( j3 >= 2 ) for ( int o = 0; o < nb_in; ++o ) move( 2 + o, j3 + o );
+        else           for ( int o = nb_in - 1; o >= 0; --o ) move( 2 + o, j3 + o );
+        put( 0, pax, pay, pid );
+        put( 1, pbx, pby, bid );
+    }
+    c.nb = new_nb;
+    if ( ev ) { ev->new_vertices += 2; ev->cuts += 1; }
+    return CUT;
+}
+
+/// the shoelace area ( `cell/Local2.h:430-440` )
+static double area( const Poly &c ) {
+    double s = 0;
+    for ( int i = 0; i < c.nb; ++i ) {
+        const int j = i + 1 < c.nb ? i + 1 : 0;
+        s += c.x[ size_t( i ) ] * c.y[ size_t( j ) ] - c.x[ size_t( j ) ] * c.y[ size_t( i ) ];
+    }
+    return s / 2;
+}
+
+// ---- the walk --------------------------------------------------------------------------------
+
+/// can a seed of node `m` still cut the cell of `( p0, w0 )`? ( `cell/Pruning.h:46-76` )
+static bool can_cut( const Tree &t, I64 m, const Poly &c, double p0x, double p0y, double w0, Events *ev ) {
+    const double *bx = &t.box[ size_t( m ) * 4 ];
+    const double ax = t.wa[ size_t( 2 * m ) ], ay = t.wa[ size_t( 2 * m + 1 ) ], b = t.wb[ size_t( m ) ];
+    if ( ev ) ev->node_visits += 1;
+    for ( int i = 0; i < c.nb; ++i ) {
+        if ( ev ) ev->prune_vertex_tests += 1;
+        const double X = c.x[ size_t( i ) ] + p0x, Y = c.y[ size_t( i ) ] + p0y;
+        double yx = X + ax / 2, yy = Y + ay / 2;
+        yx = yx < bx[ 0 ] ? bx[ 0 ] : ( yx > bx[ 2 ] ? bx[ 2 ] : yx );
+        yy = yy < bx[ 1 ] ? bx[ 1 ] : ( yy > bx[ 3 ] ? bx[ 3 ] : yy );
+        const double ux = yx - X, uy = yy - Y;
+        const double s = w0 - b + ux * ux - c.x[ size_t( i ) ] * c.x[ size_t( i ) ] - ax * yx
+                                + uy * uy - c.y[ size_t( i ) ] * c.y[ size_t( i ) ] - ay * yy;
+        if ( s <= 0 ) return true;
+    }
+    return false;
+}
+
+/// squared distance from `p0` to the box of `m`: the ordering key of the walk ( `cell/Providers.h:148` )
+static double proximity( const Tree &t, I64 m, double p0x, double p0y ) {
+    const double *bx = &t.box[ size_t( m ) * 4 ];
+    double e0 = std::max( bx[ 0 ] - p0x, p0x - bx[ 2 ] ), e1 = std::max( bx[ 1 ] - p0y, p0y - bx[ 3 ] );
+    e0 = e0 > 0 ? e0 : 0;
+    e1 = e1 > 0 ? e1 : 0;
+    return e0 * e0 + e1 * e1;
+}
+
+/// the cell of rank `k0` in `c`, from the domain box, cut by the planes the tree proposes
+/// ( `diagram/Ops.h:82`, `cell/Providers.h:167-236` ). Returns false if the cell is empty.
+bool make_cell( const Tree &t, I64 k0, const double *lo, const double *hi, Poly &c, Events *ev ) {
+    const double p0x = t.px[ size_t( k0 ) ], p0y = t.py[ size_t( k0 ) ], w0 = t.w[ size_t( k0 ) ];
+    init_box( c, lo, hi, p0x, p0y );
+    I64 stack[ 128 ];
+    int top = 0;
+    stack[ top++ ] = ( I64( 0 ) << 7 ) | t.depth;
+    while ( top ) {
+        const I64 en = stack[ --top ];
+        const I64 m = en >> 7;
+        const int h = int( en & 127 );
+        const I64 b = t.beg[ size_t( m ) ], e = t.end[ size_t( m ) ];
+        if ( b >= e ) continue;
+        if ( ! can_cut( t, m, c, p0x, p0y, w0, ev ) ) continue;
+        if ( h <= 1 ) {
+            for ( I64 k = b; k < e; ++k ) {
+                if ( k == k0 ) continue;
+                const double dx = t.px[ size_t( k ) ] - p0x, dy = t.py[ size_t( k ) ] - p0y;
+                const double off = ( dx * dx + dy * dy + w0 - t.w[ size_t( k ) ] ) / 2;
+                if ( cut( c, dx, dy, off, int( k ), ev ) == EMPTY ) return false;
+            }
+            continue;
+        }
+        const I64 lc = m + 1, rc = m + ( I64( 1 ) << ( h - 1 ) );
+        const int hc = h - 1;
+        if ( proximity( t, lc, p0x, p0y ) <= proximity( t, rc, p0x, p0y ) ) {
+            stack[ top++ ] = ( rc << 7 ) | hc;
+            stack[ top++ ] = ( lc << 7 ) | hc;
+        } else {
+            stack[ top++ ] = ( lc << 7 ) | hc;
+            stack[ top++ ] = ( rc << 7 ) | hc;
+        }
+    }
+    return c.nb > 0;
+}
+
+/// the same cell, cut by EVERY other seed in rank order: no tree at all ( `cell/Providers.h:36-64` )
+bool make_cell_all( const Tree &t, I64 k0, const double *lo, const double *hi, Poly &c ) {
+    const double p0x = t.px[ size_t( k0 ) ], p0y = t.py[ size_t( k0 ) ], w0 = t.w[ size_t( k0 ) ];
+    init_box( c, lo, hi, p0x, p0y );
+    for ( I64 k = 0; k < t.n; ++k ) {
+        if ( k == k0 ) continue;
+        const double dx = t.px[ size_t( k ) ] - p0x, dy = t.py[ size_t( k ) ] - p0y;
+        const double off = ( dx * dx + dy * dy + w0 - t.w[ size_t( k ) ] ) / 2;
+        if ( cut( c, dx, dy, off, int( k ), nullptr ) == EMPTY ) return false;
+    }
+    return true;
+}
+
+// ---- integrals against a density ---------------------------------------------------------------
+
+/// moments of one piece of constant density `rho`, in the seed's frame: `m = int rho`,
+/// `mx = int v rho`, `m2 = int |v|^2 rho` by the fan from vertex 0 ( `diagram/Ops.h:187-228` )
+static void piece_moments( const Poly &c, double rho, double &m, double *mx, double &m2 ) {
+    for ( int i = 1; i + 1 < c.nb; ++i ) {
+        const double ax = c.x[ 0 ], ay = c.y[ 0 ];
+        const double bx = c.x[ size_t( i ) ], by = c.y[ size_t( i ) ];
+        const double cx = c.x[ size_t( i + 1 ) ], cy = c.y[ size_t( i + 1 ) ];
+        const double ar = std::fabs( ( bx - ax ) * ( cy - ay ) - ( by - ay ) * ( cx - ax ) ) / 2;
+        const double w = rho * ar;
+        const double sx = ax + bx + cx, sy = ay + by + cy;
+        const double sq = ax * ax + ay * ay + bx * bx + by * by + cx * cx + cy * cy;
+        m += w;
+        mx[ 0 ] += w * sx / 3;
+        mx[ 1 ] += w * sy / 3;
+        m2 += w * ( sq + sx * sx + sy * sy ) / 12;
+    }
+}
+
+/// `func( piece, value )` for each piece of the cell on which the density is constant
+/// ( `Image.cxx:106-232`: the tiles the cell's box meets, each cut by the tile's planes when the
+/// cell crosses them ); `p0` is the seed, the cell is in its frame
+template<class F>
+static void for_each_piece( const Density &dn, const Poly &c, double p0x, double p0y, Poly &piece, F &&func ) {
+    if ( c.nb == 0 ) return;
+    if ( dn.kind == 0 ) { func( c, dn.value ); return; }
+    const double org[ 2 ] = { dn.x0 - p0x, dn.y0 - p0y }, h[ 2 ] = { dn.hx, dn.hy };
+    const int nbk[ 2 ] = { dn.nx, dn.ny };
+    I64 k0[ 2 ], k1[ 2 ];
+    double tlo[ 2 ], thi[ 2 ];
+    for ( int a = 0; a < 2; ++a ) {
+        double mn = 0, mx = 0;
+        for ( int v = 0; v < c.nb; ++v ) {
+            const double t = ( ( a ? c.y[ size_t( v ) ] : c.x[ size_t( v ) ] ) - org[ a ] ) / h[ a ];
+            if ( v == 0 || t < mn ) mn = t;
+            if ( v == 0 || t > mx ) mx = t;
+        }
+        if ( mx < 0 || mn > nbk[ a ] ) return;
+        tlo[ a ] = mn;
+        thi[ a ] = mx;
+        auto knot_index = [&]( double t ) {
+            I64 b = 0, e = nbk[ a ];
+            while ( e - b > 1 ) { const I64 m = ( b + e ) / 2; if ( double( m ) <= t ) b = m; else e = m; }
+            return b;
+        };
+        k0[ a ] = knot_index( mn );
+        k1[ a ] = knot_index( mx ) + 1;
+    }
+    for ( I64 j = k0[ 1 ]; j < k1[ 1 ]; ++j ) {
+        for ( I64 i = k0[ 0 ]; i < k1[ 0 ]; ++i ) {
+            const I64 k[ 2 ] = { i, j };
+            bool alive = true, started = false;
+            for ( int a = 0; a < 2 && alive; ++a ) {
+                const double tile_lo = double( k[ a ] ), tile_hi = double( k[ a ] + 1 );
+                const double lo = org[ a ] + tile_lo * h[ a ], hi = org[ a ] + tile_hi * h[ a ];
+                const double dx = a == 0, dy = a == 1;
+                if ( thi[ a ] > tile_hi ) {
+                    if ( ! started ) { piece.reserve( c.nb + 8 ); piece.nb = c.nb; for ( int q = 0; q < c.nb; ++q ) { piece.x[ size_t( q ) ] = c.x[ size_t( q ) ]; piece.y[ size_t( q ) ] = c.y[ size_t( q ) ]; piece.id[ size_t( q ) ] = c.id[ size_t( q ) ]; } }
+                    started = true;
+                    if ( cut( piece, dx, dy, hi, PIECE_ID, nullptr ) == EMPTY ) alive = false;
+                }
+                if ( alive && tlo[ a ] < tile_lo ) {
+                    if ( ! started ) { piece.reserve( c.nb + 8 ); piece.nb = c.nb; for ( int q = 0; q < c.nb; ++q ) { piece.x[ size_t( q ) ] = c.x[ size_t( q ) ]; piece.y[ size_t( q ) ] = c.y[ size_t( q ) ]; piece.id[ size_t( q ) ] = c.id[ size_t( q ) ]; } }
+                    started = true;
+                    if ( cut( piece, -dx, -dy, -lo, PIECE_ID, nullptr ) == EMPTY ) alive = false;
+                }
+            }
+            if ( ! alive ) continue;
+            const double value = dn.img[ size_t( j ) * size_t( dn.nx ) + size_t( i ) ];
+            func( started ? piece : c, value );
+        }
+    }
+}
+
+/// one cell's mass and its facets `c_kj = int_facet rho / ( 2 |p_k - p_j| )` ( `sdotplan/Sweep.h:64-128` );
+/// `facet( j_rank, c )` per neighbour, in the cell's edge order
+template<class F>
+static double mass_and_facets( const Tree &t, const Density &dn, I64 k, const Poly &c, Poly &piece, bool with_facets, F &&facet ) {
+    const double p0x = t.px[ size_t( k ) ], p0y = t.py[ size_t( k ) ];
+    double mass = 0;
+    std::vector<double> vals( sz( c.nb ), 0.0 );
+    for_each_piece( dn, c, p0x, p0y, piece, [&]( const Poly &pc, double value ) {
+        mass += value * area( pc );
+        if ( ! with_facets ) return;
+        for ( int q = 0; q < pc.nb; ++q ) {
+            const int id = pc.id[ size_t( q ) ];
+            if ( id < 0 ) continue;
+            const int r = q + 1 < pc.nb ? q + 1 : 0;
+            const double dx = pc.x[ size_t( r ) ] - pc.x[ size_t( q ) ], dy = pc.y[ size_t( r ) ] - pc.y[ size_t( q ) ];
+            const double val = value * std::sqrt( dx * dx + dy * dy );
+            for ( int u = 0; u < c.nb; ++u )
+                if ( c.id[ size_t( u ) ] == id ) { vals[ size_t( u ) ] += val; break; }
+        }
+    } );
+    if ( with_facets ) {
+        for ( int u = 0; u < c.nb; ++u ) {
+            const int id = c.id[ size_t( u ) ];
+            if ( id < 0 ) continue;
+            const double dx = t.px[ size_t( id ) ] - p0x, dy = t.py[ size_t( id ) ] - p0y;
+            const double d2 = dx * dx + dy * dy;
+            if ( d2 > 0 ) facet( I64( id ), vals[ size_t( u ) ] / ( 2 * std::sqrt( d2 ) ) );
+        }
+    }
+    return mass;
+}
+
+// ---- the Laplacian and the CG -----------------------------------------------------------------
+
+struct Facet { I64 i, j; double c; };
+
+/// `sdotplan/Laplacian.h:40-72`: each row keeps its own view of a facet; `L_ii = sum_j c_ij`
+struct Laplacian {
+    I64 n = 0;
+    std::vector<I64> row, col;
+    std::vector<double> c, dia;
+    void assemble( I64 nb, const std::vector<Facet> &fa ) {
+        n = nb;
+        row.assign( size_t( n + 1 ), 0 );
+        for ( const Facet &e : fa ) ++row[ size_t( e.i + 1 ) ];
+        for ( I64 i = 0; i < n; ++i ) row[ size_t( i + 1 ) ] += row[ size_t( i ) ];
+        col.resize( size_t( row[ size_t( n ) ] ) );
+        c.resize( size_t( row[ size_t( n ) ] ) );
+        dia.assign( size_t( n ), 0.0 );
+        std::vector<I64> at( row.begin(), row.end() - 1 );
+        for ( const Facet &e : fa ) {
+            const I64 p = at[ size_t( e.i ) ]++;
+            col[ size_t( p ) ] = e.j;
+            c[ size_t( p ) ] = e.c;
+            dia[ size_t( e.i ) ] += e.c;
+        }
+        for ( I64 i = 0; i < n; ++i ) if ( ! ( dia[ size_t( i ) ] > 0 ) ) dia[ size_t( i ) ] = 1;
+    }
+    void product( const double *x, double *y ) const {
+        for ( I64 i = 0; i < n; ++i ) {
+            double s = dia[ size_t( i ) ] * x[ i ];
+            for ( I64 e = row[ size_t( i ) ]; e < row[ size_t( i + 1 ) ]; ++e ) s -= c[ size_t( e ) ] * x[ col[ size_t( e ) ] ];
+            y[ i ] = s;
+        }
+    }
+};
+
+struct LinStats { int nb_iter = 0; double worst = 0, t = 0; };
+
+/// Jacobi-preconditioned CG, the gauge `d_0 = 0` ( `sdotplan/Linear.cpp:44-88` )
+bool cg_solve( const Laplacian &L, const std::vector<double> &b, std::vector<double> &d, double tol, int maxit, LinStats &st ) {
+    const double t0 = now();
+    const I64 n = L.n;
+    std::vector<double> r( sz( n ) ), z( sz( n ) ), p( sz( n ) ), q( sz( n ) );
+    d.assign( size_t( n ), 0.0 );
+    double nb = 0;
+    for ( I64 i = 1; i < n; ++i ) { r[ size_t( i ) ] = b[ size_t( i ) ]; nb += b[ size_t( i ) ] * b[ size_t( i ) ]; }
+    r[ 0 ] = 0;
+    nb = std::sqrt( nb );
+    if ( ! ( nb > 0 ) ) { st.t += now() - t0; return true; }
+    double rz = 0;
+    for ( I64 i = 1; i < n; ++i ) { z[ size_t( i ) ] = r[ size_t( i ) ] / L.dia[ size_t( i ) ]; p[ size_t( i ) ] = z[ size_t( i ) ]; rz += r[ size_t( i ) ] * z[ size_t( i ) ]; }
+    double err = 1;
+    int it = 0;
+    for ( ; it < maxit; ++it ) {
+        L.product( p.data(), q.data() );
+        q[ 0 ] = 0;
+        double pq = 0;
+        for ( I64 i = 1; i < n; ++i ) pq += p[ size_t( i ) ] * q[ size_t( i ) ];
+        if ( ! ( pq > 0 ) ) break;
+        const double alpha = rz / pq;
+        double nr = 0;
+        for ( I64 i = 1; i < n; ++i ) { d[ size_t( i ) ] += alpha * p[ size_t( i ) ]; r[ size_t( i ) ] -= alpha * q[ size_t( i ) ]; nr += r[ size_t( i ) ] * r[ size_t( i ) ]; }
+        err = std::sqrt( nr ) / nb;
+        if ( err <= tol ) break;
+        double rz2 = 0;
+        for ( I64 i = 1; i < n; ++i ) { z[ size_t( i ) ] = r[ size_t( i ) ] / L.dia[ size_t( i ) ]; rz2 += r[ size_t( i ) ] * z[ size_t( i ) ]; }
+        const double beta = rz2 / rz;
+        rz = rz2;
+        for ( I64 i = 1; i < n; ++i ) p[ size_t( i ) ] = z[ size_t( i ) ] + beta * p[ size_t( i ) ];
+    }
+    st.nb_iter += it;
+    st.worst = std::max( st.worst, err );
+    st.t += now() - t0;
+    return err < 1;
+}
+
+// ---- the diagram as Newton sees it ( `sdotplan/Sweep.h:142-364` ) ------------------------------
+
+struct Sweep {
+    Tree   &t;
+    const double *lo, *hi;
+    Density dn;
+    int    nb_diag = 0;
+    double t_majorant = 0, t_diag = 0;
+    Events ev;
+
+    Sweep( Tree &t, const double *lo, const double *hi, const Density &dn ) : t( t ), lo( lo ), hi( hi ), dn( dn ) {}
+
+    void set_weights( const std::vector<double> &W ) {
+        const double t0 = now();
+        orc::set_weights( t, W.data() );
+        t_majorant += now() - t0;
+    }
+
+    /// masses ( user order ) and, if asked, the facets ( user indices )
+    void measures( std::vector<double> &a, std::vector<Facet> *fa ) {
+        const double t0 = now();
+        const I64 n = t.n;
+        a.assign( size_t( n ), 0.0 );
+        const int nt = nb_threads();
+        std::vector<std::vector<Facet>> fth( sz( nt ) );
+        std::vector<Events> eth( sz( nt ) );
+        parallel_blocks( n, 256, [&]( int th, I64 b, I64 e ) {
+            Poly c, piece;
+            c.reserve( 64 );
+            piece.reserve( 64 );
+            for ( I64 k = b; k < e; ++k ) {
+                const I64 i = t.order[ size_t( k ) ];
+                if ( ! make_cell( t, k, lo, hi, c, &eth[ size_t( th ) ] ) ) { a[ size_t( i ) ] = 0; continue; }
+                a[ size_t( i ) ] = mass_and_facets( t, dn, k, c, piece, fa != nullptr, [&]( I64 j, double cij ) {
+                    fth[ size_t( th ) ].push_back( Facet{ i, t.order[ size_t( j ) ], cij } );
+                } );
+            }
+        } );
+        if ( fa ) {
+            fa->clear();
+            for ( auto &v : fth ) fa->insert( fa->end(), v.begin(), v.end() );
+            // a deterministic order whatever the thread count: by row, then in the cell's edge order
+            std::stable_sort( fa->begin(), fa->end(), []( const Facet &x, const Facet &y ) { return x.i < y.i; } );
+        }
+        for ( auto &e : eth ) ev.add( e );
+        t_diag += now() - t0;
+        ++nb_diag;
+    }
+
+    /// barycenters ( the seed for an empty cell ) and the transport cost ( `sdotplan/Sweep.h:305-362` )
+    void moments( std::vector<double> &bary, double &cost ) {
+        const I64 n = t.n;
+        bary.assign( size_t( n ) * 2, 0.0 );
+        const int nt = nb_threads();
+        std::vector<double> cth( sz( nt ), 0.0 );
+        parallel_blocks( n, 256, [&]( int th, I64 b, I64 e ) {
+            Poly c, piece;
+            c.reserve( 64 );
+            piece.reserve( 64 );
+            for ( I64 k = b; k < e; ++k ) {
+                const I64 i = t.order[ size_t( k ) ];
+                const double p0x = t.px[ size_t( k ) ], p0y = t.py[ size_t( k ) ];
+                double m = 0, mx[ 2 ] = { 0, 0 }, m2 = 0;
+                if ( make_cell( t, k, lo, hi, c, nullptr ) )
+                    for_each_piece( dn, c, p0x, p0y, piece, [&]( const Poly &pc, double value ) { piece_moments( pc, value, m, mx, m2 ); } );
+                cth[ size_t( th ) ] += m2;
+                bary[ size_t( 2 * i ) ] = m > 0 ? p0x + mx[ 0 ] / m : p0x;
+                bary[ size_t( 2 * i + 1 ) ] = m > 0 ? p0y + mx[ 1 ] / m : p0y;
+            }
+        } );
+        cost = 0;
+        for ( double v : cth ) cost += v;
+    }
+};
+
+// ---- Newton ( `sdotplan/Newton.h` ) ------------------------------------------------------------
+
+enum Stat {
+    STATUS = 0, RESIDUAL, RESIDUAL0, NB_ITER, NB_DIAG, NB_BACKTRACKS, T_MAJORANT, T_DIAG, T_ASM, T_LIN, T_LIM, EPS,
+    DOMAIN_MASS, NB_OVERFLOWED, NB_CELL_LIM, NB_LIMIT_ROUNDS, LIN_NB_HIERARCHIES, LIN_NB_ITER, LIN_WORST, START, T_TOTAL,
+    NB_CONTINUATION_STEPS, MIN_START_MASS, IT_SWITCH, IT_DOUBLE, SCRATCH_BYTES,
+    NB_CLUSTERS, NB_AGGREGATED, NB_DUPLICATES, RESIDUAL_FULL, NB_POLISH, NB_STATS
+};
+enum Hist { H_STEP = 0, H_T, H_RESIDUAL_L2, H_MIN_MASS, H_MAX_RESIDUAL, H_NB_DIAG, H_NB_EVALS, H_S, NB_HIST };
+enum Residual { LIN = 0, LOG = 1, POWER = 2 };
+
+struct Options {
+    double tol_abs = 1e-8, tol_rel = 0, t_min = 1e-10, mult_ok = 4, power = 0.5, switch_residual = 2, lin_tol = 1e-12;
+    int    maxit = 100, max_backtracks = 60, residual = LOG, lin_maxit = 20000;
+    int    keep_start = 0;
+};
+
+/// `Report.h:37-57`: two refused trials whose secant slopes agree and extrapolate under `nr / 4`
+struct Hopeless {
+    double t_prev = 0, s_prev = 0;
+    bool has_prev = false;
+    bool refused( double t, double n2r, double nr, bool ok ) {
+        if ( ! ok || ! ( nr > 0 ) ) { has_prev = false; return false; }
+        const double s = ( nr - n2r ) / t;
+        const double as = s_prev > 0 ? s_prev : -s_prev;
+        const bool hopeless = has_prev && t_prev == 2 * t && s < nr / 4 && s_prev < nr / 4 &&
+                              s - s_prev <= 0.25 * as && s_prev - s <= 0.25 * as && 2 * s - s_prev < nr / 4;
+        t_prev = t; s_prev = s; has_prev = true;
+        return hopeless;
+    }
+};
+
+static double g_of( double x, int r, double p ) {
+    x = std::max( x, 1e-300 );
+    if ( r == POWER ) return p == 0 ? std::log( x ) : ( std::pow( x, p ) - 1 ) / p;
+    return r == LOG ? std::log( x ) : x - 1;
+}
+static double gp_of( double x, int r, double p ) {
+    x = std::max( x, 1e-300 );
+    if ( r == POWER ) return std::pow( x, p - 1 );
+    return r == LOG ? 1 / x : 1;
+}
+
+struct Solver {
+    Sweep &sw;
+    Options o;
+    std::vector<double> nu, w, a, d;
+    std::vector<Facet> fa;
+    int res_cur = LIN;
+    double t_last = 1;
+    // stats
+    int status = 0, nb_iter = 0, nb_backtracks = 0, it_switch = -1;
+    double residual = 0, residual0 = 0, eps = 0, t_asm = 0;
+    LinStats lst;
+    std::function<void( int, double, int )> after_step;
+
+    Solver( Sweep &sw, const Options &o ) : sw( sw ), o( o ) {}
+
+    void rhs( const std::vector<double> &A, std::vector<double> &b ) const {
+        const size_t n = A.size();
+        b.assign( n, 0.0 );
+        if ( res_cur == LIN ) { for ( size_t i = 0; i < n; ++i ) b[ i ] = nu[ i ] > 0 ? nu[ i ] - A[ i ] : 0.0; return; }
+        double su = 0, sug = 0;
+        for ( size_t i = 0; i < n; ++i ) {
+            if ( ! ( nu[ i ] > 0 ) ) continue;
+            const double x = A[ i ] / nu[ i ], u = nu[ i ] / gp_of( x, res_cur, o.power );
+            su += u;
+            sug += u * g_of( x, res_cur, o.power );
+        }
+        const double c = sug / su;
+        for ( size_t i = 0; i < n; ++i ) {
+            if ( ! ( nu[ i ] > 0 ) ) continue;
+            const double x = A[ i ] / nu[ i ];
+            b[ i ] = nu[ i ] / gp_of( x, res_cur, o.power ) * ( c - g_of( x, res_cur, o.power ) );
+        }
+    }
+    static void project_on_range( std::vector<double> &b ) {
+        if ( b.empty() ) return;
+        double s = 0;
+        for ( double v : b ) s += v;
+        s /= double( b.size() );
+        for ( double &v : b ) v -= s;
+    }
+    double merit( const std::vector<double> &A ) const {
+        const size_t n = A.size();
+        double s = 0;
+        if ( res_cur == LIN ) {
+            for ( size_t i = 0; i < n; ++i ) s += ( nu[ i ] - A[ i ] ) * ( nu[ i ] - A[ i ] );
+            return std::sqrt( s );
+        }
+        double m = 0;
+        for ( size_t i = 0; i < n; ++i ) m += g_of( A[ i ] / nu[ i ], res_cur, o.power );
+        m /= double( n );
+        for ( size_t i = 0; i < n; ++i ) { const double e = g_of( A[ i ] / nu[ i ], res_cur, o.power ) - m; s += e * e; }
+        return std::sqrt( s );
+    }
+    void measures_and_facets( std::vector<double> &W, std::vector<double> &res, std::vector<Facet> &f ) {
+        sw.set_weights( W );
+        sw.measures( res, &f );
+    }
+
+    /// `Newton.h:243-400` ( trials, no aggregation ): `a` and `fa` already measured for `w_init`
+    bool solves( const std::vector<double> &w_init ) {
+        const I64 n = I64( w_init.size() );
+        std::vector<double> a2, b, w2;
+        std::vector<Facet> fa2;
+        Laplacian L;
+        w = w_init;
+        const double gauge = w[ 0 ];
+        for ( auto &v : w ) v -= gauge;
+        t_last = 1;
+        if ( after_step ) after_step( 0, 0, 1 );
+        res_cur = o.residual;
+        for ( int it = 0; it < o.maxit; ++it ) {
+            double worst = 0, worst_rel = 0;
+            for ( I64 i = 0; i < n; ++i ) {
+                worst = std::max( worst, std::fabs( nu[ size_t( i ) ] - a[ size_t( i ) ] ) );
+                worst_rel = std::max( worst_rel, std::fabs( nu[ size_t( i ) ] - a[ size_t( i ) ] ) / nu[ size_t( i ) ] );
+            }
+            if ( o.switch_residual > 0 && res_cur != LIN && worst_rel <= o.switch_residual ) { res_cur = LIN; it_switch = it; }
+            rhs( a, b );
+            project_on_range( b );
+            if ( it == 0 ) {
+                double am = a[ 0 ], nm = nu[ 0 ];
+                for ( I64 i = 0; i < n; ++i ) { am = std::min( am, a[ size_t( i ) ] ); nm = std::min( nm, nu[ size_t( i ) ] ); }
+                eps = 0.5 * std::min( nm, am );
+                residual0 = worst;
+            }
+            const double nr = merit( a );
+            residual = worst;
+            if ( worst <= o.tol_abs || ( o.tol_rel > 0 && worst_rel <= o.tol_rel ) ) { status = 1; return true; }
+            ++nb_iter;
+            double t0 = now();
+            L.assemble( n, fa );
+            t_asm += now() - t0;
+            if ( ! cg_solve( L, b, d, o.lin_tol, o.lin_maxit, lst ) ) { status = 4; return false; }
+            double t = std::min( 1.0, o.mult_ok * t_last );
+            bool taken = false;
+            Hopeless hope;
+            int nb_evals = 0;
+            w2.resize( size_t( n ) );
+            for ( int trial = 0; trial < o.max_backtracks; ++trial ) {
+                for ( I64 i = 0; i < n; ++i ) w2[ size_t( i ) ] = w[ size_t( i ) ] + t * d[ size_t( i ) ];
+                w2[ 0 ] = 0;
+                measures_and_facets( w2, a2, fa2 );
+                ++nb_evals;
+                double m2 = a2[ 0 ];
+                for ( double v : a2 ) m2 = std::min( m2, v );
+                const double n2r = merit( a2 );
+                if ( m2 >= eps && std::isfinite( n2r ) && n2r <= ( 1 - t / 2 ) * nr && n2r < nr ) { taken = true; break; }
+                const bool hopeless = hope.refused( t, n2r, nr, m2 >= eps && std::isfinite( n2r ) );
+                t /= 2;
+                ++nb_backtracks;
+                if ( t < o.t_min || hopeless ) break;
+            }
+            if ( ! taken && res_cur != LIN ) {
+                sw.set_weights( w );
+                res_cur = LIN;
+                it_switch = it;
+                --nb_iter;
+                --it;
+                continue;
+            }
+            if ( ! taken ) { sw.set_weights( w ); status = 3; return false; }
+            t_last = t;
+            w.swap( w2 );
+            a.swap( a2 );
+            fa.swap( fa2 );
+            if ( after_step ) after_step( it + 1, t, nb_evals );
+        }
+        double worst = 0;
+        for ( I64 i = 0; i < n; ++i ) worst = std::max( worst, std::fabs( nu[ size_t( i ) ] - a[ size_t( i ) ] ) );
+        residual = worst;
+        status = 2;
+        return false;
+    }
+};
+
+/// the similarity start ( `sdotplan/Solve.h:69-100` )
+static void similarity( const Tree &t, const double *pos, const double *lo, const double *hi, std::vector<double> &w, double margin = 0.1 ) {
+    const I64 n = t.n;
+    double plo[ 2 ] = { 1e300, 1e300 }, phi[ 2 ] = { -1e300, -1e300 };
+    for ( I64 k = 0; k < n; ++k )
+        for ( int d = 0; d < 2; ++d ) { plo[ d ] = std::min( plo[ d ], pos[ 2 * k + d ] ); phi[ d ] = std::max( phi[ d ], pos[ 2 * k + d ] ); }
+    double a = 1;
+    for ( int d = 0; d < 2; ++d ) {
+        const double span_dom = ( hi[ d ] - lo[ d ] ) * ( 1 - 2 * margin );
+        const double span_pts = std::max( phi[ d ] - plo[ d ], 1e-300 );
+        a = std::min( a, span_dom / span_pts );
+    }
+    double b[ 2 ];
+    for ( int d = 0; d < 2; ++d ) b[ d ] = ( lo[ d ] + hi[ d ] ) / 2 - a * ( plo[ d ] + phi[ d ] ) / 2;
+    w.assign( size_t( n ), 0.0 );
+    for ( I64 k = 0; k < n; ++k ) {
+        double pp = 0, qq = 0;
+        for ( int d = 0; d < 2; ++d ) {
+            const double q = a * pos[ 2 * k + d ] + b[ d ];
+            pp += pos[ 2 * k + d ] * pos[ 2 * k + d ];
+            qq += q * q;
+        }
+        w[ size_t( k ) ] = pp - qq / a;
+    }
+}
+
+} // namespace orc
+
+// ============================================================================================
+// the C entry points, for ctypes
+// ============================================================================================
+
+using orc::I64;
+using orc::sz;
+
+static orc::Density make_density( int kind, double value, int nx, int ny, const double *geo, const double *img ) {
+    orc::Density d;
+    d.kind = kind;
+    d.value = value;
+    if ( kind == 1 ) {
+        d.nx = nx; d.ny = ny;
+        d.x0 = geo[ 0 ]; d.y0 = geo[ 1 ]; d.hx = geo[ 2 ]; d.hy = geo[ 3 ];
+        d.img = img;
+    }
+    return d;
+}
+
+extern "C" {
+
+void oracle_set_threads( int nt ) { orc::g_threads = nt; }
+int  oracle_get_threads() { return orc::nb_threads(); }
+
+int oracle_max_depth_for( I64 n, I64 leaf ) { return orc::max_depth_for( n, leaf ); }
+
+/// the tree of `pos[ n ][ 2 ]` with the majorants of `w` ( user order; null: zeros ). Outputs sized by
+/// the caller: `order[ n ]`, `beg/end[ N ]`, `box[ N * 4 ]`, `wa[ N * 2 ]`, `wb[ N ]`, N = 2^depth - 1
+int oracle_build_tree( I64 n, const double *pos, const double *w, I64 leaf, I64 *order, I64 *beg, I64 *end, double *box,
+                       double *wa, double *wb ) {
+    orc::Tree t;
+    orc::build_tree( t, n, pos, leaf );
+    std::vector<double> W( sz( n ), 0.0 );
+    if ( w ) for ( I64 i = 0; i < n; ++i ) W[ size_t( i ) ] = w[ i ];
+    orc::set_weights( t, W.data() );
+    std::memcpy( order, t.order.data(), sizeof( I64 ) * size_t( n ) );
+    std::memcpy( beg, t.beg.data(), sizeof( I64 ) * size_t( t.nb_nodes ) );
+    std::memcpy( end, t.end.data(), sizeof( I64 ) * size_t( t.nb_nodes ) );
+    std::memcpy( box, t.box.data(), sizeof( double ) * size_t( t.nb_nodes ) * 4 );
+    std::memcpy( wa, t.wa.data(), sizeof( double ) * size_t( t.nb_nodes ) * 2 );
+    std::memcpy( wb, t.wb.data(), sizeof( double ) * size_t( t.nb_nodes ) );
+    return t.depth;
+}
+
+/// the majorant of one slice ( testing hook )
+void oracle_weight_majorant( I64 m, const double *pos, const double *w, double *wa, double *wb ) {
+    std::vector<double> px( sz( m ) ), py( sz( m ) );
+    for ( I64 k = 0; k < m; ++k ) { px[ size_t( k ) ] = pos[ 2 * k ]; py[ size_t( k ) ] = pos[ 2 * k + 1 ]; }
+    orc::weight_majorant( px.data(), py.data(), w, 0, m, wa, *wb );
+}
+
+/// the masses ( user order ) of the diagram of `( pos, w )` in the box `lo/hi` against a density
+/// ( kind 0: constant `value`; kind 1: image `img[ ny ][ nx ]`, `geo = x0, y0, hx, hy` ), and if
+/// `fi` is not null the facets `( fi, fj, fc )` ( user indices, rows in order ), at most `cap`;
+/// returns the number of facets ( -1 if more than `cap` ). `ev[ 5 ]`: the event counts ( or null ).
+I64 oracle_measures( I64 n, const double *pos, const double *w, const double *lo, const double *hi, int kind, double value,
+                     int nx, int ny, const double *geo, const double *img, double *masses, I64 *fi, I64 *fj, double *fc, I64 cap,
+                     I64 *ev, int brute_force ) {
+    orc::Tree t;
+    orc::build_tree( t, n, pos, 10 );
+    std::vector<double> W( sz( n ), 0.0 );
+    if ( w ) for ( I64 i = 0; i < n; ++i ) W[ size_t( i ) ] = w[ i ];
+    orc::Sweep sw( t, lo, hi, make_density( kind, value, nx, ny, geo, img ) );
+    sw.set_weights( W );
+    std::vector<double> a;
+    std::vector<orc::Facet> fa;
+    if ( brute_force ) {
+        a.assign( size_t( n ), 0.0 );
+        orc::Poly c, piece;
+        for ( I64 k = 0; k < n; ++k ) {
+            const I64 i = t.order[ size_t( k ) ];
+            if ( ! orc::make_cell_all( t, k, lo, hi, c ) ) continue;
+            a[ size_t( i ) ] = orc::mass_and_facets( t, sw.dn, k, c, piece, fi != nullptr, [&]( I64 j, double cij ) {
+                fa.push_back( orc::Facet{ i, t.order[ size_t( j ) ], cij } );
+            } );
+        }
+        std::stable_sort( fa.begin(), fa.end(), []( const orc::Facet &x, const orc::Facet &y ) { return x.i < y.i; } );
+    } else
+        sw.measures( a, fi ? &fa : nullptr );
+    std::memcpy( masses, a.data(), sizeof( double ) * size_t( n ) );
+    if ( ev ) {
+        ev[ 0 ] = sw.ev.side_tests; ev[ 1 ] = sw.ev.prune_vertex_tests; ev[ 2 ] = sw.ev.new_vertices;
+        ev[ 3 ] = sw.ev.node_visits; ev[ 4 ] = sw.ev.cuts;
+    }
+    if ( ! fi ) return 0;
+    if ( I64( fa.size() ) > cap ) return -1;
+    for ( size_t q = 0; q < fa.size(); ++q ) { fi[ q ] = fa[ q ].i; fj[ q ] = fa[ q ].j; fc[ q ] = fa[ q ].c; }
+    return I64( fa.size() );
+}
+
+/// the vertices of the cell of user seed `i` ( absolute coordinates, counter-clockwise ) and its
+/// cut ids ( user indices of the neighbours, -1 for the domain ); returns the vertex count
+int oracle_cell( I64 n, const double *pos, const double *w, const double *lo, const double *hi, I64 i, double *xy, I64 *ids,
+                 int cap, int brute_force ) {
+    orc::Tree t;
+    orc::build_tree( t, n, pos, 10 );
+    std::vector<double> W( sz( n ), 0.0 );
+    if ( w ) for ( I64 q = 0; q < n; ++q ) W[ size_t( q ) ] = w[ q ];
+    orc::set_weights( t, W.data() );
+    orc::Poly c;
+    const I64 k = t.rank_of[ size_t( i ) ];
+    const bool ok = brute_force ? orc::make_cell_all( t, k, lo, hi, c ) : orc::make_cell( t, k, lo, hi, c, nullptr );
+    if ( ! ok ) return 0;
+    if ( c.nb > cap ) return -c.nb;
+    for ( int q = 0; q < c.nb; ++q ) {
+        xy[ 2 * q ] = c.x[ size_t( q ) ] + t.px[ size_t( k ) ];
+        xy[ 2 * q + 1 ] = c.y[ size_t( q ) ] + t.py[ size_t( k ) ];
+        ids[ q ] = c.id[ size_t( q ) ] >= 0 ? t.order[ size_t( c.id[ size_t( q ) ] ) ] : c.id[ size_t( q ) ];
+    }
+    return c.nb;
+}
+
+/// the Jacobi-PCG alone on the Laplacian of facets `( fi, fj, fc )`: `L d = b`, `d[ 0 ] = 0`
+int oracle_cg( I64 n, I64 nf, const I64 *fi, const I64 *fj, const double *fc, const double *b, double tol, int maxit, double *d ) {
+    std::vector<orc::Facet> fa( sz( nf ) );
+    for ( I64 q = 0; q < nf; ++q ) fa[ size_t( q ) ] = orc::Facet{ fi[ q ], fj[ q ], fc[ q ] };
+    orc::Laplacian L;
+    L.assemble( n, fa );
+    std::vector<double> B( b, b + n ), D;
+    orc::LinStats st;
+    const bool ok = orc::cg_solve( L, B, D, tol, maxit, st );
+    std::memcpy( d, D.data(), sizeof( double ) * size_t( n ) );
+    return ok ? st.nb_iter : -1;
+}
+
+/// THE SOLVE ( `sdotplan/Solve.h:111-371`, trials, no aggregation, no continuation ).
+/// `opts`: tol_abs, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol,
+/// lin_maxit, keep_start ( 12 doubles ). `start_box`: flag, lo[ 2 ], hi[ 2 ].
+/// Outputs: `weights[ n ]`, `stats[ 31 ]`, `hist[ ( maxit + 1 ) * 8 ]` ( returns the row count ),
+/// `masses[ n ]`, `bary[ n * 2 ]`, `cost[ 1 ]`.
+int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w0, const double *lo, const double *hi, int kind,
+                  double value, int nx, int ny, const double *geo, const double *img, const double *opts, const double *start_box,
+                  double *weights, double *stats, double *hist, double *masses, double *bary, double *cost ) {
+    using namespace orc;
+    const double t_begin = now();
+    Options o;
+    o.tol_abs = opts[ 0 ]; o.tol_rel = opts[ 1 ]; o.maxit = int( opts[ 2 ] ); o.max_backtracks = int( opts[ 3 ] );
+    o.t_min = opts[ 4 ]; o.mult_ok = opts[ 5 ]; o.residual = int( opts[ 6 ] ); o.power = opts[ 7 ]; o.switch_residual = opts[ 8 ];
+    o.lin_tol = opts[ 9 ]; o.lin_maxit = int( opts[ 10 ] ); o.keep_start = int( opts[ 11 ] );
+    Tree t;
+    build_tree( t, n, pos, 10 );
+    Sweep sw( t, lo, hi, make_density( kind, value, nx, ny, geo, img ) );
+    Solver nt( sw, o );
+    const int cap_steps = o.maxit + 1;
+    int nb_steps = 0;
+    nt.after_step = [&]( int it, double tt, int nb_evals ) {
+        if ( nb_steps >= cap_steps ) return;
+        double mn = nt.a.empty() ? 0 : nt.a[ 0 ], mx = 0, l2 = 0;
+        for ( I64 i = 0; i < n; ++i ) {
+            mn = std::min( mn, nt.a[ size_t( i ) ] );
+            mx = std::max( mx, std::fabs( nt.a[ size_t( i ) ] - nt.nu[ size_t( i ) ] ) );
+            l2 += ( nt.a[ size_t( i ) ] - nt.nu[ size_t( i ) ] ) * ( nt.a[ size_t( i ) ] - nt.nu[ size_t( i ) ] );
+        }
+        double *r = hist + size_t( nb_steps ) * NB_HIST;
+        r[ H_STEP ] = it; r[ H_T ] = tt; r[ H_RESIDUAL_L2 ] = std::sqrt( l2 ); r[ H_MIN_MASS ] = mn;
+        r[ H_MAX_RESIDUAL ] = mx; r[ H_NB_DIAG ] = sw.nb_diag; r[ H_NB_EVALS ] = nb_evals; r[ H_S ] = 0;
+        ++nb_steps;
+    };
+    std::vector<double> nu( nu_in, nu_in + n ), w( sz( n ), 0.0 );
+    bool given = false;
+    if ( w0 ) for ( I64 i = 0; i < n; ++i ) { w[ size_t( i ) ] = w0[ i ]; given |= w[ size_t( i ) ] != 0; }
+    int start = given ? 0 : 1;
+    const double gauge = w[ 0 ];
+    for ( auto &v : w ) v -= gauge;
+    std::vector<double> a;
+    std::vector<Facet> fa;
+    nt.measures_and_facets( w, a, fa );
+    double nu_min = nu[ 0 ];
+    for ( double v : nu ) nu_min = std::min( nu_min, v );
+    auto min_mass = []( const std::vector<double> &m ) { double r = m.empty() ? 0 : m[ 0 ]; for ( double v : m ) r = std::min( r, v ); return r; };
+    if ( given && ! o.keep_start && min_mass( a ) < 1e-3 * nu_min ) {
+        std::vector<double> wz( sz( n ), 0.0 ), az;
+        std::vector<Facet> fz;
+        nt.measures_and_facets( wz, az, fz );
+        if ( min_mass( az ) > min_mass( a ) ) { w.swap( wz ); a.swap( az ); fa.swap( fz ); start = 1; }
+        else sw.set_weights( w );
+    }
+    if ( min_mass( a ) <= 0 && ! ( given && o.keep_start ) ) {
+        double blo[ 2 ] = { lo[ 0 ], lo[ 1 ] }, bhi[ 2 ] = { hi[ 0 ], hi[ 1 ] };
+        if ( start_box && start_box[ 0 ] > 0 ) { blo[ 0 ] = start_box[ 1 ]; blo[ 1 ] = start_box[ 2 ]; bhi[ 0 ] = start_box[ 3 ]; bhi[ 1 ] = start_box[ 4 ]; }
+        std::vector<double> w1, a1;
+        std::vector<Facet> f1;
+        similarity( t, pos, blo, bhi, w1 );
+        nt.measures_and_facets( w1, a1, f1 );
+        if ( min_mass( a1 ) > min_mass( a ) ) { w.swap( w1 ); a.swap( a1 ); fa.swap( f1 ); start = 2; }
+        else sw.set_weights( w );
+    }
+    const double min_start_mass = min_mass( a );
+    double domain_mass = 0, nu_mass = 0;
+    for ( I64 i = 0; i < n; ++i ) { domain_mass += a[ size_t( i ) ]; nu_mass += nu[ size_t( i ) ]; }
+    std::vector<double> nu_s = nu;
+    if ( domain_mass > 0 && nu_mass > 0 && domain_mass != nu_mass )
+        for ( auto &v : nu_s ) v *= domain_mass / nu_mass;
+    nt.nu = nu_s;
+    nt.a = a;
+    nt.fa = fa;
+    nt.solves( w );
+    w = nt.w;
+    a = nt.a;
+    for ( I64 i = 0; i < n; ++i ) { weights[ i ] = w[ size_t( i ) ]; masses[ i ] = a[ size_t( i ) ]; }
+    const int nb_diag_descent = sw.nb_diag;
+    {
+        sw.set_weights( w );
+        std::vector<double> bc;
+        double c = 0;
+        sw.moments( bc, c );
+        std::memcpy( bary, bc.data(), sizeof( double ) * size_t( n ) * 2 );
+        *cost = c;
+    }
+    for ( int q = 0; q < NB_STATS; ++q ) stats[ q ] = 0;
+    stats[ STATUS ] = nt.status;
+    stats[ RESIDUAL ] = nt.residual;
+    stats[ RESIDUAL0 ] = nt.residual0;
+    stats[ NB_ITER ] = nt.nb_iter;
+    stats[ NB_DIAG ] = nb_diag_descent;
+    stats[ NB_BACKTRACKS ] = nt.nb_backtracks;
+    stats[ T_MAJORANT ] = sw.t_majorant;
+    stats[ T_DIAG ] = sw.t_diag;
+    stats[ T_ASM ] = nt.t_asm;
+    stats[ T_LIN ] = nt.lst.t;
+    stats[ EPS ] = nt.eps;
+    stats[ DOMAIN_MASS ] = domain_mass;
+    stats[ LIN_NB_ITER ] = nt.lst.nb_iter;
+    stats[ LIN_WORST ] = nt.lst.worst;
+    stats[ START ] = start;
+    stats[ T_TOTAL ] = now() - t_begin;
+    stats[ NB_CONTINUATION_STEPS ] = 1;
+    stats[ MIN_START_MASS ] = min_start_mass;
+    stats[ IT_SWITCH ] = nt.it_switch;
+    stats[ RESIDUAL_FULL ] = nt.residual;
+    return nb_steps;
+}
+
+} // extern "C"

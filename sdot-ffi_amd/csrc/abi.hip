@@ -1,0 +1,284 @@
+// The extern "C" entry points declared in include/sdot_amd.h: argument checks, the error record of
+// the calling thread, and the conversions between the reference's layouts ( int64 indices, tree in
+// preorder ) and the library's own.
+
+#include "../../include/sdot_amd.h"
+#include "solver.h"
+
+#include <cmath>
+
+namespace sdot_amd {
+
+Arena &arena() {
+    static Arena a[ 16 ];
+    int dev = 0;
+    if ( hipGetDevice( &dev ) != hipSuccess || dev < 0 || dev >= 16 ) dev = 0;
+    return a[ dev ];
+}
+
+template<class T>
+T *pinned( const std::string &name, size_t count ) {
+    static std::mutex mtx;
+    static std::map<std::string, std::pair<void *, size_t>> bufs;
+    std::lock_guard<std::mutex> lk( mtx );
+    auto &b = bufs[ name ];
+    const size_t bytes = std::max<size_t>( count * sizeof( T ), 64 );
+    if ( b.second < bytes ) {
+        if ( b.first ) (void) hipHostFree( b.first );
+        b.first = nullptr;
+        SDOT_HIP( hipHostMalloc( &b.first, bytes, hipHostMallocDefault ) );
+        b.second = bytes;
+    }
+    return reinterpret_cast<T *>( b.first );
+}
+template double *pinned<double>( const std::string &, size_t );
+template unsigned *pinned<unsigned>( const std::string &, size_t );
+
+namespace {
+
+struct LastError {
+    int kind = 0, code = 0;
+    long long value = 0;
+    std::string msg;
+};
+thread_local LastError g_err;
+
+template<class F>
+int guarded( F &&f ) {
+    try {
+        f();
+        g_err = LastError{};
+        return 0;
+    } catch ( const Failure &e ) {
+        g_err.kind = e.kind; g_err.code = e.code; g_err.value = e.value; g_err.msg = e.what();
+        return e.kind;
+    } catch ( const std::exception &e ) {
+        g_err.kind = 1; g_err.code = 0; g_err.value = 0; g_err.msg = e.what();
+        return 1;
+    }
+}
+
+Density make_density( int kind, double rho, const double *img, int nx, int ny, const double *geo ) {
+    Density d;
+    d.kind = kind;
+    d.rho = rho;
+    if ( kind == SDOT_AMD_IMAGE ) {
+        SDOT_CHECK( img != nullptr && nx > 0 && ny > 0, "an image density needs its values and a positive shape" );
+        SDOT_CHECK( geo[ 2 ] > 0 && geo[ 3 ] > 0, "an image density needs positive pixel sizes" );
+        d.img = img; d.nx = nx; d.ny = ny;
+        d.x0 = geo[ 0 ]; d.y0 = geo[ 1 ]; d.hx = geo[ 2 ]; d.hy = geo[ 3 ];
+    } else
+        SDOT_CHECK( kind == SDOT_AMD_CONSTANT, "unknown density kind" );
+    return d;
+}
+
+void check_box( const double *box ) {
+    SDOT_CHECK( box != nullptr, "the domain box is required" );
+    SDOT_CHECK( box[ 2 ] > box[ 0 ] && box[ 3 ] > box[ 1 ], "the domain box is empty" );
+}
+
+/// heap rank -> preorder node, and each preorder node's height
+void preorder_tables( int depth, std::vector<int> &pre, std::vector<int> &height ) {
+    const I64 N = ( I64( 1 ) << depth ) - 1;
+    pre.assign( size_t( N ), 0 );
+    height.assign( size_t( N ), 0 );
+    for ( int level = 0; level + 1 < depth; ++level ) {
+        const int h = depth - level;
+        for ( I64 i = ( I64( 1 ) << level ) - 1; i < ( I64( 1 ) << ( level + 1 ) ) - 1; ++i ) {
+            pre[ size_t( 2 * i + 1 ) ] = pre[ size_t( i ) ] + 1;
+            pre[ size_t( 2 * i + 2 ) ] = pre[ size_t( i ) ] + int( I64( 1 ) << ( h - 1 ) );
+        }
+    }
+    for ( int level = 0; level < depth; ++level )
+        for ( I64 i = ( I64( 1 ) << level ) - 1; i < ( I64( 1 ) << ( level + 1 ) ) - 1; ++i ) height[ size_t( pre[ size_t( i ) ] ) ] = depth - level;
+}
+
+__global__ void k_i32_to_i64( I64 n, const int *a, long long *b ) {
+    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( i < n ) b[ i ] = a[ i ];
+}
+__global__ void k_i64_to_i32( I64 n, const long long *a, int *b ) {
+    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( i < n ) b[ i ] = int( a[ i ] );
+}
+__global__ void k_seeds_from( I64 n, const double *pos, const double *w, double4 *seeds ) {
+    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( i < n ) seeds[ i ] = make_double4( pos[ 2 * i ], pos[ 2 * i + 1 ], w[ i ], 0.0 );
+}
+__global__ void k_majorants_out( I64 N, const NodeRec *nodes, double *wa, double *wb ) {
+    const I64 m = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( m >= N ) return;
+    wa[ 2 * m ] = nodes[ m ].wa_x;
+    wa[ 2 * m + 1 ] = nodes[ m ].wa_y;
+    wb[ m ] = nodes[ m ].wb;
+}
+__global__ void k_csr_from( I64 n, const long long *row, const long long *col, int *row_off, int *row_cnt, int *col32, I64 nnz ) {
+    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( i < n ) { row_off[ i ] = int( row[ i ] ); row_cnt[ i ] = int( row[ i + 1 ] - row[ i ] ); }
+    if ( i < nnz ) col32[ i ] = int( col[ i ] );
+}
+
+} // namespace
+} // namespace sdot_amd
+
+using namespace sdot_amd;
+
+extern "C" {
+
+int sdot_amd_abi_version( void ) { return SDOT_AMD_ABI_VERSION; }
+
+const char *sdot_amd_last_error( int *kind, int *code, long long *value ) {
+    if ( kind ) *kind = g_err.kind;
+    if ( code ) *code = g_err.code;
+    if ( value ) *value = g_err.value;
+    return g_err.msg.c_str();
+}
+
+long long sdot_amd_device_bytes( void ) { return (long long) arena().bytes_held(); }
+
+void sdot_amd_release( void ) {
+    (void) hipDeviceSynchronize();
+    arena().release();
+}
+
+int sdot_amd_measures_2d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                          double rho, const double *img, int nx, int ny, const double *img_geo, double *masses, long long *facet_i,
+                          long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets, unsigned long long *events ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && masses, "measures: positions and masses are required" );
+        check_box( box );
+        SDOT_CHECK( ! facet_i || ( facet_j && facet_c ), "measures: facet_i, facet_j and facet_c go together" );
+        const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
+        measures_2d( hipStream_t( stream ), n, positions, weights, box, dn, masses, facet_i, facet_j, facet_c, facet_cap, nb_facets, events );
+    } );
+}
+
+int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const double *nu, const double *w0, const long long *dups,
+                       long long nb_dup_values, const double *options, const double *box, int density_kind, double rho, const double *img,
+                       double *weights, double *history, long long *nb_steps, double *stats, double *cell_masses, double *barycenters,
+                       double *cost, long long *clusters ) {
+    (void) dups;
+    (void) nb_dup_values;
+    return guarded( [&] {
+        SDOT_CHECK( n >= 2 && positions && nu && options && weights && cell_masses && barycenters, "solve: a required array is null" );
+        check_box( box );
+        SDOT_CHECK( options[ O_MAXIT ] >= 0, "solve: negative maxit" );
+        SolveInputs in;
+        in.n = n;
+        in.positions = positions;
+        in.nu = nu;
+        in.w0 = w0;
+        in.options = options;
+        for ( int c = 0; c < 4; ++c ) in.box[ c ] = box[ c ];
+        const double geo[ 4 ] = { options[ O_IMG_X0 ], options[ O_IMG_Y0 ], options[ O_IMG_HX ], options[ O_IMG_HY ] };
+        in.dens = make_density( density_kind, rho, img, int( options[ O_IMG_NX ] ), int( options[ O_IMG_NY ] ), geo );
+        SolveOutputs out;
+        out.weights = weights;
+        out.cell_masses = cell_masses;
+        out.barycenters = barycenters;
+        out.clusters = clusters;
+        out.history = history;
+        out.nb_steps = nb_steps;
+        out.stats = stats;
+        out.cost = cost;
+        solve_2d( in, out, hipStream_t( stream ) );
+    } );
+}
+
+long long sdot_amd_tree_nodes( long long n, int leaf ) { return ( 1ll << max_depth_for( n, leaf ) ) - 1; }
+
+int sdot_amd_build_tree_2d( void *stream, long long n, const double *positions, int leaf, long long *seed_indices, long long *rank_of,
+                            long long *node_begin, long long *node_end, long long *node_left, long long *node_right, double *node_box ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && leaf >= 1, "build_tree: positions, n >= 1 and leaf >= 1 are required" );
+        hipStream_t st = hipStream_t( stream );
+        std::vector<double> pos( size_t( n ) * 2 );
+        SDOT_HIP( hipMemcpyAsync( pos.data(), positions, sizeof( double ) * size_t( n ) * 2, hipMemcpyDeviceToHost, st ) );
+        SDOT_HIP( hipStreamSynchronize( st ) );
+        DevTree t;
+        build_tree( t, n, pos.data(), leaf, st, "abi_tree" );
+        const I64 N = t.nb_nodes;
+        std::vector<int> pre, height;
+        preorder_tables( t.depth, pre, height );
+        std::vector<long long> lft( ( size_t( N ) ) ), rgt( ( size_t( N ) ) );
+        for ( I64 m = 0; m < N; ++m ) {
+            const int h = height[ size_t( m ) ];
+            lft[ size_t( m ) ] = h > 1 ? m + 1 : -1;
+            rgt[ size_t( m ) ] = h > 1 ? m + ( 1ll << ( h - 1 ) ) : -1;
+        }
+        if ( seed_indices ) k_i32_to_i64<<<blocks_for( n, 256 ), 256, 0, st>>>( n, t.order, seed_indices );
+        if ( rank_of ) k_i32_to_i64<<<blocks_for( n, 256 ), 256, 0, st>>>( n, t.rank_of, rank_of );
+        if ( node_begin ) k_i32_to_i64<<<blocks_for( N, 256 ), 256, 0, st>>>( N, t.node_beg, node_begin );
+        if ( node_end ) k_i32_to_i64<<<blocks_for( N, 256 ), 256, 0, st>>>( N, t.node_end, node_end );
+        check_launch( "k_i32_to_i64" );
+        if ( node_left ) SDOT_HIP( hipMemcpyAsync( node_left, lft.data(), sizeof( long long ) * size_t( N ), hipMemcpyHostToDevice, st ) );
+        if ( node_right ) SDOT_HIP( hipMemcpyAsync( node_right, rgt.data(), sizeof( long long ) * size_t( N ), hipMemcpyHostToDevice, st ) );
+        if ( node_box ) SDOT_HIP( hipMemcpyAsync( node_box, t.node_box, sizeof( double ) * size_t( N ) * 4, hipMemcpyDeviceToDevice, st ) );
+        SDOT_HIP( hipStreamSynchronize( st ) );
+    } );
+}
+
+int sdot_amd_refresh_majorants_2d( void *stream, long long n, long long nb_nodes, const double *node_box, const long long *node_begin,
+                                   const long long *node_end, const double *sorted_positions, const double *sorted_weights, double *node_wa,
+                                   double *node_wb ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && node_box && node_begin && node_end && sorted_positions && sorted_weights && node_wa && node_wb,
+                    "refresh_majorants: a required array is null" );
+        int depth = 0;
+        while ( ( 1ll << depth ) - 1 < nb_nodes ) ++depth;
+        SDOT_CHECK( ( 1ll << depth ) - 1 == nb_nodes, "refresh_majorants: nb_nodes is not 2^depth - 1" );
+        hipStream_t st = hipStream_t( stream );
+        Arena &A = arena();
+        DevTree t;
+        t.n = n;
+        t.nb_nodes = nb_nodes;
+        t.depth = depth;
+        t.nodes = A.get<NodeRec>( "rm.nodes", size_t( nb_nodes ) );
+        t.node_box = const_cast<double *>( node_box );
+        t.node_beg = A.get<int>( "rm.beg", size_t( nb_nodes ) );
+        t.node_end = A.get<int>( "rm.end", size_t( nb_nodes ) );
+        t.lvl_nodes = A.get<int>( "rm.lvl", size_t( nb_nodes ) );
+        t.seeds = A.get<double4>( "rm.seeds", size_t( n ) );
+        t.stat = A.get<double>( "rm.stat", size_t( nb_nodes ) * 11 );
+        t.fit = A.get<double>( "rm.fit", size_t( nb_nodes ) * 2 );
+        t.rkeys = A.get<unsigned long long>( "rm.rkeys", size_t( nb_nodes ) * 3 );
+        std::vector<int> pre, height;
+        preorder_tables( depth, pre, height );
+        SDOT_HIP( hipMemcpyAsync( t.lvl_nodes, pre.data(), sizeof( int ) * size_t( nb_nodes ), hipMemcpyHostToDevice, st ) );
+        k_i64_to_i32<<<blocks_for( nb_nodes, 256 ), 256, 0, st>>>( nb_nodes, node_begin, t.node_beg );
+        k_i64_to_i32<<<blocks_for( nb_nodes, 256 ), 256, 0, st>>>( nb_nodes, node_end, t.node_end );
+        k_seeds_from<<<blocks_for( n, 256 ), 256, 0, st>>>( n, sorted_positions, sorted_weights, t.seeds );
+        check_launch( "refresh_majorants inputs" );
+        refresh_majorants( t, st );
+        k_majorants_out<<<blocks_for( nb_nodes, 256 ), 256, 0, st>>>( nb_nodes, t.nodes, node_wa, node_wb );
+        check_launch( "k_majorants_out" );
+        SDOT_HIP( hipStreamSynchronize( st ) );
+    } );
+}
+
+int sdot_amd_linear_solves_2d( void *stream, long long n, const long long *row, const long long *col, const double *val, const double *dia,
+                               const double *rhs, long long k, const double *options, double *solution, double *iterations ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 2 && row && col && val && dia && rhs && solution && k >= 1, "linear_solves: a required array is null" );
+        hipStream_t st = hipStream_t( stream );
+        long long nnz = 0;
+        SDOT_HIP( hipMemcpyAsync( &nnz, row + n, sizeof( long long ), hipMemcpyDeviceToHost, st ) );
+        SDOT_HIP( hipStreamSynchronize( st ) );
+        Arena &A = arena();
+        int *row_off = A.get<int>( "ls.row_off", size_t( n ) ), *row_cnt = A.get<int>( "ls.row_cnt", size_t( n ) );
+        int *col32 = A.get<int>( "ls.col", size_t( std::max<long long>( nnz, 1 ) ) );
+        k_csr_from<<<blocks_for( std::max<long long>( n, nnz ), 256 ), 256, 0, st>>>( n, row, col, row_off, row_cnt, col32, nnz );
+        check_launch( "k_csr_from" );
+        Csr L;
+        L.n = n; L.row_off = row_off; L.row_cnt = row_cnt; L.col = col32; L.val = val; L.dia = dia;
+        const double tol = options && options[ 3 ] > 0 ? options[ 3 ] : 1e-6;
+        for ( long long j = 0; j < k; ++j ) {
+            CgStats cs;
+            const int its = cg_solve( L, 0, rhs + j * n, solution + j * n, tol, 20000, st, cs );
+            if ( iterations ) iterations[ j ] = its;
+        }
+        SDOT_HIP( hipStreamSynchronize( st ) );
+    } );
+}
+
+} // extern "C"

@@ -1,0 +1,553 @@
+// The cells of a 2D power diagram on CDNA4.
+//
+// ONE LANE PER CELL, the lanes of a wave holding consecutive tree ranks ( neighbours in space, so the
+// 64 walks of a wave touch the same nodes ). A cell is a convex polygon kept in LDS as three arrays
+// strided by the block ( `x[ i * BLOCK + lane ]` ), in the seed's frame ( a vertex is `x - p0` ).
+// Reference semantics followed here ( paths under `src/sdot/include/sdot/` ):
+//   * the cut: `cell/Local2.h:206-298` ( in place, the outside a single cyclic run, intersections
+//     anchored on the inside vertex ), the several-runs rule `cell/Local2.h:300-315`;
+//   * the walk: `cell/Providers.h:167-236` ( depth first, nearest child first, a node tested when it
+//     is reached ), the test `cell/Pruning.h:46-76` ( exact per-axis clamp of the free minimum );
+//   * the bisector in the seed's frame ( `gpu/Cell2D.cuh:139-160`, `cell/Plane.h:39` );
+//   * the facets `c_kj = int_facet rho / ( 2 |p_k - p_j| )` ( `sdotplan/Sweep.h:64-128` ).
+// The DFS keeps no stack: the heights on the current path carry two bits each ( a sibling is still
+// pending; the path went left ), and a pop climbs to the lowest pending height -- the node indices
+// are arithmetic in the preorder tree ( left `m + 1`, right `m + 2^( h - 1 )` ).
+// An image density is integrated on the cell's edges ( Green: `mass = loop_integral F dy`,
+// `F( x, y ) = int_{x_ref}^x rho( s, y ) ds`, exact piece by piece since `F` is linear in `x` inside a
+// pixel ), so the cell is never cut by the pixel grid.
+// Cells that outgrow `CAP1` LDS slots are listed and redone by a second pass with global slots.
+// Arithmetic is written with explicit `fma` and no contraction, the same expressions as the
+// oracle's, so a constant density gives the oracle's masses to the bit on the same tree.
+
+#include "cells.h"
+
+#pragma clang fp contract( off )
+
+namespace sdot_amd {
+
+namespace {
+
+enum { CUT_UNCHANGED = 0, CUT_DONE = 1, CUT_EMPTY = 2, CUT_NO_ROOM = 3 };
+constexpr int DOMAIN_ID = -1;
+
+struct Ev {
+    unsigned side = 0, prune = 0, newv = 0, visits = 0, cuts = 0;
+};
+
+/// a polygon's arrays: `STRIDE` lanes apart ( LDS ) or contiguous ( a global slot )
+template<int STRIDE>
+struct Store {
+    double *x, *y;
+    int    *id;
+    __device__ __forceinline__ double &X( int i ) const { return x[ i * STRIDE ]; }
+    __device__ __forceinline__ double &Y( int i ) const { return y[ i * STRIDE ]; }
+    __device__ __forceinline__ int    &I( int i ) const { return id[ i * STRIDE ]; }
+};
+
+template<class S>
+__device__ __forceinline__ double side( const S &c, int i, double dx, double dy, double off ) {
+    return fma( dx, c.X( i ), fma( dy, c.Y( i ), -off ) );
+}
+
+/// the outside run holding the largest `s`, when the signs come in several runs
+template<class S>
+__device__ void main_run( const S &c, int nb, double dx, double dy, double off, int &i1, int &nb_out ) {
+    int a = 0;
+    double sa = side( c, 0, dx, dy, off );
+    for ( int i = 1; i < nb; ++i ) {
+        const double si = side( c, i, dx, dy, off );
+        if ( si > sa ) { a = i; sa = si; }
+    }
+    i1 = a;
+    for ( int q = a ? a - 1 : nb - 1; side( c, q, dx, dy, off ) > 0; q = q ? q - 1 : nb - 1 ) i1 = q;
+    nb_out = 1;
+    for ( int q = a + 1 < nb ? a + 1 : 0; side( c, q, dx, dy, off ) > 0; q = q + 1 < nb ? q + 1 : 0 ) ++nb_out;
+    nb_out += ( a - i1 + nb ) % nb;
+}
+
+/// the cut by `dx . v <= off`, in place; `CUT_NO_ROOM` leaves the cell intact
+template<class S>
+__device__ int cut( const S &c, int &nb, int cap, double dx, double dy, double off, int pid, Ev &ev ) {
+    int nb_out = 0;
+    for ( int i = 0; i < nb; ++i ) nb_out += side( c, i, dx, dy, off ) > 0;
+    ev.side += nb;
+    if ( nb_out == 0 ) return CUT_UNCHANGED;
+    if ( nb_out == nb ) { nb = 0; return CUT_EMPTY; }
+    int i1 = 0;
+    {
+        bool prev = side( c, nb - 1, dx, dy, off ) > 0;
+        for ( int i = 0; i < nb; ++i ) {
+            const bool o = side( c, i, dx, dy, off ) > 0;
+            if ( o && ! prev ) { i1 = i; break; }
+            prev = o;
+        }
+    }
+    for ( int o = 1, i = i1 + 1; o < nb_out; ++o, ++i ) {
+        if ( i == nb ) i = 0;
+        if ( ! ( side( c, i, dx, dy, off ) > 0 ) ) { main_run( c, nb, dx, dy, off, i1, nb_out ); break; }
+    }
+    const int nb_in = nb - nb_out, new_nb = nb_in + 2;
+    if ( new_nb > cap ) return CUT_NO_ROOM;
+    const int j0 = ( i1 + nb - 1 ) % nb, j2 = ( i1 + nb_out - 1 ) % nb, j3 = ( j2 + 1 ) % nb;
+    const double s0 = side( c, j0, dx, dy, off ), s1 = side( c, i1, dx, dy, off );
+    const double s2 = side( c, j2, dx, dy, off ), s3 = side( c, j3, dx, dy, off );
+    const double ta = s0 / ( s0 - s1 );
+    const double pax = fma( c.X( i1 ) - c.X( j0 ), ta, c.X( j0 ) );
+    const double pay = fma( c.Y( i1 ) - c.Y( j0 ), ta, c.Y( j0 ) );
+    const double tb = s3 / ( s3 - s2 );
+    const double pbx = fma( c.X( j2 ) - c.X( j3 ), tb, c.X( j3 ) );
+    const double pby = fma( c.Y( j2 ) - c.Y( j3 ), tb, c.Y( j3 ) );
+    const int bid = c.I( j2 );
+    auto move = [&]( int d, int s ) { c.X( d ) = c.X( s ); c.Y( d ) = c.Y( s ); c.I( d ) = c.I( s ); };
+    auto put = [&]( int d, double x, double y, int id ) { c.X( d ) = x; c.Y( d ) = y; c.I( d ) = id; };
+    if ( i1 <= j2 ) {
+        if ( nb_out == 1 ) {
+            for ( int i = nb; i > i1 + 1; --i ) move( i, i - 1 );
+        } else if ( nb_out > 2 ) {
+            const int gap = nb_out - 2;
+            for ( int i = j2 + 1; i < nb; ++i ) move( i - gap, i );
+        }
+        put( i1, pax, pay, pid );
+        put( i1 + 1, pbx, pby, bid );
+    } else {
+        if ( j3 >= 2 ) for ( int o = 0; o < nb_in; ++o ) move( 2 + o, j3 + o );
+        else           for ( int o = nb_in - 1; o >= 0; --o ) move( 2 + o, j3 + o );
+        put( 0, pax, pay, pid );
+        put( 1, pbx, pby, bid );
+    }
+    nb = new_nb;
+    ev.newv += 2;
+    ev.cuts += 1;
+    return CUT_DONE;
+}
+
+/// can a seed of node `r` still cut the cell of `( p0, w0 )`? ( exact: per-axis clamp )
+template<class S>
+__device__ __forceinline__ bool can_cut( const S &c, int nb, const NodeRec &r, double p0x, double p0y, double w0, Ev &ev ) {
+    ev.visits += 1;
+    const double hax = r.wa_x / 2, hay = r.wa_y / 2, cb = w0 - r.wb;
+    for ( int i = 0; i < nb; ++i ) {
+        const double vx = c.X( i ), vy = c.Y( i );
+        const double X = vx + p0x, Y = vy + p0y;
+        double yx = X + hax, yy = Y + hay;
+        yx = yx < r.lo_x ? r.lo_x : ( yx > r.hi_x ? r.hi_x : yx );
+        yy = yy < r.lo_y ? r.lo_y : ( yy > r.hi_y ? r.hi_y : yy );
+        const double ux = yx - X, uy = yy - Y;
+        const double s = cb + ux * ux - vx * vx - r.wa_x * yx + uy * uy - vy * vy - r.wa_y * yy;
+        if ( s <= 0 ) { ev.prune += i + 1; return true; }
+    }
+    ev.prune += nb;
+    return false;
+}
+
+__device__ __forceinline__ double proximity( const NodeRec *nodes, int m, double p0x, double p0y ) {
+    const double4 b = *reinterpret_cast<const double4 *>( &nodes[ m ] );
+    double e0 = fmax( b.x - p0x, p0x - b.z ), e1 = fmax( b.y - p0y, p0y - b.w );
+    e0 = e0 > 0 ? e0 : 0;
+    e1 = e1 > 0 ? e1 : 0;
+    return e0 * e0 + e1 * e1;
+}
+
+struct Walk {
+    const NodeRec *nodes;
+    const double4 *seeds;
+    int depth;
+};
+
+/// builds the cell of rank `k` in `c`; returns the vertex count ( 0: empty ), or -1 when the slots
+/// were too few ( the cell is then left for the next pass )
+template<class S>
+__device__ int make_cell( const Walk &wk, int k, const double *lo, const double *hi, const S &c, int cap, Ev &ev ) {
+    const double4 p = wk.seeds[ k ];
+    const double p0x = p.x, p0y = p.y, w0 = p.z;
+    c.X( 0 ) = lo[ 0 ] - p0x; c.Y( 0 ) = lo[ 1 ] - p0y;
+    c.X( 1 ) = hi[ 0 ] - p0x; c.Y( 1 ) = lo[ 1 ] - p0y;
+    c.X( 2 ) = hi[ 0 ] - p0x; c.Y( 2 ) = hi[ 1 ] - p0y;
+    c.X( 3 ) = lo[ 0 ] - p0x; c.Y( 3 ) = hi[ 1 ] - p0y;
+    for ( int i = 0; i < 4; ++i ) c.I( i ) = DOMAIN_ID;
+    int nb = 4;
+    int m = 0, h = wk.depth;
+    unsigned long long pend = 0, left = 0;
+    for ( ;; ) {
+        const NodeRec r = wk.nodes[ m ];
+        if ( r.beg < r.end && can_cut( c, nb, r, p0x, p0y, w0, ev ) ) {
+            if ( h <= 1 ) {
+                for ( int q = r.beg; q < r.end; ++q ) {
+                    if ( q == k ) continue;
+                    const double4 s = wk.seeds[ q ];
+                    const double dx = s.x - p0x, dy = s.y - p0y;
+                    const double off = ( dx * dx + dy * dy + w0 - s.z ) / 2;
+                    const int res = cut( c, nb, cap, dx, dy, off, q, ev );
+                    if ( res == CUT_EMPTY ) return 0;
+                    if ( res == CUT_NO_ROOM ) return -1;
+                }
+            } else {
+                const int lc = m + 1, rc = m + ( 1 << ( h - 1 ) );
+                const bool go_left = proximity( wk.nodes, lc, p0x, p0y ) <= proximity( wk.nodes, rc, p0x, p0y );
+                const unsigned long long bit = 1ull << ( h - 1 );
+                pend |= bit;
+                left = go_left ? ( left | bit ) : ( left & ~bit );
+                m = go_left ? lc : rc;
+                h -= 1;
+                continue;
+            }
+        }
+        if ( ! pend ) break;
+        const int hp = __ffsll( (long long) pend ) - 1;
+        while ( h < hp ) {
+            m = ( ( left >> h ) & 1 ) ? m - 1 : m - ( 1 << h );
+            ++h;
+        }
+        const unsigned long long bit = 1ull << hp;
+        if ( left & bit ) { m = m - 1 + ( 1 << hp ); left &= ~bit; }
+        else              { m = m + 1 - ( 1 << hp ); left |= bit; }
+        pend &= ~bit;
+    }
+    return nb;
+}
+
+// ---- integrals -------------------------------------------------------------------------------------
+
+template<class S>
+__device__ __forceinline__ double shoelace( const S &c, int nb ) {
+    double s = 0;
+    for ( int i = 0; i < nb; ++i ) {
+        const int j = i + 1 < nb ? i + 1 : 0;
+        s += c.X( i ) * c.Y( j ) - c.X( j ) * c.Y( i );
+    }
+    return s / 2;
+}
+
+/// the pixel grid as seen from one seed: columns `u_i = x0 + i hx - p0x`, rows `v_j = y0 + j hy - p0y`
+struct Grid {
+    const double *img;
+    int nx, ny;
+    double x0, y0, hx, hy;   ///< x0, y0 in the seed's frame
+    int iref;                ///< the column the row integrals start from
+    __device__ double u( int i ) const { return x0 + double( i ) * hx; }
+    __device__ double v( int j ) const { return y0 + double( j ) * hy; }
+    __device__ double val( int i, int j ) const { return img[ size_t( j ) * size_t( nx ) + size_t( i ) ]; }
+    /// `int_{u_iref}^{u} rho( s, row j ) ds` for `u` in column `i`
+    __device__ double F0( int i, int j, double x ) const {
+        double acc = 0;
+        if ( i >= iref ) for ( int q = iref; q < i; ++q ) acc += val( q, j ) * hx;
+        else             for ( int q = i; q < iref; ++q ) acc -= val( q, j ) * hx;
+        return acc + val( i, j ) * ( x - u( i ) );
+    }
+};
+
+/// walks the segment `a -> b` ( seed frame ) across the pixel grid; `f( i, j, ta, tb )` per piece
+template<class F>
+__device__ void grid_walk( const Grid &g, double ax, double ay, double bx, double by, F &&f ) {
+    const double gax = ( ax - g.x0 ) / g.hx, gay = ( ay - g.y0 ) / g.hy;
+    const double gbx = ( bx - g.x0 ) / g.hx, gby = ( by - g.y0 ) / g.hy;
+    const double ddx = gbx - gax, ddy = gby - gay;
+    // the next vertical / horizontal grid line crossed, as a parameter
+    int sx = ddx > 0 ? 1 : ( ddx < 0 ? -1 : 0 ), sy = ddy > 0 ? 1 : ( ddy < 0 ? -1 : 0 );
+    double nxl = sx > 0 ? floor( gax ) + 1 : ceil( gax ) - 1;
+    double nyl = sy > 0 ? floor( gay ) + 1 : ceil( gay ) - 1;
+    double t0 = 0;
+    for ( int guard = 0; guard < 1 << 20; ++guard ) {
+        const double tx = sx ? ( nxl - gax ) / ddx : 2.0;
+        const double ty = sy ? ( nyl - gay ) / ddy : 2.0;
+        double t1 = fmin( fmin( tx, ty ), 1.0 );
+        if ( t1 < t0 ) t1 = t0;
+        const double tm = 0.5 * ( t0 + t1 );
+        int i = int( floor( gax + tm * ddx ) ), j = int( floor( gay + tm * ddy ) );
+        i = i < 0 ? 0 : ( i >= g.nx ? g.nx - 1 : i );
+        j = j < 0 ? 0 : ( j >= g.ny ? g.ny - 1 : j );
+        if ( t1 > t0 ) f( i, j, t0, t1 );
+        if ( t1 >= 1.0 ) break;
+        if ( tx <= t1 ) nxl += sx;
+        if ( ty <= t1 ) nyl += sy;
+        t0 = t1;
+    }
+}
+
+template<class S>
+__device__ void image_mass( const Grid &g, const S &c, int nb, double &mass ) {
+    double acc = 0;
+    for ( int e = 0; e < nb; ++e ) {
+        const int f = e + 1 < nb ? e + 1 : 0;
+        const double ax = c.X( e ), ay = c.Y( e ), bx = c.X( f ), by = c.Y( f );
+        if ( ay == by ) continue;                        // no `dy`: nothing for the mass
+        grid_walk( g, ax, ay, bx, by, [&]( int i, int j, double t0, double t1 ) {
+            const double tm = 0.5 * ( t0 + t1 );
+            const double xm = fma( bx - ax, tm, ax );
+            acc += g.F0( i, j, xm ) * ( ( by - ay ) * ( t1 - t0 ) );
+        } );
+    }
+    mass = acc;
+}
+
+/// `int_edge rho dl` of edge `e`
+template<class S>
+__device__ double image_edge_mass( const Grid &g, const S &c, int nb, int e ) {
+    const int f = e + 1 < nb ? e + 1 : 0;
+    const double ax = c.X( e ), ay = c.Y( e ), bx = c.X( f ), by = c.Y( f );
+    const double len = sqrt( ( bx - ax ) * ( bx - ax ) + ( by - ay ) * ( by - ay ) );
+    double acc = 0;
+    grid_walk( g, ax, ay, bx, by, [&]( int i, int j, double t0, double t1 ) { acc += g.val( i, j ) * ( t1 - t0 ); } );
+    return acc * len;
+}
+
+/// moments of the cell against the image: `m`, `int v rho`, `int |v|^2 rho` ( seed frame ), by Green
+/// with the row integrals of `rho`, `rho s`, `rho s^2`; two Gauss points per piece ( exact: cubic )
+template<class S>
+__device__ void image_moments( const Grid &g, const S &c, int nb, double &m, double &mx, double &my, double &m2 ) {
+    m = mx = my = m2 = 0;
+    const double gq = 0.5 / sqrt( 3.0 );
+    for ( int e = 0; e < nb; ++e ) {
+        const int f = e + 1 < nb ? e + 1 : 0;
+        const double ax = c.X( e ), ay = c.Y( e ), bx = c.X( f ), by = c.Y( f );
+        if ( ay == by ) continue;
+        grid_walk( g, ax, ay, bx, by, [&]( int i, int j, double t0, double t1 ) {
+            // row prefixes at the column's left side
+            double c0 = 0, c1 = 0, c2 = 0;
+            if ( i >= g.iref )
+                for ( int q = g.iref; q < i; ++q ) {
+                    const double v = g.val( q, j ), a = g.u( q ), b = g.u( q + 1 );
+                    c0 += v * ( b - a ); c1 += v * ( b * b - a * a ) / 2; c2 += v * ( b * b * b - a * a * a ) / 3;
+                }
+            else
+                for ( int q = i; q < g.iref; ++q ) {
+                    const double v = g.val( q, j ), a = g.u( q ), b = g.u( q + 1 );
+                    c0 -= v * ( b - a ); c1 -= v * ( b * b - a * a ) / 2; c2 -= v * ( b * b * b - a * a * a ) / 3;
+                }
+            const double v = g.val( i, j ), ui = g.u( i );
+            const double dt = t1 - t0, tm = 0.5 * ( t0 + t1 );
+            const double dy = ( by - ay ) * dt;
+            for ( int s = -1; s <= 1; s += 2 ) {
+                const double t = tm + s * gq * dt;
+                const double x = fma( bx - ax, t, ax ), y = fma( by - ay, t, ay );
+                const double f0 = c0 + v * ( x - ui );
+                const double f1 = c1 + v * ( x * x - ui * ui ) / 2;
+                const double f2 = c2 + v * ( x * x * x - ui * ui * ui ) / 3;
+                m += 0.5 * dy * f0;
+                mx += 0.5 * dy * f1;
+                my += 0.5 * dy * y * f0;
+                m2 += 0.5 * dy * ( f2 + y * y * f0 );
+            }
+        } );
+    }
+}
+
+/// moments of a constant density: the fan from vertex 0 ( `diagram/Ops.h:187-228` )
+template<class S>
+__device__ void const_moments( const S &c, int nb, double rho, double &m, double &mx, double &my, double &m2 ) {
+    m = mx = my = m2 = 0;
+    const double ax = c.X( 0 ), ay = c.Y( 0 );
+    for ( int i = 1; i + 1 < nb; ++i ) {
+        const double bx = c.X( i ), by = c.Y( i ), cx = c.X( i + 1 ), cy = c.Y( i + 1 );
+        const double ar = fabs( ( bx - ax ) * ( cy - ay ) - ( by - ay ) * ( cx - ax ) ) / 2;
+        const double w = rho * ar;
+        const double sx = ax + bx + cx, sy = ay + by + cy;
+        const double sq = ax * ax + ay * ay + bx * bx + by * by + cx * cx + cy * cy;
+        m += w;
+        mx += w * sx / 3;
+        my += w * sy / 3;
+        m2 += w * ( sq + sx * sx + sy * sy ) / 12;
+    }
+}
+
+// ---- wave helpers -------------------------------------------------------------------------------------
+
+/// exclusive prefix sum of `v` over the wave, and the wave's total
+__device__ __forceinline__ unsigned wave_scan( unsigned v, unsigned &total ) {
+    const int lane = threadIdx.x & 63;
+    unsigned x = v;
+    for ( int off = 1; off < 64; off <<= 1 ) {
+        const unsigned y = __shfl_up( x, off, 64 );
+        if ( lane >= off ) x += y;
+    }
+    total = __shfl( x, 63, 64 );
+    return x - v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum64( unsigned long long v ) {
+    for ( int off = 32; off > 0; off >>= 1 ) v += __shfl_down( v, off, 64 );
+    return v;
+}
+
+struct Params {
+    Walk          wk;
+    int           n;
+    double        lo[ 2 ], hi[ 2 ];
+    Density       dn;
+    DiagramOut    out;
+    int           mode;
+};
+
+/// the end of a cell: its mass, and its facets ( one atomic per wave ) or its moments
+template<class S>
+__device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool active, bool wave_full ) {
+    const double4 p = P.wk.seeds[ k < P.n ? k : 0 ];
+    // facets: the live cuts that face a seed
+    unsigned cnt = 0;
+    if ( active && P.mode == MODE_FACETS )
+        for ( int e = 0; e < nb; ++e ) cnt += c.I( e ) >= 0;
+    unsigned total = 0, base = 0, pre = 0;
+    if ( P.mode == MODE_FACETS ) {
+        pre = wave_scan( cnt, total );
+        const int lane = threadIdx.x & 63;
+        if ( lane == 0 && total ) base = atomicAdd( &P.out.counters[ 0 ], total );
+        base = __shfl( base, 0, 64 );
+    }
+    if ( ! active ) return;
+    Grid g;
+    if ( P.dn.kind == 1 ) {
+        g.img = P.dn.img; g.nx = P.dn.nx; g.ny = P.dn.ny; g.hx = P.dn.hx; g.hy = P.dn.hy;
+        g.x0 = P.dn.x0 - p.x; g.y0 = P.dn.y0 - p.y;
+        int ir = int( floor( -g.x0 / g.hx ) );
+        g.iref = ir < 0 ? 0 : ( ir >= g.nx ? g.nx - 1 : ir );
+    }
+    if ( P.mode == MODE_MOMENTS ) {
+        double m = 0, mx = 0, my = 0, m2 = 0;
+        if ( nb > 0 ) {
+            if ( P.dn.kind == 1 ) image_moments( g, c, nb, m, mx, my, m2 );
+            else                  const_moments( c, nb, P.dn.rho, m, mx, my, m2 );
+        }
+        P.out.bary[ 2 * k ] = m > 0 ? p.x + mx / m : p.x;
+        P.out.bary[ 2 * k + 1 ] = m > 0 ? p.y + my / m : p.y;
+        P.out.cost[ k ] = m2;
+        P.out.mass[ k ] = m;
+        return;
+    }
+    double mass = 0;
+    if ( nb > 0 ) {
+        if ( P.dn.kind == 1 ) image_mass( g, c, nb, mass );
+        else                  mass = P.dn.rho * shoelace( c, nb );
+    }
+    P.out.mass[ k ] = mass;
+    if ( P.mode != MODE_FACETS ) return;
+    const I64 at = I64( base ) + pre;
+    const bool fits = I64( base ) + total <= P.out.facet_cap;
+    P.out.row_off[ k ] = int( fits ? at : 0 );
+    P.out.row_cnt[ k ] = int( cnt );
+    double dia = 0;
+    int q = 0;
+    for ( int e = 0; e < nb; ++e ) {
+        const int id = c.I( e );
+        if ( id < 0 ) continue;
+        const int f = e + 1 < nb ? e + 1 : 0;
+        const double ex = c.X( f ) - c.X( e ), ey = c.Y( f ) - c.Y( e );
+        const double fm = P.dn.kind == 1 ? image_edge_mass( g, c, nb, e ) : P.dn.rho * sqrt( ex * ex + ey * ey );
+        const double4 s = P.wk.seeds[ id ];
+        const double dx = s.x - p.x, dy = s.y - p.y;
+        const double d2 = dx * dx + dy * dy;
+        const double cij = d2 > 0 ? fm / ( 2 * sqrt( d2 ) ) : 0.0;
+        if ( fits ) {
+            P.out.col[ at + q ] = id;
+            P.out.val[ at + q ] = cij;
+        }
+        dia += cij;
+        ++q;
+    }
+    P.out.dia[ k ] = dia > 0 ? dia : 1.0;
+}
+
+__device__ __forceinline__ void flush_events( const Params &P, const Ev &ev ) {
+    const unsigned long long a = wave_sum64( ev.side ), b = wave_sum64( ev.prune ), c = wave_sum64( ev.newv );
+    const unsigned long long d = wave_sum64( ev.visits ), e = wave_sum64( ev.cuts );
+    if ( ( threadIdx.x & 63 ) == 0 ) {
+        atomicAdd( &P.out.events[ 0 ], a );
+        atomicAdd( &P.out.events[ 1 ], b );
+        atomicAdd( &P.out.events[ 2 ], c );
+        atomicAdd( &P.out.events[ 3 ], d );
+        atomicAdd( &P.out.events[ 4 ], e );
+    }
+}
+
+} // namespace
+
+/// the first pass: one lane per cell, `CAP1` vertex slots in LDS; the blocks of one XCD take a
+/// contiguous range of ranks ( blocks are dealt round-robin to the 8 XCDs, each with its own L2 )
+__global__ __launch_bounds__( CELL_BLOCK ) void k_cells_pass1( Params P ) {
+    __shared__ double sx[ CAP1 * CELL_BLOCK ], sy[ CAP1 * CELL_BLOCK ];
+    __shared__ int sid[ CAP1 * CELL_BLOCK ];
+    const int nblk = int( gridDim.x ), b = int( blockIdx.x );
+    const int per = nblk / 8, rem = nblk % 8, xcd = b % 8, slot = b / 8;
+    const int lb = xcd < rem ? xcd * ( per + 1 ) + slot : rem * ( per + 1 ) + ( xcd - rem ) * per + slot;
+    const int k = lb * CELL_BLOCK + int( threadIdx.x );
+    const Store<CELL_BLOCK> c{ sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x };
+    Ev ev;
+    int nb = 0;
+    bool active = k < P.n;
+    if ( active ) {
+        nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP1, ev );
+        if ( nb < 0 ) {
+            const unsigned at = atomicAdd( &P.out.counters[ 1 ], 1u );
+            P.out.ovf_list[ at ] = k;
+            active = false;
+        }
+    }
+    finish_cell( P, k, c, nb, active, true );
+    flush_events( P, ev );
+}
+
+/// the second pass: the overflowed cells, `CAP2` slots each in global memory, a fixed grid of
+/// `OVF_SLOTS` lanes striding over the list ( its length only the device knows )
+__global__ __launch_bounds__( 64 ) void k_cells_pass2( Params P, double *gx, double *gy, int *gid ) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const Store<1> c{ gx + size_t( slot ) * CAP2, gy + size_t( slot ) * CAP2, gid + size_t( slot ) * CAP2 };
+    const unsigned count = P.out.counters[ 1 ];
+    const unsigned rounds = ( count + OVF_SLOTS - 1 ) / OVF_SLOTS;
+    Ev ev;
+    for ( unsigned r = 0; r < rounds; ++r ) {
+        const unsigned idx = r * OVF_SLOTS + unsigned( slot );
+        const bool live = idx < count;
+        const int k = live ? P.out.ovf_list[ idx ] : 0;
+        int nb = 0;
+        bool active = live;
+        if ( live ) {
+            nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP2, ev );
+            if ( nb < 0 ) {
+                if ( atomicAdd( &P.out.counters[ 2 ], 1u ) == 0 ) P.out.counters[ 3 ] = unsigned( k );
+                active = false;
+                P.out.mass[ k ] = 0;
+                if ( P.mode == MODE_FACETS ) { P.out.row_cnt[ k ] = 0; P.out.row_off[ k ] = 0; P.out.dia[ k ] = 1; }
+            }
+        }
+        finish_cell( P, k, c, nb, active, false );
+    }
+    flush_events( P, ev );
+}
+
+__global__ void k_reset_counters( unsigned *c ) {
+    if ( threadIdx.x < 4 ) c[ threadIdx.x ] = 0;
+}
+
+void cells_scratch( DiagramOut &out, I64 n, const std::string &tag ) {
+    Arena &A = arena();
+    out.counters = A.get<unsigned>( tag + ".counters", 4 );
+    out.ovf_list = A.get<int>( tag + ".ovf_list", size_t( n ) );
+    if ( ! out.events ) {
+        out.events = A.get<unsigned long long>( "events", 8 );
+        SDOT_HIP( hipMemset( out.events, 0, 8 * sizeof( unsigned long long ) ) );
+    }
+}
+
+void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st ) {
+    Params P;
+    P.wk.nodes = t.nodes;
+    P.wk.seeds = t.seeds;
+    P.wk.depth = t.depth;
+    P.n = int( t.n );
+    P.lo[ 0 ] = lo[ 0 ]; P.lo[ 1 ] = lo[ 1 ]; P.hi[ 0 ] = hi[ 0 ]; P.hi[ 1 ] = hi[ 1 ];
+    P.dn = dn;
+    P.out = out;
+    P.mode = mode;
+    k_reset_counters<<<1, 64, 0, st>>>( out.counters );
+    check_launch( "k_reset_counters" );
+    k_cells_pass1<<<blocks_for( t.n, CELL_BLOCK ), CELL_BLOCK, 0, st>>>( P );
+    check_launch( "k_cells_pass1" );
+    Arena &A = arena();
+    double *gx = A.get<double>( "ovf.x", size_t( OVF_SLOTS ) * CAP2 );
+    double *gy = A.get<double>( "ovf.y", size_t( OVF_SLOTS ) * CAP2 );
+    int *gid = A.get<int>( "ovf.id", size_t( OVF_SLOTS ) * CAP2 );
+    k_cells_pass2<<<OVF_SLOTS / 64, 64, 0, st>>>( P, gx, gy, gid );
+    check_launch( "k_cells_pass2" );
+}
+
+} // namespace sdot_amd

@@ -1,0 +1,163 @@
+// Shared definitions of the MI355X semi-discrete OT library: error plumbing, the device arena,
+// launch helpers and the small device utilities every kernel file uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sdot_amd {
+
+using I64 = long long;
+
+// ---- errors ------------------------------------------------------------------------------------
+
+/// what crosses the C ABI on failure: `kind` 1 = HIP / argument error, 2 = a kernel failure
+/// ( `code` 1: a cell past the vertex limit, `value` its seed ), 3 = capacity, 4 = out of memory
+struct Failure : std::runtime_error {
+    int kind, code;
+    I64 value;
+    Failure( int kind, int code, I64 value, const std::string &msg ) : std::runtime_error( msg ), kind( kind ), code( code ), value( value ) {}
+};
+
+#define SDOT_HIP( expr )                                                                                     \
+    do {                                                                                                     \
+        hipError_t e_ = ( expr );                                                                            \
+        if ( e_ != hipSuccess )                                                                              \
+            throw ::sdot_amd::Failure( 1, int( e_ ), 0, std::string( #expr " -> " ) + hipGetErrorString( e_ ) ); \
+    } while ( 0 )
+
+#define SDOT_CHECK( cond, msg )                                      \
+    do {                                                             \
+        if ( ! ( cond ) ) throw ::sdot_amd::Failure( 1, 0, 0, msg ); \
+    } while ( 0 )
+
+inline void check_launch( const char *name ) {
+    hipError_t e = hipGetLastError();
+    if ( e != hipSuccess )
+        throw Failure( 1, int( e ), 0, std::string( "launch of " ) + name + " -> " + hipGetErrorString( e ) );
+}
+
+// ---- the device arena --------------------------------------------------------------------------
+
+/// Library-owned device memory, grow-only, one named buffer per role: a solve of `n` seeds takes
+/// its buffers once and the next solve of the same size reuses them ( hipMalloc is a
+/// synchronizing call and costs far more than a diagram ). `bytes_held()` is the footprint.
+struct Arena {
+    struct Buf { void *ptr = nullptr; size_t bytes = 0; };
+    std::map<std::string, Buf> bufs;
+    std::mutex mtx;
+
+    template<class T>
+    T *get( const std::string &name, size_t count ) {
+        std::lock_guard<std::mutex> lk( mtx );
+        const size_t bytes = std::max<size_t>( count * sizeof( T ), 256 );
+        Buf &b = bufs[ name ];
+        if ( b.bytes < bytes ) {
+            if ( b.ptr ) SDOT_HIP( hipFree( b.ptr ) );
+            b.ptr = nullptr;
+            b.bytes = 0;
+            hipError_t e = hipMalloc( &b.ptr, bytes );
+            if ( e != hipSuccess ) {
+                (void) hipGetLastError();
+                throw Failure( 4, int( e ), I64( bytes ), "out of device memory for buffer '" + name + "' (" + std::to_string( bytes ) + " bytes)" );
+            }
+            b.bytes = bytes;
+        }
+        return reinterpret_cast<T *>( b.ptr );
+    }
+
+    size_t bytes_held() {
+        std::lock_guard<std::mutex> lk( mtx );
+        size_t s = 0;
+        for ( auto &kv : bufs ) s += kv.second.bytes;
+        return s;
+    }
+
+    void release() {
+        std::lock_guard<std::mutex> lk( mtx );
+        for ( auto &kv : bufs )
+            if ( kv.second.ptr ) (void) hipFree( kv.second.ptr );
+        bufs.clear();
+    }
+};
+
+/// the arena of the current device
+Arena &arena();
+
+/// pinned host staging, grow-only ( readbacks of the few scalars Newton decides on )
+template<class T>
+T *pinned( const std::string &name, size_t count );
+
+// ---- launch shapes -----------------------------------------------------------------------------
+
+constexpr int RED_BLOCKS  = 512;    ///< partial sums of every deterministic reduction ( fixed shape )
+constexpr int RED_THREADS = 256;
+
+inline unsigned blocks_for( I64 n, int threads ) { return unsigned( ( n + threads - 1 ) / threads ); }
+
+// ---- device helpers ----------------------------------------------------------------------------
+
+/// a monotone map from doubles to unsigned 64-bit integers: `a < b` iff `key( a ) < key( b )`,
+/// so that an integer atomic min / max is a double min / max
+__device__ __forceinline__ unsigned long long ordered_key( double x ) {
+    unsigned long long u = __double_as_longlong( x );
+    return ( u >> 63 ) ? ~u : ( u | 0x8000000000000000ull );
+}
+__device__ __forceinline__ double ordered_value( unsigned long long k ) {
+    const unsigned long long u = ( k >> 63 ) ? ( k & 0x7fffffffffffffffull ) : ~k;
+    return __longlong_as_double( u );
+}
+inline unsigned long long ordered_key_host( double x ) {
+    unsigned long long u;
+    std::memcpy( &u, &x, 8 );
+    return ( u >> 63 ) ? ~u : ( u | 0x8000000000000000ull );
+}
+
+/// block-wide sum of a double, deterministic ( fixed tree over the block )
+template<int THREADS>
+__device__ __forceinline__ double block_sum( double v, double *sh ) {
+    sh[ threadIdx.x ] = v;
+    __syncthreads();
+    for ( int s = THREADS / 2; s > 0; s >>= 1 ) {
+        if ( int( threadIdx.x ) < s ) sh[ threadIdx.x ] += sh[ threadIdx.x + s ];
+        __syncthreads();
+    }
+    const double r = sh[ 0 ];
+    __syncthreads();
+    return r;
+}
+
+template<int THREADS>
+__device__ __forceinline__ double block_min( double v, double *sh ) {
+    sh[ threadIdx.x ] = v;
+    __syncthreads();
+    for ( int s = THREADS / 2; s > 0; s >>= 1 ) {
+        if ( int( threadIdx.x ) < s ) sh[ threadIdx.x ] = fmin( sh[ threadIdx.x ], sh[ threadIdx.x + s ] );
+        __syncthreads();
+    }
+    const double r = sh[ 0 ];
+    __syncthreads();
+    return r;
+}
+
+template<int THREADS>
+__device__ __forceinline__ double block_max( double v, double *sh ) {
+    sh[ threadIdx.x ] = v;
+    __syncthreads();
+    for ( int s = THREADS / 2; s > 0; s >>= 1 ) {
+        if ( int( threadIdx.x ) < s ) sh[ threadIdx.x ] = fmax( sh[ threadIdx.x ], sh[ threadIdx.x + s ] );
+        __syncthreads();
+    }
+    const double r = sh[ 0 ];
+    __syncthreads();
+    return r;
+}
+
+} // namespace sdot_amd

@@ -1,0 +1,155 @@
+"""`SdotPlanNd`: the solution of a semi-discrete transport in `d >= 2`, on the MI355X path.
+
+Mirrors `src/sdot/SdotPlanNd.py` of the reference: the whole fit is ONE call of the C ABI
+( `sdot_amd_solve_2d`, standing for `sdotplan_solve_card_2d[_image]`, `SdotPlanNd.py:864-890` ); the
+options cross as the reference card's 41-double vector, `stats` / `history` come back in the
+reference's layouts ( `_STATS`, `_HISTORY` ) and `OtNotConverged` is raised the same way.
+"""
+
+import numpy as np
+
+from . import _native
+from .distributions import Image, UniformBox
+
+_STATS = [ "status", "residual", "residual0", "nb_iter", "nb_diag", "nb_backtracks", "t_majorant", "t_diag", "t_asm", "t_lin", "t_lim", "eps",
+           "domain_mass", "nb_overflowed", "nb_cell_lim", "nb_limit_rounds", "lin_nb_hierarchies", "lin_nb_iter", "lin_worst", "start", "t_total",
+           "nb_continuation_steps", "min_start_mass", "it_switch", "it_double", "scratch_bytes",
+           "nb_clusters", "nb_aggregated", "nb_duplicates", "residual_full", "nb_polish" ]
+_HISTORY = [ "step", "t", "residual_l2", "min_measure", "max_abs_residual", "nb_diag", "nb_evals", "s" ]
+_STATUS = { 0: "running", 1: "converged", 2: "max iterations", 3: "stagnation", 4: "linear solver failure", 5: "card capacity",
+            6: "failure", 7: "converged (aggregated)" }
+_START = { 0: "weights0", 1: "voronoi", 2: "similarity" }
+_LIN = { "auto": 0, "cholesky": 1, "amg": 2, "cg": 3, "mg": 4 }
+_STEP = { "trials": 0, "limits": 1 }
+_RESIDUAL = { "lin": 0, "log": 1, "power": 2 }
+_CONTINUATION = { "never": 0, "auto": 1, "always": 2 }
+
+
+def card_options( settings, density, verbose = False ):
+    """the 41-double option vector ( `SdotPlanNd.py:806-833` of the reference, same names and meanings )"""
+    tun = settings.tuning
+    step = { "auto": "trials" }.get( tun.step, tun.step )
+    if step not in _STEP:
+        raise ValueError( f"unknown step: { tun.step!r} ( 'auto', 'trials' or 'limits' )" )
+    if step == "limits":
+        raise NotImplementedError( "step = 'limits' is not on the MI355X path yet: 'trials' ( or 'auto' )" )
+    if tun.residual not in _RESIDUAL:
+        raise ValueError( f"unknown residual: { tun.residual!r}" )
+    if tun.linear_solver not in _LIN:
+        raise ValueError( f"unknown linear_solver: { tun.linear_solver!r}" )
+    opts = [ 0.0 ] * len( _native.CARD_OPTIONS )
+
+    def put( name, v ):
+        opts[ _native.CARD_OPTIONS.index( name ) ] = float( v )
+
+    put( "tol_abs", settings.tol ); put( "tol_rel", tun.mass_rtol ); put( "t_min", tun.t_min ); put( "mult_ok", tun.restart_factor )
+    put( "factor", 0.9 ); put( "maxit", settings.max_iter ); put( "max_backtracks", tun.max_backtracks ); put( "step", _STEP[ step ] )
+    put( "residual", _RESIDUAL[ tun.residual ] ); put( "power", tun.residual_power ); put( "switch", tun.residual_switch )
+    put( "lin", 0 if tun.linear_solver == "cg" else 1 ); put( "host_method", _LIN[ tun.linear_solver ] ); put( "lin_tol", tun.linear_tol or 0.0 )
+    put( "lin_maxit", 0 ); put( "trace", int( bool( verbose ) ) )
+    put( "continuation", _CONTINUATION[ settings.continuation ] ); put( "conv_threshold", tun.conv_threshold )
+    put( "conv_s0", tun.conv_start or 0.0 ); put( "conv_ratio", tun.conv_ratio ); put( "conv_min", tun.conv_min or 0.0 )
+    if isinstance( density, Image ):
+        lo, hi = density.box()
+        h = np.diag( density.frame )
+        nx, ny = density.shape
+        for name, v in zip( ( "img_x0", "img_y0", "img_hx", "img_hy", "img_nx", "img_ny" ), ( lo[ 0 ], lo[ 1 ], h[ 0 ], h[ 1 ], nx, ny ) ):
+            put( name, v )
+    return np.asarray( opts )
+
+
+class SdotPlanNd:
+    """see the module docstring; obtained from `OtProblem.solve()`"""
+
+    @classmethod
+    def _solve( cls, problem, settings, verbose, warm = None ):
+        self = cls.__new__( cls )
+        self._build( problem, settings, verbose, warm )
+        self._check_converged( settings )
+        return self
+
+    def _build( self, problem, settings, verbose, warm = None ):
+        self.problem = problem
+        src, dst = problem.source, problem.target
+        d = problem.nb_dims
+        if d != 2:
+            raise NotImplementedError( f"SdotPlanNd on the MI355X path: 2D ( not { d }D )" )
+        if settings.precision == "fp32":
+            raise NotImplementedError( "precision = 'fp32' is not on the MI355X path yet ( 'auto', 'fp64' or 'mixed' run FP64 cells )" )
+        plan = settings.ot_plan if settings.ot_plan is not None else warm
+        if settings.weights0 is not None:
+            w0, self._warm_start = settings.weights0, "weights0"
+        elif plan is not None:
+            n_plan = int( plan.weights.shape[ 0 ] )
+            if n_plan != src.nb_diracs:
+                if settings.ot_plan is not None:
+                    raise ValueError( f"Iterative( ot_plan = ... ): this plan carries { n_plan } weights and the source has { src.nb_diracs }" )
+                w0, self._warm_start = None, f"none ( the kept plan carries { n_plan } weights, the source has { src.nb_diracs } )"
+            else:
+                w0, self._warm_start = plan.weights, "ot_plan"
+        else:
+            w0, self._warm_start = None, "none"
+        problem.domain
+        lo, hi = dst.box()
+        box = [ lo[ 0 ], lo[ 1 ], hi[ 0 ], hi[ 1 ] ]
+        if not isinstance( dst, ( Image, UniformBox ) ):
+            dst.native_args( None )                      # ( raises: not on this path )
+        opts = card_options( settings, dst, verbose )
+        res = _native.solve_2d( src.positions, src.weights, w0, opts, box, dst )
+        #: the fitted weights ( `weights[ 0 ] == 0` ), device tensor, user order
+        self.weights = res[ "weights" ]
+        self._weights_fit = self.weights
+        #: the measure of each cell at the fitted weights
+        self.cell_masses = res[ "cell_masses" ]
+        #: the barycenter of each cell ( its seed if empty ), `[ n, 2 ]`
+        self.barycenters = res[ "barycenters" ]
+        #: the transport cost `sum_i int_{cell_i} |x - p_i|^2 rho`
+        self.cost = res[ "cost" ]
+        self._clusters = res[ "clusters" ]
+        self._masses = np.asarray( src.weights, dtype = np.float64 )
+        self._read_stats( res[ "stats" ], settings )
+        self._read_history( res[ "history" ] )
+
+    def _read_stats( self, st, settings ):
+        self.stats = { name: float( st[ k ] ) for k, name in enumerate( _STATS ) }
+        self.stats[ "status" ] = _STATUS.get( int( self.stats[ "status" ] ), "?" )
+        self.stats[ "it_switch" ] = int( self.stats[ "it_switch" ] )
+        self.stats[ "it_double" ] = int( self.stats[ "it_double" ] )
+        self.stats[ "start" ] = _START.get( int( self.stats[ "start" ] ), "?" )
+        for name in ( "nb_iter", "nb_diag", "nb_backtracks", "nb_overflowed", "nb_cell_lim", "nb_limit_rounds", "lin_nb_hierarchies",
+                      "lin_nb_iter", "nb_continuation_steps", "scratch_bytes", "nb_clusters", "nb_aggregated", "nb_duplicates", "nb_polish" ):
+            self.stats[ name ] = int( self.stats[ name ] )
+        self.stats[ "aggregation" ] = "off" if not settings.aggregate else "none"
+        self.stats[ "duplicates" ] = "not checked ( this path )"
+        self.stats[ "warm_start" ] = self._warm_start
+
+    def _read_history( self, rows ):
+        self.history = []
+        for r in rows:
+            entry = { name: float( r[ k ] ) for k, name in enumerate( _HISTORY ) }
+            entry[ "step" ] = int( entry[ "step" ] )
+            self.history.append( entry )
+
+    def _check_converged( self, settings ):
+        if getattr( settings, "on_failure", "raise" ) != "raise":
+            return
+        if not self.converged:
+            from .OtProblem import OtNotConverged
+            raise OtNotConverged( self )
+
+    @property
+    def converged( self ):
+        return self.stats[ "status" ] in ( "converged", "converged (aggregated)" )
+
+    @property
+    def clusters( self ):
+        return None if self.stats[ "nb_clusters" ] == 0 else self._clusters
+
+    @property
+    def target_masses( self ):
+        """`nu` rescaled to the mass of the domain"""
+        return self._masses * ( self.stats[ "domain_mass" ] / float( self._masses.sum() ) )
+
+    def power_diagram( self, weights = None ):
+        from .PowerDiagram import PowerDiagram
+        return PowerDiagram( self.problem.source.positions, self._weights_fit if weights is None else weights, distribution = self.problem.target )

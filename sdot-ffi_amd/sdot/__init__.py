@@ -1,0 +1,13 @@
+"""sdot on MI355X: the semi-discrete optimal transport solve of the reference `sdot`
+( `OtProblem( SumOfDiracs, density ).solve( Iterative( ... ) )` -> `SdotPlanNd` ), with its
+power diagrams, on hand-written HIP kernels behind the C ABI of `include/sdot_amd.h`.
+"""
+
+from .distributions import Image, SumOfDiracs, SumOfGaussians, UniformBox
+from .OtProblem import Direct, Iterative, OtNotConverged, OtProblem, Tuning, ot_solve
+from .PowerDiagram import PowerDiagram, axis_aligned_box, box_half_spaces
+from .SdotPlanNd import SdotPlanNd
+from ._native import KernelFailure, SdotError
+
+__all__ = [ "Image", "SumOfDiracs", "SumOfGaussians", "UniformBox", "Direct", "Iterative", "OtNotConverged", "OtProblem", "Tuning",
+            "ot_solve", "PowerDiagram", "axis_aligned_box", "box_half_spaces", "SdotPlanNd", "KernelFailure", "SdotError" ]

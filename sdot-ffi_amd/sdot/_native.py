@@ -1,0 +1,229 @@
+"""The C ABI of `libsdot_amd.so` ( `include/sdot_amd.h` ), loaded with ctypes.
+
+Device memory is torch's ( ROCm ): every array crosses as a `data_ptr()` with its size, on torch's
+current stream. There is no fallback: without the library, or without a GPU, the calls raise.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname( os.path.abspath( __file__ ) )
+LIB_PATH = os.path.join( _HERE, "_lib", "libsdot_amd.so" )
+_lib = None
+
+_P = ctypes.c_void_p
+_LL = ctypes.c_longlong
+_D = ctypes.c_double
+_I = ctypes.c_int
+
+#: the option vector of the solve, in the reference card's order ( `src/sdot/SdotPlanNd.py:197-201` )
+CARD_OPTIONS = [ "tol_abs", "tol_rel", "t_min", "mult_ok", "factor", "maxit", "max_backtracks", "step", "residual", "power", "switch",
+                 "lin", "host_method", "lin_tol", "amg_variant", "mg_shift", "mg_recycle", "mg_rebuild", "mg_stop", "mg_nu", "mg_kcycle",
+                 "lin_maxit", "trace", "mg_float", "mg_smoothed",
+                 "continuation", "conv_threshold", "conv_s0", "conv_ratio", "conv_min", "conv_possible", "min_scale",
+                 "img_x0", "img_y0", "img_hx", "img_hy", "img_nx", "img_ny", "agg_margin", "agg_gap", "agg_nb_dups" ]
+
+#: what the library exports ( every symbol of `include/sdot_amd.h` )
+EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_bytes", "sdot_amd_release", "sdot_amd_measures_2d",
+            "sdot_amd_solve_2d", "sdot_amd_tree_nodes", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
+            "sdot_amd_linear_solves_2d" ]
+
+CONSTANT, IMAGE = 0, 1
+
+
+class SdotError( RuntimeError ):
+    """a failure reported by the library: `kind` ( 1 argument / HIP, 2 kernel failure, 3 capacity, 4 memory ), `code`, `value`"""
+
+    def __init__( self, msg, kind, code, value ):
+        super().__init__( msg )
+        self.kind, self.code, self.value = kind, code, value
+
+
+class KernelFailure( SdotError ):
+    """a cell the kernels could not build ( `code` 1: more vertices than the limit, `value` the seed )"""
+
+
+def lib():
+    """the loaded library ( raises when it is not built )"""
+    global _lib
+    if _lib is None:
+        if not os.path.exists( LIB_PATH ):
+            raise RuntimeError( f"libsdot_amd.so is not built ( expected at { LIB_PATH } ): `make -C sdot-ffi_amd/csrc` "
+                                "or `python -c 'import __graft_entry__ as g; g.build()'`" )
+        L = ctypes.CDLL( LIB_PATH )
+        L.sdot_amd_abi_version.restype = _I
+        L.sdot_amd_last_error.restype = ctypes.c_char_p
+        L.sdot_amd_last_error.argtypes = [ ctypes.POINTER( _I ), ctypes.POINTER( _I ), ctypes.POINTER( _LL ) ]
+        L.sdot_amd_device_bytes.restype = _LL
+        L.sdot_amd_tree_nodes.restype = _LL
+        L.sdot_amd_tree_nodes.argtypes = [ _LL, _I ]
+        L.sdot_amd_measures_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P, _P, _LL, _P, _P ]
+        L.sdot_amd_solve_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _LL, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_build_tree_2d.argtypes = [ _P, _LL, _P, _I, _P, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_refresh_majorants_2d.argtypes = [ _P, _LL, _LL, _P, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_linear_solves_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _LL, _P, _P, _P ]
+        for name in ( "sdot_amd_measures_2d", "sdot_amd_solve_2d", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
+                      "sdot_amd_linear_solves_2d" ):
+            getattr( L, name ).restype = _I
+        if L.sdot_amd_abi_version() != 1:
+            raise RuntimeError( "libsdot_amd.so: unexpected ABI version" )
+        _lib = L
+    return _lib
+
+
+def _check( rc ):
+    if rc == 0:
+        return
+    k, c, v = _I(), _I(), _LL()
+    msg = lib().sdot_amd_last_error( ctypes.byref( k ), ctypes.byref( c ), ctypes.byref( v ) ).decode()
+    cls = KernelFailure if k.value == 2 else SdotError
+    if k.value == 4:
+        raise MemoryError( msg )
+    raise cls( msg, k.value, c.value, v.value )
+
+
+# ---- torch plumbing ------------------------------------------------------------------------------------
+
+
+def torch():
+    import torch as T
+    return T
+
+
+def device():
+    """the GPU the library runs on ( raises without one: there is no CPU path )"""
+    T = torch()
+    if not T.cuda.is_available():
+        raise RuntimeError( "sdot (MI355X): no GPU visible -- this implementation runs on the device only" )
+    return T.device( "cuda", T.cuda.current_device() )
+
+
+def on_device( a, dtype = None ):
+    """`a` as a contiguous device tensor ( float64 unless told otherwise )"""
+    T = torch()
+    dtype = dtype or T.float64
+    if isinstance( a, T.Tensor ):
+        return a.detach().to( device = device(), dtype = dtype ).contiguous()
+    return T.as_tensor( np.ascontiguousarray( np.asarray( a ) ), dtype = dtype, device = device() ).contiguous()
+
+
+def _ptr( t ):
+    return None if t is None else _P( t.data_ptr() )
+
+
+def _stream():
+    return _P( torch().cuda.current_stream().cuda_stream )
+
+
+def _host( a ):
+    a = np.ascontiguousarray( np.asarray( a, dtype = np.float64 ) )
+    return a, a.ctypes.data_as( _P )
+
+
+# ---- the calls -------------------------------------------------------------------------------------------
+
+
+def measures_2d( positions, weights, box, density, facets = False, events = False ):
+    """masses ( device tensor, user order ); with `facets` also `( fi, fj, fc )` ( device tensors ); with `events` the counts"""
+    T = torch()
+    pos = on_device( positions ).reshape( -1, 2 ).contiguous()
+    n = pos.shape[ 0 ]
+    w = None if weights is None else on_device( weights ).reshape( -1 )
+    boxh, boxp = _host( box )
+    kind, rho, img, nx, ny, geo = density.native_args( device_tensor = on_device )
+    geoh, geop = _host( geo )
+    masses = T.empty( n, dtype = T.float64, device = device() )
+    cap = 8 * n + 1024 if facets else 0
+    fi = T.empty( cap, dtype = T.int64, device = device() ) if facets else None
+    fj = T.empty( cap, dtype = T.int64, device = device() ) if facets else None
+    fc = T.empty( cap, dtype = T.float64, device = device() ) if facets else None
+    nbf = np.zeros( 1, np.int64 )
+    ev = np.zeros( 5, np.uint64 )
+    _check( lib().sdot_amd_measures_2d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop, _ptr( masses ),
+                                        _ptr( fi ), _ptr( fj ), _ptr( fc ), cap, nbf.ctypes.data_as( _P ),
+                                        ev.ctypes.data_as( _P ) if events else None ) )
+    res = [ masses ]
+    if facets:
+        k = int( nbf[ 0 ] )
+        res.append( ( fi[ :k ], fj[ :k ], fc[ :k ] ) )
+    if events:
+        res.append( dict( zip( ( "side_tests", "prune_vertex_tests", "new_vertices", "node_visits", "cuts" ), [ int( x ) for x in ev ] ) ) )
+    return res[ 0 ] if len( res ) == 1 else tuple( res )
+
+
+def solve_2d( positions, nu, w0, options, box, density ):
+    """one solve; returns a dict of device tensors and host arrays ( see `sdot_amd_solve_2d` )"""
+    T = torch()
+    pos = on_device( positions ).reshape( -1, 2 ).contiguous()
+    n = pos.shape[ 0 ]
+    nu = on_device( nu ).reshape( -1 )
+    w0 = None if w0 is None else on_device( w0 ).reshape( -1 )
+    opts, optp = _host( options )
+    boxh, boxp = _host( box )
+    kind, rho, img, nx, ny, geo = density.native_args( device_tensor = on_device )
+    maxit = int( opts[ CARD_OPTIONS.index( "maxit" ) ] )
+    dev = device()
+    weights = T.empty( n, dtype = T.float64, device = dev )
+    masses = T.empty( n, dtype = T.float64, device = dev )
+    bary = T.empty( ( n, 2 ), dtype = T.float64, device = dev )
+    clusters = T.empty( n, dtype = T.int64, device = dev )
+    hist = np.zeros( ( maxit + 1, 8 ) )
+    nb_steps = np.zeros( 1, np.int64 )
+    stats = np.zeros( 31 )
+    cost = np.zeros( 1 )
+    _check( lib().sdot_amd_solve_2d( _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), None, 0, optp, boxp, kind, rho, _ptr( img ),
+                                     _ptr( weights ), hist.ctypes.data_as( _P ), nb_steps.ctypes.data_as( _P ), stats.ctypes.data_as( _P ),
+                                     _ptr( masses ), _ptr( bary ), cost.ctypes.data_as( _P ), _ptr( clusters ) ) )
+    return dict( weights = weights, cell_masses = masses, barycenters = bary, clusters = clusters, cost = float( cost[ 0 ] ),
+                 stats = stats, history = hist[ :int( nb_steps[ 0 ] ) ] )
+
+
+def build_tree_2d( positions, leaf = 10 ):
+    T = torch()
+    pos = on_device( positions ).reshape( -1, 2 ).contiguous()
+    n = pos.shape[ 0 ]
+    N = int( lib().sdot_amd_tree_nodes( n, leaf ) )
+    dev = device()
+    out = dict( seed_indices = T.empty( n, dtype = T.int64, device = dev ), rank_of = T.empty( n, dtype = T.int64, device = dev ),
+                node_begin = T.empty( N, dtype = T.int64, device = dev ), node_end = T.empty( N, dtype = T.int64, device = dev ),
+                node_left = T.empty( N, dtype = T.int64, device = dev ), node_right = T.empty( N, dtype = T.int64, device = dev ),
+                node_box = T.empty( ( N, 2, 2 ), dtype = T.float64, device = dev ) )
+    _check( lib().sdot_amd_build_tree_2d( _stream(), n, _ptr( pos ), leaf, *[ _ptr( out[ k ] ) for k in (
+        "seed_indices", "rank_of", "node_begin", "node_end", "node_left", "node_right", "node_box" ) ] ) )
+    return out
+
+
+def refresh_majorants_2d( tree, sorted_positions, sorted_weights ):
+    T = torch()
+    sp = on_device( sorted_positions ).reshape( -1, 2 ).contiguous()
+    sw = on_device( sorted_weights ).reshape( -1 )
+    N = tree[ "node_begin" ].shape[ 0 ]
+    wa = T.empty( ( N, 2 ), dtype = T.float64, device = device() )
+    wb = T.empty( N, dtype = T.float64, device = device() )
+    _check( lib().sdot_amd_refresh_majorants_2d( _stream(), sp.shape[ 0 ], N, _ptr( tree[ "node_box" ] ), _ptr( tree[ "node_begin" ] ),
+                                                 _ptr( tree[ "node_end" ] ), _ptr( sp ), _ptr( sw ), _ptr( wa ), _ptr( wb ) ) )
+    return wa, wb
+
+
+def linear_solves_2d( row, col, val, dia, rhs, tol = 1e-6 ):
+    T = torch()
+    row = on_device( row, T.int64 )
+    col = on_device( col, T.int64 )
+    val = on_device( val )
+    dia = on_device( dia )
+    rhs = on_device( rhs )
+    if rhs.dim() == 1:
+        rhs = rhs[ None ]
+    k, n = rhs.shape
+    sol = T.empty( ( k, n ), dtype = T.float64, device = device() )
+    its = np.zeros( k )
+    opts, optp = _host( [ n, k, 0, tol, -1, 0, 0, 0, -1, 0 ] )
+    _check( lib().sdot_amd_linear_solves_2d( _stream(), n, _ptr( row ), _ptr( col ), _ptr( val ), _ptr( dia ), _ptr( rhs.contiguous() ), k,
+                                             optp, _ptr( sol ), its.ctypes.data_as( _P ) ) )
+    return sol, its.astype( int )
+
+
+def device_bytes():
+    return int( lib().sdot_amd_device_bytes() )

@@ -1,0 +1,165 @@
+"""The distributions of the solve: the discrete source and the target densities.
+
+Mirrors `src/sdot/distributions/{SumOfDiracs,Image,SumOfGaussians}.py` of the reference: the same
+constructors, `normalized_version()` ( total mass 1 ) and the support the target declares
+( `bounding_half_spaces`, the domain of the problem ).
+"""
+
+import numpy as np
+
+
+def _host( a ):
+    try:
+        import torch
+        if isinstance( a, torch.Tensor ):
+            return a.detach().cpu().numpy()
+    except ImportError:
+        pass
+    return np.asarray( a )
+
+
+class SumOfDiracs:
+    """weighted Dirac masses at `positions[ n, d ]` ( reference `distributions/SumOfDiracs.py:16` )"""
+
+    _is_dirac_source = True
+
+    def __init__( self, positions, weights = None, target_mass = 1.0 ):
+        self.positions = positions
+        self.weights = weights
+        self.target_mass = target_mass
+
+    @property
+    def nb_diracs( self ):
+        return int( self.positions.shape[ 0 ] )
+
+    @property
+    def nb_dims( self ):
+        return int( self.positions.shape[ 1 ] )
+
+    @property
+    def mass( self ):
+        return float( self.nb_diracs if self.weights is None else _host( self.weights ).sum() )
+
+    def normalized_version( self, nb_dims = None ):
+        """the weights scaled to `target_mass` ( `1 / n` each when none were given ), `SumOfDiracs.py:43`"""
+        if self.weights is None:
+            w = np.full( self.nb_diracs, self.target_mass / self.nb_diracs )
+        else:
+            w = _host( self.weights ).astype( np.float64 ).reshape( -1 )
+            w = w * ( self.target_mass / w.sum() )
+        res = SumOfDiracs( self.positions, w, self.target_mass )
+        return res
+
+
+class Image:
+    """piecewise-constant density on a grid: pixel `( i, j )` ( first index along x ) covers
+    `origin + frame^T [ i, i + 1 ] x [ j, j + 1 ]` ( reference `distributions/Image.py:12`, `Image.cxx:97` ).
+    Regular grids with a diagonal frame."""
+
+    def __init__( self, values, origin = None, frame = None, target_mass = 1.0 ):
+        self.values = values
+        v = _host( values )
+        self.nb_dims = v.ndim
+        self.origin = np.zeros( v.ndim ) if origin is None else np.asarray( _host( origin ), dtype = np.float64 ).reshape( -1 )
+        self.frame = np.eye( v.ndim ) if frame is None else np.asarray( _host( frame ), dtype = np.float64 ).reshape( v.ndim, v.ndim )
+        self.target_mass = target_mass
+        if np.any( self.frame != np.diag( np.diag( self.frame ) ) ) or np.any( np.diag( self.frame ) <= 0 ):
+            raise NotImplementedError( "Image: this implementation takes a diagonal frame with positive steps ( a regular grid )" )
+
+    @property
+    def shape( self ):
+        return tuple( _host( self.values ).shape )
+
+    @property
+    def mass( self ):
+        return float( _host( self.values ).sum() * abs( np.linalg.det( self.frame ) ) )
+
+    def normalized_version( self, nb_dims = None ):
+        v = _host( self.values ).astype( np.float64 )
+        m = self.mass
+        if not ( m > 0 ):
+            raise ValueError( "Image: the image has no mass" )
+        return Image( v * ( self.target_mass / m ), self.origin, self.frame, self.target_mass )
+
+    def box( self ):
+        """`( lo, hi )` of the grid"""
+        h = np.diag( self.frame )
+        return self.origin.copy(), self.origin + h * np.asarray( self.shape, dtype = np.float64 )
+
+    def bounding_half_spaces( self ):
+        """the block of the grid as half-spaces `dir . x <= off` ( `Image.py:66-80` )"""
+        d = self.nb_dims
+        lo, hi = self.box()
+        return np.concatenate( [ np.eye( d ), -np.eye( d ) ] ), np.concatenate( [ hi, -lo ] )
+
+    def native_args( self, device_tensor ):
+        """`( kind, rho, img, nx, ny, geo )` for the C ABI: `img[ j * nx + i ]` ( `PowerDiagram_Bsp.py:266-268` )"""
+        v = _host( self.values ).astype( np.float64 )
+        if v.ndim != 2:
+            raise NotImplementedError( "Image: 2D images on this path" )
+        if v.size and np.all( v == v.flat[ 0 ] ):        # equal values: the constant density ( `PowerDiagram_Bsp.py:223-231` )
+            return 0, float( v.flat[ 0 ] ), None, 0, 0, [ 0.0, 0.0, 1.0, 1.0 ]
+        nx, ny = v.shape
+        h = np.diag( self.frame )
+        return 1, 0.0, device_tensor( np.ascontiguousarray( v.T ).reshape( -1 ) ), nx, ny, [ self.origin[ 0 ], self.origin[ 1 ], h[ 0 ], h[ 1 ] ]
+
+
+class UniformBox:
+    """the Lebesgue measure of a box, normalized: `rho = 1 / |box|` ( the constant density of the reference card )"""
+
+    def __init__( self, lo, hi, target_mass = 1.0 ):
+        self.lo = np.asarray( lo, dtype = np.float64 ).reshape( -1 )
+        self.hi = np.asarray( hi, dtype = np.float64 ).reshape( -1 )
+        self.nb_dims = len( self.lo )
+        self.target_mass = target_mass
+
+    def normalized_version( self, nb_dims = None ):
+        return self
+
+    def box( self ):
+        return self.lo.copy(), self.hi.copy()
+
+    def bounding_half_spaces( self ):
+        d = self.nb_dims
+        return np.concatenate( [ np.eye( d ), -np.eye( d ) ] ), np.concatenate( [ self.hi, -self.lo ] )
+
+    @property
+    def rho( self ):
+        return self.target_mass / float( np.prod( self.hi - self.lo ) )
+
+    def native_args( self, device_tensor ):
+        return 0, self.rho, None, 0, 0, [ 0.0, 0.0, 1.0, 1.0 ]
+
+
+class SumOfGaussians:
+    """isotropic Gaussians `weights_k N( positions_k, sigmas_k^2 I )` truncated to `support_box`
+    ( reference `distributions/SumOfGaussians.py:11` ). Declared with its normalization and support;
+    its integrals on cells are not on this device path yet."""
+
+    def __init__( self, positions, sigmas, weights = None, support_box = None, support_sigmas = 6.0, target_mass = 1.0 ):
+        self.positions = np.asarray( _host( positions ), dtype = np.float64 )
+        self.sigmas = np.asarray( _host( sigmas ), dtype = np.float64 ).reshape( -1 )
+        k = self.positions.shape[ 0 ]
+        self.weights = np.full( k, 1.0 / k ) if weights is None else np.asarray( _host( weights ), dtype = np.float64 ).reshape( -1 )
+        self.nb_dims = self.positions.shape[ 1 ]
+        self.target_mass = target_mass
+        if support_box is None:
+            lo = ( self.positions - support_sigmas * self.sigmas[ :, None ] ).min( axis = 0 )
+            hi = ( self.positions + support_sigmas * self.sigmas[ :, None ] ).max( axis = 0 )
+            support_box = ( lo, hi )
+        self.support_box = ( np.asarray( support_box[ 0 ], dtype = np.float64 ), np.asarray( support_box[ 1 ], dtype = np.float64 ) )
+
+    def normalized_version( self, nb_dims = None ):
+        return SumOfGaussians( self.positions, self.sigmas, self.weights * ( self.target_mass / self.weights.sum() ), self.support_box,
+                               target_mass = self.target_mass )
+
+    def box( self ):
+        return self.support_box[ 0 ].copy(), self.support_box[ 1 ].copy()
+
+    def bounding_half_spaces( self ):
+        d = self.nb_dims
+        lo, hi = self.box()
+        return np.concatenate( [ np.eye( d ), -np.eye( d ) ] ), np.concatenate( [ hi, -lo ] )
+
+    def native_args( self, device_tensor ):
+        raise NotImplementedError( "SumOfGaussians: the Gaussian integrals of the cells are not on the MI355X path yet" )

@@ -1,0 +1,140 @@
+"""Independent checks shared by the tests: clouds of the reference's test-suite, the power diagram
+from scipy's convex hull of the lifted seeds, and exact polygon / pixel clipping in numpy."""
+
+import numpy as np
+
+
+# ---- clouds ( reference tests/test_CardCells.py:75-102 ) ------------------------------------------------------
+
+
+def ring( k, rng, n_back = 2000 ):
+    """a seed in the middle of a ring of `k` seeds ( its cell is a k-gon ), plus a uniform background"""
+    a = 2 * np.pi * ( np.arange( k ) + rng.uniform( -0.1, 0.1, k ) ) / k
+    rg = 0.5 + 0.2 * np.stack( [ np.cos( a ), np.sin( a ) ], axis = 1 )
+    back = rng.uniform( 0.001, 0.999, size = ( n_back, 2 ) )
+    back = back[ np.linalg.norm( back - 0.5, axis = 1 ) > 0.3 ]
+    return np.concatenate( [ [ [ 0.5, 0.5 ] ], rg, back ] )
+
+
+def lines( n, rng ):
+    """five lines with a 0.005 normal spread ( thin, long cells )"""
+    ends = rng.uniform( 0.05, 0.95, size = ( 5, 2, 2 ) )
+    which = rng.integers( 0, 5, n )
+    t = rng.uniform( 0, 1, n )[ :, None ]
+    pos = ends[ which, 0 ] + t * ( ends[ which, 1 ] - ends[ which, 0 ] ) + rng.normal( 0, 0.005, size = ( n, 2 ) )
+    return np.clip( pos, 0.001, 0.999 )
+
+
+def clouds( rng, n = 3000 ):
+    """`( label, positions, weights )`: uniform, lines, weighted lines"""
+    uni = rng.uniform( 0.001, 0.999, size = ( n, 2 ) )
+    lin = lines( n, rng )
+    h2 = 1.0 / n
+    return [ ( "uniform", uni, None ),
+             ( "lines", lin, None ),
+             ( "lines weighted", lin, rng.uniform( -0.5, 0.5, n ) * h2 + 0.05 * np.sin( 4 * lin[ :, 0 ] ) ) ]
+
+
+# ---- the power diagram by lifting -------------------------------------------------------------------------------
+
+
+def lifted_neighbours( pos, w, lo = ( 0, 0 ), hi = ( 1, 1 ) ):
+    """the neighbour pairs `{ ( i, j ) }` of the power diagram restricted to the box, as the lower convex hull of the
+    lifted points `( p, |p|^2 - w )` gives them ( a regular triangulation ), keeping the pairs whose common facet crosses
+    the box. Independent of the oracle: scipy's Qhull."""
+    from scipy.spatial import ConvexHull
+    pos = np.asarray( pos, dtype = np.float64 )
+    w = np.zeros( len( pos ) ) if w is None else np.asarray( w, dtype = np.float64 )
+    lift = np.column_stack( [ pos, ( pos ** 2 ).sum( 1 ) - w ] )
+    hull = ConvexHull( lift )
+    pairs = set()
+    for simplex, eq in zip( hull.simplices, hull.equations ):
+        if eq[ 2 ] >= 0:                                 # upper faces are not in the regular triangulation
+            continue
+        for a in range( 3 ):
+            i, j = int( simplex[ a ] ), int( simplex[ ( a + 1 ) % 3 ] )
+            pairs.add( ( min( i, j ), max( i, j ) ) )
+    return pairs
+
+
+def halfspace_cell( pos, w, i, lo = ( 0, 0 ), hi = ( 1, 1 ) ):
+    """the vertices of cell `i` ( counter-clockwise ) by half-space intersection ( scipy ), or `None` if empty"""
+    from scipy.spatial import HalfspaceIntersection, ConvexHull
+    pos = np.asarray( pos, dtype = np.float64 )
+    w = np.zeros( len( pos ) ) if w is None else np.asarray( w, dtype = np.float64 )
+    p0 = pos[ i ]
+    hs = []
+    for j in range( len( pos ) ):
+        if j == i:
+            continue
+        d = pos[ j ] - p0
+        off = ( d @ d + w[ i ] - w[ j ] ) / 2
+        hs.append( [ d[ 0 ], d[ 1 ], -off ] )
+    hs += [ [ 1, 0, -( hi[ 0 ] - p0[ 0 ] ) ], [ -1, 0, lo[ 0 ] - p0[ 0 ] ], [ 0, 1, -( hi[ 1 ] - p0[ 1 ] ) ], [ 0, -1, lo[ 1 ] - p0[ 1 ] ] ]
+    hs = np.asarray( hs )
+    # an interior point: the seed's cell may not contain the seed; use a linear program
+    from scipy.optimize import linprog
+    A = hs[ :, :2 ]
+    nrm = np.linalg.norm( A, axis = 1 )
+    res = linprog( [ 0, 0, -1 ], A_ub = np.column_stack( [ A, nrm ] ), b_ub = -hs[ :, 2 ], bounds = [ ( None, None ), ( None, None ), ( 0, None ) ] )
+    if not res.success or res.x[ 2 ] < 1e-14:
+        return None
+    hsi = HalfspaceIntersection( hs, res.x[ :2 ] )
+    v = hsi.intersections
+    hull = ConvexHull( v )
+    return v[ hull.vertices ] + p0
+
+
+def polygon_area( xy ):
+    x, y = xy[ :, 0 ], xy[ :, 1 ]
+    return 0.5 * float( np.sum( x * np.roll( y, -1 ) - np.roll( x, -1 ) * y ) )
+
+
+def clip_polygon( xy, nx, ny, off ):
+    """Sutherland-Hodgman: the part of polygon `xy` where `nx x + ny y <= off`"""
+    out = []
+    m = len( xy )
+    for k in range( m ):
+        a, b = xy[ k ], xy[ ( k + 1 ) % m ]
+        sa, sb = nx * a[ 0 ] + ny * a[ 1 ] - off, nx * b[ 0 ] + ny * b[ 1 ] - off
+        if sa <= 0:
+            out.append( a )
+        if ( sa <= 0 ) != ( sb <= 0 ):
+            t = sa / ( sa - sb )
+            out.append( a + t * ( b - a ) )
+    return np.asarray( out ).reshape( -1, 2 )
+
+
+def image_mass_of_polygon( xy, img, x0, y0, hx, hy ):
+    """`int_polygon rho` for the image `img[ j, i ]` by exact clipping of the polygon against every pixel it meets"""
+    if len( xy ) < 3:
+        return 0.0
+    ny, nx = img.shape
+    i0 = max( int( np.floor( ( xy[ :, 0 ].min() - x0 ) / hx ) ), 0 )
+    i1 = min( int( np.floor( ( xy[ :, 0 ].max() - x0 ) / hx ) ), nx - 1 )
+    j0 = max( int( np.floor( ( xy[ :, 1 ].min() - y0 ) / hy ) ), 0 )
+    j1 = min( int( np.floor( ( xy[ :, 1 ].max() - y0 ) / hy ) ), ny - 1 )
+    tot = 0.0
+    for j in range( j0, j1 + 1 ):
+        for i in range( i0, i1 + 1 ):
+            p = clip_polygon( xy, 1, 0, x0 + ( i + 1 ) * hx )
+            if len( p ) >= 3: p = clip_polygon( p, -1, 0, -( x0 + i * hx ) )
+            if len( p ) >= 3: p = clip_polygon( p, 0, 1, y0 + ( j + 1 ) * hy )
+            if len( p ) >= 3: p = clip_polygon( p, 0, -1, -( y0 + j * hy ) )
+            if len( p ) >= 3:
+                tot += img[ j, i ] * polygon_area( p )
+    return tot
+
+
+def bumps_image( nx, ny, rng, k = 8, floor = 0.1, sig = ( 0.03, 0.15 ) ):
+    """`v[ i, j ] = floor + sum_k exp( -|x_ij - c_k|^2 / ( 2 s_k^2 ) )` at pixel centres ( first index along x ):
+    the image density of the 1M-Dirac 2D benchmark, at any resolution"""
+    c = rng.uniform( 0, 1, ( k, 2 ) )
+    s = rng.uniform( sig[ 0 ], sig[ 1 ], k )
+    xs = ( np.arange( nx ) + 0.5 ) / nx
+    ys = ( np.arange( ny ) + 0.5 ) / ny
+    X, Y = np.meshgrid( xs, ys, indexing = "ij" )
+    v = np.full( ( nx, ny ), float( floor ) )
+    for q in range( k ):
+        v += np.exp( -( ( X - c[ q, 0 ] ) ** 2 + ( Y - c[ q, 1 ] ) ** 2 ) / ( 2 * s[ q ] ** 2 ) )
+    return v

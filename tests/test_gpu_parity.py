@@ -1,0 +1,283 @@
+"""The HIP library against the oracle, through the C ABI, on the GPU.
+
+* constant density: the masses and facets are the oracle's TO THE BIT ( same tree, same walk, same
+  cut arithmetic );
+* image density: within 1e-12 relative ( Green's theorem on the edges against the oracle's pixel
+  pieces: two exact quadratures, different roundings );
+* the solve: the same iterations and diagrams as the oracle, weights to 1e-10 of their scale, and at
+  full size the properties that do not need the oracle.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def say( *a ):
+    print( *a, flush = True )
+    sys.stdout.flush()
+
+
+def _img( values_xy, lo = ( 0, 0 ), hi = ( 1, 1 ) ):
+    """the oracle's image args from a `[ nx, ny ]` array ( first index along x )"""
+    nx, ny = values_xy.shape
+    return dict( values = np.ascontiguousarray( values_xy.T ), x0 = lo[ 0 ], y0 = lo[ 1 ], hx = ( hi[ 0 ] - lo[ 0 ] ) / nx, hy = ( hi[ 1 ] - lo[ 1 ] ) / ny )
+
+
+def _gpu_measures( native, pos, w, density = None, facets = False ):
+    from sdot.distributions import Image, UniformBox
+    dens = UniformBox( [ 0, 0 ], [ 1, 1 ] ) if density is None else density
+    res = native.measures_2d( pos, w, [ 0, 0, 1, 1 ], dens, facets = facets )
+    if facets:
+        m, ( fi, fj, fc ) = res
+        return m.cpu().numpy(), ( fi.cpu().numpy(), fj.cpu().numpy(), fc.cpu().numpy() )
+    return res.cpu().numpy()
+
+
+CLOUDS = [ "uniform", "lines", "lines weighted", "ring", "grid", "tiny" ]
+
+
+def _cloud( label, rng ):
+    if label == "ring":
+        return H.ring( 60, rng, 1500 ), None
+    if label == "grid":
+        g = ( np.arange( 40 ) + 0.5 ) / 40
+        return np.array( [ [ x, y ] for x in g for y in g ] ), None
+    if label == "tiny":
+        return np.array( [ [ 0.2, 0.3 ], [ 0.7, 0.6 ] ] ), np.array( [ 0.01, -0.02 ] )
+    for lab, pos, w in H.clouds( rng, 5000 ):
+        if lab == label:
+            return pos, w
+
+
+@pytest.mark.parametrize( "label", CLOUDS )
+def test_constant_density_masses_and_facets_are_the_oracles( native, oracle, label ):
+    rng = np.random.default_rng( 100 + CLOUDS.index( label ) )
+    pos, w = _cloud( label, rng )
+    say( "case constant", label, len( pos ) )
+    m, ( fi, fj, fc ) = _gpu_measures( native, pos, w, facets = True )
+    mo, ( oi, oj, oc ) = oracle.measures( pos, w, facets = True )
+    assert np.array_equal( m, mo ), ( label, np.abs( m - mo ).max() )
+    # facets: the same rows ( user order ), each in its cell's edge order
+    assert np.array_equal( fi, oi ) and np.array_equal( fj, oj ) and np.array_equal( fc, oc )
+
+
+def test_event_counts_are_the_oracles( native, oracle ):
+    rng = np.random.default_rng( 7 )
+    pos = rng.uniform( 0, 1, ( 20000, 2 ) )
+    say( "case events 20000" )
+    _, ev = native.measures_2d( pos, None, [ 0, 0, 1, 1 ], __import__( "sdot" ).UniformBox( [ 0, 0 ], [ 1, 1 ] ), events = True )
+    _, oev = oracle.measures( pos, events = True )
+    assert ev == oev
+
+
+@pytest.mark.parametrize( "res", [ 64, 1024 ] )
+def test_image_masses_match_the_oracle( native, oracle, res ):
+    from sdot import Image
+    rng = np.random.default_rng( 200 + res )
+    n = 4000 if res == 64 else 60000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    w = rng.uniform( -0.5, 0.5, n ) / n
+    img = H.bumps_image( res, res, rng )
+    dens = Image( img, origin = [ 0, 0 ], frame = np.eye( 2 ) / res ).normalized_version()
+    say( "case image", res, n )
+    m = _gpu_measures( native, pos, w, dens )
+    mo = oracle.measures( pos, w, density = _img( dens.values ) )
+    rel = np.abs( m - mo ) / np.maximum( mo, 1e-3 / n )
+    assert rel.max() < 1e-12, rel.max()
+    assert abs( m.sum() - 1 ) < 1e-12
+
+
+def test_image_facets_match_the_oracle( native, oracle ):
+    from sdot import Image
+    rng = np.random.default_rng( 300 )
+    n = 3000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    img = H.bumps_image( 48, 48, rng )
+    dens = Image( img, origin = [ 0, 0 ], frame = np.eye( 2 ) / 48 ).normalized_version()
+    say( "case image facets" )
+    m, ( fi, fj, fc ) = _gpu_measures( native, pos, None, dens, facets = True )
+    mo, ( oi, oj, oc ) = oracle.measures( pos, None, density = _img( dens.values ), facets = True )
+    assert np.array_equal( fi, oi ) and np.array_equal( fj, oj )
+    assert np.abs( fc - oc ).max() <= 1e-12 * np.abs( oc ).max()
+
+
+def test_neighbour_sets_match_the_lifted_hull( native ):
+    rng = np.random.default_rng( 5 )
+    n = 300
+    pos = rng.uniform( 0.05, 0.95, ( n, 2 ) )
+    w = rng.uniform( -2e-3, 2e-3, n )
+    say( "case neighbours" )
+    _, ( fi, fj, _ ) = _gpu_measures( native, pos, w, facets = True )
+    ours = { ( min( i, j ), max( i, j ) ) for i, j in zip( fi.tolist(), fj.tolist() ) }
+    assert ours <= H.lifted_neighbours( pos, w )
+
+
+def test_overflowed_cells_go_through_the_second_pass( native, oracle ):
+    # a 300-gon: far past the first pass's LDS slots, redone with global slots
+    rng = np.random.default_rng( 9 )
+    pos = H.ring( 300, rng, 800 )
+    say( "case overflow ring 300" )
+    m = _gpu_measures( native, pos, None )
+    mo = oracle.measures( pos )
+    assert np.array_equal( m, mo )
+    xy, _ = oracle.cell( pos, 0 )
+    assert len( xy ) == 300
+
+
+def test_a_cell_past_the_vertex_limit_is_a_clean_failure( native ):
+    from sdot import KernelFailure
+    rng = np.random.default_rng( 10 )
+    pos = H.ring( 1500, rng, 10 )
+    say( "case failure ring 1500" )
+    with pytest.raises( KernelFailure ) as e:
+        _gpu_measures( native, pos, None )
+    assert e.value.code == 1 and e.value.value == 0
+
+
+def test_tree_through_the_abi_is_the_oracles( native, oracle ):
+    rng = np.random.default_rng( 11 )
+    pos = rng.uniform( 0, 1, ( 5000, 2 ) )
+    say( "case tree" )
+    t = { k: v.cpu().numpy() for k, v in native.build_tree_2d( pos ).items() }
+    to = oracle.build_tree( pos )
+    assert np.array_equal( t[ "seed_indices" ], to[ "order" ] )
+    assert np.array_equal( t[ "node_begin" ], to[ "beg" ] ) and np.array_equal( t[ "node_end" ], to[ "end" ] )
+    assert np.array_equal( t[ "node_box" ], to[ "box" ] )
+    assert np.array_equal( t[ "rank_of" ][ t[ "seed_indices" ] ], np.arange( 5000 ) )
+
+
+def test_majorants_through_the_abi_majorate( native ):
+    rng = np.random.default_rng( 12 )
+    pos = rng.uniform( 0, 1, ( 5000, 2 ) )
+    t = native.build_tree_2d( pos )
+    order = t[ "seed_indices" ].cpu().numpy()
+    beg, end = t[ "node_begin" ].cpu().numpy(), t[ "node_end" ].cpu().numpy()
+    sp = pos[ order ]
+    say( "case majorants" )
+    for w in ( 0.3 * pos[ :, 0 ] - 0.7 * pos[ :, 1 ], rng.uniform( -1, 1, 5000 ), 1e-2 * np.sin( 5 * pos[ :, 0 ] ) ):
+        sw = w[ order ]
+        wa, wb = native.refresh_majorants_2d( t, sp, sw )
+        wa, wb = wa.cpu().numpy(), wb.cpu().numpy()
+        for m in range( len( beg ) ):
+            if end[ m ] > beg[ m ]:
+                s = slice( beg[ m ], end[ m ] )
+                assert np.all( sw[ s ] <= sp[ s ] @ wa[ m ] + wb[ m ] ), m
+        # an affine weight is majorized by its own plane at the root, tightly
+        if w is not None and np.allclose( w, 0.3 * pos[ :, 0 ] - 0.7 * pos[ :, 1 ] ):
+            assert np.allclose( wa[ 0 ], [ 0.3, -0.7 ], atol = 1e-9 ) and abs( wb[ 0 ] ) < 1e-5
+
+
+def test_linear_solves_match_a_dense_solve( native, oracle ):
+    rng = np.random.default_rng( 13 )
+    n = 500
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    _, ( fi, fj, fc ) = oracle.measures( pos, facets = True )
+    row = np.zeros( n + 1, np.int64 )
+    np.add.at( row, fi + 1, 1 )
+    row = np.cumsum( row )
+    dia = np.zeros( n )
+    np.add.at( dia, fi, fc )
+    L = -np.zeros( ( n, n ) )
+    for i, j, c in zip( fi, fj, fc ):
+        L[ i, j ] -= c
+    L[ np.arange( n ), np.arange( n ) ] = dia
+    b = rng.normal( size = ( 2, n ) )
+    b -= b.mean( 1, keepdims = True )
+    say( "case linear" )
+    sol, its = native.linear_solves_2d( row, fj, fc, dia, b, tol = 1e-12 )
+    sol = sol.cpu().numpy()
+    for k in range( 2 ):
+        ref = np.zeros( n )
+        ref[ 1: ] = np.linalg.solve( L[ 1:, 1: ], b[ k, 1: ] )
+        assert its[ k ] > 0 and sol[ k, 0 ] == 0
+        assert np.abs( sol[ k ] - ref ).max() < 1e-9 * np.abs( ref ).max()
+
+
+def _solve_both( oracle, pos, density_gpu, density_oracle, tol, lin_tol ):
+    from sdot import Iterative, OtProblem, SumOfDiracs, Tuning
+    sol = OtProblem( SumOfDiracs( pos ), density_gpu ).solve( Iterative( tol = tol, continuation = "never",
+                                                                          tuning = Tuning( linear_solver = "cg", linear_tol = lin_tol ) ) )
+    ro = oracle.solve( pos, tol = tol, lin_tol = lin_tol, density = density_oracle )
+    return sol, ro
+
+
+def test_solve_uniform_10k_matches_the_oracle( oracle, native ):
+    # the plain method at 10 000 Diracs: same decisions, same iterations, same plan
+    from sdot import UniformBox
+    pos = np.random.default_rng( 0 ).uniform( 0, 1, ( 10000, 2 ) )
+    say( "case solve uniform 10k" )
+    sol, ro = _solve_both( oracle, pos, UniformBox( [ 0, 0 ], [ 1, 1 ] ), None, 1e-9, 1e-10 )
+    assert sol.converged and ro[ "stats" ][ "status" ] == 1
+    assert sol.stats[ "nb_iter" ] == ro[ "stats" ][ "nb_iter" ] and sol.stats[ "nb_diag" ] == ro[ "stats" ][ "nb_diag" ]
+    w = sol.weights.cpu().numpy()
+    assert np.abs( w - ro[ "weights" ] ).max() < 1e-10 * np.abs( ro[ "weights" ] ).max()
+    m = sol.cell_masses.cpu().numpy()
+    assert np.abs( m - 1e-4 ).max() <= 1e-9
+    assert abs( sol.cost - ro[ "cost" ] ) < 1e-12 * ro[ "cost" ]
+    assert np.abs( sol.barycenters.cpu().numpy() - ro[ "barycenters" ] ).max() < 1e-10
+    # the cells of the returned weights, measured again, have the returned masses
+    assert np.array_equal( _gpu_measures( native, pos, w ), m )
+
+
+def test_solve_image_matches_the_oracle( oracle, native ):
+    from sdot import Image
+    rng = np.random.default_rng( 1 )
+    n = 20000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    img = H.bumps_image( 128, 128, rng )
+    dens = Image( img, origin = [ 0, 0 ], frame = np.eye( 2 ) / 128 ).normalized_version()
+    say( "case solve image 20k" )
+    sol, ro = _solve_both( oracle, pos, dens, _img( dens.values ), 1e-9 / 10, 1e-10 )
+    assert sol.converged and ro[ "stats" ][ "status" ] == 1
+    assert sol.stats[ "nb_iter" ] == ro[ "stats" ][ "nb_iter" ]
+    w = sol.weights.cpu().numpy()
+    assert np.abs( w - ro[ "weights" ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max()
+    assert abs( sol.cost - ro[ "cost" ] ) < 1e-10 * ro[ "cost" ]
+
+
+def test_solve_is_bitwise_reproducible( native ):
+    from sdot import Iterative, OtProblem, SumOfDiracs, UniformBox
+    pos = np.random.default_rng( 2 ).uniform( 0, 1, ( 30000, 2 ) )
+    say( "case determinism" )
+    a = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( tol = 1e-10 ) )
+    b = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( tol = 1e-10 ) )
+    assert np.array_equal( a.weights.cpu().numpy(), b.weights.cpu().numpy() )
+    assert a.stats[ "nb_diag" ] == b.stats[ "nb_diag" ]
+
+
+def test_similarity_start_on_the_gpu( native ):
+    from sdot import Iterative, OtProblem, SumOfDiracs, UniformBox
+    pos = np.random.default_rng( 3 ).uniform( 2, 3, ( 2000, 2 ) )
+    say( "case similarity" )
+    sol = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( tol = 1e-10 ) )
+    assert sol.stats[ "start" ] == "similarity" and sol.converged
+
+
+def test_warm_restart_takes_few_steps( native ):
+    from sdot import Iterative, OtProblem, SumOfDiracs, UniformBox
+    rng = np.random.default_rng( 4 )
+    pos = rng.uniform( 0, 1, ( 20000, 2 ) )
+    pb = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) )
+    say( "case warm restart" )
+    first = pb.solve( Iterative( tol = 1e-10 ) )
+    pb.source = SumOfDiracs( pos + 1e-5 * rng.normal( size = pos.shape ).clip( -3, 3 ) )
+    second = pb.solve( Iterative( tol = 1e-10 ) )
+    assert second.stats[ "warm_start" ] == "ot_plan" and second.stats[ "nb_iter" ] <= 3 < first.stats[ "nb_iter" ]
+
+
+def test_full_size_uniform_million( native ):
+    # the 1M-Dirac shape: converged to 1e-9, masses summing to the domain, the gauge held
+    from sdot import Iterative, OtProblem, SumOfDiracs, UniformBox
+    pos = np.random.default_rng( 1 ).uniform( 0, 1, ( 1_000_000, 2 ) )
+    say( "case 1M uniform" )
+    sol = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( tol = 1e-9 ) )
+    m = sol.cell_masses.cpu().numpy()
+    assert sol.converged and np.abs( m - 1e-6 ).max() <= 1e-9
+    assert abs( m.sum() - 1 ) < 1e-10
+    assert sol.weights.cpu().numpy()[ 0 ] == 0

@@ -1,0 +1,334 @@
+"""The oracle pinned: each function of `oracle/sdot_oracle.cpp` against what the mathematics fixes
+( closed forms, invariants, an independent library, brute force, finite differences ), mirroring the
+reference's own tests ( file:line cited per test ). CPU only."""
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+
+# ---- the tree ( reference tests/test_PowerDiagram.py:773-900 ) -----------------------------------------------
+
+
+def _check_tree( O, pos, w, leaf = 10 ):
+    n = len( pos )
+    t = O.build_tree( pos, w, leaf )
+    depth = t[ "depth" ]
+    assert depth == O.max_depth_for( n, leaf )
+    N = 2 ** depth - 1
+    assert len( t[ "beg" ] ) == N
+    # every seed exactly once ( test_PowerDiagram.py:773 )
+    assert np.array_equal( np.sort( t[ "order" ] ), np.arange( n ) )
+    sp = pos[ t[ "order" ] ]
+    ww = None if w is None else w[ t[ "order" ] ]
+
+    def walk( m, h ):
+        b, e = t[ "beg" ][ m ], t[ "end" ][ m ]
+        if e > b:
+            # the box contains the subtree ( test_PowerDiagram.py:835 )
+            lo, hi = t[ "box" ][ m ]
+            assert np.all( sp[ b:e ] >= lo ) and np.all( sp[ b:e ] <= hi )
+            assert np.array_equal( lo, sp[ b:e ].min( 0 ) ) and np.array_equal( hi, sp[ b:e ].max( 0 ) )
+            # the majorant majorates ( test_PowerDiagram.py:870 )
+            if ww is not None:
+                assert np.all( ww[ b:e ] <= sp[ b:e ] @ t[ "wa" ][ m ] + t[ "wb" ][ m ] )
+        if h <= 1:
+            assert e - b <= max( leaf, 1 ) or np.all( sp[ b:e ] == sp[ b ] )
+            return
+        lc, rc = m + 1, m + 2 ** ( h - 1 )
+        # the children partition the slice
+        assert t[ "beg" ][ lc ] == b and t[ "end" ][ rc ] == e and t[ "end" ][ lc ] == t[ "beg" ][ rc ]
+        walk( lc, h - 1 )
+        walk( rc, h - 1 )
+
+    walk( 0, depth )
+    return t
+
+
+def test_tree_shape_depends_on_n_only( oracle ):
+    # the tree's shape is a function of n alone, degenerate clouds included ( test_PowerDiagram.py:801 )
+    rng = np.random.default_rng( 0 )
+    for n in ( 1, 2, 10, 11, 57, 1000 ):
+        for pos in ( rng.uniform( 0, 1, ( n, 2 ) ), np.full( ( n, 2 ), 0.3 ), np.column_stack( [ np.linspace( 0, 1, n ), np.zeros( n ) ] ) ):
+            _check_tree( oracle, pos, None )
+    assert oracle.max_depth_for( 10 ) == 1 and oracle.max_depth_for( 11 ) == 2 and oracle.max_depth_for( 10 ** 6 ) == 18
+
+
+def test_tree_median_split( oracle ):
+    # a median cut on the longest axis: the halves differ by at most one seed and are separated along that axis
+    rng = np.random.default_rng( 1 )
+    pos = rng.uniform( 0, 1, ( 500, 2 ) ) * [ 3.0, 1.0 ]
+    t = _check_tree( oracle, pos, None )
+    sp = pos[ t[ "order" ] ]
+    h = t[ "depth" ]
+    lc, rc = 1, 2 ** ( h - 1 )
+    nl = t[ "end" ][ lc ] - t[ "beg" ][ lc ]
+    nr = t[ "end" ][ rc ] - t[ "beg" ][ rc ]
+    assert abs( nl - nr ) <= 1
+    assert sp[ :nl, 0 ].max() <= sp[ nl:, 0 ].min()
+
+
+def test_majorants_majorate( oracle ):
+    rng = np.random.default_rng( 2 )
+    pos = rng.uniform( 0, 1, ( 2000, 2 ) )
+    for w in ( rng.uniform( -1, 1, 2000 ), 0.3 * pos[ :, 0 ] - 0.7 * pos[ :, 1 ] + 1e-3 * rng.normal( size = 2000 ),
+               np.sin( 5 * pos[ :, 0 ] ) * 1e-2 ):
+        _check_tree( oracle, pos, w )
+
+
+def test_majorant_takes_an_affine_weight( oracle ):
+    # an exactly affine weight: the least-squares slope is the slope, and the bound is tight up to its margin
+    rng = np.random.default_rng( 3 )
+    pos = rng.uniform( 0, 1, ( 40, 2 ) )
+    w = 0.3 * pos[ :, 0 ] - 0.7 * pos[ :, 1 ] + 2.0
+    wa, wb = oracle.weight_majorant( pos, w )
+    assert np.allclose( wa, [ 0.3, -0.7 ], atol = 1e-12 )
+    assert abs( wb - 2.0 ) < 1e-5
+    # random weights: the slope is refused ( it does not tighten the spread by more than chance ), b = max w
+    w = rng.uniform( -1, 1, 40 )
+    wa, wb = oracle.weight_majorant( pos, w )
+    assert np.all( wa == 0 ) and wb >= w.max() and wb - w.max() < 1e-5
+
+
+# ---- the cells ( reference tests/test_PowerDiagram.py:46-330 ) --------------------------------------------------
+
+
+def test_one_seed_takes_everything( oracle ):
+    m = oracle.measures( np.array( [ [ 0.3, 0.3 ] ] ), lo = ( 0, 0 ), hi = ( 2, 2 ) )
+    assert abs( m[ 0 ] - 4.0 ) < 1e-12
+
+
+def test_sum_is_the_domain( oracle ):
+    rng = np.random.default_rng( 14 )
+    for label, pos, w in H.clouds( rng, 3000 ):
+        m = oracle.measures( pos, w )
+        assert abs( m.sum() - 1 ) < 1e-12, label
+        assert np.all( m >= 0 )
+
+
+def test_bisector_shifted_by_the_weight_gap( oracle ):
+    # two seeds at x = 0 and 1: the plane is at ( 1 + w0 - w1 ) / 2 ( test_PowerDiagram.py:262 )
+    for gap in ( -0.6, -0.25, 0.0, 0.25, 0.6 ):
+        m = oracle.measures( np.array( [ [ 0.0, 0.5 ], [ 1.0, 0.5 ] ] ), np.array( [ gap, 0.0 ] ) )
+        assert abs( m[ 0 ] - ( 1 + gap ) / 2 ) < 1e-14, gap
+
+
+def test_seeds_on_a_grid( oracle ):
+    # four cells meet at each grid point, the bisectors pass through vertices ( test_PowerDiagram.py:217 )
+    xs = np.array( [ 1, 3, 5 ] ) / 6
+    pos = np.array( [ [ x, y ] for x in xs for y in xs ] )
+    assert np.allclose( oracle.measures( pos ), 1 / 9, atol = 1e-15 )
+    g = ( np.arange( 32 ) + 0.5 ) / 32
+    pos = np.array( [ [ x, y ] for x in g for y in g ] )
+    assert np.allclose( oracle.measures( pos ), 1 / 1024, atol = 1e-17 )
+
+
+def test_a_seed_outside_the_domain_has_no_cell( oracle ):
+    m = oracle.measures( np.array( [ [ 0.5, 0.5 ], [ 0.2, 0.2 ], [ 5.0, 5.0 ] ] ) )
+    assert abs( m[ 2 ] ) < 1e-14 and abs( m.sum() - 1 ) < 1e-12
+
+
+def test_weight_shift_changes_nothing( oracle ):
+    rng = np.random.default_rng( 33 )
+    pos = rng.uniform( 0.1, 0.9, ( 25, 2 ) )
+    w = rng.uniform( -0.05, 0.05, 25 )
+    assert np.allclose( oracle.measures( pos, w ), oracle.measures( pos, w + 2.5 ), atol = 1e-12 )
+    assert np.allclose( oracle.measures( pos ), oracle.measures( pos, np.full( 25, -3.0 ) ), atol = 1e-12 )
+
+
+def test_tree_walk_equals_brute_force( oracle ):
+    # pruning never loses a cut: the tree's cells are the all-seeds cells ( test_PowerDiagram.py:955-1036 )
+    rng = np.random.default_rng( 4 )
+    for label, pos, w in H.clouds( rng, 1500 ) + [ ( "ring", H.ring( 40, rng, 400 ), None ) ]:
+        a = oracle.measures( pos, w )
+        b = oracle.measures( pos, w, brute_force = True )
+        assert np.abs( a - b ).max() <= 1e-15, label
+
+
+def test_neighbours_and_areas_match_the_lifted_hull( oracle ):
+    # an independent diagram: the lower convex hull of ( p, |p|^2 - w ) ( scipy / Qhull ), and half-space
+    # intersections for the areas
+    rng = np.random.default_rng( 5 )
+    n = 300
+    pos = rng.uniform( 0.05, 0.95, ( n, 2 ) )
+    for w in ( None, rng.uniform( -2e-3, 2e-3, n ) ):
+        m, ( fi, fj, fc ) = oracle.measures( pos, w, facets = True )
+        ours = { ( min( i, j ), max( i, j ) ) for i, j in zip( fi.tolist(), fj.tolist() ) }
+        hull = H.lifted_neighbours( pos, w )
+        # the hull also links seeds whose common facet lies outside the box: ours must be a subset, and the extra
+        # pairs must be pairs whose bisector misses the box
+        assert ours <= hull
+        for i in range( 0, n, 7 ):
+            v = H.halfspace_cell( pos, w, i )
+            ref = 0.0 if v is None else H.polygon_area( v )
+            assert abs( m[ i ] - ref ) <= 1e-11 * max( ref, 1e-6 ), i
+
+
+def test_cell_vertices_and_ids( oracle ):
+    rng = np.random.default_rng( 6 )
+    pos = rng.uniform( 0, 1, ( 200, 2 ) )
+    w = rng.uniform( -1e-3, 1e-3, 200 )
+    for i in ( 0, 17, 123 ):
+        xy, ids = oracle.cell( pos, i, w )
+        xy2, ids2 = oracle.cell( pos, i, w, brute_force = True )
+        # the same cyclic sequence ( the cut order changes where the list starts )
+        assert len( xy ) == len( xy2 )
+        r = int( np.argmin( np.abs( xy2 - xy[ 0 ] ).sum( 1 ) ) )
+        assert np.allclose( xy, np.roll( xy2, -r, axis = 0 ), atol = 1e-15 ) and np.array_equal( ids, np.roll( ids2, -r ) )
+        assert H.polygon_area( xy ) > 0
+        # each vertex is on the bisectors of its two cuts ( cut q carries the edge [ v_q, v_q+1 ] )
+        for q in range( len( xy ) ):
+            for qq in ( q, q - 1 ):
+                j = ids[ qq ]
+                if j < 0:
+                    continue
+                lhs = ( xy[ q ] - pos[ i ] ) @ ( xy[ q ] - pos[ i ] ) - w[ i ]
+                rhs = ( xy[ q ] - pos[ j ] ) @ ( xy[ q ] - pos[ j ] ) - w[ j ]
+                assert abs( lhs - rhs ) < 1e-12
+
+
+# ---- image ( reference tests/test_PowerDiagram.py:1527-1660 ) ---------------------------------------------------
+
+
+def _img( values, lo = ( 0, 0 ), hi = ( 1, 1 ) ):
+    ny, nx = values.shape
+    return dict( values = values, x0 = lo[ 0 ], y0 = lo[ 1 ], hx = ( hi[ 0 ] - lo[ 0 ] ) / nx, hy = ( hi[ 1 ] - lo[ 1 ] ) / ny )
+
+
+def test_constant_image_is_lebesgue( oracle ):
+    rng = np.random.default_rng( 7 )
+    pos = rng.uniform( 0, 1, ( 400, 2 ) )
+    w = rng.uniform( -1e-3, 1e-3, 400 )
+    a = oracle.measures( pos, w, density = 2.5 )
+    b = oracle.measures( pos, w, density = _img( np.full( ( 13, 17 ), 2.5 ) ) )
+    assert np.allclose( a, b, rtol = 1e-12, atol = 1e-16 )
+
+
+def test_image_against_exact_clipping( oracle ):
+    rng = np.random.default_rng( 8 )
+    pos = rng.uniform( 0, 1, ( 120, 2 ) )
+    w = rng.uniform( -2e-3, 2e-3, 120 )
+    img = rng.uniform( 0.1, 2.0, ( 9, 11 ) )          # [ ny, nx ]
+    d = _img( img )
+    m = oracle.measures( pos, w, density = d )
+    for i in range( 0, 120, 5 ):
+        xy, _ = oracle.cell( pos, i, w )
+        ref = H.image_mass_of_polygon( xy, img, 0, 0, d[ "hx" ], d[ "hy" ] )
+        assert abs( m[ i ] - ref ) <= 1e-13 * max( ref, 1e-8 ), i
+    assert abs( m.sum() - img.mean() ) < 1e-13
+
+
+def test_one_pixel_moments_closed_form( oracle ):
+    # the single seed in a box against a constant density: barycentre = the box centre, cost = int |x - p|^2
+    r = oracle.solve( np.array( [ [ 0.3, 0.6 ], [ 0.7, 0.4 ] ] ), tol = 1e-12 )
+    # two seeds symmetric about the centre: at the solution, each takes half of the square
+    assert r[ "stats" ][ "status" ] == 1
+    assert np.allclose( r[ "masses" ], 0.5, atol = 1e-12 )
+    # closed form of the cost for a half-square cut by the bisector through the centre, by quadrature on a fine grid
+    g = ( np.arange( 2000 ) + 0.5 ) / 2000
+    X, Y = np.meshgrid( g, g, indexing = "ij" )
+    d0 = ( X - 0.3 ) ** 2 + ( Y - 0.6 ) ** 2
+    d1 = ( X - 0.7 ) ** 2 + ( Y - 0.4 ) ** 2
+    w = r[ "weights" ]
+    cost = np.where( d0 - w[ 0 ] <= d1 - w[ 1 ], d0, d1 ).mean()
+    assert abs( r[ "cost" ] - cost ) < 1e-6
+
+
+# ---- facets and the Hessian ( reference tests/test_SdotPlanNd.py:184-243 ) ---------------------------------------
+
+
+def _dense_laplacian( n, fi, fj, fc ):
+    L = np.zeros( ( n, n ) )
+    for i, j, c in zip( fi, fj, fc ):
+        L[ i, j ] -= c
+        L[ i, i ] += c
+    return L
+
+
+@pytest.mark.parametrize( "image", [ False, True ] )
+def test_hessian_is_the_jacobian_of_the_masses( oracle, image ):
+    rng = np.random.default_rng( 9 )
+    n = 40
+    pos = rng.uniform( 0.05, 0.95, ( n, 2 ) )
+    w = rng.uniform( -3e-3, 3e-3, n )
+    dens = _img( rng.uniform( 0.2, 3.0, ( 7, 6 ) ) ) if image else None
+    m, ( fi, fj, fc ) = oracle.measures( pos, w, density = dens, facets = True )
+    L = _dense_laplacian( n, fi, fj, fc )
+    h = 1e-6
+    for j in range( 0, n, 3 ):
+        e = np.zeros( n )
+        e[ j ] = h
+        J = ( oracle.measures( pos, w + e, density = dens ) - oracle.measures( pos, w - e, density = dens ) ) / ( 2 * h )
+        assert np.abs( J - L[ :, j ] ).max() <= 1e-7 * np.abs( L ).max(), j
+    # symmetric and zero row sums
+    assert np.abs( L - L.T ).max() <= 1e-12 * np.abs( L ).max()
+    assert np.abs( L.sum( 1 ) ).max() <= 1e-12 * np.abs( L ).max()
+
+
+def test_cg_solves_the_gauged_laplacian( oracle ):
+    rng = np.random.default_rng( 10 )
+    n = 400
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    _, ( fi, fj, fc ) = oracle.measures( pos, facets = True )
+    b = rng.normal( size = n )
+    b -= b.mean()
+    d, its = oracle.cg( n, fi, fj, fc, b, tol = 1e-13 )
+    assert its > 0 and d[ 0 ] == 0
+    L = _dense_laplacian( n, fi, fj, fc )
+    ref = np.zeros( n )
+    ref[ 1: ] = np.linalg.solve( L[ 1:, 1: ], b[ 1: ] )
+    assert np.abs( d - ref ).max() <= 1e-9 * np.abs( ref ).max()
+
+
+# ---- the solve ( reference tests/test_SdotPlanNd.py:39-330 ) -----------------------------------------------------
+
+
+def test_newton_matches_the_target_masses( oracle ):
+    rng = np.random.default_rng( 11 )
+    n = 2000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    nu = rng.uniform( 0.5, 1.5, n )
+    nu /= nu.sum()
+    r = oracle.solve( pos, nu, tol = 1e-10 / n * 1e3 )
+    assert r[ "stats" ][ "status" ] == 1
+    assert np.abs( r[ "masses" ] - nu ).max() <= 1e-10 / n * 1e3
+    assert r[ "weights" ][ 0 ] == 0
+    # the masses of the returned weights, measured again
+    assert np.abs( oracle.measures( pos, r[ "weights" ] ) - r[ "masses" ] ).max() < 1e-15
+
+
+def test_quadratic_tail_and_warm_start( oracle ):
+    # the last steps gain an order of magnitude each, a warm start from the solution ends in a few steps
+    # ( test_SdotPlanNd.py:245-276 )
+    rng = np.random.default_rng( 12 )
+    n = 3000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    img = H.bumps_image( 32, 32, rng ).T
+    dens = _img( img / img.mean() )
+    r = oracle.solve( pos, density = dens, tol = 1e-10 / n )
+    assert r[ "stats" ][ "status" ] == 1
+    res = r[ "history" ][ :, 4 ]
+    assert res[ -1 ] < 1e-10 / n
+    assert res[ -1 ] < 1e-2 * res[ -2 ]
+    r2 = oracle.solve( pos, density = dens, tol = 1e-10 / n, w0 = r[ "weights" ] + 1e-7 * rng.normal( size = n ) * 0 )
+    assert r2[ "stats" ][ "nb_iter" ] <= 1 and r2[ "stats" ][ "start" ] == 0 or r2[ "stats" ][ "nb_iter" ] == 0
+
+
+def test_similarity_start( oracle ):
+    # seeds outside the domain: the Voronoi start has empty cells, the similarity start has none ( test_SdotPlanNd.py:279-305 )
+    rng = np.random.default_rng( 13 )
+    pos = rng.uniform( 2, 3, ( 500, 2 ) )
+    r = oracle.solve( pos, tol = 1e-9 )
+    assert r[ "stats" ][ "start" ] == 2 and r[ "stats" ][ "status" ] == 1
+    assert r[ "stats" ][ "min_start_mass" ] > 0
+
+
+def test_residual_variants_reach_the_same_plan( oracle ):
+    # log / lin residuals give the same plan ( test_SdotPlanNd.py:327-347 )
+    rng = np.random.default_rng( 15 )
+    pos = rng.uniform( 0, 1, ( 800, 2 ) )
+    a = oracle.solve( pos, tol = 1e-12, residual = 1 )
+    b = oracle.solve( pos, tol = 1e-12, residual = 0 )
+    assert a[ "stats" ][ "status" ] == 1 and b[ "stats" ][ "status" ] == 1
+    assert np.abs( a[ "weights" ] - b[ "weights" ] ).max() < 1e-9 * np.abs( a[ "weights" ] ).max()
