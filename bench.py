@@ -1,0 +1,152 @@
+"""Throughput of the semi-discrete OT solve on one MI355X: Diracs solved per second to a max mass error
+of 1e-9.
+
+Workload ( one "step" = one complete solve, cold start, tree build included, inputs resident in HBM ):
+1 000 000 Diracs uniform in [0,1]^2 ( numpy default_rng(1) ), masses 1/n, target a 1024 x 1024
+piecewise-constant image density = 0.1 + a sum of 8 isotropic Gaussian bumps ( centres uniform in
+[0,1]^2, sigma uniform in [0.03, 0.15], default_rng(2) ), normalized; FP64 cells, damped Newton,
+stop at max_i |m_i - nu_i| <= 1e-9.
+
+Prints ONE JSON line ( see the keys below ), with the roofline of the dominant kernel measured live
+with HIP events on the library's stream, and a CPU baseline: the oracle's solve of a 100 000-Dirac
+sample of the same workload on the host cores.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname( os.path.abspath( __file__ ) )
+sys.path.insert( 0, os.path.join( ROOT, "sdot-ffi_amd" ) )
+
+METRIC = "Diracs/s to 1e-9 mass err (2D/3D 1M) vs plain GPU path; served p50/p99 latency"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+# algorithmic flops of the cell kernel's events ( 2D ): a plane / vertex side test, one vertex of a pruning test,
+# a new vertex ( intersection )
+FLOP_SIDE, FLOP_PRUNE, FLOP_NEWV = 4, 11, 10
+
+
+def workload( n = 1_000_000, res = 1024 ):
+    pos = np.random.default_rng( 1 ).uniform( 0, 1, ( n, 2 ) )
+    rng = np.random.default_rng( 2 )
+    c = rng.uniform( 0, 1, ( 8, 2 ) )
+    s = rng.uniform( 0.03, 0.15, 8 )
+    xs = ( np.arange( res ) + 0.5 ) / res
+    X, Y = np.meshgrid( xs, xs, indexing = "ij" )
+    v = np.full( ( res, res ), 0.1 )
+    for k in range( 8 ):
+        v += np.exp( -( ( X - c[ k, 0 ] ) ** 2 + ( Y - c[ k, 1 ] ) ** 2 ) / ( 2 * s[ k ] ** 2 ) )
+    return pos, v
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument( "--gpus", type = int, default = 1 )
+    ap.add_argument( "--steps", type = int, default = 3 )
+    ap.add_argument( "--warmup", type = int, default = 1 )
+    ap.add_argument( "--n", type = int, default = 1_000_000 )
+    ap.add_argument( "--res", type = int, default = 1024 )
+    ap.add_argument( "--tol", type = float, default = 1e-9 )
+    ap.add_argument( "--cpu-sample", type = int, default = 100_000 )
+    ap.add_argument( "--no-cpu-baseline", action = "store_true" )
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print( "bench.py: this build runs on ONE GPU ( the multi-GPU design is documented, not built ); use --gpus 1", file = sys.stderr )
+        sys.exit( 2 )
+
+    import torch
+    import sdot
+    from sdot import _native
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    dev = torch.device( "cuda", 0 )
+    torch.cuda.set_device( dev )
+
+    pos_h, img_h = workload( args.n, args.res )
+    pos = torch.as_tensor( pos_h, device = dev )
+    target = sdot.Image( img_h, origin = [ 0, 0 ], frame = np.eye( 2 ) / args.res )
+    settings = sdot.Iterative( tol = args.tol, continuation = "never", on_failure = "ignore" )
+    print( f"bench: n = { args.n }, image { args.res }^2, tol { args.tol }", file = sys.stderr, flush = True )
+
+    def step():
+        return sdot.OtProblem( sdot.SumOfDiracs( pos ), target ).solve( settings )
+
+    for w in range( args.warmup ):
+        sol = step()
+        print( f"bench: warmup { w }: { sol.stats[ 'status' ] }, { sol.stats[ 'nb_iter' ] } it, t { sol.stats[ 't_total' ]:.3f} s", file = sys.stderr, flush = True )
+    _native.set_timing( True )
+    _native.kernel_times( reset = True )
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sols = []
+    for k in range( args.steps ):
+        sols.append( step() )
+        print( f"bench: step { k }: { sols[ -1 ].stats[ 'status' ] }, { sols[ -1 ].stats[ 'nb_iter' ] } it", file = sys.stderr, flush = True )
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    times, spmv_bytes, ev = _native.kernel_times()
+    _native.set_timing( False )
+
+    st = sols[ -1 ].stats
+    nu = 1.0 / args.n
+    ms_step = 1e3 * elapsed / args.steps
+    # ---- rooflines: the cell kernel ( FP64 VALU ) and the CG's SpMV ( HBM ), the dominant one first
+    cells_ms, cells_launches = times[ "cells" ]
+    flops = FLOP_SIDE * ev[ "side_tests" ] + FLOP_PRUNE * ev[ "prune_vertex_tests" ] + FLOP_NEWV * ev[ "new_vertices" ]
+    cells_avg_ms = cells_ms / max( cells_launches, 1 )
+    cells_tf = ( flops / max( cells_launches, 1 ) ) / ( cells_avg_ms * 1e-3 ) / 1e12 if cells_launches else 0.0
+    roof_cells = dict( kernel = "k_cells_pass1", bound = "fp64-valu", achieved = cells_tf, peak = FP64_PEAK_TFLOPS, unit = "TFLOP/s",
+                       frac = cells_tf / FP64_PEAK_TFLOPS, traffic = None, avg_launch_ms = cells_avg_ms, launches = cells_launches,
+                       flops_per_launch = flops / max( cells_launches, 1 ) )
+    spmv_ms, spmv_launches = times[ "spmv" ]
+    spmv_avg_ms = spmv_ms / max( spmv_launches, 1 )
+    spmv_gbs = ( spmv_bytes / max( spmv_launches, 1 ) ) / ( spmv_avg_ms * 1e-3 ) / 1e9 if spmv_launches else 0.0
+    roof_spmv = dict( kernel = "k_cg_spmv_dot", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s", frac = spmv_gbs / HBM_PEAK_GBS,
+                      traffic = None, avg_launch_ms = spmv_avg_ms, launches_timed = spmv_launches,
+                      bytes_per_launch = spmv_bytes / max( spmv_launches, 1 ) )
+    # the SpMV is timed one launch in sixteen: its share of the step is estimated from the CG iteration count
+    spmv_total_ms = spmv_avg_ms * st[ "lin_nb_iter" ] * args.steps
+    dominant, other = ( roof_spmv, roof_cells ) if spmv_total_ms > cells_ms else ( roof_cells, roof_spmv )
+
+    line = dict( metric = METRIC, value = args.n * args.steps / elapsed, unit = "Diracs/s", n_gpus = 1, steps = args.steps, warmup = args.warmup,
+                 ms_per_step = ms_step, higher_is_better = True, scaling = "weak", vs_baseline = None, dtype = "f64", data = "synthetic",
+                 config = dict( workload = f"2D { args.n } Diracs uniform in [0,1]^2, { args.res }^2 image of 8 Gaussian bumps + 0.1 floor, "
+                                           f"max mass error <= { args.tol:g} ( absolute )",
+                                n = args.n, image = [ args.res, args.res ], tol_abs = args.tol, start = "cold ( Voronoi ), tree included",
+                                step = "trials", linear = "jacobi-pcg", cells = "fp64" ),
+                 roofline = dominant, roofline_secondary = other,
+                 solve = dict( status = st[ "status" ], residual_abs = st[ "residual" ], residual_rel = st[ "residual" ] / nu,
+                               nb_iter = st[ "nb_iter" ], nb_diag = st[ "nb_diag" ], lin_nb_iter = st[ "lin_nb_iter" ],
+                               t_diag = st[ "t_diag" ], t_majorant = st[ "t_majorant" ], t_lin = st[ "t_lin" ], t_total = st[ "t_total" ],
+                               device_bytes = _native.device_bytes() ) )
+
+    if not args.no_cpu_baseline:
+        line[ "cpu_baseline" ] = cpu_baseline( pos_h, img_h, args )
+    print( json.dumps( line ), flush = True )
+
+
+def cpu_baseline( pos_h, img_h, args ):
+    """the oracle's solve of the first `cpu_sample` Diracs of the same workload, on the host cores"""
+    sys.path.insert( 0, os.path.join( ROOT, "oracle" ) )
+    import oracle
+    m = min( args.cpu_sample, len( pos_h ) )
+    cores = min( os.cpu_count() or 1, 16 )
+    oracle.set_threads( cores )
+    v = img_h / ( img_h.mean() )                         # ( the unit square: mean value = mass )
+    dens = dict( values = np.ascontiguousarray( v.T ), x0 = 0.0, y0 = 0.0, hx = 1.0 / args.res, hy = 1.0 / args.res )
+    tol = args.tol * len( pos_h ) / m                    # the same relative accuracy per Dirac
+    t0 = time.perf_counter()
+    r = oracle.solve( pos_h[ :m ], density = dens, tol = tol, lin_tol = 1e-6 )
+    dt = time.perf_counter() - t0
+    return dict( value = m / dt, unit = "Diracs/s", cores = cores, kind = "port",
+                 sample = f"the oracle ( C++ FP64 restatement, Jacobi-PCG ) solving the first { m } Diracs of the same workload to "
+                          f"max |m - nu| <= { tol:.1e} ( the same error relative to nu ), { r[ 'stats' ][ 'nb_iter' ]:.0f} Newton iterations, "
+                          f"{ dt:.1f} s, status { r[ 'stats' ][ 'status' ]:.0f}" )
+
+
+if __name__ == "__main__":
+    main()

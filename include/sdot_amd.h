@@ -39,6 +39,17 @@ const char *sdot_amd_last_error( int *kind, int *code, long long *value );
 long long   sdot_amd_device_bytes( void );
 void        sdot_amd_release( void );
 
+/* LIVE KERNEL TIMING ( no counterpart in the reference, which times stages on the host: `Newton2D.cuh`'s
+ * `StageTimer` ). When on, HIP events are recorded on the launch stream around each launch of the cell
+ * kernels ( index 0: first pass, 1: overflow pass ), one SpMV launch in sixteen of the CG ( index 2 ) and
+ * the majorant refresh ( index 3 ).
+ * `sdot_amd_kernel_times` returns, per index, the summed milliseconds and the launches timed, the
+ * algorithmic bytes of the timed SpMV launches, and the cell kernels' event counters summed over every
+ * diagram since the last reset ( HOST [ 5 ]: side tests, pruning vertex tests, new vertices, node
+ * visits, cuts ); `reset` clears them afterwards. */
+int sdot_amd_set_timing( int on );
+int sdot_amd_kernel_times( double *ms, long long *launches, double *spmv_bytes, unsigned long long *events, int reset );
+
 /* ONE DIAGRAM: the masses of the power cells of `( positions, weights )` in the box
  * `box = { lo_x, lo_y, hi_x, hi_y }` ( HOST ) against a density, and optionally the facets
  * `c_ij = int_facet rho / ( 2 |p_i - p_j| )` as triplets ( rows in user order, each row in its cell's

@@ -16,6 +16,11 @@ Arena &arena() {
     return a[ dev ];
 }
 
+Timing &timing() {
+    static Timing t;
+    return t;
+}
+
 template<class T>
 T *pinned( const std::string &name, size_t count ) {
     static std::mutex mtx;
@@ -141,6 +146,31 @@ void sdot_amd_release( void ) {
     arena().release();
 }
 
+int sdot_amd_set_timing( int on ) {
+    return guarded( [&] {
+        timing().resolve();
+        timing().on = on != 0;
+    } );
+}
+
+int sdot_amd_kernel_times( double *ms, long long *launches, double *spmv_bytes, unsigned long long *events, int reset ) {
+    return guarded( [&] {
+        Timing &T = timing();
+        T.resolve();
+        for ( int k = 0; k < TK_NB; ++k ) {
+            if ( ms ) ms[ k ] = T.ms[ k ];
+            if ( launches ) launches[ k ] = T.launches[ k ];
+        }
+        if ( spmv_bytes ) *spmv_bytes = T.spmv_bytes;
+        unsigned long long *ev = arena().get<unsigned long long>( "events", 8 );
+        if ( events ) SDOT_HIP( hipMemcpy( events, ev, 5 * sizeof( unsigned long long ), hipMemcpyDeviceToHost ) );
+        if ( reset ) {
+            T.reset();
+            SDOT_HIP( hipMemset( ev, 0, 8 * sizeof( unsigned long long ) ) );
+        }
+    } );
+}
+
 int sdot_amd_measures_2d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
                           double rho, const double *img, int nx, int ny, const double *img_geo, double *masses, long long *facet_i,
                           long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets, unsigned long long *events ) {
@@ -192,11 +222,8 @@ int sdot_amd_build_tree_2d( void *stream, long long n, const double *positions, 
     return guarded( [&] {
         SDOT_CHECK( n >= 1 && positions && leaf >= 1, "build_tree: positions, n >= 1 and leaf >= 1 are required" );
         hipStream_t st = hipStream_t( stream );
-        std::vector<double> pos( size_t( n ) * 2 );
-        SDOT_HIP( hipMemcpyAsync( pos.data(), positions, sizeof( double ) * size_t( n ) * 2, hipMemcpyDeviceToHost, st ) );
-        SDOT_HIP( hipStreamSynchronize( st ) );
         DevTree t;
-        build_tree( t, n, pos.data(), leaf, st, "abi_tree" );
+        build_tree_device( t, n, positions, leaf, st, "abi_tree" );
         const I64 N = t.nb_nodes;
         std::vector<int> pre, height;
         preorder_tables( t.depth, pre, height );

@@ -26,8 +26,8 @@ struct DiagramOut {
     I64       facet_cap = 0;
     // moments ( instead of facets ): barycentres ( absolute, rank order ) and per-cell cost
     double   *bary = nullptr, *cost = nullptr;
-    // counters on the device: [ 0 ] facets used, [ 1 ] overflowed cells, [ 2 ] failures,
-    // [ 3 ] the first failing rank
+    // counters on the device: [ 0 ] facets used, [ 1 ] cells past the first pass, [ 2 ] failures,
+    // [ 3 ] the first failing rank, [ 4 ] cells past the second pass
     unsigned *counters = nullptr;
     int      *ovf_list = nullptr;
     unsigned long long *events = nullptr;   ///< side tests, pruning vertex tests, new vertices, node visits, cuts
@@ -35,8 +35,10 @@ struct DiagramOut {
 
 /// the vertex slots of the passes ( the first in LDS, the second in global slots )
 constexpr int CELL_BLOCK = 128;
-constexpr int CAP1 = 16;
-constexpr int CAP2 = 1024;
+constexpr int CAP1 = 16;             ///< first pass, LDS
+constexpr int CAP2 = 96;             ///< second pass, LDS ( one wave per block )
+constexpr int CAP3 = 2048;           ///< third pass, global slots: the vertex limit of a cell
+constexpr int PASS2_BLOCKS = 1024;
 constexpr int OVF_SLOTS = 4096;
 
 enum CellMode { MODE_MASSES = 0, MODE_FACETS = 1, MODE_MOMENTS = 2 };

@@ -156,9 +156,11 @@ struct Walk {
 };
 
 /// builds the cell of rank `k` in `c`; returns the vertex count ( 0: empty ), or -1 when the slots
-/// were too few ( the cell is then left for the next pass )
+/// were too few ( the cell is then left for the next pass ). One loop iteration does ONE thing of about
+/// the same cost -- a cut by one seed of the open leaf, or one node's test ( after a pop if needed ) --
+/// so the lanes of a wave, each at its own place in its walk, stay in step.
 template<class S>
-__device__ int make_cell( const Walk &wk, int k, const double *lo, const double *hi, const S &c, int cap, Ev &ev ) {
+__device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *lo, const double *hi, const S &c, int cap, Ev &ev ) {
     const double4 p = wk.seeds[ k ];
     const double p0x = p.x, p0y = p.y, w0 = p.z;
     c.X( 0 ) = lo[ 0 ] - p0x; c.Y( 0 ) = lo[ 1 ] - p0y;
@@ -169,19 +171,38 @@ __device__ int make_cell( const Walk &wk, int k, const double *lo, const double 
     int nb = 4;
     int m = 0, h = wk.depth;
     unsigned long long pend = 0, left = 0;
+    int q = 0, qe = 0;                                   // the open leaf's remaining seeds
+    bool visit = true;                                   // ( m, h ) is still to be tested
     for ( ;; ) {
+        if ( q < qe ) {
+            const int s = q++;
+            if ( s == k ) continue;
+            const double4 sd = wk.seeds[ s ];
+            const double dx = sd.x - p0x, dy = sd.y - p0y;
+            const double off = ( dx * dx + dy * dy + w0 - sd.z ) / 2;
+            const int res = cut( c, nb, cap, dx, dy, off, s, ev );
+            if ( res == CUT_EMPTY ) return 0;
+            if ( res == CUT_NO_ROOM ) return -1;
+            continue;
+        }
+        if ( ! visit ) {                                  // pop: climb to the lowest pending height, take the sibling
+            if ( ! pend ) break;
+            const int hp = __ffsll( (long long) pend ) - 1;
+            while ( h < hp ) {
+                m = ( ( left >> h ) & 1 ) ? m - 1 : m - ( 1 << h );
+                ++h;
+            }
+            const unsigned long long bit = 1ull << hp;
+            if ( left & bit ) { m = m - 1 + ( 1 << hp ); left &= ~bit; }
+            else              { m = m + 1 - ( 1 << hp ); left |= bit; }
+            pend &= ~bit;
+        }
+        visit = false;
         const NodeRec r = wk.nodes[ m ];
         if ( r.beg < r.end && can_cut( c, nb, r, p0x, p0y, w0, ev ) ) {
             if ( h <= 1 ) {
-                for ( int q = r.beg; q < r.end; ++q ) {
-                    if ( q == k ) continue;
-                    const double4 s = wk.seeds[ q ];
-                    const double dx = s.x - p0x, dy = s.y - p0y;
-                    const double off = ( dx * dx + dy * dy + w0 - s.z ) / 2;
-                    const int res = cut( c, nb, cap, dx, dy, off, q, ev );
-                    if ( res == CUT_EMPTY ) return 0;
-                    if ( res == CUT_NO_ROOM ) return -1;
-                }
+                q = r.beg;
+                qe = r.end;
             } else {
                 const int lc = m + 1, rc = m + ( 1 << ( h - 1 ) );
                 const bool go_left = proximity( wk.nodes, lc, p0x, p0y ) <= proximity( wk.nodes, rc, p0x, p0y );
@@ -190,19 +211,9 @@ __device__ int make_cell( const Walk &wk, int k, const double *lo, const double 
                 left = go_left ? ( left | bit ) : ( left & ~bit );
                 m = go_left ? lc : rc;
                 h -= 1;
-                continue;
+                visit = true;
             }
         }
-        if ( ! pend ) break;
-        const int hp = __ffsll( (long long) pend ) - 1;
-        while ( h < hp ) {
-            m = ( ( left >> h ) & 1 ) ? m - 1 : m - ( 1 << h );
-            ++h;
-        }
-        const unsigned long long bit = 1ull << hp;
-        if ( left & bit ) { m = m - 1 + ( 1 << hp ); left &= ~bit; }
-        else              { m = m + 1 - ( 1 << hp ); left |= bit; }
-        pend &= ~bit;
     }
     return nb;
 }
@@ -239,18 +250,19 @@ struct Grid {
 
 /// walks the segment `a -> b` ( seed frame ) across the pixel grid; `f( i, j, ta, tb )` per piece
 template<class F>
-__device__ void grid_walk( const Grid &g, double ax, double ay, double bx, double by, F &&f ) {
+__device__ __forceinline__ void grid_walk( const Grid &g, double ax, double ay, double bx, double by, F &&f ) {
     const double gax = ( ax - g.x0 ) / g.hx, gay = ( ay - g.y0 ) / g.hy;
     const double gbx = ( bx - g.x0 ) / g.hx, gby = ( by - g.y0 ) / g.hy;
     const double ddx = gbx - gax, ddy = gby - gay;
+    const int sx = ddx > 0 ? 1 : ( ddx < 0 ? -1 : 0 ), sy = ddy > 0 ? 1 : ( ddy < 0 ? -1 : 0 );
+    const double rx = sx ? 1.0 / ddx : 0.0, ry = sy ? 1.0 / ddy : 0.0;
     // the next vertical / horizontal grid line crossed, as a parameter
-    int sx = ddx > 0 ? 1 : ( ddx < 0 ? -1 : 0 ), sy = ddy > 0 ? 1 : ( ddy < 0 ? -1 : 0 );
     double nxl = sx > 0 ? floor( gax ) + 1 : ceil( gax ) - 1;
     double nyl = sy > 0 ? floor( gay ) + 1 : ceil( gay ) - 1;
     double t0 = 0;
     for ( int guard = 0; guard < 1 << 20; ++guard ) {
-        const double tx = sx ? ( nxl - gax ) / ddx : 2.0;
-        const double ty = sy ? ( nyl - gay ) / ddy : 2.0;
+        const double tx = sx ? ( nxl - gax ) * rx : 2.0;
+        const double ty = sy ? ( nyl - gay ) * ry : 2.0;
         double t1 = fmin( fmin( tx, ty ), 1.0 );
         if ( t1 < t0 ) t1 = t0;
         const double tm = 0.5 * ( t0 + t1 );
@@ -265,31 +277,21 @@ __device__ void grid_walk( const Grid &g, double ax, double ay, double bx, doubl
     }
 }
 
+/// edge `e` of the cell against the image, in one walk: its term of `mass = loop_integral F0 dy` and
+/// its integral `int_edge rho dl`
 template<class S>
-__device__ void image_mass( const Grid &g, const S &c, int nb, double &mass ) {
-    double acc = 0;
-    for ( int e = 0; e < nb; ++e ) {
-        const int f = e + 1 < nb ? e + 1 : 0;
-        const double ax = c.X( e ), ay = c.Y( e ), bx = c.X( f ), by = c.Y( f );
-        if ( ay == by ) continue;                        // no `dy`: nothing for the mass
-        grid_walk( g, ax, ay, bx, by, [&]( int i, int j, double t0, double t1 ) {
-            const double tm = 0.5 * ( t0 + t1 );
-            const double xm = fma( bx - ax, tm, ax );
-            acc += g.F0( i, j, xm ) * ( ( by - ay ) * ( t1 - t0 ) );
-        } );
-    }
-    mass = acc;
-}
-
-/// `int_edge rho dl` of edge `e`
-template<class S>
-__device__ double image_edge_mass( const Grid &g, const S &c, int nb, int e ) {
+__device__ __forceinline__ void image_edge( const Grid &g, const S &c, int nb, int e, double &mass_term, double &line ) {
     const int f = e + 1 < nb ? e + 1 : 0;
     const double ax = c.X( e ), ay = c.Y( e ), bx = c.X( f ), by = c.Y( f );
-    const double len = sqrt( ( bx - ax ) * ( bx - ax ) + ( by - ay ) * ( by - ay ) );
-    double acc = 0;
-    grid_walk( g, ax, ay, bx, by, [&]( int i, int j, double t0, double t1 ) { acc += g.val( i, j ) * ( t1 - t0 ); } );
-    return acc * len;
+    const double ex = bx - ax, ey = by - ay;
+    double m = 0, l = 0;
+    grid_walk( g, ax, ay, bx, by, [&]( int i, int j, double t0, double t1 ) {
+        const double dt = t1 - t0;
+        m += g.F0( i, j, fma( ex, 0.5 * ( t0 + t1 ), ax ) ) * ( ey * dt );
+        l += g.val( i, j ) * dt;
+    } );
+    mass_term = m;
+    line = l * sqrt( ex * ex + ey * ey );
 }
 
 /// moments of the cell against the image: `m`, `int v rho`, `int |v|^2 rho` ( seed frame ), by Green
@@ -414,25 +416,36 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
         P.out.mass[ k ] = m;
         return;
     }
-    double mass = 0;
-    if ( nb > 0 ) {
-        if ( P.dn.kind == 1 ) image_mass( g, c, nb, mass );
-        else                  mass = P.dn.rho * shoelace( c, nb );
+    if ( P.mode != MODE_FACETS ) {
+        double mass = 0;
+        if ( nb > 0 ) {
+            if ( P.dn.kind == 1 ) {
+                for ( int e = 0; e < nb; ++e ) { double mt, ln; image_edge( g, c, nb, e, mt, ln ); mass += mt; }
+            } else
+                mass = P.dn.rho * shoelace( c, nb );
+        }
+        P.out.mass[ k ] = mass;
+        return;
     }
-    P.out.mass[ k ] = mass;
-    if ( P.mode != MODE_FACETS ) return;
     const I64 at = I64( base ) + pre;
     const bool fits = I64( base ) + total <= P.out.facet_cap;
     P.out.row_off[ k ] = int( fits ? at : 0 );
     P.out.row_cnt[ k ] = int( cnt );
-    double dia = 0;
+    double dia = 0, mass = 0;
     int q = 0;
     for ( int e = 0; e < nb; ++e ) {
         const int id = c.I( e );
+        double fm = 0;
+        if ( P.dn.kind == 1 ) {
+            double mt;
+            image_edge( g, c, nb, e, mt, fm );
+            mass += mt;
+        } else if ( id >= 0 ) {
+            const int f = e + 1 < nb ? e + 1 : 0;
+            const double ex = c.X( f ) - c.X( e ), ey = c.Y( f ) - c.Y( e );
+            fm = P.dn.rho * sqrt( ex * ex + ey * ey );
+        }
         if ( id < 0 ) continue;
-        const int f = e + 1 < nb ? e + 1 : 0;
-        const double ex = c.X( f ) - c.X( e ), ey = c.Y( f ) - c.Y( e );
-        const double fm = P.dn.kind == 1 ? image_edge_mass( g, c, nb, e ) : P.dn.rho * sqrt( ex * ex + ey * ey );
         const double4 s = P.wk.seeds[ id ];
         const double dx = s.x - p.x, dy = s.y - p.y;
         const double d2 = dx * dx + dy * dy;
@@ -444,6 +457,8 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
         dia += cij;
         ++q;
     }
+    if ( P.dn.kind != 1 && nb > 0 ) mass = P.dn.rho * shoelace( c, nb );
+    P.out.mass[ k ] = mass;
     P.out.dia[ k ] = dia > 0 ? dia : 1.0;
 }
 
@@ -486,22 +501,51 @@ __global__ __launch_bounds__( CELL_BLOCK ) void k_cells_pass1( Params P ) {
     flush_events( P, ev );
 }
 
-/// the second pass: the overflowed cells, `CAP2` slots each in global memory, a fixed grid of
-/// `OVF_SLOTS` lanes striding over the list ( its length only the device knows )
-__global__ __launch_bounds__( 64 ) void k_cells_pass2( Params P, double *gx, double *gy, int *gid ) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    const Store<1> c{ gx + size_t( slot ) * CAP2, gy + size_t( slot ) * CAP2, gid + size_t( slot ) * CAP2 };
+/// the second pass: the cells that outgrew `CAP1`, one lane per cell with `CAP2` slots in LDS, one wave per
+/// block and a fixed grid striding over the list ( its length only the device knows ); the cells that
+/// outgrow `CAP2` too are listed again for the third pass
+__global__ __launch_bounds__( 64 ) void k_cells_pass2( Params P ) {
+    __shared__ double sx[ CAP2 * 64 ], sy[ CAP2 * 64 ];
+    __shared__ int sid[ CAP2 * 64 ];
+    const Store<64> c{ sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x };
     const unsigned count = P.out.counters[ 1 ];
-    const unsigned rounds = ( count + OVF_SLOTS - 1 ) / OVF_SLOTS;
+    const unsigned lanes = gridDim.x * 64;
+    const unsigned rounds = ( count + lanes - 1 ) / lanes;
     Ev ev;
     for ( unsigned r = 0; r < rounds; ++r ) {
-        const unsigned idx = r * OVF_SLOTS + unsigned( slot );
+        const unsigned idx = r * lanes + blockIdx.x * 64 + threadIdx.x;
         const bool live = idx < count;
         const int k = live ? P.out.ovf_list[ idx ] : 0;
         int nb = 0;
         bool active = live;
         if ( live ) {
             nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP2, ev );
+            if ( nb < 0 ) {
+                const unsigned at = atomicAdd( &P.out.counters[ 4 ], 1u );
+                P.out.ovf_list[ P.n + int( at ) ] = k;              // ( the second half: the first holds this pass's input )
+                active = false;
+            }
+        }
+        finish_cell( P, k, c, nb, active, false );
+    }
+    flush_events( P, ev );
+}
+
+/// the third pass: `CAP3` slots per cell in global memory, `OVF_SLOTS` lanes; past that a cell is a failure
+__global__ __launch_bounds__( 64 ) void k_cells_pass3( Params P, double *gx, double *gy, int *gid ) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const Store<1> c{ gx + size_t( slot ) * CAP3, gy + size_t( slot ) * CAP3, gid + size_t( slot ) * CAP3 };
+    const unsigned count = P.out.counters[ 4 ];
+    const unsigned rounds = ( count + OVF_SLOTS - 1 ) / OVF_SLOTS;
+    Ev ev;
+    for ( unsigned r = 0; r < rounds; ++r ) {
+        const unsigned idx = r * OVF_SLOTS + unsigned( slot );
+        const bool live = idx < count;
+        const int k = live ? P.out.ovf_list[ P.n + int( idx ) ] : 0;
+        int nb = 0;
+        bool active = live;
+        if ( live ) {
+            nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP3, ev );
             if ( nb < 0 ) {
                 if ( atomicAdd( &P.out.counters[ 2 ], 1u ) == 0 ) P.out.counters[ 3 ] = unsigned( k );
                 active = false;
@@ -515,13 +559,13 @@ __global__ __launch_bounds__( 64 ) void k_cells_pass2( Params P, double *gx, dou
 }
 
 __global__ void k_reset_counters( unsigned *c ) {
-    if ( threadIdx.x < 4 ) c[ threadIdx.x ] = 0;
+    if ( threadIdx.x < 8 ) c[ threadIdx.x ] = 0;
 }
 
 void cells_scratch( DiagramOut &out, I64 n, const std::string &tag ) {
     Arena &A = arena();
-    out.counters = A.get<unsigned>( tag + ".counters", 4 );
-    out.ovf_list = A.get<int>( tag + ".ovf_list", size_t( n ) );
+    out.counters = A.get<unsigned>( tag + ".counters", 8 );
+    out.ovf_list = A.get<int>( tag + ".ovf_list", 2 * size_t( n ) );
     if ( ! out.events ) {
         out.events = A.get<unsigned long long>( "events", 8 );
         SDOT_HIP( hipMemset( out.events, 0, 8 * sizeof( unsigned long long ) ) );
@@ -540,14 +584,21 @@ void run_cells( const DevTree &t, const double *lo, const double *hi, const Dens
     P.mode = mode;
     k_reset_counters<<<1, 64, 0, st>>>( out.counters );
     check_launch( "k_reset_counters" );
+    Timing::Pending tp;
+    bool timed = timing().begin( TK_CELLS, st, tp );
     k_cells_pass1<<<blocks_for( t.n, CELL_BLOCK ), CELL_BLOCK, 0, st>>>( P );
     check_launch( "k_cells_pass1" );
-    Arena &A = arena();
-    double *gx = A.get<double>( "ovf.x", size_t( OVF_SLOTS ) * CAP2 );
-    double *gy = A.get<double>( "ovf.y", size_t( OVF_SLOTS ) * CAP2 );
-    int *gid = A.get<int>( "ovf.id", size_t( OVF_SLOTS ) * CAP2 );
-    k_cells_pass2<<<OVF_SLOTS / 64, 64, 0, st>>>( P, gx, gy, gid );
+    if ( timed ) timing().end( tp, st );
+    timed = timing().begin( TK_CELLS_OVF, st, tp );
+    k_cells_pass2<<<PASS2_BLOCKS, 64, 0, st>>>( P );
     check_launch( "k_cells_pass2" );
+    Arena &A = arena();
+    double *gx = A.get<double>( "ovf.x", size_t( OVF_SLOTS ) * CAP3 );
+    double *gy = A.get<double>( "ovf.y", size_t( OVF_SLOTS ) * CAP3 );
+    int *gid = A.get<int>( "ovf.id", size_t( OVF_SLOTS ) * CAP3 );
+    k_cells_pass3<<<OVF_SLOTS / 64, 64, 0, st>>>( P, gx, gy, gid );
+    check_launch( "k_cells_pass3" );
+    if ( timed ) timing().end( tp, st );
 }
 
 } // namespace sdot_amd

@@ -95,6 +95,63 @@ Arena &arena();
 template<class T>
 T *pinned( const std::string &name, size_t count );
 
+// ---- live kernel timing ------------------------------------------------------------------------
+
+/// the kernels whose launches can be timed live ( HIP events on their own stream, resolved on demand )
+enum TimedKernel { TK_CELLS = 0, TK_CELLS_OVF, TK_SPMV, TK_MAJORANTS, TK_NB };
+
+struct Timing {
+    bool on = false;
+    struct Pending { int k; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[ TK_NB ] = {};
+    long long launches[ TK_NB ] = {};
+    double spmv_bytes = 0;          ///< algorithmic bytes of the timed SpMV launches
+    std::mutex mtx;
+
+    hipEvent_t take() {
+        if ( pool.empty() ) { hipEvent_t e; if ( hipEventCreate( &e ) != hipSuccess ) return nullptr; return e; }
+        hipEvent_t e = pool.back(); pool.pop_back(); return e;
+    }
+    /// a pair of events around one launch; `begin` returns false when timing is off
+    bool begin( int k, hipStream_t st, Pending &p ) {
+        if ( ! on ) return false;
+        std::lock_guard<std::mutex> lk( mtx );
+        p.k = k; p.a = take(); p.b = take();
+        if ( ! p.a || ! p.b ) return false;
+        (void) hipEventRecord( p.a, st );
+        return true;
+    }
+    void end( Pending &p, hipStream_t st ) {
+        (void) hipEventRecord( p.b, st );
+        std::lock_guard<std::mutex> lk( mtx );
+        pending.push_back( p );
+    }
+    /// waits for the recorded pairs and adds them up
+    void resolve() {
+        std::lock_guard<std::mutex> lk( mtx );
+        for ( auto &p : pending ) {
+            float t = 0;
+            if ( hipEventSynchronize( p.b ) == hipSuccess && hipEventElapsedTime( &t, p.a, p.b ) == hipSuccess ) {
+                ms[ p.k ] += t;
+                launches[ p.k ] += 1;
+            }
+            pool.push_back( p.a );
+            pool.push_back( p.b );
+        }
+        pending.clear();
+    }
+    void reset() {
+        resolve();
+        std::lock_guard<std::mutex> lk( mtx );
+        for ( int k = 0; k < TK_NB; ++k ) { ms[ k ] = 0; launches[ k ] = 0; }
+        spmv_bytes = 0;
+    }
+};
+
+Timing &timing();
+
 // ---- launch shapes -----------------------------------------------------------------------------
 
 constexpr int RED_BLOCKS  = 512;    ///< partial sums of every deterministic reduction ( fixed shape )

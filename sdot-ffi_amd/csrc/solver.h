@@ -28,7 +28,7 @@ enum Residual { R_LIN = 0, R_LOG = 1, R_POWER = 2 };
 
 /// the sparse Laplacian of one diagram, rank order ( rows of `DiagramOut` )
 struct Csr {
-    I64 n = 0;
+    I64 n = 0, nnz = 0;
     const int *row_off = nullptr, *row_cnt = nullptr, *col = nullptr;
     const double *val = nullptr, *dia = nullptr;
 };

@@ -46,6 +46,9 @@ int max_depth_for( I64 n, I64 leaf );
 /// buffers are named with `tag` so two trees can live side by side
 void build_tree( DevTree &t, I64 n, const double *pos_host, int leaf, hipStream_t st, const std::string &tag );
 
+/// builds the same tree on the device from `pos_dev[ n ][ 2 ]` ( device, user order ): no host round trip
+void build_tree_device( DevTree &t, I64 n, const double *pos_dev, int leaf, hipStream_t st, const std::string &tag );
+
 /// rank-ordered weights are in `t.seeds[ k ].z`: the majorants of every node and the walk records
 void refresh_majorants( DevTree &t, hipStream_t st );
 

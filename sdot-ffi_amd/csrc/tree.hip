@@ -401,6 +401,8 @@ __global__ void k_major_final( const double *S, int N, const double *fit, const 
 
 void refresh_majorants( DevTree &t, hipStream_t st ) {
     const int N = int( t.nb_nodes ), depth = t.depth;
+    Timing::Pending tp;
+    const bool timed = timing().begin( TK_MAJORANTS, st, tp );
     // the leaves, then each level up
     {
         const int L = depth - 1, first = ( 1 << L ) - 1, count = 1 << L;
@@ -418,6 +420,7 @@ void refresh_majorants( DevTree &t, hipStream_t st ) {
     check_launch( "k_major_spread" );
     k_major_final<<<blocks_for( N, 256 ), 256, 0, st>>>( t.stat, N, t.fit, t.rkeys, t.node_box, t.node_beg, t.node_end, t.nodes );
     check_launch( "k_major_final" );
+    if ( timed ) timing().end( tp, st );
 }
 
 } // namespace sdot_amd

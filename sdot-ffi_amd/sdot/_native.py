@@ -26,7 +26,8 @@ CARD_OPTIONS = [ "tol_abs", "tol_rel", "t_min", "mult_ok", "factor", "maxit", "m
                  "img_x0", "img_y0", "img_hx", "img_hy", "img_nx", "img_ny", "agg_margin", "agg_gap", "agg_nb_dups" ]
 
 #: what the library exports ( every symbol of `include/sdot_amd.h` )
-EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_bytes", "sdot_amd_release", "sdot_amd_measures_2d",
+EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_bytes", "sdot_amd_release", "sdot_amd_set_timing",
+            "sdot_amd_kernel_times", "sdot_amd_measures_2d",
             "sdot_amd_solve_2d", "sdot_amd_tree_nodes", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
             "sdot_amd_linear_solves_2d" ]
 
@@ -52,6 +53,9 @@ def lib():
         if not os.path.exists( LIB_PATH ):
             raise RuntimeError( f"libsdot_amd.so is not built ( expected at { LIB_PATH } ): `make -C sdot-ffi_amd/csrc` "
                                 "or `python -c 'import __graft_entry__ as g; g.build()'`" )
+        # ONE HIP runtime per process: torch's is loaded first, so that the library's `libamdhip64.so.7` resolves to
+        # it ( the same soname ) instead of loading the system's beside it -- two runtimes do not share devices
+        torch().cuda.is_available()
         L = ctypes.CDLL( LIB_PATH )
         L.sdot_amd_abi_version.restype = _I
         L.sdot_amd_last_error.restype = ctypes.c_char_p
@@ -64,6 +68,10 @@ def lib():
         L.sdot_amd_build_tree_2d.argtypes = [ _P, _LL, _P, _I, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_refresh_majorants_2d.argtypes = [ _P, _LL, _LL, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_linear_solves_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _LL, _P, _P, _P ]
+        L.sdot_amd_set_timing.argtypes = [ _I ]
+        L.sdot_amd_set_timing.restype = _I
+        L.sdot_amd_kernel_times.argtypes = [ _P, _P, _P, _P, _I ]
+        L.sdot_amd_kernel_times.restype = _I
         for name in ( "sdot_amd_measures_2d", "sdot_amd_solve_2d", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
                       "sdot_amd_linear_solves_2d" ):
             getattr( L, name ).restype = _I
@@ -223,6 +231,25 @@ def linear_solves_2d( row, col, val, dia, rhs, tol = 1e-6 ):
     _check( lib().sdot_amd_linear_solves_2d( _stream(), n, _ptr( row ), _ptr( col ), _ptr( val ), _ptr( dia ), _ptr( rhs.contiguous() ), k,
                                              optp, _ptr( sol ), its.ctypes.data_as( _P ) ) )
     return sol, its.astype( int )
+
+
+TIMED_KERNELS = [ "cells", "cells_overflow", "spmv", "majorants" ]
+
+
+def set_timing( on ):
+    _check( lib().sdot_amd_set_timing( int( bool( on ) ) ) )
+
+
+def kernel_times( reset = False ):
+    """per timed kernel: `( ms, launches )`, the timed SpMV bytes, and the cell kernels' event counts"""
+    ms = np.zeros( 4 )
+    la = np.zeros( 4, np.int64 )
+    sb = np.zeros( 1 )
+    ev = np.zeros( 5, np.uint64 )
+    _check( lib().sdot_amd_kernel_times( ms.ctypes.data_as( _P ), la.ctypes.data_as( _P ), sb.ctypes.data_as( _P ), ev.ctypes.data_as( _P ),
+                                         int( bool( reset ) ) ) )
+    return ( { k: ( float( ms[ i ] ), int( la[ i ] ) ) for i, k in enumerate( TIMED_KERNELS ) }, float( sb[ 0 ] ),
+             dict( zip( ( "side_tests", "prune_vertex_tests", "new_vertices", "node_visits", "cuts" ), [ int( x ) for x in ev ] ) ) )
 
 
 def device_bytes():

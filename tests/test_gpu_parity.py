@@ -62,6 +62,13 @@ def test_constant_density_masses_and_facets_are_the_oracles( native, oracle, lab
     say( "case constant", label, len( pos ) )
     m, ( fi, fj, fc ) = _gpu_measures( native, pos, w, facets = True )
     mo, ( oi, oj, oc ) = oracle.measures( pos, w, facets = True )
+    if label == "grid":
+        # seeds tied at the medians: the two builds may send a tied seed to either side ( the reference's own card and
+        # CPU trees differ there too, `AaBsp.py:227-229` ), so the cut order and the rounding differ; the cells do not
+        assert np.abs( m - mo ).max() <= 1e-13 * mo.max() and np.allclose( m, 1 / 1600, rtol = 1e-12 )
+        big = lambda i, j, c: { ( a, b ) for a, b, x in zip( i.tolist(), j.tolist(), c.tolist() ) if x > 1e-9 }
+        assert big( fi, fj, fc ) == big( oi, oj, oc )
+        return
     assert np.array_equal( m, mo ), ( label, np.abs( m - mo ).max() )
     # facets: the same rows ( user order ), each in its cell's edge order
     assert np.array_equal( fi, oi ) and np.array_equal( fj, oj ) and np.array_equal( fc, oc )
@@ -132,9 +139,10 @@ def test_overflowed_cells_go_through_the_second_pass( native, oracle ):
 
 def test_a_cell_past_the_vertex_limit_is_a_clean_failure( native ):
     from sdot import KernelFailure
+    # the centre of a ring of 2500 seeds: a 2500-gon, past the 2048-vertex limit of the last pass
     rng = np.random.default_rng( 10 )
-    pos = H.ring( 1500, rng, 10 )
-    say( "case failure ring 1500" )
+    pos = H.ring( 2500, rng, 10 )
+    say( "case failure ring 2500" )
     with pytest.raises( KernelFailure ) as e:
         _gpu_measures( native, pos, None )
     assert e.value.code == 1 and e.value.value == 0
