@@ -54,6 +54,11 @@ def main():
     ap.add_argument( "--tol", type = float, default = 1e-9 )
     ap.add_argument( "--cpu-sample", type = int, default = 100_000 )
     ap.add_argument( "--no-cpu-baseline", action = "store_true" )
+    ap.add_argument( "--step", default = "limits", choices = [ "limits", "trials" ] )
+    ap.add_argument( "--continuation", default = "never", choices = [ "never", "auto", "always" ] )
+    ap.add_argument( "--linear", default = "auto", choices = [ "auto", "cg" ] )
+    ap.add_argument( "--copy-calibration", action = "store_true",
+                     help = "first copy 512 MiB device to device ( the known-size kernel that scales the HBM counters )" )
     args = ap.parse_args()
     if args.gpus != 1:
         print( "bench.py: this build runs on ONE GPU ( the multi-GPU design is documented, not built ); use --gpus 1", file = sys.stderr )
@@ -66,10 +71,20 @@ def main():
     dev = torch.device( "cuda", 0 )
     torch.cuda.set_device( dev )
 
+    if args.copy_calibration:
+        src = torch.ones( 64 << 20, dtype = torch.float64, device = dev )
+        dst = torch.empty_like( src )
+        dst.copy_( src )
+        torch.cuda.synchronize()
+        print( f"bench: copy calibration: { src.numel() * 8 } bytes read, as many written", file = sys.stderr, flush = True )
+        del src, dst
+
     pos_h, img_h = workload( args.n, args.res )
     pos = torch.as_tensor( pos_h, device = dev )
     target = sdot.Image( img_h, origin = [ 0, 0 ], frame = np.eye( 2 ) / args.res )
-    settings = sdot.Iterative( tol = args.tol, continuation = "never", on_failure = "ignore" )
+    settings = sdot.Iterative( tol = args.tol, continuation = args.continuation, on_failure = "ignore",
+                               tuning = sdot.Tuning( step = args.step, linear_solver = args.linear ) )
+    linear = "jacobi-pcg" if args.linear == "cg" else "multigrid-pcg ( aggregates of 4 tree ranks, damped-Jacobi smoothing )"
     print( f"bench: n = { args.n }, image { args.res }^2, tol { args.tol }", file = sys.stderr, flush = True )
 
     def step():
@@ -105,7 +120,7 @@ def main():
     spmv_ms, spmv_launches = times[ "spmv" ]
     spmv_avg_ms = spmv_ms / max( spmv_launches, 1 )
     spmv_gbs = ( spmv_bytes / max( spmv_launches, 1 ) ) / ( spmv_avg_ms * 1e-3 ) / 1e9 if spmv_launches else 0.0
-    roof_spmv = dict( kernel = "k_cg_spmv_dot", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s", frac = spmv_gbs / HBM_PEAK_GBS,
+    roof_spmv = dict( kernel = "k_cg_spmv_dot" if args.linear == "cg" else "k_pcg_spmv", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s", frac = spmv_gbs / HBM_PEAK_GBS,
                       traffic = None, avg_launch_ms = spmv_avg_ms, launches_timed = spmv_launches,
                       bytes_per_launch = spmv_bytes / max( spmv_launches, 1 ) )
     # the SpMV is timed one launch in sixteen: its share of the step is estimated from the CG iteration count
@@ -117,10 +132,11 @@ def main():
                  config = dict( workload = f"2D { args.n } Diracs uniform in [0,1]^2, { args.res }^2 image of 8 Gaussian bumps + 0.1 floor, "
                                            f"max mass error <= { args.tol:g} ( absolute )",
                                 n = args.n, image = [ args.res, args.res ], tol_abs = args.tol, start = "cold ( Voronoi ), tree included",
-                                step = "trials", linear = "jacobi-pcg", cells = "fp64" ),
+                                step = args.step, continuation = args.continuation, linear = linear, cells = "fp64" ),
                  roofline = dominant, roofline_secondary = other,
                  solve = dict( status = st[ "status" ], residual_abs = st[ "residual" ], residual_rel = st[ "residual" ] / nu,
                                nb_iter = st[ "nb_iter" ], nb_diag = st[ "nb_diag" ], lin_nb_iter = st[ "lin_nb_iter" ],
+                               nb_continuation_steps = st[ "nb_continuation_steps" ], t_lim = st[ "t_lim" ],
                                t_diag = st[ "t_diag" ], t_majorant = st[ "t_majorant" ], t_lin = st[ "t_lin" ], t_total = st[ "t_total" ],
                                device_bytes = _native.device_bytes() ) )
 
@@ -140,7 +156,7 @@ def cpu_baseline( pos_h, img_h, args ):
     dens = dict( values = np.ascontiguousarray( v.T ), x0 = 0.0, y0 = 0.0, hx = 1.0 / args.res, hy = 1.0 / args.res )
     tol = args.tol * len( pos_h ) / m                    # the same relative accuracy per Dirac
     t0 = time.perf_counter()
-    r = oracle.solve( pos_h[ :m ], density = dens, tol = tol, lin_tol = 1e-6 )
+    r = oracle.solve( pos_h[ :m ], density = dens, tol = tol, lin_tol = 1e-6, step = args.step, continuation = args.continuation )
     dt = time.perf_counter() - t0
     return dict( value = m / dt, unit = "Diracs/s", cores = cores, kind = "port",
                  sample = f"the oracle ( C++ FP64 restatement, Jacobi-PCG ) solving the first { m } Diracs of the same workload to "

@@ -24,11 +24,14 @@ extern "C" {
 
 #define SDOT_AMD_ABI_VERSION 1
 
-/* the density: SDOT_AMD_CONSTANT ( `rho` on the domain box ) or SDOT_AMD_IMAGE ( piecewise constant on
+/* the density: SDOT_AMD_CONSTANT ( `rho` on the domain box ), SDOT_AMD_IMAGE ( piecewise constant on
  * `nx * ny` pixels, `img[ j * nx + i ]` the value of pixel `( i, j )` covering
- * `[ x0 + i hx, x0 + ( i + 1 ) hx ] x [ y0 + j hy, y0 + ( j + 1 ) hy ]` ) */
-#define SDOT_AMD_CONSTANT 0
-#define SDOT_AMD_IMAGE    1
+ * `[ x0 + i hx, x0 + ( i + 1 ) hx ] x [ y0 + j hy, y0 + ( j + 1 ) hy ]` ) or SDOT_AMD_GAUSSIANS ( a sum of
+ * `nx` isotropic Gaussians truncated to the domain box, `img[ 4 q .. 4 q + 3 ]` = centre x, centre y, standard
+ * deviation and mass of Gaussian `q`: the reference card's `g_pos`, `g_sigma`, `g_mass`, SdotPlanNd.py:837-862 ) */
+#define SDOT_AMD_CONSTANT  0
+#define SDOT_AMD_IMAGE     1
+#define SDOT_AMD_GAUSSIANS 2
 
 int         sdot_amd_abi_version( void );
 
@@ -58,7 +61,8 @@ int sdot_amd_kernel_times( double *ms, long long *launches, double *spmv_bytes, 
  * ( src/sdot/include/sdot/gpu/Cell2D.cuh:1801-1811, called from src/sdot/PowerDiagram_Bsp.py:271-311 ).
  *   positions    DEVICE [ n ][ 2 ]
  *   weights      DEVICE [ n ]       ( null: all zero, a Voronoi diagram )
- *   img          DEVICE [ ny * nx ] ( SDOT_AMD_IMAGE only ); img_geo HOST { x0, y0, hx, hy }
+ *   img          DEVICE [ ny * nx ] ( SDOT_AMD_IMAGE; img_geo HOST { x0, y0, hx, hy } ) or [ nx ][ 4 ]
+ *                                   ( SDOT_AMD_GAUSSIANS, `nx` Gaussians; `ny`, `img_geo` unused )
  *   masses       DEVICE [ n ]       ( output )
  *   facet_i/j/c  DEVICE [ facet_cap ] ( outputs; null: no facets )
  *   nb_facets    HOST   [ 1 ]       ( output, may be null )
@@ -70,8 +74,10 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
                           long long *nb_facets, unsigned long long *events );
 
 /* THE SOLVE: weights `w` such that every cell's mass matches its target `nu_i`, by the damped Newton of
- * Kitagawa-Merigot-Thibert ( trials step ), FP64 cells, Jacobi-PCG directions.
- * Replaces `sdotplan_solve_card_2d` / `_image` -> `sdot::gpu2d::solve`
+ * Kitagawa-Merigot-Thibert, FP64 cells; the step by trials or by the limits of the cells ( option `step` ),
+ * directions by Jacobi-PCG or multigrid-PCG ( option `lin` ), the width continuation of an image or of
+ * Gaussians ( options `continuation`, `conv_*` ).
+ * Replaces `sdotplan_solve_card_2d` / `_image` / `_gauss` -> `sdot::gpu2d::solve`
  * ( src/sdot/include/sdot/gpu/Newton2D.cuh:846-850, called from src/sdot/SdotPlanNd.py:864-890 ).
  *   positions    DEVICE [ n ][ 2 ]
  *   nu           DEVICE [ n ]       target masses ( rescaled to the mass of the domain )
@@ -79,9 +85,9 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
  *   dups         DEVICE [ 2 k ]     exact duplicates ( dup, rep ); accepted, not used by this version
  *   options      HOST   [ 41 ]      the reference card's option vector, same order
  *                                   ( src/sdot/SdotPlanNd.py:197-201 ); the image geometry is in
- *                                   img_x0 .. img_ny as there
+ *                                   img_x0 .. img_ny as there, the Gaussians' count in img_nx
  *   box          HOST   [ 4 ]       the domain { lo_x, lo_y, hi_x, hi_y }
- *   img          DEVICE [ ny * nx ] ( SDOT_AMD_IMAGE only )
+ *   img          DEVICE [ ny * nx ] ( SDOT_AMD_IMAGE ) or [ img_nx ][ 4 ] ( SDOT_AMD_GAUSSIANS )
  *   weights      DEVICE [ n ]       output, `weights[ 0 ] == 0` ( the gauge )
  *   history      HOST   [ ( maxit + 1 ) * 8 ] output rows ( src/sdot/include/sdot/sdotplan/Report.h:25 )
  *   nb_steps     HOST   [ 1 ]       output, rows written

@@ -93,9 +93,13 @@ def weight_majorant( pos, w ):
 
 
 def _density_args( density ):
-    """`density`: `None` / a float ( constant ), or `dict( values = img[ ny, nx ], x0, y0, hx, hy )`"""
+    """`density`: `None` / a float ( constant ), `dict( values = img[ ny, nx ], x0, y0, hx, hy )` ( an image ), or
+    `dict( gaussians = [ ng, 4 ] )` ( centre x, y, sigma, mass per row: a sum of isotropic Gaussians )"""
     if density is None or np.isscalar( density ):
         return 0, 1.0 if density is None else float( density ), 0, 0, None, None, None
+    if "gaussians" in density:
+        g = _f64( density[ "gaussians" ] ).reshape( -1, 4 )
+        return 2, 0.0, g.shape[ 0 ], 0, None, g, ( g, )
     img = _f64( density[ "values" ] )
     geo = _f64( [ density[ "x0" ], density[ "y0" ], density[ "hx" ], density[ "hy" ] ] )
     return 1, 0.0, img.shape[ 1 ], img.shape[ 0 ], geo, img, ( img, geo )
@@ -142,6 +146,30 @@ def cell( pos, i, w = None, lo = ( 0, 0 ), hi = ( 1, 1 ), brute_force = False, c
     return xy[ :nv ].copy(), ids[ :nv ].copy()
 
 
+def along( pos, w, d, t = 0.0, eps = 0.0, lo = ( 0, 0 ), hi = ( 1, 1 ), density = None ):
+    """the step's frozen cells of the diagram of ( pos, w ) along d: dict( poly[ n, 3 ], ok[ n ], frozen_mass[ n ] at t,
+    forward[ n ] at floor eps, backward[ n ] from the trial t )"""
+    pos = _f64( pos )
+    n = pos.shape[ 0 ]
+    kind, value, nx, ny, geo, img, keep = _density_args( density )
+    poly = np.zeros( ( n, 3 ) )
+    ok = np.zeros( n, np.int32 )
+    fm, fw, bw = np.zeros( n ), np.zeros( n ), np.zeros( n )
+    lib().oracle_along( ctypes.c_longlong( n ), _d( pos ), _d( _f64( w ) ), _d( _f64( d ) ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
+                        ctypes.c_double( value ), ctypes.c_int( nx ), ctypes.c_int( ny ), _d( geo ), _d( img ), ctypes.c_double( t ),
+                        ctypes.c_double( eps ), _d( poly ), ok.ctypes.data_as( ctypes.c_void_p ), _d( fm ), _d( fw ), _d( bw ) )
+    return dict( poly = poly, ok = ok.astype( bool ), frozen_mass = fm, forward = fw, backward = bw )
+
+
+def blur( values, s, hx, hy ):
+    """the image `values[ ny, nx ]` at width s ( the continuation's filter )"""
+    v = _f64( values )
+    ny, nx = v.shape
+    out = np.zeros_like( v )
+    lib().oracle_blur( ctypes.c_int( nx ), ctypes.c_int( ny ), ctypes.c_double( hx ), ctypes.c_double( hy ), _d( v ), ctypes.c_double( s ), _d( out ) )
+    return out
+
+
 def cg( n, fi, fj, fc, b, tol = 1e-12, maxit = 20000 ):
     """Jacobi-PCG on the Laplacian of the facets; returns `( d, iterations )`"""
     fi = np.ascontiguousarray( fi, dtype = np.int64 )
@@ -163,14 +191,18 @@ HISTORY = [ "step", "t", "residual_l2", "min_measure", "max_abs_residual", "nb_d
 
 def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = None, tol = 1e-9, tol_rel = 0.0, maxit = 100, max_backtracks = 60,
            t_min = 1e-10, mult_ok = 4.0, residual = 1, power = 0.5, switch = 2.0, lin_tol = 1e-12, lin_maxit = 20000, keep_start = False,
-           start_box = None ):
-    """the damped-Newton solve ( trials ); returns a dict ( weights, masses, barycenters, cost, stats, history )"""
+           start_box = None, step = "trials", factor = 0.9, continuation = "never", conv_threshold = 1e-2, conv_s0 = 0.0,
+           conv_ratio = 2 ** 0.5, conv_min = 0.0 ):
+    """the damped-Newton solve ( step "trials" or "limits"; continuation "never", "auto" or "always" ); returns a dict
+    ( weights, masses, barycenters, cost, stats, history )"""
     pos = _f64( pos )
     n = pos.shape[ 0 ]
     nu = np.full( n, 1.0 / n ) if nu is None else _f64( nu )
     w0 = None if w0 is None else _f64( w0 )
     kind, value, nx, ny, geo, img, keep = _density_args( density )
-    opts = _f64( [ tol, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol, lin_maxit, int( keep_start ) ] )
+    opts = _f64( [ tol, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol, lin_maxit, int( keep_start ),
+                   { "trials": 0, "limits": 1 }[ step ], factor, { "never": 0, "auto": 1, "always": 2 }[ continuation ], conv_threshold,
+                   conv_s0, conv_ratio, conv_min ] )
     sb = _f64( [ 0, 0, 0, 0, 0 ] if start_box is None else [ 1, *start_box[ 0 ], *start_box[ 1 ] ] )
     weights = np.zeros( n )
     stats = np.zeros( len( STATS ) )

@@ -68,13 +68,19 @@ static double now() {
 // ---- densities -------------------------------------------------------------------------------
 
 /// kind 0: constant `value` on the box; kind 1: an image, `img[ j * nx + i ]` the value of pixel
-/// `( i, j )`, pixel `( i, j )` covering `[ x0 + i hx, x0 + ( i + 1 ) hx ] x [ y0 + j hy, ... ]`
+/// `( i, j )`, pixel `( i, j )` covering `[ x0 + i hx, x0 + ( i + 1 ) hx ] x [ y0 + j hy, ... ]`;
+/// kind 2: a sum of `ng` isotropic Gaussians, `gauss[ 4 q .. ]` = centre x, y, sigma, mass, each widened to
+/// `sqrt( sigma^2 + conv_s^2 )` ( the width continuation, `sdotplan/convolved/SumOfGaussians.h:34` )
 struct Density {
     int           kind  = 0;
     double        value = 1;
     int           nx = 0, ny = 0;
     double        x0 = 0, y0 = 0, hx = 1, hy = 1;
     const double *img = nullptr;
+    int           ng = 0;
+    const double *gauss = nullptr;
+    double        conv_s = 0;
+    double sigma( int q ) const { const double s = gauss[ 4 * q + 2 ]; return conv_s > 0 ? std::sqrt( s * s + conv_s * conv_s ) : s; }
 };
 
 // ---- the tree --------------------------------------------------------------------------------
@@ -344,9 +350,9 @@ void set_weights( Tree &t, const double *w_user ) {
 
 enum CutStatus { UNCHANGED = 0, CUT = 1, EMPTY = 2 };
 
-/// cut ids: >= 0 a seed rank, `DOMAIN_ID` a side of the domain, `PIECE_ID` a pixel plane
-constexpr int DOMAIN_ID = -1;
-constexpr int PIECE_ID  = -2;
+/// cut ids: >= 0 a seed rank, `-1 - f` side `f` of the domain box ( 0 bottom, 1 right, 2 top, 3 left,
+/// as `gpu/Newton2D.cuh:529-534` reads them ), `PIECE_ID` a pixel plane
+constexpr int PIECE_ID  = -16;
 
 /// events of the cut loop ( side tests, pruning tests, new vertices ), for the roofline's count
 struct Events {
@@ -374,7 +380,7 @@ static void init_box( Poly &c, const double *lo, const double *hi, double p0x, d
     c.x[ 1 ] = hi[ 0 ] - p0x; c.y[ 1 ] = lo[ 1 ] - p0y;
     c.x[ 2 ] = hi[ 0 ] - p0x; c.y[ 2 ] = hi[ 1 ] - p0y;
     c.x[ 3 ] = lo[ 0 ] - p0x; c.y[ 3 ] = hi[ 1 ] - p0y;
-    for ( int i = 0; i < 4; ++i ) c.id[ size_t( i ) ] = DOMAIN_ID;
+    for ( int i = 0; i < 4; ++i ) c.id[ size_t( i ) ] = -1 - i;
 }
 
 /// the outside run holding the largest `s` when the signs come in several runs
@@ -553,6 +559,137 @@ static void piece_moments( const Poly &c, double rho, double &m, double *mx, dou
     }
 }
 
+
+// ---- a sum of isotropic Gaussians ( `SumOfGaussians.cxx:210-266`, `gpu/Density2D.cuh:153-280` ) ----------------------
+//
+// The standard normal measure of the triangle ( 0, P, Q ) is `( 1 / 2 pi ) int ( 1 - exp( -r^2 / 2 ) ) dtheta` over its
+// angle, `r` the distance from 0 to the segment along the ray; with `t` the abscissa along PQ and `p` the distance from 0
+// to its line, `dtheta = p dt / ( p^2 + t^2 )`. Beyond `|t| = 8` the exponential is under 1e-14: the tails are the exact
+// arctangents, the core a composite 8-point Gauss-Legendre on panels of at most half a unit.
+
+static double gauss_wedge( double px, double py, double qx, double qy ) {
+    const double dx = qx - px, dy = qy - py;
+    const double L = std::sqrt( dx * dx + dy * dy );
+    if ( ! ( L > 0 ) ) return 0;
+    const double ux = dx / L, uy = dy / L;
+    const double p = uy * px - ux * py, ap = std::fabs( p );
+    if ( ! ( ap > 0 ) ) return 0;
+    const double t0 = ux * px + uy * py, t1 = ux * qx + uy * qy, cut = 8;
+    double acc = 0;
+    if ( ap >= cut ) acc = std::atan( t1 / ap ) - std::atan( t0 / ap );
+    else {
+        if ( t0 < -cut ) acc += std::atan( std::min( t1, -cut ) / ap ) - std::atan( t0 / ap );
+        if ( t1 > cut ) acc += std::atan( t1 / ap ) - std::atan( std::max( t0, cut ) / ap );
+        const double c0 = std::max( t0, -cut ), c1 = std::min( t1, cut );
+        if ( c1 > c0 ) {
+            static const double gx[ 4 ] = { 0.1834346424956498, 0.5255324099163290, 0.7966664774136267, 0.9602898564975363 };
+            static const double gw[ 4 ] = { 0.3626837833783620, 0.3137066458778873, 0.2223810344533745, 0.1012285362903763 };
+            const int np = int( std::ceil( ( c1 - c0 ) * 2 ) );
+            const double h = ( c1 - c0 ) / ( 2 * np );
+            for ( int k = 0; k < np; ++k ) {
+                const double m = c0 + ( 2 * k + 1 ) * h;
+                for ( int q = 0; q < 4; ++q )
+                    for ( int sg = -1; sg <= 1; sg += 2 ) {
+                        const double t = m + sg * h * gx[ q ];
+                        const double r2 = ap * ap + t * t;
+                        acc += h * gw[ q ] * ( 1 - std::exp( -r2 / 2 ) ) * ap / r2;
+                    }
+            }
+        }
+    }
+    return ( p < 0 ? -acc : acc ) / 6.283185307179586476925286766559;
+}
+
+/// the mass of the convex polygon `c` ( seed frame, the seed at `( ox, oy )` ): the fan from vertex 0, each triangle's
+/// measure the absolute value of its three wedges ( `SumOfGaussians.cxx:282-288`, `:316-343` without the quadrature shortcut )
+static double gauss_polygon_mass( const Density &dn, const Poly &c, double ox, double oy ) {
+    double m = 0;
+    for ( int q = 0; q < dn.ng; ++q ) {
+        const double s = dn.sigma( q ), cx = dn.gauss[ 4 * q ] - ox, cy = dn.gauss[ 4 * q + 1 ] - oy;
+        auto Y = [&]( int v, double &x, double &y ) { x = ( c.x[ sz( v ) ] - cx ) / s; y = ( c.y[ sz( v ) ] - cy ) / s; };
+        double x0, y0;
+        Y( 0, x0, y0 );
+        double acc = 0;
+        for ( int v = 1; v + 1 < c.nb; ++v ) {
+            double x1, y1, x2, y2;
+            Y( v, x1, y1 );
+            Y( v + 1, x2, y2 );
+            const double w = gauss_wedge( x0, y0, x1, y1 ) + gauss_wedge( x1, y1, x2, y2 ) + gauss_wedge( x2, y2, x0, y0 );
+            acc += std::fabs( w );
+        }
+        m += dn.gauss[ 4 * q + 3 ] * acc;
+    }
+    return m;
+}
+
+/// the signed mass of a polygon given by its vertices ( a frozen cell: a cell turned inside out is negative )
+static double gauss_signed_mass( const Density &dn, const double *vx, const double *vy, int nb, double ox, double oy ) {
+    double m = 0;
+    for ( int q = 0; q < dn.ng; ++q ) {
+        const double s = dn.sigma( q ), cx = dn.gauss[ 4 * q ] - ox, cy = dn.gauss[ 4 * q + 1 ] - oy;
+        double W = 0;
+        for ( int v = 0; v < nb; ++v ) {
+            const int r = v + 1 < nb ? v + 1 : 0;
+            W += gauss_wedge( ( vx[ v ] - cx ) / s, ( vy[ v ] - cy ) / s, ( vx[ r ] - cx ) / s, ( vy[ r ] - cy ) / s );
+        }
+        m += dn.gauss[ 4 * q + 3 ] * W;
+    }
+    return m;
+}
+
+/// `int rho ds` along the segment `a -> b`: per Gaussian `w exp( -d^2 / 2 s^2 ) ( erf( u1 ) - erf( u0 ) ) / ( 2 s sqrt( 2 pi ) )`,
+/// `d` the distance from the centre to the line ( `SumOfGaussians.cxx:127-152` )
+static double gauss_segment_line( const Density &dn, double ox, double oy, double ax, double ay, double bx, double by ) {
+    const double ex = bx - ax, ey = by - ay;
+    const double L = std::sqrt( ex * ex + ey * ey );
+    if ( ! ( L > 0 ) ) return 0;
+    const double ux = ex / L, uy = ey / L;
+    double res = 0;
+    for ( int q = 0; q < dn.ng; ++q ) {
+        const double s = dn.sigma( q ), w = dn.gauss[ 4 * q + 3 ];
+        const double px = ax - ( dn.gauss[ 4 * q ] - ox ), py = ay - ( dn.gauss[ 4 * q + 1 ] - oy );
+        const double d = -px * uy + py * ux;
+        const double t0 = px * ux + py * uy, t1 = t0 + L;
+        const double qq = d * d / ( 2 * s * s );
+        if ( qq > 700 ) continue;
+        const double is2 = 0.70710678118654752440 / s;
+        res += w * std::exp( -qq ) * ( std::erf( t1 * is2 ) - std::erf( t0 * is2 ) ) / ( 2 * s * 2.50662827463100050242 );
+    }
+    return res;
+}
+
+/// the moments of the convex polygon `c` about the seed: `m`, `int ( x - p ) rho`, `int |x - p|^2 rho`. Per Gaussian,
+/// `int ( x - c ) phi = -s^2 circ phi n ds` and `int |x - c|^2 phi = 2 s^2 m - s^2 circ ( ( x - c ) . n ) phi ds`
+/// ( divergence theorem on `grad phi = -( x - c ) phi / s^2` ), each edge's `int phi ds` in closed form
+static void gauss_polygon_moments( const Density &dn, const Poly &c, double ox, double oy, double &m, double *mx, double &m2 ) {
+    for ( int q = 0; q < dn.ng; ++q ) {
+        Density one = dn;
+        one.ng = 1;
+        one.gauss = dn.gauss + 4 * q;
+        const double s = dn.sigma( q ), s2 = s * s;
+        const double cx = dn.gauss[ 4 * q ] - ox, cy = dn.gauss[ 4 * q + 1 ] - oy;   // the centre in the seed frame
+        const double mq = gauss_polygon_mass( one, c, ox, oy );
+        double fx = 0, fy = 0, fr = 0;                  // circ phi n ds, circ ( ( x - c ) . n ) phi ds
+        for ( int v = 0; v < c.nb; ++v ) {
+            const int r = v + 1 < c.nb ? v + 1 : 0;
+            const double ex = c.x[ sz( r ) ] - c.x[ sz( v ) ], ey = c.y[ sz( r ) ] - c.y[ sz( v ) ];
+            const double L = std::sqrt( ex * ex + ey * ey );
+            if ( ! ( L > 0 ) ) continue;
+            const double nx = ey / L, ny = -ex / L;     // outward: the polygon is counter-clockwise
+            const double J = gauss_segment_line( one, ox, oy, c.x[ sz( v ) ], c.y[ sz( v ) ], c.x[ sz( r ) ], c.y[ sz( r ) ] );
+            fx += nx * J;
+            fy += ny * J;
+            fr += ( ( c.x[ sz( v ) ] - cx ) * nx + ( c.y[ sz( v ) ] - cy ) * ny ) * J;
+        }
+        // about the seed: int ( x - p ) = int ( x - c ) + ( c - p ) m, int |x - p|^2 = int |x - c|^2 + 2 ( c - p ) . int ( x - c ) + |c - p|^2 m
+        const double ix = -s2 * fx, iy = -s2 * fy, i2 = 2 * s2 * mq - s2 * fr;
+        m += mq;
+        mx[ 0 ] += ix + cx * mq;
+        mx[ 1 ] += iy + cy * mq;
+        m2 += i2 + 2 * ( cx * ix + cy * iy ) + ( cx * cx + cy * cy ) * mq;
+    }
+}
+
 /// `func( piece, value )` for each piece of the cell on which the density is constant
 /// ( `Image.cxx:106-232`: the tiles the cell's box meets, each cut by the tile's planes when the
 /// cell crosses them ); `p0` is the seed, the cell is in its frame
@@ -613,6 +750,19 @@ static void for_each_piece( const Density &dn, const Poly &c, double p0x, double
 template<class F>
 static double mass_and_facets( const Tree &t, const Density &dn, I64 k, const Poly &c, Poly &piece, bool with_facets, F &&facet ) {
     const double p0x = t.px[ size_t( k ) ], p0y = t.py[ size_t( k ) ];
+    if ( dn.kind == 2 ) {                                // the whole cell at once: wedges and erf lines
+        if ( with_facets )
+            for ( int u = 0; u < c.nb; ++u ) {
+                const int id = c.id[ size_t( u ) ];
+                if ( id < 0 ) continue;
+                const int r = u + 1 < c.nb ? u + 1 : 0;
+                const double line = gauss_segment_line( dn, p0x, p0y, c.x[ sz( u ) ], c.y[ sz( u ) ], c.x[ sz( r ) ], c.y[ sz( r ) ] );
+                const double dx = t.px[ size_t( id ) ] - p0x, dy = t.py[ size_t( id ) ] - p0y;
+                const double d2 = dx * dx + dy * dy;
+                if ( d2 > 0 ) facet( I64( id ), line / ( 2 * std::sqrt( d2 ) ) );
+            }
+        return c.nb >= 3 ? gauss_polygon_mass( dn, c, p0x, p0y ) : 0.0;
+    }
     double mass = 0;
     std::vector<double> vals( sz( c.nb ), 0.0 );
     for_each_piece( dn, c, p0x, p0y, piece, [&]( const Poly &pc, double value ) {
@@ -717,6 +867,14 @@ bool cg_solve( const Laplacian &L, const std::vector<double> &b, std::vector<dou
 
 // ---- the diagram as Newton sees it ( `sdotplan/Sweep.h:142-364` ) ------------------------------
 
+/// the cuts of a cell's edges in polygon order ( user index of the neighbour, or `-1 - f` a side of the box ), at most
+/// `EDGE_CAP` ( more: the count is -1 ): what the step by the limits rebuilds a cell from ( `gpu/Cell2D.cuh:1088-1096` )
+constexpr int EDGE_CAP = 16;
+struct Edges {
+    std::vector<int> id;     ///< [ EDGE_CAP ][ n ]
+    std::vector<int> nb;     ///< [ n ]
+};
+
 struct Sweep {
     Tree   &t;
     const double *lo, *hi;
@@ -724,6 +882,7 @@ struct Sweep {
     int    nb_diag = 0;
     double t_majorant = 0, t_diag = 0;
     Events ev;
+    Edges *edges = nullptr;  ///< filled by `measures` when set
 
     Sweep( Tree &t, const double *lo, const double *hi, const Density &dn ) : t( t ), lo( lo ), hi( hi ), dn( dn ) {}
 
@@ -741,13 +900,24 @@ struct Sweep {
         const int nt = nb_threads();
         std::vector<std::vector<Facet>> fth( sz( nt ) );
         std::vector<Events> eth( sz( nt ) );
+        if ( edges ) { edges->id.assign( sz( n ) * EDGE_CAP, 0 ); edges->nb.assign( sz( n ), 0 ); }
         parallel_blocks( n, 256, [&]( int th, I64 b, I64 e ) {
             Poly c, piece;
             c.reserve( 64 );
             piece.reserve( 64 );
             for ( I64 k = b; k < e; ++k ) {
                 const I64 i = t.order[ size_t( k ) ];
-                if ( ! make_cell( t, k, lo, hi, c, &eth[ size_t( th ) ] ) ) { a[ size_t( i ) ] = 0; continue; }
+                if ( ! make_cell( t, k, lo, hi, c, &eth[ size_t( th ) ] ) ) { a[ size_t( i ) ] = 0; if ( edges ) edges->nb[ size_t( i ) ] = 0; continue; }
+                if ( edges ) {
+                    int ne = c.nb < 3 ? 0 : c.nb;
+                    if ( ne > EDGE_CAP ) ne = -1;
+                    else
+                        for ( int q = 0; q < ne; ++q ) {
+                            const int id = c.id[ size_t( q ) ];
+                            edges->id[ size_t( q ) * size_t( n ) + size_t( i ) ] = id >= 0 ? int( t.order[ size_t( id ) ] ) : id;
+                        }
+                    edges->nb[ size_t( i ) ] = ne;
+                }
                 a[ size_t( i ) ] = mass_and_facets( t, dn, k, c, piece, fa != nullptr, [&]( I64 j, double cij ) {
                     fth[ size_t( th ) ].push_back( Facet{ i, t.order[ size_t( j ) ], cij } );
                 } );
@@ -778,8 +948,10 @@ struct Sweep {
                 const I64 i = t.order[ size_t( k ) ];
                 const double p0x = t.px[ size_t( k ) ], p0y = t.py[ size_t( k ) ];
                 double m = 0, mx[ 2 ] = { 0, 0 }, m2 = 0;
-                if ( make_cell( t, k, lo, hi, c, nullptr ) )
-                    for_each_piece( dn, c, p0x, p0y, piece, [&]( const Poly &pc, double value ) { piece_moments( pc, value, m, mx, m2 ); } );
+                if ( make_cell( t, k, lo, hi, c, nullptr ) && c.nb >= 3 ) {
+                    if ( dn.kind == 2 ) gauss_polygon_moments( dn, c, p0x, p0y, m, mx, m2 );
+                    else for_each_piece( dn, c, p0x, p0y, piece, [&]( const Poly &pc, double value ) { piece_moments( pc, value, m, mx, m2 ); } );
+                }
                 cth[ size_t( th ) ] += m2;
                 bary[ size_t( 2 * i ) ] = m > 0 ? p0x + mx[ 0 ] / m : p0x;
                 bary[ size_t( 2 * i + 1 ) ] = m > 0 ? p0y + mx[ 1 ] / m : p0y;
@@ -789,6 +961,265 @@ struct Sweep {
         for ( double v : cth ) cost += v;
     }
 };
+
+
+// ---- the width continuation of an image ( `sdotplan/Continuation.h:31-39`, `sdotplan/convolved/Image.h:40-85` ) -------
+
+/// the widths of the stages: `s0, s0 / ratio, ...` while `>= s_min`, then 0
+static std::vector<double> continuation_steps( double s0, double ratio, double s_min ) {
+    std::vector<double> res;
+    if ( s0 > 0 && ratio > 1 )
+        for ( double s = s0; s >= s_min * ( 1 - 1e-12 ) && s > 0; s /= ratio ) res.push_back( s );
+    res.push_back( 0 );
+    return res;
+}
+
+/// the image at width `s`: along x, then along y, a Gaussian of standard deviation `s / h` pixels truncated at four of
+/// them, each value divided by the sum of the weights that fall inside the image ( so a constant stays that constant )
+static void blur_image( const Density &dn, double s, std::vector<double> &out ) {
+    const size_t nb = size_t( dn.nx ) * size_t( dn.ny );
+    out.assign( dn.img, dn.img + nb );
+    if ( ! ( s > 0 ) ) return;
+    std::vector<double> tmp( nb );
+    for ( int a = 0; a < 2; ++a ) {
+        const double sp = s / ( a ? dn.hy : dn.hx );
+        if ( ! ( sp > 1e-3 ) ) continue;
+        const I64 len = a ? dn.ny : dn.nx, stride = a ? dn.nx : 1;
+        const I64 r = std::min<I64>( I64( std::ceil( 4 * sp ) ), len );
+        std::vector<double> ker( sz( 2 * r + 1 ) );
+        for ( I64 k = -r; k <= r; ++k ) ker[ sz( k + r ) ] = std::exp( -0.5 * double( k * k ) / ( sp * sp ) );
+        for ( I64 j = 0; j < dn.ny; ++j )
+            for ( I64 i = 0; i < dn.nx; ++i ) {
+                const I64 at = j * dn.nx + i, pos = a ? j : i;
+                const I64 kb = std::max<I64>( -r, -pos ), ke = std::min<I64>( r, len - 1 - pos );
+                double sum = 0, wsum = 0;
+                for ( I64 k = kb; k <= ke; ++k ) { sum += ker[ sz( k + r ) ] * out[ sz( at + k * stride ) ]; wsum += ker[ sz( k + r ) ]; }
+                tmp[ sz( at ) ] = sum / wsum;
+            }
+        out.swap( tmp );
+    }
+}
+
+// ---- the step by the limits ( the card's, `gpu/Newton2D.cuh:505-770`, `:1615-1670` ) ---------------------------------
+//
+// With its edges frozen, a cell along `w + t d` is known at every `t`: a bisector's offset is affine in the weights and its
+// normal fixed, so each vertex ( the crossing of two consecutive edge lines ) is affine in `t` and the area a polynomial of
+// degree 2. FORWARD ( the accepted diagram ): the first `t > 0` where a cell's mass reaches the floor `eps`; BACKWARD ( a
+// trial diagram, its cells under the floor ): where they crossed it going back. A density that is not a constant bisects
+// the frozen cell's mass instead.
+
+constexpr double DET_MIN = 1e-6;
+
+/// a diagram's cells, edges frozen, along `w + t d`: positions, weights, direction in USER order
+struct Along {
+    const double *pos, *w, *d, *lo, *hi;
+    const Edges  *e;
+    I64           n;
+    const Density *dn;
+
+    /// the line of cut `c` of cell `k` in its seed's frame at `w + t0 d`: `nx x + ny y = off + s delta` at `w + ( t0 + s ) d`
+    void line( I64 k, int c, double t0, double &nx, double &ny, double &off, double &delta ) const {
+        const double px = pos[ 2 * k ], py = pos[ 2 * k + 1 ];
+        if ( c >= 0 ) {
+            nx = pos[ 2 * c ] - px;
+            ny = pos[ 2 * c + 1 ] - py;
+            off = 0.5 * ( nx * nx + ny * ny ) + 0.5 * ( ( w[ k ] + t0 * d[ k ] ) - ( w[ c ] + t0 * d[ c ] ) );
+            delta = 0.5 * ( d[ k ] - d[ c ] );
+            return;
+        }
+        const int f = -1 - c;
+        nx = ( f & 1 ) ? 1.0 : 0.0;
+        ny = ( f & 1 ) ? 0.0 : 1.0;
+        off = f == 0 ? lo[ 1 ] - py : f == 1 ? hi[ 0 ] - px : f == 2 ? hi[ 1 ] - py : lo[ 0 ] - px;
+        delta = 0;
+    }
+    int nb_edges( I64 k ) const { return e->nb[ sz( k ) ]; }
+    int edge( I64 k, int q ) const { return e->id[ size_t( q ) * size_t( n ) + size_t( k ) ]; }
+
+    /// vertex `q` of the frozen cell at `w + t0 d` ( crossing of the lines of edges `q - 1` and `q` ) and its velocity;
+    /// `false` if two consecutive lines are nearly parallel
+    bool vertices( I64 k, double t0, int nb, double *vx, double *vy, double *ux, double *uy ) const {
+        double ax, ay, ao, ad;
+        line( k, edge( k, nb - 1 ), t0, ax, ay, ao, ad );
+        for ( int q = 0; q < nb; ++q ) {
+            double bx, by, bo, bd;
+            line( k, edge( k, q ), t0, bx, by, bo, bd );
+            const double det = ax * by - ay * bx;
+            if ( ! ( det * det > DET_MIN * DET_MIN * ( ax * ax + ay * ay ) * ( bx * bx + by * by ) ) ) return false;
+            const double inv = 1 / det;
+            vx[ q ] = ( ao * by - bo * ay ) * inv;
+            vy[ q ] = ( ax * bo - bx * ao ) * inv;
+            ux[ q ] = ( ad * by - bd * ay ) * inv;
+            uy[ q ] = ( ax * bd - bx * ad ) * inv;
+            ax = bx; ay = by; ao = bo; ad = bd;
+        }
+        return true;
+    }
+
+    /// `area( t0 + s ) = a0 + a1 s + a2 s^2` while the edges hold
+    bool area_polynomial( I64 k, double t0, double &a0, double &a1, double &a2 ) const {
+        const int nb = nb_edges( k );
+        if ( nb < 3 ) return false;
+        double vx[ EDGE_CAP ], vy[ EDGE_CAP ], ux[ EDGE_CAP ], uy[ EDGE_CAP ];
+        if ( ! vertices( k, t0, nb, vx, vy, ux, uy ) ) return false;
+        double c0 = 0, c1 = 0, c2 = 0;
+        for ( int q = 0; q < nb; ++q ) {
+            const int r = q + 1 < nb ? q + 1 : 0;
+            c0 += vx[ q ] * vy[ r ] - vx[ r ] * vy[ q ];
+            c1 += vx[ q ] * uy[ r ] + ux[ q ] * vy[ r ] - vx[ r ] * uy[ q ] - ux[ r ] * vy[ q ];
+            c2 += ux[ q ] * uy[ r ] - ux[ r ] * uy[ q ];
+        }
+        const double sg = c0 < 0 ? -0.5 : 0.5;
+        a0 = sg * c0; a1 = sg * c1; a2 = sg * c2;
+        return true;
+    }
+
+    /// the mass of the frozen cell at `w + t d` against the density ( counter-clockwise as the cell it comes from: a cell
+    /// turned inside out has a negative mass ); `false` if two consecutive lines are nearly parallel
+    bool frozen_mass( I64 k, double t, double &m ) const;
+};
+
+/// `int F0 dy` along the segment `a -> b` of the seed frame, `F0( x, y ) = int rho( s, y ) ds` from the seed's column to `x`
+/// along the row of `y`: exact per pixel piece ( `F0` linear there: its middle times the piece's `dy` ). The pieces come from
+/// the sorted crossings of the segment with the grid lines ( `gpu/Density2D.cuh:95` walks them instead ).
+static double image_edge_mass( const Density &dn, double px, double py, double ax, double ay, double bx, double by ) {
+    const double ox = dn.x0 - px, oy = dn.y0 - py;
+    int iref = int( std::floor( -ox / dn.hx ) );
+    iref = std::min( std::max( iref, 0 ), dn.nx - 1 );
+    std::vector<double> ts{ 0.0, 1.0 };
+    for ( int a = 0; a < 2; ++a ) {
+        const double o = a ? oy : ox, h = a ? dn.hy : dn.hx;
+        const double ga = ( ( a ? ay : ax ) - o ) / h, gb = ( ( a ? by : bx ) - o ) / h;
+        if ( ga == gb ) continue;
+        for ( double g = std::ceil( std::min( ga, gb ) ); g <= std::floor( std::max( ga, gb ) ); g += 1 ) {
+            const double t = ( g - ga ) / ( gb - ga );
+            if ( t > 0 && t < 1 ) ts.push_back( t );
+        }
+    }
+    std::sort( ts.begin(), ts.end() );
+    double m = 0;
+    for ( size_t q = 0; q + 1 < ts.size(); ++q ) {
+        const double t0 = ts[ q ], t1 = ts[ q + 1 ];
+        if ( ! ( t1 > t0 ) ) continue;
+        const double tm = 0.5 * ( t0 + t1 );
+        const double x = ax + tm * ( bx - ax ), y = ay + tm * ( by - ay );
+        int i = int( std::floor( ( x - ox ) / dn.hx ) ), j = int( std::floor( ( y - oy ) / dn.hy ) );
+        i = std::min( std::max( i, 0 ), dn.nx - 1 );
+        j = std::min( std::max( j, 0 ), dn.ny - 1 );
+        const double *row = dn.img + size_t( j ) * size_t( dn.nx );
+        double f0 = 0;
+        if ( i >= iref ) for ( int c = iref; c < i; ++c ) f0 += row[ c ] * dn.hx;
+        else             for ( int c = i; c < iref; ++c ) f0 -= row[ c ] * dn.hx;
+        f0 += row[ i ] * ( x - ( ox + double( i ) * dn.hx ) );
+        m += f0 * ( ( by - ay ) * ( t1 - t0 ) );
+    }
+    return m;
+}
+
+bool Along::frozen_mass( I64 k, double t, double &m ) const {
+    const int nb = nb_edges( k );
+    double vx[ EDGE_CAP ], vy[ EDGE_CAP ], ux[ EDGE_CAP ], uy[ EDGE_CAP ];
+    if ( nb < 3 || ! vertices( k, t, nb, vx, vy, ux, uy ) ) return false;
+    m = 0;
+    if ( dn->kind == 0 ) {
+        for ( int q = 0; q < nb; ++q ) {
+            const int r = q + 1 < nb ? q + 1 : 0;
+            m += vx[ q ] * vy[ r ] - vx[ r ] * vy[ q ];
+        }
+        m *= 0.5 * dn->value;
+        return true;
+    }
+    const double px = pos[ 2 * k ], py = pos[ 2 * k + 1 ];
+    if ( dn->kind == 2 ) { m = gauss_signed_mass( *dn, vx, vy, nb, px, py ); return true; }
+    for ( int q = 0; q < nb; ++q ) {
+        const int r = q + 1 < nb ? q + 1 : 0;
+        m += image_edge_mass( *dn, px, py, vx[ q ], vy[ q ], vx[ r ], vy[ r ] );
+    }
+    return true;
+}
+
+/// the real roots of `a2 s^2 + a1 s + c = 0`, sorted ( `sdotplan/Bounds.h:170-185` )
+static int quadratic_roots( double a2, double a1, double c, double &r1, double &r2 ) {
+    if ( std::fabs( a2 ) <= 1e-300 ) {
+        if ( a1 == 0 ) return 0;
+        r1 = r2 = -c / a1;
+        return 1;
+    }
+    const double disc = a1 * a1 - 4 * a2 * c;
+    if ( disc < 0 ) return 0;
+    const double sq = std::sqrt( disc );
+    const double q = -0.5 * ( a1 + ( a1 >= 0 ? sq : -sq ) );
+    double x1 = q / a2, x2 = q != 0 ? c / q : x1;
+    if ( x1 > x2 ) std::swap( x1, x2 );
+    r1 = x1; r2 = x2;
+    return 2;
+}
+
+/// the admissible end of `[ lo, hi ]` ( `mass( lo ) >= eps > mass( hi )` ), to 1e-2 relative and above zero
+template<class F>
+static double mass_bisection( double lo, double hi, double eps, const F &mass ) {
+    for ( int round = 0; round < 64; ++round ) {
+        if ( hi - lo <= 1e-2 * hi && lo > 0 ) break;
+        const double mid = 0.5 * ( lo + hi );
+        double m;
+        if ( mass( mid, m ) && m >= eps ) lo = mid;
+        else                              hi = mid;
+    }
+    return lo;
+}
+
+/// FORWARD, one cell of the accepted diagram: the first `t > 0` where its mass reaches `eps` ( 1e300: never, or no say )
+static double alpha_forward( const Along &al, I64 k, double eps ) {
+    double a0, a1, a2;
+    if ( al.dn->kind == 0 ) {
+        if ( ! al.area_polynomial( k, 0.0, a0, a1, a2 ) ) return 1e300;
+        const double rho = al.dn->value;
+        double r1, r2;
+        const int nr = quadratic_roots( rho * a2, rho * a1, std::max( rho * a0 - eps, 0.0 ), r1, r2 );
+        if ( nr >= 1 && r1 > 0 ) return r1;
+        if ( nr >= 2 && r2 > 0 ) return r2;
+        return 1e300;
+    }
+    // a density: the frozen mass at `t = 1`, and at the bottom of the area's parabola
+    if ( al.nb_edges( k ) < 3 ) return 1e300;
+    auto mass = [&]( double t, double &m ) { return al.frozen_mass( k, t, m ); };
+    double m1, hi = -1;
+    if ( ! mass( 1.0, m1 ) ) return 1e300;
+    if ( m1 < eps ) hi = 1;
+    else if ( al.area_polynomial( k, 0.0, a0, a1, a2 ) && a2 > 0 ) {
+        const double ts = -a1 / ( 2 * a2 );
+        double ms;
+        if ( ts > 0 && ts < 1 && mass( ts, ms ) && ms < eps ) hi = ts;
+    }
+    if ( hi < 0 ) return 1e300;
+    return mass_bisection( 0.0, hi, eps, mass );
+}
+
+/// BACKWARD, one cell under the floor in the trial diagram at `t_trial`: where it crossed it ( half the trial if unknown )
+static double alpha_backward( const Along &al, I64 k, double eps, double t_trial ) {
+    double target = 0.5 * t_trial;
+    const int nb = al.nb_edges( k );
+    if ( al.dn->kind == 0 ) {
+        double a0, a1, a2;
+        if ( nb >= 3 && al.area_polynomial( k, t_trial, a0, a1, a2 ) ) {
+            const double rho = al.dn->value;
+            double r1, r2, beta = -1e300;
+            const int nr = quadratic_roots( rho * a2, rho * a1, rho * a0 - eps, r1, r2 );
+            if ( nr >= 1 && r1 < 0 ) beta = r1;
+            if ( nr >= 2 && r2 < 0 ) beta = r2;
+            const double tg = t_trial + beta;
+            if ( tg > 0 && tg < t_trial ) target = tg;
+        }
+        return target;
+    }
+    auto mass = [&]( double t, double &m ) { return al.frozen_mass( k, t, m ); };
+    double m0;
+    if ( nb >= 3 && mass( 0.0, m0 ) && m0 >= eps ) {
+        const double tg = mass_bisection( 0.0, t_trial, eps, mass );
+        if ( tg > 0 && tg < t_trial ) target = tg;
+    }
+    return target;
+}
 
 // ---- Newton ( `sdotplan/Newton.h` ) ------------------------------------------------------------
 
@@ -801,10 +1232,14 @@ enum Stat {
 enum Hist { H_STEP = 0, H_T, H_RESIDUAL_L2, H_MIN_MASS, H_MAX_RESIDUAL, H_NB_DIAG, H_NB_EVALS, H_S, NB_HIST };
 enum Residual { LIN = 0, LOG = 1, POWER = 2 };
 
+enum Step { TRIALS = 0, LIMITS = 1 };
+
 struct Options {
     double tol_abs = 1e-8, tol_rel = 0, t_min = 1e-10, mult_ok = 4, power = 0.5, switch_residual = 2, lin_tol = 1e-12;
     int    maxit = 100, max_backtracks = 60, residual = LOG, lin_maxit = 20000;
     int    keep_start = 0;
+    int    step = TRIALS;
+    double factor = 0.9;     ///< LIMITS: the trial is `factor` times the limit
 };
 
 /// `Report.h:37-57`: two refused trials whose secant slopes agree and extrapolate under `nr / 4`
@@ -836,13 +1271,16 @@ static double gp_of( double x, int r, double p ) {
 struct Solver {
     Sweep &sw;
     Options o;
+    const double *pos = nullptr;                         ///< user order ( the step by the limits reads it )
     std::vector<double> nu, w, a, d;
     std::vector<Facet> fa;
+    Edges ea;                                            ///< the accepted diagram's edges
     int res_cur = LIN;
     double t_last = 1;
     // stats
     int status = 0, nb_iter = 0, nb_backtracks = 0, it_switch = -1;
-    double residual = 0, residual0 = 0, eps = 0, t_asm = 0;
+    I64 nb_cell_lim = 0, nb_limit_rounds = 0;
+    double residual = 0, residual0 = 0, eps = 0, t_asm = 0, t_lim = 0;
     LinStats lst;
     std::function<void( int, double, int )> after_step;
 
@@ -886,16 +1324,21 @@ struct Solver {
         for ( size_t i = 0; i < n; ++i ) { const double e = g_of( A[ i ] / nu[ i ], res_cur, o.power ) - m; s += e * e; }
         return std::sqrt( s );
     }
-    void measures_and_facets( std::vector<double> &W, std::vector<double> &res, std::vector<Facet> &f ) {
+    void measures_and_facets( std::vector<double> &W, std::vector<double> &res, std::vector<Facet> &f, Edges *E = nullptr ) {
         sw.set_weights( W );
+        sw.edges = E;
         sw.measures( res, &f );
+        sw.edges = nullptr;
     }
 
-    /// `Newton.h:243-400` ( trials, no aggregation ): `a` and `fa` already measured for `w_init`
+    /// `Newton.h:243-400` ( no aggregation ), the step by trials or by the limits ( `gpu/Newton2D.cuh:1615-1690` ): `a`, `fa`
+    /// and `ea` already measured for `w_init`
     bool solves( const std::vector<double> &w_init ) {
         const I64 n = I64( w_init.size() );
         std::vector<double> a2, b, w2;
         std::vector<Facet> fa2;
+        Edges e2;
+        const bool limits = o.step == LIMITS;
         Laplacian L;
         w = w_init;
         const double gauge = w[ 0 ];
@@ -927,15 +1370,47 @@ struct Solver {
             t_asm += now() - t0;
             if ( ! cg_solve( L, b, d, o.lin_tol, o.lin_maxit, lst ) ) { status = 4; return false; }
             double t = std::min( 1.0, o.mult_ok * t_last );
-            bool taken = false;
+            bool taken = false, already = false;
             Hopeless hope;
             int nb_evals = 0;
             w2.resize( size_t( n ) );
-            for ( int trial = 0; trial < o.max_backtracks; ++trial ) {
-                for ( I64 i = 0; i < n; ++i ) w2[ size_t( i ) ] = w[ size_t( i ) ] + t * d[ size_t( i ) ];
+            auto trial_diagram = [&]( double tt ) {
+                for ( I64 i = 0; i < n; ++i ) w2[ size_t( i ) ] = w[ size_t( i ) ] + tt * d[ size_t( i ) ];
                 w2[ 0 ] = 0;
-                measures_and_facets( w2, a2, fa2 );
+                measures_and_facets( w2, a2, fa2, limits ? &e2 : nullptr );
                 ++nb_evals;
+            };
+            if ( limits ) {
+                // FORWARD on the accepted diagram, then up to eight rounds of BACKWARD on the trial diagrams
+                double th = now();
+                const Along fw{ pos, w.data(), d.data(), sw.lo, sw.hi, &ea, n, &sw.dn };
+                double am = 1e300;
+                for ( I64 k = 0; k < n; ++k ) am = std::min( am, alpha_forward( fw, k, eps ) );
+                t_lim += now() - th;
+                t = am >= 1 ? 1.0 : o.factor * am;
+                if ( sw.dn.kind != 0 && ! ( t >= o.t_min ) ) t = std::min( 1.0, o.mult_ok * t_last );
+                double t_done = -1;
+                for ( int round = 0; round < 8; ++round ) {
+                    trial_diagram( t );
+                    t_done = t;
+                    I64 nb_below = 0;
+                    for ( double v : a2 ) nb_below += v < eps;
+                    if ( nb_below == 0 ) break;
+                    ++nb_limit_rounds;
+                    th = now();
+                    const Along bw{ pos, w.data(), d.data(), sw.lo, sw.hi, &e2, n, &sw.dn };
+                    double mb = 1e300;
+                    for ( I64 k = 0; k < n; ++k )
+                        if ( a2[ size_t( k ) ] < eps ) { ++nb_cell_lim; mb = std::min( mb, alpha_backward( bw, k, eps, t ) ); }
+                    t_lim += now() - th;
+                    t = o.factor * std::min( t, mb );
+                    if ( t < o.t_min ) break;
+                }
+                if ( t < o.t_min ) t = t_done / 2;
+                already = t == t_done;
+            }
+            for ( int trial = 0; trial < o.max_backtracks; ++trial ) {
+                if ( ! ( trial == 0 && already ) ) trial_diagram( t );
                 double m2 = a2[ 0 ];
                 for ( double v : a2 ) m2 = std::min( m2, v );
                 const double n2r = merit( a2 );
@@ -958,6 +1433,7 @@ struct Solver {
             w.swap( w2 );
             a.swap( a2 );
             fa.swap( fa2 );
+            if ( limits ) std::swap( ea, e2 );
             if ( after_step ) after_step( it + 1, t, nb_evals );
         }
         double worst = 0;
@@ -1011,6 +1487,10 @@ static orc::Density make_density( int kind, double value, int nx, int ny, const 
         d.nx = nx; d.ny = ny;
         d.x0 = geo[ 0 ]; d.y0 = geo[ 1 ]; d.hx = geo[ 2 ]; d.hy = geo[ 3 ];
         d.img = img;
+    }
+    if ( kind == 2 ) {                                   // `img` holds the Gaussians ( x, y, sigma, mass ), `nx` their count
+        d.ng = nx;
+        d.gauss = img;
     }
     return d;
 }
@@ -1108,6 +1588,42 @@ int oracle_cell( I64 n, const double *pos, const double *w, const double *lo, co
     return c.nb;
 }
 
+/// the step's cell polynomials ( testing hook ): the diagram of `( pos, w )` gives each cell's edges; with them frozen,
+/// `poly[ 3 i .. ]` = the area polynomial at `w + 0 d` ( `ok[ i ]` = 0 when there is none ), `fmass[ i ]` the frozen mass at
+/// `w + t d`, `fwd[ i ]` the forward limit at floor `eps`, `bwd[ i ]` the backward target from the trial `t` ( all user order )
+void oracle_along( I64 n, const double *pos, const double *w, const double *d, const double *lo, const double *hi, int kind, double value,
+                   int nx, int ny, const double *geo, const double *img, double t, double eps, double *poly, int *ok, double *fmass,
+                   double *fwd, double *bwd ) {
+    orc::Tree tr;
+    orc::build_tree( tr, n, pos, 10 );
+    std::vector<double> W( w, w + n );
+    const orc::Density dn = make_density( kind, value, nx, ny, geo, img );
+    orc::Sweep sw( tr, lo, hi, dn );
+    sw.set_weights( W );
+    orc::Edges E;
+    sw.edges = &E;
+    std::vector<double> a;
+    sw.measures( a, nullptr );
+    const orc::Along al{ pos, w, d, lo, hi, &E, n, &sw.dn };
+    for ( I64 i = 0; i < n; ++i ) {
+        double a0 = 0, a1 = 0, a2 = 0, m = 0;
+        ok[ i ] = al.area_polynomial( i, 0.0, a0, a1, a2 ) ? 1 : 0;
+        poly[ 3 * i ] = a0; poly[ 3 * i + 1 ] = a1; poly[ 3 * i + 2 ] = a2;
+        fmass[ i ] = al.frozen_mass( i, t, m ) ? m : std::nan( "" );
+        fwd[ i ] = orc::alpha_forward( al, i, eps );
+        bwd[ i ] = orc::alpha_backward( al, i, eps, t );
+    }
+}
+
+/// the image at width `s` ( testing hook ): `out[ ny * nx ]`
+void oracle_blur( int nx, int ny, double hx, double hy, const double *img, double s, double *out ) {
+    orc::Density dn;
+    dn.kind = 1; dn.nx = nx; dn.ny = ny; dn.hx = hx; dn.hy = hy; dn.img = img;
+    std::vector<double> b;
+    orc::blur_image( dn, s, b );
+    std::memcpy( out, b.data(), sizeof( double ) * b.size() );
+}
+
 /// the Jacobi-PCG alone on the Laplacian of facets `( fi, fj, fc )`: `L d = b`, `d[ 0 ] = 0`
 int oracle_cg( I64 n, I64 nf, const I64 *fi, const I64 *fj, const double *fc, const double *b, double tol, int maxit, double *d ) {
     std::vector<orc::Facet> fa( sz( nf ) );
@@ -1121,9 +1637,10 @@ int oracle_cg( I64 n, I64 nf, const I64 *fi, const I64 *fj, const double *fc, co
     return ok ? st.nb_iter : -1;
 }
 
-/// THE SOLVE ( `sdotplan/Solve.h:111-371`, trials, no aggregation, no continuation ).
-/// `opts`: tol_abs, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol,
-/// lin_maxit, keep_start ( 12 doubles ). `start_box`: flag, lo[ 2 ], hi[ 2 ].
+/// THE SOLVE ( `sdotplan/Solve.h:111-371`, no aggregation ).
+/// `opts` ( 19 doubles ): tol_abs, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol,
+/// lin_maxit, keep_start, step ( 0 trials, 1 limits ), factor, continuation ( 0 never, 1 auto, 2 always ), conv_threshold,
+/// conv_s0, conv_ratio, conv_min. `start_box`: flag, lo[ 2 ], hi[ 2 ].
 /// Outputs: `weights[ n ]`, `stats[ 31 ]`, `hist[ ( maxit + 1 ) * 8 ]` ( returns the row count ),
 /// `masses[ n ]`, `bary[ n * 2 ]`, `cost[ 1 ]`.
 int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w0, const double *lo, const double *hi, int kind,
@@ -1135,12 +1652,36 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     o.tol_abs = opts[ 0 ]; o.tol_rel = opts[ 1 ]; o.maxit = int( opts[ 2 ] ); o.max_backtracks = int( opts[ 3 ] );
     o.t_min = opts[ 4 ]; o.mult_ok = opts[ 5 ]; o.residual = int( opts[ 6 ] ); o.power = opts[ 7 ]; o.switch_residual = opts[ 8 ];
     o.lin_tol = opts[ 9 ]; o.lin_maxit = int( opts[ 10 ] ); o.keep_start = int( opts[ 11 ] );
+    o.step = int( opts[ 12 ] ); o.factor = opts[ 13 ] > 0 ? opts[ 13 ] : 0.9;
+    const int continuation = int( opts[ 14 ] );
+    const double conv_threshold = opts[ 15 ], conv_s0 = opts[ 16 ], conv_ratio = opts[ 17 ], conv_min = opts[ 18 ];
     Tree t;
     build_tree( t, n, pos, 10 );
-    Sweep sw( t, lo, hi, make_density( kind, value, nx, ny, geo, img ) );
+    const Density dn0 = make_density( kind, value, nx, ny, geo, img );
+    Sweep sw( t, lo, hi, dn0 );
     Solver nt( sw, o );
+    nt.pos = pos;
+    // the stages of the width continuation ( `Solve.h:176-200`; an image only: a constant is positive everywhere )
+    const bool possible = kind == 1 || kind == 2;
+    const double s0 = conv_s0 > 0 ? conv_s0 : 0.5 * std::sqrt( ( hi[ 0 ] - lo[ 0 ] ) * ( hi[ 0 ] - lo[ 0 ] ) + ( hi[ 1 ] - lo[ 1 ] ) * ( hi[ 1 ] - lo[ 1 ] ) );
+    double smallest = 0;                                 // the image's pixel, or the narrowest Gaussian ( `convolved/*.h::min_scale` )
+    if ( kind == 1 ) smallest = std::min( dn0.hx, dn0.hy );
+    if ( kind == 2 ) { smallest = 1e300; for ( int q = 0; q < dn0.ng; ++q ) smallest = std::min( smallest, dn0.gauss[ 4 * q + 2 ] ); }
+    const double s_min = conv_min > 0 ? conv_min : smallest / 4;
+    std::vector<double> scales = continuation == 2 && possible ? continuation_steps( s0, conv_ratio, s_min ) : std::vector<double>{ 0.0 };
+    std::vector<double> blurred;
+    double s_current = 0;
+    auto set_stage = [&]( double sc ) {
+        s_current = sc;
+        if ( ! possible ) return;
+        Density dn = dn0;
+        if ( sc > 0 && kind == 1 ) { blur_image( dn0, sc, blurred ); dn.img = blurred.data(); }
+        if ( kind == 2 ) dn.conv_s = sc;
+        sw.dn = dn;
+    };
+    set_stage( scales[ 0 ] );
     const int cap_steps = o.maxit + 1;
-    int nb_steps = 0;
+    int nb_steps = 0, nb_steps_before = 0;
     nt.after_step = [&]( int it, double tt, int nb_evals ) {
         if ( nb_steps >= cap_steps ) return;
         double mn = nt.a.empty() ? 0 : nt.a[ 0 ], mx = 0, l2 = 0;
@@ -1150,8 +1691,8 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
             l2 += ( nt.a[ size_t( i ) ] - nt.nu[ size_t( i ) ] ) * ( nt.a[ size_t( i ) ] - nt.nu[ size_t( i ) ] );
         }
         double *r = hist + size_t( nb_steps ) * NB_HIST;
-        r[ H_STEP ] = it; r[ H_T ] = tt; r[ H_RESIDUAL_L2 ] = std::sqrt( l2 ); r[ H_MIN_MASS ] = mn;
-        r[ H_MAX_RESIDUAL ] = mx; r[ H_NB_DIAG ] = sw.nb_diag; r[ H_NB_EVALS ] = nb_evals; r[ H_S ] = 0;
+        r[ H_STEP ] = nb_steps_before + it; r[ H_T ] = tt; r[ H_RESIDUAL_L2 ] = std::sqrt( l2 ); r[ H_MIN_MASS ] = mn;
+        r[ H_MAX_RESIDUAL ] = mx; r[ H_NB_DIAG ] = sw.nb_diag; r[ H_NB_EVALS ] = nb_evals; r[ H_S ] = s_current;
         ++nb_steps;
     };
     std::vector<double> nu( nu_in, nu_in + n ), w( sz( n ), 0.0 );
@@ -1162,15 +1703,17 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     for ( auto &v : w ) v -= gauge;
     std::vector<double> a;
     std::vector<Facet> fa;
-    nt.measures_and_facets( w, a, fa );
+    Edges ea;
+    nt.measures_and_facets( w, a, fa, &ea );
     double nu_min = nu[ 0 ];
     for ( double v : nu ) nu_min = std::min( nu_min, v );
     auto min_mass = []( const std::vector<double> &m ) { double r = m.empty() ? 0 : m[ 0 ]; for ( double v : m ) r = std::min( r, v ); return r; };
     if ( given && ! o.keep_start && min_mass( a ) < 1e-3 * nu_min ) {
         std::vector<double> wz( sz( n ), 0.0 ), az;
         std::vector<Facet> fz;
-        nt.measures_and_facets( wz, az, fz );
-        if ( min_mass( az ) > min_mass( a ) ) { w.swap( wz ); a.swap( az ); fa.swap( fz ); start = 1; }
+        Edges ez;
+        nt.measures_and_facets( wz, az, fz, &ez );
+        if ( min_mass( az ) > min_mass( a ) ) { w.swap( wz ); a.swap( az ); fa.swap( fz ); std::swap( ea, ez ); start = 1; }
         else sw.set_weights( w );
     }
     if ( min_mass( a ) <= 0 && ! ( given && o.keep_start ) ) {
@@ -1178,26 +1721,50 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
         if ( start_box && start_box[ 0 ] > 0 ) { blo[ 0 ] = start_box[ 1 ]; blo[ 1 ] = start_box[ 2 ]; bhi[ 0 ] = start_box[ 3 ]; bhi[ 1 ] = start_box[ 4 ]; }
         std::vector<double> w1, a1;
         std::vector<Facet> f1;
+        Edges e1;
         similarity( t, pos, blo, bhi, w1 );
-        nt.measures_and_facets( w1, a1, f1 );
-        if ( min_mass( a1 ) > min_mass( a ) ) { w.swap( w1 ); a.swap( a1 ); fa.swap( f1 ); start = 2; }
+        nt.measures_and_facets( w1, a1, f1, &e1 );
+        if ( min_mass( a1 ) > min_mass( a ) ) { w.swap( w1 ); a.swap( a1 ); fa.swap( f1 ); std::swap( ea, e1 ); start = 2; }
         else sw.set_weights( w );
     }
     const double min_start_mass = min_mass( a );
-    double domain_mass = 0, nu_mass = 0;
-    for ( I64 i = 0; i < n; ++i ) { domain_mass += a[ size_t( i ) ]; nu_mass += nu[ size_t( i ) ]; }
-    std::vector<double> nu_s = nu;
-    if ( domain_mass > 0 && nu_mass > 0 && domain_mass != nu_mass )
-        for ( auto &v : nu_s ) v *= domain_mass / nu_mass;
-    nt.nu = nu_s;
-    nt.a = a;
-    nt.fa = fa;
-    nt.solves( w );
-    w = nt.w;
-    a = nt.a;
+    // AUTO: the density is missing where cells are -> the continuation, from the same start ( `Solve.h:228-236` )
+    if ( continuation == 1 && possible && scales.size() == 1 && min_start_mass < conv_threshold * nu_min ) {
+        scales = continuation_steps( s0, conv_ratio, s_min );
+        set_stage( scales[ 0 ] );
+        nt.measures_and_facets( w, a, fa, &ea );
+    }
+    double domain_mass = 0, residual0 = 0, eps0 = 0;
+    int it_switch = -1;
+    for ( size_t stage = 0; stage < scales.size(); ++stage ) {
+        if ( stage > 0 ) {                               // the next density: the measures of the start redone
+            set_stage( scales[ stage ] );
+            nt.measures_and_facets( w, a, fa, &ea );
+        }
+        double nu_mass = 0;
+        domain_mass = 0;
+        for ( I64 i = 0; i < n; ++i ) { domain_mass += a[ size_t( i ) ]; nu_mass += nu[ size_t( i ) ]; }
+        std::vector<double> nu_s = nu;
+        if ( domain_mass > 0 && nu_mass > 0 && domain_mass != nu_mass )
+            for ( auto &v : nu_s ) v *= domain_mass / nu_mass;
+        nt.nu = nu_s;
+        nt.a = a;
+        nt.fa = fa;
+        nt.ea = ea;
+        nt.it_switch = -1;
+        nt.solves( w );
+        w = nt.w;
+        a = nt.a;
+        fa = nt.fa;
+        ea = nt.ea;
+        if ( stage == 0 ) { residual0 = nt.residual0; eps0 = nt.eps; it_switch = nt.it_switch; }
+        nb_steps_before = nb_steps > 0 ? int( hist[ size_t( nb_steps - 1 ) * NB_HIST + H_STEP ] ) + 1 : 0;
+        if ( nt.status == 4 ) break;
+    }
     for ( I64 i = 0; i < n; ++i ) { weights[ i ] = w[ size_t( i ) ]; masses[ i ] = a[ size_t( i ) ]; }
     const int nb_diag_descent = sw.nb_diag;
     {
+        set_stage( 0 );                                  // the moments on the TRUE density
         sw.set_weights( w );
         std::vector<double> bc;
         double c = 0;
@@ -1208,7 +1775,7 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     for ( int q = 0; q < NB_STATS; ++q ) stats[ q ] = 0;
     stats[ STATUS ] = nt.status;
     stats[ RESIDUAL ] = nt.residual;
-    stats[ RESIDUAL0 ] = nt.residual0;
+    stats[ RESIDUAL0 ] = residual0;
     stats[ NB_ITER ] = nt.nb_iter;
     stats[ NB_DIAG ] = nb_diag_descent;
     stats[ NB_BACKTRACKS ] = nt.nb_backtracks;
@@ -1216,15 +1783,18 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     stats[ T_DIAG ] = sw.t_diag;
     stats[ T_ASM ] = nt.t_asm;
     stats[ T_LIN ] = nt.lst.t;
-    stats[ EPS ] = nt.eps;
+    stats[ EPS ] = eps0;
+    stats[ T_LIM ] = nt.t_lim;
+    stats[ NB_CELL_LIM ] = double( nt.nb_cell_lim );
+    stats[ NB_LIMIT_ROUNDS ] = double( nt.nb_limit_rounds );
     stats[ DOMAIN_MASS ] = domain_mass;
     stats[ LIN_NB_ITER ] = nt.lst.nb_iter;
     stats[ LIN_WORST ] = nt.lst.worst;
     stats[ START ] = start;
     stats[ T_TOTAL ] = now() - t_begin;
-    stats[ NB_CONTINUATION_STEPS ] = 1;
+    stats[ NB_CONTINUATION_STEPS ] = double( scales.size() );
     stats[ MIN_START_MASS ] = min_start_mass;
-    stats[ IT_SWITCH ] = nt.it_switch;
+    stats[ IT_SWITCH ] = it_switch;
     stats[ RESIDUAL_FULL ] = nt.residual;
     return nb_steps;
 }

@@ -72,6 +72,9 @@ Density make_density( int kind, double rho, const double *img, int nx, int ny, c
         SDOT_CHECK( geo[ 2 ] > 0 && geo[ 3 ] > 0, "an image density needs positive pixel sizes" );
         d.img = img; d.nx = nx; d.ny = ny;
         d.x0 = geo[ 0 ]; d.y0 = geo[ 1 ]; d.hx = geo[ 2 ]; d.hy = geo[ 3 ];
+    } else if ( kind == SDOT_AMD_GAUSSIANS ) {
+        SDOT_CHECK( img != nullptr && nx > 0, "a sum of Gaussians needs at least one Gaussian" );
+        d.gauss = img; d.ng = nx;
     } else
         SDOT_CHECK( kind == SDOT_AMD_CONSTANT, "unknown density kind" );
     return d;

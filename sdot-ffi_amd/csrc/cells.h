@@ -5,14 +5,16 @@
 
 namespace sdot_amd {
 
-/// the target density: a constant on the domain box, or a piecewise-constant image
-/// `img[ j * nx + i ]` on pixels `[ x0 + i hx, x0 + ( i + 1 ) hx ] x [ y0 + j hy, ... ]`
+/// the target density: a constant on the domain box, a piecewise-constant image `img[ j * nx + i ]` on pixels
+/// `[ x0 + i hx, x0 + ( i + 1 ) hx ] x [ y0 + j hy, ... ]`, or a sum of isotropic Gaussians truncated to the box
 struct Density {
-    int           kind = 0;        ///< 0 constant, 1 image
+    int           kind = 0;        ///< 0 constant, 1 image, 2 a sum of isotropic Gaussians
     double        rho = 1;
     const double *img = nullptr;   ///< device
     int           nx = 0, ny = 0;
     double        x0 = 0, y0 = 0, hx = 1, hy = 1;
+    const double *gauss = nullptr; ///< device, `[ ng ][ 4 ]`: centre x, y, width, mass
+    int           ng = 0;
 };
 
 /// what one diagram writes, in RANK order: the masses, the rows of the Laplacian ( each cell's
@@ -31,6 +33,9 @@ struct DiagramOut {
     unsigned *counters = nullptr;
     int      *ovf_list = nullptr;
     unsigned long long *events = nullptr;   ///< side tests, pruning vertex tests, new vertices, node visits, cuts
+    // facets mode, when set: each cell's cuts in polygon order ( `edges[ q * n + k ]`: a seed rank, or `-1 - f` side `f`
+    // of the box, 0 bottom, 1 right, 2 top, 3 left ) and their count ( 0 for an empty cell, -1 past `EDGE_CAP` or failed )
+    int      *edges = nullptr, *nb_edges = nullptr;
 };
 
 /// the vertex slots of the passes ( the first in LDS, the second in global slots )
@@ -40,6 +45,7 @@ constexpr int CAP2 = 96;             ///< second pass, LDS ( one wave per block 
 constexpr int CAP3 = 2048;           ///< third pass, global slots: the vertex limit of a cell
 constexpr int PASS2_BLOCKS = 1024;
 constexpr int OVF_SLOTS = 4096;
+constexpr int EDGE_CAP = 16;         ///< the edges a cell keeps for the step by the limits ( `gpu/Cell2D.cuh:97` )
 
 enum CellMode { MODE_MASSES = 0, MODE_FACETS = 1, MODE_MOMENTS = 2 };
 

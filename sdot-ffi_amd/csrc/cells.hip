@@ -21,6 +21,8 @@
 // oracle's, so a constant density gives the oracle's masses to the bit on the same tree.
 
 #include "cells.h"
+#include "gauss.h"
+#include "image.h"
 
 #pragma clang fp contract( off )
 
@@ -29,7 +31,6 @@ namespace sdot_amd {
 namespace {
 
 enum { CUT_UNCHANGED = 0, CUT_DONE = 1, CUT_EMPTY = 2, CUT_NO_ROOM = 3 };
-constexpr int DOMAIN_ID = -1;
 
 struct Ev {
     unsigned side = 0, prune = 0, newv = 0, visits = 0, cuts = 0;
@@ -167,7 +168,7 @@ __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *l
     c.X( 1 ) = hi[ 0 ] - p0x; c.Y( 1 ) = lo[ 1 ] - p0y;
     c.X( 2 ) = hi[ 0 ] - p0x; c.Y( 2 ) = hi[ 1 ] - p0y;
     c.X( 3 ) = lo[ 0 ] - p0x; c.Y( 3 ) = hi[ 1 ] - p0y;
-    for ( int i = 0; i < 4; ++i ) c.I( i ) = DOMAIN_ID;
+    for ( int i = 0; i < 4; ++i ) c.I( i ) = -1 - i;     // the box's sides: bottom, right, top, left
     int nb = 4;
     int m = 0, h = wk.depth;
     unsigned long long pend = 0, left = 0;
@@ -230,53 +231,6 @@ __device__ __forceinline__ double shoelace( const S &c, int nb ) {
     return s / 2;
 }
 
-/// the pixel grid as seen from one seed: columns `u_i = x0 + i hx - p0x`, rows `v_j = y0 + j hy - p0y`
-struct Grid {
-    const double *img;
-    int nx, ny;
-    double x0, y0, hx, hy;   ///< x0, y0 in the seed's frame
-    int iref;                ///< the column the row integrals start from
-    __device__ double u( int i ) const { return x0 + double( i ) * hx; }
-    __device__ double v( int j ) const { return y0 + double( j ) * hy; }
-    __device__ double val( int i, int j ) const { return img[ size_t( j ) * size_t( nx ) + size_t( i ) ]; }
-    /// `int_{u_iref}^{u} rho( s, row j ) ds` for `u` in column `i`
-    __device__ double F0( int i, int j, double x ) const {
-        double acc = 0;
-        if ( i >= iref ) for ( int q = iref; q < i; ++q ) acc += val( q, j ) * hx;
-        else             for ( int q = i; q < iref; ++q ) acc -= val( q, j ) * hx;
-        return acc + val( i, j ) * ( x - u( i ) );
-    }
-};
-
-/// walks the segment `a -> b` ( seed frame ) across the pixel grid; `f( i, j, ta, tb )` per piece
-template<class F>
-__device__ __forceinline__ void grid_walk( const Grid &g, double ax, double ay, double bx, double by, F &&f ) {
-    const double gax = ( ax - g.x0 ) / g.hx, gay = ( ay - g.y0 ) / g.hy;
-    const double gbx = ( bx - g.x0 ) / g.hx, gby = ( by - g.y0 ) / g.hy;
-    const double ddx = gbx - gax, ddy = gby - gay;
-    const int sx = ddx > 0 ? 1 : ( ddx < 0 ? -1 : 0 ), sy = ddy > 0 ? 1 : ( ddy < 0 ? -1 : 0 );
-    const double rx = sx ? 1.0 / ddx : 0.0, ry = sy ? 1.0 / ddy : 0.0;
-    // the next vertical / horizontal grid line crossed, as a parameter
-    double nxl = sx > 0 ? floor( gax ) + 1 : ceil( gax ) - 1;
-    double nyl = sy > 0 ? floor( gay ) + 1 : ceil( gay ) - 1;
-    double t0 = 0;
-    for ( int guard = 0; guard < 1 << 20; ++guard ) {
-        const double tx = sx ? ( nxl - gax ) * rx : 2.0;
-        const double ty = sy ? ( nyl - gay ) * ry : 2.0;
-        double t1 = fmin( fmin( tx, ty ), 1.0 );
-        if ( t1 < t0 ) t1 = t0;
-        const double tm = 0.5 * ( t0 + t1 );
-        int i = int( floor( gax + tm * ddx ) ), j = int( floor( gay + tm * ddy ) );
-        i = i < 0 ? 0 : ( i >= g.nx ? g.nx - 1 : i );
-        j = j < 0 ? 0 : ( j >= g.ny ? g.ny - 1 : j );
-        if ( t1 > t0 ) f( i, j, t0, t1 );
-        if ( t1 >= 1.0 ) break;
-        if ( tx <= t1 ) nxl += sx;
-        if ( ty <= t1 ) nyl += sy;
-        t0 = t1;
-    }
-}
-
 /// edge `e` of the cell against the image, in one walk: its term of `mass = loop_integral F0 dy` and
 /// its integral `int_edge rho dl`
 template<class S>
@@ -332,6 +286,38 @@ __device__ void image_moments( const Grid &g, const S &c, int nb, double &m, dou
                 m2 += 0.5 * dy * ( f2 + y * y * f0 );
             }
         } );
+    }
+}
+
+/// a sum of Gaussians on the cell ( seed at `( ox, oy )` ): per Gaussian its mass `w | corner sum |`, and for `mom` the
+/// moments about the seed from the boundary fluxes of each edge
+template<class S>
+__device__ void gauss_cell( const Density &dn, double ox, double oy, const S &c, int nb, bool mom, double &m, double &mx, double &my, double &m2 ) {
+    m = mx = my = m2 = 0;
+    for ( int q = 0; q < dn.ng; ++q ) {
+        const GaussTerm G = gauss_term( dn, q, ox, oy );
+        const double W = gauss_corner_sum( G, [&]( auto &&f ) {
+            for ( int e = 0; e < nb; ++e ) { const int r = e + 1 < nb ? e + 1 : 0; f( c.X( e ), c.Y( e ), c.X( r ), c.Y( r ) ); }
+        } );
+        const double mq = G.w * fabs( W );
+        m += mq;
+        if ( ! mom ) continue;
+        double fx = 0, fy = 0, fr = 0;
+        for ( int e = 0; e < nb; ++e ) {
+            const int r = e + 1 < nb ? e + 1 : 0;
+            const double ex = c.X( r ) - c.X( e ), ey = c.Y( r ) - c.Y( e );
+            const double L = sqrt( ex * ex + ey * ey );
+            if ( ! ( L > 0 ) ) continue;
+            const double nx = ey / L, ny = -ex / L;      // outward: the cell is counter-clockwise
+            const double J = gauss_segment_line( G, c.X( e ), c.Y( e ), c.X( r ), c.Y( r ) );
+            fx += nx * J;
+            fy += ny * J;
+            fr += ( ( c.X( e ) - G.cx ) * nx + ( c.Y( e ) - G.cy ) * ny ) * J;
+        }
+        const double s2 = G.s * G.s, ix = -s2 * fx, iy = -s2 * fy, i2 = 2 * s2 * mq - s2 * fr;
+        mx += ix + G.cx * mq;
+        my += iy + G.cy * mq;
+        m2 += i2 + 2 * ( G.cx * ix + G.cy * iy ) + ( G.cx * G.cx + G.cy * G.cy ) * mq;
     }
 }
 
@@ -397,18 +383,14 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
         base = __shfl( base, 0, 64 );
     }
     if ( ! active ) return;
-    Grid g;
-    if ( P.dn.kind == 1 ) {
-        g.img = P.dn.img; g.nx = P.dn.nx; g.ny = P.dn.ny; g.hx = P.dn.hx; g.hy = P.dn.hy;
-        g.x0 = P.dn.x0 - p.x; g.y0 = P.dn.y0 - p.y;
-        int ir = int( floor( -g.x0 / g.hx ) );
-        g.iref = ir < 0 ? 0 : ( ir >= g.nx ? g.nx - 1 : ir );
-    }
+    Grid g{};
+    if ( P.dn.kind == 1 ) g = grid_from( P.dn, p.x, p.y );
     if ( P.mode == MODE_MOMENTS ) {
         double m = 0, mx = 0, my = 0, m2 = 0;
         if ( nb > 0 ) {
-            if ( P.dn.kind == 1 ) image_moments( g, c, nb, m, mx, my, m2 );
-            else                  const_moments( c, nb, P.dn.rho, m, mx, my, m2 );
+            if ( P.dn.kind == 1 )      image_moments( g, c, nb, m, mx, my, m2 );
+            else if ( P.dn.kind == 2 ) gauss_cell( P.dn, p.x, p.y, c, nb, true, m, mx, my, m2 );
+            else                       const_moments( c, nb, P.dn.rho, m, mx, my, m2 );
         }
         P.out.bary[ 2 * k ] = m > 0 ? p.x + mx / m : p.x;
         P.out.bary[ 2 * k + 1 ] = m > 0 ? p.y + my / m : p.y;
@@ -421,11 +403,21 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
         if ( nb > 0 ) {
             if ( P.dn.kind == 1 ) {
                 for ( int e = 0; e < nb; ++e ) { double mt, ln; image_edge( g, c, nb, e, mt, ln ); mass += mt; }
+            } else if ( P.dn.kind == 2 ) {
+                double u0, u1, u2;
+                gauss_cell( P.dn, p.x, p.y, c, nb, false, mass, u0, u1, u2 );
             } else
                 mass = P.dn.rho * shoelace( c, nb );
         }
         P.out.mass[ k ] = mass;
         return;
+    }
+    if ( P.out.edges ) {                                 // the cuts in polygon order, for the step by the limits
+        int ne = nb < 3 ? 0 : nb;
+        if ( ne > EDGE_CAP ) ne = -1;
+        else
+            for ( int e = 0; e < ne; ++e ) P.out.edges[ size_t( e ) * size_t( P.n ) + size_t( k ) ] = c.I( e );
+        P.out.nb_edges[ k ] = ne;
     }
     const I64 at = I64( base ) + pre;
     const bool fits = I64( base ) + total <= P.out.facet_cap;
@@ -440,6 +432,11 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
             double mt;
             image_edge( g, c, nb, e, mt, fm );
             mass += mt;
+        } else if ( P.dn.kind == 2 ) {
+            if ( id >= 0 ) {
+                const int f = e + 1 < nb ? e + 1 : 0;
+                for ( int u = 0; u < P.dn.ng; ++u ) fm += gauss_segment_line( gauss_term( P.dn, u, p.x, p.y ), c.X( e ), c.Y( e ), c.X( f ), c.Y( f ) );
+            }
         } else if ( id >= 0 ) {
             const int f = e + 1 < nb ? e + 1 : 0;
             const double ex = c.X( f ) - c.X( e ), ey = c.Y( f ) - c.Y( e );
@@ -457,7 +454,8 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
         dia += cij;
         ++q;
     }
-    if ( P.dn.kind != 1 && nb > 0 ) mass = P.dn.rho * shoelace( c, nb );
+    if ( P.dn.kind == 0 && nb > 0 ) mass = P.dn.rho * shoelace( c, nb );
+    if ( P.dn.kind == 2 && nb > 0 ) { double u0, u1, u2; gauss_cell( P.dn, p.x, p.y, c, nb, false, mass, u0, u1, u2 ); }
     P.out.mass[ k ] = mass;
     P.out.dia[ k ] = dia > 0 ? dia : 1.0;
 }
@@ -550,7 +548,10 @@ __global__ __launch_bounds__( 64 ) void k_cells_pass3( Params P, double *gx, dou
                 if ( atomicAdd( &P.out.counters[ 2 ], 1u ) == 0 ) P.out.counters[ 3 ] = unsigned( k );
                 active = false;
                 P.out.mass[ k ] = 0;
-                if ( P.mode == MODE_FACETS ) { P.out.row_cnt[ k ] = 0; P.out.row_off[ k ] = 0; P.out.dia[ k ] = 1; }
+                if ( P.mode == MODE_FACETS ) {
+                    P.out.row_cnt[ k ] = 0; P.out.row_off[ k ] = 0; P.out.dia[ k ] = 1;
+                    if ( P.out.edges ) P.out.nb_edges[ k ] = -1;
+                }
             }
         }
         finish_cell( P, k, c, nb, active, false );

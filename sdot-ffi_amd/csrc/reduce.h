@@ -1,0 +1,74 @@
+// Fixed-shape reductions on one stream: `RED_BLOCKS` partials in a grid-stride loop, then one block in a fixed
+// order, so a result is bitwise the same run to run.
+#pragma once
+
+#include "common.h"
+
+namespace sdot_amd {
+
+namespace {
+
+// ---- fixed-shape reductions -------------------------------------------------------------------------
+
+enum { OP_SUM = 0, OP_MIN = 1, OP_MAX = 2 };
+
+__device__ __forceinline__ double op_init( int op ) { return op == OP_SUM ? 0.0 : ( op == OP_MIN ? INFINITY : -INFINITY ); }
+__device__ __forceinline__ double op_apply( int op, double a, double b ) { return op == OP_SUM ? a + b : ( op == OP_MIN ? fmin( a, b ) : fmax( a, b ) ); }
+
+template<int K>
+struct Ops { int op[ K ]; };
+
+__device__ __forceinline__ double block_op( int op, double v, double *sh ) {
+    if ( op == OP_SUM ) return block_sum<RED_THREADS>( v, sh );
+    if ( op == OP_MIN ) return block_min<RED_THREADS>( v, sh );
+    return block_max<RED_THREADS>( v, sh );
+}
+
+template<int K, class F>
+__global__ __launch_bounds__( RED_THREADS ) void k_partials( I64 n, F f, Ops<K> ops, double *part ) {
+    __shared__ double sh[ RED_THREADS ];
+    double acc[ K ];
+    for ( int q = 0; q < K; ++q ) acc[ q ] = op_init( ops.op[ q ] );
+    for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < n; i += I64( RED_BLOCKS ) * RED_THREADS ) {
+        double v[ K ];
+        f( i, v );
+        for ( int q = 0; q < K; ++q ) acc[ q ] = op_apply( ops.op[ q ], acc[ q ], v[ q ] );
+    }
+    for ( int q = 0; q < K; ++q ) {
+        const double r = block_op( ops.op[ q ], acc[ q ], sh );
+        if ( threadIdx.x == 0 ) part[ q * RED_BLOCKS + blockIdx.x ] = r;
+    }
+}
+
+/// the partials of every quantity reduced by one block; `E` then runs on thread 0 with the results
+template<int K, class E>
+__global__ __launch_bounds__( RED_THREADS ) void k_final( const double *part, Ops<K> ops, double *out, E epilogue ) {
+    __shared__ double sh[ RED_THREADS ];
+    __shared__ double res[ K ];
+    for ( int q = 0; q < K; ++q ) {
+        double a = op_init( ops.op[ q ] );
+        for ( int j = threadIdx.x; j < RED_BLOCKS; j += RED_THREADS ) a = op_apply( ops.op[ q ], a, part[ q * RED_BLOCKS + j ] );
+        const double r = block_op( ops.op[ q ], a, sh );
+        if ( threadIdx.x == 0 ) res[ q ] = r;
+    }
+    __syncthreads();
+    if ( threadIdx.x == 0 ) {
+        for ( int q = 0; q < K; ++q ) out[ q ] = res[ q ];
+        epilogue( res );
+    }
+}
+
+struct NoEpilogue { __device__ void operator()( const double * ) const {} };
+
+template<int K, class F, class E = NoEpilogue>
+void reduce( I64 n, const F &f, Ops<K> ops, double *out, hipStream_t st, E e = E{} ) {
+    double *part = arena().get<double>( "red.part", size_t( K ) * RED_BLOCKS );
+    k_partials<K, F><<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, f, ops, part );
+    check_launch( "k_partials" );
+    k_final<K, E><<<1, RED_THREADS, 0, st>>>( part, ops, out, e );
+    check_launch( "k_final" );
+}
+
+} // namespace
+
+} // namespace sdot_amd

@@ -9,7 +9,7 @@ reference's layouts ( `_STATS`, `_HISTORY` ) and `OtNotConverged` is raised the 
 import numpy as np
 
 from . import _native
-from .distributions import Image, UniformBox
+from .distributions import Image, SumOfGaussians, UniformBox
 
 _STATS = [ "status", "residual", "residual0", "nb_iter", "nb_diag", "nb_backtracks", "t_majorant", "t_diag", "t_asm", "t_lin", "t_lim", "eps",
            "domain_mass", "nb_overflowed", "nb_cell_lim", "nb_limit_rounds", "lin_nb_hierarchies", "lin_nb_iter", "lin_worst", "start", "t_total",
@@ -28,11 +28,9 @@ _CONTINUATION = { "never": 0, "auto": 1, "always": 2 }
 def card_options( settings, density, verbose = False ):
     """the 41-double option vector ( `SdotPlanNd.py:806-833` of the reference, same names and meanings )"""
     tun = settings.tuning
-    step = { "auto": "trials" }.get( tun.step, tun.step )
+    step = { "auto": "limits" }.get( tun.step, tun.step )       # ( 2D: `SdotPlanNd.py:462` )
     if step not in _STEP:
         raise ValueError( f"unknown step: { tun.step!r} ( 'auto', 'trials' or 'limits' )" )
-    if step == "limits":
-        raise NotImplementedError( "step = 'limits' is not on the MI355X path yet: 'trials' ( or 'auto' )" )
     if tun.residual not in _RESIDUAL:
         raise ValueError( f"unknown residual: { tun.residual!r}" )
     if tun.linear_solver not in _LIN:
@@ -55,6 +53,11 @@ def card_options( settings, density, verbose = False ):
         nx, ny = density.shape
         for name, v in zip( ( "img_x0", "img_y0", "img_hx", "img_hy", "img_nx", "img_ny" ), ( lo[ 0 ], lo[ 1 ], h[ 0 ], h[ 1 ], nx, ny ) ):
             put( name, v )
+        # the blur of the width continuation stops at a quarter pixel ( `sdotplan/convolved/Image.h:49-53` )
+        put( "conv_possible", 1 ); put( "min_scale", min( abs( h[ 0 ] ), abs( h[ 1 ] ) ) / 4 )
+    if isinstance( density, SumOfGaussians ):
+        # the Gaussians' count; the widening stops at a quarter of the narrowest ( `convolved/SumOfGaussians.h:39` )
+        put( "img_nx", len( density.sigmas ) ); put( "conv_possible", 1 ); put( "min_scale", float( density.sigmas.min() ) / 4 )
     return np.asarray( opts )
 
 
@@ -92,8 +95,8 @@ class SdotPlanNd:
         problem.domain
         lo, hi = dst.box()
         box = [ lo[ 0 ], lo[ 1 ], hi[ 0 ], hi[ 1 ] ]
-        if not isinstance( dst, ( Image, UniformBox ) ):
-            dst.native_args( None )                      # ( raises: not on this path )
+        if not isinstance( dst, ( Image, UniformBox, SumOfGaussians ) ):
+            raise NotImplementedError( f"SdotPlanNd on the MI355X path: an Image, a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
         opts = card_options( settings, dst, verbose )
         res = _native.solve_2d( src.positions, src.weights, w0, opts, box, dst )
         #: the fitted weights ( `weights[ 0 ] == 0` ), device tensor, user order

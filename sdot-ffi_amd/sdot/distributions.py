@@ -133,14 +133,16 @@ class UniformBox:
 
 class SumOfGaussians:
     """isotropic Gaussians `weights_k N( positions_k, sigmas_k^2 I )` truncated to `support_box`
-    ( reference `distributions/SumOfGaussians.py:11` ). Declared with its normalization and support;
-    its integrals on cells are not on this device path yet."""
+    ( reference `distributions/SumOfGaussians.py:11-101` ): `weights_k` is the MASS of Gaussian `k` ( all 1 by
+    default ), the declared support is `support_box`, else the box of `support_sigmas` deviations around them."""
 
-    def __init__( self, positions, sigmas, weights = None, support_box = None, support_sigmas = 6.0, target_mass = 1.0 ):
+    def __init__( self, positions, sigmas, weights = None, target_mass = 1.0, support_sigmas = 6.0, support_box = None ):
         self.positions = np.asarray( _host( positions ), dtype = np.float64 )
         self.sigmas = np.asarray( _host( sigmas ), dtype = np.float64 ).reshape( -1 )
         k = self.positions.shape[ 0 ]
-        self.weights = np.full( k, 1.0 / k ) if weights is None else np.asarray( _host( weights ), dtype = np.float64 ).reshape( -1 )
+        self.weights = np.ones( k ) if weights is None else np.asarray( _host( weights ), dtype = np.float64 ).reshape( -1 )
+        if np.any( self.sigmas <= 0 ):
+            raise ValueError( "SumOfGaussians: the widths must be positive" )
         self.nb_dims = self.positions.shape[ 1 ]
         self.target_mass = target_mass
         if support_box is None:
@@ -149,9 +151,14 @@ class SumOfGaussians:
             support_box = ( lo, hi )
         self.support_box = ( np.asarray( support_box[ 0 ], dtype = np.float64 ), np.asarray( support_box[ 1 ], dtype = np.float64 ) )
 
+    @property
+    def mass( self ):
+        return float( self.weights.sum() )
+
     def normalized_version( self, nb_dims = None ):
-        return SumOfGaussians( self.positions, self.sigmas, self.weights * ( self.target_mass / self.weights.sum() ), self.support_box,
-                               target_mass = self.target_mass )
+        """the masses scaled to `target_mass` ( a division: the total is the sum of the masses )"""
+        return SumOfGaussians( self.positions, self.sigmas, self.weights * ( self.target_mass / self.weights.sum() ),
+                               target_mass = self.target_mass, support_box = self.support_box )
 
     def box( self ):
         return self.support_box[ 0 ].copy(), self.support_box[ 1 ].copy()
@@ -162,4 +169,8 @@ class SumOfGaussians:
         return np.concatenate( [ np.eye( d ), -np.eye( d ) ] ), np.concatenate( [ hi, -lo ] )
 
     def native_args( self, device_tensor ):
-        raise NotImplementedError( "SumOfGaussians: the Gaussian integrals of the cells are not on the MI355X path yet" )
+        """`( kind, rho, gauss, ng, 0, geo )`: the Gaussians as rows ( centre x, y, width, mass ) ( `SdotPlanNd.py:837-862` )"""
+        if self.nb_dims != 2:
+            raise NotImplementedError( "SumOfGaussians: 2D on this path" )
+        g = np.concatenate( [ self.positions, self.sigmas[ :, None ], self.weights[ :, None ] ], axis = 1 )
+        return 2, 0.0, device_tensor( np.ascontiguousarray( g ).reshape( -1 ) ), g.shape[ 0 ], 0, [ 0.0, 0.0, 1.0, 1.0 ]

@@ -207,11 +207,11 @@ def test_linear_solves_match_a_dense_solve( native, oracle ):
         assert np.abs( sol[ k ] - ref ).max() < 1e-9 * np.abs( ref ).max()
 
 
-def _solve_both( oracle, pos, density_gpu, density_oracle, tol, lin_tol ):
+def _solve_both( oracle, pos, density_gpu, density_oracle, tol, lin_tol, step = "trials", continuation = "never", maxit = 100 ):
     from sdot import Iterative, OtProblem, SumOfDiracs, Tuning
-    sol = OtProblem( SumOfDiracs( pos ), density_gpu ).solve( Iterative( tol = tol, continuation = "never",
-                                                                          tuning = Tuning( linear_solver = "cg", linear_tol = lin_tol ) ) )
-    ro = oracle.solve( pos, tol = tol, lin_tol = lin_tol, density = density_oracle )
+    sol = OtProblem( SumOfDiracs( pos ), density_gpu ).solve( Iterative( tol = tol, continuation = continuation, max_iter = maxit,
+                                                                          tuning = Tuning( step = step, linear_solver = "cg", linear_tol = lin_tol ) ) )
+    ro = oracle.solve( pos, tol = tol, lin_tol = lin_tol, density = density_oracle, step = step, continuation = continuation, maxit = maxit )
     return sol, ro
 
 
@@ -247,6 +247,118 @@ def test_solve_image_matches_the_oracle( oracle, native ):
     w = sol.weights.cpu().numpy()
     assert np.abs( w - ro[ "weights" ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max()
     assert abs( sol.cost - ro[ "cost" ] ) < 1e-10 * ro[ "cost" ]
+
+
+def _stats( raw ):
+    from sdot.SdotPlanNd import _STATS
+    return dict( zip( _STATS, np.asarray( raw ).tolist() ) )
+
+
+def test_limits_step_matches_the_oracle( oracle, native ):
+    # the step by the limits ( the 2D default ): the forward limits of the accepted diagram, and the backward rounds on
+    # the trials that fall under the floor ( which a bolder factor than 0.9 provokes ): the oracle's decisions
+    from sdot import Iterative, Tuning, UniformBox
+    from sdot.SdotPlanNd import card_options
+    rng = np.random.default_rng( 24 )
+    pos = np.clip( np.concatenate( [ rng.normal( c, 0.01, ( 250, 2 ) ) for c in rng.uniform( 0.2, 0.8, ( 8, 2 ) ) ] ), 1e-3, 1 - 1e-3 )
+    n = len( pos )
+    box = UniformBox( [ 0, 0 ], [ 1, 1 ] )
+    for factor in ( 0.9, 1.5 ):
+        opts = card_options( Iterative( tol = 1e-10, continuation = "never", tuning = Tuning( step = "limits", linear_solver = "cg", linear_tol = 1e-11 ) ), box )
+        opts[ native.CARD_OPTIONS.index( "factor" ) ] = factor
+        say( "case limits clusters factor", factor )
+        r = native.solve_2d( pos, np.full( n, 1 / n ), None, opts, [ 0, 0, 1, 1 ], box )
+        st = _stats( r[ "stats" ] )
+        ro = oracle.solve( pos, tol = 1e-10, step = "limits", factor = factor, lin_tol = 1e-11 )
+        assert st[ "status" ] == 1 and ro[ "stats" ][ "status" ] == 1
+        for key in ( "nb_iter", "nb_diag", "nb_backtracks", "nb_limit_rounds", "nb_cell_lim" ):
+            assert st[ key ] == ro[ "stats" ][ key ], ( factor, key )
+        if factor > 1:
+            assert st[ "nb_limit_rounds" ] > 0
+        w = r[ "weights" ].cpu().numpy()
+        assert np.abs( w - ro[ "weights" ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max()
+
+
+def test_limits_step_on_an_image_matches_the_oracle( oracle, native ):
+    from sdot import Image
+    rng = np.random.default_rng( 1 )
+    n = 20000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    img = H.bumps_image( 128, 128, rng )
+    dens = Image( img, origin = [ 0, 0 ], frame = np.eye( 2 ) / 128 ).normalized_version()
+    say( "case limits image 20k" )
+    sol, ro = _solve_both( oracle, pos, dens, _img( dens.values ), 1e-9 / 10, 1e-10, step = "limits" )
+    assert sol.converged and ro[ "stats" ][ "status" ] == 1
+    assert sol.stats[ "nb_iter" ] == ro[ "stats" ][ "nb_iter" ] and sol.stats[ "nb_diag" ] == ro[ "stats" ][ "nb_diag" ]
+    w = sol.weights.cpu().numpy()
+    assert np.abs( w - ro[ "weights" ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max()
+
+
+def test_continuation_matches_the_oracle( oracle, native ):
+    # the width continuation: the same stages ( the image blurred on the device by the oracle's filter ), a Newton per
+    # stage, the last one on the true density
+    from sdot import Image
+    rng = np.random.default_rng( 25 )
+    n = 5000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    img = H.bumps_image( 64, 64, rng )
+    dens = Image( img, origin = [ 0, 0 ], frame = np.eye( 2 ) / 64 ).normalized_version()
+    say( "case continuation 5k" )
+    sol, ro = _solve_both( oracle, pos, dens, _img( dens.values ), 1e-11, 1e-12, step = "limits", continuation = "always", maxit = 200 )
+    assert sol.converged and ro[ "stats" ][ "status" ] == 1
+    for key in ( "nb_continuation_steps", "nb_iter", "nb_diag" ):
+        assert sol.stats[ key ] == ro[ "stats" ][ key ], key
+    hist = np.array( [ [ row[ "step" ], row[ "s" ] ] for row in sol.history ] )
+    assert np.array_equal( hist, ro[ "history" ][ :, [ 0, 7 ] ] )
+    w = sol.weights.cpu().numpy()
+    assert np.abs( w - ro[ "weights" ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max()
+    m = sol.cell_masses.cpu().numpy()
+    assert np.abs( m - oracle.measures( pos, w, density = _img( dens.values ) ) ).max() < 1e-12 * m.max()
+
+
+_GAUSS = np.array( [ [ 0.3, 0.35, 0.06, 1.0 ], [ 0.7, 0.6, 0.1, 1.5 ], [ 0.45, 0.8, 0.2, 0.5 ] ] )
+
+
+def _sog( rows = _GAUSS ):
+    from sdot import SumOfGaussians
+    return SumOfGaussians( rows[ :, :2 ], rows[ :, 2 ], rows[ :, 3 ], support_box = ( [ 0, 0 ], [ 1, 1 ] ) ).normalized_version()
+
+
+def test_gaussian_masses_and_facets_match_the_oracle( oracle, native ):
+    # the corners of each edge ( polygon sums, device ) against the oracle's fan triangles, the erf facets
+    rng = np.random.default_rng( 40 )
+    dens = _sog()
+    rows = np.concatenate( [ dens.positions, dens.sigmas[ :, None ], dens.weights[ :, None ] ], axis = 1 )
+    for n, spread in ( ( 3000, 1e-4 ), ( 50000, 0.0 ) ):
+        pos = rng.uniform( 0, 1, ( n, 2 ) )
+        w = rng.normal( 0, spread, n ) if spread else np.zeros( n )
+        say( "case gaussians", n )
+        m, ( fi, fj, fc ) = _gpu_measures( native, pos, w, dens, facets = True )
+        mo, ( oi, oj, oc ) = oracle.measures( pos, w, density = dict( gaussians = rows ), facets = True )
+        assert np.abs( m - mo ).max() <= 1e-12 * mo.max()
+        a = { ( i, j ): c for i, j, c in zip( fi.tolist(), fj.tolist(), fc.tolist() ) }
+        b = { ( i, j ): c for i, j, c in zip( oi.tolist(), oj.tolist(), oc.tolist() ) }
+        assert a.keys() == b.keys()
+        ref = max( b.values() )
+        assert max( abs( a[ k ] - b[ k ] ) for k in b ) <= 1e-12 * ref
+
+
+def test_gaussian_solve_matches_the_oracle( oracle, native ):
+    rng = np.random.default_rng( 41 )
+    n = 4000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    dens = _sog()
+    rows = dict( gaussians = np.concatenate( [ dens.positions, dens.sigmas[ :, None ], dens.weights[ :, None ] ], axis = 1 ) )
+    for continuation in ( "never", "always" ):
+        say( "case gaussian solve", continuation )
+        sol, ro = _solve_both( oracle, pos, dens, rows, 1e-11, 1e-12, step = "limits", continuation = continuation, maxit = 200 )
+        assert sol.converged and ro[ "stats" ][ "status" ] == 1
+        for key in ( "nb_continuation_steps", "nb_iter" ):
+            assert sol.stats[ key ] == ro[ "stats" ][ key ], ( continuation, key )
+        w = sol.weights.cpu().numpy()
+        assert np.abs( w - ro[ "weights" ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max()
+        assert np.abs( sol.barycenters.cpu().numpy() - ro[ "barycenters" ] ).max() < 1e-10
+        assert abs( sol.cost - ro[ "cost" ] ) < 1e-10 * ro[ "cost" ]
 
 
 def test_solve_is_bitwise_reproducible( native ):

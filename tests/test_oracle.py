@@ -332,3 +332,225 @@ def test_residual_variants_reach_the_same_plan( oracle ):
     b = oracle.solve( pos, tol = 1e-12, residual = 0 )
     assert a[ "stats" ][ "status" ] == 1 and b[ "stats" ][ "status" ] == 1
     assert np.abs( a[ "weights" ] - b[ "weights" ] ).max() < 1e-9 * np.abs( a[ "weights" ] ).max()
+
+
+# ---- the step by the limits ( reference gpu/Newton2D.cuh:505-770, sdotplan/Bounds.h:142-230 ) --------------------
+
+
+def _weighted_cloud( seed, n = 400, spread = 1e-4 ):
+    rng = np.random.default_rng( seed )
+    return rng.uniform( 0, 1, ( n, 2 ) ), rng.normal( 0, spread, n ), rng.normal( 0, spread, n )
+
+
+def test_area_polynomial_is_the_area_and_the_laplacian( oracle ):
+    # a0 is the cell's area; a1 = d area_i / dt = sum_j c_ij ( d_i - d_j ) with the facets of the Hessian test
+    # ( a wrong sign or a swapped operand in the moving offsets breaks it ); while the edges hold, the polynomial
+    # is the exact area at w + t d
+    pos, w, d = _weighted_cloud( 20 )
+    r = oracle.along( pos, w, d )
+    a, ( fi, fj, fc ) = oracle.measures( pos, w, facets = True )
+    assert np.array_equal( r[ "ok" ], a > 0 ) and r[ "ok" ].sum() > 390   # ( an empty cell has no polynomial: area 0 )
+    assert np.abs( r[ "poly" ][ :, 0 ] - a ).max() <= 1e-14 * a.max()
+    a1 = np.zeros( len( pos ) )
+    np.add.at( a1, fi, fc * ( d[ fi ] - d[ fj ] ) )
+    assert np.abs( r[ "poly" ][ :, 1 ] - a1 ).max() <= 1e-12 * np.abs( a1 ).max()
+    t = 1e-3
+    at = oracle.measures( pos, w + t * d )
+    pred = r[ "poly" ] @ np.array( [ 1, t, t * t ] )
+    assert np.abs( pred - at ).max() <= 1e-13 * a.max()
+    # the frozen cell's mass is the polynomial, whatever t ( constant density )
+    ok = r[ "ok" ]
+    fm = oracle.along( pos, w, d, t = 0.5 )[ "frozen_mass" ]
+    assert np.abs( fm[ ok ] - r[ "poly" ][ ok ] @ np.array( [ 1, 0.5, 0.25 ] ) ).max() <= 1e-14 * a.max()
+
+
+def test_frozen_image_mass_is_the_clipped_mass( oracle ):
+    # the Green walk of a frozen cell against the pixel-clipping quadrature of the measures
+    pos, w, d = _weighted_cloud( 21 )
+    rng = np.random.default_rng( 22 )
+    dens = _img( rng.uniform( 0.5, 2, ( 48, 64 ) ) )
+    for t in ( 0.0, 1e-3 ):
+        fm = oracle.along( pos, w, d, t = t, density = dens )[ "frozen_mass" ]
+        ref = oracle.measures( pos, w + t * d, density = dens )
+        assert np.abs( fm - ref ).max() <= 1e-13 * ref.max(), t
+
+
+def test_forward_limit_is_the_first_root( oracle ):
+    pos, w, d = _weighted_cloud( 23, spread = 2e-5 )
+    a = oracle.measures( pos, w )
+    eps = 0.9 * a.min()
+    assert eps > 0
+    r = oracle.along( pos, w, 30 * d, eps = eps )
+    f = r[ "forward" ]
+    hit = f < 1e299
+    assert hit.any() and np.all( f > 0 )
+    p = r[ "poly" ]
+    # the mass at the limit is the floor, and above it before
+    at = p[ hit, 0 ] + f[ hit ] * p[ hit, 1 ] + f[ hit ] ** 2 * p[ hit, 2 ]
+    assert np.abs( at - eps ).max() <= 1e-12 * a.max()
+    half = p[ hit, 0 ] + 0.5 * f[ hit ] * p[ hit, 1 ] + 0.25 * f[ hit ] ** 2 * p[ hit, 2 ]
+    assert np.all( half > eps )
+    # an image density bisects the frozen mass on ( 0, 1 ]: within 1e-2 under the same limit on a constant image of value 1
+    ri = oracle.along( pos, w, 30 * d, eps = eps, density = _img( np.ones( ( 16, 16 ) ) ) )[ "forward" ]
+    in1 = hit & ( f < 1 )
+    assert in1.sum() > 10 and np.all( ri[ in1 ] <= f[ in1 ] ) and np.all( ri[ in1 ] >= 0.98 * f[ in1 ] )
+
+
+def test_the_limits_step_solves_with_fewer_diagrams( oracle ):
+    # the same plan as the trials, with fewer diagrams ( reference test_SdotPlanNd.py:917-920 compares the steps )
+    rng = np.random.default_rng( 24 )
+    pos = np.clip( np.concatenate( [ rng.normal( c, 0.01, ( 250, 2 ) ) for c in rng.uniform( 0.2, 0.8, ( 8, 2 ) ) ] ), 1e-3, 1 - 1e-3 )
+    a = oracle.solve( pos, tol = 1e-10, step = "trials", lin_tol = 1e-11 )
+    b = oracle.solve( pos, tol = 1e-10, step = "limits", lin_tol = 1e-11 )
+    assert a[ "stats" ][ "status" ] == 1 and b[ "stats" ][ "status" ] == 1
+    assert b[ "stats" ][ "nb_diag" ] < a[ "stats" ][ "nb_diag" ]
+    assert np.abs( a[ "weights" ] - b[ "weights" ] ).max() < 1e-8 * np.abs( a[ "weights" ] ).max()
+    # a bolder factor makes trials fall under the floor: the backward rounds then correct them
+    c = oracle.solve( pos, tol = 1e-10, step = "limits", factor = 1.5, lin_tol = 1e-11 )
+    assert c[ "stats" ][ "status" ] == 1 and c[ "stats" ][ "nb_limit_rounds" ] > 0 and c[ "stats" ][ "nb_cell_lim" ] > 0
+
+
+# ---- the width continuation ( reference sdotplan/Continuation.h, sdotplan/convolved/Image.h ) ----------------------
+
+
+def test_blur_keeps_constants_and_spreads_a_pixel( oracle ):
+    ones = np.full( ( 40, 50 ), 3.0 )
+    assert np.abs( oracle.blur( ones, 0.05, 1 / 50, 1 / 40 ) - ones ).max() <= 4e-15 * 3
+    # one bright pixel far from the borders: the outer product of the normalized 1D Gaussians of s / h pixels
+    img = np.zeros( ( 61, 61 ) )
+    img[ 30, 30 ] = 1.0
+    s, h = 0.02, 1 / 61
+    b = oracle.blur( img, s, h, h )
+    sp = s / h
+    r = int( np.ceil( 4 * sp ) )
+    k = np.exp( -0.5 * np.arange( -r, r + 1 ) ** 2 / sp ** 2 )
+    k /= k.sum()
+    ref = np.zeros_like( img )
+    ref[ 30 - r:30 + r + 1, 30 - r:30 + r + 1 ] = np.outer( k, k )
+    assert np.abs( b - ref ).max() <= 1e-15
+    assert abs( b.sum() - 1 ) < 1e-14
+    # no blur under a thousandth of a pixel
+    assert np.array_equal( oracle.blur( img, 1e-4 * h, h, h ), img )
+
+
+def test_continuation_ends_on_the_true_density( oracle ):
+    # the stages s0, s0 / r, ... >= s_min, then 0 ( Continuation.h:31-39 ); the last stage is the density itself: the
+    # returned masses are its measures at the returned weights, within the tolerance of the targets
+    rng = np.random.default_rng( 25 )
+    n = 1500
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    img = H.bumps_image( 32, 32, rng )
+    dens = _img( np.ascontiguousarray( img.T ) / img.mean() )
+    r = oracle.solve( pos, tol = 1e-11, density = dens, step = "limits", continuation = "always", lin_tol = 1e-12 )
+    st = r[ "stats" ]
+    s0, smin = 0.5 * np.sqrt( 2 ), ( 1 / 32 ) / 4
+    assert st[ "nb_continuation_steps" ] == int( np.floor( np.log( s0 / smin ) / np.log( np.sqrt( 2 ) ) + 1e-9 ) ) + 2
+    assert st[ "status" ] == 1
+    m = oracle.measures( pos, r[ "weights" ], density = dens )
+    assert np.abs( m - r[ "masses" ] ).max() <= 1e-15
+    assert np.abs( m - m.sum() / n ).max() <= 1e-11
+    hs = r[ "history" ][ :, 7 ]
+    assert hs[ 0 ] == s0 and hs[ -1 ] == 0 and np.all( np.diff( hs ) <= 0 )
+    # "never" reaches the same plan directly
+    q = oracle.solve( pos, tol = 1e-11, density = dens, step = "limits", continuation = "never", lin_tol = 1e-12 )
+    assert q[ "stats" ][ "status" ] == 1
+    assert np.abs( q[ "weights" ] - r[ "weights" ] ).max() < 1e-7 * np.abs( q[ "weights" ] ).max()
+
+
+# ---- a sum of Gaussians ( reference tests/test_SumOfGaussians.py:20-73 ) ------------------------------------------------
+
+
+def _gauss( rows ):
+    return dict( gaussians = np.asarray( rows, dtype = float ) )
+
+
+def _rho_gauss( g ):
+    def rho( x, y ):
+        return sum( w * np.exp( -( ( x - cx ) ** 2 + ( y - cy ) ** 2 ) / ( 2 * s * s ) ) / ( 2 * np.pi * s * s ) for cx, cy, s, w in g )
+    return rho
+
+
+def test_gaussian_mass_of_a_box_is_a_product_of_erfs( oracle ):
+    from scipy.special import erf
+    g = np.array( [ [ 0.2, 0.7, 0.05, 2.0 ], [ 0.8, 0.3, 0.2, 0.5 ], [ 0.5, 0.5, 0.1, 1.0 ] ] )
+    box = [ sum( w / 4 * ( erf( ( 1 - cx ) / ( np.sqrt( 2 ) * s ) ) + erf( cx / ( np.sqrt( 2 ) * s ) ) )
+                 * ( erf( ( 1 - cy ) / ( np.sqrt( 2 ) * s ) ) + erf( cy / ( np.sqrt( 2 ) * s ) ) ) for cx, cy, s, w in g ) ]
+    one = oracle.measures( np.array( [ [ 0.3, 0.6 ] ] ), density = _gauss( g ) )
+    assert abs( one[ 0 ] - box[ 0 ] ) <= 1e-14 * box[ 0 ]
+    # any diagram shares it out exactly
+    rng = np.random.default_rng( 30 )
+    pos = rng.uniform( 0, 1, ( 300, 2 ) )
+    m = oracle.measures( pos, rng.normal( 0, 1e-4, 300 ), density = _gauss( g ) )
+    assert np.all( m >= 0 ) and abs( m.sum() - box[ 0 ] ) <= 1e-13 * box[ 0 ]
+
+
+def test_gaussian_cells_and_facets_against_quadrature( oracle ):
+    from scipy import integrate
+    g = np.array( [ [ 0.2, 0.7, 0.05, 2.0 ], [ 0.8, 0.3, 0.2, 0.5 ] ] )
+    rho = _rho_gauss( g )
+    rng = np.random.default_rng( 31 )
+    pos = rng.uniform( 0, 1, ( 40, 2 ) )
+    w = rng.normal( 0, 1e-3, 40 )
+    a, ( fi, fj, fc ) = oracle.measures( pos, w, density = _gauss( g ), facets = True )
+    for i in ( 0, 7, 19 ):
+        xy, ids = oracle.cell( pos, i, w )
+        ref = 0.0
+        for v in range( 1, len( xy ) - 1 ):
+            A, B, C = xy[ 0 ], xy[ v ], xy[ v + 1 ]
+            jac = abs( ( B - A )[ 0 ] * ( C - A )[ 1 ] - ( B - A )[ 1 ] * ( C - A )[ 0 ] )
+            ref += integrate.dblquad( lambda t, u: rho( *( A + u * ( B - A ) + t * ( C - A ) ) ) * jac, 0, 1, 0, lambda u: 1 - u,
+                                      epsabs = 1e-15, epsrel = 1e-12 )[ 0 ]
+        assert abs( a[ i ] - ref ) <= 1e-11 * ref, i
+        # the facets of the cell: int rho ds / ( 2 |p_i - p_j| )
+        for q in np.where( fi == i )[ 0 ]:
+            j = fj[ q ]
+            e = list( ids ).index( j )
+            P, Q = xy[ e ], xy[ ( e + 1 ) % len( xy ) ]
+            L = np.linalg.norm( Q - P )
+            line = integrate.quad( lambda t: rho( *( P + t * ( Q - P ) ) ) * L, 0, 1, epsabs = 1e-16, epsrel = 1e-13 )[ 0 ]
+            assert abs( fc[ q ] - line / ( 2 * np.linalg.norm( pos[ j ] - pos[ i ] ) ) ) <= 1e-12 * fc[ q ] + 1e-300
+
+
+def test_gaussian_moments_against_quadrature( oracle ):
+    # barycentres and cost from the boundary fluxes against a direct quadrature of the converged cells
+    from scipy import integrate
+    g = np.array( [ [ 0.35, 0.6, 0.15, 1.0 ], [ 0.7, 0.4, 0.2, 1.0 ] ] )
+    rho = _rho_gauss( g )
+    rng = np.random.default_rng( 32 )
+    pos = rng.uniform( 0, 1, ( 60, 2 ) )
+    r = oracle.solve( pos, density = _gauss( g ), tol = 1e-12, step = "limits", continuation = "auto", lin_tol = 1e-13 )
+    assert r[ "stats" ][ "status" ] == 1
+    cost = 0.0
+    for i in range( len( pos ) ):
+        xy, _ = oracle.cell( pos, i, r[ "weights" ] )
+        m = mx = my = c2 = 0.0
+        for v in range( 1, len( xy ) - 1 ):
+            A, B, C = xy[ 0 ], xy[ v ], xy[ v + 1 ]
+            jac = abs( ( B - A )[ 0 ] * ( C - A )[ 1 ] - ( B - A )[ 1 ] * ( C - A )[ 0 ] )
+            def integral( f ):
+                return integrate.dblquad( lambda t, u: f( *( A + u * ( B - A ) + t * ( C - A ) ) ) * jac, 0, 1, 0, lambda u: 1 - u,
+                                          epsabs = 1e-15, epsrel = 1e-11 )[ 0 ]
+            m += integral( rho )
+            mx += integral( lambda x, y: x * rho( x, y ) )
+            my += integral( lambda x, y: y * rho( x, y ) )
+            c2 += integral( lambda x, y: ( ( x - pos[ i, 0 ] ) ** 2 + ( y - pos[ i, 1 ] ) ** 2 ) * rho( x, y ) )
+        cost += c2
+        if i % 7 == 0:
+            assert np.abs( r[ "barycenters" ][ i ] - [ mx / m, my / m ] ).max() <= 1e-10, i
+    assert abs( r[ "cost" ] - cost ) <= 1e-10 * cost
+
+
+def test_gaussian_continuation_widens_the_gaussians( oracle ):
+    # the stages of a sum of Gaussians stop at a quarter of the narrowest ( convolved/SumOfGaussians.h:39 ), and the plan is
+    # the one the direct Newton reaches
+    g = np.array( [ [ 0.3, 0.3, 0.06, 1.0 ], [ 0.7, 0.65, 0.1, 1.5 ] ] )
+    rng = np.random.default_rng( 33 )
+    pos = rng.uniform( 0, 1, ( 800, 2 ) )
+    r = oracle.solve( pos, density = _gauss( g ), tol = 1e-11, step = "limits", continuation = "always", lin_tol = 1e-12 )
+    st = r[ "stats" ]
+    s0, smin = 0.5 * np.sqrt( 2 ), 0.06 / 4
+    assert st[ "status" ] == 1
+    assert st[ "nb_continuation_steps" ] == int( np.floor( np.log( s0 / smin ) / np.log( np.sqrt( 2 ) ) + 1e-9 ) ) + 2
+    q = oracle.solve( pos, density = _gauss( g ), tol = 1e-11, step = "limits", continuation = "never", lin_tol = 1e-12, maxit = 200 )
+    assert q[ "stats" ][ "status" ] == 1
+    assert np.abs( q[ "weights" ] - r[ "weights" ] ).max() < 1e-7 * np.abs( q[ "weights" ] ).max()
