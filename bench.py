@@ -115,13 +115,14 @@ def main():
     cells_avg_ms = cells_ms / max( cells_launches, 1 )
     cells_tf = ( flops / max( cells_launches, 1 ) ) / ( cells_avg_ms * 1e-3 ) / 1e12 if cells_launches else 0.0
     roof_cells = dict( kernel = "k_cells_pass1", bound = "fp64-valu", achieved = cells_tf, peak = FP64_PEAK_TFLOPS, unit = "TFLOP/s",
-                       frac = cells_tf / FP64_PEAK_TFLOPS, traffic = None, avg_launch_ms = cells_avg_ms, launches = cells_launches,
+                       frac = cells_tf / FP64_PEAK_TFLOPS, traffic = measured_traffic( "k_cells_pass1", args ), avg_launch_ms = cells_avg_ms, launches = cells_launches,
                        flops_per_launch = flops / max( cells_launches, 1 ) )
     spmv_ms, spmv_launches = times[ "spmv" ]
     spmv_avg_ms = spmv_ms / max( spmv_launches, 1 )
     spmv_gbs = ( spmv_bytes / max( spmv_launches, 1 ) ) / ( spmv_avg_ms * 1e-3 ) / 1e9 if spmv_launches else 0.0
     roof_spmv = dict( kernel = "k_cg_spmv_dot" if args.linear == "cg" else "k_pcg_spmv", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s", frac = spmv_gbs / HBM_PEAK_GBS,
-                      traffic = None, avg_launch_ms = spmv_avg_ms, launches_timed = spmv_launches,
+                      traffic = measured_traffic( "k_pcg_spmv" if args.linear != "cg" else "k_cg_spmv_dot", args ), avg_launch_ms = spmv_avg_ms,
+                      launches_timed = spmv_launches,
                       bytes_per_launch = spmv_bytes / max( spmv_launches, 1 ) )
     # the SpMV is timed one launch in sixteen: its share of the step is estimated from the CG iteration count
     spmv_total_ms = spmv_avg_ms * st[ "lin_nb_iter" ] * args.steps
@@ -143,6 +144,19 @@ def main():
     if not args.no_cpu_baseline:
         line[ "cpu_baseline" ] = cpu_baseline( pos_h, img_h, args )
     print( json.dumps( line ), flush = True )
+
+
+def measured_traffic( kernel, args ):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary of this workload ( FETCH_SIZE and
+    WRITE_SIZE, scaled by a 512 MiB copy: profiles/hbm_traffic_bench_1M_image.md ), for the default workload only"""
+    if ( args.n, args.res, args.step, args.continuation ) != ( 1_000_000, 1024, "limits", "never" ):
+        return None
+    try:
+        with open( os.path.join( ROOT, "profiles", "hbm_traffic_bench_1M_image.json" ) ) as f:
+            k = json.load( f )[ "kernels" ].get( kernel )
+    except OSError:
+        return None
+    return None if k is None else k[ "bytes" ]
 
 
 def cpu_baseline( pos_h, img_h, args ):
