@@ -82,7 +82,10 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
  *   positions    DEVICE [ n ][ 2 ]
  *   nu           DEVICE [ n ]       target masses ( rescaled to the mass of the domain )
  *   w0           DEVICE [ n ]       starting weights ( null: zero, the Voronoi diagram )
- *   dups         DEVICE [ 2 k ]     exact duplicates ( dup, rep ); accepted, not used by this version
+ *   dups         DEVICE [ 2 k ]     exact duplicates, ( dup, rep ) user-id pairs, 0 <= rep < dup, each dup once and
+ *                                   no rep a dup; the first `agg_nb_dups` pairs are read ( `nb_dup_values` >= 2 x that ).
+ *                                   Each duplicate's weight is tied `j agg_gap` under its representative's ( its j-th ),
+ *                                   its cell left empty, its target carried by the representative; may be null
  *   options      HOST   [ 41 ]      the reference card's option vector, same order
  *                                   ( src/sdot/SdotPlanNd.py:197-201 ); the image geometry is in
  *                                   img_x0 .. img_ny as there, the Gaussians' count in img_nx
@@ -95,7 +98,7 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
  *   cell_masses  DEVICE [ n ]       output
  *   barycenters  DEVICE [ n ][ 2 ]  output ( the seed for an empty cell )
  *   cost         HOST   [ 1 ]       output, sum_i int_{cell_i} |x - p_i|^2 rho
- *   clusters     DEVICE [ n ] int64 output ( each seed its own representative; may be null )
+ *   clusters     DEVICE [ n ] int64 output ( each seed its own index, a duplicate its representative's; may be null )
  */
 int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const double *nu, const double *w0,
                        const long long *dups, long long nb_dup_values, const double *options, const double *box,

@@ -1275,6 +1275,44 @@ struct Solver {
     std::vector<double> nu, w, a, d;
     std::vector<Facet> fa;
     Edges ea;                                            ///< the accepted diagram's edges
+    // THE EXACT DUPLICATES ( user indices, `sdotplan/Aggregation.h:107-220`, `gpu/Newton2D.cuh:340-366` ): the k-th duplicate
+    // of a representative has `w_dup = w_rep - k gap` ( its cell empty ), its target carried by the representative ( `nue` );
+    // the tests read each member's share of its cluster's mass, `nu_m a_rep / nue_rep`
+    std::vector<I64> dup, rep, dup_rep;                  ///< the pairs; per seed its representative, -1 if not a duplicate
+    std::vector<double> dup_gap, nue;
+    std::vector<char> member;                            ///< per seed: in a cluster ( a representative or a duplicate )
+    double c_link = 1;                                   ///< the coefficient of the facet linking a duplicate to its representative
+    void set_duplicates( const std::vector<I64> &d, const std::vector<I64> &r, double gap, I64 n ) {
+        dup = d; rep = r;
+        dup_gap.clear();
+        if ( d.empty() ) return;
+        dup_rep.assign( sz( n ), -1 );
+        member.assign( sz( n ), 0 );
+        std::vector<int> count( sz( n ), 0 );
+        for ( size_t q = 0; q < d.size(); ++q ) {
+            dup_rep[ sz( d[ q ] ) ] = r[ q ];
+            dup_gap.push_back( gap * double( ++count[ sz( r[ q ] ) ] ) );
+            member[ sz( d[ q ] ) ] = member[ sz( r[ q ] ) ] = 1;
+        }
+    }
+    bool is_dup( size_t i ) const { return ! dup_rep.empty() && dup_rep[ i ] >= 0; }
+    /// `nue` for the current `nu` ( `Aggregation::prepare` )
+    void prepare() {
+        if ( dup.empty() ) { nue.clear(); return; }
+        nue = nu;
+        for ( size_t q = 0; q < dup.size(); ++q ) { nue[ sz( rep[ q ] ) ] += nu[ sz( dup[ q ] ) ]; nue[ sz( dup[ q ] ) ] = 0; }
+    }
+    /// the masses the tests read: a member's share of its cluster ( `Aggregation::shares` )
+    const std::vector<double> &shares( const std::vector<double> &A, std::vector<double> &ae ) const {
+        if ( dup.empty() ) return A;
+        ae = A;
+        for ( size_t i = 0; i < A.size(); ++i ) {
+            if ( ! member[ i ] ) continue;
+            const size_t r = is_dup( i ) ? sz( dup_rep[ i ] ) : i;
+            ae[ i ] = nu[ i ] * ( nue[ r ] > 0 ? A[ r ] / nue[ r ] : 0.0 );
+        }
+        return ae;
+    }
     int res_cur = LIN;
     double t_last = 1;
     // stats
@@ -1286,22 +1324,24 @@ struct Solver {
 
     Solver( Sweep &sw, const Options &o ) : sw( sw ), o( o ) {}
 
+    /// `Newton.h:176-203`: on the targets `nue` when there are duplicates ( a duplicate's `b` is 0 )
     void rhs( const std::vector<double> &A, std::vector<double> &b ) const {
+        const std::vector<double> &N = dup.empty() ? nu : nue;
         const size_t n = A.size();
         b.assign( n, 0.0 );
-        if ( res_cur == LIN ) { for ( size_t i = 0; i < n; ++i ) b[ i ] = nu[ i ] > 0 ? nu[ i ] - A[ i ] : 0.0; return; }
+        if ( res_cur == LIN ) { for ( size_t i = 0; i < n; ++i ) b[ i ] = N[ i ] > 0 ? N[ i ] - A[ i ] : 0.0; return; }
         double su = 0, sug = 0;
         for ( size_t i = 0; i < n; ++i ) {
-            if ( ! ( nu[ i ] > 0 ) ) continue;
-            const double x = A[ i ] / nu[ i ], u = nu[ i ] / gp_of( x, res_cur, o.power );
+            if ( ! ( N[ i ] > 0 ) ) continue;
+            const double x = A[ i ] / N[ i ], u = N[ i ] / gp_of( x, res_cur, o.power );
             su += u;
             sug += u * g_of( x, res_cur, o.power );
         }
         const double c = sug / su;
         for ( size_t i = 0; i < n; ++i ) {
-            if ( ! ( nu[ i ] > 0 ) ) continue;
-            const double x = A[ i ] / nu[ i ];
-            b[ i ] = nu[ i ] / gp_of( x, res_cur, o.power ) * ( c - g_of( x, res_cur, o.power ) );
+            if ( ! ( N[ i ] > 0 ) ) continue;
+            const double x = A[ i ] / N[ i ];
+            b[ i ] = N[ i ] / gp_of( x, res_cur, o.power ) * ( c - g_of( x, res_cur, o.power ) );
         }
     }
     static void project_on_range( std::vector<double> &b ) {
@@ -1324,7 +1364,36 @@ struct Solver {
         for ( size_t i = 0; i < n; ++i ) { const double e = g_of( A[ i ] / nu[ i ], res_cur, o.power ) - m; s += e * e; }
         return std::sqrt( s );
     }
+    /// the smallest mass the floor reads, the duplicates ( empty by construction ) left out ( `Aggregation::floor_min` )
+    double floor_min( const std::vector<double> &A ) const {
+        double m = 1e300;
+        for ( size_t i = 0; i < A.size(); ++i ) if ( ! is_dup( i ) ) m = std::min( m, A[ i ] );
+        return A.empty() ? 0 : m;
+    }
+    /// `max |a - nue|` and its relative form over the seeds that are not duplicates ( `Aggregation::full_residual` )
+    void full_residual( const std::vector<double> &A, double &worst, double &worst_rel ) const {
+        const std::vector<double> &N = dup.empty() ? nu : nue;
+        worst = worst_rel = 0;
+        for ( size_t i = 0; i < A.size(); ++i ) {
+            if ( is_dup( i ) ) continue;
+            const double e = std::fabs( A[ i ] - N[ i ] );
+            worst = std::max( worst, e );
+            worst_rel = std::max( worst_rel, e / N[ i ] );
+        }
+    }
+    void tie( std::vector<double> &W ) const {
+        for ( size_t q = 0; q < dup.size(); ++q ) W[ sz( dup[ q ] ) ] = W[ sz( rep[ q ] ) ] - dup_gap[ q ];
+    }
+    /// the duplicates' rows linked to their representatives by a facet of coefficient `c_link` ( `Aggregation::link_duplicates` )
+    void link( const std::vector<Facet> &f, std::vector<Facet> &out ) const {
+        out = f;
+        for ( size_t q = 0; q < dup.size(); ++q ) {
+            out.push_back( Facet{ dup[ q ], rep[ q ], c_link } );
+            out.push_back( Facet{ rep[ q ], dup[ q ], c_link } );
+        }
+    }
     void measures_and_facets( std::vector<double> &W, std::vector<double> &res, std::vector<Facet> &f, Edges *E = nullptr ) {
+        tie( W );
         sw.set_weights( W );
         sw.edges = E;
         sw.measures( res, &f );
@@ -1340,33 +1409,39 @@ struct Solver {
         Edges e2;
         const bool limits = o.step == LIMITS;
         Laplacian L;
+        std::vector<double> ae, ae2;
+        std::vector<Facet> fa_lin;
+        prepare();
         w = w_init;
         const double gauge = w[ 0 ];
         for ( auto &v : w ) v -= gauge;
+        tie( w );
         t_last = 1;
         if ( after_step ) after_step( 0, 0, 1 );
         res_cur = o.residual;
         for ( int it = 0; it < o.maxit; ++it ) {
+            const std::vector<double> &A = shares( a, ae );
             double worst = 0, worst_rel = 0;
             for ( I64 i = 0; i < n; ++i ) {
-                worst = std::max( worst, std::fabs( nu[ size_t( i ) ] - a[ size_t( i ) ] ) );
-                worst_rel = std::max( worst_rel, std::fabs( nu[ size_t( i ) ] - a[ size_t( i ) ] ) / nu[ size_t( i ) ] );
+                worst = std::max( worst, std::fabs( nu[ size_t( i ) ] - A[ size_t( i ) ] ) );
+                worst_rel = std::max( worst_rel, std::fabs( nu[ size_t( i ) ] - A[ size_t( i ) ] ) / nu[ size_t( i ) ] );
             }
             if ( o.switch_residual > 0 && res_cur != LIN && worst_rel <= o.switch_residual ) { res_cur = LIN; it_switch = it; }
             rhs( a, b );
             project_on_range( b );
             if ( it == 0 ) {
-                double am = a[ 0 ], nm = nu[ 0 ];
-                for ( I64 i = 0; i < n; ++i ) { am = std::min( am, a[ size_t( i ) ] ); nm = std::min( nm, nu[ size_t( i ) ] ); }
-                eps = 0.5 * std::min( nm, am );
+                double nm = nu[ 0 ];
+                for ( I64 i = 0; i < n; ++i ) nm = std::min( nm, nu[ size_t( i ) ] );
+                eps = 0.5 * std::min( nm, floor_min( a ) );
                 residual0 = worst;
             }
-            const double nr = merit( a );
+            const double nr = merit( A );
             residual = worst;
             if ( worst <= o.tol_abs || ( o.tol_rel > 0 && worst_rel <= o.tol_rel ) ) { status = 1; return true; }
             ++nb_iter;
             double t0 = now();
-            L.assemble( n, fa );
+            if ( ! dup.empty() ) { link( fa, fa_lin ); L.assemble( n, fa_lin ); }
+            else L.assemble( n, fa );
             t_asm += now() - t0;
             if ( ! cg_solve( L, b, d, o.lin_tol, o.lin_maxit, lst ) ) { status = 4; return false; }
             double t = std::min( 1.0, o.mult_ok * t_last );
@@ -1385,7 +1460,7 @@ struct Solver {
                 double th = now();
                 const Along fw{ pos, w.data(), d.data(), sw.lo, sw.hi, &ea, n, &sw.dn };
                 double am = 1e300;
-                for ( I64 k = 0; k < n; ++k ) am = std::min( am, alpha_forward( fw, k, eps ) );
+                for ( I64 k = 0; k < n; ++k ) if ( ! is_dup( sz( k ) ) ) am = std::min( am, alpha_forward( fw, k, eps ) );
                 t_lim += now() - th;
                 t = am >= 1 ? 1.0 : o.factor * am;
                 if ( sw.dn.kind != 0 && ! ( t >= o.t_min ) ) t = std::min( 1.0, o.mult_ok * t_last );
@@ -1394,14 +1469,14 @@ struct Solver {
                     trial_diagram( t );
                     t_done = t;
                     I64 nb_below = 0;
-                    for ( double v : a2 ) nb_below += v < eps;
+                    for ( I64 k = 0; k < n; ++k ) nb_below += ! is_dup( sz( k ) ) && a2[ sz( k ) ] < eps;
                     if ( nb_below == 0 ) break;
                     ++nb_limit_rounds;
                     th = now();
                     const Along bw{ pos, w.data(), d.data(), sw.lo, sw.hi, &e2, n, &sw.dn };
                     double mb = 1e300;
                     for ( I64 k = 0; k < n; ++k )
-                        if ( a2[ size_t( k ) ] < eps ) { ++nb_cell_lim; mb = std::min( mb, alpha_backward( bw, k, eps, t ) ); }
+                        if ( ! is_dup( sz( k ) ) && a2[ size_t( k ) ] < eps ) { ++nb_cell_lim; mb = std::min( mb, alpha_backward( bw, k, eps, t ) ); }
                     t_lim += now() - th;
                     t = o.factor * std::min( t, mb );
                     if ( t < o.t_min ) break;
@@ -1411,9 +1486,8 @@ struct Solver {
             }
             for ( int trial = 0; trial < o.max_backtracks; ++trial ) {
                 if ( ! ( trial == 0 && already ) ) trial_diagram( t );
-                double m2 = a2[ 0 ];
-                for ( double v : a2 ) m2 = std::min( m2, v );
-                const double n2r = merit( a2 );
+                const double m2 = floor_min( a2 );
+                const double n2r = merit( shares( a2, ae2 ) );
                 if ( m2 >= eps && std::isfinite( n2r ) && n2r <= ( 1 - t / 2 ) * nr && n2r < nr ) { taken = true; break; }
                 const bool hopeless = hope.refused( t, n2r, nr, m2 >= eps && std::isfinite( n2r ) );
                 t /= 2;
@@ -1436,8 +1510,9 @@ struct Solver {
             if ( limits ) std::swap( ea, e2 );
             if ( after_step ) after_step( it + 1, t, nb_evals );
         }
+        const std::vector<double> &A = shares( a, ae );
         double worst = 0;
-        for ( I64 i = 0; i < n; ++i ) worst = std::max( worst, std::fabs( nu[ size_t( i ) ] - a[ size_t( i ) ] ) );
+        for ( I64 i = 0; i < n; ++i ) worst = std::max( worst, std::fabs( nu[ size_t( i ) ] - A[ size_t( i ) ] ) );
         residual = worst;
         status = 2;
         return false;
@@ -1566,6 +1641,27 @@ I64 oracle_measures( I64 n, const double *pos, const double *w, const double *lo
     return I64( fa.size() );
 }
 
+/// the event counts of each cell ( testing hook, RANK order: `ev[ 5 k .. ]` side tests, pruning vertex tests, new
+/// vertices, node visits, cuts of the cell of rank `k` ), and the user index of each rank in `order`
+void oracle_cell_events( I64 n, const double *pos, const double *w, const double *lo, const double *hi, I64 *ev, I64 *order ) {
+    orc::Tree t;
+    orc::build_tree( t, n, pos, 10 );
+    std::vector<double> W( sz( n ), 0.0 );
+    if ( w ) for ( I64 i = 0; i < n; ++i ) W[ size_t( i ) ] = w[ i ];
+    orc::set_weights( t, W.data() );
+    orc::parallel_blocks( n, 256, [&]( int, I64 b, I64 e ) {
+        orc::Poly c;
+        c.reserve( 64 );
+        for ( I64 k = b; k < e; ++k ) {
+            orc::Events E;
+            orc::make_cell( t, k, lo, hi, c, &E );
+            I64 *o = ev + 5 * k;
+            o[ 0 ] = E.side_tests; o[ 1 ] = E.prune_vertex_tests; o[ 2 ] = E.new_vertices; o[ 3 ] = E.node_visits; o[ 4 ] = E.cuts;
+        }
+    } );
+    std::memcpy( order, t.order.data(), sizeof( I64 ) * size_t( n ) );
+}
+
 /// the vertices of the cell of user seed `i` ( absolute coordinates, counter-clockwise ) and its
 /// cut ids ( user indices of the neighbours, -1 for the domain ); returns the vertex count
 int oracle_cell( I64 n, const double *pos, const double *w, const double *lo, const double *hi, I64 i, double *xy, I64 *ids,
@@ -1645,7 +1741,7 @@ int oracle_cg( I64 n, I64 nf, const I64 *fi, const I64 *fj, const double *fc, co
 /// `masses[ n ]`, `bary[ n * 2 ]`, `cost[ 1 ]`.
 int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w0, const double *lo, const double *hi, int kind,
                   double value, int nx, int ny, const double *geo, const double *img, const double *opts, const double *start_box,
-                  double *weights, double *stats, double *hist, double *masses, double *bary, double *cost ) {
+                  const I64 *dups, double *weights, double *stats, double *hist, double *masses, double *bary, double *cost ) {
     using namespace orc;
     const double t_begin = now();
     Options o;
@@ -1661,6 +1757,16 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     Sweep sw( t, lo, hi, dn0 );
     Solver nt( sw, o );
     nt.pos = pos;
+    // the exact duplicates: `opts[ 19 ]` the gap, `opts[ 20 ]` the count of `( dup, rep )` pairs in `dups` ( user ids )
+    const I64 nb_dups = dups ? I64( opts[ 20 ] ) : 0;
+    {
+        std::vector<I64> dd, rr;
+        for ( I64 q = 0; q < nb_dups; ++q ) { dd.push_back( dups[ 2 * q ] ); rr.push_back( dups[ 2 * q + 1 ] ); }
+        nt.set_duplicates( dd, rr, opts[ 19 ], n );
+        // the link's coefficient: the scale of a facet, the density or the mean density of the box ( `gpu/Newton2D.cuh:1273` )
+        const double vol = ( hi[ 0 ] - lo[ 0 ] ) * ( hi[ 1 ] - lo[ 1 ] );
+        nt.c_link = kind == 0 ? value : 1.0 / std::max( vol, 1e-300 );
+    }
     // the stages of the width continuation ( `Solve.h:176-200`; an image only: a constant is positive everywhere )
     const bool possible = kind == 1 || kind == 2;
     const double s0 = conv_s0 > 0 ? conv_s0 : 0.5 * std::sqrt( ( hi[ 0 ] - lo[ 0 ] ) * ( hi[ 0 ] - lo[ 0 ] ) + ( hi[ 1 ] - lo[ 1 ] ) * ( hi[ 1 ] - lo[ 1 ] ) );
@@ -1684,11 +1790,13 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     int nb_steps = 0, nb_steps_before = 0;
     nt.after_step = [&]( int it, double tt, int nb_evals ) {
         if ( nb_steps >= cap_steps ) return;
-        double mn = nt.a.empty() ? 0 : nt.a[ 0 ], mx = 0, l2 = 0;
+        std::vector<double> ae;
+        const std::vector<double> &A = nt.shares( nt.a, ae );
+        const double mn = nt.floor_min( nt.a );
+        double mx = 0, l2 = 0;
         for ( I64 i = 0; i < n; ++i ) {
-            mn = std::min( mn, nt.a[ size_t( i ) ] );
-            mx = std::max( mx, std::fabs( nt.a[ size_t( i ) ] - nt.nu[ size_t( i ) ] ) );
-            l2 += ( nt.a[ size_t( i ) ] - nt.nu[ size_t( i ) ] ) * ( nt.a[ size_t( i ) ] - nt.nu[ size_t( i ) ] );
+            mx = std::max( mx, std::fabs( A[ size_t( i ) ] - nt.nu[ size_t( i ) ] ) );
+            l2 += ( A[ size_t( i ) ] - nt.nu[ size_t( i ) ] ) * ( A[ size_t( i ) ] - nt.nu[ size_t( i ) ] );
         }
         double *r = hist + size_t( nb_steps ) * NB_HIST;
         r[ H_STEP ] = nb_steps_before + it; r[ H_T ] = tt; r[ H_RESIDUAL_L2 ] = std::sqrt( l2 ); r[ H_MIN_MASS ] = mn;
@@ -1707,7 +1815,7 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     nt.measures_and_facets( w, a, fa, &ea );
     double nu_min = nu[ 0 ];
     for ( double v : nu ) nu_min = std::min( nu_min, v );
-    auto min_mass = []( const std::vector<double> &m ) { double r = m.empty() ? 0 : m[ 0 ]; for ( double v : m ) r = std::min( r, v ); return r; };
+    auto min_mass = [&]( const std::vector<double> &m ) { return nt.floor_min( m ); };
     if ( given && ! o.keep_start && min_mass( a ) < 1e-3 * nu_min ) {
         std::vector<double> wz( sz( n ), 0.0 ), az;
         std::vector<Facet> fz;
@@ -1762,6 +1870,15 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
         if ( nt.status == 4 ) break;
     }
     for ( I64 i = 0; i < n; ++i ) { weights[ i ] = w[ size_t( i ) ]; masses[ i ] = a[ size_t( i ) ]; }
+    // the full problem's residual; with clusters, `converged` only when it passes the test too ( `gpu/Newton2D.cuh:1752-1760` )
+    double full = nt.residual;
+    int status = nt.status;
+    if ( nb_dups > 0 ) {
+        double wf, wfr;
+        nt.full_residual( a, wf, wfr );
+        full = wf;
+        if ( status == 1 && ! ( wf <= o.tol_abs || ( o.tol_rel > 0 && wfr <= o.tol_rel ) ) ) status = 7;
+    }
     const int nb_diag_descent = sw.nb_diag;
     {
         set_stage( 0 );                                  // the moments on the TRUE density
@@ -1773,7 +1890,7 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
         *cost = c;
     }
     for ( int q = 0; q < NB_STATS; ++q ) stats[ q ] = 0;
-    stats[ STATUS ] = nt.status;
+    stats[ STATUS ] = status;
     stats[ RESIDUAL ] = nt.residual;
     stats[ RESIDUAL0 ] = residual0;
     stats[ NB_ITER ] = nt.nb_iter;
@@ -1795,7 +1912,16 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     stats[ NB_CONTINUATION_STEPS ] = double( scales.size() );
     stats[ MIN_START_MASS ] = min_start_mass;
     stats[ IT_SWITCH ] = it_switch;
-    stats[ RESIDUAL_FULL ] = nt.residual;
+    stats[ RESIDUAL_FULL ] = full;
+    if ( nb_dups > 0 ) {                                 // each representative and its duplicates: one cluster
+        std::vector<char> is_rep( sz( n ), 0 );
+        for ( I64 r : nt.rep ) is_rep[ sz( r ) ] = 1;
+        double nc = 0;
+        for ( char c : is_rep ) nc += c;
+        stats[ NB_CLUSTERS ] = nc;
+        stats[ NB_AGGREGATED ] = nc + double( nb_dups );
+        stats[ NB_DUPLICATES ] = double( nb_dups );
+    }
     return nb_steps;
 }
 

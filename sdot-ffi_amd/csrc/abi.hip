@@ -4,6 +4,7 @@
 
 #include "../../include/sdot_amd.h"
 #include "solver.h"
+#include "multigrid.h"
 
 #include <cmath>
 
@@ -190,8 +191,6 @@ int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const
                        long long nb_dup_values, const double *options, const double *box, int density_kind, double rho, const double *img,
                        double *weights, double *history, long long *nb_steps, double *stats, double *cell_masses, double *barycenters,
                        double *cost, long long *clusters ) {
-    (void) dups;
-    (void) nb_dup_values;
     return guarded( [&] {
         SDOT_CHECK( n >= 2 && positions && nu && options && weights && cell_masses && barycenters, "solve: a required array is null" );
         check_box( box );
@@ -202,6 +201,11 @@ int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const
         in.nu = nu;
         in.w0 = w0;
         in.options = options;
+        // the exact duplicates: the first `agg_nb_dups` pairs of `dups` ( the reference passes one dummy pair when there is none )
+        in.nb_dups = dups ? std::min<long long>( (long long) options[ O_AGG_NB_DUPS ], nb_dup_values / 2 ) : 0;
+        SDOT_CHECK( in.nb_dups >= 0, "solve: a negative count of duplicates" );
+        SDOT_CHECK( in.nb_dups == 0 || options[ O_AGG_GAP ] > 0, "solve: duplicates need a positive gap ( option agg_gap )" );
+        in.dups = dups;
         for ( int c = 0; c < 4; ++c ) in.box[ c ] = box[ c ];
         const double geo[ 4 ] = { options[ O_IMG_X0 ], options[ O_IMG_Y0 ], options[ O_IMG_HX ], options[ O_IMG_HY ] };
         in.dens = make_density( density_kind, rho, img, int( options[ O_IMG_NX ] ), int( options[ O_IMG_NY ] ), geo );
@@ -301,10 +305,17 @@ int sdot_amd_linear_solves_2d( void *stream, long long n, const long long *row, 
         check_launch( "k_csr_from" );
         Csr L;
         L.n = n; L.row_off = row_off; L.row_cnt = row_cnt; L.col = col32; L.val = val; L.dia = dia;
+        // options ( `SdotPlanNd.py:204-222` ): n, k, method ( 0 cg, 1 mg ), tol, smoothed, float, recycle, rebuild, kcycle, nu
         const double tol = options && options[ 3 ] > 0 ? options[ 3 ] : 1e-6;
+        const bool mg = options && options[ 2 ] == 1;
+        MgOptions mo;
+        if ( options && options[ 8 ] >= 0 ) mo.kcycle = int( options[ 8 ] );
+        if ( options && options[ 9 ] > 0 ) mo.nu = int( options[ 9 ] );
         for ( long long j = 0; j < k; ++j ) {
             CgStats cs;
-            const int its = cg_solve( L, 0, rhs + j * n, solution + j * n, tol, 20000, st, cs );
+            int levels = 0;
+            int its = mg ? mg_solve( L, 0, rhs + j * n, solution + j * n, tol, 2000, mo, st, cs, levels ) : -2;
+            if ( its == -2 ) its = cg_solve( L, 0, rhs + j * n, solution + j * n, tol, 20000, st, cs );
             if ( iterations ) iterations[ j ] = its;
         }
         SDOT_HIP( hipStreamSynchronize( st ) );

@@ -18,8 +18,9 @@ struct Density {
 };
 
 /// what one diagram writes, in RANK order: the masses, the rows of the Laplacian ( each cell's
-/// own view of its facets: `col[ row_off[ k ] + q ]`, `val[ ... ]`, `q < row_cnt[ k ]`, and
-/// `dia[ k ]` the row sum ), and the diagnostics
+/// own view of its facets: `col[ row_off[ k ] + q * ROW_STEP ]`, `val[ ... ]`, `q < row_cnt[ k ]`, and
+/// `dia[ k ]` the row sum: the rows of a wave interleaved, entry `q` of its 64 rows in 64 consecutive words ),
+/// and the diagnostics
 struct DiagramOut {
     double   *mass = nullptr;
     int      *row_off = nullptr, *row_cnt = nullptr;
@@ -36,7 +37,13 @@ struct DiagramOut {
     // facets mode, when set: each cell's cuts in polygon order ( `edges[ q * n + k ]`: a seed rank, or `-1 - f` side `f`
     // of the box, 0 bottom, 1 right, 2 top, 3 left ) and their count ( 0 for an empty cell, -1 past `EDGE_CAP` or failed )
     int      *edges = nullptr, *nb_edges = nullptr;
+    // facets mode: slots left free at the end of every row ( a wave's rows are `64 x ( longest + row_extra )` ), for the
+    // links the solver appends after the cells ( the exact duplicates' rows )
+    int       row_extra = 0;
 };
+
+/// the Laplacian rows of a wave's 64 cells are interleaved: entry `q` of the row of lane `l` at `base + q * 64 + l`
+constexpr int ROW_STEP = 64;
 
 /// the vertex slots of the passes ( the first in LDS, the second in global slots )
 constexpr int CELL_BLOCK = 128;

@@ -341,6 +341,12 @@ __device__ void const_moments( const S &c, int nb, double rho, double &m, double
 
 // ---- wave helpers -------------------------------------------------------------------------------------
 
+/// the largest `v` of the wave
+__device__ __forceinline__ unsigned wave_max( unsigned v ) {
+    for ( int off = 32; off > 0; off >>= 1 ) { const unsigned o = __shfl_xor( v, off, 64 ); v = o > v ? o : v; }
+    return v;
+}
+
 /// exclusive prefix sum of `v` over the wave, and the wave's total
 __device__ __forceinline__ unsigned wave_scan( unsigned v, unsigned &total ) {
     const int lane = threadIdx.x & 63;
@@ -375,11 +381,12 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
     unsigned cnt = 0;
     if ( active && P.mode == MODE_FACETS )
         for ( int e = 0; e < nb; ++e ) cnt += c.I( e ) >= 0;
-    unsigned total = 0, base = 0, pre = 0;
+    // the wave's rows interleaved: `64 x` its longest row, one atomic for the wave
+    const int lane = threadIdx.x & 63;
+    unsigned wmax = 0, base = 0;
     if ( P.mode == MODE_FACETS ) {
-        pre = wave_scan( cnt, total );
-        const int lane = threadIdx.x & 63;
-        if ( lane == 0 && total ) base = atomicAdd( &P.out.counters[ 0 ], total );
+        wmax = wave_max( cnt ) + unsigned( P.out.row_extra );
+        if ( lane == 0 && wmax ) base = atomicAdd( &P.out.counters[ 0 ], unsigned( ROW_STEP ) * wmax );
         base = __shfl( base, 0, 64 );
     }
     if ( ! active ) return;
@@ -419,8 +426,8 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
             for ( int e = 0; e < ne; ++e ) P.out.edges[ size_t( e ) * size_t( P.n ) + size_t( k ) ] = c.I( e );
         P.out.nb_edges[ k ] = ne;
     }
-    const I64 at = I64( base ) + pre;
-    const bool fits = I64( base ) + total <= P.out.facet_cap;
+    const I64 at = I64( base ) + lane;
+    const bool fits = I64( base ) + I64( ROW_STEP ) * wmax <= P.out.facet_cap;
     P.out.row_off[ k ] = int( fits ? at : 0 );
     P.out.row_cnt[ k ] = int( cnt );
     double dia = 0, mass = 0;
@@ -448,8 +455,8 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
         const double d2 = dx * dx + dy * dy;
         const double cij = d2 > 0 ? fm / ( 2 * sqrt( d2 ) ) : 0.0;
         if ( fits ) {
-            P.out.col[ at + q ] = id;
-            P.out.val[ at + q ] = cij;
+            P.out.col[ at + I64( q ) * ROW_STEP ] = id;
+            P.out.val[ at + I64( q ) * ROW_STEP ] = cij;
         }
         dia += cij;
         ++q;

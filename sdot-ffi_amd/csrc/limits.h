@@ -20,6 +20,7 @@ struct Along {
     I64            n = 0;
     double         lo[ 2 ] = { 0, 0 }, hi[ 2 ] = { 1, 1 };
     Density        dn;
+    const double  *nue = nullptr;       ///< the targets with the exact duplicates folded in ( a duplicate's 0: no say ); may be null
 };
 
 /// FORWARD: `out[ 0 ]` = the smallest, over the cells, first `t > 0` where the frozen mass reaches `eps`

@@ -204,7 +204,7 @@ struct BackwardF {
     const double *a;
     double eps, t_trial;
     __device__ void operator()( I64 k, double *v ) const {
-        const bool below = a[ k ] < eps;
+        const bool below = a[ k ] < eps && ! ( al.nue && ! ( al.nue[ k ] > 0 ) );   // ( a duplicate is empty by construction )
         v[ 0 ] = below ? 1.0 : 0.0;
         v[ 1 ] = below ? backward_cell( al, k, eps, t_trial ) : 1e300;
     }
@@ -230,11 +230,11 @@ __global__ void k_blur_axis( int nx, int ny, int axis, int r, const double *ker,
 } // namespace
 
 void alpha_forward( const Along &al, double eps, double *out, hipStream_t st ) {
-    reduce<1>( al.n, ForwardF{ al, eps }, Ops<1>{ { OP_MIN } }, out, st );
+    reduce_wide<1>( al.n, ForwardF{ al, eps }, Ops<1>{ { OP_MIN } }, out, st );
 }
 
 void alpha_backward( const Along &al, const double *a, double eps, double t_trial, double *out, hipStream_t st ) {
-    reduce<2>( al.n, BackwardF{ al, a, eps, t_trial }, Ops<2>{ { OP_SUM, OP_MIN } }, out, st );
+    reduce_wide<2>( al.n, BackwardF{ al, a, eps, t_trial }, Ops<2>{ { OP_SUM, OP_MIN } }, out, st );
 }
 
 void blur_image( const double *src, int nx, int ny, double hx, double hy, double s, double *dst, double *tmp, hipStream_t st ) {

@@ -9,11 +9,11 @@ struct MgOptions {
     int    pack = 4;          ///< seeds per aggregate ( consecutive tree ranks )
     int    nu = 2;            ///< Jacobi sweeps before and after the coarse correction
     double omega = 0.7;       ///< their damping
-    int    w_levels = 0;      ///< the top levels that visit the next one twice ( a W-cycle there )
+    int    kcycle = 3;        ///< levels 1 .. kcycle take their coarse correction by two flexible CG steps ( a K-cycle ); below, one cycle
 };
 
-/// `L d = b` ( `b` of zero sum ) by CG preconditioned with one V-cycle of an aggregation multigrid whose
-/// aggregates are `pack` consecutive tree ranks; the solution shifted so that `d[ gauge ] = 0`.
+/// `L d = b` ( `b` of zero sum ) by flexible CG preconditioned with one multigrid cycle whose aggregates are `pack`
+/// consecutive tree ranks; the solution shifted so that `d[ gauge ] = 0`.
 /// Returns the iteration count, or -1 when it did not reach `tol` ( relative residual ) in `maxit`, or
 /// -2 when the hierarchy could not be built ( a coarse row with too many neighbours ).
 int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, int maxit, const MgOptions &o, hipStream_t st, CgStats &cs,

@@ -1,20 +1,25 @@
-// The Newton direction by CG preconditioned with an aggregation multigrid.
+// The Newton direction by flexible CG preconditioned with an aggregation multigrid.
 //
 // The reference solves `L d = b` with a multigrid whose aggregates follow the BSP tree's order
 // ( `src/sdot/include/sdot/sdotplan/Multigrid.h:11-18`: consecutive tree ranks are neighbours in
 // space, so packs of consecutive ranks are compact aggregates; the card's version
-// `gpu/Linear2D.cuh:1104-1506`, packs of 4 in 2D ). This one is its own design for CDNA4:
+// `gpu/Linear2D.cuh:1104-1506`, packs of 4 in 2D, a flexible CG and a K-cycle ). This one is its own
+// design for CDNA4:
 //   * aggregates of `pack` consecutive ranks: restriction and prolongation are index arithmetic
 //     ( `I = i / pack` ), nothing to store;
 //   * Galerkin coarse Laplacians `P^T L P` ( P piecewise constant ), one lane per coarse row merging the
-//     `pack` fine rows in LDS, rows stored with a fixed stride so a level is built in one launch;
-//   * a symmetric V-cycle: damped Jacobi before and after, the coarsest level ( <= 256 rows ) solved by
-//     a dense inverse of `L_c + alpha 1 1^T` ( the rank-one term removes the constants, the kernel of a
-//     Laplacian ), computed by Gauss-Jordan in one workgroup per Newton iteration;
-//   * CG in the zero-mean gauge: constants are invisible to `L`, so the iteration needs no projection,
-//     and the solution is shifted at the end to the reference's gauge `d[ gauge ] = 0`;
-//   * every scalar of the CG stays on the device ( `alpha`, `beta`, the stop flag that turns the
-//     remaining launches of a batch into no-ops ), the host reads the state once per batch.
+//     `pack` fine rows in LDS; coarse rows stored COLUMN-MAJOR ( entry `q` of row `I` at `I + q n` ), so the
+//     lanes of a wave read consecutive words;
+//   * the cycle: damped Jacobi before and after the coarse correction; on levels 1 .. `kcycle` the coarse
+//     correction is two steps of flexible CG preconditioned by the next level's cycle ( Notay's K-cycle:
+//     the iteration count stops growing with the number of levels ); below, one cycle; the coarsest level
+//     ( <= 64 rows ) by a dense inverse of `L_c + alpha 1 1^T` ( the rank-one term removes the constants,
+//     the kernel of a Laplacian );
+//   * the outer flexible CG ( `alpha = p . r / p . q`, `beta = -z . q / p . q` ) in the zero-mean gauge:
+//     constants are invisible to `L`, so the iteration needs no projection, and the solution is shifted at
+//     the end to the reference's gauge `d[ gauge ] = 0`;
+//   * every scalar stays on the device ( the K-cycle's and the CG's, the stop flag that turns the remaining
+//     launches of a batch into no-ops ), the host reads the state once per batch.
 // Deterministic: fixed-shape reductions, each coarse row merged in a fixed order.
 
 #include "multigrid.h"
@@ -26,25 +31,34 @@ namespace {
 constexpr int MG_STRIDE = 48;      ///< neighbours of a coarse row, at most
 constexpr int MG_BLOCK = 64;
 constexpr int MAX_LEVELS = 24;
+constexpr int DENSE_MAX = 64;      ///< the coarsest level, solved by a dense inverse
 
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_NB, S_ERR, S_DONE, S_IT, S_TOL, S_FAILED, S_BAD, S_NB_STATE };
+/// the outer CG's device state
+enum { S_PQ = 0, S_PR, S_ALPHA, S_BETA, S_ZQ, S_RR, S_NB, S_ERR, S_DONE, S_IT, S_TOL, S_FAILED, S_BAD, S_NB_STATE };
+/// a K-cycle's scalars, per level
+enum { K_RHO1 = 0, K_ALPHA1, K_GAMMA, K_BETA2, K_ALPHA2, K_NB };
 
+/// a level's matrix: entry `q` of row `i` at `base + q * step`, `base = row_off[ i ]` ( CSR, `step` 1 ) or `i`
+/// ( column-major, `row_off` null, `step` the row count ); the off-diagonal values are the `c_ij > 0`
 struct Mat {
-    I64 n = 0;
+    I64 n = 0, step = 1;
     const int *row_off = nullptr, *row_cnt = nullptr, *col = nullptr;
     const double *val = nullptr, *dia = nullptr;
 };
 
 __device__ __forceinline__ double row_times( const Mat &A, I64 i, const double *x ) {
     double s = A.dia[ i ] * x[ i ];
-    const int b = A.row_off[ i ], e = b + A.row_cnt[ i ];
-    for ( int q = b; q < e; ++q ) s -= A.val[ q ] * x[ A.col[ q ] ];
+    const I64 b = A.row_off ? A.row_off[ i ] : i;
+    const int c = A.row_cnt[ i ];
+    for ( int q = 0; q < c; ++q ) {
+        const I64 e = b + q * A.step;
+        s -= A.val[ e ] * x[ A.col[ e ] ];
+    }
     return s;
 }
 
-/// the Galerkin product of one coarse row: the `pack` fine rows merged by coarse column
-__global__ __launch_bounds__( MG_BLOCK ) void k_coarse_op( Mat F, int pack, I64 nc, int *row_off, int *row_cnt, int *col, double *val, double *dia,
-                                                         double *state ) {
+/// the Galerkin product of one coarse row: the `pack` fine rows merged by coarse column, written column-major
+__global__ __launch_bounds__( MG_BLOCK ) void k_coarse_op( Mat F, int pack, I64 nc, int *row_cnt, int *col, double *val, double *dia, double *state ) {
     __shared__ int sj[ MG_STRIDE * MG_BLOCK ];
     __shared__ double sc[ MG_STRIDE * MG_BLOCK ];
     const I64 I = I64( blockIdx.x ) * MG_BLOCK + threadIdx.x;
@@ -57,10 +71,12 @@ __global__ __launch_bounds__( MG_BLOCK ) void k_coarse_op( Mat F, int pack, I64 
     const I64 i0 = I * pack, i1 = i0 + pack < F.n ? i0 + pack : F.n;
     for ( I64 i = i0; i < i1; ++i ) {
         d += F.dia[ i ];
-        const int b = F.row_off[ i ], e = b + F.row_cnt[ i ];
-        for ( int q = b; q < e; ++q ) {
-            const int jc = F.col[ q ] / pack;
-            const double c = F.val[ q ];
+        const I64 b = F.row_off ? F.row_off[ i ] : i;
+        const int cnt = F.row_cnt[ i ];
+        for ( int q = 0; q < cnt; ++q ) {
+            const I64 e = b + q * F.step;
+            const int jc = F.col[ e ] / pack;
+            const double c = F.val[ e ];
             if ( jc == int( I ) ) { d -= c; continue; }
             int k = 0;
             while ( k < u && J[ k * MG_BLOCK ] != jc ) ++k;
@@ -73,17 +89,14 @@ __global__ __launch_bounds__( MG_BLOCK ) void k_coarse_op( Mat F, int pack, I64 
             C[ k * MG_BLOCK ] += c;
         }
     }
-    row_off[ I ] = int( I * MG_STRIDE );
     row_cnt[ I ] = u;
     for ( int k = 0; k < u; ++k ) {
-        col[ I * MG_STRIDE + k ] = J[ k * MG_BLOCK ];
-        val[ I * MG_STRIDE + k ] = C[ k * MG_BLOCK ];
+        col[ I + k * nc ] = J[ k * MG_BLOCK ];
+        val[ I + k * nc ] = C[ k * MG_BLOCK ];
     }
     dia[ I ] = d > 0 ? d : 1.0;
     if ( bad ) state[ S_BAD ] = 1;
 }
-
-constexpr int DENSE_MAX = 64;       ///< the coarsest level, solved by a dense inverse
 
 /// the inverse of `L_c + alpha 1 1^T` ( `alpha` the mean diagonal over `n`: the rank-one term removes the
 /// constants, the kernel of a Laplacian ), by Gauss-Jordan in LDS, one workgroup; writes it row-major
@@ -99,8 +112,8 @@ __global__ __launch_bounds__( 256 ) void k_dense_inverse( Mat C, double *inv ) {
     if ( int( threadIdx.x ) < n ) {
         const int i = threadIdx.x;
         M[ i ][ i ] += C.dia[ i ];
-        const int b = C.row_off[ i ], e = b + C.row_cnt[ i ];
-        for ( int q = b; q < e; ++q ) M[ i ][ C.col[ q ] ] -= C.val[ q ];
+        const I64 b = C.row_off ? C.row_off[ i ] : i;
+        for ( int q = 0; q < C.row_cnt[ i ]; ++q ) M[ i ][ C.col[ b + q * C.step ] ] -= C.val[ b + q * C.step ];
     }
     __syncthreads();
     for ( int p = 0; p < n; ++p ) {
@@ -122,7 +135,7 @@ __global__ __launch_bounds__( 256 ) void k_dense_inverse( Mat C, double *inv ) {
     for ( int k = threadIdx.x; k < n * n; k += blockDim.x ) inv[ k ] = V[ k / n ][ k % n ];
 }
 
-// ---- the V-cycle --------------------------------------------------------------------------------------------
+// ---- the cycle --------------------------------------------------------------------------------------------
 
 __global__ void k_jacobi_first( I64 n, const double *r, const double *dia, double omega, double *x, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
@@ -144,7 +157,7 @@ __global__ void k_copy( I64 n, const double *a, double *b, const double *state )
 
 /// `rc[ I ] = sum_{ i in I } ( r - A x )_i`: one lane per fine row, the `pack` consecutive lanes of an aggregate
 /// summed by shuffles ( `pack` divides 64 )
-__global__ void k_restrict( Mat A, const double *r, const double *x, int pack, I64 nc, double *rc, const double *state ) {
+__global__ void k_restrict( Mat A, const double *r, const double *x, int pack, double *rc, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
     const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
     double v = i < A.n ? r[ i ] - row_times( A, i, x ) : 0.0;
@@ -168,7 +181,80 @@ __global__ void k_dense_apply( int n, const double *inv, const double *r, double
     if ( lane == 0 ) x[ row ] = s;
 }
 
-// ---- the CG around it -------------------------------------------------------------------------------------------
+// ---- the K-cycle's two flexible CG steps ( Notay ) ------------------------------------------------------------
+//
+//   c1 = C( r ), v1 = A c1, rho1 = c1 . v1, alpha1 = c1 . r, r2 = r - alpha1 / rho1 v1,
+//   c2 = C( r2 ), v2 = A c2, gamma = c2 . v1, beta = c2 . v2, alpha2 = c2 . r2, rho2 = beta - gamma^2 / rho1,
+//   x = ( alpha1 / rho1 - gamma alpha2 / ( rho1 rho2 ) ) c1 + alpha2 / rho2 c2
+
+template<int T>
+__device__ __forceinline__ double block_sum_t( double v, double *sh ) { return block_sum<T>( v, sh ); }
+
+/// `v = A c` and the partial sums of `c . v`, `c . u1` and ( `nd == 3` ) `c . u2`, over `nblk` blocks
+__global__ __launch_bounds__( RED_THREADS ) void k_spmv_dots( Mat A, const double *c, double *v, const double *u1, const double *u2, int nd, double *part,
+                                                            const double *state ) {
+    __shared__ double sh[ RED_THREADS ];
+    if ( state[ S_DONE ] != 0 ) return;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < A.n; i += I64( gridDim.x ) * RED_THREADS ) {
+        const double ci = c[ i ], vi = row_times( A, i, c );
+        v[ i ] = vi;
+        a0 += ci * vi;
+        a1 += ci * u1[ i ];
+        if ( nd == 3 ) a2 += ci * u2[ i ];
+    }
+    const double r0 = block_sum<RED_THREADS>( a0, sh );
+    const double r1 = block_sum<RED_THREADS>( a1, sh );
+    const double r2 = nd == 3 ? block_sum<RED_THREADS>( a2, sh ) : 0.0;
+    if ( threadIdx.x == 0 ) {
+        part[ blockIdx.x ] = r0;
+        part[ RED_BLOCKS + blockIdx.x ] = r1;
+        part[ 2 * RED_BLOCKS + blockIdx.x ] = r2;
+    }
+}
+
+/// the partials of `k_spmv_dots` summed into the level's scalars: first step ( `nd == 2` ): `rho1`, `alpha1`;
+/// second step: `beta`, `gamma`, `alpha2`
+__global__ __launch_bounds__( RED_THREADS ) void k_kc_scalars( const double *part, int nblk, int nd, double *ks, const double *state ) {
+    __shared__ double sh[ RED_THREADS ];
+    if ( state[ S_DONE ] != 0 ) return;
+    double s[ 3 ];
+    for ( int q = 0; q < nd; ++q ) {
+        double a = 0;
+        for ( int j = threadIdx.x; j < nblk; j += RED_THREADS ) a += part[ q * RED_BLOCKS + j ];
+        s[ q ] = block_sum<RED_THREADS>( a, sh );
+    }
+    if ( threadIdx.x == 0 ) {
+        if ( nd == 2 ) { ks[ K_RHO1 ] = s[ 0 ]; ks[ K_ALPHA1 ] = s[ 1 ]; }
+        else           { ks[ K_BETA2 ] = s[ 0 ]; ks[ K_GAMMA ] = s[ 1 ]; ks[ K_ALPHA2 ] = s[ 2 ]; }
+    }
+}
+
+/// `r2 = r - alpha1 / rho1 v1`
+__global__ void k_kc_residual( I64 n, const double *r, const double *v1, const double *ks, double *r2, const double *state ) {
+    if ( state[ S_DONE ] != 0 ) return;
+    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    const double f = ks[ K_RHO1 ] > 0 ? ks[ K_ALPHA1 ] / ks[ K_RHO1 ] : 0.0;
+    if ( i < n ) r2[ i ] = r[ i ] - f * v1[ i ];
+}
+
+/// `x = a1 c1 + a2 c2` with the K-cycle's coefficients ( one step only when the second is degenerate )
+__global__ void k_kc_combine( I64 n, const double *c1, const double *c2, const double *ks, double *x, const double *state ) {
+    if ( state[ S_DONE ] != 0 ) return;
+    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( i >= n ) return;
+    const double rho1 = ks[ K_RHO1 ];
+    if ( ! ( rho1 > 0 ) ) { x[ i ] = c1[ i ]; return; }
+    const double rho2 = ks[ K_BETA2 ] - ks[ K_GAMMA ] * ks[ K_GAMMA ] / rho1;
+    double a1 = ks[ K_ALPHA1 ] / rho1, a2 = 0;
+    if ( rho2 > 0 ) {
+        a2 = ks[ K_ALPHA2 ] / rho2;
+        a1 -= ks[ K_GAMMA ] * a2 / rho1;
+    }
+    x[ i ] = a1 * c1[ i ] + a2 * c2[ i ];
+}
+
+// ---- the outer flexible CG -------------------------------------------------------------------------------------
 
 template<int T>
 __device__ __forceinline__ void write_partial( double v, double *part, int slot ) {
@@ -199,57 +285,39 @@ __global__ __launch_bounds__( RED_THREADS ) void k_pcg_init_final( const double 
         state[ S_IT ] = 0;
         state[ S_TOL ] = tol;
         state[ S_FAILED ] = 0;
-        state[ S_RZ ] = 0;
+        state[ S_BETA ] = 0;
+        state[ S_PQ ] = 0;
     }
 }
 
-/// `rz = r . z` ( partials ); `p = z` on the first use ( `first` ), else `p = z + beta p` comes after the final
-__global__ __launch_bounds__( RED_THREADS ) void k_pcg_dot( I64 n, const double *r, const double *z, double *part, const double *state ) {
-    if ( state[ S_DONE ] != 0 ) return;
-    double acc = 0;
-    for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < n; i += I64( RED_BLOCKS ) * RED_THREADS ) acc += r[ i ] * z[ i ];
-    write_partial<RED_THREADS>( acc, part, blockIdx.x );
-}
-
-__global__ __launch_bounds__( RED_THREADS ) void k_pcg_beta( const double *part, double *state ) {
+/// `q = A p`, partial sums of `p . q` and `p . r`
+__global__ __launch_bounds__( RED_THREADS ) void k_fcg_spmv( Mat A, const double *p, const double *r, double *q, double *part, const double *state ) {
     __shared__ double sh[ RED_THREADS ];
     if ( state[ S_DONE ] != 0 ) return;
-    double a = 0;
-    for ( int j = threadIdx.x; j < RED_BLOCKS; j += RED_THREADS ) a += part[ j ];
-    const double rz2 = block_sum<RED_THREADS>( a, sh );
-    if ( threadIdx.x == 0 ) {
-        state[ S_BETA ] = state[ S_RZ ] > 0 ? rz2 / state[ S_RZ ] : 0.0;
-        state[ S_RZ ] = rz2;
-    }
-}
-
-__global__ void k_pcg_dir( I64 n, const double *z, double *p, const double *state ) {
-    if ( state[ S_DONE ] != 0 ) return;
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
-    if ( i < n ) p[ i ] = z[ i ] + state[ S_BETA ] * p[ i ];
-}
-
-__global__ __launch_bounds__( RED_THREADS ) void k_pcg_spmv( Mat A, const double *p, double *q, double *part, const double *state ) {
-    if ( state[ S_DONE ] != 0 ) return;
-    double acc = 0;
+    double pq = 0, pr = 0;
     for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < A.n; i += I64( RED_BLOCKS ) * RED_THREADS ) {
         const double s = row_times( A, i, p );
         q[ i ] = s;
-        acc += p[ i ] * s;
+        pq += p[ i ] * s;
+        pr += p[ i ] * r[ i ];
     }
-    write_partial<RED_THREADS>( acc, part, blockIdx.x );
+    const double a = block_sum<RED_THREADS>( pq, sh );
+    const double b = block_sum<RED_THREADS>( pr, sh );
+    if ( threadIdx.x == 0 ) { part[ blockIdx.x ] = a; part[ RED_BLOCKS + blockIdx.x ] = b; }
 }
 
-__global__ __launch_bounds__( RED_THREADS ) void k_pcg_alpha( const double *part, double *state ) {
+__global__ __launch_bounds__( RED_THREADS ) void k_fcg_alpha( const double *part, double *state ) {
     __shared__ double sh[ RED_THREADS ];
     if ( state[ S_DONE ] != 0 ) return;
-    double a = 0;
-    for ( int j = threadIdx.x; j < RED_BLOCKS; j += RED_THREADS ) a += part[ j ];
+    double a = 0, b = 0;
+    for ( int j = threadIdx.x; j < RED_BLOCKS; j += RED_THREADS ) { a += part[ j ]; b += part[ RED_BLOCKS + j ]; }
     const double pq = block_sum<RED_THREADS>( a, sh );
+    const double pr = block_sum<RED_THREADS>( b, sh );
     if ( threadIdx.x == 0 ) {
         state[ S_PQ ] = pq;
+        state[ S_PR ] = pr;
         if ( ! ( pq > 0 ) ) { state[ S_DONE ] = 1; state[ S_FAILED ] = 1; }
-        else state[ S_ALPHA ] = state[ S_RZ ] / pq;
+        else state[ S_ALPHA ] = pr / pq;
     }
 }
 
@@ -282,6 +350,32 @@ __global__ __launch_bounds__( RED_THREADS ) void k_pcg_check( const double *part
     }
 }
 
+/// partial sums of `z . q` ( the flexible CG's `beta` )
+__global__ __launch_bounds__( RED_THREADS ) void k_fcg_zq( I64 n, const double *z, const double *q, double *part, const double *state ) {
+    if ( state[ S_DONE ] != 0 ) return;
+    double acc = 0;
+    for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < n; i += I64( RED_BLOCKS ) * RED_THREADS ) acc += z[ i ] * q[ i ];
+    write_partial<RED_THREADS>( acc, part, blockIdx.x );
+}
+
+__global__ __launch_bounds__( RED_THREADS ) void k_fcg_beta( const double *part, double *state ) {
+    __shared__ double sh[ RED_THREADS ];
+    if ( state[ S_DONE ] != 0 ) return;
+    double a = 0;
+    for ( int j = threadIdx.x; j < RED_BLOCKS; j += RED_THREADS ) a += part[ j ];
+    const double zq = block_sum<RED_THREADS>( a, sh );
+    if ( threadIdx.x == 0 ) {
+        state[ S_ZQ ] = zq;
+        state[ S_BETA ] = state[ S_PQ ] > 0 ? -zq / state[ S_PQ ] : 0.0;
+    }
+}
+
+__global__ void k_pcg_dir( I64 n, const double *z, double *p, const double *state ) {
+    if ( state[ S_DONE ] != 0 ) return;
+    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( i < n ) p[ i ] = z[ i ] + state[ S_BETA ] * p[ i ];
+}
+
 __global__ void k_shift_gauge( I64 n, double *d, const double *g, int gauge ) {
     const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
     if ( i < n && i != gauge ) d[ i ] -= g[ 0 ];
@@ -290,85 +384,112 @@ __global__ void k_shift_gauge( I64 n, double *d, const double *g, int gauge ) {
 __global__ void k_read_gauge( const double *d, int gauge, double *g ) { if ( threadIdx.x == 0 ) g[ 0 ] = d[ gauge ]; }
 __global__ void k_zero_gauge( double *d, int gauge ) { if ( threadIdx.x == 0 ) d[ gauge ] = 0; }
 
+// ---- the hierarchy and its cycles ----------------------------------------------------------------------------------
+
 struct Hierarchy {
     int nl = 0;
     Mat A[ MAX_LEVELS ];
+    // per level: the cycle's right-hand side, solution and smoothing buffer; the K-cycle's vectors
     double *x[ MAX_LEVELS ] = {}, *t[ MAX_LEVELS ] = {}, *r[ MAX_LEVELS ] = {};
+    double *c1[ MAX_LEVELS ] = {}, *c2[ MAX_LEVELS ] = {}, *v1[ MAX_LEVELS ] = {}, *v2[ MAX_LEVELS ] = {}, *r2[ MAX_LEVELS ] = {};
+    double *ks = nullptr;              ///< [ MAX_LEVELS ][ K_NB ]
+    double *part = nullptr;            ///< [ 3 ][ RED_BLOCKS ]
     double *inv = nullptr;
     int nc = 0;
 };
 
-/// one cycle on level `l`: `x[ l ] ~ A_l^-1 r[ l ]` from zero; the next level is visited twice on the
-/// top `o.w_levels` levels ( a W-cycle there ), once below
-void cycle( Hierarchy &H, int l, const double *const *r, double *const *x, const MgOptions &o, const double *state, hipStream_t st ) {
+unsigned red_blocks_for( I64 n ) {
+    const unsigned b = blocks_for( n, RED_THREADS );
+    return b < unsigned( RED_BLOCKS ) ? b : unsigned( RED_BLOCKS );
+}
+
+void cycle( Hierarchy &H, int l, const double *r, double *x, const MgOptions &o, const double *state, hipStream_t st );
+
+/// `x ~ A_l^-1 r` on level `l >= 1` by two flexible CG steps preconditioned by the cycle of level `l`
+void kcycle( Hierarchy &H, int l, const double *r, double *x, const MgOptions &o, const double *state, hipStream_t st ) {
+    const Mat &A = H.A[ l ];
+    const unsigned nb = blocks_for( A.n, 256 ), rb = red_blocks_for( A.n );
+    double *ks = H.ks + l * K_NB;
+    cycle( H, l, r, H.c1[ l ], o, state, st );
+    k_spmv_dots<<<rb, RED_THREADS, 0, st>>>( A, H.c1[ l ], H.v1[ l ], r, nullptr, 2, H.part, state );
+    k_kc_scalars<<<1, RED_THREADS, 0, st>>>( H.part, int( rb ), 2, ks, state );
+    k_kc_residual<<<nb, 256, 0, st>>>( A.n, r, H.v1[ l ], ks, H.r2[ l ], state );
+    cycle( H, l, H.r2[ l ], H.c2[ l ], o, state, st );
+    k_spmv_dots<<<rb, RED_THREADS, 0, st>>>( A, H.c2[ l ], H.v2[ l ], H.v1[ l ], H.r2[ l ], 3, H.part, state );
+    k_kc_scalars<<<1, RED_THREADS, 0, st>>>( H.part, int( rb ), 3, ks, state );
+    k_kc_combine<<<nb, 256, 0, st>>>( A.n, H.c1[ l ], H.c2[ l ], ks, x, state );
+}
+
+/// one cycle on level `l`: `x ~ A_l^-1 r` from zero
+void cycle( Hierarchy &H, int l, const double *r, double *x, const MgOptions &o, const double *state, hipStream_t st ) {
     if ( l == H.nl - 1 ) {
-        k_dense_apply<<<blocks_for( H.nc, 4 ), 256, 0, st>>>( H.nc, H.inv, r[ l ], x[ l ], state );
+        k_dense_apply<<<blocks_for( H.nc, 4 ), 256, 0, st>>>( H.nc, H.inv, r, x, state );
         return;
     }
     const Mat &A = H.A[ l ];
     const unsigned nb = blocks_for( A.n, 256 );
-    // pre-smoothing from zero: the first sweep, then `nu - 1` sweeps ping-ponging so that the last lands in `x[ l ]`
-    double *cur = ( o.nu - 1 ) % 2 ? H.t[ l ] : x[ l ], *tmp = ( o.nu - 1 ) % 2 ? x[ l ] : H.t[ l ];
-    k_jacobi_first<<<nb, 256, 0, st>>>( A.n, r[ l ], A.dia, o.omega, cur, state );
+    // pre-smoothing from zero: the first sweep, then `nu - 1` sweeps ping-ponging so that the last lands in `x`
+    double *cur = ( o.nu - 1 ) % 2 ? H.t[ l ] : x, *tmp = ( o.nu - 1 ) % 2 ? x : H.t[ l ];
+    k_jacobi_first<<<nb, 256, 0, st>>>( A.n, r, A.dia, o.omega, cur, state );
     for ( int s = 1; s < o.nu; ++s ) {
-        k_jacobi<<<nb, 256, 0, st>>>( A, r[ l ], cur, tmp, o.omega, state );
+        k_jacobi<<<nb, 256, 0, st>>>( A, r, cur, tmp, o.omega, state );
         std::swap( cur, tmp );
     }
-    // the coarse corrections ( two in a row on the W levels: the square of an A-symmetric correction is one )
-    const int visits = l < o.w_levels ? 2 : 1;
-    for ( int v = 0; v < visits; ++v ) {
-        const I64 nc = H.A[ l + 1 ].n;
-        k_restrict<<<nb, 256, 0, st>>>( A, r[ l ], x[ l ], o.pack, nc, H.r[ l + 1 ], state );
-        cycle( H, l + 1, r, x, o, state, st );
-        k_prolong<<<nb, 256, 0, st>>>( A.n, o.pack, x[ l + 1 ], x[ l ], state );
-    }
+    // the coarse correction: the residual restricted, solved by the next level's K-cycle or cycle, prolonged
+    k_restrict<<<nb, 256, 0, st>>>( A, r, x, o.pack, H.r[ l + 1 ], state );
+    if ( l + 1 <= o.kcycle && l + 1 < H.nl - 1 ) kcycle( H, l + 1, H.r[ l + 1 ], H.x[ l + 1 ], o, state, st );
+    else                                         cycle( H, l + 1, H.r[ l + 1 ], H.x[ l + 1 ], o, state, st );
+    k_prolong<<<nb, 256, 0, st>>>( A.n, o.pack, H.x[ l + 1 ], x, state );
     // post-smoothing, the pre-smoothing's mirror
-    double *c2 = x[ l ], *t2 = H.t[ l ];
+    double *c2 = x, *t2 = H.t[ l ];
     for ( int s = 0; s < o.nu; ++s ) {
-        k_jacobi<<<nb, 256, 0, st>>>( A, r[ l ], c2, t2, o.omega, state );
+        k_jacobi<<<nb, 256, 0, st>>>( A, r, c2, t2, o.omega, state );
         std::swap( c2, t2 );
     }
-    if ( c2 != x[ l ] ) k_copy<<<nb, 256, 0, st>>>( A.n, c2, x[ l ], state );
-}
-
-/// `z = V( r )` on level 0, every launch a no-op once the CG has stopped
-void vcycle( Hierarchy &H, const double *r0, double *z0, const MgOptions &o, const double *state, hipStream_t st ) {
-    const double *r[ MAX_LEVELS ];
-    double *x[ MAX_LEVELS ];
-    for ( int l = 0; l < H.nl; ++l ) { r[ l ] = l ? H.r[ l ] : r0; x[ l ] = l ? H.x[ l ] : z0; }
-    cycle( H, 0, r, x, o, state, st );
-    check_launch( "vcycle" );
+    if ( c2 != x ) k_copy<<<nb, 256, 0, st>>>( A.n, c2, x, state );
 }
 
 } // namespace
 
 int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, int maxit, const MgOptions &o, hipStream_t st, CgStats &cs,
               int &nb_levels ) {
+    SDOT_CHECK( o.pack >= 2 && o.pack <= 64 && ( o.pack & ( o.pack - 1 ) ) == 0, "multigrid: the aggregates must be a power of two from 2 to 64" );
+    SDOT_CHECK( o.nu >= 1, "multigrid: at least one smoothing sweep" );
     Arena &A = arena();
     double *state = A.get<double>( "mg.state", S_NB_STATE );
-    double *part = A.get<double>( "mg.part", RED_BLOCKS );
     double *hs = pinned<double>( "mg.hstate", S_NB_STATE );
     Hierarchy H;
-    H.A[ 0 ].n = L.n; H.A[ 0 ].row_off = L.row_off; H.A[ 0 ].row_cnt = L.row_cnt; H.A[ 0 ].col = L.col; H.A[ 0 ].val = L.val; H.A[ 0 ].dia = L.dia;
+    H.part = A.get<double>( "mg.part", 3 * size_t( RED_BLOCKS ) );
+    H.ks = A.get<double>( "mg.ks", size_t( MAX_LEVELS ) * K_NB );
+    H.A[ 0 ].n = L.n; H.A[ 0 ].step = L.step;
+    H.A[ 0 ].row_off = L.row_off; H.A[ 0 ].row_cnt = L.row_cnt; H.A[ 0 ].col = L.col; H.A[ 0 ].val = L.val; H.A[ 0 ].dia = L.dia;
     H.nl = 1;
     SDOT_HIP( hipMemsetAsync( state, 0, sizeof( double ) * S_NB_STATE, st ) );
     // the hierarchy
     for ( int l = 0; H.A[ l ].n > DENSE_MAX && l + 1 < MAX_LEVELS; ++l ) {
         const I64 nc = ( H.A[ l ].n + o.pack - 1 ) / o.pack;
         const std::string tag = "mg.l" + std::to_string( l + 1 );
-        int *ro = A.get<int>( tag + ".ro", size_t( nc ) ), *rc = A.get<int>( tag + ".rc", size_t( nc ) );
+        int *rc = A.get<int>( tag + ".rc", size_t( nc ) );
         int *col = A.get<int>( tag + ".col", size_t( nc ) * MG_STRIDE );
         double *val = A.get<double>( tag + ".val", size_t( nc ) * MG_STRIDE ), *dia = A.get<double>( tag + ".dia", size_t( nc ) );
-        k_coarse_op<<<blocks_for( nc, MG_BLOCK ), MG_BLOCK, 0, st>>>( H.A[ l ], o.pack, nc, ro, rc, col, val, dia, state );
+        k_coarse_op<<<blocks_for( nc, MG_BLOCK ), MG_BLOCK, 0, st>>>( H.A[ l ], o.pack, nc, rc, col, val, dia, state );
         check_launch( "k_coarse_op" );
         Mat &C = H.A[ l + 1 ];
-        C.n = nc; C.row_off = ro; C.row_cnt = rc; C.col = col; C.val = val; C.dia = dia;
+        C.n = nc; C.step = nc; C.row_off = nullptr; C.row_cnt = rc; C.col = col; C.val = val; C.dia = dia;
         H.nl = l + 2;
     }
     for ( int l = 0; l < H.nl; ++l ) {
         const std::string tag = "mg.v" + std::to_string( l );
-        H.t[ l ] = A.get<double>( tag + ".t", size_t( H.A[ l ].n ) );
-        if ( l ) { H.x[ l ] = A.get<double>( tag + ".x", size_t( H.A[ l ].n ) ); H.r[ l ] = A.get<double>( tag + ".r", size_t( H.A[ l ].n ) ); }
+        const size_t m = size_t( H.A[ l ].n );
+        H.t[ l ] = A.get<double>( tag + ".t", m );
+        if ( l ) {
+            H.x[ l ] = A.get<double>( tag + ".x", m ); H.r[ l ] = A.get<double>( tag + ".r", m );
+            if ( l <= o.kcycle && l < H.nl - 1 ) {
+                H.c1[ l ] = A.get<double>( tag + ".c1", m ); H.c2[ l ] = A.get<double>( tag + ".c2", m );
+                H.v1[ l ] = A.get<double>( tag + ".v1", m ); H.v2[ l ] = A.get<double>( tag + ".v2", m );
+                H.r2[ l ] = A.get<double>( tag + ".r2", m );
+            }
+        }
     }
     H.nc = int( H.A[ H.nl - 1 ].n );
     SDOT_CHECK( H.nc <= DENSE_MAX, "multigrid: the coarsest level is too large for its dense inverse" );
@@ -377,17 +498,16 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
     check_launch( "coarsest inverse" );
     nb_levels = H.nl;
 
-    // the CG
+    // the flexible CG
     const I64 n = L.n;
     double *r = A.get<double>( "mg.r", size_t( n ) ), *z = A.get<double>( "mg.z", size_t( n ) );
     double *p = A.get<double>( "mg.p", size_t( n ) ), *q = A.get<double>( "mg.q", size_t( n ) );
     const Mat &A0 = H.A[ 0 ];
+    double *part = H.part;
     k_pcg_init<<<RED_BLOCKS, RED_THREADS, 0, st>>>( A0, b, d, r, part );
     k_pcg_init_final<<<1, RED_THREADS, 0, st>>>( part, state, tol );
-    // z = V( r ), rz = r . z, p = z ( beta = 0 on the first pass: `S_RZ` is 0 )
-    vcycle( H, r, z, o, state, st );
-    k_pcg_dot<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, r, z, part, state );
-    k_pcg_beta<<<1, RED_THREADS, 0, st>>>( part, state );
+    // z = M( r ), p = z ( beta = 0 on the first pass )
+    cycle( H, 0, r, z, o, state, st );
     SDOT_HIP( hipMemsetAsync( p, 0, sizeof( double ) * size_t( n ), st ) );
     k_pcg_dir<<<blocks_for( n, 256 ), 256, 0, st>>>( n, z, p, state );
     check_launch( "pcg init" );
@@ -397,17 +517,17 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
         for ( int c = 0; c < batch; ++c ) {
             Timing::Pending tp;
             const bool timed = c == 0 && timing().begin( TK_SPMV, st, tp );
-            k_pcg_spmv<<<RED_BLOCKS, RED_THREADS, 0, st>>>( A0, p, q, part, state );
+            k_fcg_spmv<<<RED_BLOCKS, RED_THREADS, 0, st>>>( A0, p, r, q, part, state );
             if ( timed ) {
                 timing().end( tp, st );
-                timing().spmv_bytes += double( n ) * ( 4 + 4 + 8 + 8 + 8 ) + double( L.nnz ) * ( 4 + 8 );
+                timing().spmv_bytes += double( n ) * ( 4 + 4 + 8 + 8 + 8 + 8 ) + double( L.nnz ) * ( 4 + 8 );
             }
-            k_pcg_alpha<<<1, RED_THREADS, 0, st>>>( part, state );
+            k_fcg_alpha<<<1, RED_THREADS, 0, st>>>( part, state );
             k_pcg_update<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, p, q, d, r, part, state );
             k_pcg_check<<<1, RED_THREADS, 0, st>>>( part, state );
-            vcycle( H, r, z, o, state, st );
-            k_pcg_dot<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, r, z, part, state );
-            k_pcg_beta<<<1, RED_THREADS, 0, st>>>( part, state );
+            cycle( H, 0, r, z, o, state, st );
+            k_fcg_zq<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, z, q, part, state );
+            k_fcg_beta<<<1, RED_THREADS, 0, st>>>( part, state );
             k_pcg_dir<<<blocks_for( n, 256 ), 256, 0, st>>>( n, z, p, state );
         }
         check_launch( "pcg iterations" );

@@ -4,6 +4,8 @@
 
 #include "common.h"
 
+#include <algorithm>
+
 namespace sdot_amd {
 
 namespace {
@@ -40,14 +42,29 @@ __global__ __launch_bounds__( RED_THREADS ) void k_partials( I64 n, F f, Ops<K> 
     }
 }
 
-/// the partials of every quantity reduced by one block; `E` then runs on thread 0 with the results
+/// one element per lane ( for functors heavy enough that a grid-stride loop over `RED_BLOCKS` blocks would leave the chip
+/// half idle ): one partial per block, `nb` blocks, quantity `q`'s partials at `part[ q * nb + b ]`
+template<int K, class F>
+__global__ __launch_bounds__( RED_THREADS ) void k_partials_wide( I64 n, F f, Ops<K> ops, double *part, int nb ) {
+    __shared__ double sh[ RED_THREADS ];
+    double v[ K ];
+    const I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x;
+    if ( i < n ) f( i, v );
+    else for ( int q = 0; q < K; ++q ) v[ q ] = op_init( ops.op[ q ] );
+    for ( int q = 0; q < K; ++q ) {
+        const double r = block_op( ops.op[ q ], v[ q ], sh );
+        if ( threadIdx.x == 0 ) part[ I64( q ) * nb + blockIdx.x ] = r;
+    }
+}
+
+/// the `nb` partials of every quantity reduced by one block; `E` then runs on thread 0 with the results
 template<int K, class E>
-__global__ __launch_bounds__( RED_THREADS ) void k_final( const double *part, Ops<K> ops, double *out, E epilogue ) {
+__global__ __launch_bounds__( RED_THREADS ) void k_final( const double *part, Ops<K> ops, double *out, E epilogue, int nb = RED_BLOCKS ) {
     __shared__ double sh[ RED_THREADS ];
     __shared__ double res[ K ];
     for ( int q = 0; q < K; ++q ) {
         double a = op_init( ops.op[ q ] );
-        for ( int j = threadIdx.x; j < RED_BLOCKS; j += RED_THREADS ) a = op_apply( ops.op[ q ], a, part[ q * RED_BLOCKS + j ] );
+        for ( int j = threadIdx.x; j < nb; j += RED_THREADS ) a = op_apply( ops.op[ q ], a, part[ I64( q ) * nb + j ] );
         const double r = block_op( ops.op[ q ], a, sh );
         if ( threadIdx.x == 0 ) res[ q ] = r;
     }
@@ -66,6 +83,19 @@ void reduce( I64 n, const F &f, Ops<K> ops, double *out, hipStream_t st, E e = E
     k_partials<K, F><<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, f, ops, part );
     check_launch( "k_partials" );
     k_final<K, E><<<1, RED_THREADS, 0, st>>>( part, ops, out, e );
+    check_launch( "k_final" );
+}
+
+/// the same reduction with one element per lane ( `k_partials_wide` ): for per-element work of many loads and many
+/// registers, where the chip needs every element in flight at once. Fixed shape for a given `n`, so still bitwise
+/// reproducible
+template<int K, class F>
+void reduce_wide( I64 n, const F &f, Ops<K> ops, double *out, hipStream_t st ) {
+    const int nb = int( std::max<I64>( 1, ( n + RED_THREADS - 1 ) / RED_THREADS ) );
+    double *part = arena().get<double>( "red.wide", size_t( K ) * size_t( nb ) );
+    k_partials_wide<K, F><<<nb, RED_THREADS, 0, st>>>( n, f, ops, part, nb );
+    check_launch( "k_partials_wide" );
+    k_final<K, NoEpilogue><<<1, RED_THREADS, 0, st>>>( part, ops, out, NoEpilogue{}, nb );
     check_launch( "k_final" );
 }
 

@@ -23,12 +23,13 @@ enum Stat {
     NB_CLUSTERS, NB_AGGREGATED, NB_DUPLICATES, RESIDUAL_FULL, NB_POLISH, NB_STATS
 };
 enum Hist { H_STEP = 0, H_T, H_RESIDUAL_L2, H_MIN_MASS, H_MAX_RESIDUAL, H_NB_DIAG, H_NB_EVALS, H_S, NB_HIST };
-enum Status { S_RUNNING = 0, S_CONVERGED = 1, S_MAX_ITERATIONS = 2, S_STAGNATION = 3, S_LINEAR_FAILURE = 4, S_CAPACITY = 5, S_FAILURE = 6 };
+enum Status { S_RUNNING = 0, S_CONVERGED = 1, S_MAX_ITERATIONS = 2, S_STAGNATION = 3, S_LINEAR_FAILURE = 4, S_CAPACITY = 5, S_FAILURE = 6,
+              S_CONVERGED_AGGREGATED = 7 };
 enum Residual { R_LIN = 0, R_LOG = 1, R_POWER = 2 };
 
 /// the sparse Laplacian of one diagram, rank order ( rows of `DiagramOut` )
 struct Csr {
-    I64 n = 0, nnz = 0;
+    I64 n = 0, nnz = 0, step = 1;       ///< entry `q` of row `i` at `row_off[ i ] + q * step`
     const int *row_off = nullptr, *row_cnt = nullptr, *col = nullptr;
     const double *val = nullptr, *dia = nullptr;
 };
@@ -44,6 +45,8 @@ struct SolveInputs {
     I64 n = 0;
     const double *positions = nullptr, *nu = nullptr, *w0 = nullptr;   ///< device, user order
     const double *options = nullptr;                                     ///< host [ 41 ]
+    const long long *dups = nullptr;                                     ///< device [ 2 nb_dups ], ( dup, rep ) user ids
+    I64 nb_dups = 0;
     double box[ 4 ] = { 0, 0, 1, 1 };
     Density dens;
 };

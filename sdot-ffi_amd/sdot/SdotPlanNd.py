@@ -9,7 +9,7 @@ reference's layouts ( `_STATS`, `_HISTORY` ) and `OtNotConverged` is raised the 
 import numpy as np
 
 from . import _native
-from .distributions import Image, SumOfGaussians, UniformBox
+from .distributions import Image, SumOfGaussians, UniformBox, _host
 
 _STATS = [ "status", "residual", "residual0", "nb_iter", "nb_diag", "nb_backtracks", "t_majorant", "t_diag", "t_asm", "t_lin", "t_lim", "eps",
            "domain_mass", "nb_overflowed", "nb_cell_lim", "nb_limit_rounds", "lin_nb_hierarchies", "lin_nb_iter", "lin_worst", "start", "t_total",
@@ -23,9 +23,58 @@ _LIN = { "auto": 0, "cholesky": 1, "amg": 2, "cg": 3, "mg": 4 }
 _STEP = { "trials": 0, "limits": 1 }
 _RESIDUAL = { "lin": 0, "log": 1, "power": 2 }
 _CONTINUATION = { "never": 0, "auto": 1, "always": 2 }
+#: exact duplicates: `w_dup = w_rep - gap`, `gap` this fraction of the larger of the squared norm and the squared extent of the
+#: cloud -- any positive gap empties a duplicate's cell, this one far above the rounding of the cuts ( `SdotPlanNd.py:119-121` )
+_DUPLICATE_GAP = 1e-6
 
 
-def card_options( settings, density, verbose = False ):
+def _mix64( z ):
+    """splitmix64's finalizer on a `uint64` array ( a bijection: distinct inputs, distinct outputs )"""
+    z = ( z ^ ( z >> np.uint64( 30 ) ) ) * np.uint64( 0xBF58476D1CE4E5B9 )
+    z = ( z ^ ( z >> np.uint64( 27 ) ) ) * np.uint64( 0x94D049BB133111EB )
+    return z ^ ( z >> np.uint64( 31 ) )
+
+
+def exact_duplicates( positions ):
+    """THE EXACT DUPLICATES of a cloud `[ n, d ]`: `( dup, rep )`, each seed whose position an earlier one already holds and
+    the smallest index at that position, both increasing with `dup` ( empty `int64` arrays when there is none; `-0.0` and
+    `0.0` alike ). Reference `src/sdot/SdotPlanNd.py:124-155`. A 64-bit print of each row's bits, sorted: when no two prints
+    are equal there is no duplicate; otherwise the rows whose prints collide are grouped exactly."""
+    P = np.ascontiguousarray( _host( positions ), dtype = np.float64 )
+    n = P.shape[ 0 ]
+    none = ( np.zeros( 0, dtype = np.int64 ), np.zeros( 0, dtype = np.int64 ) )
+    if n < 2:
+        return none
+    P = P.reshape( n, -1 ) + 0.0
+    bits = P.view( np.uint64 )
+    h = np.zeros( n, dtype = np.uint64 )
+    for k in range( bits.shape[ 1 ] ):
+        h = _mix64( h ^ bits[ :, k ] )
+    if not ( np.diff( np.sort( h ) ) == 0 ).any():
+        return none
+    order = np.argsort( h, kind = "stable" )
+    hs = h[ order ]
+    eq = hs[ 1: ] == hs[ :-1 ]
+    cand = np.unique( np.concatenate( [ order[ 1: ][ eq ], order[ :-1 ][ eq ] ] ) )
+    _, inv = np.unique( P[ cand ], axis = 0, return_inverse = True )
+    inv = inv.reshape( -1 )
+    first = np.full( int( inv.max() ) + 1, n, dtype = np.int64 )
+    np.minimum.at( first, inv, cand )
+    rep = first[ inv ]
+    keep = rep != cand
+    return cand[ keep ].astype( np.int64 ), rep[ keep ].astype( np.int64 )
+
+
+def duplicate_gap( positions ):
+    """the gap of the exact duplicates' weights ( `SdotPlanNd.py:669-674` )"""
+    P = np.asarray( _host( positions ), dtype = np.float64 )
+    P = P.reshape( P.shape[ 0 ], -1 )
+    ext = P.max( axis = 0 ) - P.min( axis = 0 )
+    s2 = max( float( ( P * P ).sum( axis = 1 ).max() ), float( ( ext * ext ).sum() ) )
+    return _DUPLICATE_GAP * ( s2 if s2 > 0 else 1.0 )
+
+
+def card_options( settings, density, verbose = False, nb_dups = 0, gap = 0.0 ):
     """the 41-double option vector ( `SdotPlanNd.py:806-833` of the reference, same names and meanings )"""
     tun = settings.tuning
     step = { "auto": "limits" }.get( tun.step, tun.step )       # ( 2D: `SdotPlanNd.py:462` )
@@ -45,8 +94,13 @@ def card_options( settings, density, verbose = False ):
     put( "residual", _RESIDUAL[ tun.residual ] ); put( "power", tun.residual_power ); put( "switch", tun.residual_switch )
     put( "lin", 0 if tun.linear_solver == "cg" else 1 ); put( "host_method", _LIN[ tun.linear_solver ] ); put( "lin_tol", tun.linear_tol or 0.0 )
     put( "lin_maxit", 0 ); put( "trace", int( bool( verbose ) ) )
+    # the multigrid: aggregates of `mg_pack` ranks ( a power of two ), `mg_nu` sweeps, K-cycle on the top `mg_kcycle` levels
+    if tun.mg_pack:
+        put( "mg_shift", int( tun.mg_pack ).bit_length() - 1 )
+    put( "mg_nu", tun.mg_nu or 0 ); put( "mg_kcycle", -1 if tun.mg_kcycle is None else tun.mg_kcycle )
     put( "continuation", _CONTINUATION[ settings.continuation ] ); put( "conv_threshold", tun.conv_threshold )
     put( "conv_s0", tun.conv_start or 0.0 ); put( "conv_ratio", tun.conv_ratio ); put( "conv_min", tun.conv_min or 0.0 )
+    put( "agg_gap", gap ); put( "agg_nb_dups", nb_dups )
     if isinstance( density, Image ):
         lo, hi = density.box()
         h = np.diag( density.frame )
@@ -97,8 +151,18 @@ class SdotPlanNd:
         box = [ lo[ 0 ], lo[ 1 ], hi[ 0 ], hi[ 1 ] ]
         if not isinstance( dst, ( Image, UniformBox, SumOfGaussians ) ):
             raise NotImplementedError( f"SdotPlanNd on the MI355X path: an Image, a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
-        opts = card_options( settings, dst, verbose )
-        res = _native.solve_2d( src.positions, src.weights, w0, opts, box, dst )
+        # THE EXACT DUPLICATES ( `SdotPlanNd.py:650-685` ): found here on the host, merged by the solve
+        dup = rep = np.zeros( 0, dtype = np.int64 )
+        gap = 0.0
+        self._dup_note = "not checked ( aggregate = False )"
+        if settings.aggregate:
+            dup, rep = exact_duplicates( src.positions )
+            self._dup_note = "checked"
+            if len( dup ):
+                gap = duplicate_gap( src.positions )
+        opts = card_options( settings, dst, verbose, len( dup ), gap )
+        pairs = np.stack( [ dup, rep ], axis = 1 ) if len( dup ) else None
+        res = _native.solve_2d( src.positions, src.weights, w0, opts, box, dst, dups = pairs )
         #: the fitted weights ( `weights[ 0 ] == 0` ), device tensor, user order
         self.weights = res[ "weights" ]
         self._weights_fit = self.weights
@@ -122,8 +186,14 @@ class SdotPlanNd:
         for name in ( "nb_iter", "nb_diag", "nb_backtracks", "nb_overflowed", "nb_cell_lim", "nb_limit_rounds", "lin_nb_hierarchies",
                       "lin_nb_iter", "nb_continuation_steps", "scratch_bytes", "nb_clusters", "nb_aggregated", "nb_duplicates", "nb_polish" ):
             self.stats[ name ] = int( self.stats[ name ] )
-        self.stats[ "aggregation" ] = "off" if not settings.aggregate else "none"
-        self.stats[ "duplicates" ] = "not checked ( this path )"
+        if not settings.aggregate:
+            self.stats[ "aggregation" ] = "off"
+        elif self.stats[ "nb_clusters" ] == 0:
+            self.stats[ "aggregation" ] = "none"
+        else:
+            self.stats[ "aggregation" ] = ( f"{ self.stats[ 'nb_aggregated' ] } seeds in { self.stats[ 'nb_clusters' ] } clusters "
+                                            f"( { self.stats[ 'nb_duplicates' ] } exact duplicates ), re-split in { self.stats[ 'nb_polish' ] } diagrams" )
+        self.stats[ "duplicates" ] = self._dup_note
         self.stats[ "warm_start" ] = self._warm_start
 
     def _read_history( self, rows ):

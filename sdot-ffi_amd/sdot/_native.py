@@ -161,8 +161,9 @@ def measures_2d( positions, weights, box, density, facets = False, events = Fals
     return res[ 0 ] if len( res ) == 1 else tuple( res )
 
 
-def solve_2d( positions, nu, w0, options, box, density ):
-    """one solve; returns a dict of device tensors and host arrays ( see `sdot_amd_solve_2d` )"""
+def solve_2d( positions, nu, w0, options, box, density, dups = None ):
+    """one solve; returns a dict of device tensors and host arrays ( see `sdot_amd_solve_2d` ); `dups`: the exact
+    duplicates, `[ k, 2 ]` rows `( dup, rep )` of user ids"""
     T = torch()
     pos = on_device( positions ).reshape( -1, 2 ).contiguous()
     n = pos.shape[ 0 ]
@@ -181,7 +182,9 @@ def solve_2d( positions, nu, w0, options, box, density ):
     nb_steps = np.zeros( 1, np.int64 )
     stats = np.zeros( 31 )
     cost = np.zeros( 1 )
-    _check( lib().sdot_amd_solve_2d( _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), None, 0, optp, boxp, kind, rho, _ptr( img ),
+    dp = None if dups is None or len( dups ) == 0 else on_device( np.asarray( dups, dtype = np.int64 ).reshape( -1 ), dtype = T.int64 )
+    _check( lib().sdot_amd_solve_2d( _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), _ptr( dp ), 0 if dp is None else dp.numel(), optp, boxp,
+                                     kind, rho, _ptr( img ),
                                      _ptr( weights ), hist.ctypes.data_as( _P ), nb_steps.ctypes.data_as( _P ), stats.ctypes.data_as( _P ),
                                      _ptr( masses ), _ptr( bary ), cost.ctypes.data_as( _P ), _ptr( clusters ) ) )
     return dict( weights = weights, cell_masses = masses, barycenters = bary, clusters = clusters, cost = float( cost[ 0 ] ),
@@ -215,7 +218,8 @@ def refresh_majorants_2d( tree, sorted_positions, sorted_weights ):
     return wa, wb
 
 
-def linear_solves_2d( row, col, val, dia, rhs, tol = 1e-6 ):
+def linear_solves_2d( row, col, val, dia, rhs, tol = 1e-6, method = "cg", kcycle = -1, nu = 0 ):
+    """`L d_j = rhs[ j ]` ( `SdotPlanNd._card_linear_solves` of the reference: `method` "cg" or "mg" )"""
     T = torch()
     row = on_device( row, T.int64 )
     col = on_device( col, T.int64 )
@@ -227,7 +231,9 @@ def linear_solves_2d( row, col, val, dia, rhs, tol = 1e-6 ):
     k, n = rhs.shape
     sol = T.empty( ( k, n ), dtype = T.float64, device = device() )
     its = np.zeros( k )
-    opts, optp = _host( [ n, k, 0, tol, -1, 0, 0, 0, -1, 0 ] )
+    if method not in ( "cg", "mg" ):
+        raise ValueError( f"linear_solves_2d: method 'cg' or 'mg' ( not { method!r} )" )
+    opts, optp = _host( [ n, k, 0 if method == "cg" else 1, tol, -1, 0, 0, 0, kcycle, nu ] )
     _check( lib().sdot_amd_linear_solves_2d( _stream(), n, _ptr( row ), _ptr( col ), _ptr( val ), _ptr( dia ), _ptr( rhs.contiguous() ), k,
                                              optp, _ptr( sol ), its.ctypes.data_as( _P ) ) )
     return sol, its.astype( int )

@@ -138,3 +138,58 @@ def bumps_image( nx, ny, rng, k = 8, floor = 0.1, sig = ( 0.03, 0.15 ) ):
     for q in range( k ):
         v += np.exp( -( ( X - c[ q, 0 ] ) ** 2 + ( Y - c[ q, 1 ] ) ** 2 ) / ( 2 * s[ q ] ** 2 ) )
     return v
+
+
+def degenerate_clouds( seed = 17, n = 400 ):
+    """`{ name: ( positions, groups ) }`: a uniform cloud with seeds moved exactly onto others; `groups` the expected
+    clusters as `( members, exact )` ( reference tests/test_SdotPlanNd.py:1377-1404, its exact cases )"""
+    rng = np.random.default_rng( seed )
+    base = rng.uniform( 0.05, 0.95, size = ( n, 2 ) )
+
+    def moved( *groups ):
+        pos = base.copy()
+        for members in groups:
+            for m in members[ 1: ]:
+                pos[ m ] = pos[ members[ 0 ] ]
+        return pos, [ ( list( g ), True ) for g in groups ]
+
+    return { "exact pair": moved( [ 5, 7 ] ), "exact triple": moved( [ 3, 11, 29 ] ), "two pairs": moved( [ 1, 6 ], [ 12, 15 ] ) }
+
+
+def brute_duplicates( pos ):
+    """the exact duplicates by comparing every pair: `( dup, rep )` rows, `rep` the smallest index at that position"""
+    n = len( pos )
+    out = []
+    for i in range( n ):
+        same = np.nonzero( ( pos[ :i ] == pos[ i ] ).all( axis = 1 ) )[ 0 ]
+        if len( same ):
+            out.append( ( i, int( same[ 0 ] ) ) )
+    return np.asarray( out, dtype = np.int64 ).reshape( -1, 2 )
+
+
+def duplicate_gap( pos ):
+    """`1e-6` of the larger of the squared norm and the squared extent of the cloud ( reference src/sdot/SdotPlanNd.py:671-674 )"""
+    ext = pos.max( axis = 0 ) - pos.min( axis = 0 )
+    s2 = max( float( ( pos * pos ).sum( axis = 1 ).max() ), float( ( ext * ext ).sum() ) )
+    return 1e-6 * ( s2 if s2 > 0 else 1.0 )
+
+
+def check_aggregated( name, st, weights, masses, nu, clusters, groups, tol ):
+    """a plan of a cloud with exact duplicates ( reference tests/test_SdotPlanNd.py:1407-1433 ): converged, each cluster's
+    mass its target ( `k` members: `k tol` ), a duplicate's cell empty and its weight under its representative's, every other
+    cell its own target"""
+    n = len( nu )
+    assert st[ "status" ] in ( 1, "converged" ), ( name, st[ "status" ] )
+    assert st[ "residual" ] <= tol, ( name, st[ "residual" ] )
+    alone = np.ones( n, dtype = bool )
+    for members, _ in groups:
+        if clusters is not None:
+            assert ( clusters[ members ] == min( members ) ).all(), ( name, members, clusters[ members ] )
+        alone[ members ] = False
+        assert abs( masses[ members ].sum() - nu[ members ].sum() ) <= len( members ) * tol * ( 1 + 1e-9 ), name
+        assert ( masses[ members[ 1: ] ] == 0 ).all() and ( weights[ members[ 1: ] ] < weights[ members[ 0 ] ] ).all(), name
+    if clusters is not None:
+        assert ( clusters[ alone ] == np.arange( n )[ alone ] ).all(), name
+    assert np.abs( masses[ alone ] - nu[ alone ] ).max() <= tol, name
+    assert st[ "nb_clusters" ] == len( groups ) and st[ "nb_aggregated" ] == sum( len( g ) for g, _ in groups ), name
+    assert st[ "nb_duplicates" ] == sum( len( g ) - 1 for g, _ in groups ), name

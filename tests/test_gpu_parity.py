@@ -197,14 +197,45 @@ def test_linear_solves_match_a_dense_solve( native, oracle ):
     L[ np.arange( n ), np.arange( n ) ] = dia
     b = rng.normal( size = ( 2, n ) )
     b -= b.mean( 1, keepdims = True )
-    say( "case linear" )
-    sol, its = native.linear_solves_2d( row, fj, fc, dia, b, tol = 1e-12 )
-    sol = sol.cpu().numpy()
-    for k in range( 2 ):
-        ref = np.zeros( n )
-        ref[ 1: ] = np.linalg.solve( L[ 1:, 1: ], b[ k, 1: ] )
-        assert its[ k ] > 0 and sol[ k, 0 ] == 0
-        assert np.abs( sol[ k ] - ref ).max() < 1e-9 * np.abs( ref ).max()
+    for method in ( "cg", "mg" ):
+        say( "case linear", method )
+        sol, its = native.linear_solves_2d( row, fj, fc, dia, b, tol = 1e-12, method = method )
+        sol = sol.cpu().numpy()
+        for k in range( 2 ):
+            ref = np.zeros( n )
+            ref[ 1: ] = np.linalg.solve( L[ 1:, 1: ], b[ k, 1: ] )
+            assert its[ k ] > 0 and sol[ k, 0 ] == 0, method
+            assert np.abs( sol[ k ] - ref ).max() < 1e-9 * np.abs( ref ).max(), method
+
+
+def test_multigrid_solves_a_large_laplacian( native, oracle ):
+    # the K-cycle multigrid on the Laplacian of a weighted 100k diagram: the relative residual it claims, the
+    # Jacobi-PCG's solution, and far fewer iterations ( reference test_SdotPlanNd.py:680-727 checks the residual )
+    import scipy.sparse as sps
+    rng = np.random.default_rng( 14 )
+    n = 100000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    pos = pos[ oracle.build_tree( pos )[ "order" ] ]      # ( in tree order: the aggregates of consecutive rows are compact )
+    w = rng.normal( 0, 1e-10, n )
+    _, ( fi, fj, fc ) = oracle.measures( pos, w, facets = True )
+    row = np.zeros( n + 1, np.int64 )
+    np.add.at( row, fi + 1, 1 )
+    row = np.cumsum( row )
+    dia = np.zeros( n )
+    np.add.at( dia, fi, fc )
+    assert np.all( dia > 0 )                             # ( no empty cell: the graph is connected, `b` of zero sum is in the range )
+    L = sps.csr_matrix( ( -fc, ( fi, fj ) ), shape = ( n, n ) ) + sps.diags( dia )
+    b = rng.normal( size = ( 1, n ) )
+    b -= b.mean()
+    say( "case multigrid 100k" )
+    xm, im = native.linear_solves_2d( row, fj, fc, dia, b, tol = 1e-10, method = "mg" )
+    xc, ic = native.linear_solves_2d( row, fj, fc, dia, b, tol = 1e-10, method = "cg" )
+    xm, xc = xm.cpu().numpy()[ 0 ], xc.cpu().numpy()[ 0 ]
+    # the residual in the gauge: row 0 is replaced by x_0 = 0, so compare on the other rows
+    res = ( L @ xm - b[ 0 ] )[ 1: ]
+    assert np.linalg.norm( res ) <= 3e-10 * np.linalg.norm( b[ 0 ] )
+    assert np.abs( xm - xc ).max() <= 1e-7 * np.abs( xc ).max()
+    assert 0 < im[ 0 ] < ic[ 0 ] / 10
 
 
 def _solve_both( oracle, pos, density_gpu, density_oracle, tol, lin_tol, step = "trials", continuation = "never", maxit = 100 ):
@@ -359,6 +390,44 @@ def test_gaussian_solve_matches_the_oracle( oracle, native ):
         assert np.abs( w - ro[ "weights" ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max()
         assert np.abs( sol.barycenters.cpu().numpy() - ro[ "barycenters" ] ).max() < 1e-10
         assert abs( sol.cost - ro[ "cost" ] ) < 1e-10 * ro[ "cost" ]
+
+
+@pytest.mark.parametrize( "step,lin", [ ( "trials", "cg" ), ( "limits", "cg" ), ( "limits", "mg" ) ] )
+def test_exact_duplicates_match_the_oracle( oracle, native, step, lin ):
+    # exact duplicates merged ( reference tests/test_SdotPlanNd.py:1436-1452 ): each duplicate's cell empty, its weight
+    # `k gap` under its representative's, the cluster's mass its target; the oracle's iterations and weights
+    from sdot import Iterative, OtProblem, SumOfDiracs, Tuning, UniformBox
+    for name, ( pos, groups ) in H.degenerate_clouds().items():
+        n = len( pos )
+        tol = 1e-11 / n
+        say( "case duplicates", name, step, lin )
+        it = Iterative( tol = tol, max_iter = 60, continuation = "never", tuning = Tuning( step = step, linear_solver = lin, linear_tol = 1e-13 ) )
+        sol = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( it )
+        st = sol.stats
+        w = sol.weights.cpu().numpy()
+        m = sol.cell_masses.cpu().numpy()
+        H.check_aggregated( name, st, w, m, np.full( n, 1 / n ), sol.clusters.cpu().numpy(), groups, tol )
+        assert st[ "status" ] == "converged" and st[ "duplicates" ] == "checked" and "exact duplicates" in st[ "aggregation" ]
+        gap = H.duplicate_gap( pos )
+        for members, _ in groups:
+            for k, mm in enumerate( members[ 1: ], 1 ):
+                assert w[ mm ] == w[ members[ 0 ] ] - k * gap
+        ro = oracle.solve( pos, tol = tol, maxit = 60, step = step, dups = H.brute_duplicates( pos ), gap = gap, lin_tol = 1e-13 )
+        assert ro[ "stats" ][ "status" ] == 1
+        if lin == "cg":                                  # ( the multigrid's directions differ within the linear tolerance )
+            assert st[ "nb_iter" ] == ro[ "stats" ][ "nb_iter" ] and st[ "nb_diag" ] == ro[ "stats" ][ "nb_diag" ], name
+        assert np.abs( w - ro[ "weights" ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max(), name
+
+
+def test_without_the_merge_duplicates_do_not_converge( native ):
+    # two equal points cannot split a cell ( reference tests/test_SdotPlanNd.py:1449-1452 )
+    from sdot import Iterative, OtProblem, SumOfDiracs, Tuning, UniformBox
+    pos, _ = H.degenerate_clouds()[ "exact pair" ]
+    say( "case duplicates without the merge" )
+    it = Iterative( tol = 1e-11 / len( pos ), max_iter = 30, aggregate = False, on_failure = "ignore", continuation = "never",
+                    tuning = Tuning( linear_solver = "cg" ) )
+    off = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( it )
+    assert not off.converged and off.stats[ "aggregation" ] == "off" and off.clusters is None
 
 
 def test_solve_is_bitwise_reproducible( native ):

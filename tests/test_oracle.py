@@ -554,3 +554,57 @@ def test_gaussian_continuation_widens_the_gaussians( oracle ):
     q = oracle.solve( pos, density = _gauss( g ), tol = 1e-11, step = "limits", continuation = "never", lin_tol = 1e-12, maxit = 200 )
     assert q[ "stats" ][ "status" ] == 1
     assert np.abs( q[ "weights" ] - r[ "weights" ] ).max() < 1e-7 * np.abs( q[ "weights" ] ).max()
+
+
+# ---- exact duplicates ( reference sdotplan/Aggregation.h:107-220, tests/test_SdotPlanNd.py:1377-1470 ) --------------------
+
+
+@pytest.mark.parametrize( "step", [ "trials", "limits" ] )
+@pytest.mark.parametrize( "name", [ "exact pair", "exact triple", "two pairs" ] )
+def test_exact_duplicates_are_merged( oracle, name, step ):
+    # each duplicate's weight tied under its representative's, its target carried by it: converged, the clusters' masses
+    # their targets ( reference tests/test_SdotPlanNd.py:1436-1452 )
+    pos, groups = H.degenerate_clouds()[ name ]
+    n = len( pos )
+    tol = 1e-11 / n
+    dups = H.brute_duplicates( pos )
+    r = oracle.solve( pos, tol = tol, maxit = 60, step = step, dups = dups, gap = H.duplicate_gap( pos ), lin_tol = 1e-13 )
+    nu = np.full( n, 1.0 / n )
+    H.check_aggregated( f"{ name } { step }", r[ "stats" ], r[ "weights" ], r[ "masses" ], nu, None, groups, tol )
+    # the k-th duplicate of a representative sits `k gap` under it ( Aggregation.h:111-117 )
+    gap = H.duplicate_gap( pos )
+    for members, _ in groups:
+        for k, m in enumerate( members[ 1: ], 1 ):
+            assert r[ "weights" ][ m ] == r[ "weights" ][ members[ 0 ] ] - k * gap
+    assert r[ "stats" ][ "residual_full" ] <= 2 * tol
+
+
+def test_exact_duplicates_solve_the_merged_problem( oracle ):
+    # the representatives' weights are those of the problem without the duplicates, whose targets their representatives
+    # carry: the same cells, the same equations ( an invariant of the merge, independent of how it is implemented )
+    pos, groups = H.degenerate_clouds()[ "exact triple" ]
+    n = len( pos )
+    rng = np.random.default_rng( 5 )
+    nu = rng.uniform( 0.5, 1.5, n )
+    nu /= nu.sum()
+    dups = H.brute_duplicates( pos )
+    r = oracle.solve( pos, nu = nu, tol = 1e-14, maxit = 60, dups = dups, gap = H.duplicate_gap( pos ), lin_tol = 1e-13 )
+    keep = np.setdiff1d( np.arange( n ), dups[ :, 0 ] )
+    nu_m = nu.copy()
+    for d, p in dups:
+        nu_m[ p ] += nu[ d ]
+    q = oracle.solve( pos[ keep ], nu = nu_m[ keep ], tol = 1e-14, maxit = 60, lin_tol = 1e-13 )
+    assert r[ "stats" ][ "status" ] == 1 and q[ "stats" ][ "status" ] == 1
+    assert np.abs( r[ "weights" ][ keep ] - q[ "weights" ] ).max() < 1e-9
+    assert np.abs( r[ "masses" ][ keep ] - q[ "masses" ] ).max() < 1e-13
+    assert ( r[ "masses" ][ dups[ :, 0 ] ] == 0 ).all()
+    # the cost is blind to the duplicates: each of them holds nothing
+    assert abs( r[ "cost" ] - q[ "cost" ] ) < 1e-9 * q[ "cost" ]
+
+
+def test_without_the_merge_the_duplicates_cannot_converge( oracle ):
+    # two equal points cannot split a cell: without the merge, no weight vector reaches the targets
+    # ( reference tests/test_SdotPlanNd.py:1449-1452 )
+    pos, _ = H.degenerate_clouds()[ "exact pair" ]
+    r = oracle.solve( pos, tol = 1e-11 / len( pos ), maxit = 30 )
+    assert r[ "stats" ][ "status" ] != 1
