@@ -85,7 +85,10 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
  *   dups         DEVICE [ 2 k ]     exact duplicates, ( dup, rep ) user-id pairs, 0 <= rep < dup, each dup once and
  *                                   no rep a dup; the first `agg_nb_dups` pairs are read ( `nb_dup_values` >= 2 x that ).
  *                                   Each duplicate's weight is tied `j agg_gap` under its representative's ( its j-th ),
- *                                   its cell left empty, its target carried by the representative; may be null
+ *                                   its cell left empty, its target carried by the representative; may be null.
+ *                                   With `agg_margin` > 0 the solve also merges the near-coincident pairs it finds
+ *                                   on its diagrams ( src/sdot/include/sdot/sdotplan/Aggregation.h ); `clusters` and
+ *                                   `stats` report both kinds of merge
  *   options      HOST   [ 41 ]      the reference card's option vector, same order
  *                                   ( src/sdot/SdotPlanNd.py:197-201 ); the image geometry is in
  *                                   img_x0 .. img_ny as there, the Gaussians' count in img_nx

@@ -192,10 +192,10 @@ HISTORY = [ "step", "t", "residual_l2", "min_measure", "max_abs_residual", "nb_d
 def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = None, tol = 1e-9, tol_rel = 0.0, maxit = 100, max_backtracks = 60,
            t_min = 1e-10, mult_ok = 4.0, residual = 1, power = 0.5, switch = 2.0, lin_tol = 1e-12, lin_maxit = 20000, keep_start = False,
            start_box = None, step = "trials", factor = 0.9, continuation = "never", conv_threshold = 1e-2, conv_s0 = 0.0,
-           conv_ratio = 2 ** 0.5, conv_min = 0.0, dups = None, gap = 0.0 ):
+           conv_ratio = 2 ** 0.5, conv_min = 0.0, dups = None, gap = 0.0, margin = 0.0 ):
     """the damped-Newton solve ( step "trials" or "limits"; continuation "never", "auto" or "always" ); `dups`: the exact
-    duplicates, `( dup, rep )` rows of user ids, whose weights are tied at `w_rep - k gap`; returns a dict
-    ( weights, masses, barycenters, cost, stats, history )"""
+    duplicates, `( dup, rep )` rows of user ids, whose weights are tied at `w_rep - k gap`; `margin`: the aggregation of
+    near-coincident pairs ( 0: off ); returns a dict ( weights, masses, barycenters, cost, stats, history, clusters )"""
     pos = _f64( pos )
     n = pos.shape[ 0 ]
     nu = np.full( n, 1.0 / n ) if nu is None else _f64( nu )
@@ -203,7 +203,7 @@ def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = No
     kind, value, nx, ny, geo, img, keep = _density_args( density )
     opts = _f64( [ tol, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol, lin_maxit, int( keep_start ),
                    { "trials": 0, "limits": 1 }[ step ], factor, { "never": 0, "auto": 1, "always": 2 }[ continuation ], conv_threshold,
-                   conv_s0, conv_ratio, conv_min, gap, 0 if dups is None else len( dups ) ] )
+                   conv_s0, conv_ratio, conv_min, gap, 0 if dups is None else len( dups ), margin ] )
     dp = None if dups is None or len( dups ) == 0 else np.ascontiguousarray( np.asarray( dups, dtype = np.int64 ).reshape( -1, 2 ) )
     sb = _f64( [ 0, 0, 0, 0, 0 ] if start_box is None else [ 1, *start_box[ 0 ], *start_box[ 1 ] ] )
     weights = np.zeros( n )
@@ -212,8 +212,10 @@ def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = No
     masses = np.zeros( n )
     bary = np.zeros( ( n, 2 ) )
     cost = np.zeros( 1 )
+    clusters = np.zeros( n, dtype = np.int64 )
     nb = lib().oracle_solve( ctypes.c_longlong( n ), _d( pos ), _d( nu ), _d( w0 ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
                              ctypes.c_double( value ), ctypes.c_int( nx ), ctypes.c_int( ny ), _d( geo ), _d( img ), _d( opts ), _d( sb ),
-                             None if dp is None else dp.ctypes.data_as( ctypes.POINTER( ctypes.c_longlong ) ), _d( weights ), _d( stats ), _d( hist ), _d( masses ), _d( bary ), _d( cost ) )
+                             None if dp is None else dp.ctypes.data_as( ctypes.POINTER( ctypes.c_longlong ) ), _d( weights ), _d( stats ), _d( hist ), _d( masses ), _d( bary ), _d( cost ),
+                             clusters.ctypes.data_as( ctypes.POINTER( ctypes.c_longlong ) ) )
     return dict( weights = weights, masses = masses, barycenters = bary, cost = float( cost[ 0 ] ),
-                 stats = dict( zip( STATS, stats.tolist() ) ), history = hist[ :nb ].copy() )
+                 stats = dict( zip( STATS, stats.tolist() ) ), history = hist[ :nb ].copy(), clusters = clusters )

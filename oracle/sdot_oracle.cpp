@@ -23,10 +23,12 @@
 //       `sdotplan/Solve.h:112-371`, `sdotplan/Report.h:15-31`.
 //
 // Geometry runs in the SEED'S FRAME ( a vertex is stored as `x - p0` ): the half-space of the
-// bisector of ( p0, w0 ) and ( q, wq ) is then `d . v <= ( |d|^2 + w0 - wq ) / 2`, `d = q - p0`,
+// bisector of ( p0, w0 ) and ( q, wq ) is then `d . v <= ( ( w0 - wq ) + |d|^2 ) / 2`, `d = q - p0`,
 // which is the reference's `Plane.h:39` half-space translated by `p0` ( the reference card does
-// the same, `gpu/Cell2D.cuh:139-160` ). The HIP library uses the same frame, the same walk and the
-// same cut, so its cells agree with these to rounding.
+// the same, `gpu/Cell2D.cuh:139-160` ). The weights' difference is taken first: `|d|^2` added to a
+// weight would be lost for a pair far closer than `sqrt( |w| )`, and the two cells of such a pair
+// would then leave a gap of width `|d|` between them. The HIP library uses the same frame, the same
+// walk and the same cut, so its cells agree with these to rounding.
 //
 // Parity pins ( the tests that keep this restatement honest, all in `tests/` ):
 //   tree           shape = f( n ) only, every seed once, boxes contain their subtree, majorant
@@ -508,7 +510,7 @@ bool make_cell( const Tree &t, I64 k0, const double *lo, const double *hi, Poly 
             for ( I64 k = b; k < e; ++k ) {
                 if ( k == k0 ) continue;
                 const double dx = t.px[ size_t( k ) ] - p0x, dy = t.py[ size_t( k ) ] - p0y;
-                const double off = ( dx * dx + dy * dy + w0 - t.w[ size_t( k ) ] ) / 2;
+                const double off = ( ( w0 - t.w[ size_t( k ) ] ) + ( dx * dx + dy * dy ) ) / 2;
                 if ( cut( c, dx, dy, off, int( k ), ev ) == EMPTY ) return false;
             }
             continue;
@@ -533,7 +535,7 @@ bool make_cell_all( const Tree &t, I64 k0, const double *lo, const double *hi, P
     for ( I64 k = 0; k < t.n; ++k ) {
         if ( k == k0 ) continue;
         const double dx = t.px[ size_t( k ) ] - p0x, dy = t.py[ size_t( k ) ] - p0y;
-        const double off = ( dx * dx + dy * dy + w0 - t.w[ size_t( k ) ] ) / 2;
+        const double off = ( ( w0 - t.w[ size_t( k ) ] ) + ( dx * dx + dy * dy ) ) / 2;
         if ( cut( c, dx, dy, off, int( k ), nullptr ) == EMPTY ) return false;
     }
     return true;
@@ -1275,43 +1277,182 @@ struct Solver {
     std::vector<double> nu, w, a, d;
     std::vector<Facet> fa;
     Edges ea;                                            ///< the accepted diagram's edges
-    // THE EXACT DUPLICATES ( user indices, `sdotplan/Aggregation.h:107-220`, `gpu/Newton2D.cuh:340-366` ): the k-th duplicate
-    // of a representative has `w_dup = w_rep - k gap` ( its cell empty ), its target carried by the representative ( `nue` );
-    // the tests read each member's share of its cluster's mass, `nu_m a_rep / nue_rep`
+    // THE AGGREGATION ( user indices, `sdotplan/Aggregation.h:65-320` ). Clusters are the union-find classes of more than one
+    // seed, numbered by their smallest index, members increasing; the tests and the merit read each member's share of its
+    // cluster's mass, `nu_m sum_C a / sum_C nu`.
+    // * EXACT DUPLICATES ( given ): the k-th duplicate of a representative has `w_dup = w_rep - k gap` ( its cell empty ), its
+    //   target carried by the representative ( `nue` ), its row of the Laplacian linked to the representative's;
+    // * NEAR-COINCIDENT PAIRS ( found on the diagrams, `detect` ): merged when one ulp of their weights moves more than
+    //   `tau / margin` of mass between them; latched. Once the aggregated problem converged, the clusters' members are re-split
+    //   by local Newton steps on their `k x k` blocks ( `resplit` ).
     std::vector<I64> dup, rep, dup_rep;                  ///< the pairs; per seed its representative, -1 if not a duplicate
     std::vector<double> dup_gap, nue;
-    std::vector<char> member;                            ///< per seed: in a cluster ( a representative or a duplicate )
     double c_link = 1;                                   ///< the coefficient of the facet linking a duplicate to its representative
+    double margin = 0, tau_min = 0;                      ///< the criterion's `kappa` ( 0: no detection ), `tau` of the smallest target
+    std::vector<I64> parent, cl_begin, cl_members, cluster_of;
+    I64 nb_merged_pairs = 0;
+    int nb_polish = 0;
+    I64 find( I64 i ) {
+        while ( parent[ sz( i ) ] != i ) { parent[ sz( i ) ] = parent[ sz( parent[ sz( i ) ] ) ]; i = parent[ sz( i ) ]; }
+        return i;
+    }
+    bool unite( I64 i, I64 j, I64 n ) {
+        if ( parent.empty() ) { parent.resize( sz( n ) ); for ( I64 k = 0; k < n; ++k ) parent[ sz( k ) ] = k; }
+        i = find( i ); j = find( j );
+        if ( i == j ) return false;
+        parent[ sz( std::max( i, j ) ) ] = std::min( i, j );
+        return true;
+    }
+    void rebuild( I64 n ) {
+        cl_begin.clear(); cl_members.clear();
+        cluster_of.assign( sz( n ), -1 );
+        if ( parent.empty() ) return;
+        std::vector<I64> size( sz( n ), 0 ), id( sz( n ), -1 );
+        for ( I64 i = 0; i < n; ++i ) ++size[ sz( find( i ) ) ];
+        cl_begin.push_back( 0 );
+        for ( I64 i = 0; i < n; ++i )
+            if ( parent[ sz( i ) ] == i && size[ sz( i ) ] > 1 ) { id[ sz( i ) ] = I64( cl_begin.size() ) - 1; cl_begin.push_back( cl_begin.back() + size[ sz( i ) ] ); }
+        cl_members.resize( sz( cl_begin.back() ) );
+        std::vector<I64> at( cl_begin.begin(), cl_begin.end() - 1 );
+        for ( I64 i = 0; i < n; ++i ) {
+            const I64 c = id[ sz( find( i ) ) ];
+            if ( c < 0 ) continue;
+            cluster_of[ sz( i ) ] = c;
+            cl_members[ sz( at[ sz( c ) ]++ ) ] = i;
+        }
+    }
+    I64 nb_clusters() const { return cl_begin.empty() ? 0 : I64( cl_begin.size() ) - 1; }
+    bool any() const { return nb_clusters() > 0; }
     void set_duplicates( const std::vector<I64> &d, const std::vector<I64> &r, double gap, I64 n ) {
         dup = d; rep = r;
         dup_gap.clear();
         if ( d.empty() ) return;
         dup_rep.assign( sz( n ), -1 );
-        member.assign( sz( n ), 0 );
         std::vector<int> count( sz( n ), 0 );
         for ( size_t q = 0; q < d.size(); ++q ) {
             dup_rep[ sz( d[ q ] ) ] = r[ q ];
             dup_gap.push_back( gap * double( ++count[ sz( r[ q ] ) ] ) );
-            member[ sz( d[ q ] ) ] = member[ sz( r[ q ] ) ] = 1;
+            unite( d[ q ], r[ q ], n );
         }
+        rebuild( n );
     }
     bool is_dup( size_t i ) const { return ! dup_rep.empty() && dup_rep[ i ] >= 0; }
-    /// `nue` for the current `nu` ( `Aggregation::prepare` )
+    /// the residual a seed of target `nu_i` must reach ( `Aggregation::tau` )
+    double tau( double nu_i ) const { return std::max( { o.tol_abs, o.tol_rel * nu_i, 1e-12 * nu_i } ); }
+    /// `nue` and `tau_min` for the current `nu` ( `Aggregation::prepare` )
     void prepare() {
+        double nm = nu.empty() ? 0 : nu[ 0 ];
+        for ( double v : nu ) nm = std::min( nm, v );
+        tau_min = tau( nm );
         if ( dup.empty() ) { nue.clear(); return; }
         nue = nu;
         for ( size_t q = 0; q < dup.size(); ++q ) { nue[ sz( rep[ q ] ) ] += nu[ sz( dup[ q ] ) ]; nue[ sz( dup[ q ] ) ] = 0; }
     }
-    /// the masses the tests read: a member's share of its cluster ( `Aggregation::shares` )
+    const std::vector<double> &targets() const { return dup.empty() ? nu : nue; }
+    /// the masses the tests read: each member's share of its cluster's mass ( `Aggregation::shares` )
     const std::vector<double> &shares( const std::vector<double> &A, std::vector<double> &ae ) const {
-        if ( dup.empty() ) return A;
+        if ( ! any() ) return A;
         ae = A;
-        for ( size_t i = 0; i < A.size(); ++i ) {
-            if ( ! member[ i ] ) continue;
-            const size_t r = is_dup( i ) ? sz( dup_rep[ i ] ) : i;
-            ae[ i ] = nu[ i ] * ( nue[ r ] > 0 ? A[ r ] / nue[ r ] : 0.0 );
+        for ( I64 c = 0; c < nb_clusters(); ++c ) {
+            double sa = 0, sn = 0;
+            for ( I64 k = cl_begin[ sz( c ) ]; k < cl_begin[ sz( c + 1 ) ]; ++k ) { sa += A[ sz( cl_members[ sz( k ) ] ) ]; sn += nu[ sz( cl_members[ sz( k ) ] ) ]; }
+            const double r = sn > 0 ? sa / sn : 0;
+            for ( I64 k = cl_begin[ sz( c ) ]; k < cl_begin[ sz( c + 1 ) ]; ++k ) ae[ sz( cl_members[ sz( k ) ] ) ] = nu[ sz( cl_members[ sz( k ) ] ) ] * r;
         }
         return ae;
+    }
+    /// the spacing of the doubles at `x`
+    static double ulp_of( double x ) { x = std::fabs( x ); return std::nextafter( x, INFINITY ) - x; }
+    /// THE DETECTION ( `Aggregation::detect` ) on the Laplacian of the accepted diagram and its weights: `true` if a cluster
+    /// was made or grew. The scan is skipped while no diagonal passes the bound `tau_min / ( margin ulp( max |w| ) )`
+    bool detect( const Laplacian &L, const std::vector<double> &W ) {
+        if ( ! ( margin > 0 ) || ! ( tau_min > 0 ) ) return false;
+        double wmax = 0, dmax = 0;
+        for ( double x : W ) wmax = std::max( wmax, std::fabs( x ) );
+        const double c_star = tau_min / ( margin * ulp_of( wmax ) );
+        for ( I64 i = 0; i < L.n; ++i ) dmax = std::max( dmax, L.dia[ sz( i ) ] );
+        if ( ! ( dmax > c_star ) ) return false;
+        bool changed = false;
+        for ( I64 i = 0; i < L.n; ++i ) {
+            if ( ! ( L.dia[ sz( i ) ] > c_star ) ) continue;
+            for ( I64 e = L.row[ sz( i ) ]; e < L.row[ sz( i + 1 ) ]; ++e ) {
+                const I64 j = L.col[ sz( e ) ];
+                if ( j <= i || ! ( L.c[ sz( e ) ] > c_star ) ) continue;
+                const double m = L.c[ sz( e ) ] * ulp_of( std::max( std::fabs( W[ sz( i ) ] ), std::fabs( W[ sz( j ) ] ) ) );
+                if ( m * margin > tau( std::min( nu[ sz( i ) ], nu[ sz( j ) ] ) ) && unite( i, j, L.n ) ) { changed = true; ++nb_merged_pairs; }
+            }
+        }
+        if ( changed ) rebuild( L.n );
+        return changed;
+    }
+    /// the members of cluster `c` the re-splitting moves ( the duplicates left out ), increasing
+    void movable_members( I64 c, std::vector<I64> &mem ) const {
+        mem.clear();
+        for ( I64 k = cl_begin[ sz( c ) ]; k < cl_begin[ sz( c + 1 ) ]; ++k )
+            if ( ! is_dup( sz( cl_members[ sz( k ) ] ) ) ) mem.push_back( cl_members[ sz( k ) ] );
+    }
+    /// the members' largest residual ( the duplicates left out )
+    double members_residual( const std::vector<double> &A ) const {
+        const std::vector<double> &N = targets();
+        double worst = 0;
+        for ( I64 i : cl_members ) if ( ! is_dup( sz( i ) ) ) worst = std::max( worst, std::fabs( A[ sz( i ) ] - N[ sz( i ) ] ) );
+        return worst;
+    }
+    /// `M x = r` for a symmetric positive definite `k x k` `M` ( row-major, overwritten ) by Cholesky; `false` if not definite
+    static bool cholesky( I64 k, std::vector<double> &M, std::vector<double> &r ) {
+        for ( I64 j = 0; j < k; ++j ) {
+            double dj = M[ sz( j * k + j ) ];
+            for ( I64 q = 0; q < j; ++q ) dj -= M[ sz( j * k + q ) ] * M[ sz( j * k + q ) ];
+            if ( ! ( dj > 0 ) ) return false;
+            const double lj = std::sqrt( dj );
+            M[ sz( j * k + j ) ] = lj;
+            for ( I64 i = j + 1; i < k; ++i ) {
+                double v = M[ sz( i * k + j ) ];
+                for ( I64 q = 0; q < j; ++q ) v -= M[ sz( i * k + q ) ] * M[ sz( j * k + q ) ];
+                M[ sz( i * k + j ) ] = v / lj;
+            }
+        }
+        for ( I64 i = 0; i < k; ++i ) {
+            double v = r[ sz( i ) ];
+            for ( I64 q = 0; q < i; ++q ) v -= M[ sz( i * k + q ) ] * r[ sz( q ) ];
+            r[ sz( i ) ] = v / M[ sz( i * k + i ) ];
+        }
+        for ( I64 i = k - 1; i >= 0; --i ) {
+            double v = r[ sz( i ) ];
+            for ( I64 q = i + 1; q < k; ++q ) v -= M[ sz( q * k + i ) ] * r[ sz( q ) ];
+            r[ sz( i ) ] = v / M[ sz( i * k + i ) ];
+        }
+        return true;
+    }
+    /// one local Newton step per cluster ( `Aggregation::local_step` ): `L_CC dw_C = nu_C - a_C` on its movable members, the
+    /// rest fixed ( `dw` 0 elsewhere; clusters of one movable member or more than 64 left as they are ); returns the
+    /// members' largest residual before the step
+    double local_step( const Laplacian &L, const std::vector<double> &A, std::vector<double> &dw ) const {
+        const std::vector<double> &N = targets();
+        dw.assign( sz( L.n ), 0.0 );
+        double worst = 0;
+        std::vector<I64> mem;
+        std::vector<double> M, r;
+        for ( I64 c = 0; c < nb_clusters(); ++c ) {
+            movable_members( c, mem );
+            const I64 k = I64( mem.size() );
+            for ( I64 i : mem ) worst = std::max( worst, std::fabs( A[ sz( i ) ] - N[ sz( i ) ] ) );
+            if ( k < 2 || k > 64 ) continue;
+            M.assign( sz( k * k ), 0.0 );
+            r.resize( sz( k ) );
+            for ( I64 u = 0; u < k; ++u ) {
+                const I64 i = mem[ sz( u ) ];
+                M[ sz( u * k + u ) ] = L.dia[ sz( i ) ];
+                r[ sz( u ) ] = N[ sz( i ) ] - A[ sz( i ) ];
+                for ( I64 e = L.row[ sz( i ) ]; e < L.row[ sz( i + 1 ) ]; ++e ) {
+                    const auto it = std::lower_bound( mem.begin(), mem.end(), L.col[ sz( e ) ] );
+                    if ( it != mem.end() && *it == L.col[ sz( e ) ] ) M[ sz( u * k + I64( it - mem.begin() ) ) ] -= L.c[ sz( e ) ];
+                }
+            }
+            if ( ! cholesky( k, M, r ) ) continue;
+            for ( I64 u = 0; u < k; ++u ) dw[ sz( mem[ sz( u ) ] ) ] = r[ sz( u ) ];
+        }
+        return worst;
     }
     int res_cur = LIN;
     double t_last = 1;
@@ -1400,6 +1541,44 @@ struct Solver {
         sw.edges = nullptr;
     }
 
+    /// THE RE-SPLITTING once the aggregated problem converged ( `Newton.h:431-470` ): local Newton steps on the clusters'
+    /// members, one diagram each, kept while their residual decreases, the floor holds and the aggregated test still
+    /// passes; stops as soon as the full problem passes the test. Works on the accepted `w`, `a`, `fa`.
+    void resplit() {
+        if ( ! any() ) return;
+        const I64 n = I64( w.size() );
+        Laplacian L;
+        std::vector<double> dw, a2, w2, ae;
+        std::vector<Facet> fa2;
+        for ( int round = 0; round < 4; ++round ) {
+            double wf, wfr;
+            full_residual( a, wf, wfr );
+            if ( wf <= o.tol_abs || ( o.tol_rel > 0 && wfr <= o.tol_rel ) ) return;
+            L.assemble( n, fa );
+            const double before = local_step( L, a, dw );
+            const double d0 = dw[ 0 ];                   // the gauge `w_0 = 0`, kept by a shift of everyone
+            w2.resize( sz( n ) );
+            for ( I64 i = 0; i < n; ++i ) w2[ sz( i ) ] = w[ sz( i ) ] + dw[ sz( i ) ] - d0;
+            measures_and_facets( w2, a2, fa2 );
+            ++nb_polish;
+            const double after = members_residual( a2 );
+            const std::vector<double> &A2 = shares( a2, ae );
+            double worst = 0, worst_rel = 0;
+            for ( I64 i = 0; i < n; ++i ) {
+                worst = std::max( worst, std::fabs( nu[ sz( i ) ] - A2[ sz( i ) ] ) );
+                worst_rel = std::max( worst_rel, std::fabs( nu[ sz( i ) ] - A2[ sz( i ) ] ) / nu[ sz( i ) ] );
+            }
+            if ( ! ( after < before ) || ! ( floor_min( a2 ) > 0 ) || ! ( worst <= o.tol_abs || ( o.tol_rel > 0 && worst_rel <= o.tol_rel ) ) ) {
+                sw.set_weights( w );                     // the diagram takes the accepted weights back
+                return;
+            }
+            w.swap( w2 );
+            a.swap( a2 );
+            fa.swap( fa2 );
+            residual = worst;
+        }
+    }
+
     /// `Newton.h:243-400` ( no aggregation ), the step by trials or by the limits ( `gpu/Newton2D.cuh:1615-1690` ): `a`, `fa`
     /// and `ea` already measured for `w_init`
     bool solves( const std::vector<double> &w_init ) {
@@ -1435,13 +1614,15 @@ struct Solver {
                 eps = 0.5 * std::min( nm, floor_min( a ) );
                 residual0 = worst;
             }
-            const double nr = merit( A );
+            double nr = merit( A );
             residual = worst;
-            if ( worst <= o.tol_abs || ( o.tol_rel > 0 && worst_rel <= o.tol_rel ) ) { status = 1; return true; }
+            if ( worst <= o.tol_abs || ( o.tol_rel > 0 && worst_rel <= o.tol_rel ) ) { status = 1; resplit(); return true; }
             ++nb_iter;
             double t0 = now();
             if ( ! dup.empty() ) { link( fa, fa_lin ); L.assemble( n, fa_lin ); }
             else L.assemble( n, fa );
+            // the pairs whose plane the doubles can no longer place are merged: the merit reads their clusters from here on
+            if ( detect( L, w ) ) nr = merit( shares( a, ae ) );
             t_asm += now() - t0;
             if ( ! cg_solve( L, b, d, o.lin_tol, o.lin_maxit, lst ) ) { status = 4; return false; }
             double t = std::min( 1.0, o.mult_ok * t_last );
@@ -1741,7 +1922,8 @@ int oracle_cg( I64 n, I64 nf, const I64 *fi, const I64 *fj, const double *fc, co
 /// `masses[ n ]`, `bary[ n * 2 ]`, `cost[ 1 ]`.
 int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w0, const double *lo, const double *hi, int kind,
                   double value, int nx, int ny, const double *geo, const double *img, const double *opts, const double *start_box,
-                  const I64 *dups, double *weights, double *stats, double *hist, double *masses, double *bary, double *cost ) {
+                  const I64 *dups, double *weights, double *stats, double *hist, double *masses, double *bary, double *cost,
+                  I64 *clusters ) {
     using namespace orc;
     const double t_begin = now();
     Options o;
@@ -1757,8 +1939,10 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     Sweep sw( t, lo, hi, dn0 );
     Solver nt( sw, o );
     nt.pos = pos;
-    // the exact duplicates: `opts[ 19 ]` the gap, `opts[ 20 ]` the count of `( dup, rep )` pairs in `dups` ( user ids )
+    // the exact duplicates: `opts[ 19 ]` the gap, `opts[ 20 ]` the count of `( dup, rep )` pairs in `dups` ( user ids );
+    // `opts[ 21 ]` the aggregation's margin ( 0: near-coincident pairs are not merged )
     const I64 nb_dups = dups ? I64( opts[ 20 ] ) : 0;
+    nt.margin = opts[ 21 ];
     {
         std::vector<I64> dd, rr;
         for ( I64 q = 0; q < nb_dups; ++q ) { dd.push_back( dups[ 2 * q ] ); rr.push_back( dups[ 2 * q + 1 ] ); }
@@ -1873,7 +2057,7 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     // the full problem's residual; with clusters, `converged` only when it passes the test too ( `gpu/Newton2D.cuh:1752-1760` )
     double full = nt.residual;
     int status = nt.status;
-    if ( nb_dups > 0 ) {
+    if ( nt.any() ) {
         double wf, wfr;
         nt.full_residual( a, wf, wfr );
         full = wf;
@@ -1913,14 +2097,14 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     stats[ MIN_START_MASS ] = min_start_mass;
     stats[ IT_SWITCH ] = it_switch;
     stats[ RESIDUAL_FULL ] = full;
-    if ( nb_dups > 0 ) {                                 // each representative and its duplicates: one cluster
-        std::vector<char> is_rep( sz( n ), 0 );
-        for ( I64 r : nt.rep ) is_rep[ sz( r ) ] = 1;
-        double nc = 0;
-        for ( char c : is_rep ) nc += c;
-        stats[ NB_CLUSTERS ] = nc;
-        stats[ NB_AGGREGATED ] = nc + double( nb_dups );
-        stats[ NB_DUPLICATES ] = double( nb_dups );
+    stats[ NB_CLUSTERS ] = double( nt.nb_clusters() );
+    stats[ NB_AGGREGATED ] = double( nt.cl_members.size() );
+    stats[ NB_DUPLICATES ] = double( nb_dups );
+    stats[ NB_POLISH ] = nt.nb_polish;
+    if ( clusters ) {                                    // every seed its own index, a member its cluster's smallest
+        for ( I64 i = 0; i < n; ++i ) clusters[ i ] = i;
+        for ( I64 c = 0; c < nt.nb_clusters(); ++c )
+            for ( I64 k = nt.cl_begin[ sz( c ) ]; k < nt.cl_begin[ sz( c + 1 ) ]; ++k ) clusters[ nt.cl_members[ sz( k ) ] ] = nt.cl_members[ sz( nt.cl_begin[ sz( c ) ] ) ];
     }
     return nb_steps;
 }

@@ -142,9 +142,8 @@ __device__ __forceinline__ bool can_cut( const S &c, int nb, const NodeRec &r, d
     return false;
 }
 
-__device__ __forceinline__ double proximity( const NodeRec *nodes, int m, double p0x, double p0y ) {
-    const double4 b = *reinterpret_cast<const double4 *>( &nodes[ m ] );
-    double e0 = fmax( b.x - p0x, p0x - b.z ), e1 = fmax( b.y - p0y, p0y - b.w );
+__device__ __forceinline__ double proximity( const NodeRec &b, double p0x, double p0y ) {
+    double e0 = fmax( b.lo_x - p0x, p0x - b.hi_x ), e1 = fmax( b.lo_y - p0y, p0y - b.hi_y );
     e0 = e0 > 0 ? e0 : 0;
     e1 = e1 > 0 ? e1 : 0;
     return e0 * e0 + e1 * e1;
@@ -159,7 +158,9 @@ struct Walk {
 /// builds the cell of rank `k` in `c`; returns the vertex count ( 0: empty ), or -1 when the slots
 /// were too few ( the cell is then left for the next pass ). One loop iteration does ONE thing of about
 /// the same cost -- a cut by one seed of the open leaf, or one node's test ( after a pop if needed ) --
-/// so the lanes of a wave, each at its own place in its walk, stay in step.
+/// so the lanes of a wave, each at its own place in its walk, stay in step. The loads run one step
+/// ahead of their use: a descent loads both children's records ( the nearer is the next node tested,
+/// from registers ), and a leaf's next seed is loaded while the current one cuts.
 template<class S>
 __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *lo, const double *hi, const S &c, int cap, Ev &ev ) {
     const double4 p = wk.seeds[ k ];
@@ -174,13 +175,17 @@ __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *l
     unsigned long long pend = 0, left = 0;
     int q = 0, qe = 0;                                   // the open leaf's remaining seeds
     bool visit = true;                                   // ( m, h ) is still to be tested
+    NodeRec rec;                                         // the record of ( m, h ) when the descent loaded it
+    bool have = false;
+    double4 nxt = make_double4( 0, 0, 0, 0 );            // the open leaf's seed `q`, loaded ahead
     for ( ;; ) {
         if ( q < qe ) {
             const int s = q++;
+            const double4 sd = nxt;
+            if ( q < qe ) nxt = wk.seeds[ q ];
             if ( s == k ) continue;
-            const double4 sd = wk.seeds[ s ];
             const double dx = sd.x - p0x, dy = sd.y - p0y;
-            const double off = ( dx * dx + dy * dy + w0 - sd.z ) / 2;
+            const double off = ( ( w0 - sd.z ) + ( dx * dx + dy * dy ) ) / 2;   // ( `|d|^2` after the weights' difference )
             const int res = cut( c, nb, cap, dx, dy, off, s, ev );
             if ( res == CUT_EMPTY ) return 0;
             if ( res == CUT_NO_ROOM ) return -1;
@@ -199,18 +204,23 @@ __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *l
             pend &= ~bit;
         }
         visit = false;
-        const NodeRec r = wk.nodes[ m ];
+        const NodeRec r = have ? rec : wk.nodes[ m ];
+        have = false;
         if ( r.beg < r.end && can_cut( c, nb, r, p0x, p0y, w0, ev ) ) {
             if ( h <= 1 ) {
                 q = r.beg;
                 qe = r.end;
+                nxt = wk.seeds[ q ];
             } else {
                 const int lc = m + 1, rc = m + ( 1 << ( h - 1 ) );
-                const bool go_left = proximity( wk.nodes, lc, p0x, p0y ) <= proximity( wk.nodes, rc, p0x, p0y );
+                const NodeRec rl = wk.nodes[ lc ], rr = wk.nodes[ rc ];
+                const bool go_left = proximity( rl, p0x, p0y ) <= proximity( rr, p0x, p0y );
                 const unsigned long long bit = 1ull << ( h - 1 );
                 pend |= bit;
                 left = go_left ? ( left | bit ) : ( left & ~bit );
                 m = go_left ? lc : rc;
+                rec = go_left ? rl : rr;
+                have = true;
                 h -= 1;
                 visit = true;
             }

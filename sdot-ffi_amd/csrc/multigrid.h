@@ -10,6 +10,7 @@ struct MgOptions {
     int    nu = 2;            ///< Jacobi sweeps before and after the coarse correction
     double omega = 0.7;       ///< their damping
     int    kcycle = 3;        ///< levels 1 .. kcycle take their coarse correction by two flexible CG steps ( a K-cycle ); below, one cycle
+    I64    bottom = 4096;     ///< the levels from the first of at most this many rows run as one single-workgroup kernel ( 0: none )
 };
 
 /// `L d = b` ( `b` of zero sum ) by flexible CG preconditioned with one multigrid cycle whose aggregates are `pack`

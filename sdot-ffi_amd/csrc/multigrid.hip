@@ -35,8 +35,8 @@ constexpr int DENSE_MAX = 64;      ///< the coarsest level, solved by a dense in
 
 /// the outer CG's device state
 enum { S_PQ = 0, S_PR, S_ALPHA, S_BETA, S_ZQ, S_RR, S_NB, S_ERR, S_DONE, S_IT, S_TOL, S_FAILED, S_BAD, S_NB_STATE };
-/// a K-cycle's scalars, per level
-enum { K_RHO1 = 0, K_ALPHA1, K_GAMMA, K_BETA2, K_ALPHA2, K_NB };
+/// a K-cycle's partial sums, per level: `rho1 = c1 . v1`, `alpha1 = c1 . r`, `beta = c2 . v2`, `gamma = c2 . v1`, `alpha2 = c2 . r2`
+enum { K_RHO1 = 0, K_ALPHA1, K_BETA2, K_GAMMA, K_ALPHA2, K_NB };
 
 /// a level's matrix: entry `q` of row `i` at `base + q * step`, `base = row_off[ i ]` ( CSR, `step` 1 ) or `i`
 /// ( column-major, `row_off` null, `step` the row count ); the off-diagonal values are the `c_ij > 0`
@@ -149,10 +149,47 @@ __global__ void k_jacobi( Mat A, const double *r, const double *x, double *xo, d
     if ( i < A.n ) xo[ i ] = x[ i ] + omega * ( r[ i ] - row_times( A, i, x ) ) / A.dia[ i ];
 }
 
-__global__ void k_copy( I64 n, const double *a, double *b, const double *state ) {
+/// the first two sweeps from zero in one pass: `x1 = omega D^-1 r` read on the fly, `xo = x1 + omega D^-1 ( r - A x1 )`
+__global__ void k_jacobi_two( Mat A, const double *r, double *xo, double omega, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
     const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
-    if ( i < n ) b[ i ] = a[ i ];
+    if ( i >= A.n ) return;
+    const double x1 = omega * r[ i ] / A.dia[ i ];
+    double s = A.dia[ i ] * x1;
+    const I64 b = A.row_off ? A.row_off[ i ] : i;
+    const int c = A.row_cnt[ i ];
+    for ( int q = 0; q < c; ++q ) {
+        const I64 e = b + q * A.step;
+        const int j = A.col[ e ];
+        s -= A.val[ e ] * ( omega * r[ j ] / A.dia[ j ] );
+    }
+    xo[ i ] = x1 + omega * ( r[ i ] - s ) / A.dia[ i ];
+}
+
+/// the coarse correction prolonged and the first post-sweep in one pass: `x' = x + P xc` read on the fly,
+/// `xo = x' + omega D^-1 ( r - A x' )`
+__global__ void k_prolong_jacobi( Mat A, const double *r, const double *x, const double *xc, int pack, double *xo, double omega,
+                                  const double *state ) {
+    if ( state[ S_DONE ] != 0 ) return;
+    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( i >= A.n ) return;
+    const double xi = x[ i ] + xc[ i / pack ];
+    double s = A.dia[ i ] * xi;
+    const I64 b = A.row_off ? A.row_off[ i ] : i;
+    const int c = A.row_cnt[ i ];
+    for ( int q = 0; q < c; ++q ) {
+        const I64 e = b + q * A.step;
+        const int j = A.col[ e ];
+        s -= A.val[ e ] * ( x[ j ] + xc[ j / pack ] );
+    }
+    xo[ i ] = xi + omega * ( r[ i ] - s ) / A.dia[ i ];
+}
+
+/// `xo = x + P xc`
+__global__ void k_prolong_to( I64 n, int pack, const double *x, const double *xc, double *xo, const double *state ) {
+    if ( state[ S_DONE ] != 0 ) return;
+    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( i < n ) xo[ i ] = x[ i ] + xc[ i / pack ];
 }
 
 /// `rc[ I ] = sum_{ i in I } ( r - A x )_i`: one lane per fine row, the `pack` consecutive lanes of an aggregate
@@ -165,11 +202,6 @@ __global__ void k_restrict( Mat A, const double *r, const double *x, int pack, d
     if ( i < A.n && ( i % pack ) == 0 ) rc[ i / pack ] = v;
 }
 
-__global__ void k_prolong( I64 n, int pack, const double *xc, double *x, const double *state ) {
-    if ( state[ S_DONE ] != 0 ) return;
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
-    if ( i < n ) x[ i ] += xc[ i / pack ];
-}
 
 /// `x = Inv r`, one wave per row
 __global__ void k_dense_apply( int n, const double *inv, const double *r, double *x, const double *state ) {
@@ -181,6 +213,82 @@ __global__ void k_dense_apply( int n, const double *inv, const double *r, double
     if ( lane == 0 ) x[ row ] = s;
 }
 
+// ---- the bottom of the hierarchy in one workgroup ---------------------------------------------------------------
+//
+// Below a few thousand rows a level's kernels are too small to fill the chip, and a launch costs more than its work: the
+// levels from `lb` down run as ONE kernel of one workgroup ( V-cycles: pre-smoothing, restriction, ... the dense
+// coarsest solve ... prolongation, post-smoothing ), the steps separated by barriers. Their matrices and vectors stay in
+// the L2 of the workgroup's XCD.
+
+struct BottomLevel {
+    I64 n = 0;
+    const int *cnt = nullptr, *col = nullptr;            ///< column-major: entry `q` of row `i` at `i + q n`
+    const double *val = nullptr, *dia = nullptr;
+    double *x = nullptr, *t = nullptr, *r = nullptr;
+};
+struct Bottom {
+    int nl = 0, pack = 4, nu = 2, nc = 0;
+    double omega = 0.7;
+    const double *inv = nullptr;
+    BottomLevel L[ MAX_LEVELS ];
+};
+constexpr int BOTTOM_THREADS = 1024;
+
+__device__ __forceinline__ double b_row_times( const BottomLevel &A, I64 i, const double *x ) {
+    double s = A.dia[ i ] * x[ i ];
+    for ( int q = 0; q < A.cnt[ i ]; ++q ) s -= A.val[ i + q * A.n ] * x[ A.col[ i + q * A.n ] ];
+    return s;
+}
+
+/// `x ~ A^-1 r` on the bottom levels ( `B.L[ 0 ].r` in, `B.L[ 0 ].x` out ), one V-cycle
+__global__ __launch_bounds__( BOTTOM_THREADS ) void k_bottom( Bottom B, const double *state ) {
+    if ( state[ S_DONE ] != 0 ) return;
+    const int tid = threadIdx.x;
+    for ( int l = 0; l + 1 < B.nl; ++l ) {
+        const BottomLevel &A = B.L[ l ];
+        // pre-smoothing from zero, `nu` sweeps ping-ponging so that the last lands in `x`
+        double *cur = ( B.nu - 1 ) % 2 ? A.t : A.x, *oth = ( B.nu - 1 ) % 2 ? A.x : A.t;
+        for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) cur[ i ] = B.omega * A.r[ i ] / A.dia[ i ];
+        __syncthreads();
+        for ( int s = 1; s < B.nu; ++s ) {
+            for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) oth[ i ] = cur[ i ] + B.omega * ( A.r[ i ] - b_row_times( A, i, cur ) ) / A.dia[ i ];
+            __syncthreads();
+            double *sw = cur; cur = oth; oth = sw;
+        }
+        // the residual restricted
+        const BottomLevel &C = B.L[ l + 1 ];
+        for ( I64 I = tid; I < C.n; I += BOTTOM_THREADS ) {
+            double v = 0;
+            const I64 e = ( I + 1 ) * B.pack < A.n ? ( I + 1 ) * B.pack : A.n;
+            for ( I64 i = I * B.pack; i < e; ++i ) v += A.r[ i ] - b_row_times( A, i, A.x );
+            C.r[ I ] = v;
+        }
+        __syncthreads();
+    }
+    {
+        const BottomLevel &D = B.L[ B.nl - 1 ];                  // the coarsest: the dense inverse
+        for ( int row = tid; row < B.nc; row += BOTTOM_THREADS ) {
+            double v = 0;
+            for ( int j = 0; j < B.nc; ++j ) v += B.inv[ size_t( row ) * B.nc + j ] * D.r[ j ];
+            D.x[ row ] = v;
+        }
+        __syncthreads();
+    }
+    for ( int l = B.nl - 2; l >= 0; --l ) {
+        const BottomLevel &A = B.L[ l ];
+        const double *xc = B.L[ l + 1 ].x;
+        // prolongation, then `nu` sweeps ending in `x`: from `t` when `nu` is odd, else from `x` through `t`
+        double *cur = B.nu % 2 ? A.t : A.x, *oth = B.nu % 2 ? A.x : A.t;
+        for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) cur[ i ] = A.x[ i ] + xc[ i / B.pack ];
+        __syncthreads();
+        for ( int s = 0; s < B.nu; ++s ) {
+            for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) oth[ i ] = cur[ i ] + B.omega * ( A.r[ i ] - b_row_times( A, i, cur ) ) / A.dia[ i ];
+            __syncthreads();
+            double *sw = cur; cur = oth; oth = sw;
+        }
+    }
+}
+
 // ---- the K-cycle's two flexible CG steps ( Notay ) ------------------------------------------------------------
 //
 //   c1 = C( r ), v1 = A c1, rho1 = c1 . v1, alpha1 = c1 . r, r2 = r - alpha1 / rho1 v1,
@@ -190,7 +298,15 @@ __global__ void k_dense_apply( int n, const double *inv, const double *r, double
 template<int T>
 __device__ __forceinline__ double block_sum_t( double v, double *sh ) { return block_sum<T>( v, sh ); }
 
-/// `v = A c` and the partial sums of `c . v`, `c . u1` and ( `nd == 3` ) `c . u2`, over `nblk` blocks
+/// the fixed-order sum of `nb` partials: every block that calls it gets the same value ( all its threads take part )
+__device__ __forceinline__ double sum_parts( const double *p, int nb, double *sh ) {
+    double a = 0;
+    for ( int j = threadIdx.x; j < nb; j += RED_THREADS ) a += p[ j ];
+    return block_sum<RED_THREADS>( a, sh );
+}
+
+/// `v = A c` and the partial sums of `c . v`, `c . u1` and ( `nd == 3` ) `c . u2`, over `nblk` blocks, into the slots
+/// `part[ q * RED_BLOCKS ]`
 __global__ __launch_bounds__( RED_THREADS ) void k_spmv_dots( Mat A, const double *c, double *v, const double *u1, const double *u2, int nd, double *part,
                                                             const double *state ) {
     __shared__ double sh[ RED_THREADS ];
@@ -213,45 +329,36 @@ __global__ __launch_bounds__( RED_THREADS ) void k_spmv_dots( Mat A, const doubl
     }
 }
 
-/// the partials of `k_spmv_dots` summed into the level's scalars: first step ( `nd == 2` ): `rho1`, `alpha1`;
-/// second step: `beta`, `gamma`, `alpha2`
-__global__ __launch_bounds__( RED_THREADS ) void k_kc_scalars( const double *part, int nblk, int nd, double *ks, const double *state ) {
+/// `r2 = r - alpha1 / rho1 v1`, `rho1` and `alpha1` summed by every block from the first step's partials ( slots 0, 1 );
+/// a grid-stride loop over `nblk` blocks
+__global__ __launch_bounds__( RED_THREADS ) void k_kc_residual( I64 n, const double *r, const double *v1, const double *part, int nblk, double *r2,
+                                                              const double *state ) {
     __shared__ double sh[ RED_THREADS ];
     if ( state[ S_DONE ] != 0 ) return;
-    double s[ 3 ];
-    for ( int q = 0; q < nd; ++q ) {
-        double a = 0;
-        for ( int j = threadIdx.x; j < nblk; j += RED_THREADS ) a += part[ q * RED_BLOCKS + j ];
-        s[ q ] = block_sum<RED_THREADS>( a, sh );
-    }
-    if ( threadIdx.x == 0 ) {
-        if ( nd == 2 ) { ks[ K_RHO1 ] = s[ 0 ]; ks[ K_ALPHA1 ] = s[ 1 ]; }
-        else           { ks[ K_BETA2 ] = s[ 0 ]; ks[ K_GAMMA ] = s[ 1 ]; ks[ K_ALPHA2 ] = s[ 2 ]; }
-    }
+    const double rho1 = sum_parts( part, nblk, sh ), alpha1 = sum_parts( part + RED_BLOCKS, nblk, sh );
+    const double f = rho1 > 0 ? alpha1 / rho1 : 0.0;
+    for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < n; i += I64( gridDim.x ) * RED_THREADS ) r2[ i ] = r[ i ] - f * v1[ i ];
 }
 
-/// `r2 = r - alpha1 / rho1 v1`
-__global__ void k_kc_residual( I64 n, const double *r, const double *v1, const double *ks, double *r2, const double *state ) {
+/// `x = a1 c1 + a2 c2` with the K-cycle's coefficients, summed by every block from the two steps' partials ( slots 0 .. 4:
+/// `rho1`, `alpha1`, `beta`, `gamma`, `alpha2` ); one step only when the second is degenerate
+__global__ __launch_bounds__( RED_THREADS ) void k_kc_combine( I64 n, const double *c1, const double *c2, const double *part, int nblk, double *x,
+                                                             const double *state ) {
+    __shared__ double sh[ RED_THREADS ];
     if ( state[ S_DONE ] != 0 ) return;
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
-    const double f = ks[ K_RHO1 ] > 0 ? ks[ K_ALPHA1 ] / ks[ K_RHO1 ] : 0.0;
-    if ( i < n ) r2[ i ] = r[ i ] - f * v1[ i ];
-}
-
-/// `x = a1 c1 + a2 c2` with the K-cycle's coefficients ( one step only when the second is degenerate )
-__global__ void k_kc_combine( I64 n, const double *c1, const double *c2, const double *ks, double *x, const double *state ) {
-    if ( state[ S_DONE ] != 0 ) return;
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
-    if ( i >= n ) return;
-    const double rho1 = ks[ K_RHO1 ];
-    if ( ! ( rho1 > 0 ) ) { x[ i ] = c1[ i ]; return; }
-    const double rho2 = ks[ K_BETA2 ] - ks[ K_GAMMA ] * ks[ K_GAMMA ] / rho1;
-    double a1 = ks[ K_ALPHA1 ] / rho1, a2 = 0;
-    if ( rho2 > 0 ) {
-        a2 = ks[ K_ALPHA2 ] / rho2;
-        a1 -= ks[ K_GAMMA ] * a2 / rho1;
+    double k[ K_NB ];
+    for ( int q = 0; q < K_NB; ++q ) k[ q ] = sum_parts( part + q * RED_BLOCKS, nblk, sh );
+    const double rho1 = k[ 0 ], alpha1 = k[ 1 ], beta = k[ 2 ], gamma = k[ 3 ], alpha2 = k[ 4 ];
+    double a1 = 1, a2 = 0;
+    if ( rho1 > 0 ) {
+        const double rho2 = beta - gamma * gamma / rho1;
+        a1 = alpha1 / rho1;
+        if ( rho2 > 0 ) {
+            a2 = alpha2 / rho2;
+            a1 -= gamma * a2 / rho1;
+        }
     }
-    x[ i ] = a1 * c1[ i ] + a2 * c2[ i ];
+    for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < n; i += I64( gridDim.x ) * RED_THREADS ) x[ i ] = a1 * c1[ i ] + a2 * c2[ i ];
 }
 
 // ---- the outer flexible CG -------------------------------------------------------------------------------------
@@ -306,25 +413,19 @@ __global__ __launch_bounds__( RED_THREADS ) void k_fcg_spmv( Mat A, const double
     if ( threadIdx.x == 0 ) { part[ blockIdx.x ] = a; part[ RED_BLOCKS + blockIdx.x ] = b; }
 }
 
-__global__ __launch_bounds__( RED_THREADS ) void k_fcg_alpha( const double *part, double *state ) {
+/// `alpha = p . r / p . q` summed by every block from the partials ( slots 0, 1 ), then `d += alpha p`, `r -= alpha q`
+/// and the partials of `r . r` ( slot 2 ). `p . q <= 0` ( a breakdown ) stops the iteration as a failure
+__global__ __launch_bounds__( RED_THREADS ) void k_pcg_update( I64 n, const double *p, const double *q, double *d, double *r, double *part,
+                                                             double *state ) {
     __shared__ double sh[ RED_THREADS ];
     if ( state[ S_DONE ] != 0 ) return;
-    double a = 0, b = 0;
-    for ( int j = threadIdx.x; j < RED_BLOCKS; j += RED_THREADS ) { a += part[ j ]; b += part[ RED_BLOCKS + j ]; }
-    const double pq = block_sum<RED_THREADS>( a, sh );
-    const double pr = block_sum<RED_THREADS>( b, sh );
-    if ( threadIdx.x == 0 ) {
-        state[ S_PQ ] = pq;
-        state[ S_PR ] = pr;
-        if ( ! ( pq > 0 ) ) { state[ S_DONE ] = 1; state[ S_FAILED ] = 1; }
-        else state[ S_ALPHA ] = pr / pq;
+    const double pq = sum_parts( part, RED_BLOCKS, sh ), pr = sum_parts( part + RED_BLOCKS, RED_BLOCKS, sh );
+    if ( ! ( pq > 0 ) ) {
+        if ( blockIdx.x == 0 && threadIdx.x == 0 ) { state[ S_PQ ] = pq; state[ S_DONE ] = 1; state[ S_FAILED ] = 1; }
+        return;
     }
-}
-
-__global__ __launch_bounds__( RED_THREADS ) void k_pcg_update( I64 n, const double *p, const double *q, double *d, double *r, double *part,
-                                                             const double *state ) {
-    if ( state[ S_DONE ] != 0 ) return;
-    const double alpha = state[ S_ALPHA ];
+    const double alpha = pr / pq;
+    if ( blockIdx.x == 0 && threadIdx.x == 0 ) { state[ S_PQ ] = pq; state[ S_PR ] = pr; state[ S_ALPHA ] = alpha; }
     double acc = 0;
     for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < n; i += I64( RED_BLOCKS ) * RED_THREADS ) {
         d[ i ] += alpha * p[ i ];
@@ -332,15 +433,14 @@ __global__ __launch_bounds__( RED_THREADS ) void k_pcg_update( I64 n, const doub
         r[ i ] = ri;
         acc += ri * ri;
     }
-    write_partial<RED_THREADS>( acc, part, blockIdx.x );
+    const double rr = block_sum<RED_THREADS>( acc, sh );
+    if ( threadIdx.x == 0 ) part[ 2 * RED_BLOCKS + blockIdx.x ] = rr;
 }
 
 __global__ __launch_bounds__( RED_THREADS ) void k_pcg_check( const double *part, double *state ) {
     __shared__ double sh[ RED_THREADS ];
     if ( state[ S_DONE ] != 0 ) return;
-    double a = 0;
-    for ( int j = threadIdx.x; j < RED_BLOCKS; j += RED_THREADS ) a += part[ j ];
-    const double rr = block_sum<RED_THREADS>( a, sh );
+    const double rr = sum_parts( part + 2 * RED_BLOCKS, RED_BLOCKS, sh );
     if ( threadIdx.x == 0 ) {
         state[ S_RR ] = rr;
         const double err = sqrt( rr ) / state[ S_NB ];
@@ -350,30 +450,29 @@ __global__ __launch_bounds__( RED_THREADS ) void k_pcg_check( const double *part
     }
 }
 
-/// partial sums of `z . q` ( the flexible CG's `beta` )
+/// partial sums of `z . q` ( slot 3: the flexible CG's `beta` )
 __global__ __launch_bounds__( RED_THREADS ) void k_fcg_zq( I64 n, const double *z, const double *q, double *part, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
     double acc = 0;
     for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < n; i += I64( RED_BLOCKS ) * RED_THREADS ) acc += z[ i ] * q[ i ];
-    write_partial<RED_THREADS>( acc, part, blockIdx.x );
+    write_partial<RED_THREADS>( acc, part, 3 * RED_BLOCKS + blockIdx.x );
 }
 
-__global__ __launch_bounds__( RED_THREADS ) void k_fcg_beta( const double *part, double *state ) {
+/// `beta = - z . q / p . q` summed by every block from the partials ( slot 3 ), then `p = z + beta p`
+__global__ __launch_bounds__( RED_THREADS ) void k_pcg_dir( I64 n, const double *z, double *p, const double *part, double *state ) {
     __shared__ double sh[ RED_THREADS ];
     if ( state[ S_DONE ] != 0 ) return;
-    double a = 0;
-    for ( int j = threadIdx.x; j < RED_BLOCKS; j += RED_THREADS ) a += part[ j ];
-    const double zq = block_sum<RED_THREADS>( a, sh );
-    if ( threadIdx.x == 0 ) {
-        state[ S_ZQ ] = zq;
-        state[ S_BETA ] = state[ S_PQ ] > 0 ? -zq / state[ S_PQ ] : 0.0;
-    }
+    const double zq = sum_parts( part + 3 * RED_BLOCKS, RED_BLOCKS, sh ), pq = state[ S_PQ ];
+    const double beta = pq > 0 ? -zq / pq : 0.0;
+    if ( blockIdx.x == 0 && threadIdx.x == 0 ) { state[ S_ZQ ] = zq; state[ S_BETA ] = beta; }
+    for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < n; i += I64( RED_BLOCKS ) * RED_THREADS ) p[ i ] = z[ i ] + beta * p[ i ];
 }
 
-__global__ void k_pcg_dir( I64 n, const double *z, double *p, const double *state ) {
+/// the first direction, `p = z`
+__global__ void k_pcg_first_dir( I64 n, const double *z, double *p, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
     const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
-    if ( i < n ) p[ i ] = z[ i ] + state[ S_BETA ] * p[ i ];
+    if ( i < n ) p[ i ] = z[ i ];
 }
 
 __global__ void k_shift_gauge( I64 n, double *d, const double *g, int gauge ) {
@@ -392,10 +491,12 @@ struct Hierarchy {
     // per level: the cycle's right-hand side, solution and smoothing buffer; the K-cycle's vectors
     double *x[ MAX_LEVELS ] = {}, *t[ MAX_LEVELS ] = {}, *r[ MAX_LEVELS ] = {};
     double *c1[ MAX_LEVELS ] = {}, *c2[ MAX_LEVELS ] = {}, *v1[ MAX_LEVELS ] = {}, *v2[ MAX_LEVELS ] = {}, *r2[ MAX_LEVELS ] = {};
-    double *ks = nullptr;              ///< [ MAX_LEVELS ][ K_NB ]
+    double *kpart = nullptr;           ///< [ MAX_LEVELS ][ K_NB ][ RED_BLOCKS ] the K-cycles' partial sums
     double *part = nullptr;            ///< [ 3 ][ RED_BLOCKS ]
     double *inv = nullptr;
     int nc = 0;
+    int lb = 0;                        ///< the first level of the bottom kernel ( `nl`: none )
+    Bottom bottom;                     ///< its levels `lb` .. `nl - 1`
 };
 
 unsigned red_blocks_for( I64 n ) {
@@ -408,45 +509,59 @@ void cycle( Hierarchy &H, int l, const double *r, double *x, const MgOptions &o,
 /// `x ~ A_l^-1 r` on level `l >= 1` by two flexible CG steps preconditioned by the cycle of level `l`
 void kcycle( Hierarchy &H, int l, const double *r, double *x, const MgOptions &o, const double *state, hipStream_t st ) {
     const Mat &A = H.A[ l ];
-    const unsigned nb = blocks_for( A.n, 256 ), rb = red_blocks_for( A.n );
-    double *ks = H.ks + l * K_NB;
+    const unsigned rb = red_blocks_for( A.n );
+    double *kp = H.kpart + size_t( l ) * K_NB * RED_BLOCKS;
     cycle( H, l, r, H.c1[ l ], o, state, st );
-    k_spmv_dots<<<rb, RED_THREADS, 0, st>>>( A, H.c1[ l ], H.v1[ l ], r, nullptr, 2, H.part, state );
-    k_kc_scalars<<<1, RED_THREADS, 0, st>>>( H.part, int( rb ), 2, ks, state );
-    k_kc_residual<<<nb, 256, 0, st>>>( A.n, r, H.v1[ l ], ks, H.r2[ l ], state );
+    k_spmv_dots<<<rb, RED_THREADS, 0, st>>>( A, H.c1[ l ], H.v1[ l ], r, nullptr, 2, kp, state );
+    k_kc_residual<<<rb, RED_THREADS, 0, st>>>( A.n, r, H.v1[ l ], kp, int( rb ), H.r2[ l ], state );
     cycle( H, l, H.r2[ l ], H.c2[ l ], o, state, st );
-    k_spmv_dots<<<rb, RED_THREADS, 0, st>>>( A, H.c2[ l ], H.v2[ l ], H.v1[ l ], H.r2[ l ], 3, H.part, state );
-    k_kc_scalars<<<1, RED_THREADS, 0, st>>>( H.part, int( rb ), 3, ks, state );
-    k_kc_combine<<<nb, 256, 0, st>>>( A.n, H.c1[ l ], H.c2[ l ], ks, x, state );
+    k_spmv_dots<<<rb, RED_THREADS, 0, st>>>( A, H.c2[ l ], H.v2[ l ], H.v1[ l ], H.r2[ l ], 3, kp + K_BETA2 * RED_BLOCKS, state );
+    k_kc_combine<<<rb, RED_THREADS, 0, st>>>( A.n, H.c1[ l ], H.c2[ l ], kp, int( rb ), x, state );
 }
 
 /// one cycle on level `l`: `x ~ A_l^-1 r` from zero
 void cycle( Hierarchy &H, int l, const double *r, double *x, const MgOptions &o, const double *state, hipStream_t st ) {
+    if ( l >= H.lb ) {                                   // the bottom levels, one workgroup ( `r` and `x` are its level's own )
+        if ( l == H.lb ) k_bottom<<<1, BOTTOM_THREADS, 0, st>>>( H.bottom, state );
+        else k_dense_apply<<<blocks_for( H.nc, 4 ), 256, 0, st>>>( H.nc, H.inv, r, x, state );   // ( `lb` past the coarsest )
+        return;
+    }
     if ( l == H.nl - 1 ) {
         k_dense_apply<<<blocks_for( H.nc, 4 ), 256, 0, st>>>( H.nc, H.inv, r, x, state );
         return;
     }
     const Mat &A = H.A[ l ];
     const unsigned nb = blocks_for( A.n, 256 );
-    // pre-smoothing from zero: the first sweep, then `nu - 1` sweeps ping-ponging so that the last lands in `x`
-    double *cur = ( o.nu - 1 ) % 2 ? H.t[ l ] : x, *tmp = ( o.nu - 1 ) % 2 ? x : H.t[ l ];
-    k_jacobi_first<<<nb, 256, 0, st>>>( A.n, r, A.dia, o.omega, cur, state );
-    for ( int s = 1; s < o.nu; ++s ) {
+    // pre-smoothing from zero, `nu` sweeps ( the first two in one pass ) ping-ponging so that the last lands in `x`
+    const int nk = o.nu >= 2 ? o.nu - 1 : 1;             // the launches
+    double *cur = ( nk - 1 ) % 2 ? H.t[ l ] : x, *tmp = ( nk - 1 ) % 2 ? x : H.t[ l ];
+    if ( o.nu >= 2 ) k_jacobi_two<<<nb, 256, 0, st>>>( A, r, cur, o.omega, state );
+    else             k_jacobi_first<<<nb, 256, 0, st>>>( A.n, r, A.dia, o.omega, cur, state );
+    for ( int s = 1; s < nk; ++s ) {
         k_jacobi<<<nb, 256, 0, st>>>( A, r, cur, tmp, o.omega, state );
         std::swap( cur, tmp );
     }
     // the coarse correction: the residual restricted, solved by the next level's K-cycle or cycle, prolonged
     k_restrict<<<nb, 256, 0, st>>>( A, r, x, o.pack, H.r[ l + 1 ], state );
-    if ( l + 1 <= o.kcycle && l + 1 < H.nl - 1 ) kcycle( H, l + 1, H.r[ l + 1 ], H.x[ l + 1 ], o, state, st );
-    else                                         cycle( H, l + 1, H.r[ l + 1 ], H.x[ l + 1 ], o, state, st );
-    k_prolong<<<nb, 256, 0, st>>>( A.n, o.pack, H.x[ l + 1 ], x, state );
-    // post-smoothing, the pre-smoothing's mirror
-    double *c2 = x, *t2 = H.t[ l ];
-    for ( int s = 0; s < o.nu; ++s ) {
-        k_jacobi<<<nb, 256, 0, st>>>( A, r, c2, t2, o.omega, state );
-        std::swap( c2, t2 );
+    if ( l + 1 <= o.kcycle && l + 1 < H.nl - 1 && l + 1 < H.lb ) kcycle( H, l + 1, H.r[ l + 1 ], H.x[ l + 1 ], o, state, st );
+    else                                                          cycle( H, l + 1, H.r[ l + 1 ], H.x[ l + 1 ], o, state, st );
+    // post-smoothing: the prolongation fused with the first sweep ( into `t`, since it reads `x` around each row ), or for
+    // an odd `nu` prolonged into `t` alone, then the sweeps ping-ponging so that the last lands in `x`
+    if ( o.nu % 2 == 0 ) {
+        k_prolong_jacobi<<<nb, 256, 0, st>>>( A, r, x, H.x[ l + 1 ], o.pack, H.t[ l ], o.omega, state );
+        double *c2 = H.t[ l ], *t2 = x;
+        for ( int s = 1; s < o.nu; ++s ) {
+            k_jacobi<<<nb, 256, 0, st>>>( A, r, c2, t2, o.omega, state );
+            std::swap( c2, t2 );
+        }
+    } else {
+        k_prolong_to<<<nb, 256, 0, st>>>( A.n, o.pack, x, H.x[ l + 1 ], H.t[ l ], state );
+        double *c2 = H.t[ l ], *t2 = x;
+        for ( int s = 0; s < o.nu; ++s ) {
+            k_jacobi<<<nb, 256, 0, st>>>( A, r, c2, t2, o.omega, state );
+            std::swap( c2, t2 );
+        }
     }
-    if ( c2 != x ) k_copy<<<nb, 256, 0, st>>>( A.n, c2, x, state );
 }
 
 } // namespace
@@ -459,8 +574,8 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
     double *state = A.get<double>( "mg.state", S_NB_STATE );
     double *hs = pinned<double>( "mg.hstate", S_NB_STATE );
     Hierarchy H;
-    H.part = A.get<double>( "mg.part", 3 * size_t( RED_BLOCKS ) );
-    H.ks = A.get<double>( "mg.ks", size_t( MAX_LEVELS ) * K_NB );
+    H.part = A.get<double>( "mg.part", 4 * size_t( RED_BLOCKS ) );
+    H.kpart = A.get<double>( "mg.kpart", size_t( MAX_LEVELS ) * K_NB * RED_BLOCKS );
     H.A[ 0 ].n = L.n; H.A[ 0 ].step = L.step;
     H.A[ 0 ].row_off = L.row_off; H.A[ 0 ].row_cnt = L.row_cnt; H.A[ 0 ].col = L.col; H.A[ 0 ].val = L.val; H.A[ 0 ].dia = L.dia;
     H.nl = 1;
@@ -494,6 +609,21 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
     H.nc = int( H.A[ H.nl - 1 ].n );
     SDOT_CHECK( H.nc <= DENSE_MAX, "multigrid: the coarsest level is too large for its dense inverse" );
     H.inv = A.get<double>( "mg.inv", size_t( DENSE_MAX ) * DENSE_MAX );
+    // the bottom kernel: the levels from the first of at most `bottom` rows ( never the finest, whose rows are CSR )
+    H.lb = H.nl;
+    if ( o.bottom > 0 )
+        for ( int l = 1; l < H.nl; ++l )
+            if ( H.A[ l ].n <= o.bottom ) { H.lb = l; break; }
+    if ( H.lb < H.nl - 1 ) {
+        Bottom &B = H.bottom;
+        B.nl = H.nl - H.lb; B.pack = o.pack; B.nu = o.nu; B.omega = o.omega; B.nc = H.nc; B.inv = H.inv;
+        for ( int l = H.lb; l < H.nl; ++l ) {
+            BottomLevel &L = B.L[ l - H.lb ];
+            L.n = H.A[ l ].n; L.cnt = H.A[ l ].row_cnt; L.col = H.A[ l ].col; L.val = H.A[ l ].val; L.dia = H.A[ l ].dia;
+            L.x = H.x[ l ]; L.t = H.t[ l ]; L.r = H.r[ l ];
+        }
+    } else
+        H.lb = H.nl;                                     // ( a bottom of the coarsest alone: the dense apply as before )
     k_dense_inverse<<<1, 256, 0, st>>>( H.A[ H.nl - 1 ], H.inv );
     check_launch( "coarsest inverse" );
     nb_levels = H.nl;
@@ -506,10 +636,9 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
     double *part = H.part;
     k_pcg_init<<<RED_BLOCKS, RED_THREADS, 0, st>>>( A0, b, d, r, part );
     k_pcg_init_final<<<1, RED_THREADS, 0, st>>>( part, state, tol );
-    // z = M( r ), p = z ( beta = 0 on the first pass )
+    // z = M( r ), p = z
     cycle( H, 0, r, z, o, state, st );
-    SDOT_HIP( hipMemsetAsync( p, 0, sizeof( double ) * size_t( n ), st ) );
-    k_pcg_dir<<<blocks_for( n, 256 ), 256, 0, st>>>( n, z, p, state );
+    k_pcg_first_dir<<<blocks_for( n, 256 ), 256, 0, st>>>( n, z, p, state );
     check_launch( "pcg init" );
     const int batch = 4;
     int launched = 0;
@@ -522,13 +651,11 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
                 timing().end( tp, st );
                 timing().spmv_bytes += double( n ) * ( 4 + 4 + 8 + 8 + 8 + 8 ) + double( L.nnz ) * ( 4 + 8 );
             }
-            k_fcg_alpha<<<1, RED_THREADS, 0, st>>>( part, state );
             k_pcg_update<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, p, q, d, r, part, state );
             k_pcg_check<<<1, RED_THREADS, 0, st>>>( part, state );
             cycle( H, 0, r, z, o, state, st );
             k_fcg_zq<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, z, q, part, state );
-            k_fcg_beta<<<1, RED_THREADS, 0, st>>>( part, state );
-            k_pcg_dir<<<blocks_for( n, 256 ), 256, 0, st>>>( n, z, p, state );
+            k_pcg_dir<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, z, p, part, state );
         }
         check_launch( "pcg iterations" );
         launched += batch;

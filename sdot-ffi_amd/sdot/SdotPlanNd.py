@@ -23,6 +23,9 @@ _LIN = { "auto": 0, "cholesky": 1, "amg": 2, "cg": 3, "mg": 4 }
 _STEP = { "trials": 0, "limits": 1 }
 _RESIDUAL = { "lin": 0, "log": 1, "power": 2 }
 _CONTINUATION = { "never": 0, "auto": 1, "always": 2 }
+#: THE AGGREGATION ( `sdotplan/Aggregation.h` ): a pair is merged when one ulp of its weights moves more than `tol / margin` of
+#: mass between them ( `Tuning( aggregation_margin = ... )`, `SdotPlanNd.py:115-118` )
+_AGGREGATE_MARGIN = 4.0
 #: exact duplicates: `w_dup = w_rep - gap`, `gap` this fraction of the larger of the squared norm and the squared extent of the
 #: cloud -- any positive gap empties a duplicate's cell, this one far above the rounding of the cuts ( `SdotPlanNd.py:119-121` )
 _DUPLICATE_GAP = 1e-6
@@ -98,9 +101,14 @@ def card_options( settings, density, verbose = False, nb_dups = 0, gap = 0.0 ):
     if tun.mg_pack:
         put( "mg_shift", int( tun.mg_pack ).bit_length() - 1 )
     put( "mg_nu", tun.mg_nu or 0 ); put( "mg_kcycle", -1 if tun.mg_kcycle is None else tun.mg_kcycle )
+    # the rows under which the hierarchy's bottom runs in one workgroup ( None: the default; 0 or less: never )
+    if tun.mg_stop is not None:
+        put( "mg_stop", tun.mg_stop if tun.mg_stop > 0 else -1 )
     put( "continuation", _CONTINUATION[ settings.continuation ] ); put( "conv_threshold", tun.conv_threshold )
     put( "conv_s0", tun.conv_start or 0.0 ); put( "conv_ratio", tun.conv_ratio ); put( "conv_min", tun.conv_min or 0.0 )
     put( "agg_gap", gap ); put( "agg_nb_dups", nb_dups )
+    if settings.aggregate:
+        put( "agg_margin", _AGGREGATE_MARGIN if tun.aggregation_margin is None else tun.aggregation_margin )
     if isinstance( density, Image ):
         lo, hi = density.box()
         h = np.diag( density.frame )

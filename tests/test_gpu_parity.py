@@ -393,30 +393,41 @@ def test_gaussian_solve_matches_the_oracle( oracle, native ):
 
 
 @pytest.mark.parametrize( "step,lin", [ ( "trials", "cg" ), ( "limits", "cg" ), ( "limits", "mg" ) ] )
-def test_exact_duplicates_match_the_oracle( oracle, native, step, lin ):
-    # exact duplicates merged ( reference tests/test_SdotPlanNd.py:1436-1452 ): each duplicate's cell empty, its weight
-    # `k gap` under its representative's, the cluster's mass its target; the oracle's iterations and weights
+@pytest.mark.parametrize( "name", H.EXACT_CLOUDS + H.NEAR_CLOUDS )
+def test_aggregation_matches_the_oracle( oracle, native, name, step, lin ):
+    # exact duplicates and near-coincident seeds merged ( reference tests/test_SdotPlanNd.py:1436-1452 ): converged ( aggregated
+    # unless only exact duplicates ), the clusters found, each cluster's mass its target, a duplicate's cell empty and its
+    # weight `k gap` under its representative's; the oracle's iterations, weights and clusters
     from sdot import Iterative, OtProblem, SumOfDiracs, Tuning, UniformBox
-    for name, ( pos, groups ) in H.degenerate_clouds().items():
-        n = len( pos )
-        tol = 1e-11 / n
-        say( "case duplicates", name, step, lin )
-        it = Iterative( tol = tol, max_iter = 60, continuation = "never", tuning = Tuning( step = step, linear_solver = lin, linear_tol = 1e-13 ) )
-        sol = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( it )
-        st = sol.stats
-        w = sol.weights.cpu().numpy()
-        m = sol.cell_masses.cpu().numpy()
-        H.check_aggregated( name, st, w, m, np.full( n, 1 / n ), sol.clusters.cpu().numpy(), groups, tol )
-        assert st[ "status" ] == "converged" and st[ "duplicates" ] == "checked" and "exact duplicates" in st[ "aggregation" ]
-        gap = H.duplicate_gap( pos )
-        for members, _ in groups:
-            for k, mm in enumerate( members[ 1: ], 1 ):
+    pos, groups = H.degenerate_clouds()[ name ]
+    n = len( pos )
+    tol = 1e-11 / n
+    say( "case aggregation", name, step, lin )
+    it = Iterative( tol = tol, max_iter = 60, continuation = "never", tuning = Tuning( step = step, linear_solver = lin, linear_tol = 1e-14 ) )
+    sol = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( it )
+    st = sol.stats
+    w = sol.weights.cpu().numpy()
+    m = sol.cell_masses.cpu().numpy()
+    cl = sol.clusters.cpu().numpy()
+    H.check_aggregated( name, st, w, m, np.full( n, 1 / n ), cl, groups, tol )
+    assert st[ "duplicates" ] == "checked"
+    assert st[ "nb_duplicates" ] == sum( len( g ) - 1 for g, exact in groups if exact )
+    gap = H.duplicate_gap( pos )
+    for members, exact in groups:
+        for k, mm in enumerate( members[ 1: ], 1 ):
+            if exact:
                 assert w[ mm ] == w[ members[ 0 ] ] - k * gap
-        ro = oracle.solve( pos, tol = tol, maxit = 60, step = step, dups = H.brute_duplicates( pos ), gap = gap, lin_tol = 1e-13 )
-        assert ro[ "stats" ][ "status" ] == 1
-        if lin == "cg":                                  # ( the multigrid's directions differ within the linear tolerance )
-            assert st[ "nb_iter" ] == ro[ "stats" ][ "nb_iter" ] and st[ "nb_diag" ] == ro[ "stats" ][ "nb_diag" ], name
-        assert np.abs( w - ro[ "weights" ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max(), name
+    dups = H.brute_duplicates( pos )
+    ro = oracle.solve( pos, tol = tol, maxit = 60, step = step, dups = dups if len( dups ) else None, gap = gap, margin = 4.0,
+                       lin_tol = 1e-14, lin_maxit = 200000 )
+    assert ro[ "stats" ][ "status" ] in ( 1, 7 )
+    assert st[ "status" ] == ( "converged" if ro[ "stats" ][ "status" ] == 1 else "converged (aggregated)" )
+    assert np.array_equal( cl, ro[ "clusters" ] )
+    if lin == "cg":                                      # ( the multigrid's directions differ within the linear tolerance )
+        assert st[ "nb_iter" ] == ro[ "stats" ][ "nb_iter" ], name
+    # the weights agree where the doubles place them: a merged pair's split is only known to its floor
+    alone = np.setdiff1d( np.arange( n ), [ i for g, _ in groups for i in g ] )
+    assert np.abs( w[ alone ] - ro[ "weights" ][ alone ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max(), name
 
 
 def test_without_the_merge_duplicates_do_not_converge( native ):

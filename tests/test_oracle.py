@@ -560,7 +560,7 @@ def test_gaussian_continuation_widens_the_gaussians( oracle ):
 
 
 @pytest.mark.parametrize( "step", [ "trials", "limits" ] )
-@pytest.mark.parametrize( "name", [ "exact pair", "exact triple", "two pairs" ] )
+@pytest.mark.parametrize( "name", H.EXACT_CLOUDS )
 def test_exact_duplicates_are_merged( oracle, name, step ):
     # each duplicate's weight tied under its representative's, its target carried by it: converged, the clusters' masses
     # their targets ( reference tests/test_SdotPlanNd.py:1436-1452 )
@@ -570,7 +570,8 @@ def test_exact_duplicates_are_merged( oracle, name, step ):
     dups = H.brute_duplicates( pos )
     r = oracle.solve( pos, tol = tol, maxit = 60, step = step, dups = dups, gap = H.duplicate_gap( pos ), lin_tol = 1e-13 )
     nu = np.full( n, 1.0 / n )
-    H.check_aggregated( f"{ name } { step }", r[ "stats" ], r[ "weights" ], r[ "masses" ], nu, None, groups, tol )
+    H.check_aggregated( f"{ name } { step }", r[ "stats" ], r[ "weights" ], r[ "masses" ], nu, r[ "clusters" ], groups, tol )
+    assert r[ "stats" ][ "nb_duplicates" ] == sum( len( g ) - 1 for g, _ in groups )
     # the k-th duplicate of a representative sits `k gap` under it ( Aggregation.h:111-117 )
     gap = H.duplicate_gap( pos )
     for members, _ in groups:
@@ -608,3 +609,30 @@ def test_without_the_merge_the_duplicates_cannot_converge( oracle ):
     pos, _ = H.degenerate_clouds()[ "exact pair" ]
     r = oracle.solve( pos, tol = 1e-11 / len( pos ), maxit = 30 )
     assert r[ "stats" ][ "status" ] != 1
+
+
+# ---- the aggregation of near-coincident seeds ( reference sdotplan/Aggregation.h, tests/test_SdotPlanNd.py:1436-1452 ) ------
+
+
+@pytest.mark.parametrize( "step", [ "trials", "limits" ] )
+@pytest.mark.parametrize( "name", H.NEAR_CLOUDS )
+def test_near_coincident_seeds_are_aggregated( oracle, name, step ):
+    # pairs the doubles cannot separate are merged ( margin 4, the reference's default ): converged ( aggregated ), the
+    # clusters found, each cluster's mass its target, every other cell its own
+    pos, groups = H.degenerate_clouds()[ name ]
+    n = len( pos )
+    tol = 1e-11 / n
+    dups = H.brute_duplicates( pos )
+    r = oracle.solve( pos, tol = tol, maxit = 60, step = step, dups = dups if len( dups ) else None, gap = H.duplicate_gap( pos ),
+                      margin = 4.0, lin_tol = 1e-14, lin_maxit = 200000 )
+    H.check_aggregated( f"{ name } { step }", r[ "stats" ], r[ "weights" ], r[ "masses" ], np.full( n, 1.0 / n ), r[ "clusters" ], groups, tol )
+    # the full problem's residual is the floor of the doubles: above the tolerance, far below the cells' masses
+    assert tol < r[ "stats" ][ "residual_full" ] < 1e-3 / n
+
+
+def test_without_the_aggregation_near_pairs_stagnate( oracle ):
+    # ( reference tests/test_SdotPlanNd.py:1449-1452: "without the aggregation: the floor" )
+    pos, _ = H.degenerate_clouds()[ "pair at 1e-12" ]
+    n = len( pos )
+    r = oracle.solve( pos, tol = 1e-11 / n, maxit = 60, lin_tol = 1e-14, lin_maxit = 200000 )
+    assert r[ "stats" ][ "status" ] != 1 and r[ "stats" ][ "nb_clusters" ] == 0
