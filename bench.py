@@ -120,8 +120,8 @@ def main():
     spmv_ms, spmv_launches = times[ "spmv" ]
     spmv_avg_ms = spmv_ms / max( spmv_launches, 1 )
     spmv_gbs = ( spmv_bytes / max( spmv_launches, 1 ) ) / ( spmv_avg_ms * 1e-3 ) / 1e9 if spmv_launches else 0.0
-    roof_spmv = dict( kernel = "k_cg_spmv_dot" if args.linear == "cg" else "k_pcg_spmv", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s", frac = spmv_gbs / HBM_PEAK_GBS,
-                      traffic = measured_traffic( "k_pcg_spmv" if args.linear != "cg" else "k_cg_spmv_dot", args ), avg_launch_ms = spmv_avg_ms,
+    roof_spmv = dict( kernel = "k_cg_spmv_dot" if args.linear == "cg" else "k_fcg_spmv", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s", frac = spmv_gbs / HBM_PEAK_GBS,
+                      traffic = measured_traffic( "k_fcg_spmv" if args.linear != "cg" else "k_cg_spmv_dot", args ), avg_launch_ms = spmv_avg_ms,
                       launches_timed = spmv_launches,
                       bytes_per_launch = spmv_bytes / max( spmv_launches, 1 ) )
     # the SpMV is timed one launch in sixteen: its share of the step is estimated from the CG iteration count

@@ -109,6 +109,32 @@ int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const
                        double *weights, double *history, long long *nb_steps, double *stats,
                        double *cell_masses, double *barycenters, double *cost, long long *clusters );
 
+/* THE BATCH: `k` independent solves at once ( the reference solves one problem per call; this entry point has no
+ * counterpart there and serves the batched configuration: many instances of uneven size ). Instance q owns the seeds
+ * `offsets[ q ] .. offsets[ q + 1 ]` of the concatenated arrays and has its own box and density; all share `options`
+ * ( the 41 doubles of `sdot_amd_solve_2d`, minus the image geometry, which is per instance ). The instances run on
+ * `nb_streams` HIP streams ( 1 .. 32 ), largest first, each stream driven by its own host thread with its own device
+ * buffers: an instance's results do not depend on what it ran beside. `stream` is waited on before the start; every
+ * output is complete when the call returns. No duplicates are merged ( `aggregate` is per-instance work done by
+ * `sdot_amd_solve_2d` ).
+ *   offsets      HOST   [ k + 1 ]  int64, offsets[ 0 ] == 0, increasing by at least 2
+ *   positions    DEVICE [ N ][ 2 ], nu DEVICE [ N ], w0 DEVICE [ N ] or null   ( N = offsets[ k ] )
+ *   boxes        HOST   [ k ][ 4 ]  each instance's domain
+ *   rho          HOST   [ k ]       the constant density ( SDOT_AMD_CONSTANT ); may be null otherwise
+ *   imgs         HOST   [ k ]       device pointers: each instance's image [ ny * nx ] ( SDOT_AMD_IMAGE ) or Gaussians
+ *                                   [ ng ][ 4 ] ( SDOT_AMD_GAUSSIANS ); may be null for a constant density
+ *   img_geo      HOST   [ k ][ 6 ]  x0, y0, hx, hy, nx, ny ( an image ) or ng in nx ( Gaussians ); may be null for a constant
+ *   weights, cell_masses DEVICE [ N ], barycenters DEVICE [ N ][ 2 ]  outputs, per instance as `sdot_amd_solve_2d`'s
+ *   stats        HOST   [ k ][ 31 ], costs HOST [ k ]  outputs
+ *   errors       HOST   [ k ][ 3 ]  output: per instance 0, or the error kind, code and value that stopped it
+ *   times        HOST   [ k ][ 2 ]  output ( may be null ): each instance's start and end, seconds from the call's start
+ * Returns 0, or an error of the batch itself ( an instance's failure is only in `errors` ). */
+int sdot_amd_solve_2d_batch( void *stream, int nb_streams, long long k, const long long *offsets, const double *positions,
+                             const double *nu, const double *w0, const double *options, const double *boxes, int density_kind,
+                             const double *rho, const double *const *imgs, const double *img_geo,
+                             double *weights, double *cell_masses, double *barycenters, double *stats, double *costs,
+                             long long *errors, double *times );
+
 /* THE TREE: the BSP tree of a 2D cloud ( median cut on the longest axis, at most `leaf` seeds per leaf,
  * perfect binary tree of depth ceil( log2( n / leaf ) ) + 1 in preorder ).
  * Replaces `bsp_build_card` -> `sdot::gpu2d::build_tree`

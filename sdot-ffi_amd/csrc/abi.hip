@@ -5,17 +5,25 @@
 #include "../../include/sdot_amd.h"
 #include "solver.h"
 #include "multigrid.h"
+#include "batch.h"
 
 #include <cmath>
 
 namespace sdot_amd {
 
+namespace {
+thread_local Arena *tl_arena = nullptr;
+}
+
 Arena &arena() {
+    if ( tl_arena ) return *tl_arena;
     static Arena a[ 16 ];
     int dev = 0;
     if ( hipGetDevice( &dev ) != hipSuccess || dev < 0 || dev >= 16 ) dev = 0;
     return a[ dev ];
 }
+
+void set_thread_arena( Arena *a ) { tl_arena = a; }
 
 Timing &timing() {
     static Timing t;
@@ -23,20 +31,7 @@ Timing &timing() {
 }
 
 template<class T>
-T *pinned( const std::string &name, size_t count ) {
-    static std::mutex mtx;
-    static std::map<std::string, std::pair<void *, size_t>> bufs;
-    std::lock_guard<std::mutex> lk( mtx );
-    auto &b = bufs[ name ];
-    const size_t bytes = std::max<size_t>( count * sizeof( T ), 64 );
-    if ( b.second < bytes ) {
-        if ( b.first ) (void) hipHostFree( b.first );
-        b.first = nullptr;
-        SDOT_HIP( hipHostMalloc( &b.first, bytes, hipHostMallocDefault ) );
-        b.second = bytes;
-    }
-    return reinterpret_cast<T *>( b.first );
-}
+T *pinned( const std::string &name, size_t count ) { return arena().host<T>( name, count ); }
 template double *pinned<double>( const std::string &, size_t );
 template unsigned *pinned<unsigned>( const std::string &, size_t );
 
@@ -219,6 +214,62 @@ int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const
         out.stats = stats;
         out.cost = cost;
         solve_2d( in, out, hipStream_t( stream ) );
+    } );
+}
+
+int sdot_amd_solve_2d_batch( void *stream, int nb_streams, long long k, const long long *offsets, const double *positions,
+                             const double *nu, const double *w0, const double *options, const double *boxes, int density_kind,
+                             const double *rho, const double *const *imgs, const double *img_geo,
+                             double *weights, double *cell_masses, double *barycenters, double *stats, double *costs,
+                             long long *errors, double *times ) {
+    return guarded( [&] {
+        SDOT_CHECK( k >= 1 && offsets && positions && nu && options && boxes && weights && cell_masses && barycenters && stats && costs && errors,
+                    "batch: a required array is null" );
+        SDOT_CHECK( offsets[ 0 ] == 0, "batch: offsets[ 0 ] must be 0" );
+        for ( long long q = 0; q < k; ++q ) {
+            SDOT_CHECK( offsets[ q + 1 ] - offsets[ q ] >= 2, "batch: every instance needs at least two seeds" );
+            check_box( boxes + 4 * q );
+        }
+        SDOT_CHECK( options[ O_MAXIT ] >= 0, "batch: negative maxit" );
+        SDOT_CHECK( density_kind == SDOT_AMD_CONSTANT || ( imgs && img_geo ), "batch: an image or Gaussians need imgs and img_geo" );
+        SDOT_CHECK( density_kind != SDOT_AMD_CONSTANT || rho, "batch: a constant density needs rho" );
+        SDOT_HIP( hipStreamSynchronize( hipStream_t( stream ) ) );
+        std::vector<BatchJob> jobs( ( size_t( k ) ) );
+        std::vector<std::vector<double>> opts( ( size_t( k ) ) );
+        for ( long long q = 0; q < k; ++q ) {
+            BatchJob &J = jobs[ size_t( q ) ];
+            const I64 b = offsets[ q ], n = offsets[ q + 1 ] - b;
+            opts[ size_t( q ) ].assign( options, options + NB_OPTS );
+            double *o = opts[ size_t( q ) ].data();
+            o[ O_AGG_NB_DUPS ] = 0;
+            o[ O_AGG_MARGIN ] = 0;
+            o[ O_MIN_SCALE ] = 0;                        // ( the continuation's smallest width: from each instance's own density )
+            double geo[ 4 ] = { 0, 0, 1, 1 };
+            int nx = 0, ny = 0;
+            if ( density_kind != SDOT_AMD_CONSTANT ) {
+                const double *g = img_geo + 6 * q;
+                for ( int c = 0; c < 4; ++c ) { geo[ c ] = g[ c ]; o[ O_IMG_X0 + c ] = g[ c ]; }
+                nx = int( g[ 4 ] ); ny = int( g[ 5 ] );
+                o[ O_IMG_NX ] = g[ 4 ]; o[ O_IMG_NY ] = g[ 5 ];
+            }
+            J.in.n = n;
+            J.in.positions = positions + 2 * b;
+            J.in.nu = nu + b;
+            J.in.w0 = w0 ? w0 + b : nullptr;
+            J.in.options = o;
+            for ( int c = 0; c < 4; ++c ) J.in.box[ c ] = boxes[ 4 * q + c ];
+            J.in.dens = make_density( density_kind, rho ? rho[ q ] : 1.0, imgs ? imgs[ q ] : nullptr, nx, ny, geo );
+            J.out.weights = weights + b;
+            J.out.cell_masses = cell_masses + b;
+            J.out.barycenters = barycenters + 2 * b;
+            J.out.stats = stats + size_t( NB_STATS ) * size_t( q );
+            J.out.cost = costs + q;
+        }
+        solve_2d_batch( jobs, nb_streams );
+        for ( long long q = 0; q < k; ++q ) {
+            errors[ 3 * q ] = jobs[ size_t( q ) ].kind; errors[ 3 * q + 1 ] = jobs[ size_t( q ) ].code; errors[ 3 * q + 2 ] = jobs[ size_t( q ) ].value;
+            if ( times ) { times[ 2 * q ] = jobs[ size_t( q ) ].t_start; times[ 2 * q + 1 ] = jobs[ size_t( q ) ].t_end; }
+        }
     } );
 }
 

@@ -51,8 +51,24 @@ inline void check_launch( const char *name ) {
 /// synchronizing call and costs far more than a diagram ). `bytes_held()` is the footprint.
 struct Arena {
     struct Buf { void *ptr = nullptr; size_t bytes = 0; };
-    std::map<std::string, Buf> bufs;
+    std::map<std::string, Buf> bufs, hbufs;          ///< device buffers; pinned host staging
     std::mutex mtx;
+
+    /// pinned host staging, grow-only ( readbacks of the few scalars Newton decides on )
+    template<class T>
+    T *host( const std::string &name, size_t count ) {
+        std::lock_guard<std::mutex> lk( mtx );
+        Buf &b = hbufs[ name ];
+        const size_t bytes = std::max<size_t>( count * sizeof( T ), 64 );
+        if ( b.bytes < bytes ) {
+            if ( b.ptr ) (void) hipHostFree( b.ptr );
+            b.ptr = nullptr;
+            b.bytes = 0;
+            SDOT_HIP( hipHostMalloc( &b.ptr, bytes, hipHostMallocDefault ) );
+            b.bytes = bytes;
+        }
+        return reinterpret_cast<T *>( b.ptr );
+    }
 
     template<class T>
     T *get( const std::string &name, size_t count ) {
@@ -85,11 +101,16 @@ struct Arena {
         for ( auto &kv : bufs )
             if ( kv.second.ptr ) (void) hipFree( kv.second.ptr );
         bufs.clear();
+        for ( auto &kv : hbufs )
+            if ( kv.second.ptr ) (void) hipHostFree( kv.second.ptr );
+        hbufs.clear();
     }
 };
 
-/// the arena of the current device
+/// the arena of the calling thread: its own when `set_thread_arena` gave it one ( a worker of a batch ), else the
+/// current device's
 Arena &arena();
+void set_thread_arena( Arena *a );
 
 /// pinned host staging, grow-only ( readbacks of the few scalars Newton decides on )
 template<class T>

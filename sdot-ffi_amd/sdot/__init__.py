@@ -8,6 +8,8 @@ from .OtProblem import Direct, Iterative, OtNotConverged, OtProblem, Tuning, ot_
 from .PowerDiagram import PowerDiagram, axis_aligned_box, box_half_spaces
 from .SdotPlanNd import SdotPlanNd
 from ._native import KernelFailure, SdotError
+from .batch import solve_batch
 
 __all__ = [ "Image", "SumOfDiracs", "SumOfGaussians", "UniformBox", "Direct", "Iterative", "OtNotConverged", "OtProblem", "Tuning",
-            "ot_solve", "PowerDiagram", "axis_aligned_box", "box_half_spaces", "SdotPlanNd", "KernelFailure", "SdotError" ]
+            "ot_solve", "PowerDiagram", "axis_aligned_box", "box_half_spaces", "SdotPlanNd", "KernelFailure", "SdotError",
+            "solve_batch" ]

@@ -29,7 +29,7 @@ CARD_OPTIONS = [ "tol_abs", "tol_rel", "t_min", "mult_ok", "factor", "maxit", "m
 EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_bytes", "sdot_amd_release", "sdot_amd_set_timing",
             "sdot_amd_kernel_times", "sdot_amd_measures_2d",
             "sdot_amd_solve_2d", "sdot_amd_tree_nodes", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
-            "sdot_amd_linear_solves_2d" ]
+            "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch" ]
 
 CONSTANT, IMAGE = 0, 1
 
@@ -68,12 +68,13 @@ def lib():
         L.sdot_amd_build_tree_2d.argtypes = [ _P, _LL, _P, _I, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_refresh_majorants_2d.argtypes = [ _P, _LL, _LL, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_linear_solves_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _LL, _P, _P, _P ]
+        L.sdot_amd_solve_2d_batch.argtypes = [ _P, _I, _LL, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_set_timing.argtypes = [ _I ]
         L.sdot_amd_set_timing.restype = _I
         L.sdot_amd_kernel_times.argtypes = [ _P, _P, _P, _P, _I ]
         L.sdot_amd_kernel_times.restype = _I
         for name in ( "sdot_amd_measures_2d", "sdot_amd_solve_2d", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
-                      "sdot_amd_linear_solves_2d" ):
+                      "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch" ):
             getattr( L, name ).restype = _I
         if L.sdot_amd_abi_version() != 1:
             raise RuntimeError( "libsdot_amd.so: unexpected ABI version" )
@@ -189,6 +190,64 @@ def solve_2d( positions, nu, w0, options, box, density, dups = None ):
                                      _ptr( masses ), _ptr( bary ), cost.ctypes.data_as( _P ), _ptr( clusters ) ) )
     return dict( weights = weights, cell_masses = masses, barycenters = bary, clusters = clusters, cost = float( cost[ 0 ] ),
                  stats = stats, history = hist[ :int( nb_steps[ 0 ] ) ] )
+
+
+def solve_2d_batch( instances, options, streams = 8 ):
+    """`k` independent solves at once ( see `sdot_amd_solve_2d_batch` ): `instances` is a list of `( positions, nu, w0,
+    box, density )`, every density of the same kind; `options` the 41-double vector they share. Returns one dict per
+    instance: device tensors ( views of the concatenated outputs ), `stats`, `cost`, `error` ( kind, code, value: 0 when
+    it converged or stopped on its own ), `times` ( start, end in seconds from the call's start )."""
+    T = torch()
+    k = len( instances )
+    if k == 0:
+        return []
+    dev = device()
+    pos = [ on_device( p ).reshape( -1, 2 ) for p, _, _, _, _ in instances ]
+    sizes = [ int( p.shape[ 0 ] ) for p in pos ]
+    offsets = np.zeros( k + 1, np.int64 )
+    offsets[ 1: ] = np.cumsum( sizes )
+    P = T.cat( pos ).contiguous()
+    nu = T.cat( [ on_device( q[ 1 ] ).reshape( -1 ) for q in instances ] ).contiguous()
+    has_w0 = [ q[ 2 ] is not None for q in instances ]
+    w0 = None
+    if any( has_w0 ):
+        w0 = T.cat( [ on_device( q[ 2 ] ).reshape( -1 ) if q[ 2 ] is not None else T.zeros( sizes[ i ], dtype = T.float64, device = dev )
+                      for i, q in enumerate( instances ) ] ).contiguous()
+    boxes = np.ascontiguousarray( np.asarray( [ q[ 3 ] for q in instances ], dtype = np.float64 ).reshape( k, 4 ) )
+    args = [ q[ 4 ].native_args( device_tensor = on_device ) for q in instances ]
+    kinds = { a[ 0 ] for a in args }
+    if len( kinds ) != 1:
+        raise ValueError( "solve_2d_batch: every instance's density must be of the same kind" )
+    kind = kinds.pop()
+    rho = np.asarray( [ a[ 1 ] for a in args ], dtype = np.float64 )
+    imgs = [ a[ 2 ] for a in args ]                       # ( kept alive through the call )
+    img_ptrs = ( ctypes.c_void_p * k )( *[ ( None if im is None else im.data_ptr() ) for im in imgs ] )
+    geo = np.zeros( ( k, 6 ) )
+    for i, a in enumerate( args ):
+        g = a[ 5 ]
+        if g is not None:
+            geo[ i, :4 ] = np.asarray( g, dtype = np.float64 ).reshape( -1 )[ :4 ]
+        geo[ i, 4 ], geo[ i, 5 ] = a[ 3 ], a[ 4 ]
+    opts, optp = _host( options )
+    N = int( offsets[ -1 ] )
+    weights = T.empty( N, dtype = T.float64, device = dev )
+    masses = T.empty( N, dtype = T.float64, device = dev )
+    bary = T.empty( ( N, 2 ), dtype = T.float64, device = dev )
+    stats = np.zeros( ( k, 31 ) )
+    costs = np.zeros( k )
+    errors = np.zeros( ( k, 3 ), np.int64 )
+    times = np.zeros( ( k, 2 ) )
+    _check( lib().sdot_amd_solve_2d_batch( _stream(), int( streams ), k, offsets.ctypes.data_as( _P ), _ptr( P ), _ptr( nu ), _ptr( w0 ), optp,
+                                           boxes.ctypes.data_as( _P ), kind, rho.ctypes.data_as( _P ), ctypes.cast( img_ptrs, _P ),
+                                           geo.ctypes.data_as( _P ), _ptr( weights ), _ptr( masses ), _ptr( bary ),
+                                           stats.ctypes.data_as( _P ), costs.ctypes.data_as( _P ), errors.ctypes.data_as( _P ),
+                                           times.ctypes.data_as( _P ) ) )
+    out = []
+    for i in range( k ):
+        a, b = int( offsets[ i ] ), int( offsets[ i + 1 ] )
+        out.append( dict( weights = weights[ a:b ], cell_masses = masses[ a:b ], barycenters = bary[ a:b ], stats = stats[ i ],
+                          cost = float( costs[ i ] ), error = tuple( int( v ) for v in errors[ i ] ), times = tuple( times[ i ] ) ) )
+    return out
 
 
 def build_tree_2d( positions, leaf = 10 ):

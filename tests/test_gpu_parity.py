@@ -411,13 +411,11 @@ def test_aggregation_matches_the_oracle( oracle, native, name, step, lin ):
     cl = sol.clusters.cpu().numpy()
     H.check_aggregated( name, st, w, m, np.full( n, 1 / n ), cl, groups, tol )
     assert st[ "duplicates" ] == "checked"
-    assert st[ "nb_duplicates" ] == sum( len( g ) - 1 for g, exact in groups if exact )
-    gap = H.duplicate_gap( pos )
-    for members, exact in groups:
-        for k, mm in enumerate( members[ 1: ], 1 ):
-            if exact:
-                assert w[ mm ] == w[ members[ 0 ] ] - k * gap
     dups = H.brute_duplicates( pos )
+    assert st[ "nb_duplicates" ] == len( dups )
+    gap = H.duplicate_gap( pos )
+    for k, ( d, r ) in enumerate( dups ):
+        assert w[ d ] == w[ r ] - ( 1 + int( ( dups[ :k, 1 ] == r ).sum() ) ) * gap
     ro = oracle.solve( pos, tol = tol, maxit = 60, step = step, dups = dups if len( dups ) else None, gap = gap, margin = 4.0,
                        lin_tol = 1e-14, lin_maxit = 200000 )
     assert ro[ "stats" ][ "status" ] in ( 1, 7 )
@@ -481,3 +479,43 @@ def test_full_size_uniform_million( native ):
     assert sol.converged and np.abs( m - 1e-6 ).max() <= 1e-9
     assert abs( m.sum() - 1 ) < 1e-10
     assert sol.weights.cpu().numpy()[ 0 ] == 0
+
+
+# ---- many instances at once ( the batched configuration: uneven sizes, one image each ) -------------------------------
+
+
+def _batch_problems( k = 10, seed = 11 ):
+    from sdot import Image, OtProblem, SumOfDiracs
+    rng = np.random.default_rng( seed )
+    sizes = np.round( np.exp( rng.uniform( np.log( 1e3 ), np.log( 4e4 ), k ) ) ).astype( int )
+    out = []
+    for q, n in enumerate( sizes ):
+        img = H.bumps_image( 64, 64, rng, k = 4 )
+        out.append( OtProblem( SumOfDiracs( rng.uniform( 0, 1, ( int( n ), 2 ) ) ), Image( img, origin = [ 0, 0 ], frame = np.eye( 2 ) / 64 ) ) )
+    return out
+
+
+def test_batch_results_are_those_of_single_solves( native ):
+    # each instance's plan is the one it gets alone, to the bit: nothing is shared between the streams but the device
+    from sdot import Iterative, solve_batch
+    probs = _batch_problems()
+    it = Iterative( tol = 1e-9, continuation = "never" )
+    say( "case batch of", len( probs ), "sizes", [ p.nb_diracs for p in probs ] )
+    plans = solve_batch( probs, it, streams = 4 )
+    assert len( plans ) == len( probs )
+    for pb, pl in zip( probs, plans ):
+        alone = pb.solve( it )
+        assert pl.converged and alone.converged
+        assert pl.stats[ "nb_iter" ] == alone.stats[ "nb_iter" ] and pl.stats[ "nb_diag" ] == alone.stats[ "nb_diag" ]
+        assert torch_equal( pl.weights, alone.weights ) and torch_equal( pl.cell_masses, alone.cell_masses )
+        assert pl.cost == alone.cost
+        assert np.abs( pl.cell_masses.cpu().numpy() - 1.0 / pb.nb_diracs ).max() <= 1e-9
+    # and again with one stream, then with as many streams as instances: the same plans
+    for streams in ( 1, len( probs ) ):
+        again = solve_batch( probs, it, streams = streams )
+        for a, b in zip( plans, again ):
+            assert torch_equal( a.weights, b.weights )
+
+
+def torch_equal( a, b ):
+    return bool( ( a == b ).all().item() )
