@@ -135,6 +135,28 @@ int sdot_amd_solve_2d_batch( void *stream, int nb_streams, long long k, const lo
                              double *weights, double *cell_masses, double *barycenters, double *stats, double *costs,
                              long long *errors, double *times );
 
+/* THE SERVER: continuous batching of solves ( no counterpart in the reference, which solves one problem per call; for
+ * callers that serve a stream of requests ). `nb_slots` slots ( 1 .. 32 ), each a HIP stream driven by its own host
+ * thread with its own device buffers, kept between its requests. A free slot takes the oldest waiting request whose
+ * estimated footprint fits `memory_budget` bytes beside what the other slots hold; requests that arrive while others
+ * run are admitted as slots and memory free up, and their work overlaps on the device. A request's result does not
+ * depend on what ran beside it. Returns a handle, or 0 on error ( `sdot_amd_last_error` ). */
+long long sdot_amd_server_start( int nb_slots, double memory_budget );
+
+/* A request: the arguments of `sdot_amd_solve_2d` minus the stream, the duplicates, the history and the clusters. Every
+ * array must stay alive, and the outputs untouched, until `sdot_amd_server_wait` returns for its ticket. */
+int sdot_amd_server_submit( long long server, long long n, const double *positions, const double *nu, const double *w0,
+                            const double *options, const double *box, int density_kind, double rho, const double *img,
+                            double *weights, double *cell_masses, double *barycenters, double *stats, double *cost,
+                            long long *ticket );
+
+/* Blocks until the request is done. `error` HOST [ 3 ]: 0, or the kind, code and value that stopped it; `times` HOST [ 3 ]:
+ * its arrival, start and end, seconds from the server's start ( may be null ). */
+int sdot_amd_server_wait( long long server, long long ticket, long long *error, double *times );
+
+/* Stops the server once its running requests are done ( waiting requests are dropped ) and frees its slots. */
+int sdot_amd_server_stop( long long server );
+
 /* THE TREE: the BSP tree of a 2D cloud ( median cut on the longest axis, at most `leaf` seeds per leaf,
  * perfect binary tree of depth ceil( log2( n / leaf ) ) + 1 in preorder ).
  * Replaces `bsp_build_card` -> `sdot::gpu2d::build_tree`

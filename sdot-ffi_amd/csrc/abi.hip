@@ -6,6 +6,7 @@
 #include "solver.h"
 #include "multigrid.h"
 #include "batch.h"
+#include "server.h"
 
 #include <cmath>
 
@@ -270,6 +271,81 @@ int sdot_amd_solve_2d_batch( void *stream, int nb_streams, long long k, const lo
             errors[ 3 * q ] = jobs[ size_t( q ) ].kind; errors[ 3 * q + 1 ] = jobs[ size_t( q ) ].code; errors[ 3 * q + 2 ] = jobs[ size_t( q ) ].value;
             if ( times ) { times[ 2 * q ] = jobs[ size_t( q ) ].t_start; times[ 2 * q + 1 ] = jobs[ size_t( q ) ].t_end; }
         }
+    } );
+}
+
+namespace {
+std::mutex g_servers_mtx;
+std::map<long long, std::unique_ptr<Server>> g_servers;
+long long g_next_server = 1;
+
+Server *find_server( long long h ) {
+    std::lock_guard<std::mutex> lk( g_servers_mtx );
+    auto it = g_servers.find( h );
+    SDOT_CHECK( it != g_servers.end(), "server: unknown handle" );
+    return it->second.get();
+}
+}
+
+long long sdot_amd_server_start( int nb_slots, double memory_budget ) {
+    long long h = 0;
+    const int rc = guarded( [&] {
+        auto s = std::make_unique<Server>( nb_slots, memory_budget );
+        std::lock_guard<std::mutex> lk( g_servers_mtx );
+        h = g_next_server++;
+        g_servers[ h ] = std::move( s );
+    } );
+    return rc == 0 ? h : 0;
+}
+
+int sdot_amd_server_submit( long long server, long long n, const double *positions, const double *nu, const double *w0,
+                            const double *options, const double *box, int density_kind, double rho, const double *img,
+                            double *weights, double *cell_masses, double *barycenters, double *stats, double *cost,
+                            long long *ticket ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 2 && positions && nu && options && box && weights && cell_masses && barycenters && stats && cost && ticket,
+                    "server: a required array is null" );
+        check_box( box );
+        SDOT_CHECK( options[ O_MAXIT ] >= 0, "server: negative maxit" );
+        Server *S = find_server( server );
+        auto r = std::make_unique<Request>();
+        r->opts.assign( options, options + NB_OPTS );
+        r->opts[ O_AGG_NB_DUPS ] = 0;
+        const double geo[ 4 ] = { options[ O_IMG_X0 ], options[ O_IMG_Y0 ], options[ O_IMG_HX ], options[ O_IMG_HY ] };
+        const int nx = int( options[ O_IMG_NX ] ), ny = int( options[ O_IMG_NY ] );
+        BatchJob &J = r->job;
+        J.in.n = n; J.in.positions = positions; J.in.nu = nu; J.in.w0 = w0;
+        for ( int c = 0; c < 4; ++c ) J.in.box[ c ] = box[ c ];
+        J.in.dens = make_density( density_kind, rho, img, nx, ny, geo );
+        J.out.weights = weights; J.out.cell_masses = cell_masses; J.out.barycenters = barycenters; J.out.stats = stats; J.out.cost = cost;
+        // the footprint estimate: the buffers of a solve per seed ( two diagram slots with their rows, the multigrid's
+        // levels, the step's edges ), the image's blurred copies, and a fixed part
+        const double img_bytes = density_kind == SDOT_AMD_IMAGE ? 3.0 * 8.0 * double( nx ) * double( ny ) : 0.0;
+        r->bytes = 2048.0 * double( n ) + img_bytes + double( 32 << 20 );
+        *ticket = S->submit( std::move( r ) );
+    } );
+}
+
+int sdot_amd_server_wait( long long server, long long ticket, long long *error, double *times ) {
+    return guarded( [&] {
+        Server *S = find_server( server );
+        std::unique_ptr<Request> r = S->wait( ticket );
+        if ( error ) { error[ 0 ] = r->job.kind; error[ 1 ] = r->job.code; error[ 2 ] = r->job.value; }
+        if ( times ) { times[ 0 ] = r->t_arrival; times[ 1 ] = r->job.t_start; times[ 2 ] = r->job.t_end; }
+    } );
+}
+
+int sdot_amd_server_stop( long long server ) {
+    return guarded( [&] {
+        std::unique_ptr<Server> s;
+        {
+            std::lock_guard<std::mutex> lk( g_servers_mtx );
+            auto it = g_servers.find( server );
+            SDOT_CHECK( it != g_servers.end(), "server: unknown handle" );
+            s = std::move( it->second );
+            g_servers.erase( it );
+        }
+        s.reset();
     } );
 }
 

@@ -9,7 +9,8 @@ from .PowerDiagram import PowerDiagram, axis_aligned_box, box_half_spaces
 from .SdotPlanNd import SdotPlanNd
 from ._native import KernelFailure, SdotError
 from .batch import solve_batch
+from .serve import Server
 
 __all__ = [ "Image", "SumOfDiracs", "SumOfGaussians", "UniformBox", "Direct", "Iterative", "OtNotConverged", "OtProblem", "Tuning",
             "ot_solve", "PowerDiagram", "axis_aligned_box", "box_half_spaces", "SdotPlanNd", "KernelFailure", "SdotError",
-            "solve_batch" ]
+            "solve_batch", "Server" ]

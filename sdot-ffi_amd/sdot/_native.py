@@ -29,7 +29,8 @@ CARD_OPTIONS = [ "tol_abs", "tol_rel", "t_min", "mult_ok", "factor", "maxit", "m
 EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_bytes", "sdot_amd_release", "sdot_amd_set_timing",
             "sdot_amd_kernel_times", "sdot_amd_measures_2d",
             "sdot_amd_solve_2d", "sdot_amd_tree_nodes", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
-            "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch" ]
+            "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_start", "sdot_amd_server_submit",
+            "sdot_amd_server_wait", "sdot_amd_server_stop" ]
 
 CONSTANT, IMAGE = 0, 1
 
@@ -69,12 +70,18 @@ def lib():
         L.sdot_amd_refresh_majorants_2d.argtypes = [ _P, _LL, _LL, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_linear_solves_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _LL, _P, _P, _P ]
         L.sdot_amd_solve_2d_batch.argtypes = [ _P, _I, _LL, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_server_start.argtypes = [ _I, _D ]
+        L.sdot_amd_server_start.restype = _LL
+        L.sdot_amd_server_submit.argtypes = [ _LL, _LL, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_server_wait.argtypes = [ _LL, _LL, _P, _P ]
+        L.sdot_amd_server_stop.argtypes = [ _LL ]
         L.sdot_amd_set_timing.argtypes = [ _I ]
         L.sdot_amd_set_timing.restype = _I
         L.sdot_amd_kernel_times.argtypes = [ _P, _P, _P, _P, _I ]
         L.sdot_amd_kernel_times.restype = _I
         for name in ( "sdot_amd_measures_2d", "sdot_amd_solve_2d", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
-                      "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch" ):
+                      "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_submit", "sdot_amd_server_wait",
+                      "sdot_amd_server_stop" ):
             getattr( L, name ).restype = _I
         if L.sdot_amd_abi_version() != 1:
             raise RuntimeError( "libsdot_amd.so: unexpected ABI version" )

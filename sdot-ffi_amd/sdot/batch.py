@@ -44,7 +44,7 @@ def solve_batch( problems, settings = None, streams = 8 ):
     for pb, r in zip( problems, res ):
         if r[ "error" ][ 0 ] != 0:
             raise _native.SdotError( f"solve_batch: an instance failed ( kind { r[ 'error' ][ 0 ] }, code { r[ 'error' ][ 1 ] }, "
-                                     f"value { r[ 'error' ][ 2 ] } )" )
+                                     f"value { r[ 'error' ][ 2 ] } )", *r[ "error" ] )
         plan = SdotPlanNd.__new__( SdotPlanNd )
         plan.problem = pb
         plan.weights = r[ "weights" ]

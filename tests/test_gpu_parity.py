@@ -519,3 +519,46 @@ def test_batch_results_are_those_of_single_solves( native ):
 
 def torch_equal( a, b ):
     return bool( ( a == b ).all().item() )
+
+
+# ---- serving: requests admitted as they come ( continuous batching ) ---------------------------------------------------
+
+
+def test_server_results_are_those_of_single_solves( native ):
+    # requests arriving at Poisson times while others run: each plan is the one it gets alone, to the bit, and the
+    # requests' work overlaps on the device
+    import time
+    from sdot import Iterative, Server
+    probs = _batch_problems( k = 12, seed = 12 )
+    it = Iterative( tol = 1e-9, continuation = "never" )
+    gaps = np.random.default_rng( 3 ).exponential( 0.01, len( probs ) )
+    say( "case server of", len( probs ), "requests" )
+    with Server( slots = 4 ) as srv:
+        tickets = []
+        for pb, g in zip( probs, gaps ):
+            time.sleep( float( g ) )
+            tickets.append( srv.submit( pb, it ) )
+        plans = [ t.result() for t in tickets ]
+    for pb, pl in zip( probs, plans ):
+        alone = pb.solve( it )
+        assert pl.converged and alone.converged
+        assert torch_equal( pl.weights, alone.weights ) and torch_equal( pl.cell_masses, alone.cell_masses ) and pl.cost == alone.cost
+    times = np.asarray( [ t.times for t in tickets ] )
+    assert ( times[ :, 0 ] <= times[ :, 1 ] ).all() and ( times[ :, 1 ] < times[ :, 2 ] ).all()
+    overlap = any( times[ i, 1 ] < times[ j, 2 ] and times[ j, 1 ] < times[ i, 2 ] for i in range( len( times ) ) for j in range( i ) )
+    assert overlap
+
+
+def test_server_admission_keeps_to_the_memory_budget( native ):
+    # a budget that holds one request at a time: the requests run one after the other, in arrival order
+    from sdot import Iterative, Server
+    probs = _batch_problems( k = 4, seed = 13 )
+    it = Iterative( tol = 1e-9, continuation = "never" )
+    say( "case server with a small budget" )
+    with Server( slots = 4, memory = 1.0 ) as srv:
+        tickets = [ srv.submit( pb, it ) for pb in probs ]
+        plans = [ t.result() for t in tickets ]
+    assert all( p.converged for p in plans )
+    times = sorted( t.times[ 1: ] for t in tickets )
+    for ( s0, e0 ), ( s1, e1 ) in zip( times, times[ 1: ] ):
+        assert e0 <= s1
