@@ -70,23 +70,39 @@ __device__ void main_run( const S &c, int nb, double dx, double dy, double off, 
 /// the cut by `dx . v <= off`, in place; `CUT_NO_ROOM` leaves the cell intact
 template<class S>
 __device__ int cut( const S &c, int &nb, int cap, double dx, double dy, double off, int pid, Ev &ev ) {
+    // the signs, four vertices a step ( their reads issued together: the slots past `nb`, up to the next multiple of
+    // four, exist since `cap` is one, and are masked ), with the mask of the outside ones among the first 64
     int nb_out = 0;
-    for ( int i = 0; i < nb; ++i ) nb_out += side( c, i, dx, dy, off ) > 0;
+    unsigned long long out = 0;
+    for ( int i = 0; i < nb; i += 4 ) {
+        const double s0 = side( c, i, dx, dy, off ), s1 = side( c, i + 1, dx, dy, off );
+        const double s2 = side( c, i + 2, dx, dy, off ), s3 = side( c, i + 3, dx, dy, off );
+        const unsigned o = unsigned( s0 > 0 ) | unsigned( i + 1 < nb && s1 > 0 ) << 1 | unsigned( i + 2 < nb && s2 > 0 ) << 2 |
+                           unsigned( i + 3 < nb && s3 > 0 ) << 3;
+        nb_out += __popc( o );
+        if ( i < 64 ) out |= ( unsigned long long )( o ) << i;
+    }
     ev.side += nb;
     if ( nb_out == 0 ) return CUT_UNCHANGED;
     if ( nb_out == nb ) { nb = 0; return CUT_EMPTY; }
     int i1 = 0;
-    {
+    if ( nb <= 64 ) {
+        // the first outside vertex after an inside one; the outside vertices make one run iff there is one such
+        const unsigned long long full = nb == 64 ? ~0ull : ( 1ull << nb ) - 1;
+        const unsigned long long starts = out & ~( ( ( out << 1 ) | ( out >> ( nb - 1 ) ) ) & full );
+        i1 = __ffsll( ( long long )( starts ) ) - 1;
+        if ( __popcll( starts ) != 1 ) main_run( c, nb, dx, dy, off, i1, nb_out );
+    } else {
         bool prev = side( c, nb - 1, dx, dy, off ) > 0;
         for ( int i = 0; i < nb; ++i ) {
             const bool o = side( c, i, dx, dy, off ) > 0;
             if ( o && ! prev ) { i1 = i; break; }
             prev = o;
         }
-    }
-    for ( int o = 1, i = i1 + 1; o < nb_out; ++o, ++i ) {
-        if ( i == nb ) i = 0;
-        if ( ! ( side( c, i, dx, dy, off ) > 0 ) ) { main_run( c, nb, dx, dy, off, i1, nb_out ); break; }
+        for ( int o = 1, i = i1 + 1; o < nb_out; ++o, ++i ) {
+            if ( i == nb ) i = 0;
+            if ( ! ( side( c, i, dx, dy, off ) > 0 ) ) { main_run( c, nb, dx, dy, off, i1, nb_out ); break; }
+        }
     }
     const int nb_in = nb - nb_out, new_nb = nb_in + 2;
     if ( new_nb > cap ) return CUT_NO_ROOM;
@@ -128,15 +144,19 @@ template<class S>
 __device__ __forceinline__ bool can_cut( const S &c, int nb, const NodeRec &r, double p0x, double p0y, double w0, Ev &ev ) {
     ev.visits += 1;
     const double hax = r.wa_x / 2, hay = r.wa_y / 2, cb = w0 - r.wb;
-    for ( int i = 0; i < nb; ++i ) {
+    auto test = [&]( int i ) {
         const double vx = c.X( i ), vy = c.Y( i );
         const double X = vx + p0x, Y = vy + p0y;
         double yx = X + hax, yy = Y + hay;
         yx = yx < r.lo_x ? r.lo_x : ( yx > r.hi_x ? r.hi_x : yx );
         yy = yy < r.lo_y ? r.lo_y : ( yy > r.hi_y ? r.hi_y : yy );
         const double ux = yx - X, uy = yy - Y;
-        const double s = cb + ux * ux - vx * vx - r.wa_x * yx + uy * uy - vy * vy - r.wa_y * yy;
-        if ( s <= 0 ) { ev.prune += i + 1; return true; }
+        return cb + ux * ux - vx * vx - r.wa_x * yx + uy * uy - vy * vy - r.wa_y * yy <= 0;
+    };
+    for ( int i = 0; i < nb; i += 2 ) {                  // ( two vertices a step, their reads issued together; `cap` is even )
+        const bool t0 = test( i ), u1 = test( i + 1 ), t1 = i + 1 < nb && u1;
+        if ( t0 ) { ev.prune += i + 1; return true; }
+        if ( t1 ) { ev.prune += i + 2; return true; }
     }
     ev.prune += nb;
     return false;
@@ -194,9 +214,10 @@ __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *l
         if ( ! visit ) {                                  // pop: climb to the lowest pending height, take the sibling
             if ( ! pend ) break;
             const int hp = __ffsll( (long long) pend ) - 1;
-            while ( h < hp ) {
-                m = ( ( left >> h ) & 1 ) ? m - 1 : m - ( 1 << h );
-                ++h;
+            if ( h < hp ) {                              // up from `h` to `hp`: a left child's parent is 1 before it, a right
+                const unsigned long long R = ( ( 1ull << hp ) - 1 ) & ~( ( 1ull << h ) - 1 );   // child's `2^height` before
+                m -= __popcll( left & R ) + int( R & ~left );
+                h = hp;
             }
             const unsigned long long bit = 1ull << hp;
             if ( left & bit ) { m = m - 1 + ( 1 << hp ); left &= ~bit; }

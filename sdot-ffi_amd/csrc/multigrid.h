@@ -6,11 +6,12 @@
 namespace sdot_amd {
 
 struct MgOptions {
-    int    pack = 4;          ///< seeds per aggregate ( consecutive tree ranks )
+    int    pack = 8;          ///< seeds per aggregate ( consecutive tree ranks )
     int    nu = 2;            ///< Jacobi sweeps before and after the coarse correction
     double omega = 0.7;       ///< their damping
-    int    kcycle = 3;        ///< levels 1 .. kcycle take their coarse correction by two flexible CG steps ( a K-cycle ); below, one cycle
-    I64    bottom = 4096;     ///< the levels from the first of at most this many rows run as one single-workgroup kernel ( 0: none )
+    int    kcycle = 2;        ///< levels 1 .. kcycle take their coarse correction by two flexible CG steps ( a K-cycle ); below, one cycle
+    I64    bottom = 1024;     ///< the levels from the first of at most this many rows run as one single-workgroup kernel ( 0: none )
+    bool   graphs = true;     ///< the batches of iterations replayed from HIP graphs
 };
 
 /// `L d = b` ( `b` of zero sum ) by flexible CG preconditioned with one multigrid cycle whose aggregates are `pack`

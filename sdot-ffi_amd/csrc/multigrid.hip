@@ -24,6 +24,9 @@
 
 #include "multigrid.h"
 
+#include <algorithm>
+#include <cstdint>
+
 namespace sdot_amd {
 
 namespace {
@@ -564,6 +567,56 @@ void cycle( Hierarchy &H, int l, const double *r, double *x, const MgOptions &o,
     }
 }
 
+/// THE ITERATIONS AS A GRAPH: a batch of flexible CG iterations is ~100 launches of small kernels, whose issue costs
+/// more than their work on the coarse levels; replayed from a HIP graph it is one launch. A graph is valid as long as
+/// every kernel argument is: the key hashes them all ( the hierarchy's levels, vectors and sizes, the options ). The
+/// arena keeps the buffers of a solve of the same size, so Newton's successive systems ( two slots of rows ) replay
+/// two graphs. A few per thread are kept ( least recently used out ).
+struct GraphCache {
+    struct Entry { uint64_t key; hipGraphExec_t exec; hipGraph_t graph; uint64_t used; };
+    std::vector<Entry> entries;
+    uint64_t clock = 0;
+    ~GraphCache() {
+        for ( auto &e : entries ) { (void) hipGraphExecDestroy( e.exec ); (void) hipGraphDestroy( e.graph ); }
+    }
+    hipGraphExec_t find( uint64_t key ) {
+        for ( auto &e : entries ) if ( e.key == key ) { e.used = ++clock; return e.exec; }
+        return nullptr;
+    }
+    void put( uint64_t key, hipGraph_t g, hipGraphExec_t x ) {
+        if ( entries.size() >= 8 ) {
+            auto old = std::min_element( entries.begin(), entries.end(), []( const Entry &a, const Entry &b ) { return a.used < b.used; } );
+            (void) hipGraphExecDestroy( old->exec );
+            (void) hipGraphDestroy( old->graph );
+            entries.erase( old );
+        }
+        entries.push_back( Entry{ key, x, g, ++clock } );
+    }
+};
+thread_local GraphCache g_graphs;
+
+/// the stream graphs are captured on ( a capture needs a stream of its own: the legacy default stream cannot be
+/// captured ); one per thread, default priority
+struct CaptureStream {
+    hipStream_t s = nullptr;
+    ~CaptureStream() { if ( s ) (void) hipStreamDestroy( s ); }
+    hipStream_t get() {
+        if ( ! s && hipStreamCreateWithFlags( &s, hipStreamNonBlocking ) != hipSuccess ) { (void) hipGetLastError(); s = nullptr; }
+        return s;
+    }
+};
+thread_local CaptureStream g_capture;
+
+/// FNV-1a over bytes
+struct Hash {
+    uint64_t h = 1469598103934665603ull;
+    void add( const void *p, size_t nb ) {
+        const unsigned char *c = static_cast<const unsigned char *>( p );
+        for ( size_t i = 0; i < nb; ++i ) { h ^= c[ i ]; h *= 1099511628211ull; }
+    }
+    template<class T> void val( const T &v ) { add( &v, sizeof( T ) ); }
+};
+
 } // namespace
 
 int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, int maxit, const MgOptions &o, hipStream_t st, CgStats &cs,
@@ -641,11 +694,11 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
     k_pcg_first_dir<<<blocks_for( n, 256 ), 256, 0, st>>>( n, z, p, state );
     check_launch( "pcg init" );
     const int batch = 4;
-    int launched = 0;
-    for ( ;; ) {
+    // one batch of iterations on stream `st` ( the first launch of its SpMV timed when `timed` )
+    auto iterations = [&]( hipStream_t st, bool timed_first ) {
         for ( int c = 0; c < batch; ++c ) {
             Timing::Pending tp;
-            const bool timed = c == 0 && timing().begin( TK_SPMV, st, tp );
+            const bool timed = timed_first && c == 0 && timing().begin( TK_SPMV, st, tp );
             k_fcg_spmv<<<RED_BLOCKS, RED_THREADS, 0, st>>>( A0, p, r, q, part, state );
             if ( timed ) {
                 timing().end( tp, st );
@@ -657,6 +710,37 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
             k_fcg_zq<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, z, q, part, state );
             k_pcg_dir<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, z, p, part, state );
         }
+    };
+    // the graph of a batch, found by the hash of every argument its kernels take
+    hipGraphExec_t exec = nullptr;
+    if ( o.graphs ) {
+        Hash key;
+        key.val( A0 ); key.val( H.nl ); key.val( H.lb ); key.val( H.nc ); key.val( H.inv ); key.val( H.part ); key.val( H.kpart );
+        for ( int l = 0; l < H.nl; ++l ) {
+            key.val( H.A[ l ] ); key.val( H.x[ l ] ); key.val( H.t[ l ] ); key.val( H.r[ l ] ); key.val( H.c1[ l ] ); key.val( H.c2[ l ] );
+            key.val( H.v1[ l ] ); key.val( H.v2[ l ] ); key.val( H.r2[ l ] );
+        }
+        key.val( H.bottom ); key.val( o.pack ); key.val( o.nu ); key.val( o.omega ); key.val( o.kcycle ); key.val( o.bottom );
+        key.val( p ); key.val( q ); key.val( r ); key.val( z ); key.val( d ); key.val( state ); key.val( batch );
+        exec = g_graphs.find( key.h );
+        hipStream_t cs = exec ? nullptr : g_capture.get();
+        if ( ! exec && cs && hipStreamBeginCapture( cs, hipStreamCaptureModeThreadLocal ) == hipSuccess ) {
+            hipGraph_t g = nullptr;
+            iterations( cs, false );
+            const hipError_t e = hipStreamEndCapture( cs, &g );
+            if ( e == hipSuccess && g && hipGraphInstantiate( &exec, g, nullptr, nullptr, 0 ) == hipSuccess ) g_graphs.put( key.h, g, exec );
+            else {                                       // ( no graph: the batches are launched one by one )
+                exec = nullptr;
+                if ( g ) (void) hipGraphDestroy( g );
+            }
+        }
+        (void) hipGetLastError();
+    }
+    int launched = 0;
+    for ( ;; ) {
+        // the first batch launched one by one ( its SpMV timed live ); the next ones replayed from the graph
+        if ( launched == 0 || ! exec ) iterations( st, launched == 0 );
+        else SDOT_HIP( hipGraphLaunch( exec, st ) );
         check_launch( "pcg iterations" );
         launched += batch;
         SDOT_HIP( hipMemcpyAsync( hs, state, sizeof( double ) * S_NB_STATE, hipMemcpyDeviceToHost, st ) );
