@@ -12,6 +12,19 @@
 
 namespace sdot_amd {
 
+/// the smallest pixel value over blocks of `2^k x 2^k` pixels, level by level ( level 0: the image itself; none when
+/// `nl == 0` ): a lower bound of an image density over any box in two or four reads
+struct MinPyramid {
+    const double *v = nullptr;
+    int nl = 0;
+    int w[ 16 ] = {}, h[ 16 ] = {};
+    I64 off[ 16 ] = {};
+};
+
+/// the pyramid of the image `img[ ny ][ nx ]` into `buf` ( `min_pyramid_size( nx, ny )` doubles )
+I64 min_pyramid_size( int nx, int ny );
+MinPyramid build_min_pyramid( const double *img, int nx, int ny, double *buf, hipStream_t st );
+
 /// a diagram's cells with their edges frozen, along `w + t d` ( RANK order )
 struct Along {
     const double4 *seeds = nullptr;     ///< the positions ( x, y ) by rank
@@ -21,6 +34,7 @@ struct Along {
     double         lo[ 2 ] = { 0, 0 }, hi[ 2 ] = { 1, 1 };
     Density        dn;
     const double  *nue = nullptr;       ///< the targets with the exact duplicates folded in ( a duplicate's 0: no say ); may be null
+    MinPyramid     pyr;                 ///< an image density's pyramid of minima ( the forward limit's shortcut ); may be empty
 };
 
 /// FORWARD: `out[ 0 ]` = the smallest, over the cells, first `t > 0` where the frozen mass reaches `eps`

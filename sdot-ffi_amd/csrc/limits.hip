@@ -142,6 +142,54 @@ __device__ double mass_bisection( const Along &al, I64 k, int nb, double lo, dou
     return lo;
 }
 
+/// A CERTIFICATE that the frozen cell's mass stays above `eps` on `t in [ 0, 1 ]` ( an image density ): every edge keeps
+/// its orientation ( its length is affine in `t`, its direction fixed: positive at both ends, it stays positive, and the
+/// cell stays convex ), the cell then lies in the box its vertices sweep, and its smallest area on `[ 0, 1 ]` times the
+/// image's smallest value over that box passes `eps` with a margin far above the rounding of the exact integral. The
+/// exact computation would then find no crossing: the cell has no limit, without its image integrals.
+__device__ bool stays_above( const Along &al, I64 k, int nb, double eps ) {
+    const double4 p = al.seeds[ k ];
+    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+    double qx = 0, qy = 0, qux = 0, quy = 0, fx = 0, fy = 0, fux = 0, fuy = 0;
+    bool kept = true;
+    auto edge = [&]( double ax, double ay, double aux, double auy, double bx, double by, double bux, double buy ) {
+        const double e0x = bx - ax, e0y = by - ay, e1x = e0x + ( bux - aux ), e1y = e0y + ( buy - auy );
+        kept = kept && e0x * e1x + e0y * e1y > 0;
+    };
+    const bool ok = for_each_vertex( al, k, nb, 0.0, [&]( int q, double vx, double vy, double ux, double uy ) {
+        xmin = fmin( xmin, fmin( vx, vx + ux ) ); xmax = fmax( xmax, fmax( vx, vx + ux ) );
+        ymin = fmin( ymin, fmin( vy, vy + uy ) ); ymax = fmax( ymax, fmax( vy, vy + uy ) );
+        if ( q == 0 ) { fx = vx; fy = vy; fux = ux; fuy = uy; }
+        else edge( qx, qy, qux, quy, vx, vy, ux, uy );
+        qx = vx; qy = vy; qux = ux; quy = uy;
+    } );
+    if ( ! ok ) return false;
+    edge( qx, qy, qux, quy, fx, fy, fux, fuy );
+    if ( ! kept ) return false;
+    double a0, a1, a2;
+    if ( ! area_polynomial( al, k, nb, 0.0, a0, a1, a2 ) ) return false;
+    double amin = fmin( a0, a0 + a1 + a2 );
+    if ( a2 > 0 ) {
+        const double ts = -a1 / ( 2 * a2 );
+        if ( ts > 0 && ts < 1 ) amin = fmin( amin, a0 + ts * ( a1 + ts * a2 ) );
+    }
+    if ( ! ( amin > 0 ) ) return false;
+    // the pixels of the box, a pixel more on each side, and the pyramid's level where it spans two blocks at most
+    const Density &d = al.dn;
+    auto clampi = []( double v, int hi ) { return v < 0 ? 0 : ( v > double( hi ) ? hi : int( v ) ); };
+    const int i0 = clampi( floor( ( p.x + xmin - d.x0 ) / d.hx ) - 1, d.nx - 1 ), i1 = clampi( floor( ( p.x + xmax - d.x0 ) / d.hx ) + 1, d.nx - 1 );
+    const int j0 = clampi( floor( ( p.y + ymin - d.y0 ) / d.hy ) - 1, d.ny - 1 ), j1 = clampi( floor( ( p.y + ymax - d.y0 ) / d.hy ) + 1, d.ny - 1 );
+    const MinPyramid &P = al.pyr;
+    int lv = 0;
+    while ( lv + 1 < P.nl && ( ( i1 >> lv ) - ( i0 >> lv ) > 1 || ( j1 >> lv ) - ( j0 >> lv ) > 1 ) ) ++lv;
+    if ( ( i1 >> lv ) - ( i0 >> lv ) > 1 || ( j1 >> lv ) - ( j0 >> lv ) > 1 ) return false;
+    const double *v = P.v + P.off[ lv ];
+    const int w = P.w[ lv ];
+    const double rho = fmin( fmin( v[ ( j0 >> lv ) * w + ( i0 >> lv ) ], v[ ( j0 >> lv ) * w + ( i1 >> lv ) ] ),
+                             fmin( v[ ( j1 >> lv ) * w + ( i0 >> lv ) ], v[ ( j1 >> lv ) * w + ( i1 >> lv ) ] ) );
+    return amin * rho > eps * ( 1 + 1e-6 );
+}
+
 __device__ double forward_cell( const Along &al, I64 k, double eps ) {
     const int nb = al.nb_edges[ k ];
     if ( nb < 3 ) return 1e300;
@@ -155,7 +203,9 @@ __device__ double forward_cell( const Along &al, I64 k, double eps ) {
         if ( nr >= 2 && r2 > 0 ) return r2;
         return 1e300;
     }
-    // an image: the frozen mass at `t = 1`, and at the bottom of the area's parabola ( a cell that dips, then grows back )
+    // an image: the frozen mass at `t = 1`, and at the bottom of the area's parabola ( a cell that dips, then grows back ),
+    // unless the cell is certified to stay above `eps`
+    if ( al.dn.kind == 1 && al.pyr.nl > 0 && stays_above( al, k, nb, eps ) ) return 1e300;
     double m1, hi = -1;
     if ( ! frozen_mass( al, k, nb, 1.0, m1 ) ) return 1e300;
     if ( m1 < eps ) hi = 1;
@@ -210,6 +260,20 @@ struct BackwardF {
     }
 };
 
+/// one level of the pyramid of minima from the one below ( blocks of 2 x 2, cut at the borders )
+__global__ void k_min_down( const double *src, int sw, int sh, double *dst, int dw, int dh ) {
+    const I64 at = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( at >= I64( dw ) * dh ) return;
+    const int I = int( at % dw ), J = int( at / dw );
+    double m = src[ size_t( 2 * J ) * sw + size_t( 2 * I ) ];
+    if ( 2 * I + 1 < sw ) m = fmin( m, src[ size_t( 2 * J ) * sw + size_t( 2 * I + 1 ) ] );
+    if ( 2 * J + 1 < sh ) {
+        m = fmin( m, src[ size_t( 2 * J + 1 ) * sw + size_t( 2 * I ) ] );
+        if ( 2 * I + 1 < sw ) m = fmin( m, src[ size_t( 2 * J + 1 ) * sw + size_t( 2 * I + 1 ) ] );
+    }
+    dst[ at ] = m;
+}
+
 /// one axis of the blur: `dst[ at ] = sum_k ker[ k ] src[ at + k stride ] / sum_k ker[ k ]` over the taps inside the image
 __global__ void k_blur_axis( int nx, int ny, int axis, int r, const double *ker, const double *src, double *dst ) {
     const I64 at = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
@@ -235,6 +299,36 @@ void alpha_forward( const Along &al, double eps, double *out, hipStream_t st ) {
 
 void alpha_backward( const Along &al, const double *a, double eps, double t_trial, double *out, hipStream_t st ) {
     reduce_wide<2>( al.n, BackwardF{ al, a, eps, t_trial }, Ops<2>{ { OP_SUM, OP_MIN } }, out, st );
+}
+
+I64 min_pyramid_size( int nx, int ny ) {
+    I64 tot = 0;
+    for ( int w = nx, h = ny, l = 0; l < 16; ++l ) {
+        tot += I64( w ) * h;
+        if ( w == 1 && h == 1 ) break;
+        w = ( w + 1 ) / 2; h = ( h + 1 ) / 2;
+    }
+    return tot;
+}
+
+MinPyramid build_min_pyramid( const double *img, int nx, int ny, double *buf, hipStream_t st ) {
+    MinPyramid P;
+    P.v = buf;
+    int w = nx, h = ny;
+    I64 at = 0;
+    SDOT_HIP( hipMemcpyAsync( buf, img, sizeof( double ) * size_t( w ) * size_t( h ), hipMemcpyDeviceToDevice, st ) );
+    for ( int l = 0; l < 16; ++l ) {
+        P.w[ l ] = w; P.h[ l ] = h; P.off[ l ] = at; P.nl = l + 1;
+        if ( w == 1 && h == 1 ) break;
+        const int dw = ( w + 1 ) / 2, dh = ( h + 1 ) / 2;
+        if ( l + 1 < 16 ) {
+            k_min_down<<<blocks_for( I64( dw ) * dh, 256 ), 256, 0, st>>>( buf + at, w, h, buf + at + I64( w ) * h, dw, dh );
+            check_launch( "k_min_down" );
+        }
+        at += I64( w ) * h;
+        w = dw; h = dh;
+    }
+    return P;
 }
 
 void blur_image( const double *src, int nx, int ny, double hx, double hy, double s, double *dst, double *tmp, hipStream_t st ) {

@@ -228,68 +228,140 @@ struct BottomLevel {
     const int *cnt = nullptr, *col = nullptr;            ///< column-major: entry `q` of row `i` at `i + q n`
     const double *val = nullptr, *dia = nullptr;
     double *x = nullptr, *t = nullptr, *r = nullptr;
+    __device__ __forceinline__ double row_times( I64 i, const double *v ) const {
+        double s = dia[ i ] * v[ i ];
+        for ( int q = 0; q < cnt[ i ]; ++q ) s -= val[ i + q * n ] * v[ col[ i + q * n ] ];
+        return s;
+    }
 };
 struct Bottom {
     int nl = 0, pack = 4, nu = 2, nc = 0;
     double omega = 0.7;
     const double *inv = nullptr;
+    int rows = 0;                                        ///< the levels' rows together
     BottomLevel L[ MAX_LEVELS ];
 };
 constexpr int BOTTOM_THREADS = 1024;
+constexpr int BOTTOM_ROWS = 640, BOTTOM_ENTRIES = 6144;   ///< what the LDS copy of the bottom holds
 
-__device__ __forceinline__ double b_row_times( const BottomLevel &A, I64 i, const double *x ) {
-    double s = A.dia[ i ] * x[ i ];
-    for ( int q = 0; q < A.cnt[ i ]; ++q ) s -= A.val[ i + q * A.n ] * x[ A.col[ i + q * A.n ] ];
-    return s;
-}
+/// a bottom level copied in LDS: CSR rows, its vectors
+struct LdsLevel {
+    int n = 0;
+    const int *off = nullptr, *col = nullptr;
+    const double *val = nullptr, *dia = nullptr;
+    double *x = nullptr, *t = nullptr, *r = nullptr;
+    __device__ __forceinline__ double row_times( I64 i, const double *v ) const {
+        double s = dia[ i ] * v[ i ];
+        for ( int e = off[ i ]; e < off[ i + 1 ]; ++e ) s -= val[ e ] * v[ col[ e ] ];
+        return s;
+    }
+};
 
-/// `x ~ A^-1 r` on the bottom levels ( `B.L[ 0 ].r` in, `B.L[ 0 ].x` out ), one V-cycle
-__global__ __launch_bounds__( BOTTOM_THREADS ) void k_bottom( Bottom B, const double *state ) {
-    if ( state[ S_DONE ] != 0 ) return;
+/// one V-cycle over the levels `L[ 0 .. nl - 1 ]` ( `L[ 0 ].r` in, `L[ 0 ].x` out ), the steps separated by barriers
+template<class Lv>
+__device__ void bottom_cycle( const Lv *L, int nl, int pack, int nu, double omega, const double *inv, int nc ) {
     const int tid = threadIdx.x;
-    for ( int l = 0; l + 1 < B.nl; ++l ) {
-        const BottomLevel &A = B.L[ l ];
+    for ( int l = 0; l + 1 < nl; ++l ) {
+        const Lv &A = L[ l ];
         // pre-smoothing from zero, `nu` sweeps ping-ponging so that the last lands in `x`
-        double *cur = ( B.nu - 1 ) % 2 ? A.t : A.x, *oth = ( B.nu - 1 ) % 2 ? A.x : A.t;
-        for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) cur[ i ] = B.omega * A.r[ i ] / A.dia[ i ];
+        double *cur = ( nu - 1 ) % 2 ? A.t : A.x, *oth = ( nu - 1 ) % 2 ? A.x : A.t;
+        for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) cur[ i ] = omega * A.r[ i ] / A.dia[ i ];
         __syncthreads();
-        for ( int s = 1; s < B.nu; ++s ) {
-            for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) oth[ i ] = cur[ i ] + B.omega * ( A.r[ i ] - b_row_times( A, i, cur ) ) / A.dia[ i ];
+        for ( int s = 1; s < nu; ++s ) {
+            for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) oth[ i ] = cur[ i ] + omega * ( A.r[ i ] - A.row_times( i, cur ) ) / A.dia[ i ];
             __syncthreads();
             double *sw = cur; cur = oth; oth = sw;
         }
         // the residual restricted
-        const BottomLevel &C = B.L[ l + 1 ];
+        const Lv &C = L[ l + 1 ];
         for ( I64 I = tid; I < C.n; I += BOTTOM_THREADS ) {
             double v = 0;
-            const I64 e = ( I + 1 ) * B.pack < A.n ? ( I + 1 ) * B.pack : A.n;
-            for ( I64 i = I * B.pack; i < e; ++i ) v += A.r[ i ] - b_row_times( A, i, A.x );
+            const I64 e = ( I + 1 ) * pack < A.n ? ( I + 1 ) * pack : A.n;
+            for ( I64 i = I * pack; i < e; ++i ) v += A.r[ i ] - A.row_times( i, A.x );
             C.r[ I ] = v;
         }
         __syncthreads();
     }
     {
-        const BottomLevel &D = B.L[ B.nl - 1 ];                  // the coarsest: the dense inverse
-        for ( int row = tid; row < B.nc; row += BOTTOM_THREADS ) {
+        const Lv &D = L[ nl - 1 ];                       // the coarsest: the dense inverse
+        for ( int row = tid; row < nc; row += BOTTOM_THREADS ) {
             double v = 0;
-            for ( int j = 0; j < B.nc; ++j ) v += B.inv[ size_t( row ) * B.nc + j ] * D.r[ j ];
+            for ( int j = 0; j < nc; ++j ) v += inv[ size_t( row ) * nc + j ] * D.r[ j ];
             D.x[ row ] = v;
         }
         __syncthreads();
     }
-    for ( int l = B.nl - 2; l >= 0; --l ) {
-        const BottomLevel &A = B.L[ l ];
-        const double *xc = B.L[ l + 1 ].x;
+    for ( int l = nl - 2; l >= 0; --l ) {
+        const Lv &A = L[ l ];
+        const double *xc = L[ l + 1 ].x;
         // prolongation, then `nu` sweeps ending in `x`: from `t` when `nu` is odd, else from `x` through `t`
-        double *cur = B.nu % 2 ? A.t : A.x, *oth = B.nu % 2 ? A.x : A.t;
-        for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) cur[ i ] = A.x[ i ] + xc[ i / B.pack ];
+        double *cur = nu % 2 ? A.t : A.x, *oth = nu % 2 ? A.x : A.t;
+        for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) cur[ i ] = A.x[ i ] + xc[ i / pack ];
         __syncthreads();
-        for ( int s = 0; s < B.nu; ++s ) {
-            for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) oth[ i ] = cur[ i ] + B.omega * ( A.r[ i ] - b_row_times( A, i, cur ) ) / A.dia[ i ];
+        for ( int s = 0; s < nu; ++s ) {
+            for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) oth[ i ] = cur[ i ] + omega * ( A.r[ i ] - A.row_times( i, cur ) ) / A.dia[ i ];
             __syncthreads();
             double *sw = cur; cur = oth; oth = sw;
         }
     }
+}
+
+/// `x ~ A^-1 r` on the bottom levels ( `B.L[ 0 ].r` in, `B.L[ 0 ].x` out ), one V-cycle. When the levels fit, they are
+/// first copied in LDS ( rows to CSR, a block scan of the counts ): the cycle's dependent reads then cost LDS latencies
+/// instead of L2 ones
+__global__ __launch_bounds__( BOTTOM_THREADS ) void k_bottom( Bottom B, const double *state ) {
+    __shared__ int s_off[ BOTTOM_ROWS + MAX_LEVELS ], s_col[ BOTTOM_ENTRIES ];
+    __shared__ double s_val[ BOTTOM_ENTRIES ], s_dia[ BOTTOM_ROWS ], s_x[ BOTTOM_ROWS ], s_t[ BOTTOM_ROWS ], s_r[ BOTTOM_ROWS ];
+    __shared__ double sh[ BOTTOM_THREADS ];
+    __shared__ int s_tot;
+    if ( state[ S_DONE ] != 0 ) return;
+    const int tid = threadIdx.x;
+    // the rows of every level, one thread each: `g` the row across the levels, `l` its level, `i` its index there
+    int cnt = 0, lv = -1, li = 0;
+    if ( tid < B.rows ) {
+        int g = tid;
+        for ( int l = 0; l < B.nl; ++l ) {
+            if ( g < int( B.L[ l ].n ) ) { lv = l; li = g; break; }
+            g -= int( B.L[ l ].n );
+        }
+        cnt = B.L[ lv ].cnt[ li ];
+    }
+    const double tot = block_sum<BOTTOM_THREADS>( double( cnt ), sh );
+    const bool lds = B.rows <= BOTTOM_ROWS && B.rows <= BOTTOM_THREADS && tot <= double( BOTTOM_ENTRIES );
+    if ( ! lds ) { bottom_cycle( B.L, B.nl, B.pack, B.nu, B.omega, B.inv, B.nc ); return; }
+    // the offsets: an exclusive scan of the counts ( Hillis-Steele in LDS )
+    sh[ tid ] = double( cnt );
+    __syncthreads();
+    for ( int d = 1; d < BOTTOM_THREADS; d <<= 1 ) {
+        const double v = tid >= d ? sh[ tid - d ] : 0.0;
+        __syncthreads();
+        sh[ tid ] += v;
+        __syncthreads();
+    }
+    const int incl = int( sh[ tid ] ), excl = incl - cnt;
+    if ( tid < B.rows ) {
+        s_off[ tid ] = excl;
+        const BottomLevel &A = B.L[ lv ];
+        for ( int q = 0; q < cnt; ++q ) { s_col[ excl + q ] = A.col[ li + q * A.n ]; s_val[ excl + q ] = A.val[ li + q * A.n ]; }
+        s_dia[ tid ] = A.dia[ li ];
+        if ( lv == 0 ) s_r[ tid ] = A.r[ li ];
+    }
+    if ( tid == B.rows - 1 ) s_tot = incl;
+    __syncthreads();
+    // the levels' views: row `i` of level `l` is row `base + i`, its columns local to the level
+    LdsLevel V[ MAX_LEVELS ];
+    int base = 0;
+    for ( int l = 0; l < B.nl; ++l ) {
+        V[ l ].n = int( B.L[ l ].n );
+        V[ l ].off = s_off + base;                       // ( `off[ n ]`: the next level's first offset, or the total )
+        V[ l ].col = s_col; V[ l ].val = s_val;
+        V[ l ].dia = s_dia + base; V[ l ].x = s_x + base; V[ l ].t = s_t + base; V[ l ].r = s_r + base;
+        base += int( B.L[ l ].n );
+    }
+    if ( tid == 0 ) s_off[ B.rows ] = s_tot;
+    __syncthreads();
+    bottom_cycle( V, B.nl, B.pack, B.nu, B.omega, B.inv, B.nc );
+    for ( int i = tid; i < V[ 0 ].n; i += BOTTOM_THREADS ) B.L[ 0 ].x[ i ] = s_x[ i ];
 }
 
 // ---- the K-cycle's two flexible CG steps ( Notay ) ------------------------------------------------------------
@@ -669,11 +741,12 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
             if ( H.A[ l ].n <= o.bottom ) { H.lb = l; break; }
     if ( H.lb < H.nl - 1 ) {
         Bottom &B = H.bottom;
-        B.nl = H.nl - H.lb; B.pack = o.pack; B.nu = o.nu; B.omega = o.omega; B.nc = H.nc; B.inv = H.inv;
+        B.nl = H.nl - H.lb; B.pack = o.pack; B.nu = o.nu; B.omega = o.omega; B.nc = H.nc; B.inv = H.inv; B.rows = 0;
         for ( int l = H.lb; l < H.nl; ++l ) {
             BottomLevel &L = B.L[ l - H.lb ];
             L.n = H.A[ l ].n; L.cnt = H.A[ l ].row_cnt; L.col = H.A[ l ].col; L.val = H.A[ l ].val; L.dia = H.A[ l ].dia;
             L.x = H.x[ l ]; L.t = H.t[ l ]; L.r = H.r[ l ];
+            B.rows += int( L.n );
         }
     } else
         H.lb = H.nl;                                     // ( a bottom of the coarsest alone: the dense apply as before )
