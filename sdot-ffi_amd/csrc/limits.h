@@ -13,12 +13,12 @@
 namespace sdot_amd {
 
 /// the smallest pixel value over blocks of `2^k x 2^k` pixels, level by level ( level 0: the image itself; none when
-/// `nl == 0` ): a lower bound of an image density over any box in two or four reads
+/// `nl == 0` ): a lower bound of an image density over any box in two or four reads. Level `l + 1` is `ceil( w / 2 ) x
+/// ceil( h / 2 )` for level `l` of `w x h`, the levels one after the other in `v`
 struct MinPyramid {
     const double *v = nullptr;
     int nl = 0;
-    int w[ 16 ] = {}, h[ 16 ] = {};
-    I64 off[ 16 ] = {};
+    int nx = 0, ny = 0;                 ///< level 0's size
 };
 
 /// the pyramid of the image `img[ ny ][ nx ]` into `buf` ( `min_pyramid_size( nx, ny )` doubles )

@@ -183,8 +183,13 @@ __device__ bool stays_above( const Along &al, I64 k, int nb, double eps ) {
     int lv = 0;
     while ( lv + 1 < P.nl && ( ( i1 >> lv ) - ( i0 >> lv ) > 1 || ( j1 >> lv ) - ( j0 >> lv ) > 1 ) ) ++lv;
     if ( ( i1 >> lv ) - ( i0 >> lv ) > 1 || ( j1 >> lv ) - ( j0 >> lv ) > 1 ) return false;
-    const double *v = P.v + P.off[ lv ];
-    const int w = P.w[ lv ];
+    int w = P.nx, h = P.ny;                             // ( level `lv`'s width and place: sizes, not a table, so that
+    I64 off = 0;                                        //   no array of the arguments is indexed per lane )
+    for ( int l = 0; l < lv; ++l ) {
+        off += I64( w ) * h;
+        w = ( w + 1 ) / 2; h = ( h + 1 ) / 2;
+    }
+    const double *v = P.v + off;
     const double rho = fmin( fmin( v[ ( j0 >> lv ) * w + ( i0 >> lv ) ], v[ ( j0 >> lv ) * w + ( i1 >> lv ) ] ),
                              fmin( v[ ( j1 >> lv ) * w + ( i0 >> lv ) ], v[ ( j1 >> lv ) * w + ( i1 >> lv ) ] ) );
     return amin * rho > eps * ( 1 + 1e-6 );
@@ -313,12 +318,12 @@ I64 min_pyramid_size( int nx, int ny ) {
 
 MinPyramid build_min_pyramid( const double *img, int nx, int ny, double *buf, hipStream_t st ) {
     MinPyramid P;
-    P.v = buf;
+    P.v = buf; P.nx = nx; P.ny = ny;
     int w = nx, h = ny;
     I64 at = 0;
     SDOT_HIP( hipMemcpyAsync( buf, img, sizeof( double ) * size_t( w ) * size_t( h ), hipMemcpyDeviceToDevice, st ) );
     for ( int l = 0; l < 16; ++l ) {
-        P.w[ l ] = w; P.h[ l ] = h; P.off[ l ] = at; P.nl = l + 1;
+        P.nl = l + 1;
         if ( w == 1 && h == 1 ) break;
         const int dw = ( w + 1 ) / 2, dh = ( h + 1 ) / 2;
         if ( l + 1 < 16 ) {

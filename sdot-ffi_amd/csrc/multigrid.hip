@@ -244,19 +244,6 @@ struct Bottom {
 constexpr int BOTTOM_THREADS = 1024;
 constexpr int BOTTOM_ROWS = 640, BOTTOM_ENTRIES = 6144;   ///< what the LDS copy of the bottom holds
 
-/// a bottom level copied in LDS: CSR rows, its vectors
-struct LdsLevel {
-    int n = 0;
-    const int *off = nullptr, *col = nullptr;
-    const double *val = nullptr, *dia = nullptr;
-    double *x = nullptr, *t = nullptr, *r = nullptr;
-    __device__ __forceinline__ double row_times( I64 i, const double *v ) const {
-        double s = dia[ i ] * v[ i ];
-        for ( int e = off[ i ]; e < off[ i + 1 ]; ++e ) s -= val[ e ] * v[ col[ e ] ];
-        return s;
-    }
-};
-
 /// one V-cycle over the levels `L[ 0 .. nl - 1 ]` ( `L[ 0 ].r` in, `L[ 0 ].x` out ), the steps separated by barriers
 template<class Lv>
 __device__ void bottom_cycle( const Lv *L, int nl, int pack, int nu, double omega, const double *inv, int nc ) {
@@ -307,61 +294,110 @@ __device__ void bottom_cycle( const Lv *L, int nl, int pack, int nu, double omeg
 }
 
 /// `x ~ A^-1 r` on the bottom levels ( `B.L[ 0 ].r` in, `B.L[ 0 ].x` out ), one V-cycle. When the levels fit, they are
-/// first copied in LDS ( rows to CSR, a block scan of the counts ): the cycle's dependent reads then cost LDS latencies
-/// instead of L2 ones
+/// first copied in LDS as one CSR matrix over all their rows ( row `g` of the copy is row `g - base[ l ]` of level `l`,
+/// columns shifted likewise; offsets from a block scan of the row counts ): the cycle's dependent reads then cost LDS
+/// latencies instead of L2 ones. Every level index is uniform, so no per-thread array of level views is needed ( such an
+/// array lands in scratch memory )
 __global__ __launch_bounds__( BOTTOM_THREADS ) void k_bottom( Bottom B, const double *state ) {
-    __shared__ int s_off[ BOTTOM_ROWS + MAX_LEVELS ], s_col[ BOTTOM_ENTRIES ];
-    __shared__ double s_val[ BOTTOM_ENTRIES ], s_dia[ BOTTOM_ROWS ], s_x[ BOTTOM_ROWS ], s_t[ BOTTOM_ROWS ], s_r[ BOTTOM_ROWS ];
-    __shared__ double sh[ BOTTOM_THREADS ];
-    __shared__ int s_tot;
+    __shared__ int s_base[ MAX_LEVELS + 1 ], s_off[ BOTTOM_ROWS + 1 ], s_col[ BOTTOM_ENTRIES ], s_wave[ BOTTOM_THREADS / 64 ];
+    __shared__ double s_val[ BOTTOM_ENTRIES ], s_dia[ BOTTOM_ROWS ], s_r[ BOTTOM_ROWS ], s_xt[ 2 ][ BOTTOM_ROWS ];
     if ( state[ S_DONE ] != 0 ) return;
-    const int tid = threadIdx.x;
-    // the rows of every level, one thread each: `g` the row across the levels, `l` its level, `i` its index there
-    int cnt = 0, lv = -1, li = 0;
-    if ( tid < B.rows ) {
-        int g = tid;
-        for ( int l = 0; l < B.nl; ++l ) {
-            if ( g < int( B.L[ l ].n ) ) { lv = l; li = g; break; }
-            g -= int( B.L[ l ].n );
-        }
-        cnt = B.L[ lv ].cnt[ li ];
+    if ( B.rows > BOTTOM_ROWS ) { bottom_cycle( B.L, B.nl, B.pack, B.nu, B.omega, B.inv, B.nc ); return; }
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    // this thread's row: level `lv`, index `li` there, `base` the level's first row in the copy
+    int cnt = 0, lv = -1, li = 0, base = 0;
+    for ( int l = 0, b = 0; l < B.nl; ++l ) {
+        const int n = int( B.L[ l ].n );
+        if ( tid >= b && tid < b + n ) { lv = l; li = tid - b; base = b; cnt = B.L[ l ].cnt[ li ]; }
+        if ( tid == 0 ) s_base[ l ] = b;
+        b += n;
     }
-    const double tot = block_sum<BOTTOM_THREADS>( double( cnt ), sh );
-    const bool lds = B.rows <= BOTTOM_ROWS && B.rows <= BOTTOM_THREADS && tot <= double( BOTTOM_ENTRIES );
-    if ( ! lds ) { bottom_cycle( B.L, B.nl, B.pack, B.nu, B.omega, B.inv, B.nc ); return; }
-    // the offsets: an exclusive scan of the counts ( Hillis-Steele in LDS )
-    sh[ tid ] = double( cnt );
+    // the offsets: an exclusive scan of the counts ( within the waves by shuffles, then over the wave totals )
+    int inc = cnt;
+    for ( int d = 1; d < 64; d <<= 1 ) {
+        const int v = __shfl_up( inc, d, 64 );
+        if ( lane >= d ) inc += v;
+    }
+    if ( lane == 63 ) s_wave[ wid ] = inc;
     __syncthreads();
-    for ( int d = 1; d < BOTTOM_THREADS; d <<= 1 ) {
-        const double v = tid >= d ? sh[ tid - d ] : 0.0;
-        __syncthreads();
-        sh[ tid ] += v;
-        __syncthreads();
+    int before = 0, tot = 0;
+    for ( int w = 0; w < BOTTOM_THREADS / 64; ++w ) {
+        const int v = s_wave[ w ];
+        before += w < wid ? v : 0;
+        tot += v;
     }
-    const int incl = int( sh[ tid ] ), excl = incl - cnt;
-    if ( tid < B.rows ) {
-        s_off[ tid ] = excl;
-        const BottomLevel &A = B.L[ lv ];
-        for ( int q = 0; q < cnt; ++q ) { s_col[ excl + q ] = A.col[ li + q * A.n ]; s_val[ excl + q ] = A.val[ li + q * A.n ]; }
-        s_dia[ tid ] = A.dia[ li ];
-        if ( lv == 0 ) s_r[ tid ] = A.r[ li ];
-    }
-    if ( tid == B.rows - 1 ) s_tot = incl;
-    __syncthreads();
-    // the levels' views: row `i` of level `l` is row `base + i`, its columns local to the level
-    LdsLevel V[ MAX_LEVELS ];
-    int base = 0;
+    if ( tot > BOTTOM_ENTRIES ) { bottom_cycle( B.L, B.nl, B.pack, B.nu, B.omega, B.inv, B.nc ); return; }   // ( uniform )
+    const int excl = before + inc - cnt;
     for ( int l = 0; l < B.nl; ++l ) {
-        V[ l ].n = int( B.L[ l ].n );
-        V[ l ].off = s_off + base;                       // ( `off[ n ]`: the next level's first offset, or the total )
-        V[ l ].col = s_col; V[ l ].val = s_val;
-        V[ l ].dia = s_dia + base; V[ l ].x = s_x + base; V[ l ].t = s_t + base; V[ l ].r = s_r + base;
-        base += int( B.L[ l ].n );
+        if ( lv != l ) continue;
+        const BottomLevel &A = B.L[ l ];
+        s_off[ tid ] = excl;
+        for ( int q = 0; q < cnt; ++q ) {
+            s_col[ excl + q ] = base + A.col[ li + I64( q ) * A.n ];
+            s_val[ excl + q ] = A.val[ li + I64( q ) * A.n ];
+        }
+        s_dia[ tid ] = A.dia[ li ];
+        s_r[ tid ] = l == 0 ? A.r[ li ] : 0.0;
     }
-    if ( tid == 0 ) s_off[ B.rows ] = s_tot;
+    if ( tid == 0 ) { s_off[ B.rows ] = tot; s_base[ B.nl ] = B.rows; }
     __syncthreads();
-    bottom_cycle( V, B.nl, B.pack, B.nu, B.omega, B.inv, B.nc );
-    for ( int i = tid; i < V[ 0 ].n; i += BOTTOM_THREADS ) B.L[ 0 ].x[ i ] = s_x[ i ];
+
+    auto row_times = [ & ]( int g, int v ) {
+        double s = s_dia[ g ] * s_xt[ v ][ g ];
+        for ( int e = s_off[ g ]; e < s_off[ g + 1 ]; ++e ) s -= s_val[ e ] * s_xt[ v ][ s_col[ e ] ];
+        return s;
+    };
+    const int pack = B.pack, nu = B.nu;
+    const double omega = B.omega;
+    for ( int l = 0; l + 1 < B.nl; ++l ) {
+        const int b = s_base[ l ], n = s_base[ l + 1 ] - b;
+        // pre-smoothing from zero, `nu` sweeps ping-ponging between `t` ( 1 ) and `x` ( 0 ) so that the last lands in `x`
+        int cur = ( nu - 1 ) % 2, oth = 1 - cur;
+        for ( int i = tid; i < n; i += BOTTOM_THREADS ) s_xt[ cur ][ b + i ] = omega * s_r[ b + i ] / s_dia[ b + i ];
+        __syncthreads();
+        for ( int s = 1; s < nu; ++s ) {
+            for ( int i = tid; i < n; i += BOTTOM_THREADS ) {
+                const int g = b + i;
+                s_xt[ oth ][ g ] = s_xt[ cur ][ g ] + omega * ( s_r[ g ] - row_times( g, cur ) ) / s_dia[ g ];
+            }
+            __syncthreads();
+            cur = oth; oth = 1 - cur;
+        }
+        // the residual restricted
+        const int bc = s_base[ l + 1 ], nc = s_base[ l + 2 ] - bc;
+        for ( int I = tid; I < nc; I += BOTTOM_THREADS ) {
+            double v = 0;
+            const int e = ( I + 1 ) * pack < n ? ( I + 1 ) * pack : n;
+            for ( int i = I * pack; i < e; ++i ) v += s_r[ b + i ] - row_times( b + i, 0 );
+            s_r[ bc + I ] = v;
+        }
+        __syncthreads();
+    }
+    {
+        const int bd = s_base[ B.nl - 1 ];              // the coarsest: the dense inverse
+        for ( int row = tid; row < B.nc; row += BOTTOM_THREADS ) {
+            double v = 0;
+            for ( int j = 0; j < B.nc; ++j ) v += B.inv[ size_t( row ) * B.nc + j ] * s_r[ bd + j ];
+            s_xt[ 0 ][ bd + row ] = v;
+        }
+        __syncthreads();
+    }
+    for ( int l = B.nl - 2; l >= 0; --l ) {
+        const int b = s_base[ l ], n = s_base[ l + 1 ] - b, bc = s_base[ l + 1 ];
+        // prolongation, then `nu` sweeps ending in `x`: from `t` when `nu` is odd, else from `x` through `t`
+        int cur = nu % 2, oth = 1 - cur;
+        for ( int i = tid; i < n; i += BOTTOM_THREADS ) s_xt[ cur ][ b + i ] = s_xt[ 0 ][ b + i ] + s_xt[ 0 ][ bc + i / pack ];
+        __syncthreads();
+        for ( int s = 0; s < nu; ++s ) {
+            for ( int i = tid; i < n; i += BOTTOM_THREADS ) {
+                const int g = b + i;
+                s_xt[ oth ][ g ] = s_xt[ cur ][ g ] + omega * ( s_r[ g ] - row_times( g, cur ) ) / s_dia[ g ];
+            }
+            __syncthreads();
+            cur = oth; oth = 1 - cur;
+        }
+    }
+    for ( int i = tid; i < int( B.L[ 0 ].n ); i += BOTTOM_THREADS ) B.L[ 0 ].x[ i ] = s_xt[ 0 ][ i ];
 }
 
 // ---- the K-cycle's two flexible CG steps ( Notay ) ------------------------------------------------------------
