@@ -38,6 +38,10 @@ def lib():
         _lib.oracle_solve.restype = ctypes.c_int
         _lib.oracle_max_depth_for.restype = ctypes.c_int
         _lib.oracle_get_threads.restype = ctypes.c_int
+        _lib.oracle_build_tree_3d.restype = ctypes.c_int
+        _lib.oracle_measures_3d.restype = ctypes.c_longlong
+        _lib.oracle_cell_3d.restype = ctypes.c_int
+        _lib.oracle_solve_3d.restype = ctypes.c_int
     return _lib
 
 
@@ -219,3 +223,108 @@ def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = No
                              clusters.ctypes.data_as( ctypes.POINTER( ctypes.c_longlong ) ) )
     return dict( weights = weights, masses = masses, barycenters = bary, cost = float( cost[ 0 ] ),
                  stats = dict( zip( STATS, stats.tolist() ) ), history = hist[ :nb ].copy(), clusters = clusters )
+
+
+# ---- 3D ------------------------------------------------------------------------------------------------------------
+
+def _density3_args( density ):
+    """`density`: `None` / a float ( constant ), or `dict( gaussians = [ ng, 5 ] )` ( centre x, y, z, sigma, mass per row )"""
+    if density is None or np.isscalar( density ):
+        return 0, 1.0 if density is None else float( density ), 0, None
+    g = _f64( density[ "gaussians" ] ).reshape( -1, 5 )
+    return 2, 0.0, g.shape[ 0 ], g
+
+
+def rules():
+    """the quadrature rules of the 3D integrals: `tet[ 14, 5 ]` ( barycentric coordinates, weight ), `tri[ 7, 4 ]`"""
+    tet = np.zeros( ( 14, 5 ) )
+    tri = np.zeros( ( 7, 4 ) )
+    lib().oracle_rules( _d( tet ), _d( tri ) )
+    return tet, tri
+
+
+def build_tree_3d( pos, w = None, leaf = 10 ):
+    """dict( order, beg, end, box[ N, 6 ], wa[ N, 3 ], wb[ N ], depth ) of the 3D tree"""
+    pos = _f64( pos )
+    n = pos.shape[ 0 ]
+    depth = max_depth_for( n, leaf )
+    N = ( 1 << depth ) - 1
+    order = np.zeros( n, np.int64 )
+    beg = np.zeros( N, np.int64 )
+    end = np.zeros( N, np.int64 )
+    box = np.zeros( ( N, 6 ) )
+    wa = np.zeros( ( N, 3 ) )
+    wb = np.zeros( N )
+    ww = None if w is None else _f64( w )
+    d = lib().oracle_build_tree_3d( ctypes.c_longlong( n ), _d( pos ), _d( ww ), ctypes.c_longlong( leaf ), _l( order ), _l( beg ), _l( end ),
+                                    _d( box ), _d( wa ), _d( wb ) )
+    return dict( order = order, beg = beg, end = end, box = box, wa = wa, wb = wb, depth = d )
+
+
+def measures_3d( pos, w = None, lo = ( 0, 0, 0 ), hi = ( 1, 1, 1 ), density = None, facets = False, events = False ):
+    """masses `[ n ]` of the 3D diagram ( and if asked the facets `( fi, fj, fc )` in row order, and the event counts with
+    `broken`, the cells whose cut could not be closed )"""
+    pos = _f64( pos )
+    n = pos.shape[ 0 ]
+    ww = np.zeros( n ) if w is None else _f64( w )
+    kind, value, ng, g = _density3_args( density )
+    masses = np.zeros( n )
+    cap = 24 * n + 64 if facets else 0
+    fi = np.zeros( cap, np.int64 )
+    fj = np.zeros( cap, np.int64 )
+    fc = np.zeros( cap )
+    info = np.zeros( 6, np.int64 )
+    nf = lib().oracle_measures_3d( ctypes.c_longlong( n ), _d( pos ), _d( ww ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
+                                   ctypes.c_double( value ), ctypes.c_int( ng ), _d( g ), _d( masses ), _l( fi ), _l( fj ), _d( fc ),
+                                   ctypes.c_longlong( cap ), _l( info ) )
+    res = [ masses ]
+    if facets:
+        if nf > cap:
+            raise RuntimeError( "oracle: facet capacity exceeded" )
+        res.append( ( fi[ :nf ], fj[ :nf ], fc[ :nf ] ) )
+    if events:
+        res.append( dict( zip( ( "broken", "side_tests", "prune_vertex_tests", "new_vertices", "node_visits", "cuts" ), info.tolist() ) ) )
+    return res[ 0 ] if len( res ) == 1 else tuple( res )
+
+
+def cell_3d( pos, i, w = None, lo = ( 0, 0, 0 ), hi = ( 1, 1, 1 ), brute_force = False, cap = 512 ):
+    """the vertices `[ nv, 3 ]` of the 3D cell of seed `i`, the ids of its faces ( user index, or `-1 - f` for face `f` of
+    the box: x lo, x hi, y lo, y hi, z lo, z hi ) and their areas"""
+    pos = _f64( pos )
+    n = pos.shape[ 0 ]
+    ww = np.zeros( n ) if w is None else _f64( w )
+    xyz = np.zeros( ( cap, 3 ) )
+    ids = np.zeros( cap, np.int64 )
+    areas = np.zeros( cap )
+    nf = ctypes.c_int( 0 )
+    nv = lib().oracle_cell_3d( ctypes.c_longlong( n ), _d( pos ), _d( ww ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_longlong( i ),
+                               ctypes.c_int( int( brute_force ) ), _d( xyz ), ctypes.c_int( cap ), _l( ids ), _d( areas ), ctypes.byref( nf ) )
+    if nv < 0:
+        raise RuntimeError( "oracle: the cell's topology broke" )
+    if nv > cap or nf.value > cap:
+        raise RuntimeError( "oracle: cell capacity exceeded" )
+    return xyz[ :nv ].copy(), ids[ :nf.value ].copy(), areas[ :nf.value ].copy()
+
+
+def solve_3d( pos, nu = None, w0 = None, lo = ( 0, 0, 0 ), hi = ( 1, 1, 1 ), density = None, tol = 1e-9, tol_rel = 0.0, maxit = 100,
+              max_backtracks = 60, t_min = 1e-10, mult_ok = 4.0, residual = 1, power = 0.5, switch = 2.0, lin_tol = 1e-12, lin_maxit = 20000,
+              keep_start = False, continuation = "never", conv_threshold = 1e-2, conv_s0 = 0.0, conv_ratio = 2 ** 0.5, conv_min = 0.0 ):
+    """the 3D damped-Newton solve ( step by trials ); returns a dict ( weights, masses, barycenters, cost, stats, history )"""
+    pos = _f64( pos )
+    n = pos.shape[ 0 ]
+    nu = np.full( n, 1.0 / n ) if nu is None else _f64( nu )
+    w0 = None if w0 is None else _f64( w0 )
+    kind, value, ng, g = _density3_args( density )
+    opts = _f64( [ tol, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol, lin_maxit, int( keep_start ),
+                   0, 0.9, { "never": 0, "auto": 1, "always": 2 }[ continuation ], conv_threshold, conv_s0, conv_ratio, conv_min, 0, 0, 0 ] )
+    weights = np.zeros( n )
+    stats = np.zeros( len( STATS ) )
+    hist = np.zeros( ( maxit + 1, len( HISTORY ) ) )
+    masses = np.zeros( n )
+    bary = np.zeros( ( n, 3 ) )
+    cost = np.zeros( 1 )
+    nb = lib().oracle_solve_3d( ctypes.c_longlong( n ), _d( pos ), _d( nu ), _d( w0 ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
+                                ctypes.c_double( value ), ctypes.c_int( ng ), _d( g ), _d( opts ), _d( weights ), _d( stats ), _d( hist ),
+                                _d( masses ), _d( bary ), _d( cost ) )
+    return dict( weights = weights, masses = masses, barycenters = bary, cost = float( cost[ 0 ] ),
+                 stats = dict( zip( STATS, stats.tolist() ) ), history = hist[ :nb ].copy() )

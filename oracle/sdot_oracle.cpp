@@ -878,6 +878,7 @@ struct Edges {
 };
 
 struct Sweep {
+    static constexpr int DIM = 2;
     Tree   &t;
     const double *lo, *hi;
     Density dn;
@@ -1236,6 +1237,8 @@ enum Residual { LIN = 0, LOG = 1, POWER = 2 };
 
 enum Step { TRIALS = 0, LIMITS = 1 };
 
+#include "oracle3d.h"
+
 struct Options {
     double tol_abs = 1e-8, tol_rel = 0, t_min = 1e-10, mult_ok = 4, power = 0.5, switch_residual = 2, lin_tol = 1e-12;
     int    maxit = 100, max_backtracks = 60, residual = LOG, lin_maxit = 20000;
@@ -1270,8 +1273,10 @@ static double gp_of( double x, int r, double p ) {
     return r == LOG ? 1 / x : 1;
 }
 
-struct Solver {
-    Sweep &sw;
+/// the Newton solve over a sweep `SW` ( `Sweep`: 2D; `Sweep3`: 3D, where the step is by trials only )
+template<class SW>
+struct SolverT {
+    SW &sw;
     Options o;
     const double *pos = nullptr;                         ///< user order ( the step by the limits reads it )
     std::vector<double> nu, w, a, d;
@@ -1463,7 +1468,7 @@ struct Solver {
     LinStats lst;
     std::function<void( int, double, int )> after_step;
 
-    Solver( Sweep &sw, const Options &o ) : sw( sw ), o( o ) {}
+    SolverT( SW &sw, const Options &o ) : sw( sw ), o( o ) {}
 
     /// `Newton.h:176-203`: on the targets `nue` when there are duplicates ( a duplicate's `b` is 0 )
     void rhs( const std::vector<double> &A, std::vector<double> &b ) const {
@@ -1586,7 +1591,7 @@ struct Solver {
         std::vector<double> a2, b, w2;
         std::vector<Facet> fa2;
         Edges e2;
-        const bool limits = o.step == LIMITS;
+        const bool limits = SW::DIM == 2 && o.step == LIMITS;
         Laplacian L;
         std::vector<double> ae, ae2;
         std::vector<Facet> fa_lin;
@@ -1636,7 +1641,7 @@ struct Solver {
                 measures_and_facets( w2, a2, fa2, limits ? &e2 : nullptr );
                 ++nb_evals;
             };
-            if ( limits ) {
+            if constexpr ( SW::DIM == 2 ) if ( limits ) {
                 // FORWARD on the accepted diagram, then up to eight rounds of BACKWARD on the trial diagrams
                 double th = now();
                 const Along fw{ pos, w.data(), d.data(), sw.lo, sw.hi, &ea, n, &sw.dn };
@@ -1699,6 +1704,8 @@ struct Solver {
         return false;
     }
 };
+
+using Solver = SolverT<Sweep>;
 
 /// the similarity start ( `sdotplan/Solve.h:69-100` )
 static void similarity( const Tree &t, const double *pos, const double *lo, const double *hi, std::vector<double> &w, double margin = 0.1 ) {
@@ -2106,6 +2113,238 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
         for ( I64 c = 0; c < nt.nb_clusters(); ++c )
             for ( I64 k = nt.cl_begin[ sz( c ) ]; k < nt.cl_begin[ sz( c + 1 ) ]; ++k ) clusters[ nt.cl_members[ sz( k ) ] ] = nt.cl_members[ sz( nt.cl_begin[ sz( c ) ] ) ];
     }
+    return nb_steps;
+}
+
+// ---- 3D ---------------------------------------------------------------------------------------------------------------
+
+static orc::Density3 make_density3( int kind, double value, int ng, const double *gauss ) {
+    orc::Density3 d;
+    d.kind = kind;
+    d.value = value;
+    if ( kind == 2 ) { d.ng = ng; d.gauss = gauss; }
+    return d;
+}
+
+/// the quadrature rules: `tet[ 14 ][ 5 ]` ( four barycentric coordinates, weight ), `tri[ 7 ][ 4 ]`
+void oracle_rules( double *tet, double *tri ) {
+    const orc::TetRule &T = orc::tet_rule();
+    for ( int q = 0; q < 14; ++q ) { for ( int i = 0; i < 4; ++i ) tet[ 5 * q + i ] = T.b[ q ][ i ]; tet[ 5 * q + 4 ] = T.w[ q ]; }
+    const orc::TriRule &R = orc::tri_rule();
+    for ( int q = 0; q < 7; ++q ) { for ( int i = 0; i < 3; ++i ) tri[ 4 * q + i ] = R.b[ q ][ i ]; tri[ 4 * q + 3 ] = R.w[ q ]; }
+}
+
+/// the 3D tree: `order[ n ]`, `beg / end[ N ]`, `box[ N ][ 6 ]`, and with weights the majorants `wa[ N ][ 3 ]`, `wb[ N ]`
+int oracle_build_tree_3d( I64 n, const double *pos, const double *w, I64 leaf, I64 *order, I64 *beg, I64 *end, double *box, double *wa,
+                          double *wb ) {
+    orc::Tree3 t;
+    orc::build_tree3( t, n, pos, leaf );
+    if ( w ) orc::set_weights3( t, w );
+    for ( I64 k = 0; k < n; ++k ) order[ k ] = t.order[ sz( k ) ];
+    for ( I64 m = 0; m < t.nb_nodes; ++m ) {
+        beg[ m ] = t.beg[ sz( m ) ]; end[ m ] = t.end[ sz( m ) ];
+        for ( int c = 0; c < 6; ++c ) box[ 6 * m + c ] = t.box[ sz( 6 * m + c ) ];
+        if ( wa ) for ( int d = 0; d < 3; ++d ) wa[ 3 * m + d ] = t.wa[ sz( 3 * m + d ) ];
+        if ( wb ) wb[ m ] = t.wb[ sz( m ) ];
+    }
+    return t.depth;
+}
+
+/// the masses ( user order ) and, up to `cap`, the facets ( user indices ); returns the facet count ( all of them ).
+/// `info[ 0 ]` the cells whose cut could not be closed, `info[ 1 .. 5 ]` the events ( side tests, pruning tests, new
+/// vertices, node visits, cuts )
+I64 oracle_measures_3d( I64 n, const double *pos, const double *w, const double *lo, const double *hi, int kind, double value, int ng,
+                        const double *gauss, double *masses, I64 *fi, I64 *fj, double *fc, I64 cap, I64 *info ) {
+    orc::Tree3 t;
+    orc::build_tree3( t, n, pos, 10 );
+    orc::Sweep3 sw( t, lo, hi, make_density3( kind, value, ng, gauss ) );
+    std::vector<double> W( w, w + n ), a;
+    std::vector<orc::Facet> fa;
+    sw.set_weights( W );
+    sw.measures( a, &fa );
+    for ( I64 i = 0; i < n; ++i ) masses[ i ] = a[ sz( i ) ];
+    for ( size_t q = 0; q < fa.size() && I64( q ) < cap; ++q ) { fi[ q ] = fa[ q ].i; fj[ q ] = fa[ q ].j; fc[ q ] = fa[ q ].c; }
+    if ( info ) {
+        info[ 0 ] = sw.nb_broken;
+        info[ 1 ] = sw.ev.side_tests; info[ 2 ] = sw.ev.prune_vertex_tests; info[ 3 ] = sw.ev.new_vertices;
+        info[ 4 ] = sw.ev.node_visits; info[ 5 ] = sw.ev.cuts;
+    }
+    return I64( fa.size() );
+}
+
+/// one cell ( user index `i` ): its vertices ( global frame, up to `cap` ) and the ids of its faces that carry a polygon
+/// ( user index of the seed, or `-1 - f` for face `f` of the box ); `brute`: cut by every seed instead of the tree.
+/// Returns the vertex count ( 0: empty ), `-1` if the cell's topology broke
+int oracle_cell_3d( I64 n, const double *pos, const double *w, const double *lo, const double *hi, I64 i, int brute, double *xyz, int cap,
+                    I64 *face_ids, double *face_areas, int *nb_faces ) {
+    orc::Tree3 t;
+    orc::build_tree3( t, n, pos, 10 );
+    std::vector<double> W( w, w + n );
+    orc::set_weights3( t, W.data() );
+    const I64 k = t.rank_of[ sz( i ) ];
+    orc::Poly3 c;
+    const bool ok = brute ? orc::make_cell3_all( t, k, lo, hi, c ) : orc::make_cell3( t, k, lo, hi, c, nullptr );
+    *nb_faces = 0;
+    if ( ! ok ) return 0;
+    if ( c.broken ) return -1;
+    const double *p0 = &t.p[ sz( 3 * k ) ];
+    for ( int v = 0; v < c.nv && v < cap; ++v ) { xyz[ 3 * v ] = c.x[ sz( v ) ] + p0[ 0 ]; xyz[ 3 * v + 1 ] = c.y[ sz( v ) ] + p0[ 1 ]; xyz[ 3 * v + 2 ] = c.z[ sz( v ) ] + p0[ 2 ]; }
+    orc::Density3 one;
+    int nf = 0;
+    orc::for_each_face_and_tet( c, [&]( int f, const std::vector<int> &loop ) {
+        if ( nf >= cap ) return;
+        const int id = c.f[ sz( f ) ].id;
+        face_ids[ nf ] = id >= 0 ? t.order[ sz( id ) ] : I64( id );
+        face_areas[ nf ] = orc::face_integral( one, p0, c, loop );
+        ++nf;
+    }, []( const double *, const double *, const double *, const double *, double ) {} );
+    *nb_faces = nf;
+    return c.nv;
+}
+
+/// the 3D solve: the same options, statistics and history as `oracle_solve` ( `opts[ 12 ]`, the step, is by trials: the
+/// limits are a 2D step ); a constant density ( kind 0 ) or a sum of Gaussians ( kind 2, `gauss[ ng ][ 5 ]` ); no duplicates
+int oracle_solve_3d( I64 n, const double *pos, const double *nu_in, const double *w0, const double *lo, const double *hi, int kind,
+                     double value, int ng, const double *gauss, const double *opts, double *weights, double *stats, double *hist,
+                     double *masses, double *bary, double *cost ) {
+    using namespace orc;
+    const double t_begin = now();
+    Options o;
+    o.tol_abs = opts[ 0 ]; o.tol_rel = opts[ 1 ]; o.maxit = int( opts[ 2 ] ); o.max_backtracks = int( opts[ 3 ] );
+    o.t_min = opts[ 4 ]; o.mult_ok = opts[ 5 ]; o.residual = int( opts[ 6 ] ); o.power = opts[ 7 ]; o.switch_residual = opts[ 8 ];
+    o.lin_tol = opts[ 9 ]; o.lin_maxit = int( opts[ 10 ] ); o.keep_start = int( opts[ 11 ] );
+    o.step = TRIALS;
+    const int continuation = int( opts[ 14 ] );
+    const double conv_threshold = opts[ 15 ], conv_s0 = opts[ 16 ], conv_ratio = opts[ 17 ], conv_min = opts[ 18 ];
+    Tree3 t;
+    build_tree3( t, n, pos, 10 );
+    const Density3 dn0 = make_density3( kind, value, ng, gauss );
+    Sweep3 sw( t, lo, hi, dn0 );
+    SolverT<Sweep3> nt( sw, o );
+    nt.pos = pos;
+    nt.set_duplicates( {}, {}, 0.0, n );
+    const bool possible = kind == 2;
+    double diam2 = 0;
+    for ( int d = 0; d < 3; ++d ) diam2 += ( hi[ d ] - lo[ d ] ) * ( hi[ d ] - lo[ d ] );
+    const double s0 = conv_s0 > 0 ? conv_s0 : 0.5 * std::sqrt( diam2 );
+    double smallest = 1e300;
+    for ( int q = 0; q < dn0.ng; ++q ) smallest = std::min( smallest, dn0.gauss[ 5 * q + 3 ] );
+    const double s_min = conv_min > 0 ? conv_min : smallest / 4;
+    std::vector<double> scales = continuation == 2 && possible ? continuation_steps( s0, conv_ratio, s_min ) : std::vector<double>{ 0.0 };
+    double s_current = 0;
+    auto set_stage = [&]( double sc ) {
+        s_current = sc;
+        Density3 dn = dn0;
+        dn.conv_s = possible ? sc : 0;
+        sw.dn = dn;
+    };
+    set_stage( scales[ 0 ] );
+    const int cap_steps = o.maxit + 1;
+    int nb_steps = 0, nb_steps_before = 0;
+    nt.after_step = [&]( int it, double tt, int nb_evals ) {
+        if ( nb_steps >= cap_steps ) return;
+        const double mn = nt.floor_min( nt.a );
+        double mx = 0, l2 = 0;
+        for ( I64 i = 0; i < n; ++i ) {
+            const double e = nt.a[ sz( i ) ] - nt.nu[ sz( i ) ];
+            mx = std::max( mx, std::fabs( e ) );
+            l2 += e * e;
+        }
+        double *r = hist + sz( nb_steps ) * NB_HIST;
+        r[ H_STEP ] = nb_steps_before + it; r[ H_T ] = tt; r[ H_RESIDUAL_L2 ] = std::sqrt( l2 ); r[ H_MIN_MASS ] = mn;
+        r[ H_MAX_RESIDUAL ] = mx; r[ H_NB_DIAG ] = sw.nb_diag; r[ H_NB_EVALS ] = nb_evals; r[ H_S ] = s_current;
+        ++nb_steps;
+    };
+    std::vector<double> nu( nu_in, nu_in + n ), w( sz( n ), 0.0 );
+    bool given = false;
+    if ( w0 ) for ( I64 i = 0; i < n; ++i ) { w[ sz( i ) ] = w0[ i ]; given |= w[ sz( i ) ] != 0; }
+    int start = given ? 0 : 1;
+    const double gauge = w[ 0 ];
+    for ( auto &v : w ) v -= gauge;
+    std::vector<double> a;
+    std::vector<Facet> fa;
+    nt.measures_and_facets( w, a, fa );
+    double nu_min = nu[ 0 ];
+    for ( double v : nu ) nu_min = std::min( nu_min, v );
+    auto min_mass = [&]( const std::vector<double> &m ) { return nt.floor_min( m ); };
+    if ( given && ! o.keep_start && min_mass( a ) < 1e-3 * nu_min ) {
+        std::vector<double> wz( sz( n ), 0.0 ), az;
+        std::vector<Facet> fz;
+        nt.measures_and_facets( wz, az, fz );
+        if ( min_mass( az ) > min_mass( a ) ) { w.swap( wz ); a.swap( az ); fa.swap( fz ); start = 1; }
+        else sw.set_weights( w );
+    }
+    if ( min_mass( a ) <= 0 && ! ( given && o.keep_start ) ) {
+        std::vector<double> w1, a1;
+        std::vector<Facet> f1;
+        similarity3( pos, n, lo, hi, w1 );
+        nt.measures_and_facets( w1, a1, f1 );
+        if ( min_mass( a1 ) > min_mass( a ) ) { w.swap( w1 ); a.swap( a1 ); fa.swap( f1 ); start = 2; }
+        else sw.set_weights( w );
+    }
+    const double min_start_mass = min_mass( a );
+    if ( continuation == 1 && possible && scales.size() == 1 && min_start_mass < conv_threshold * nu_min ) {
+        scales = continuation_steps( s0, conv_ratio, s_min );
+        set_stage( scales[ 0 ] );
+        nt.measures_and_facets( w, a, fa );
+    }
+    double domain_mass = 0, residual0 = 0, eps0 = 0;
+    int it_switch = -1;
+    for ( size_t stage = 0; stage < scales.size(); ++stage ) {
+        if ( stage > 0 ) {
+            set_stage( scales[ stage ] );
+            nt.measures_and_facets( w, a, fa );
+        }
+        double nu_mass = 0;
+        domain_mass = 0;
+        for ( I64 i = 0; i < n; ++i ) { domain_mass += a[ sz( i ) ]; nu_mass += nu[ sz( i ) ]; }
+        std::vector<double> nu_s = nu;
+        if ( domain_mass > 0 && nu_mass > 0 && domain_mass != nu_mass )
+            for ( auto &v : nu_s ) v *= domain_mass / nu_mass;
+        nt.nu = nu_s;
+        nt.a = a;
+        nt.fa = fa;
+        nt.it_switch = -1;
+        nt.solves( w );
+        w = nt.w;
+        a = nt.a;
+        fa = nt.fa;
+        if ( stage == 0 ) { residual0 = nt.residual0; eps0 = nt.eps; it_switch = nt.it_switch; }
+        nb_steps_before = nb_steps > 0 ? int( hist[ sz( nb_steps - 1 ) * NB_HIST + H_STEP ] ) + 1 : 0;
+        if ( nt.status == 4 ) break;
+    }
+    for ( I64 i = 0; i < n; ++i ) { weights[ i ] = w[ sz( i ) ]; masses[ i ] = a[ sz( i ) ]; }
+    const int nb_diag_descent = sw.nb_diag;
+    {
+        set_stage( 0 );
+        sw.set_weights( w );
+        std::vector<double> bc;
+        double c = 0;
+        sw.moments( bc, c );
+        std::memcpy( bary, bc.data(), sizeof( double ) * sz( n ) * 3 );
+        *cost = c;
+    }
+    for ( int q = 0; q < NB_STATS; ++q ) stats[ q ] = 0;
+    stats[ STATUS ] = nt.status;
+    stats[ RESIDUAL ] = nt.residual;
+    stats[ RESIDUAL0 ] = residual0;
+    stats[ NB_ITER ] = nt.nb_iter;
+    stats[ NB_DIAG ] = nb_diag_descent;
+    stats[ NB_BACKTRACKS ] = nt.nb_backtracks;
+    stats[ T_MAJORANT ] = sw.t_majorant;
+    stats[ T_DIAG ] = sw.t_diag;
+    stats[ T_ASM ] = nt.t_asm;
+    stats[ T_LIN ] = nt.lst.t;
+    stats[ EPS ] = eps0;
+    stats[ DOMAIN_MASS ] = domain_mass;
+    stats[ LIN_NB_ITER ] = nt.lst.nb_iter;
+    stats[ LIN_WORST ] = nt.lst.worst;
+    stats[ START ] = start;
+    stats[ T_TOTAL ] = now() - t_begin;
+    stats[ NB_CONTINUATION_STEPS ] = double( scales.size() );
+    stats[ MIN_START_MASS ] = min_start_mass;
+    stats[ IT_SWITCH ] = it_switch;
+    stats[ RESIDUAL_FULL ] = nt.residual;
     return nb_steps;
 }
 
