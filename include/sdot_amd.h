@@ -109,6 +109,41 @@ int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const
                        double *weights, double *history, long long *nb_steps, double *stats,
                        double *cell_masses, double *barycenters, double *cost, long long *clusters );
 
+/* ONE 3D DIAGRAM: the masses of the power cells of `( positions, weights )` in the box
+ * `box = { lo_x, lo_y, lo_z, hi_x, hi_y, hi_z }` ( HOST ) against a constant density or a sum of Gaussians, and optionally
+ * the facets `c_ij = int_facet rho / ( 2 |p_i - p_j| )` as triplets ( rows in user order, each row in its cell's face order ).
+ * Replaces `power_diagram_measures_card_3d` -> `sdot::gpu3d::measures` ( src/sdot/include/sdot/gpu/Cell3D.cuh:1941-1952,
+ * called from src/sdot/PowerDiagram_Bsp.py:271-311 ) and, for a sum of Gaussians, the CPU `power_diagram_measures`
+ * ( src/sdot/PowerDiagram.py:375-391 ).
+ *   positions    DEVICE [ n ][ 3 ]
+ *   weights      DEVICE [ n ]       ( null: all zero, a Voronoi diagram )
+ *   density_kind SDOT_AMD_CONSTANT ( `rho` ) or SDOT_AMD_GAUSSIANS ( `gauss` DEVICE [ ng ][ 5 ]: centre x, y, z, width, mass )
+ *   masses       DEVICE [ n ]       ( output )
+ *   facet_i/j/c  DEVICE [ facet_cap ] ( outputs; null: no facets )
+ *   nb_facets    HOST   [ 1 ]       ( output, may be null )
+ *   events       HOST   [ 5 ]       ( may be null: side tests, pruning vertex tests, new vertices, node visits, cuts )
+ * A cell of more than 64 vertices or 64 faces is a failure ( kind 2, code 1, the seed in `value` ). */
+int sdot_amd_measures_3d( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                          int density_kind, double rho, const double *gauss, int ng, double *masses, long long *facet_i,
+                          long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets, unsigned long long *events );
+
+/* THE 3D SOLVE: the same damped Newton as `sdot_amd_solve_2d` on 3D cells, the step by trials ( the limits are a 2D step ),
+ * a constant density or a sum of Gaussians ( the width continuation widens them ), no duplicates.
+ * Replaces the 3D card solve `sdot::gpu2d::solve` with `SolveCard<3,...>` ( src/sdot/include/sdot/gpu/Newton2D.cuh:819,
+ * 846-850 ) and, for a sum of Gaussians, the CPU `sdotplan_solve` ( src/sdot/include/sdot/sdotplan/Solve.h:111-114, called
+ * from src/sdot/SdotPlanNd.py:568-626 ).
+ *   positions    DEVICE [ n ][ 3 ]
+ *   nu, w0       as `sdot_amd_solve_2d`
+ *   options      HOST   [ 41 ]      the reference card's option vector ( the step, the image and duplicate entries unused )
+ *   box          HOST   [ 6 ]       the domain { lo_x, lo_y, lo_z, hi_x, hi_y, hi_z }
+ *   gauss        DEVICE [ ng ][ 5 ] ( SDOT_AMD_GAUSSIANS ); may be null for a constant density
+ *   weights, history, nb_steps, stats, cell_masses, cost   as `sdot_amd_solve_2d`
+ *   barycenters  DEVICE [ n ][ 3 ]  output ( the seed for an empty cell ) */
+int sdot_amd_solve_3d( void *stream, long long n, const double *positions, const double *nu, const double *w0,
+                       const double *options, const double *box, int density_kind, double rho, const double *gauss, int ng,
+                       double *weights, double *history, long long *nb_steps, double *stats,
+                       double *cell_masses, double *barycenters, double *cost );
+
 /* THE BATCH: `k` independent solves at once ( the reference solves one problem per call; this entry point has no
  * counterpart there and serves the batched configuration: many instances of uneven size ). Instance q owns the seeds
  * `offsets[ q ] .. offsets[ q + 1 ]` of the concatenated arrays and has its own box and density; all share `options`

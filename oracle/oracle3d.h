@@ -12,8 +12,7 @@
 // ( `PointwiseDensity.h:31-134`, relative tolerance 1e-5 ): a cell's MASS is the flux through its faces of a field whose
 // divergence is the density ( `flux_field` ), so that the masses of a diagram sum to the box's whatever the weights; a
 // face integral uses the triangles from the face's mean vertex and the 7-point degree-5 rule. The moments cut the cell in
-// tetrahedra ( apex at the mean of its vertices, bases the fan triangles of its faces ) integrated by the 14-point degree-5
-// rule. Both rules are checked on every monomial of degree <= 5 by the tests.
+// tetrahedra ( apex at the mean of its vertices, bases those triangles ) integrated by the 14-point degree-5 rule. Both rules are checked on every monomial of degree <= 5 by the tests.
 
 struct Density3 {
     int           kind  = 0;             ///< 0 a constant, 2 a sum of Gaussians
@@ -457,8 +456,8 @@ static void face_loop( const Poly3 &c, int f, std::vector<int> &loop ) {
     loop.clear();
 }
 
-/// for every face: `f( face, loop )`; then every tetrahedron ( apex the mean of the vertices, base a fan triangle of a
-/// face ) with its volume: `t( A, B, C, D, vol )`. Both in the seed's frame.
+/// for every face: `f( face, loop )`; then its tetrahedra ( apex `A` the mean of the cell's vertices, base the triangle
+/// of the face's mean vertex and one of its edges ) with their volumes: `t( A, B, C, D, vol )`. Both in the seed's frame.
 template<class FF, class TF>
 static void for_each_face_and_tet( const Poly3 &c, FF &&ff, TF &&tf ) {
     double ax = 0, ay = 0, az = 0;
@@ -470,10 +469,12 @@ static void for_each_face_and_tet( const Poly3 &c, FF &&ff, TF &&tf ) {
         face_loop( c, f, loop );
         if ( loop.size() < 3 ) continue;
         ff( f, loop );
-        const int v0 = loop[ 0 ];
-        const double B[ 3 ] = { c.x[ sz( v0 ) ], c.y[ sz( v0 ) ], c.z[ sz( v0 ) ] };
-        for ( size_t i = 1; i + 1 < loop.size(); ++i ) {
-            const int v1 = loop[ i ], v2 = loop[ i + 1 ];
+        const size_t nl = loop.size();
+        double B[ 3 ] = { 0, 0, 0 };
+        for ( int v : loop ) { B[ 0 ] += c.x[ sz( v ) ]; B[ 1 ] += c.y[ sz( v ) ]; B[ 2 ] += c.z[ sz( v ) ]; }
+        for ( double &b : B ) b /= double( nl );
+        for ( size_t i = 0; i < nl; ++i ) {
+            const int v1 = loop[ i ], v2 = loop[ ( i + 1 ) % nl ];
             const double C[ 3 ] = { c.x[ sz( v1 ) ], c.y[ sz( v1 ) ], c.z[ sz( v1 ) ] };
             const double D[ 3 ] = { c.x[ sz( v2 ) ], c.y[ sz( v2 ) ], c.z[ sz( v2 ) ] };
             const double u[ 3 ] = { B[ 0 ] - A[ 0 ], B[ 1 ] - A[ 1 ], B[ 2 ] - A[ 2 ] };
@@ -512,10 +513,15 @@ static void tet_moments( const Density3 &dn, const double *p0, const double *A, 
     }
 }
 
+/// the box's faces carry the flux out of the domain: their triangles are integrated on `BOX_SUB x BOX_SUB` sub-triangles,
+/// so that the masses' total ( the box's flux, split among the cells the box faces touch ) hardly depends on the weights
+constexpr int BOX_SUB = 4;
+
 /// `int_face rho` over a face polygon ( seed frame ), or with `flux` the flux `int_face F . n` of the field above through
 /// it ( `n` the face's outward unit normal ): triangles from the polygon's mean vertex, so that the two cells of a face
 /// integrate it alike whatever vertex their loops start from
-static double face_integral( const Density3 &dn, const double *p0, const Poly3 &c, const std::vector<int> &loop, const double *flux = nullptr ) {
+static double face_integral( const Density3 &dn, const double *p0, const Poly3 &c, const std::vector<int> &loop, const double *flux = nullptr,
+                             int sub = 1 ) {
     const size_t nl = loop.size();
     double gx = 0, gy = 0, gz = 0;
     for ( int v : loop ) { gx += c.x[ sz( v ) ]; gy += c.y[ sz( v ) ]; gz += c.z[ sz( v ) ]; }
@@ -528,18 +534,30 @@ static double face_integral( const Density3 &dn, const double *p0, const Poly3 &
         const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
         const double ar = 0.5 * std::sqrt( cx * cx + cy * cy + cz * cz );
         if ( dn.kind == 0 && ! flux ) { r += dn.value * ar; continue; }
+        // the triangle on an `m x m` grid of sub-triangles ( up `( i, j )`, `( i + 1, j )`, `( i, j + 1 )`, down
+        // `( i + 1, j )`, `( i + 1, j + 1 )`, `( i, j + 1 )` ), each by the 7-point rule
         const TriRule &R = tri_rule();
-        for ( int q = 0; q < 7; ++q ) {
-            const double x = R.b[ q ][ 0 ] * gx + R.b[ q ][ 1 ] * c.x[ sz( v1 ) ] + R.b[ q ][ 2 ] * c.x[ sz( v2 ) ];
-            const double y = R.b[ q ][ 0 ] * gy + R.b[ q ][ 1 ] * c.y[ sz( v1 ) ] + R.b[ q ][ 2 ] * c.y[ sz( v2 ) ];
-            const double z = R.b[ q ][ 0 ] * gz + R.b[ q ][ 1 ] * c.z[ sz( v1 ) ] + R.b[ q ][ 2 ] * c.z[ sz( v2 ) ];
-            if ( flux ) {
-                double F[ 3 ];
-                flux_field( dn, p0[ 0 ] + x, p0[ 1 ] + y, p0[ 2 ] + z, F );
-                r += R.w[ q ] * ar * ( F[ 0 ] * flux[ 0 ] + F[ 1 ] * flux[ 1 ] + F[ 2 ] * flux[ 2 ] );
-            } else
-                r += R.w[ q ] * ar * dn.at( p0[ 0 ] + x, p0[ 1 ] + y, p0[ 2 ] + z );
-        }
+        const double A[ 3 ] = { gx, gy, gz }, B[ 3 ] = { c.x[ sz( v1 ) ], c.y[ sz( v1 ) ], c.z[ sz( v1 ) ] }, C[ 3 ] = { c.x[ sz( v2 ) ], c.y[ sz( v2 ) ], c.z[ sz( v2 ) ] };
+        auto at = [&]( int i, int j, double *P ) { for ( int d = 0; d < 3; ++d ) P[ d ] = A[ d ] + ( B[ d ] - A[ d ] ) * i / sub + ( C[ d ] - A[ d ] ) * j / sub; };
+        const double sar = ar / ( sub * sub );
+        for ( int i = 0; i < sub; ++i )
+            for ( int j = 0; i + j < sub; ++j )
+                for ( int down = 0; down < ( i + j + 1 < sub ? 2 : 1 ); ++down ) {
+                    double P0[ 3 ], P1[ 3 ], P2[ 3 ];
+                    if ( ! down ) { at( i, j, P0 ); at( i + 1, j, P1 ); at( i, j + 1, P2 ); }
+                    else          { at( i + 1, j, P0 ); at( i + 1, j + 1, P1 ); at( i, j + 1, P2 ); }
+                    for ( int q = 0; q < 7; ++q ) {
+                        const double x = R.b[ q ][ 0 ] * P0[ 0 ] + R.b[ q ][ 1 ] * P1[ 0 ] + R.b[ q ][ 2 ] * P2[ 0 ];
+                        const double y = R.b[ q ][ 0 ] * P0[ 1 ] + R.b[ q ][ 1 ] * P1[ 1 ] + R.b[ q ][ 2 ] * P2[ 1 ];
+                        const double z = R.b[ q ][ 0 ] * P0[ 2 ] + R.b[ q ][ 1 ] * P1[ 2 ] + R.b[ q ][ 2 ] * P2[ 2 ];
+                        if ( flux ) {
+                            double F[ 3 ];
+                            flux_field( dn, p0[ 0 ] + x, p0[ 1 ] + y, p0[ 2 ] + z, F );
+                            r += R.w[ q ] * sar * ( F[ 0 ] * flux[ 0 ] + F[ 1 ] * flux[ 1 ] + F[ 2 ] * flux[ 2 ] );
+                        } else
+                            r += R.w[ q ] * sar * dn.at( p0[ 0 ] + x, p0[ 1 ] + y, p0[ 2 ] + z );
+                    }
+                }
     }
     return r;
 }
@@ -639,7 +657,7 @@ static double mass_and_facets3( const Tree3 &t, const Density3 &dn, I64 k, const
         const double d = std::sqrt( fc.nx * fc.nx + fc.ny * fc.ny + fc.nz * fc.nz );
         if ( dn.kind == 2 && d > 0 ) {                   // the mass: the flux through the face
             const double nrm[ 3 ] = { fc.nx / d, fc.ny / d, fc.nz / d };
-            m += face_integral( dn, p0, c, loop, nrm );
+            m += face_integral( dn, p0, c, loop, nrm, fc.id < 0 ? BOX_SUB : 1 );
         }
         if ( ! with_facets || fc.id < 0 ) return;
         if ( d > 0 ) facet( I64( fc.id ), face_integral( dn, p0, c, loop ) / ( 2 * d ) );

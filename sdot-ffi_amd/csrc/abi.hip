@@ -77,6 +77,23 @@ Density make_density( int kind, double rho, const double *img, int nx, int ny, c
     return d;
 }
 
+Density3 make_density3( int kind, double rho, const double *gauss, int ng ) {
+    Density3 d;
+    d.kind = kind;
+    d.rho = rho;
+    if ( kind == SDOT_AMD_GAUSSIANS ) {
+        SDOT_CHECK( gauss != nullptr && ng > 0, "a sum of Gaussians needs at least one Gaussian" );
+        d.gauss = gauss; d.ng = ng;
+    } else
+        SDOT_CHECK( kind == SDOT_AMD_CONSTANT, "a 3D density is a constant or a sum of Gaussians" );
+    return d;
+}
+
+void check_box3( const double *box ) {
+    SDOT_CHECK( box != nullptr, "the domain box is required" );
+    SDOT_CHECK( box[ 3 ] > box[ 0 ] && box[ 4 ] > box[ 1 ] && box[ 5 ] > box[ 2 ], "the domain box is empty" );
+}
+
 void check_box( const double *box ) {
     SDOT_CHECK( box != nullptr, "the domain box is required" );
     SDOT_CHECK( box[ 2 ] > box[ 0 ] && box[ 3 ] > box[ 1 ], "the domain box is empty" );
@@ -210,6 +227,46 @@ int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const
         out.cell_masses = cell_masses;
         out.barycenters = barycenters;
         out.clusters = clusters;
+        out.history = history;
+        out.nb_steps = nb_steps;
+        out.stats = stats;
+        out.cost = cost;
+        solve_2d( in, out, hipStream_t( stream ) );
+    } );
+}
+
+int sdot_amd_measures_3d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                          double rho, const double *gauss, int ng, double *masses, long long *facet_i, long long *facet_j, double *facet_c,
+                          long long facet_cap, long long *nb_facets, unsigned long long *events ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && masses, "measures: positions and masses are required" );
+        check_box3( box );
+        SDOT_CHECK( ! facet_i || ( facet_j && facet_c ), "measures: facet_i, facet_j and facet_c go together" );
+        measures_3d( hipStream_t( stream ), n, positions, weights, box, make_density3( density_kind, rho, gauss, ng ), masses, facet_i, facet_j,
+                     facet_c, facet_cap, nb_facets, events );
+    } );
+}
+
+int sdot_amd_solve_3d( void *stream, long long n, const double *positions, const double *nu, const double *w0, const double *options,
+                       const double *box, int density_kind, double rho, const double *gauss, int ng, double *weights, double *history,
+                       long long *nb_steps, double *stats, double *cell_masses, double *barycenters, double *cost ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 2 && positions && nu && options && weights && cell_masses && barycenters, "solve: a required array is null" );
+        check_box3( box );
+        SDOT_CHECK( options[ O_MAXIT ] >= 0, "solve: negative maxit" );
+        SolveInputs in;
+        in.dim = 3;
+        in.n = n;
+        in.positions = positions;
+        in.nu = nu;
+        in.w0 = w0;
+        in.options = options;
+        for ( int c = 0; c < 6; ++c ) in.box3[ c ] = box[ c ];
+        in.dens3 = make_density3( density_kind, rho, gauss, ng );
+        SolveOutputs out;
+        out.weights = weights;
+        out.cell_masses = cell_masses;
+        out.barycenters = barycenters;
         out.history = history;
         out.nb_steps = nb_steps;
         out.stats = stats;

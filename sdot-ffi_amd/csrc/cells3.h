@@ -1,0 +1,28 @@
+// The power cells of a 3D diagram on the device: one wave per cell, one lane per vertex of the cell's polytope.
+#pragma once
+
+#include "cells.h"
+#include "tree3.h"
+
+namespace sdot_amd {
+
+/// the 3D target density: a constant on the domain box, or a sum of isotropic Gaussians truncated to the box
+struct Density3 {
+    int           kind = 0;        ///< 0 constant, 2 a sum of Gaussians
+    double        rho = 1;
+    const double *gauss = nullptr; ///< device, `[ ng ][ 5 ]`: centre x, y, z, width, mass
+    int           ng = 0;
+};
+
+/// the vertex and face slots of a cell ( a cell past them is a failure, code 1 )
+constexpr int CELL3_VERTICES = 64;
+constexpr int CELL3_FACES = 64;
+
+/// one 3D diagram into `out` ( rank order ): masses, and the rows of the Laplacian ( facets mode ) or the moments. The rows
+/// of 64 consecutive ranks share a block of `64 room` entries, interleaved as the 2D rows ( entry `q` of rank `k` at
+/// `( k / 64 ) 64 room + k % 64 + 64 q` ); `out.counters[ 0 ]` receives the longest row, so that a diagram whose rows
+/// outgrew `room` is redone with more. Failures: `counters[ 2 ]`, the first failing rank in `counters[ 3 ]`.
+void run_cells_3d( const DevTree3 &t, const double *lo, const double *hi, const Density3 &dn, int mode, DiagramOut &out, int room,
+                   hipStream_t st );
+
+} // namespace sdot_amd

@@ -1,7 +1,8 @@
-// The damped-Newton solve of a 2D semi-discrete transport on the device, and its pieces.
+// The damped-Newton solve of a 2D or 3D semi-discrete transport on the device, and its pieces.
 #pragma once
 
 #include "cells.h"
+#include "cells3.h"
 
 namespace sdot_amd {
 
@@ -49,6 +50,11 @@ struct SolveInputs {
     I64 nb_dups = 0;
     double box[ 4 ] = { 0, 0, 1, 1 };
     Density dens;
+    // 3D ( `dim == 3` ): positions `[ n ][ 3 ]`, the box `lo x, y, z, hi x, y, z`, a constant or a sum of Gaussians; the
+    // step is by trials, and there are no duplicates
+    int dim = 2;
+    double box3[ 6 ] = { 0, 0, 0, 1, 1, 1 };
+    Density3 dens3;
 };
 struct SolveOutputs {
     double *weights = nullptr, *cell_masses = nullptr, *barycenters = nullptr;   ///< device, user order
@@ -59,10 +65,15 @@ struct SolveOutputs {
     double *cost = nullptr;                                                     ///< host [ 1 ]
 };
 
+/// one solve, 2D or 3D after `in.dim`
 void solve_2d( const SolveInputs &in, const SolveOutputs &out, hipStream_t st );
 
 /// the masses ( and facets ) of one diagram, user order ( see `sdot_amd_measures_2d` )
 void measures_2d( hipStream_t st, I64 n, const double *positions, const double *weights, const double *box, const Density &dn,
+                  double *masses, long long *fi, long long *fj, double *fc, I64 facet_cap, I64 *nb_facets, unsigned long long *events );
+
+/// the masses ( and facets ) of one 3D diagram, user order ( see `sdot_amd_measures_3d` )
+void measures_3d( hipStream_t st, I64 n, const double *positions, const double *weights, const double *box, const Density3 &dn,
                   double *masses, long long *fi, long long *fj, double *fc, I64 facet_cap, I64 *nb_facets, unsigned long long *events );
 
 } // namespace sdot_amd

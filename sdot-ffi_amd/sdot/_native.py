@@ -30,9 +30,9 @@ EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_byte
             "sdot_amd_kernel_times", "sdot_amd_measures_2d",
             "sdot_amd_solve_2d", "sdot_amd_tree_nodes", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
             "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_start", "sdot_amd_server_submit",
-            "sdot_amd_server_wait", "sdot_amd_server_stop" ]
+            "sdot_amd_server_wait", "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d" ]
 
-CONSTANT, IMAGE = 0, 1
+CONSTANT, IMAGE, GAUSSIANS = 0, 1, 2
 
 
 class SdotError( RuntimeError ):
@@ -70,6 +70,8 @@ def lib():
         L.sdot_amd_refresh_majorants_2d.argtypes = [ _P, _LL, _LL, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_linear_solves_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _LL, _P, _P, _P ]
         L.sdot_amd_solve_2d_batch.argtypes = [ _P, _I, _LL, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_measures_3d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _P, _P, _P, _P, _LL, _P, _P ]
+        L.sdot_amd_solve_3d.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _I, _D, _P, _I, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_server_start.argtypes = [ _I, _D ]
         L.sdot_amd_server_start.restype = _LL
         L.sdot_amd_server_submit.argtypes = [ _LL, _LL, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P ]
@@ -81,7 +83,7 @@ def lib():
         L.sdot_amd_kernel_times.restype = _I
         for name in ( "sdot_amd_measures_2d", "sdot_amd_solve_2d", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
                       "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_submit", "sdot_amd_server_wait",
-                      "sdot_amd_server_stop" ):
+                      "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d" ):
             getattr( L, name ).restype = _I
         if L.sdot_amd_abi_version() != 1:
             raise RuntimeError( "libsdot_amd.so: unexpected ABI version" )
@@ -167,6 +169,70 @@ def measures_2d( positions, weights, box, density, facets = False, events = Fals
     if events:
         res.append( dict( zip( ( "side_tests", "prune_vertex_tests", "new_vertices", "node_visits", "cuts" ), [ int( x ) for x in ev ] ) ) )
     return res[ 0 ] if len( res ) == 1 else tuple( res )
+
+
+def _density3_args( density ):
+    """`( kind, rho, gaussians device tensor [ ng, 5 ] or None, ng )` of a 3D density: `None` / a float ( a constant ), an
+    object with `native_args3`, or `[ ng, 5 ]` rows ( centre x, y, z, width, mass )"""
+    if density is None or np.isscalar( density ):
+        return CONSTANT, 1.0 if density is None else float( density ), None, 0
+    if hasattr( density, "native_args3" ):
+        return density.native_args3( device_tensor = on_device )
+    g = on_device( np.asarray( density, dtype = np.float64 ).reshape( -1, 5 ) )
+    return GAUSSIANS, 0.0, g, int( g.shape[ 0 ] )
+
+
+def measures_3d( positions, weights, box, density = None, facets = False, events = False ):
+    """masses of a 3D diagram ( device tensor, user order ); with `facets` also `( fi, fj, fc )`; with `events` the counts"""
+    T = torch()
+    pos = on_device( positions ).reshape( -1, 3 ).contiguous()
+    n = pos.shape[ 0 ]
+    w = None if weights is None else on_device( weights ).reshape( -1 )
+    boxh, boxp = _host( box )
+    kind, rho, g, ng = _density3_args( density )
+    masses = T.empty( n, dtype = T.float64, device = device() )
+    cap = 24 * n + 1024 if facets else 0
+    fi = T.empty( cap, dtype = T.int64, device = device() ) if facets else None
+    fj = T.empty( cap, dtype = T.int64, device = device() ) if facets else None
+    fc = T.empty( cap, dtype = T.float64, device = device() ) if facets else None
+    nbf = np.zeros( 1, np.int64 )
+    ev = np.zeros( 5, np.uint64 )
+    _check( lib().sdot_amd_measures_3d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( g ), ng, _ptr( masses ),
+                                        _ptr( fi ), _ptr( fj ), _ptr( fc ), cap, nbf.ctypes.data_as( _P ),
+                                        ev.ctypes.data_as( _P ) if events else None ) )
+    res = [ masses ]
+    if facets:
+        k = int( nbf[ 0 ] )
+        res.append( ( fi[ :k ], fj[ :k ], fc[ :k ] ) )
+    if events:
+        res.append( dict( zip( ( "side_tests", "prune_vertex_tests", "new_vertices", "node_visits", "cuts" ), [ int( x ) for x in ev ] ) ) )
+    return res[ 0 ] if len( res ) == 1 else tuple( res )
+
+
+def solve_3d( positions, nu, w0, options, box, density = None ):
+    """one 3D solve; returns a dict of device tensors and host arrays ( see `sdot_amd_solve_3d` )"""
+    T = torch()
+    pos = on_device( positions ).reshape( -1, 3 ).contiguous()
+    n = pos.shape[ 0 ]
+    nu = on_device( nu ).reshape( -1 )
+    w0 = None if w0 is None else on_device( w0 ).reshape( -1 )
+    opts, optp = _host( options )
+    boxh, boxp = _host( box )
+    kind, rho, g, ng = _density3_args( density )
+    maxit = int( opts[ CARD_OPTIONS.index( "maxit" ) ] )
+    dev = device()
+    weights = T.empty( n, dtype = T.float64, device = dev )
+    masses = T.empty( n, dtype = T.float64, device = dev )
+    bary = T.empty( ( n, 3 ), dtype = T.float64, device = dev )
+    hist = np.zeros( ( maxit + 1, 8 ) )
+    nb_steps = np.zeros( 1, np.int64 )
+    stats = np.zeros( 31 )
+    cost = np.zeros( 1 )
+    _check( lib().sdot_amd_solve_3d( _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), optp, boxp, kind, rho, _ptr( g ), ng,
+                                     _ptr( weights ), hist.ctypes.data_as( _P ), nb_steps.ctypes.data_as( _P ), stats.ctypes.data_as( _P ),
+                                     _ptr( masses ), _ptr( bary ), cost.ctypes.data_as( _P ) ) )
+    return dict( weights = weights, cell_masses = masses, barycenters = bary, clusters = None, cost = float( cost[ 0 ] ),
+                 stats = stats, history = hist[ :int( nb_steps[ 0 ] ) ] )
 
 
 def solve_2d( positions, nu, w0, options, box, density, dups = None ):

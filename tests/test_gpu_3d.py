@@ -1,0 +1,101 @@
+"""3D parity on the MI355X, through the C ABI: the diagram's masses and facets against the oracle ( constant density and a
+sum of Gaussians ), closed forms ( the box's volume, a grid's equal cells ), and solves that reach the tolerance with the
+oracle's plan ( weights, masses, barycenters, cost )."""
+import itertools
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BOX3 = [ 0.0, 0.0, 0.0, 1.0, 1.0, 1.0 ]
+
+
+def gaussians( rng, ng = 4, sigma = 0.1 ):
+    mu = rng.uniform( 0.2, 0.8, ( ng, 3 ) )
+    mass = rng.dirichlet( np.ones( ng ) )
+    return np.column_stack( [ mu, np.full( ng, sigma ), mass ] )
+
+
+def rows_of( fi, fj, fc ):
+    """facets as a dict ( i, j ) -> c"""
+    return { ( int( i ), int( j ) ): float( c ) for i, j, c in zip( fi, fj, fc ) }
+
+
+@pytest.mark.parametrize( "kind,n,seed", [ ( "lebesgue", 3000, 0 ), ( "gaussians", 3000, 1 ), ( "lebesgue", 20000, 2 ) ] )
+def test_3d_masses_and_facets_match_the_oracle( native, oracle, kind, n, seed ):
+    print( f"3d measures: { kind } n={ n } seed={ seed }", flush = True )
+    rng = np.random.default_rng( seed )
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    w = rng.normal( 0, 1e-4 * n ** ( -2 / 3 ) * 100, n )
+    g = gaussians( rng ) if kind == "gaussians" else None
+    m, ( fi, fj, fc ), ev = native.measures_3d( pos, w, BOX3, g, facets = True, events = True )
+    m = m.cpu().numpy()
+    mr, ( ri, rj, rc ) = oracle.measures_3d( pos, w, density = None if g is None else dict( gaussians = g ), facets = True )
+    scale = mr.mean()
+    assert np.abs( m - mr ).max() <= 1e-11 * scale, np.abs( m - mr ).max() / scale
+    got, ref = rows_of( fi.cpu().numpy(), fj.cpu().numpy(), fc.cpu().numpy() ), rows_of( ri, rj, rc )
+    assert set( got ) == set( ref )
+    cmax = max( abs( v ) for v in ref.values() )
+    assert max( abs( got[ k ] - ref[ k ] ) for k in ref ) <= 1e-11 * cmax
+    if kind == "lebesgue":
+        assert abs( m.sum() - 1 ) < 1e-12
+    assert ev[ "cuts" ] > 0 and ev[ "side_tests" ] > ev[ "cuts" ]
+
+
+def test_3d_grid_cells_are_equal_and_one_seed_takes_the_box( native ):
+    g = ( np.arange( 6 ) + 0.5 ) / 6
+    pos = np.array( list( itertools.product( g, g, g ) ) )
+    m = native.measures_3d( pos, None, BOX3 ).cpu().numpy()
+    assert np.abs( m - 1 / 216 ).max() < 1e-15
+    box = [ -1.0, 0.0, 2.0, 3.0, 0.5, 2.5 ]
+    m = native.measures_3d( np.array( [ [ 0.3, 0.2, 2.1 ], [ 2.0, 0.4, 2.4 ] ] ), None, box ).cpu().numpy()
+    assert abs( m.sum() - 4 * 0.5 * 0.5 ) < 1e-14
+
+
+def _opts( tol, linear = "cg", continuation = "never" ):
+    from sdot import Iterative, Tuning
+    from sdot.SdotPlanNd import card_options
+    return card_options( Iterative( tol = tol, continuation = continuation,
+                                    tuning = Tuning( step = "trials", linear_solver = linear, linear_tol = 1e-12 if linear == "cg" else 1e-8 ) ), None )
+
+
+@pytest.mark.parametrize( "kind,linear", [ ( "lebesgue", "cg" ), ( "gaussians", "cg" ), ( "lebesgue", "mg" ) ] )
+def test_3d_solve_matches_the_oracle( native, oracle, kind, linear ):
+    print( f"3d solve: { kind } { linear }", flush = True )
+    rng = np.random.default_rng( 11 )
+    n = 4000
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    g = gaussians( rng, sigma = 0.15 ) if kind == "gaussians" else None
+    tol = 1e-9 * 1e6 / n / 1e2                           # the same relative accuracy as 1e-11 at 1M
+    nu = np.full( n, 1.0 / n )
+    r = native.solve_3d( pos, nu, None, _opts( tol, linear ), BOX3, g )
+    st = r[ "stats" ]
+    assert st[ 0 ] == 1, st[ :6 ]
+    ref = oracle.solve_3d( pos, density = None if g is None else dict( gaussians = g ), tol = tol, lin_tol = 1e-12 )
+    assert ref[ "stats" ][ "status" ] == 1
+    w = r[ "weights" ].cpu().numpy()
+    masses = r[ "cell_masses" ].cpu().numpy()
+    target = nu * st[ 12 ]                               # ( the targets at the domain's mass )
+    assert np.abs( masses - target ).max() <= tol
+    # one plan: the oracle's diagram of these weights passes the test too, and the weights are the oracle's
+    mo = oracle.measures_3d( pos, w, density = None if g is None else dict( gaussians = g ) )
+    assert np.abs( mo - target ).max() <= 1.01 * tol
+    spread = ref[ "weights" ].max() - ref[ "weights" ].min()
+    assert np.abs( w - ref[ "weights" ] ).max() <= 1e-5 * spread
+    b = r[ "barycenters" ].cpu().numpy()
+    assert np.abs( b - ref[ "barycenters" ] ).max() < 1e-6
+    assert abs( r[ "cost" ] - ref[ "cost" ] ) < 1e-6 * ref[ "cost" ]
+
+
+def test_3d_gaussians_far_from_the_corners_take_the_continuation( native, oracle ):
+    rng = np.random.default_rng( 12 )
+    n = 3000
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    g = gaussians( rng, sigma = 0.08 )
+    tol = 1e-9 * 1e6 / n / 1e2
+    r = native.solve_3d( pos, np.full( n, 1.0 / n ), None, _opts( tol, continuation = "auto" ), BOX3, g )
+    st = r[ "stats" ]
+    ref = oracle.solve_3d( pos, density = dict( gaussians = g ), tol = tol, lin_tol = 1e-12, continuation = "auto" )
+    assert st[ 0 ] == 1 and ref[ "stats" ][ "status" ] == 1
+    assert st[ 21 ] == ref[ "stats" ][ "nb_continuation_steps" ] and st[ 21 ] > 1
