@@ -225,8 +225,10 @@ __device__ void write_cell( const Params3 &P, int k0, const double4 &p0, const C
     }
 }
 
-__device__ void record_failure( const Params3 &P, int k0, I64 row0 ) {
-    if ( atomicAdd( &P.out.counters[ 2 ], 1u ) == 0 ) P.out.counters[ 3 ] = unsigned( k0 );
+/// a cell the passes could not build: `code` 1 past the slots' vertices or faces, 2 a cut whose new face could not be
+/// closed ( the reference's error codes, `gpu/Cell2D.cuh:99-102` ); the first failure's rank and code are kept
+__device__ void record_failure( const Params3 &P, int k0, I64 row0, int code ) {
+    if ( atomicAdd( &P.out.counters[ 2 ], 1u ) == 0 ) { P.out.counters[ 3 ] = unsigned( k0 ); P.out.counters[ 5 ] = unsigned( code ); }
     P.out.mass[ k0 ] = 0;
     if ( P.mode == MODE_FACETS ) { P.out.row_off[ k0 ] = int( row0 ); P.out.row_cnt[ k0 ] = 0; P.out.dia[ k0 ] = 1; }
 }
@@ -478,7 +480,7 @@ __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3( Params3 P ) {
     }
     const I64 row0 = I64( k0 / 64 ) * 64 * P.room + ( k0 & 63 );
     if ( status == C3_FAILED ) {
-        if ( lane == 0 ) record_failure( P, k0, row0 );
+        if ( lane == 0 ) record_failure( P, k0, row0, 2 );
         return;
     }
     if ( status == C3_FULL ) {                           // past the lanes: the second pass, one thread in global slots
@@ -543,7 +545,7 @@ __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3( Params3 P ) {
 
 // ---- the second pass: cells past the lanes, one thread each in global slots ----------------------------------------
 
-constexpr int SLOT_V = 192, SLOT_F = 192;               ///< a slot's vertices and faces
+constexpr int SLOT_V = CELL3_SLOT_VERTICES, SLOT_F = CELL3_SLOT_FACES;   ///< a slot's vertices and faces
 constexpr int PASS2_THREADS = 2048;
 
 /// one thread's slot: its cell ( vertices with their faces and neighbours, faces ) and a cut's staging
@@ -699,7 +701,7 @@ __global__ __launch_bounds__( 64 ) void k_cells3_pass2( Params3 P, Slot3 *slots 
         atomicAdd( &P.out.events[ 0 ], ev_side );
         atomicAdd( &P.out.events[ 2 ], ev_new );
         const I64 row0 = I64( k0 / 64 ) * 64 * P.room + ( k0 & 63 );
-        if ( status == C3_FAILED || status == C3_FULL ) { record_failure( P, k0, row0 ); continue; }
+        if ( status == C3_FAILED || status == C3_FULL ) { record_failure( P, k0, row0, status == C3_FULL ? 1 : 2 ); continue; }
         if ( status == C3_EMPTY ) C.nv = 0;
         // the measures, face by face as the first pass
         const bool facets = P.mode == MODE_FACETS, moments = P.mode == MODE_MOMENTS;

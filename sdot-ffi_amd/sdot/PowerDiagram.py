@@ -58,24 +58,30 @@ class PowerDiagram:
             dist = UniformBox( lo, hi )
         else:
             raise ValueError( "PowerDiagram: a domain is needed ( `boundaries` or `distribution` )" )
-        if len( lo ) != 2:
-            raise NotImplementedError( "PowerDiagram on the MI355X path: 2D" )
+        if len( lo ) not in ( 2, 3 ):
+            raise NotImplementedError( "PowerDiagram on the MI355X path: 2D or 3D" )
         self.distribution = dist
         self.box_min, self.box_max = lo, hi
+        self.nb_dims = len( lo )
 
     @property
     def nb_points( self ):
         return int( self.positions.shape[ 0 ] )
 
     def _box( self ):
-        return [ self.box_min[ 0 ], self.box_min[ 1 ], self.box_max[ 0 ], self.box_max[ 1 ] ]
+        return [ *self.box_min, *self.box_max ]
+
+    def _measures( self, facets ):
+        if self.nb_dims == 3:
+            return _native.measures_3d( self.positions, self.weights, self._box(), self.distribution, facets = facets )
+        return _native.measures_2d( self.positions, self.weights, self._box(), self.distribution, facets = facets )
 
     @property
     def measures( self ):
         """the mass of each cell against the density ( device tensor, user order )"""
-        return _native.measures_2d( self.positions, self.weights, self._box(), self.distribution )
+        return self._measures( False )
 
     def facets( self ):
         """`( i, j, c_ij )`: each cell's own view of its facets, `c_ij = int_facet rho / ( 2 |p_i - p_j| )`"""
-        _, f = _native.measures_2d( self.positions, self.weights, self._box(), self.distribution, facets = True )
+        _, f = self._measures( True )
         return f

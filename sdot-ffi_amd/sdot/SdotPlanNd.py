@@ -137,8 +137,8 @@ class SdotPlanNd:
         self.problem = problem
         src, dst = problem.source, problem.target
         d = problem.nb_dims
-        if d != 2:
-            raise NotImplementedError( f"SdotPlanNd on the MI355X path: 2D ( not { d }D )" )
+        if d not in ( 2, 3 ):
+            raise NotImplementedError( f"SdotPlanNd on the MI355X path: 2D or 3D ( not { d }D )" )
         if settings.precision == "fp32":
             raise NotImplementedError( "precision = 'fp32' is not on the MI355X path yet ( 'auto', 'fp64' or 'mixed' run FP64 cells )" )
         plan = settings.ot_plan if settings.ot_plan is not None else warm
@@ -156,6 +156,22 @@ class SdotPlanNd:
             w0, self._warm_start = None, "none"
         problem.domain
         lo, hi = dst.box()
+        if d == 3:                                       # ( a constant or a sum of Gaussians; the step by trials, no duplicates )
+            if not isinstance( dst, ( UniformBox, SumOfGaussians ) ):
+                raise NotImplementedError( f"SdotPlanNd on the MI355X path, 3D: a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
+            if settings.aggregate:
+                dup, rep = exact_duplicates( src.positions )
+                if len( dup ):
+                    raise NotImplementedError( f"SdotPlanNd on the MI355X path, 3D: { len( dup ) } exact duplicates ( merge them first )" )
+            self._dup_note = "checked" if settings.aggregate else "not checked ( aggregate = False )"
+            step = settings.tuning.step
+            if step == "limits":
+                raise NotImplementedError( "SdotPlanNd on the MI355X path, 3D: the step is by trials ( the limits are a 2D step )" )
+            opts = card_options( settings, dst, verbose, 0, 0.0 )
+            opts[ _native.CARD_OPTIONS.index( "step" ) ] = 0.0   # ( 'auto': trials in 3D )
+            res = _native.solve_3d( src.positions, src.weights, w0, opts, [ *lo[ :3 ], *hi[ :3 ] ], dst )
+            self._set_result( res, src, settings )
+            return
         box = [ lo[ 0 ], lo[ 1 ], hi[ 0 ], hi[ 1 ] ]
         if not isinstance( dst, ( Image, UniformBox, SumOfGaussians ) ):
             raise NotImplementedError( f"SdotPlanNd on the MI355X path: an Image, a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
@@ -171,12 +187,15 @@ class SdotPlanNd:
         opts = card_options( settings, dst, verbose, len( dup ), gap )
         pairs = np.stack( [ dup, rep ], axis = 1 ) if len( dup ) else None
         res = _native.solve_2d( src.positions, src.weights, w0, opts, box, dst, dups = pairs )
+        self._set_result( res, src, settings )
+
+    def _set_result( self, res, src, settings ):
         #: the fitted weights ( `weights[ 0 ] == 0` ), device tensor, user order
         self.weights = res[ "weights" ]
         self._weights_fit = self.weights
         #: the measure of each cell at the fitted weights
         self.cell_masses = res[ "cell_masses" ]
-        #: the barycenter of each cell ( its seed if empty ), `[ n, 2 ]`
+        #: the barycenter of each cell ( its seed if empty ), `[ n, d ]`
         self.barycenters = res[ "barycenters" ]
         #: the transport cost `sum_i int_{cell_i} |x - p_i|^2 rho`
         self.cost = res[ "cost" ]

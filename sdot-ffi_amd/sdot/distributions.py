@@ -130,6 +130,10 @@ class UniformBox:
     def native_args( self, device_tensor ):
         return 0, self.rho, None, 0, 0, [ 0.0, 0.0, 1.0, 1.0 ]
 
+    def native_args3( self, device_tensor ):
+        """`( kind, rho, gauss, ng )` of the 3D calls"""
+        return 0, self.rho, None, 0
+
 
 class SumOfGaussians:
     """isotropic Gaussians `weights_k N( positions_k, sigmas_k^2 I )` truncated to `support_box`
@@ -174,3 +178,10 @@ class SumOfGaussians:
             raise NotImplementedError( "SumOfGaussians: 2D on this path" )
         g = np.concatenate( [ self.positions, self.sigmas[ :, None ], self.weights[ :, None ] ], axis = 1 )
         return 2, 0.0, device_tensor( np.ascontiguousarray( g ).reshape( -1 ) ), g.shape[ 0 ], 0, [ 0.0, 0.0, 1.0, 1.0 ]
+
+    def native_args3( self, device_tensor ):
+        """`( kind, rho, gauss, ng )` of the 3D calls: rows ( centre x, y, z, width, mass )"""
+        if self.nb_dims != 3:
+            raise ValueError( "SumOfGaussians: native_args3 is for 3D Gaussians" )
+        g = np.concatenate( [ self.positions, self.sigmas[ :, None ], self.weights[ :, None ] ], axis = 1 )
+        return 2, 0.0, device_tensor( np.ascontiguousarray( g ) ), g.shape[ 0 ]

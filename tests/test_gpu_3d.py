@@ -99,3 +99,20 @@ def test_3d_gaussians_far_from_the_corners_take_the_continuation( native, oracle
     ref = oracle.solve_3d( pos, density = dict( gaussians = g ), tol = tol, lin_tol = 1e-12, continuation = "auto" )
     assert st[ 0 ] == 1 and ref[ "stats" ][ "status" ] == 1
     assert st[ 21 ] == ref[ "stats" ][ "nb_continuation_steps" ] and st[ 21 ] > 1
+
+
+def test_3d_through_the_reference_api( native ):
+    """`OtProblem( SumOfDiracs, SumOfGaussians 3D ).solve( Iterative )` and `PowerDiagram.measures` in 3D"""
+    import sdot
+    rng = np.random.default_rng( 13 )
+    n = 5000
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    g = gaussians( rng, sigma = 0.12 )
+    target = sdot.SumOfGaussians( g[ :, :3 ], g[ :, 3 ], g[ :, 4 ], support_box = ( [ 0, 0, 0 ], [ 1, 1, 1 ] ) )
+    sol = sdot.OtProblem( sdot.SumOfDiracs( pos ), target ).solve( sdot.Iterative( tol = 1e-9 * 1e6 / n ) )
+    assert sol.converged and sol.barycenters.shape == ( n, 3 )
+    m = sdot.PowerDiagram( pos, sol.weights, distribution = target ).measures.cpu().numpy()
+    assert np.abs( m - sol.target_masses ).max() <= 1.01e-9 * 1e6 / n
+    box = sdot.UniformBox( [ 0, 0, 0 ], [ 2, 1, 1 ] )
+    sol = sdot.OtProblem( sdot.SumOfDiracs( pos * [ 2, 1, 1 ] ), box ).solve( sdot.Iterative( tol = 1e-9 * 1e6 / n ) )
+    assert sol.converged
