@@ -22,6 +22,16 @@ namespace {
 constexpr int WAVES3 = 4;                    ///< waves ( cells ) per block
 constexpr int STACK3 = 64;
 
+/// the cells of the first pass kept for the measure kernel ( a sum of Gaussians ): per rank, 64 vertex and 64 face slots
+/// ( `[ k ][ 64 ]`, a wave's lanes contiguous ), the vertex count ( -1: the cell is measured elsewhere ) and face count
+struct PolyBuf {
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    int    *vpl = nullptr, *vnb = nullptr;
+    double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    int    *fid = nullptr;
+    int    *nv = nullptr, *nf = nullptr;
+};
+
 struct Params3 {
     const NodeRec3 *nodes;
     const double4  *seeds;
@@ -30,6 +40,7 @@ struct Params3 {
     Density3        dn;
     DiagramOut      out;
     int             mode, room;
+    PolyBuf         pb;
 };
 
 /// what a wave keeps in LDS: its cell's faces, the staging of a cut, the walk's stack
@@ -61,6 +72,13 @@ __device__ __forceinline__ double wave_maxd( double v ) {
 }
 
 __device__ __forceinline__ int byte_of( int packed, int j ) { return ( packed >> ( 8 * j ) ) & 255; }
+
+/// lane `l`'s value ( `l` uniform ), in scalar registers
+__device__ __forceinline__ double readlane_d( double v, int l ) {
+    const long long b = __double_as_longlong( v );
+    const int lo = __builtin_amdgcn_readlane( int( b ), l ), hi = __builtin_amdgcn_readlane( int( b >> 32 ), l );
+    return __longlong_as_double( ( (long long) hi << 32 ) | (long long) (unsigned) lo );
+}
 
 // ---- the density -------------------------------------------------------------------------------------------------
 
@@ -122,8 +140,10 @@ struct CellSums {
 /// the triangle `( g, v, o )` of a face of outward unit normal `u`, and the tetrahedron it makes with the cell's mean
 /// vertex `A` ( seed frame; the seed at `p0` ): the facet's `int rho`, the mass ( a constant: the tetrahedron's volume; a
 /// sum of Gaussians: the flux through the triangle ), the moments ( the tetrahedron's )
-__device__ __forceinline__ void edge_terms( const Density3 &dn, const double4 &p0, bool facets, bool moments, bool box_face, V3 A, V3 g, V3 v,
-                                           V3 o, V3 u, CellSums &S ) {
+template<bool GAUSS>
+__device__ __forceinline__ CellSums edge_terms( const Density3 &dn, const double4 &p0, bool facets, bool moments, bool box_face, V3 A, V3 g,
+                                               V3 v, V3 o, V3 u ) {
+    CellSums S;
     const double ux = v.x - g.x, uy = v.y - g.y, uz = v.z - g.z, tx = o.x - g.x, ty = o.y - g.y, tz = o.z - g.z;
     const double cx = uy * tz - uz * ty, cy = uz * tx - ux * tz, cz = ux * ty - uy * tx;
     const double ar = 0.5 * sqrt( cx * cx + cy * cy + cz * cz );
@@ -131,9 +151,9 @@ __device__ __forceinline__ void edge_terms( const Density3 &dn, const double4 &p
     const double hx = o.x - A.x, hy = o.y - A.y, hz = o.z - A.z;
     const double det = bx * ( ey * hz - ez * hy ) - by * ( ex * hz - ez * hx ) + bz * ( ex * hy - ey * hx );
     const double vol = fabs( det ) / 6;
-    if ( dn.kind != 2 ) {
+    if ( ! GAUSS || dn.kind != 2 ) {                     // ( `GAUSS` false: a constant density, the quadrature compiled out )
         if ( facets ) S.part += dn.rho * ar;
-        if ( ! moments ) { S.mass += dn.rho * vol; return; }
+        if ( ! moments ) { S.mass += dn.rho * vol; return S; }
         const double w = dn.rho * vol;
         const double sx = A.x + g.x + v.x + o.x, sy = A.y + g.y + v.y + o.y, sz = A.z + g.z + v.z + o.z;
         const double sq = A.x * A.x + A.y * A.y + A.z * A.z + g.x * g.x + g.y * g.y + g.z * g.z +
@@ -141,7 +161,7 @@ __device__ __forceinline__ void edge_terms( const Density3 &dn, const double4 &p
         S.mass += w;
         S.mx += w * sx / 4; S.my += w * sy / 4; S.mz += w * sz / 4;
         S.m2 += w * ( sq + ( sx * sx + sy * sy + sz * sz ) ) / 20;
-        return;
+        return S;
     }
     if ( ! moments ) {                                   // the 7-point rule: the facet's density and the mass's flux
         const double s15 = sqrt( 15.0 );
@@ -178,7 +198,7 @@ __device__ __forceinline__ void edge_terms( const Density3 &dn, const double4 &p
                         S.mass += wq * sar * flux_dot( dn, X, Y, Z, u.x, u.y, u.z );
                     }
                 }
-        return;
+        return S;
     }
     // the 14-point rule on the tetrahedron ( A, g, v, o )
     const double r1 = 0.0927352503108912264, r1w = 0.0734930431163619495;
@@ -206,6 +226,7 @@ __device__ __forceinline__ void edge_terms( const Density3 &dn, const double4 &p
         S.mx += w * x; S.my += w * y; S.mz += w * z;
         S.m2 += w * ( x * x + y * y + z * z );
     }
+    return S;
 }
 
 /// the outputs of a measured cell ( `mass` and moments already totalled )
@@ -389,6 +410,82 @@ __device__ __forceinline__ double proximity3( const NodeRec3 &r, double px, doub
     return s;
 }
 
+/// a cell's faces in the wave's LDS ( the first pass ) or in the polytope buffer ( the measure kernel )
+struct LdsFaces {
+    const WaveLds &L;
+    __device__ double x( int f ) const { return L.fx[ f ]; }
+    __device__ double y( int f ) const { return L.fy[ f ]; }
+    __device__ double z( int f ) const { return L.fz[ f ]; }
+    __device__ int id( int f ) const { return L.fid[ f ]; }
+};
+struct BufFaces {
+    const PolyBuf &B;
+    I64 base;
+    __device__ double x( int f ) const { return B.fx[ base + f ]; }
+    __device__ double y( int f ) const { return B.fy[ base + f ]; }
+    __device__ double z( int f ) const { return B.fz[ base + f ]; }
+    __device__ int id( int f ) const { return B.fid[ base + f ]; }
+};
+
+/// the measures of a built cell, one wave ( this lane's vertex in `c` ), written to the outputs
+template<bool GAUSS, class Faces>
+__device__ void measure_cell( const Params3 &P, const Faces &F, const Cell3 &c, int lane, int k0, const double4 &p0, I64 row0 ) {
+    // the measures: face by face ( the faces in cut order, as the oracle lists the facets )
+    const bool act = lane < c.nv;
+    const double inv_nv = c.nv > 0 ? 1.0 / c.nv : 0.0;
+    const double Ax = wave_total( act ? c.x : 0.0 ) * inv_nv, Ay = wave_total( act ? c.y : 0.0 ) * inv_nv, Az = wave_total( act ? c.z : 0.0 ) * inv_nv;
+    double wx[ 3 ], wy[ 3 ], wz[ 3 ];
+    int nk[ 3 ];
+#pragma unroll
+    for ( int k = 0; k < 3; ++k ) {
+        nk[ k ] = byte_of( c.nb, k );
+        wx[ k ] = __shfl( c.x, nk[ k ], 64 ); wy[ k ] = __shfl( c.y, nk[ k ], 64 ); wz[ k ] = __shfl( c.z, nk[ k ], 64 );
+    }
+    const bool facets = P.mode == MODE_FACETS, moments = P.mode == MODE_MOMENTS;
+    const V3 Av{ Ax, Ay, Az };
+    CellSums S;
+    double dia = 0;
+    int q = 0;
+    for ( int f = 0; f < ( c.nv > 0 ? c.nf : 0 ); ++f ) {
+        int j = -1;
+        if ( act ) j = byte_of( c.pl, 0 ) == f ? 0 : byte_of( c.pl, 1 ) == f ? 1 : byte_of( c.pl, 2 ) == f ? 2 : -1;
+        const int cntf = __popcll( __ballot( j >= 0 ) );
+        if ( cntf < 3 ) continue;
+        const V3 g{ wave_total( j >= 0 ? c.x : 0.0 ) / cntf, wave_total( j >= 0 ? c.y : 0.0 ) / cntf, wave_total( j >= 0 ? c.z : 0.0 ) / cntf };
+        const double fnx = F.x( f ), fny = F.y( f ), fnz = F.z( f );
+        const int fid = F.id( f );
+        const double dn = sqrt( fnx * fnx + fny * fny + fnz * fnz );
+        const V3 u{ fnx / dn, fny / dn, fnz / dn };
+        S.part = 0;
+        if ( j >= 0 ) {
+            const V3 v{ c.x, c.y, c.z };
+#pragma unroll
+            for ( int e = 1; e <= 2; ++e ) {             // the two edges of the face at this vertex, each counted by its lower lane
+                const int ks = ( j + e ) % 3;           // ( selects, not an index: no scratch )
+                const int w = ks == 0 ? nk[ 0 ] : ks == 1 ? nk[ 1 ] : nk[ 2 ];
+                if ( ! ( lane < w ) ) continue;
+                const V3 o{ ks == 0 ? wx[ 0 ] : ks == 1 ? wx[ 1 ] : wx[ 2 ], ks == 0 ? wy[ 0 ] : ks == 1 ? wy[ 1 ] : wy[ 2 ],
+                            ks == 0 ? wz[ 0 ] : ks == 1 ? wz[ 1 ] : wz[ 2 ] };
+                const CellSums d = edge_terms<GAUSS>( P.dn, p0, facets, moments, fid < 0, Av, g, v, o, u );
+                S.part += d.part; S.mass += d.mass; S.mx += d.mx; S.my += d.my; S.mz += d.mz; S.m2 += d.m2;
+            }
+        }
+        if ( facets && fid >= 0 ) {
+            const double cf = wave_total( S.part ) / ( 2 * dn );
+            if ( lane == 0 && q < P.room ) {
+                P.out.col[ row0 + I64( q ) * 64 ] = fid;
+                P.out.val[ row0 + I64( q ) * 64 ] = cf;
+            }
+            dia += cf;
+            ++q;
+        }
+    }
+    S.mass = wave_total( S.mass );
+    if ( moments ) { S.mx = wave_total( S.mx ); S.my = wave_total( S.my ); S.mz = wave_total( S.mz ); S.m2 = wave_total( S.m2 ); }
+    if ( lane == 0 ) write_cell( P, k0, p0, S, dia, q, row0 );
+}
+
+template<bool SPLIT>
 __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3( Params3 P ) {
     __shared__ WaveLds lds[ WAVES3 ];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -448,14 +545,19 @@ __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3( Params3 P ) {
         }
         if ( __ballot( may ) == 0 ) continue;
         if ( h <= 1 ) {
-            for ( int k = r.beg; k < r.end; ++k ) {
-                if ( k == k0 ) continue;
-                const double4 q = P.seeds[ k ];
-                const double dx = q.x - p0.x, dy = q.y - p0.y, dz = q.z - p0.z;
-                const double off = ( ( w0 - q.w ) + ( dx * dx + dy * dy + dz * dz ) ) / 2;
-                const int res = cut_cell( L, c, lane, dx, dy, dz, off, k, ev_side, ev_new );
-                if ( res == C3_CUT ) ++ev_cut;
-                if ( res == C3_EMPTY || res == C3_FAILED || res == C3_FULL ) { status = res; break; }
+            // the leaf's seeds in one vector load ( a lane each ), then broadcast in turn
+            for ( int base = r.beg; base < r.end && status == C3_CUT; base += 64 ) {
+                const int m = r.end - base < 64 ? r.end - base : 64;
+                const double4 sq = lane < m ? P.seeds[ base + lane ] : make_double4( 0, 0, 0, 0 );
+                for ( int j = 0; j < m; ++j ) {
+                    const int k = base + j;
+                    if ( k == k0 ) continue;
+                    const double dx = readlane_d( sq.x, j ) - p0.x, dy = readlane_d( sq.y, j ) - p0.y, dz = readlane_d( sq.z, j ) - p0.z;
+                    const double off = ( ( w0 - readlane_d( sq.w, j ) ) + ( dx * dx + dy * dy + dz * dz ) ) / 2;
+                    const int res = cut_cell( L, c, lane, dx, dy, dz, off, k, ev_side, ev_new );
+                    if ( res == C3_CUT ) ++ev_cut;
+                    if ( res == C3_EMPTY || res == C3_FAILED || res == C3_FULL ) { status = res; break; }
+                }
             }
             continue;
         }
@@ -479,76 +581,141 @@ __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3( Params3 P ) {
         atomicAdd( &P.out.events[ 4 ], ev_cut );
     }
     const I64 row0 = I64( k0 / 64 ) * 64 * P.room + ( k0 & 63 );
-    if ( status == C3_FAILED ) {
-        if ( lane == 0 ) record_failure( P, k0, row0, 2 );
-        return;
-    }
-    if ( status == C3_FULL ) {                           // past the lanes: the second pass, one thread in global slots
-        if ( lane == 0 ) P.out.ovf_list[ atomicAdd( &P.out.counters[ 1 ], 1u ) ] = k0;
+    if ( status == C3_FAILED || status == C3_FULL ) {
+        if ( lane == 0 && SPLIT ) P.pb.nv[ k0 ] = -1;
+        if ( lane == 0 && status == C3_FAILED ) record_failure( P, k0, row0, 2 );
+        if ( lane == 0 && status == C3_FULL ) P.out.ovf_list[ atomicAdd( &P.out.counters[ 1 ], 1u ) ] = k0;   // the second pass
         return;
     }
     if ( status == C3_EMPTY ) c.nv = 0;
 
-    // ---- the measures: face by face ( the faces in cut order, as the oracle lists the facets )
-    const bool act = lane < c.nv;
-    const double inv_nv = c.nv > 0 ? 1.0 / c.nv : 0.0;
-    const double Ax = wave_total( act ? c.x : 0.0 ) * inv_nv, Ay = wave_total( act ? c.y : 0.0 ) * inv_nv, Az = wave_total( act ? c.z : 0.0 ) * inv_nv;
-    double wx[ 3 ], wy[ 3 ], wz[ 3 ];
-    int nk[ 3 ];
-#pragma unroll
-    for ( int k = 0; k < 3; ++k ) {
-        nk[ k ] = byte_of( c.nb, k );
-        wx[ k ] = __shfl( c.x, nk[ k ], 64 ); wy[ k ] = __shfl( c.y, nk[ k ], 64 ); wz[ k ] = __shfl( c.z, nk[ k ], 64 );
+    if ( SPLIT ) {                                       // ( the Gaussians' measures: the next kernel )
+        compact_faces( L, c, lane );
+        const I64 at = I64( k0 ) * 64 + lane;
+        if ( lane < c.nv ) { P.pb.vx[ at ] = c.x; P.pb.vy[ at ] = c.y; P.pb.vz[ at ] = c.z; P.pb.vpl[ at ] = c.pl; P.pb.vnb[ at ] = c.nb; }
+        if ( lane < c.nf ) { P.pb.fx[ at ] = L.fx[ lane ]; P.pb.fy[ at ] = L.fy[ lane ]; P.pb.fz[ at ] = L.fz[ lane ]; P.pb.fid[ at ] = L.fid[ lane ]; }
+        if ( lane == 0 ) { P.pb.nv[ k0 ] = status == C3_EMPTY ? 0 : c.nv; P.pb.nf[ k0 ] = c.nf; }
+        return;
     }
+    measure_cell<false>( P, LdsFaces{ L }, c, lane, k0, p0, row0 );
+}
+
+/// what a wave of the measure kernel keeps in LDS: the cell's vertices, its faces ( mean vertex, unit normal, the norm of
+/// the plane's normal, id, validity ), its triangles ( face, two vertices ) with their facet parts
+constexpr int MTRI = 6 * CELL3_VERTICES / 2 + 64;
+struct MeasLds {
+    double px[ CELL3_VERTICES ], py[ CELL3_VERTICES ], pz[ CELL3_VERTICES ];
+    int    pl[ CELL3_VERTICES ];
+    double gx[ CELL3_FACES ], gy[ CELL3_FACES ], gz[ CELL3_FACES ], ux[ CELL3_FACES ], uy[ CELL3_FACES ], uz[ CELL3_FACES ], dn[ CELL3_FACES ];
+    int    fid[ CELL3_FACES ], fok[ CELL3_FACES ];
+    int    tf[ MTRI ], tv[ MTRI ], tw[ MTRI ];
+    double tpart[ MTRI ];
+};
+
+/// the second kernel of a sum of Gaussians: the cells the first pass kept, one wave each. The triangles of the cell ( a
+/// face's mean vertex and one of its edges, `3 nv` of them ) are spread over the lanes, each face's sums taken in a fixed
+/// order: the same terms as the face-by-face loop of the first pass, with every lane at work
+__global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3_measure( Params3 P ) {
+    __shared__ MeasLds lds[ WAVES3 ];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    MeasLds &M = lds[ wid ];
+    const int k0 = int( blockIdx.x ) * WAVES3 + wid;
+    if ( k0 >= P.n ) return;
+    const int nv = P.pb.nv[ k0 ];
+    if ( nv < 0 ) return;                                // ( measured by the second pass, or failed )
+    const int nf = nv > 0 ? P.pb.nf[ k0 ] : 0;
+    const I64 at = I64( k0 ) * 64 + lane;
+    const double4 p0 = P.seeds[ k0 ];
+    const bool act = lane < nv;
+    double x = 0, y = 0, z = 0;
+    int pl = 0, nb = 0;
+    if ( act ) { x = P.pb.vx[ at ]; y = P.pb.vy[ at ]; z = P.pb.vz[ at ]; pl = P.pb.vpl[ at ]; nb = P.pb.vnb[ at ]; M.px[ lane ] = x; M.py[ lane ] = y; M.pz[ lane ] = z; M.pl[ lane ] = pl; }
+    const double inv_nv = nv > 0 ? 1.0 / nv : 0.0;
+    const V3 Av{ wave_total( x ) * inv_nv, wave_total( y ) * inv_nv, wave_total( z ) * inv_nv };
+    wave_sync();
+    // the faces, a lane each: the mean of their vertices ( in vertex order ), the unit normal
+    if ( lane < nf ) {
+        double sx = 0, sy = 0, sz = 0;
+        int cnt = 0;
+        for ( int v = 0; v < nv; ++v ) {
+            const int q = M.pl[ v ];
+            if ( byte_of( q, 0 ) == lane || byte_of( q, 1 ) == lane || byte_of( q, 2 ) == lane ) { sx += M.px[ v ]; sy += M.py[ v ]; sz += M.pz[ v ]; ++cnt; }
+        }
+        const double fx = P.pb.fx[ at ], fy = P.pb.fy[ at ], fz = P.pb.fz[ at ];
+        const double d = sqrt( fx * fx + fy * fy + fz * fz );
+        M.gx[ lane ] = cnt ? sx / cnt : 0; M.gy[ lane ] = cnt ? sy / cnt : 0; M.gz[ lane ] = cnt ? sz / cnt : 0;
+        M.ux[ lane ] = fx / d; M.uy[ lane ] = fy / d; M.uz[ lane ] = fz / d; M.dn[ lane ] = d;
+        M.fid[ lane ] = P.pb.fid[ at ];
+        M.fok[ lane ] = cnt >= 3;
+    }
+    wave_sync();
+    // the triangles: each edge of a face counted by its lower vertex
+    int cnt = 0;
+    if ( act )
+        for ( int j = 0; j < 3; ++j ) {
+            const int f = byte_of( pl, j );
+            if ( ! M.fok[ f ] ) continue;
+            cnt += int( lane < byte_of( nb, ( j + 1 ) % 3 ) ) + int( lane < byte_of( nb, ( j + 2 ) % 3 ) );
+        }
+    int inc = cnt;
+    for ( int d = 1; d < 64; d <<= 1 ) {
+        const int t = __shfl_up( inc, d, 64 );
+        if ( lane >= d ) inc += t;
+    }
+    const int T = __shfl( inc, 63, 64 );
+    if ( act ) {
+        int t = inc - cnt;
+        for ( int j = 0; j < 3; ++j ) {
+            const int f = byte_of( pl, j );
+            if ( ! M.fok[ f ] ) continue;
+            for ( int e = 1; e <= 2; ++e ) {
+                const int w = byte_of( nb, ( j + e ) % 3 );
+                if ( lane < w ) { M.tf[ t ] = f; M.tv[ t ] = lane; M.tw[ t ] = w; ++t; }
+            }
+        }
+    }
+    wave_sync();
     const bool facets = P.mode == MODE_FACETS, moments = P.mode == MODE_MOMENTS;
-    const V3 Av{ Ax, Ay, Az };
     CellSums S;
-    double dia = 0;
-    int q = 0;
-    for ( int f = 0; f < ( c.nv > 0 ? c.nf : 0 ); ++f ) {
-        int j = -1;
-        if ( act ) j = byte_of( c.pl, 0 ) == f ? 0 : byte_of( c.pl, 1 ) == f ? 1 : byte_of( c.pl, 2 ) == f ? 2 : -1;
-        const int cntf = __popcll( __ballot( j >= 0 ) );
-        if ( cntf < 3 ) continue;
-        const V3 g{ wave_total( j >= 0 ? c.x : 0.0 ) / cntf, wave_total( j >= 0 ? c.y : 0.0 ) / cntf, wave_total( j >= 0 ? c.z : 0.0 ) / cntf };
-        const double fnx = L.fx[ f ], fny = L.fy[ f ], fnz = L.fz[ f ];
-        const int fid = L.fid[ f ];
-        const double dn = sqrt( fnx * fnx + fny * fny + fnz * fnz );
-        const V3 u{ fnx / dn, fny / dn, fnz / dn };
-        S.part = 0;
-        if ( j >= 0 ) {
-            const V3 v{ c.x, c.y, c.z };
-#pragma unroll
-            for ( int e = 1; e <= 2; ++e ) {             // the two edges of the face at this vertex, each counted by its lower lane
-                const int ks = ( j + e ) % 3;           // ( selects, not an index: no scratch )
-                const int w = ks == 0 ? nk[ 0 ] : ks == 1 ? nk[ 1 ] : nk[ 2 ];
-                if ( ! ( lane < w ) ) continue;
-                const V3 o{ ks == 0 ? wx[ 0 ] : ks == 1 ? wx[ 1 ] : wx[ 2 ], ks == 0 ? wy[ 0 ] : ks == 1 ? wy[ 1 ] : wy[ 2 ],
-                            ks == 0 ? wz[ 0 ] : ks == 1 ? wz[ 1 ] : wz[ 2 ] };
-                edge_terms( P.dn, p0, facets, moments, fid < 0, Av, g, v, o, u, S );
-            }
-        }
-        if ( facets && fid >= 0 ) {
-            const double cf = wave_total( S.part ) / ( 2 * dn );
-            if ( lane == 0 && q < P.room ) {
-                P.out.col[ row0 + I64( q ) * 64 ] = fid;
-                P.out.val[ row0 + I64( q ) * 64 ] = cf;
-            }
-            dia += cf;
-            ++q;
-        }
+    for ( int t = lane; t < T; t += 64 ) {
+        const int f = M.tf[ t ], v = M.tv[ t ], w = M.tw[ t ];
+        const CellSums d = edge_terms<true>( P.dn, p0, facets, moments, M.fid[ f ] < 0, Av, V3{ M.gx[ f ], M.gy[ f ], M.gz[ f ] },
+                                             V3{ M.px[ v ], M.py[ v ], M.pz[ v ] }, V3{ M.px[ w ], M.py[ w ], M.pz[ w ] },
+                                             V3{ M.ux[ f ], M.uy[ f ], M.uz[ f ] } );
+        M.tpart[ t ] = d.part;
+        S.mass += d.mass; S.mx += d.mx; S.my += d.my; S.mz += d.mz; S.m2 += d.m2;
     }
+    wave_sync();
+    // the facets: a face's parts summed in triangle order by its lane, the rows' entries in face order
+    const I64 row0 = I64( k0 / 64 ) * 64 * P.room + ( k0 & 63 );
+    double cf = 0;
+    bool has = false;
+    if ( facets && lane < nf && M.fok[ lane ] && M.fid[ lane ] >= 0 ) {
+        double sp = 0;
+        for ( int t = 0; t < T; ++t ) if ( M.tf[ t ] == lane ) sp += M.tpart[ t ];
+        cf = sp / ( 2 * M.dn[ lane ] );
+        has = true;
+    }
+    const unsigned long long mh = __ballot( has );
+    const int q = __popcll( mh & ( ( 1ull << lane ) - 1 ) ), nq = __popcll( mh );
+    if ( has && q < P.room ) {
+        P.out.col[ row0 + I64( q ) * 64 ] = M.fid[ lane ];
+        P.out.val[ row0 + I64( q ) * 64 ] = cf;
+    }
+    const double dia = wave_total( cf );
     S.mass = wave_total( S.mass );
     if ( moments ) { S.mx = wave_total( S.mx ); S.my = wave_total( S.my ); S.mz = wave_total( S.mz ); S.m2 = wave_total( S.m2 ); }
-    if ( lane == 0 ) write_cell( P, k0, p0, S, dia, q, row0 );
+    if ( lane == 0 ) write_cell( P, k0, p0, S, dia, nq, row0 );
 }
 
 // ---- the second pass: cells past the lanes, one thread each in global slots ----------------------------------------
 
 constexpr int SLOT_V = CELL3_SLOT_VERTICES, SLOT_F = CELL3_SLOT_FACES;   ///< a slot's vertices and faces
-constexpr int PASS2_THREADS = 2048;
+constexpr int PASS2_BLOCKS3 = 1024;                      ///< waves of the second pass ( each takes cells in turn )
+constexpr int PASS2_VISITS = 16384;                      ///< the nodes a second-pass cell may visit ( ~100 for a typical cell )
 
-/// one thread's slot: its cell ( vertices with their faces and neighbours, faces ) and a cut's staging
+/// an overflowed cell in the LDS of its wave: vertices with their faces and neighbours, a cut's staging, the faces, the
+/// walk's stack; after the walk the staging holds the measures' faces and triangles
 struct Slot3 {
     double x[ SLOT_V ], y[ SLOT_V ], z[ SLOT_V ], s[ SLOT_V ];
     int    pl[ 3 * SLOT_V ], nb[ 3 * SLOT_V ];
@@ -557,21 +724,16 @@ struct Slot3 {
     double fx[ SLOT_F ], fy[ SLOT_F ], fz[ SLOT_F ], fo[ SLOT_F ];
     int    fid[ SLOT_F ];
     int    stk_m[ STACK3 ], stk_h[ STACK3 ];
-    int    nv, nf;
+    int    tf[ 3 * SLOT_V ], tv[ 3 * SLOT_V ], tw[ 3 * SLOT_V ];   ///< the measures' triangles ( face, vertices ), `3 nv` at most
+    double tp[ 3 * SLOT_V ];                                       ///< their facet parts
+    int    nv, nf, res;
 };
 
-/// the cut of `cut_cell`, one thread: the same classification, interpolation and numbering ( the oracle's `cut3` )
-__device__ int cut_slot( Slot3 &C, double nx, double ny, double nz, double off, int id, unsigned long long &ev_side, unsigned long long &ev_new ) {
+/// the rest of a cut once the signs `s` are known and one is positive ( lane 0 alone, on the slot in LDS ): the
+/// classification with the tie bound, the inside vertices renumbered, a new vertex per crossing edge, the new face
+/// closed -- the oracle's `cut3` step by step
+__device__ int cut_slot_rest( Slot3 &C, double nx, double ny, double nz, double off, int id, double vm, unsigned long long &ev_new ) {
     const int nv = C.nv;
-    ev_side += unsigned( nv );
-    bool any = false;
-    double vm = 0;
-    for ( int v = 0; v < nv; ++v ) {
-        C.s[ v ] = fma( nx, C.x[ v ], fma( ny, C.y[ v ], fma( nz, C.z[ v ], -off ) ) );
-        any |= C.s[ v ] > 0;
-        vm = fmax( vm, fmax( fabs( C.x[ v ] ), fmax( fabs( C.y[ v ] ), fabs( C.z[ v ] ) ) ) );
-    }
-    if ( ! any ) return C3_UNCHANGED;
     const double tie = 1e-13 * ( ( fabs( nx ) + fabs( ny ) + fabs( nz ) ) * vm + fabs( off ) );
     int nin = 0;
     for ( int v = 0; v < nv; ++v ) {
@@ -634,24 +796,48 @@ __device__ int cut_slot( Slot3 &C, double nx, double ny, double nz, double off, 
     return C3_CUT;
 }
 
-__global__ __launch_bounds__( 64 ) void k_cells3_pass2( Params3 P, Slot3 *slots ) {
-    const int tid = int( blockIdx.x ) * 64 + threadIdx.x;
-    Slot3 &C = slots[ tid ];
+/// the cut of an overflowed cell by its wave: the signs by every lane, the rest by lane 0 when a vertex is outside
+__device__ int cut_slot( Slot3 &C, int lane, double nx, double ny, double nz, double off, int id, unsigned long long &ev_side,
+                         unsigned long long &ev_new ) {
+    const int nv = C.nv;
+    ev_side += unsigned( nv );
+    bool any = false;
+    double vm = 0;
+    for ( int v = lane; v < nv; v += 64 ) {
+        const double sv = fma( nx, C.x[ v ], fma( ny, C.y[ v ], fma( nz, C.z[ v ], -off ) ) );
+        C.s[ v ] = sv;
+        any |= sv > 0;
+        vm = fmax( vm, fmax( fabs( C.x[ v ] ), fmax( fabs( C.y[ v ] ), fabs( C.z[ v ] ) ) ) );
+    }
+    if ( __ballot( any ) == 0 ) return C3_UNCHANGED;
+    vm = wave_maxd( vm );
+    __syncthreads();                                     // ( the block is the wave )
+    if ( lane == 0 ) C.res = cut_slot_rest( C, nx, ny, nz, off, id, vm, ev_new );
+    __syncthreads();
+    return C.res;
+}
+
+/// the second pass: the cells past the lanes of the first, one wave per cell with the cell in LDS ( up to `SLOT_V`
+/// vertices ); a cell past that, or one whose walk visits more than `PASS2_VISITS` nodes, is a failure ( code 1 )
+__global__ __launch_bounds__( 64 ) void k_cells3_pass2( Params3 P ) {
+    __shared__ Slot3 C;
+    const int lane = threadIdx.x;
     const int count = int( P.out.counters[ 1 ] );
-    for ( int it = tid; it < count; it += PASS2_THREADS ) {
+    for ( int it = blockIdx.x; it < count; it += PASS2_BLOCKS3 ) {
         const int k0 = P.out.ovf_list[ it ];
         const double4 p0 = P.seeds[ k0 ];
         const double w0 = p0.w;
         unsigned long long ev_side = 0, ev_new = 0;
-        C.nv = 8; C.nf = 6;
-        for ( int v = 0; v < 8; ++v ) {
+        __syncthreads();
+        if ( lane < 8 ) {
+            const int v = lane;
             C.x[ v ] = ( v & 1 ? P.hi[ 0 ] : P.lo[ 0 ] ) - p0.x;
             C.y[ v ] = ( v & 2 ? P.hi[ 1 ] : P.lo[ 1 ] ) - p0.y;
             C.z[ v ] = ( v & 4 ? P.hi[ 2 ] : P.lo[ 2 ] ) - p0.z;
             for ( int d = 0; d < 3; ++d ) { C.pl[ 3 * v + d ] = 2 * d + ( ( v >> d ) & 1 ); C.nb[ 3 * v + d ] = v ^ ( 1 << d ); }
         }
-        for ( int f = 0; f < 6; ++f ) {
-            const int d = f >> 1, hi = f & 1;
+        if ( lane < 6 ) {
+            const int f = lane, d = f >> 1, hi = f & 1;
             const double pc = d == 0 ? p0.x : d == 1 ? p0.y : p0.z;
             C.fx[ f ] = d == 0 ? ( hi ? 1.0 : -1.0 ) : 0.0;
             C.fy[ f ] = d == 1 ? ( hi ? 1.0 : -1.0 ) : 0.0;
@@ -659,17 +845,21 @@ __global__ __launch_bounds__( 64 ) void k_cells3_pass2( Params3 P, Slot3 *slots 
             C.fo[ f ] = hi ? P.hi[ d ] - pc : pc - P.lo[ d ];
             C.fid[ f ] = hi ? -2 - 2 * d : -1 - 2 * d;
         }
-        int status = C3_CUT, top = 1;
-        C.stk_m[ 0 ] = 0; C.stk_h[ 0 ] = P.depth;
+        if ( lane == 0 ) { C.nv = 8; C.nf = 6; C.stk_m[ 0 ] = 0; C.stk_h[ 0 ] = P.depth; }
+        __syncthreads();
+        int status = C3_CUT, top = 1, visits = 0;
         while ( top > 0 && status == C3_CUT ) {
             --top;
             const int m = C.stk_m[ top ], h = C.stk_h[ top ];
             const NodeRec3 r = P.nodes[ m ];
             if ( r.beg >= r.end ) continue;
+            // a cell that spans so much of the tree comes from weights no step keeps ( a trial's ): given up as too large
+            if ( ++visits > PASS2_VISITS ) { status = C3_FULL; break; }
             bool may = false;
-            for ( int v = 0; v < C.nv && ! may; ++v ) {
+            for ( int v = lane; v < C.nv && ! may; v += 64 ) {
                 const double vv[ 3 ] = { C.x[ v ], C.y[ v ], C.z[ v ] }, pp[ 3 ] = { p0.x, p0.y, p0.z };
                 double sv = w0 - r.wb;
+#pragma unroll
                 for ( int d = 0; d < 3; ++d ) {
                     const double X = vv[ d ] + pp[ d ];
                     double y = X + r.wa[ d ] / 2;
@@ -679,14 +869,14 @@ __global__ __launch_bounds__( 64 ) void k_cells3_pass2( Params3 P, Slot3 *slots 
                 }
                 may = sv <= 0;
             }
-            if ( ! may ) continue;
+            if ( __ballot( may ) == 0 ) continue;
             if ( h <= 1 ) {
                 for ( int k = r.beg; k < r.end; ++k ) {
                     if ( k == k0 ) continue;
                     const double4 q = P.seeds[ k ];
                     const double dx = q.x - p0.x, dy = q.y - p0.y, dz = q.z - p0.z;
                     const double off = ( ( w0 - q.w ) + ( dx * dx + dy * dy + dz * dz ) ) / 2;
-                    const int res = cut_slot( C, dx, dy, dz, off, k, ev_side, ev_new );
+                    const int res = cut_slot( C, lane, dx, dy, dz, off, k, ev_side, ev_new );
                     if ( res == C3_EMPTY || res == C3_FAILED || res == C3_FULL ) { status = res; break; }
                 }
                 continue;
@@ -694,56 +884,105 @@ __global__ __launch_bounds__( 64 ) void k_cells3_pass2( Params3 P, Slot3 *slots 
             const int lc = m + 1, rc = m + ( 1 << ( h - 1 ) );
             const double dl = proximity3( P.nodes[ lc ], p0.x, p0.y, p0.z ), dr = proximity3( P.nodes[ rc ], p0.x, p0.y, p0.z );
             if ( top + 2 > STACK3 ) { status = C3_FAILED; break; }
-            C.stk_m[ top ] = dl <= dr ? rc : lc; C.stk_h[ top ] = h - 1;
-            C.stk_m[ top + 1 ] = dl <= dr ? lc : rc; C.stk_h[ top + 1 ] = h - 1;
+            __syncthreads();
+            if ( lane == 0 ) {
+                C.stk_m[ top ] = dl <= dr ? rc : lc; C.stk_h[ top ] = h - 1;
+                C.stk_m[ top + 1 ] = dl <= dr ? lc : rc; C.stk_h[ top + 1 ] = h - 1;
+            }
             top += 2;
+            __syncthreads();
         }
-        atomicAdd( &P.out.events[ 0 ], ev_side );
-        atomicAdd( &P.out.events[ 2 ], ev_new );
+        if ( lane == 0 ) { atomicAdd( &P.out.events[ 0 ], ev_side ); atomicAdd( &P.out.events[ 2 ], ev_new ); }
         const I64 row0 = I64( k0 / 64 ) * 64 * P.room + ( k0 & 63 );
-        if ( status == C3_FAILED || status == C3_FULL ) { record_failure( P, k0, row0, status == C3_FULL ? 1 : 2 ); continue; }
-        if ( status == C3_EMPTY ) C.nv = 0;
-        // the measures, face by face as the first pass
-        const bool facets = P.mode == MODE_FACETS, moments = P.mode == MODE_MOMENTS;
-        V3 Av{ 0, 0, 0 };
-        for ( int v = 0; v < C.nv; ++v ) { Av.x += C.x[ v ]; Av.y += C.y[ v ]; Av.z += C.z[ v ]; }
-        const double inv_nv = C.nv > 0 ? 1.0 / C.nv : 0.0;
-        Av.x *= inv_nv; Av.y *= inv_nv; Av.z *= inv_nv;
-        CellSums S;
-        double dia = 0;
-        int q = 0;
-        for ( int f = 0; f < ( C.nv > 0 ? C.nf : 0 ); ++f ) {
-            V3 g{ 0, 0, 0 };
-            int cntf = 0;
-            for ( int v = 0; v < C.nv; ++v )
-                if ( C.pl[ 3 * v ] == f || C.pl[ 3 * v + 1 ] == f || C.pl[ 3 * v + 2 ] == f ) { g.x += C.x[ v ]; g.y += C.y[ v ]; g.z += C.z[ v ]; ++cntf; }
-            if ( cntf < 3 ) continue;
-            g.x /= cntf; g.y /= cntf; g.z /= cntf;
-            const double dn = sqrt( C.fx[ f ] * C.fx[ f ] + C.fy[ f ] * C.fy[ f ] + C.fz[ f ] * C.fz[ f ] );
-            const V3 u{ C.fx[ f ] / dn, C.fy[ f ] / dn, C.fz[ f ] / dn };
-            S.part = 0;
-            for ( int v = 0; v < C.nv; ++v ) {
-                const int j = C.pl[ 3 * v ] == f ? 0 : C.pl[ 3 * v + 1 ] == f ? 1 : C.pl[ 3 * v + 2 ] == f ? 2 : -1;
-                if ( j < 0 ) continue;
-                const V3 vv{ C.x[ v ], C.y[ v ], C.z[ v ] };
-                for ( int e = 1; e <= 2; ++e ) {
-                    const int w = C.nb[ 3 * v + ( j + e ) % 3 ];
-                    if ( ! ( v < w ) ) continue;
-                    const V3 o{ C.x[ w ], C.y[ w ], C.z[ w ] };
-                    edge_terms( P.dn, p0, facets, moments, C.fid[ f ] < 0, Av, g, vv, o, u, S );
-                }
-            }
-            if ( facets && C.fid[ f ] >= 0 ) {
-                const double cf = S.part / ( 2 * dn );
-                if ( q < P.room ) {
-                    P.out.col[ row0 + I64( q ) * 64 ] = C.fid[ f ];
-                    P.out.val[ row0 + I64( q ) * 64 ] = cf;
-                }
-                dia += cf;
-                ++q;
-            }
+        if ( status == C3_FAILED || status == C3_FULL ) {
+            if ( lane == 0 ) record_failure( P, k0, row0, status == C3_FULL ? 1 : 2 );
+            continue;
         }
-        write_cell( P, k0, p0, S, dia, q, row0 );
+        const int nv = status == C3_EMPTY ? 0 : C.nv, nf = nv > 0 ? C.nf : 0;
+        // ---- the measures, the triangles spread over the lanes as in the measure kernel ( staging arrays reused: face
+        // means in X, Y, Z, unit normals in fx, fy, fz after the norms in s, validity in idx )
+        double ax = 0, ay = 0, az = 0;
+        for ( int v = lane; v < nv; v += 64 ) { ax += C.x[ v ]; ay += C.y[ v ]; az += C.z[ v ]; }
+        const double inv_nv = nv > 0 ? 1.0 / nv : 0.0;
+        const V3 Av{ wave_total( ax ) * inv_nv, wave_total( ay ) * inv_nv, wave_total( az ) * inv_nv };
+        __syncthreads();
+        for ( int f = lane; f < nf; f += 64 ) {
+            double sx = 0, sy = 0, sz = 0;
+            int cnt = 0;
+            for ( int v = 0; v < nv; ++v )
+                if ( C.pl[ 3 * v ] == f || C.pl[ 3 * v + 1 ] == f || C.pl[ 3 * v + 2 ] == f ) { sx += C.x[ v ]; sy += C.y[ v ]; sz += C.z[ v ]; ++cnt; }
+            C.X[ f ] = cnt ? sx / cnt : 0; C.Y[ f ] = cnt ? sy / cnt : 0; C.Z[ f ] = cnt ? sz / cnt : 0;
+            const double d = sqrt( C.fx[ f ] * C.fx[ f ] + C.fy[ f ] * C.fy[ f ] + C.fz[ f ] * C.fz[ f ] );
+            C.s[ f ] = d;
+            C.idx[ f ] = cnt >= 3;
+        }
+        __syncthreads();
+        for ( int f = lane; f < nf; f += 64 ) { C.fx[ f ] /= C.s[ f ]; C.fy[ f ] /= C.s[ f ]; C.fz[ f ] /= C.s[ f ]; }
+        // the triangles, each edge of a face counted by its lower vertex, in vertex order
+        int T = 0;
+        for ( int v0 = 0; v0 < nv; v0 += 64 ) {
+            const int v = v0 + lane;
+            int cnt = 0;
+            if ( v < nv )
+                for ( int j = 0; j < 3; ++j ) {
+                    if ( ! C.idx[ C.pl[ 3 * v + j ] ] ) continue;
+                    cnt += int( v < C.nb[ 3 * v + ( j + 1 ) % 3 ] ) + int( v < C.nb[ 3 * v + ( j + 2 ) % 3 ] );
+                }
+            int inc = cnt;
+            for ( int d = 1; d < 64; d <<= 1 ) {
+                const int t = __shfl_up( inc, d, 64 );
+                if ( lane >= d ) inc += t;
+            }
+            if ( v < nv ) {
+                int t = T + inc - cnt;
+                for ( int j = 0; j < 3; ++j ) {
+                    const int f = C.pl[ 3 * v + j ];
+                    if ( ! C.idx[ f ] ) continue;
+                    for ( int e = 1; e <= 2; ++e ) {
+                        const int w = C.nb[ 3 * v + ( j + e ) % 3 ];
+                        if ( v < w ) { C.tf[ t ] = f; C.tv[ t ] = v; C.tw[ t ] = w; ++t; }
+                    }
+                }
+            }
+            T += __shfl( inc, 63, 64 );
+        }
+        __syncthreads();
+        const bool facets = P.mode == MODE_FACETS, moments = P.mode == MODE_MOMENTS;
+        CellSums S;
+        for ( int t = lane; t < T; t += 64 ) {
+            const int f = C.tf[ t ], v = C.tv[ t ], w = C.tw[ t ];
+            const CellSums d = edge_terms<true>( P.dn, p0, facets, moments, C.fid[ f ] < 0, Av, V3{ C.X[ f ], C.Y[ f ], C.Z[ f ] },
+                                                 V3{ C.x[ v ], C.y[ v ], C.z[ v ] }, V3{ C.x[ w ], C.y[ w ], C.z[ w ] }, V3{ C.fx[ f ], C.fy[ f ], C.fz[ f ] } );
+            S.mass += d.mass; S.mx += d.mx; S.my += d.my; S.mz += d.mz; S.m2 += d.m2;
+            C.tp[ t ] = d.part;
+        }
+        __syncthreads();
+        // the facets in face order: a face's parts summed in triangle order by its lane
+        double dsum = 0;
+        int qbase = 0;
+        for ( int f0 = 0; f0 < nf; f0 += 64 ) {
+            const int f = f0 + lane;
+            double cf = 0;
+            bool has = false;
+            if ( facets && f < nf && C.idx[ f ] && C.fid[ f ] >= 0 ) {
+                double sp = 0;
+                for ( int t = 0; t < T; ++t ) if ( C.tf[ t ] == f ) sp += C.tp[ t ];
+                cf = sp / ( 2 * C.s[ f ] );
+                has = true;
+            }
+            const unsigned long long mh = __ballot( has );
+            const int q = qbase + __popcll( mh & ( ( 1ull << lane ) - 1 ) );
+            if ( has && q < P.room ) {
+                P.out.col[ row0 + I64( q ) * 64 ] = C.fid[ f ];
+                P.out.val[ row0 + I64( q ) * 64 ] = cf;
+            }
+            dsum += cf;
+            qbase += __popcll( mh );
+        }
+        const double dia = wave_total( dsum );
+        S.mass = wave_total( S.mass );
+        if ( moments ) { S.mx = wave_total( S.mx ); S.my = wave_total( S.my ); S.mz = wave_total( S.mz ); S.m2 = wave_total( S.m2 ); }
+        if ( lane == 0 ) write_cell( P, k0, p0, S, dia, qbase, row0 );
     }
 }
 
@@ -771,11 +1010,25 @@ void run_cells_3d( const DevTree3 &t, const double *lo, const double *hi, const 
     check_launch( "k_reset_counters3" );
     Timing::Pending tp;
     const bool timed = timing().begin( TK_CELLS, st, tp );
-    k_cells3<<<blocks_for( t.n, WAVES3 ), 64 * WAVES3, 0, st>>>( P );
-    check_launch( "k_cells3" );
+    const bool split = dn.kind == 2;                     // ( a sum of Gaussians: the measures in a kernel of their own )
+    if ( split ) {
+        Arena &A = arena();
+        const size_t m = size_t( t.n ) * 64;
+        P.pb.vx = A.get<double>( "cells3.vx", m ); P.pb.vy = A.get<double>( "cells3.vy", m ); P.pb.vz = A.get<double>( "cells3.vz", m );
+        P.pb.vpl = A.get<int>( "cells3.vpl", m ); P.pb.vnb = A.get<int>( "cells3.vnb", m );
+        P.pb.fx = A.get<double>( "cells3.fx", m ); P.pb.fy = A.get<double>( "cells3.fy", m ); P.pb.fz = A.get<double>( "cells3.fz", m );
+        P.pb.fid = A.get<int>( "cells3.fid", m );
+        P.pb.nv = A.get<int>( "cells3.nv", size_t( t.n ) ); P.pb.nf = A.get<int>( "cells3.nf", size_t( t.n ) );
+        k_cells3<true><<<blocks_for( t.n, WAVES3 ), 64 * WAVES3, 0, st>>>( P );
+        check_launch( "k_cells3" );
+        k_cells3_measure<<<blocks_for( t.n, WAVES3 ), 64 * WAVES3, 0, st>>>( P );
+        check_launch( "k_cells3_measure" );
+    } else {
+        k_cells3<false><<<blocks_for( t.n, WAVES3 ), 64 * WAVES3, 0, st>>>( P );
+        check_launch( "k_cells3" );
+    }
     if ( timed ) timing().end( tp, st );
-    Slot3 *slots = reinterpret_cast<Slot3 *>( arena().get<char>( "cells3.slots", sizeof( Slot3 ) * size_t( PASS2_THREADS ) ) );
-    k_cells3_pass2<<<PASS2_THREADS / 64, 64, 0, st>>>( P, slots );
+    k_cells3_pass2<<<PASS2_BLOCKS3, 64, 0, st>>>( P );
     check_launch( "k_cells3_pass2" );
 }
 
