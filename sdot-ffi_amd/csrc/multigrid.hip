@@ -37,7 +37,7 @@ constexpr int MAX_LEVELS = 24;
 constexpr int DENSE_MAX = 64;      ///< the coarsest level, solved by a dense inverse
 
 /// the outer CG's device state
-enum { S_PQ = 0, S_PR, S_ALPHA, S_BETA, S_ZQ, S_RR, S_NB, S_ERR, S_DONE, S_IT, S_TOL, S_FAILED, S_BAD, S_NB_STATE };
+enum { S_PQ = 0, S_PR, S_ALPHA, S_BETA, S_ZQ, S_RR, S_NB, S_ERR, S_DONE, S_IT, S_TOL, S_FAILED, S_BAD, S_TRUE, S_NB_STATE };
 /// a K-cycle's partial sums, per level: `rho1 = c1 . v1`, `alpha1 = c1 . r`, `beta = c2 . v2`, `gamma = c2 . v1`, `alpha2 = c2 . r2`
 enum { K_RHO1 = 0, K_ALPHA1, K_BETA2, K_GAMMA, K_ALPHA2, K_NB };
 
@@ -491,6 +491,24 @@ __global__ __launch_bounds__( RED_THREADS ) void k_pcg_init( Mat A, const double
     write_partial<RED_THREADS>( acc, part, blockIdx.x );
 }
 
+/// partial sums of `| b - A d |^2`: the true residual, which the flexible CG's recurrence only estimates
+__global__ __launch_bounds__( RED_THREADS ) void k_true_res( Mat A, const double *b, const double *d, double *part ) {
+    double acc = 0;
+    for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < A.n; i += I64( RED_BLOCKS ) * RED_THREADS ) {
+        const double e = b[ i ] - row_times( A, i, d );
+        acc += e * e;
+    }
+    write_partial<RED_THREADS>( acc, part, blockIdx.x );
+}
+
+__global__ __launch_bounds__( RED_THREADS ) void k_true_res_final( const double *part, double *state ) {
+    __shared__ double sh[ RED_THREADS ];
+    double a = 0;
+    for ( int j = threadIdx.x; j < RED_BLOCKS; j += RED_THREADS ) a += part[ j ];
+    const double ee = block_sum<RED_THREADS>( a, sh );
+    if ( threadIdx.x == 0 ) state[ S_TRUE ] = state[ S_NB ] > 0 ? sqrt( ee ) / state[ S_NB ] : 0.0;
+}
+
 __global__ __launch_bounds__( RED_THREADS ) void k_pcg_init_final( const double *part, double *state, double tol ) {
     __shared__ double sh[ RED_THREADS ];
     double a = 0;
@@ -857,6 +875,12 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
         if ( hs[ S_BAD ] != 0 ) return -2;
         if ( hs[ S_DONE ] != 0 || launched >= maxit ) break;
     }
+    // the direction kept only if its TRUE residual is within ten times the tolerance ( the recurrence can drift from it )
+    k_true_res<<<RED_BLOCKS, RED_THREADS, 0, st>>>( A0, b, d, part );
+    k_true_res_final<<<1, RED_THREADS, 0, st>>>( part, state );
+    check_launch( "k_true_res" );
+    SDOT_HIP( hipMemcpyAsync( hs, state, sizeof( double ) * S_NB_STATE, hipMemcpyDeviceToHost, st ) );
+    SDOT_HIP( hipStreamSynchronize( st ) );
     double *g = A.get<double>( "mg.g", 1 );
     k_read_gauge<<<1, 64, 0, st>>>( d, gauge, g );
     k_shift_gauge<<<blocks_for( n, 256 ), 256, 0, st>>>( n, d, g, gauge );
@@ -866,7 +890,7 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
     const int its = int( hs[ S_IT ] );
     cs.nb_iter += its;
     cs.worst = std::max( cs.worst, hs[ S_ERR ] );
-    return ( hs[ S_FAILED ] == 0 && hs[ S_ERR ] <= tol ) ? its : -1;
+    return ( hs[ S_FAILED ] == 0 && hs[ S_ERR ] <= tol && hs[ S_TRUE ] <= 10 * tol ) ? its : -1;
 }
 
 } // namespace sdot_amd
