@@ -10,6 +10,11 @@ stop at max_i |m_i - nu_i| <= 1e-9.
 Prints ONE JSON line ( see the keys below ), with the roofline of the dominant kernel measured live
 with HIP events on the library's stream, and a CPU baseline: the oracle's solve of a 100 000-Dirac
 sample of the same workload on the host cores.
+
+`--config gauss3d` measures the 3D workload instead ( not the default line ): 1 000 000 Diracs uniform in
+[0,1]^3 ( default_rng(3) ), target 4 isotropic Gaussians ( means uniform in [0.2,0.8]^3, sigma 0.1, masses
+Dirichlet(1), default_rng(4) ) truncated to the unit cube, width continuation 'auto', Jacobi-PCG directions,
+stop at max_i |m_i - nu_i| <= 1e-9; its CPU baseline is the oracle's 3D solve of a 20 000-Dirac sample.
 """
 
 import argparse
@@ -44,9 +49,87 @@ def workload( n = 1_000_000, res = 1024 ):
     return pos, v
 
 
+def workload3d( n = 1_000_000 ):
+    pos = np.random.default_rng( 3 ).uniform( 0, 1, ( n, 3 ) )
+    rng = np.random.default_rng( 4 )
+    mu = rng.uniform( 0.2, 0.8, ( 4, 3 ) )
+    mass = rng.dirichlet( np.ones( 4 ) )
+    return pos, np.column_stack( [ mu, np.full( 4, 0.1 ), mass ] )
+
+
+def bench_3d( args ):
+    """the 3D four-Gaussian workload: one JSON line like the default one ( the roofline: the CG's SpMV, timed live )"""
+    import torch
+    import sdot
+    from sdot import _native
+    dev = torch.device( "cuda", 0 )
+    torch.cuda.set_device( dev )
+    pos_h, g = workload3d( args.n )
+    pos = torch.as_tensor( pos_h, device = dev )
+    target = sdot.SumOfGaussians( g[ :, :3 ], g[ :, 3 ], g[ :, 4 ], support_box = ( [ 0, 0, 0 ], [ 1, 1, 1 ] ) )
+    settings = sdot.Iterative( tol = args.tol, continuation = "auto", on_failure = "ignore", tuning = sdot.Tuning( linear_solver = "cg" ) )
+
+    def step():
+        return sdot.OtProblem( sdot.SumOfDiracs( pos ), target ).solve( settings )
+
+    for w in range( args.warmup ):
+        sol = step()
+        print( f"bench 3d: warmup { w }: { sol.stats[ 'status' ] }, { sol.stats[ 'nb_iter' ] } it", file = sys.stderr, flush = True )
+    _native.set_timing( True )
+    _native.kernel_times( reset = True )
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sols = []
+    for k in range( args.steps ):
+        sols.append( step() )
+        print( f"bench 3d: step { k }: { sols[ -1 ].stats[ 'status' ] }, { sols[ -1 ].stats[ 'nb_iter' ] } it", file = sys.stderr, flush = True )
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    times, spmv_bytes, ev = _native.kernel_times()
+    _native.set_timing( False )
+    st = sols[ -1 ].stats
+    spmv_ms, spmv_launches = times[ "spmv" ]
+    spmv_avg_ms = spmv_ms / max( spmv_launches, 1 )
+    spmv_gbs = ( spmv_bytes / max( spmv_launches, 1 ) ) / ( spmv_avg_ms * 1e-3 ) / 1e9 if spmv_launches else 0.0
+    cells_ms, cells_launches = times[ "cells" ]
+    line = dict( metric = METRIC, value = args.n * args.steps / elapsed, unit = "Diracs/s", n_gpus = 1, steps = args.steps, warmup = args.warmup,
+                 ms_per_step = 1e3 * elapsed / args.steps, higher_is_better = True, scaling = "weak", vs_baseline = None, dtype = "f64",
+                 data = "synthetic",
+                 config = dict( workload = f"3D { args.n } Diracs uniform in [0,1]^3, 4 Gaussians sigma 0.1 truncated to the unit cube, "
+                                           f"max mass error <= { args.tol:g} ( absolute )",
+                                n = args.n, tol_abs = args.tol, start = "cold ( Voronoi ), tree included", step = "trials", continuation = "auto",
+                                linear = "jacobi-pcg", cells = "fp64" ),
+                 roofline = dict( kernel = "k_cg_spmv_dot", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s",
+                                  frac = spmv_gbs / HBM_PEAK_GBS, traffic = None, avg_launch_ms = spmv_avg_ms, launches_timed = spmv_launches,
+                                  bytes_per_launch = spmv_bytes / max( spmv_launches, 1 ) ),
+                 cells = dict( kernels_ms_per_diagram = cells_ms / max( cells_launches, 1 ), diagrams = cells_launches,
+                               events_per_cell = { k: v / max( cells_launches, 1 ) / args.n for k, v in ev.items() } ),
+                 solve = dict( status = st[ "status" ], residual_abs = st[ "residual" ], residual_rel = st[ "residual" ] * args.n,
+                               nb_iter = st[ "nb_iter" ], nb_diag = st[ "nb_diag" ], nb_backtracks = st[ "nb_backtracks" ],
+                               lin_nb_iter = st[ "lin_nb_iter" ], nb_continuation_steps = st[ "nb_continuation_steps" ],
+                               nb_overflowed = st[ "nb_overflowed" ], t_diag = st[ "t_diag" ], t_lin = st[ "t_lin" ], t_total = st[ "t_total" ],
+                               device_bytes = _native.device_bytes() ) )
+    if not args.no_cpu_baseline:
+        sys.path.insert( 0, os.path.join( ROOT, "oracle" ) )
+        import oracle
+        m = min( 20_000, args.n )
+        cores = min( os.cpu_count() or 1, 16 )
+        oracle.set_threads( cores )
+        tol = args.tol * args.n / m
+        t0 = time.perf_counter()
+        r = oracle.solve_3d( pos_h[ :m ], density = dict( gaussians = g ), tol = tol, lin_tol = 1e-6, continuation = "auto" )
+        dt = time.perf_counter() - t0
+        line[ "cpu_baseline" ] = dict( value = m / dt, unit = "Diracs/s", cores = cores, kind = "port",
+                                       sample = f"the first { m } Diracs of the workload, tol { tol:g} ( the same relative accuracy ), "
+                                                f"status { r[ 'stats' ][ 'status' ] }, { dt:.1f} s" )
+    print( json.dumps( line ), flush = True )
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument( "--gpus", type = int, default = 1 )
+    ap.add_argument( "--config", default = "image2d", choices = [ "image2d", "gauss3d" ],
+                     help = "image2d: the 2D 1024^2-image workload ( the default line ); gauss3d: the 3D four-Gaussian workload" )
     ap.add_argument( "--steps", type = int, default = 3 )
     ap.add_argument( "--warmup", type = int, default = 1 )
     ap.add_argument( "--n", type = int, default = 1_000_000 )
@@ -64,6 +147,10 @@ def main():
         print( "bench.py: this build runs on ONE GPU ( the multi-GPU design is documented, not built ); use --gpus 1", file = sys.stderr )
         sys.exit( 2 )
 
+    if args.config == "gauss3d":
+        import torch
+        assert torch.cuda.is_available(), "bench.py needs the GPU"
+        return bench_3d( args )
     import torch
     import sdot
     from sdot import _native
