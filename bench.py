@@ -13,7 +13,7 @@ sample of the same workload on the host cores.
 
 `--config gauss3d` measures the 3D workload instead ( not the default line ): 1 000 000 Diracs uniform in
 [0,1]^3 ( default_rng(3) ), target 4 isotropic Gaussians ( means uniform in [0.2,0.8]^3, sigma 0.1, masses
-Dirichlet(1), default_rng(4) ) truncated to the unit cube, width continuation 'auto', Jacobi-PCG directions,
+Dirichlet(1), default_rng(4) ) truncated to the unit cube, width continuation 'auto', multigrid flexible-CG directions,
 stop at max_i |m_i - nu_i| <= 1e-9; its CPU baseline is the oracle's 3D solve of a 20 000-Dirac sample.
 """
 
@@ -58,7 +58,7 @@ def workload3d( n = 1_000_000 ):
 
 
 def bench_3d( args ):
-    """the 3D four-Gaussian workload: one JSON line like the default one ( the roofline: the CG's SpMV, timed live )"""
+    """the 3D four-Gaussian workload: one JSON line like the default one ( the roofline: the linear solver's SpMV, timed live )"""
     import torch
     import sdot
     from sdot import _native
@@ -67,7 +67,7 @@ def bench_3d( args ):
     pos_h, g = workload3d( args.n )
     pos = torch.as_tensor( pos_h, device = dev )
     target = sdot.SumOfGaussians( g[ :, :3 ], g[ :, 3 ], g[ :, 4 ], support_box = ( [ 0, 0, 0 ], [ 1, 1, 1 ] ) )
-    settings = sdot.Iterative( tol = args.tol, continuation = "auto", on_failure = "ignore", tuning = sdot.Tuning( linear_solver = "cg" ) )
+    settings = sdot.Iterative( tol = args.tol, continuation = "auto", on_failure = "ignore", tuning = sdot.Tuning( linear_solver = args.linear ) )
 
     def step():
         return sdot.OtProblem( sdot.SumOfDiracs( pos ), target ).solve( settings )
@@ -98,8 +98,8 @@ def bench_3d( args ):
                  config = dict( workload = f"3D { args.n } Diracs uniform in [0,1]^3, 4 Gaussians sigma 0.1 truncated to the unit cube, "
                                            f"max mass error <= { args.tol:g} ( absolute )",
                                 n = args.n, tol_abs = args.tol, start = "cold ( Voronoi ), tree included", step = "trials", continuation = "auto",
-                                linear = "jacobi-pcg", cells = "fp64" ),
-                 roofline = dict( kernel = "k_cg_spmv_dot", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s",
+                                linear = "jacobi-pcg" if args.linear == "cg" else "multigrid flexible CG", cells = "fp64" ),
+                 roofline = dict( kernel = "k_cg_spmv_dot" if args.linear == "cg" else "k_fcg_spmv", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s",
                                   frac = spmv_gbs / HBM_PEAK_GBS, traffic = None, avg_launch_ms = spmv_avg_ms, launches_timed = spmv_launches,
                                   bytes_per_launch = spmv_bytes / max( spmv_launches, 1 ) ),
                  cells = dict( kernels_ms_per_diagram = cells_ms / max( cells_launches, 1 ), diagrams = cells_launches,
@@ -140,6 +140,8 @@ def main():
     ap.add_argument( "--step", default = "limits", choices = [ "limits", "trials" ] )
     ap.add_argument( "--continuation", default = "never", choices = [ "never", "auto", "always" ] )
     ap.add_argument( "--linear", default = "auto", choices = [ "auto", "cg" ] )
+    ap.add_argument( "--linear-tol", type = float, default = None,
+                     help = "the linear tolerance ( default the reference's 1e-6; negative: the inexact-Newton forcing term with that floor )" )
     ap.add_argument( "--copy-calibration", action = "store_true",
                      help = "first copy 512 MiB device to device ( the known-size kernel that scales the HBM counters )" )
     args = ap.parse_args()
@@ -170,7 +172,7 @@ def main():
     pos = torch.as_tensor( pos_h, device = dev )
     target = sdot.Image( img_h, origin = [ 0, 0 ], frame = np.eye( 2 ) / args.res )
     settings = sdot.Iterative( tol = args.tol, continuation = args.continuation, on_failure = "ignore",
-                               tuning = sdot.Tuning( step = args.step, linear_solver = args.linear ) )
+                               tuning = sdot.Tuning( step = args.step, linear_solver = args.linear, linear_tol = args.linear_tol ) )
     linear = "jacobi-pcg" if args.linear == "cg" else "multigrid flexible CG ( aggregates of 8 tree ranks, damped-Jacobi smoothing, K-cycle on levels 1-2 )"
     print( f"bench: n = { args.n }, image { args.res }^2, tol { args.tol }", file = sys.stderr, flush = True )
 

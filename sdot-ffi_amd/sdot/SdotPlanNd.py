@@ -169,11 +169,6 @@ class SdotPlanNd:
                 raise NotImplementedError( "SdotPlanNd on the MI355X path, 3D: the step is by trials ( the limits are a 2D step )" )
             opts = card_options( settings, dst, verbose, 0, 0.0 )
             opts[ _native.CARD_OPTIONS.index( "step" ) ] = 0.0   # ( 'auto': trials in 3D )
-            # 'auto' directions in 3D: Jacobi-PCG. The aggregation multigrid of 8 tree ranks is a weak preconditioner for
-            # the 3D Laplacians ( hundreds of flexible-CG iterations at 1M ) and its directions stalled a 1M solve that
-            # Jacobi-PCG completes; 'mg' asks for it explicitly
-            if settings.tuning.linear_solver == "auto":
-                opts[ _native.CARD_OPTIONS.index( "lin" ) ] = 0.0
             res = _native.solve_3d( src.positions, src.weights, w0, opts, [ *lo[ :3 ], *hi[ :3 ] ], dst )
             self._set_result( res, src, settings )
             return
