@@ -91,7 +91,9 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
  *                                   `stats` report both kinds of merge
  *   options      HOST   [ 41 ]      the reference card's option vector, same order
  *                                   ( src/sdot/SdotPlanNd.py:197-201 ); the image geometry is in
- *                                   img_x0 .. img_ny as there, the Gaussians' count in img_nx
+ *                                   img_x0 .. img_ny as there, the Gaussians' count in img_nx; `lin_tol` 0 is the
+ *                                   reference's 1e-6, a negative `lin_tol` a forcing term ( inexact Newton, no
+ *                                   counterpart in the reference ) floored at its absolute value
  *   box          HOST   [ 4 ]       the domain { lo_x, lo_y, hi_x, hi_y }
  *   img          DEVICE [ ny * nx ] ( SDOT_AMD_IMAGE ) or [ img_nx ][ 4 ] ( SDOT_AMD_GAUSSIANS )
  *   weights      DEVICE [ n ]       output, `weights[ 0 ] == 0` ( the gauge )

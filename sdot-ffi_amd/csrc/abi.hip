@@ -180,10 +180,11 @@ int sdot_amd_kernel_times( double *ms, long long *launches, double *spmv_bytes, 
         }
         if ( spmv_bytes ) *spmv_bytes = T.spmv_bytes;
         unsigned long long *ev = arena().get<unsigned long long>( "events", 8 );
-        if ( events ) SDOT_HIP( hipMemcpy( events, ev, 5 * sizeof( unsigned long long ), hipMemcpyDeviceToHost ) );
+        if ( events ) copy_wait( events, ev, 5 * sizeof( unsigned long long ), hipMemcpyDeviceToHost, side_stream() );
         if ( reset ) {
             T.reset();
-            SDOT_HIP( hipMemset( ev, 0, 8 * sizeof( unsigned long long ) ) );
+            SDOT_HIP( hipMemsetAsync( ev, 0, 8 * sizeof( unsigned long long ), side_stream() ) );
+            SDOT_HIP( hipStreamSynchronize( side_stream() ) );
         }
     } );
 }

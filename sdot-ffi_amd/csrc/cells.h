@@ -61,6 +61,6 @@ enum CellMode { MODE_MASSES = 0, MODE_FACETS = 1, MODE_MOMENTS = 2 };
 void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st );
 
 /// the device scratch a diagram needs besides its outputs ( overflow list and slots )
-void cells_scratch( DiagramOut &out, I64 n, const std::string &tag );
+void cells_scratch( DiagramOut &out, I64 n, const std::string &tag, hipStream_t st );
 
 } // namespace sdot_amd

@@ -601,13 +601,13 @@ __global__ void k_reset_counters( unsigned *c ) {
     if ( threadIdx.x < 8 ) c[ threadIdx.x ] = 0;
 }
 
-void cells_scratch( DiagramOut &out, I64 n, const std::string &tag ) {
+void cells_scratch( DiagramOut &out, I64 n, const std::string &tag, hipStream_t st ) {
     Arena &A = arena();
     out.counters = A.get<unsigned>( tag + ".counters", 8 );
     out.ovf_list = A.get<int>( tag + ".ovf_list", 2 * size_t( n ) );
     if ( ! out.events ) {
         out.events = A.get<unsigned long long>( "events", 8 );
-        SDOT_HIP( hipMemset( out.events, 0, 8 * sizeof( unsigned long long ) ) );
+        SDOT_HIP( hipMemsetAsync( out.events, 0, 8 * sizeof( unsigned long long ), st ) );
     }
 }
 

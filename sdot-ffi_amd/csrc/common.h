@@ -44,6 +44,25 @@ inline void check_launch( const char *name ) {
         throw Failure( 1, int( e ), 0, std::string( "launch of " ) + name + " -> " + hipGetErrorString( e ) );
 }
 
+/// a copy on `st`, waited for. The library makes no synchronous `hipMemcpy` / `hipMemset`: those go through the legacy
+/// default stream, which HIP refuses ( hipErrorStreamCaptureImplicit ) while any thread of the process captures a graph,
+/// as the solves of a batch or a server do on their own threads
+inline void copy_wait( void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t st ) {
+    SDOT_HIP( hipMemcpyAsync( dst, src, bytes, kind, st ) );
+    SDOT_HIP( hipStreamSynchronize( st ) );
+}
+
+/// a non-blocking stream of the calling thread, for the one-off copies and fills made outside a solve
+inline hipStream_t side_stream() {
+    struct Owner {
+        hipStream_t s = nullptr;
+        ~Owner() { if ( s ) (void) hipStreamDestroy( s ); }
+    };
+    thread_local Owner o;
+    if ( ! o.s ) SDOT_HIP( hipStreamCreateWithFlags( &o.s, hipStreamNonBlocking ) );
+    return o.s;
+}
+
 // ---- the device arena --------------------------------------------------------------------------
 
 /// Library-owned device memory, grow-only, one named buffer per role: a solve of `n` seeds takes
