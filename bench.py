@@ -140,6 +140,8 @@ def main():
     ap.add_argument( "--step", default = "limits", choices = [ "limits", "trials" ] )
     ap.add_argument( "--continuation", default = "never", choices = [ "never", "auto", "always" ] )
     ap.add_argument( "--linear", default = "auto", choices = [ "auto", "cg" ] )
+    ap.add_argument( "--precision", default = "auto", choices = [ "auto", "fp64", "fp32", "mixed" ],
+                     help = "the cells' float ( 2D ): FP64, FP32 ( vertices re-solved in double ), or FP32 then FP64 from the first stagnating step" )
     ap.add_argument( "--linear-tol", type = float, default = None,
                      help = "the linear tolerance ( default the reference's 1e-6; negative: the inexact-Newton forcing term with that floor )" )
     ap.add_argument( "--copy-calibration", action = "store_true",
@@ -171,7 +173,7 @@ def main():
     pos_h, img_h = workload( args.n, args.res )
     pos = torch.as_tensor( pos_h, device = dev )
     target = sdot.Image( img_h, origin = [ 0, 0 ], frame = np.eye( 2 ) / args.res )
-    settings = sdot.Iterative( tol = args.tol, continuation = args.continuation, on_failure = "ignore",
+    settings = sdot.Iterative( tol = args.tol, continuation = args.continuation, on_failure = "ignore", precision = args.precision,
                                tuning = sdot.Tuning( step = args.step, linear_solver = args.linear, linear_tol = args.linear_tol ) )
     linear = "jacobi-pcg" if args.linear == "cg" else "multigrid flexible CG ( aggregates of 8 tree ranks, damped-Jacobi smoothing, K-cycle on levels 1-2 )"
     print( f"bench: n = { args.n }, image { args.res }^2, tol { args.tol }", file = sys.stderr, flush = True )
@@ -198,6 +200,9 @@ def main():
     st = sols[ -1 ].stats
     nu = 1.0 / args.n
     ms_step = 1e3 * elapsed / args.steps
+    from sdot.SdotPlanNd import _KERNEL_FLOAT
+    cells_float = { _native.FP64: "fp64", _native.FP32: "fp32 ( vertices re-solved in fp64 )",
+                    _native.MIXED: "mixed ( fp32, fp64 from the first stagnating step )" }[ _KERNEL_FLOAT[ args.precision ] ]
     # ---- rooflines: the cell kernel ( FP64 VALU ) and the CG's SpMV ( HBM ), the dominant one first
     cells_ms, cells_launches = times[ "cells" ]
     flops = FLOP_SIDE * ev[ "side_tests" ] + FLOP_PRUNE * ev[ "prune_vertex_tests" ] + FLOP_NEWV * ev[ "new_vertices" ]
@@ -222,11 +227,11 @@ def main():
                  config = dict( workload = f"2D { args.n } Diracs uniform in [0,1]^2, { args.res }^2 image of 8 Gaussian bumps + 0.1 floor, "
                                            f"max mass error <= { args.tol:g} ( absolute )",
                                 n = args.n, image = [ args.res, args.res ], tol_abs = args.tol, start = "cold ( Voronoi ), tree included",
-                                step = args.step, continuation = args.continuation, linear = linear, cells = "fp64" ),
+                                step = args.step, continuation = args.continuation, linear = linear, cells = cells_float ),
                  roofline = dominant, roofline_secondary = other,
                  solve = dict( status = st[ "status" ], residual_abs = st[ "residual" ], residual_rel = st[ "residual" ] / nu,
                                nb_iter = st[ "nb_iter" ], nb_diag = st[ "nb_diag" ], lin_nb_iter = st[ "lin_nb_iter" ],
-                               nb_continuation_steps = st[ "nb_continuation_steps" ], t_lim = st[ "t_lim" ],
+                               nb_continuation_steps = st[ "nb_continuation_steps" ], t_lim = st[ "t_lim" ], it_double = st[ "it_double" ],
                                t_diag = st[ "t_diag" ], t_majorant = st[ "t_majorant" ], t_lin = st[ "t_lin" ], t_total = st[ "t_total" ],
                                device_bytes = _native.device_bytes() ) )
 

@@ -111,6 +111,28 @@ int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const
                        double *weights, double *history, long long *nb_steps, double *stats,
                        double *cell_masses, double *barycenters, double *cost, long long *clusters );
 
+/* THE SOLVE with the kernel's float chosen: `sdot_amd_solve_2d`'s arguments, then `precision`
+ *   SDOT_AMD_FP64   the FP64 cells ( `sdot_amd_solve_2d` )
+ *   SDOT_AMD_FP32   the walk and the cuts in float ( seed frame, planes rounded from double ), every vertex then re-solved
+ *                   in double from its two planes ( kept within what the float vertex can be off by, else the float vertex
+ *                   projected on one of them ), measures and facets in double; cells past the first pass by the FP64 kernel
+ *   SDOT_AMD_MIXED  the FP32 kernel until one of its steps stagnates, then the FP64 kernel, the targets rescaled to the
+ *                   domain's mass as the FP64 kernel measures it; `stats[ 24 ]` ( it_double ) the iteration of the switch,
+ *                   -1 if none
+ * The barycenters and the cost come from an FP64 diagram whatever `precision`.
+ * Replaces the kernel variants of `sdotplan_solve_card_2d*`: `card_variant_for( 32 | 64, ... )`, the pair of them for
+ * 'mixed' ( src/sdot/SdotPlanNd.py:772-783 ), `sdot::gpu2d::solve<V,VD>` ( src/sdot/include/sdot/gpu/Newton2D.cuh:860-866,
+ * the switch 1701-1721 ).
+ */
+#define SDOT_AMD_FP64  0
+#define SDOT_AMD_FP32  1
+#define SDOT_AMD_MIXED 2
+int sdot_amd_solve_2d_precision( void *stream, long long n, const double *positions, const double *nu, const double *w0,
+                                 const long long *dups, long long nb_dup_values, const double *options, const double *box,
+                                 int density_kind, double rho, const double *img,
+                                 double *weights, double *history, long long *nb_steps, double *stats,
+                                 double *cell_masses, double *barycenters, double *cost, long long *clusters, int precision );
+
 /* ONE 3D DIAGRAM: the masses of the power cells of `( positions, weights )` in the box
  * `box = { lo_x, lo_y, lo_z, hi_x, hi_y, hi_z }` ( HOST ) against a constant density or a sum of Gaussians, and optionally
  * the facets `c_ij = int_facet rho / ( 2 |p_i - p_j| )` as triplets ( rows in user order, each row in its cell's face order ).

@@ -201,12 +201,14 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
     } );
 }
 
-int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const double *nu, const double *w0, const long long *dups,
-                       long long nb_dup_values, const double *options, const double *box, int density_kind, double rho, const double *img,
-                       double *weights, double *history, long long *nb_steps, double *stats, double *cell_masses, double *barycenters,
-                       double *cost, long long *clusters ) {
+int sdot_amd_solve_2d_precision( void *stream, long long n, const double *positions, const double *nu, const double *w0, const long long *dups,
+                                 long long nb_dup_values, const double *options, const double *box, int density_kind, double rho,
+                                 const double *img, double *weights, double *history, long long *nb_steps, double *stats, double *cell_masses,
+                                 double *barycenters, double *cost, long long *clusters, int precision ) {
     return guarded( [&] {
         SDOT_CHECK( n >= 2 && positions && nu && options && weights && cell_masses && barycenters, "solve: a required array is null" );
+        SDOT_CHECK( precision == SDOT_AMD_FP64 || precision == SDOT_AMD_FP32 || precision == SDOT_AMD_MIXED,
+                    "solve: the precision is SDOT_AMD_FP64, SDOT_AMD_FP32 or SDOT_AMD_MIXED" );
         check_box( box );
         SDOT_CHECK( options[ O_MAXIT ] >= 0, "solve: negative maxit" );
         SolveInputs in;
@@ -215,6 +217,7 @@ int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const
         in.nu = nu;
         in.w0 = w0;
         in.options = options;
+        in.precision = precision;
         // the exact duplicates: the first `agg_nb_dups` pairs of `dups` ( the reference passes one dummy pair when there is none )
         in.nb_dups = dups ? std::min<long long>( (long long) options[ O_AGG_NB_DUPS ], nb_dup_values / 2 ) : 0;
         SDOT_CHECK( in.nb_dups >= 0, "solve: a negative count of duplicates" );
@@ -234,6 +237,14 @@ int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const
         out.cost = cost;
         solve_2d( in, out, hipStream_t( stream ) );
     } );
+}
+
+int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const double *nu, const double *w0, const long long *dups,
+                       long long nb_dup_values, const double *options, const double *box, int density_kind, double rho, const double *img,
+                       double *weights, double *history, long long *nb_steps, double *stats, double *cell_masses, double *barycenters,
+                       double *cost, long long *clusters ) {
+    return sdot_amd_solve_2d_precision( stream, n, positions, nu, w0, dups, nb_dup_values, options, box, density_kind, rho, img, weights,
+                                        history, nb_steps, stats, cell_masses, barycenters, cost, clusters, SDOT_AMD_FP64 );
 }
 
 int sdot_amd_measures_3d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,

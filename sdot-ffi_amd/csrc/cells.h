@@ -58,7 +58,10 @@ enum CellMode { MODE_MASSES = 0, MODE_FACETS = 1, MODE_MOMENTS = 2 };
 
 /// one diagram: every cell of the tree at its current weights ( `t.seeds[ k ].z` ) in the box
 /// `lo / hi`. Resets and uses the counters; the overflow pass is launched without a read back.
-void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st );
+/// `fp32`: the walk and the cuts in float, every vertex then re-solved in double from its two planes before the measures
+/// ( the cells past the first pass run the FP64 kernel either way ).
+void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st,
+                bool fp32 = false );
 
 /// the device scratch a diagram needs besides its outputs ( overflow list and slots )
 void cells_scratch( DiagramOut &out, I64 n, const std::string &tag, hipStream_t st );

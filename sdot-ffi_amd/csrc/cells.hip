@@ -36,28 +36,30 @@ struct Ev {
     unsigned side = 0, prune = 0, newv = 0, visits = 0, cuts = 0;
 };
 
-/// a polygon's arrays: `STRIDE` lanes apart ( LDS ) or contiguous ( a global slot )
-template<int STRIDE>
+/// a polygon's arrays: `STRIDE` lanes apart ( LDS ) or contiguous ( a global slot ), coordinates in `TT` ( double; float
+/// for the walk of the FP32 kernel, whose vertices the finish re-solves in double )
+template<int STRIDE, class TT = double>
 struct Store {
-    double *x, *y;
-    int    *id;
-    __device__ __forceinline__ double &X( int i ) const { return x[ i * STRIDE ]; }
-    __device__ __forceinline__ double &Y( int i ) const { return y[ i * STRIDE ]; }
-    __device__ __forceinline__ int    &I( int i ) const { return id[ i * STRIDE ]; }
+    using T = TT;
+    T   *x, *y;
+    int *id;
+    __device__ __forceinline__ T   &X( int i ) const { return x[ i * STRIDE ]; }
+    __device__ __forceinline__ T   &Y( int i ) const { return y[ i * STRIDE ]; }
+    __device__ __forceinline__ int &I( int i ) const { return id[ i * STRIDE ]; }
 };
 
-template<class S>
-__device__ __forceinline__ double side( const S &c, int i, double dx, double dy, double off ) {
+template<class S, class T = typename S::T>
+__device__ __forceinline__ T side( const S &c, int i, T dx, T dy, T off ) {
     return fma( dx, c.X( i ), fma( dy, c.Y( i ), -off ) );
 }
 
 /// the outside run holding the largest `s`, when the signs come in several runs
-template<class S>
-__device__ void main_run( const S &c, int nb, double dx, double dy, double off, int &i1, int &nb_out ) {
+template<class S, class T = typename S::T>
+__device__ void main_run( const S &c, int nb, T dx, T dy, T off, int &i1, int &nb_out ) {
     int a = 0;
-    double sa = side( c, 0, dx, dy, off );
+    T sa = side( c, 0, dx, dy, off );
     for ( int i = 1; i < nb; ++i ) {
-        const double si = side( c, i, dx, dy, off );
+        const T si = side( c, i, dx, dy, off );
         if ( si > sa ) { a = i; sa = si; }
     }
     i1 = a;
@@ -68,15 +70,15 @@ __device__ void main_run( const S &c, int nb, double dx, double dy, double off, 
 }
 
 /// the cut by `dx . v <= off`, in place; `CUT_NO_ROOM` leaves the cell intact
-template<class S>
-__device__ int cut( const S &c, int &nb, int cap, double dx, double dy, double off, int pid, Ev &ev ) {
+template<class S, class T = typename S::T>
+__device__ int cut( const S &c, int &nb, int cap, T dx, T dy, T off, int pid, Ev &ev ) {
     // the signs, four vertices a step ( their reads issued together: the slots past `nb`, up to the next multiple of
     // four, exist since `cap` is one, and are masked ), with the mask of the outside ones among the first 64
     int nb_out = 0;
     unsigned long long out = 0;
     for ( int i = 0; i < nb; i += 4 ) {
-        const double s0 = side( c, i, dx, dy, off ), s1 = side( c, i + 1, dx, dy, off );
-        const double s2 = side( c, i + 2, dx, dy, off ), s3 = side( c, i + 3, dx, dy, off );
+        const T s0 = side( c, i, dx, dy, off ), s1 = side( c, i + 1, dx, dy, off );
+        const T s2 = side( c, i + 2, dx, dy, off ), s3 = side( c, i + 3, dx, dy, off );
         const unsigned o = unsigned( s0 > 0 ) | unsigned( i + 1 < nb && s1 > 0 ) << 1 | unsigned( i + 2 < nb && s2 > 0 ) << 2 |
                            unsigned( i + 3 < nb && s3 > 0 ) << 3;
         nb_out += __popc( o );
@@ -107,17 +109,17 @@ __device__ int cut( const S &c, int &nb, int cap, double dx, double dy, double o
     const int nb_in = nb - nb_out, new_nb = nb_in + 2;
     if ( new_nb > cap ) return CUT_NO_ROOM;
     const int j0 = ( i1 + nb - 1 ) % nb, j2 = ( i1 + nb_out - 1 ) % nb, j3 = ( j2 + 1 ) % nb;
-    const double s0 = side( c, j0, dx, dy, off ), s1 = side( c, i1, dx, dy, off );
-    const double s2 = side( c, j2, dx, dy, off ), s3 = side( c, j3, dx, dy, off );
-    const double ta = s0 / ( s0 - s1 );
-    const double pax = fma( c.X( i1 ) - c.X( j0 ), ta, c.X( j0 ) );
-    const double pay = fma( c.Y( i1 ) - c.Y( j0 ), ta, c.Y( j0 ) );
-    const double tb = s3 / ( s3 - s2 );
-    const double pbx = fma( c.X( j2 ) - c.X( j3 ), tb, c.X( j3 ) );
-    const double pby = fma( c.Y( j2 ) - c.Y( j3 ), tb, c.Y( j3 ) );
+    const T s0 = side( c, j0, dx, dy, off ), s1 = side( c, i1, dx, dy, off );
+    const T s2 = side( c, j2, dx, dy, off ), s3 = side( c, j3, dx, dy, off );
+    const T ta = s0 / ( s0 - s1 );
+    const T pax = fma( c.X( i1 ) - c.X( j0 ), ta, c.X( j0 ) );
+    const T pay = fma( c.Y( i1 ) - c.Y( j0 ), ta, c.Y( j0 ) );
+    const T tb = s3 / ( s3 - s2 );
+    const T pbx = fma( c.X( j2 ) - c.X( j3 ), tb, c.X( j3 ) );
+    const T pby = fma( c.Y( j2 ) - c.Y( j3 ), tb, c.Y( j3 ) );
     const int bid = c.I( j2 );
     auto move = [&]( int d, int s ) { c.X( d ) = c.X( s ); c.Y( d ) = c.Y( s ); c.I( d ) = c.I( s ); };
-    auto put = [&]( int d, double x, double y, int id ) { c.X( d ) = x; c.Y( d ) = y; c.I( d ) = id; };
+    auto put = [&]( int d, T x, T y, int id ) { c.X( d ) = x; c.Y( d ) = y; c.I( d ) = id; };
     if ( i1 <= j2 ) {
         if ( nb_out == 1 ) {
             for ( int i = nb; i > i1 + 1; --i ) move( i, i - 1 );
@@ -145,7 +147,7 @@ __device__ __forceinline__ bool can_cut( const S &c, int nb, const NodeRec &r, d
     ev.visits += 1;
     const double hax = r.wa_x / 2, hay = r.wa_y / 2, cb = w0 - r.wb;
     auto test = [&]( int i ) {
-        const double vx = c.X( i ), vy = c.Y( i );
+        const double vx = double( c.X( i ) ), vy = double( c.Y( i ) );
         const double X = vx + p0x, Y = vy + p0y;
         double yx = X + hax, yy = Y + hay;
         yx = yx < r.lo_x ? r.lo_x : ( yx > r.hi_x ? r.hi_x : yx );
@@ -183,12 +185,13 @@ struct Walk {
 /// from registers ), and a leaf's next seed is loaded while the current one cuts.
 template<class S>
 __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *lo, const double *hi, const S &c, int cap, Ev &ev ) {
+    using T = typename S::T;
     const double4 p = wk.seeds[ k ];
     const double p0x = p.x, p0y = p.y, w0 = p.z;
-    c.X( 0 ) = lo[ 0 ] - p0x; c.Y( 0 ) = lo[ 1 ] - p0y;
-    c.X( 1 ) = hi[ 0 ] - p0x; c.Y( 1 ) = lo[ 1 ] - p0y;
-    c.X( 2 ) = hi[ 0 ] - p0x; c.Y( 2 ) = hi[ 1 ] - p0y;
-    c.X( 3 ) = lo[ 0 ] - p0x; c.Y( 3 ) = hi[ 1 ] - p0y;
+    c.X( 0 ) = T( lo[ 0 ] - p0x ); c.Y( 0 ) = T( lo[ 1 ] - p0y );
+    c.X( 1 ) = T( hi[ 0 ] - p0x ); c.Y( 1 ) = T( lo[ 1 ] - p0y );
+    c.X( 2 ) = T( hi[ 0 ] - p0x ); c.Y( 2 ) = T( hi[ 1 ] - p0y );
+    c.X( 3 ) = T( lo[ 0 ] - p0x ); c.Y( 3 ) = T( hi[ 1 ] - p0y );
     for ( int i = 0; i < 4; ++i ) c.I( i ) = -1 - i;     // the box's sides: bottom, right, top, left
     int nb = 4;
     int m = 0, h = wk.depth;
@@ -206,7 +209,7 @@ __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *l
             if ( s == k ) continue;
             const double dx = sd.x - p0x, dy = sd.y - p0y;
             const double off = ( ( w0 - sd.z ) + ( dx * dx + dy * dy ) ) / 2;   // ( `|d|^2` after the weights' difference )
-            const int res = cut( c, nb, cap, dx, dy, off, s, ev );
+            const int res = cut( c, nb, cap, T( dx ), T( dy ), T( off ), s, ev );
             if ( res == CUT_EMPTY ) return 0;
             if ( res == CUT_NO_ROOM ) return -1;
             continue;
@@ -395,6 +398,12 @@ __device__ __forceinline__ unsigned long long wave_sum64( unsigned long long v )
     return v;
 }
 
+/// the polygons of the FP32 kernel's walk, for its finish: slot `i` of rank `k` at `i n + k`
+struct Poly32 {
+    float *x, *y;
+    int   *id, *nb;
+};
+
 struct Params {
     Walk          wk;
     int           n;
@@ -510,17 +519,43 @@ __device__ __forceinline__ void flush_events( const Params &P, const Ev &ev ) {
     }
 }
 
-} // namespace
-
-/// the first pass: one lane per cell, `CAP1` vertex slots in LDS; the blocks of one XCD take a
-/// contiguous range of ranks ( blocks are dealt round-robin to the 8 XCDs, each with its own L2 )
-__global__ __launch_bounds__( CELL_BLOCK ) void k_cells_pass1( Params P ) {
-    __shared__ double sx[ CAP1 * CELL_BLOCK ], sy[ CAP1 * CELL_BLOCK ];
-    __shared__ int sid[ CAP1 * CELL_BLOCK ];
+/// the rank of the calling lane in a first pass: the blocks of one XCD take a contiguous range of ranks ( blocks are
+/// dealt round-robin to the 8 XCDs, each with its own L2 )
+__device__ __forceinline__ int pass1_rank() {
     const int nblk = int( gridDim.x ), b = int( blockIdx.x );
     const int per = nblk / 8, rem = nblk % 8, xcd = b % 8, slot = b / 8;
     const int lb = xcd < rem ? xcd * ( per + 1 ) + slot : rem * ( per + 1 ) + ( xcd - rem ) * per + slot;
-    const int k = lb * CELL_BLOCK + int( threadIdx.x );
+    return lb * CELL_BLOCK + int( threadIdx.x );
+}
+
+/// the plane of a cut in double, from its identifier ( `gpu/Cell2D.cuh:946-971` ): the bisector with seed `id`,
+/// `n . v = ( ( w0 - w ) + |n|^2 ) / 2` in the seed's frame ( `e = |n|^2` ), or side `-1 - id` of the box ( 0 bottom,
+/// 1 right, 2 top, 3 left; `e = 1` )
+__device__ __forceinline__ void plane_of( const Params &P, int id, double p0x, double p0y, double w0, double &nx, double &ny, double &off,
+                                          double &e ) {
+    if ( id >= 0 ) {
+        const double4 q = P.wk.seeds[ id ];
+        nx = q.x - p0x;
+        ny = q.y - p0y;
+        e = nx * nx + ny * ny;
+        off = ( ( w0 - q.z ) + e ) / 2;
+        return;
+    }
+    const int f = -1 - id;
+    const bool vert = f & 1;
+    nx = vert ? 1.0 : 0.0;
+    ny = vert ? 0.0 : 1.0;
+    e = 1.0;
+    off = f == 0 ? P.lo[ 1 ] - p0y : f == 1 ? P.hi[ 0 ] - p0x : f == 2 ? P.hi[ 1 ] - p0y : P.lo[ 0 ] - p0x;
+}
+
+} // namespace
+
+/// the first pass: one lane per cell, `CAP1` vertex slots in LDS
+__global__ __launch_bounds__( CELL_BLOCK ) void k_cells_pass1( Params P ) {
+    __shared__ double sx[ CAP1 * CELL_BLOCK ], sy[ CAP1 * CELL_BLOCK ];
+    __shared__ int sid[ CAP1 * CELL_BLOCK ];
+    const int k = pass1_rank();
     const Store<CELL_BLOCK> c{ sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x };
     Ev ev;
     int nb = 0;
@@ -535,6 +570,80 @@ __global__ __launch_bounds__( CELL_BLOCK ) void k_cells_pass1( Params P ) {
     }
     finish_cell( P, k, c, nb, active, true );
     flush_events( P, ev );
+}
+
+/// the first pass of the FP32 kernel: the walk and the cuts in float ( the seed's frame; each plane computed in double,
+/// then rounded ), `CAP1` float slots in LDS; the polygon ( float vertices, the cut of each edge ) to `po`, slot `i` of
+/// rank `k` at `i n + k`, and its vertex count ( -1: listed for the second pass, which runs the FP64 kernel )
+__global__ __launch_bounds__( CELL_BLOCK ) void k_cells_walk32( Params P, Poly32 po ) {
+    __shared__ float sx[ CAP1 * CELL_BLOCK ], sy[ CAP1 * CELL_BLOCK ];
+    __shared__ int sid[ CAP1 * CELL_BLOCK ];
+    const int k = pass1_rank();
+    const Store<CELL_BLOCK, float> c{ sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x };
+    Ev ev;
+    if ( k < P.n ) {
+        const int nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP1, ev );
+        if ( nb < 0 ) {
+            const unsigned at = atomicAdd( &P.out.counters[ 1 ], 1u );
+            P.out.ovf_list[ at ] = k;
+        }
+        po.nb[ k ] = nb;
+        for ( int i = 0; i < nb; ++i ) {
+            const size_t at = size_t( i ) * size_t( P.n ) + size_t( k );
+            po.x[ at ] = c.X( i );
+            po.y[ at ] = c.Y( i );
+            po.id[ at ] = c.I( i );
+        }
+    }
+    flush_events( P, ev );
+}
+
+/// the end of the FP32 kernel's first pass: each vertex re-solved in double from the planes of the edge that arrives and
+/// the edge that leaves ( `gpu/Cell2D.cuh:973-1034` ). The solution is kept when the 2 x 2 system is well conditioned
+/// ( `det^2 > 1e-12 |n_a|^2 |n_b|^2` ) and it lies within what the float vertex can be off by ( `|v - v_f|^2 det^2 <=
+/// 1e-8 L^2 |n_a|^2 |n_b|^2`, `L` the farthest corner of the box from the seed ); else the float vertex projected on the
+/// plane of the edge that leaves it. Then the cell's measures and facets in double, as the FP64 kernel's.
+__global__ __launch_bounds__( CELL_BLOCK ) void k_cells_finish32( Params P, Poly32 po ) {
+    __shared__ double sx[ CAP1 * CELL_BLOCK ], sy[ CAP1 * CELL_BLOCK ];
+    __shared__ int sid[ CAP1 * CELL_BLOCK ];
+    const int k = pass1_rank();
+    const Store<CELL_BLOCK> c{ sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x };
+    const bool live = k < P.n;
+    const int nb = live ? po.nb[ k ] : 0;
+    const bool active = live && nb >= 0;
+    if ( active && nb > 0 ) {
+        const double4 p = P.wk.seeds[ k ];
+        const double x0 = P.lo[ 0 ] - p.x, x1 = P.hi[ 0 ] - p.x, y0 = P.lo[ 1 ] - p.y, y1 = P.hi[ 1 ] - p.y;
+        const double tol2 = 1e-8 * ( fmax( x0 * x0, x1 * x1 ) + fmax( y0 * y0, y1 * y1 ) );
+        const size_t n = size_t( P.n );
+        double ax, ay, ao, ea;
+        plane_of( P, po.id[ size_t( nb - 1 ) * n + size_t( k ) ], p.x, p.y, p.z, ax, ay, ao, ea );
+        for ( int i = 0; i < nb; ++i ) {
+            const size_t at = size_t( i ) * n + size_t( k );
+            const double x = po.x[ at ], y = po.y[ at ];
+            const int id = po.id[ at ];
+            double bx, by, bo, eb;
+            plane_of( P, id, p.x, p.y, p.z, bx, by, bo, eb );
+            const double det = ax * by - ay * bx, det2 = det * det, e12 = ea * eb;
+            double vx = 0, vy = 0;
+            bool ok = det2 > 1e-12 * e12;
+            if ( ok ) {
+                vx = ( ao * by - bo * ay ) / det;
+                vy = ( ax * bo - bx * ao ) / det;
+                ok = ( ( vx - x ) * ( vx - x ) + ( vy - y ) * ( vy - y ) ) * det2 <= tol2 * e12;
+            }
+            if ( ! ok ) {
+                const double r = eb > 0 ? ( bo - ( bx * x + by * y ) ) / eb : 0.0;
+                vx = x + r * bx;
+                vy = y + r * by;
+            }
+            c.X( i ) = vx;
+            c.Y( i ) = vy;
+            c.I( i ) = id;
+            ax = bx; ay = by; ao = bo; ea = eb;
+        }
+    }
+    finish_cell( P, k, c, nb, active, true );
 }
 
 /// the second pass: the cells that outgrew `CAP1`, one lane per cell with `CAP2` slots in LDS, one wave per
@@ -597,6 +706,32 @@ __global__ __launch_bounds__( 64 ) void k_cells_pass3( Params P, double *gx, dou
     flush_events( P, ev );
 }
 
+/// the FP32 kernel's rows made symmetric, as the reference card mirrors its upper view into both rows
+/// ( `gpu/Laplacian2D.cuh:139-232` ): facet `( k, j )` takes the value row `min( k, j )` gives it, and a facet that only one
+/// of its two cells has ( a topology the floats decided differently: a facet within the float's error, tiny ) is dropped;
+/// `dia` is the row's sum again. The value a lane reads in another row is one its owner keeps, so the rows are updated in
+/// place. Nothing is done when the rows did not fit ( the diagram is then redone )
+__global__ void k_sym_rows( int n, const int *row_off, const int *row_cnt, const int *col, double *val, double *dia, const unsigned *counters,
+                            I64 cap ) {
+    const int k = int( blockIdx.x ) * int( blockDim.x ) + int( threadIdx.x );
+    if ( k >= n || I64( counters[ 0 ] ) > cap ) return;
+    const int ok = row_off[ k ], ck = row_cnt[ k ];
+    double sum = 0;
+    for ( int q = 0; q < ck; ++q ) {
+        const int at = ok + q * ROW_STEP;
+        const int j = col[ at ];
+        const double own = val[ at ];
+        const int oj = row_off[ j ], cj = row_cnt[ j ];
+        double mirror = -1;
+        for ( int r = 0; r < cj; ++r )
+            if ( col[ oj + r * ROW_STEP ] == k ) { mirror = val[ oj + r * ROW_STEP ]; break; }
+        const double c = mirror < 0 ? 0.0 : ( j < k ? mirror : own );
+        if ( c != own ) val[ at ] = c;
+        sum += c;
+    }
+    dia[ k ] = sum > 0 ? sum : 1.0;
+}
+
 __global__ void k_reset_counters( unsigned *c ) {
     if ( threadIdx.x < 8 ) c[ threadIdx.x ] = 0;
 }
@@ -611,7 +746,7 @@ void cells_scratch( DiagramOut &out, I64 n, const std::string &tag, hipStream_t 
     }
 }
 
-void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st ) {
+void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st, bool fp32 ) {
     Params P;
     P.wk.nodes = t.nodes;
     P.wk.seeds = t.seeds;
@@ -625,18 +760,35 @@ void run_cells( const DevTree &t, const double *lo, const double *hi, const Dens
     check_launch( "k_reset_counters" );
     Timing::Pending tp;
     bool timed = timing().begin( TK_CELLS, st, tp );
-    k_cells_pass1<<<blocks_for( t.n, CELL_BLOCK ), CELL_BLOCK, 0, st>>>( P );
-    check_launch( "k_cells_pass1" );
+    Arena &A = arena();
+    if ( fp32 ) {
+        Poly32 po;
+        po.x = A.get<float>( "p32.x", size_t( CAP1 ) * size_t( t.n ) );
+        po.y = A.get<float>( "p32.y", size_t( CAP1 ) * size_t( t.n ) );
+        po.id = A.get<int>( "p32.id", size_t( CAP1 ) * size_t( t.n ) );
+        po.nb = A.get<int>( "p32.nb", size_t( t.n ) );
+        k_cells_walk32<<<blocks_for( t.n, CELL_BLOCK ), CELL_BLOCK, 0, st>>>( P, po );
+        check_launch( "k_cells_walk32" );
+        k_cells_finish32<<<blocks_for( t.n, CELL_BLOCK ), CELL_BLOCK, 0, st>>>( P, po );
+        check_launch( "k_cells_finish32" );
+    } else {
+        k_cells_pass1<<<blocks_for( t.n, CELL_BLOCK ), CELL_BLOCK, 0, st>>>( P );
+        check_launch( "k_cells_pass1" );
+    }
     if ( timed ) timing().end( tp, st );
     timed = timing().begin( TK_CELLS_OVF, st, tp );
     k_cells_pass2<<<PASS2_BLOCKS, 64, 0, st>>>( P );
     check_launch( "k_cells_pass2" );
-    Arena &A = arena();
     double *gx = A.get<double>( "ovf.x", size_t( OVF_SLOTS ) * CAP3 );
     double *gy = A.get<double>( "ovf.y", size_t( OVF_SLOTS ) * CAP3 );
     int *gid = A.get<int>( "ovf.id", size_t( OVF_SLOTS ) * CAP3 );
     k_cells_pass3<<<OVF_SLOTS / 64, 64, 0, st>>>( P, gx, gy, gid );
     check_launch( "k_cells_pass3" );
+    if ( fp32 && mode == MODE_FACETS ) {
+        k_sym_rows<<<blocks_for( t.n, 256 ), 256, 0, st>>>( int( t.n ), out.row_off, out.row_cnt, out.col, out.val, out.dia, out.counters,
+                                                           out.facet_cap );
+        check_launch( "k_sym_rows" );
+    }
     if ( timed ) timing().end( tp, st );
 }
 

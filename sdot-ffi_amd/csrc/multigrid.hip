@@ -890,6 +890,7 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
     const int its = int( hs[ S_IT ] );
     cs.nb_iter += its;
     cs.worst = std::max( cs.worst, hs[ S_ERR ] );
+    cs.last_err = hs[ S_ERR ]; cs.last_true = hs[ S_TRUE ]; cs.last_breakdown = hs[ S_FAILED ] != 0;
     return ( hs[ S_FAILED ] == 0 && hs[ S_ERR ] <= tol && hs[ S_TRUE ] <= 10 * tol ) ? its : -1;
 }
 

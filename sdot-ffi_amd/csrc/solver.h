@@ -35,7 +35,13 @@ struct Csr {
     const double *val = nullptr, *dia = nullptr;
 };
 
-struct CgStats { int nb_iter = 0; double worst = 0; };
+struct CgStats {
+    int nb_iter = 0;
+    double worst = 0;
+    // the last solve's end: its residual by the recurrence and the true one ( relative ), a breakdown ( `p . q <= 0` )
+    double last_err = 0, last_true = 0;
+    bool last_breakdown = false;
+};
 
 /// Jacobi-preconditioned CG for `L d = b`, the unknown `gauge` held at 0 ( reference
 /// `sdotplan/Linear.cpp:44-88` ); `b`, `d` device vectors of `n`. Returns the iteration count or -1.
@@ -53,6 +59,7 @@ struct SolveInputs {
     // 3D ( `dim == 3` ): positions `[ n ][ 3 ]`, the box `lo x, y, z, hi x, y, z`, a constant or a sum of Gaussians; the
     // step is by trials, and there are no duplicates
     int dim = 2;
+    int precision = 0;                                                   ///< 2D cells: 0 FP64, 1 FP32, 2 mixed
     double box3[ 6 ] = { 0, 0, 0, 1, 1, 1 };
     Density3 dens3;
 };

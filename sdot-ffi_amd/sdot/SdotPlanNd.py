@@ -21,6 +21,9 @@ _STATUS = { 0: "running", 1: "converged", 2: "max iterations", 3: "stagnation", 
 _START = { 0: "weights0", 1: "voronoi", 2: "similarity" }
 _LIN = { "auto": 0, "cholesky": 1, "amg": 2, "cg": 3, "mg": 4 }
 _STEP = { "trials": 0, "limits": 1 }
+#: the 2D cells' float per `precision` ( `src/sdot/SdotPlanNd.py:772-781`: 'mixed' the FP32 kernel, then the FP64 one from
+#: the first FP32 step that stagnates )
+_KERNEL_FLOAT = { "auto": _native.FP64, "fp64": _native.FP64, "fp32": _native.FP32, "mixed": _native.MIXED }
 _RESIDUAL = { "lin": 0, "log": 1, "power": 2 }
 _CONTINUATION = { "never": 0, "auto": 1, "always": 2 }
 #: THE AGGREGATION ( `sdotplan/Aggregation.h` ): a pair is merged when one ulp of its weights moves more than `tol / margin` of
@@ -139,8 +142,8 @@ class SdotPlanNd:
         d = problem.nb_dims
         if d not in ( 2, 3 ):
             raise NotImplementedError( f"SdotPlanNd on the MI355X path: 2D or 3D ( not { d }D )" )
-        if settings.precision == "fp32":
-            raise NotImplementedError( "precision = 'fp32' is not on the MI355X path yet ( 'auto', 'fp64' or 'mixed' run FP64 cells )" )
+        if settings.precision == "fp32" and d == 3:
+            raise NotImplementedError( "precision = 'fp32' is 2D on the MI355X path ( 3D cells are FP64: 'auto', 'fp64' or 'mixed' )" )
         plan = settings.ot_plan if settings.ot_plan is not None else warm
         if settings.weights0 is not None:
             w0, self._warm_start = settings.weights0, "weights0"
@@ -186,7 +189,7 @@ class SdotPlanNd:
                 gap = duplicate_gap( src.positions )
         opts = card_options( settings, dst, verbose, len( dup ), gap )
         pairs = np.stack( [ dup, rep ], axis = 1 ) if len( dup ) else None
-        res = _native.solve_2d( src.positions, src.weights, w0, opts, box, dst, dups = pairs )
+        res = _native.solve_2d( src.positions, src.weights, w0, opts, box, dst, dups = pairs, precision = _KERNEL_FLOAT[ settings.precision ] )
         self._set_result( res, src, settings )
 
     def _set_result( self, res, src, settings ):

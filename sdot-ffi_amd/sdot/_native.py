@@ -30,7 +30,8 @@ EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_byte
             "sdot_amd_kernel_times", "sdot_amd_measures_2d",
             "sdot_amd_solve_2d", "sdot_amd_tree_nodes", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
             "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_start", "sdot_amd_server_submit",
-            "sdot_amd_server_wait", "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d" ]
+            "sdot_amd_server_wait", "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d",
+            "sdot_amd_solve_2d_precision" ]
 
 CONSTANT, IMAGE, GAUSSIANS = 0, 1, 2
 
@@ -66,6 +67,7 @@ def lib():
         L.sdot_amd_tree_nodes.argtypes = [ _LL, _I ]
         L.sdot_amd_measures_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P, _P, _LL, _P, _P ]
         L.sdot_amd_solve_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _LL, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_solve_2d_precision.argtypes = [ _P, _LL, _P, _P, _P, _P, _LL, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I ]
         L.sdot_amd_build_tree_2d.argtypes = [ _P, _LL, _P, _I, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_refresh_majorants_2d.argtypes = [ _P, _LL, _LL, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_linear_solves_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _LL, _P, _P, _P ]
@@ -83,7 +85,7 @@ def lib():
         L.sdot_amd_kernel_times.restype = _I
         for name in ( "sdot_amd_measures_2d", "sdot_amd_solve_2d", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
                       "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_submit", "sdot_amd_server_wait",
-                      "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d" ):
+                      "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d", "sdot_amd_solve_2d_precision" ):
             getattr( L, name ).restype = _I
         if L.sdot_amd_abi_version() != 1:
             raise RuntimeError( "libsdot_amd.so: unexpected ABI version" )
@@ -235,9 +237,13 @@ def solve_3d( positions, nu, w0, options, box, density = None ):
                  stats = stats, history = hist[ :int( nb_steps[ 0 ] ) ] )
 
 
-def solve_2d( positions, nu, w0, options, box, density, dups = None ):
+#: the kernel's float ( `sdot_amd_solve_2d_precision` )
+FP64, FP32, MIXED = 0, 1, 2
+
+
+def solve_2d( positions, nu, w0, options, box, density, dups = None, precision = FP64 ):
     """one solve; returns a dict of device tensors and host arrays ( see `sdot_amd_solve_2d` ); `dups`: the exact
-    duplicates, `[ k, 2 ]` rows `( dup, rep )` of user ids"""
+    duplicates, `[ k, 2 ]` rows `( dup, rep )` of user ids; `precision`: FP64, FP32 or MIXED cells"""
     T = torch()
     pos = on_device( positions ).reshape( -1, 2 ).contiguous()
     n = pos.shape[ 0 ]
@@ -257,10 +263,10 @@ def solve_2d( positions, nu, w0, options, box, density, dups = None ):
     stats = np.zeros( 31 )
     cost = np.zeros( 1 )
     dp = None if dups is None or len( dups ) == 0 else on_device( np.asarray( dups, dtype = np.int64 ).reshape( -1 ), dtype = T.int64 )
-    _check( lib().sdot_amd_solve_2d( _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), _ptr( dp ), 0 if dp is None else dp.numel(), optp, boxp,
-                                     kind, rho, _ptr( img ),
-                                     _ptr( weights ), hist.ctypes.data_as( _P ), nb_steps.ctypes.data_as( _P ), stats.ctypes.data_as( _P ),
-                                     _ptr( masses ), _ptr( bary ), cost.ctypes.data_as( _P ), _ptr( clusters ) ) )
+    _check( lib().sdot_amd_solve_2d_precision( _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), _ptr( dp ), 0 if dp is None else dp.numel(),
+                                               optp, boxp, kind, rho, _ptr( img ),
+                                               _ptr( weights ), hist.ctypes.data_as( _P ), nb_steps.ctypes.data_as( _P ), stats.ctypes.data_as( _P ),
+                                               _ptr( masses ), _ptr( bary ), cost.ctypes.data_as( _P ), _ptr( clusters ), int( precision ) ) )
     return dict( weights = weights, cell_masses = masses, barycenters = bary, clusters = clusters, cost = float( cost[ 0 ] ),
                  stats = stats, history = hist[ :int( nb_steps[ 0 ] ) ] )
 

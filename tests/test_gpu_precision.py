@@ -1,0 +1,76 @@
+"""The FP32 cell kernel ( the walk and the cuts in float, every vertex re-solved in double from its two planes, the rows made
+symmetric ) and the mixed solve ( FP32, then FP64 from the first step the FP32 kernel cannot take ), on the GPU, through the
+C ABI: the plan such a solve returns passes the tolerance on the ORACLE's FP64 cells of its weights, its weights are the
+FP64 solve's, and its Laplacians keep the multigrid converging."""
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def say( *a ):
+    print( *a, flush = True )
+    sys.stdout.flush()
+
+
+def _density( kind, rng ):
+    """( the GPU density, the oracle's )"""
+    from sdot import Image, UniformBox
+    if kind == "uniform":
+        return UniformBox( [ 0, 0 ], [ 1, 1 ] ), None
+    img = H.bumps_image( 128, 128, rng )
+    dens = Image( img, origin = [ 0, 0 ], frame = np.eye( 2 ) / 128 ).normalized_version()
+    v = dens.values
+    return dens, dict( values = np.ascontiguousarray( np.asarray( v ).T ), x0 = 0.0, y0 = 0.0, hx = 1 / 128, hy = 1 / 128 )
+
+
+def _solve( pos, dens, precision, tol ):
+    from sdot import Iterative, OtProblem, SumOfDiracs
+    return OtProblem( SumOfDiracs( pos ), dens ).solve( Iterative( tol = tol, continuation = "never", precision = precision,
+                                                                     on_failure = "ignore" ) )
+
+
+@pytest.mark.parametrize( "kind", [ "uniform", "image" ] )
+@pytest.mark.parametrize( "precision", [ "fp32", "mixed" ] )
+def test_fp32_plan_passes_the_oracles_fp64_cells( native, oracle, kind, precision ):
+    rng = np.random.default_rng( 40 + ( kind == "image" ) )
+    n = 20000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    dens, dens_oracle = _density( kind, rng )
+    tol = 1e-10                                          # ( 2e-6 of a mean cell )
+    say( f"case precision { precision } { kind } n={ n }" )
+    sol = _solve( pos, dens, precision, tol )
+    ref = _solve( pos, dens, "fp64", tol )
+    assert sol.converged and ref.converged, ( sol.stats[ "status" ], ref.stats[ "status" ] )
+    assert sol.stats[ "it_double" ] == ( 0 if precision == "fp32" else -1 )
+    # the FP64 cells of the returned weights ( the oracle's ) have the targets to the tolerance
+    w = sol.weights.cpu().numpy()
+    mo = oracle.measures( pos, w, density = dens_oracle )
+    target = sol.target_masses.cpu().numpy() if hasattr( sol.target_masses, "cpu" ) else np.asarray( sol.target_masses )
+    assert np.abs( mo - target ).max() <= 1.01 * tol
+    # the masses the FP32 kernel reported are the oracle's cells' ( re-solved vertices ), far below the tolerance
+    assert np.abs( sol.cell_masses.cpu().numpy() - mo ).max() <= 1e-3 * tol
+    # the plan is the FP64 solve's
+    w64 = ref.weights.cpu().numpy()
+    assert np.abs( w - w64 ).max() <= 1e-6 * ( w64.max() - w64.min() )
+    # the multigrid keeps converging on the FP32 rows ( asymmetric rows broke it down into the Jacobi-PCG fallback )
+    assert sol.stats[ "lin_nb_iter" ] <= 1.5 * ref.stats[ "lin_nb_iter" ] + 50
+
+
+def test_fp32_solve_at_1m( native ):
+    """full size, through the properties: converged, and the FP64 cells of its weights pass the tolerance"""
+    import sdot
+    rng = np.random.default_rng( 42 )
+    n = 1_000_000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    tol = 1e-12
+    say( "case precision fp32 uniform 1M" )
+    sol = _solve( pos, sdot.UniformBox( [ 0, 0 ], [ 1, 1 ] ), "fp32", tol )
+    assert sol.converged
+    m = sdot.PowerDiagram( pos, sol.weights, distribution = sdot.UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).measures.cpu().numpy()
+    assert np.abs( m - 1.0 / n ).max() <= 1.01 * tol
+    assert abs( m.sum() - 1 ) < 1e-10
