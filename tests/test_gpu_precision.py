@@ -74,3 +74,23 @@ def test_fp32_solve_at_1m( native ):
     m = sdot.PowerDiagram( pos, sol.weights, distribution = sdot.UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).measures.cpu().numpy()
     assert np.abs( m - 1.0 / n ).max() <= 1.01 * tol
     assert abs( m.sum() - 1 ) < 1e-10
+
+
+def test_mixed_switches_to_fp64_where_fp32_stagnates( native ):
+    """40 pairs of seeds 1e-7 apart ( no aggregation ) and a 1e-13 tolerance: the FP32 kernel stagnates where the floats
+    decide those pairs' cuts; the mixed solve switches to the FP64 kernel there ( `it_double` the iteration ) and goes
+    further than the FP32 solve"""
+    import torch
+    import sdot
+    rng = np.random.default_rng( 7 )
+    p = rng.uniform( 0.05, 0.95, ( 5000, 2 ) )
+    pos = torch.as_tensor( np.concatenate( [ p, p[ :40 ] + 1e-7 * rng.normal( size = ( 40, 2 ) ) ] ), device = "cuda" )
+    say( "case mixed switch: 5040 seeds, 40 pairs 1e-7 apart, tol 1e-13" )
+    res = {}
+    for precision in ( "fp32", "mixed" ):
+        st = sdot.Iterative( tol = 1e-13, continuation = "never", on_failure = "ignore", precision = precision, aggregate = False )
+        res[ precision ] = sdot.OtProblem( sdot.SumOfDiracs( pos ), sdot.UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( st ).stats
+    f, m = res[ "fp32" ], res[ "mixed" ]
+    assert f[ "it_double" ] == 0 and f[ "status" ] != "converged"
+    assert 0 < m[ "it_double" ] <= m[ "nb_iter" ]
+    assert m[ "residual" ] < f[ "residual" ]
