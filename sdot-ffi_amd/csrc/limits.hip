@@ -130,10 +130,23 @@ __device__ int quadratic_roots( double a2, double a1, double c, double &r1, doub
     return 2;
 }
 
-/// the admissible end of `[ lo, hi ]` ( `mass( lo ) >= eps > mass( hi )` ), to 1e-2 relative and above zero
-__device__ double mass_bisection( const Along &al, I64 k, int nb, double lo, double hi, double eps ) {
+/// the smallest forward limit found so far by the lanes of a pass ( the bits of a non-negative double, which order as
+/// the doubles do ), read coherently
+__device__ __forceinline__ double best_so_far( const unsigned long long *best ) {
+    return __longlong_as_double( (long long) __hip_atomic_load( best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) );
+}
+
+__device__ __forceinline__ void publish_best( unsigned long long *best, double v ) {
+    if ( v < best_so_far( best ) ) atomicMin( best, (unsigned long long) __double_as_longlong( v ) );
+}
+
+/// the admissible end of `[ lo, hi ]` ( `mass( lo ) >= eps > mass( hi )` ), to 1e-2 relative and above zero. With `best`,
+/// the bisection stops once `lo` reaches the smallest limit found so far: the end it would give is at least `lo`, so it
+/// cannot be the smallest ( the value returned is then `lo` )
+__device__ double mass_bisection( const Along &al, I64 k, int nb, double lo, double hi, double eps, const unsigned long long *best = nullptr ) {
     for ( int round = 0; round < 64; ++round ) {
         if ( hi - lo <= 1e-2 * hi && lo > 0 ) break;
+        if ( best && lo > 0 && lo >= best_so_far( best ) ) break;
         const double mid = 0.5 * ( lo + hi );
         double m;
         if ( frozen_mass( al, k, nb, mid, m ) && m >= eps ) lo = mid;
@@ -142,23 +155,23 @@ __device__ double mass_bisection( const Along &al, I64 k, int nb, double lo, dou
     return lo;
 }
 
-/// A CERTIFICATE that the frozen cell's mass stays above `eps` on `t in [ 0, 1 ]` ( an image density ): every edge keeps
+/// A CERTIFICATE that the frozen cell's mass stays above `eps` on `t in [ 0, T ]` ( an image density ): every edge keeps
 /// its orientation ( its length is affine in `t`, its direction fixed: positive at both ends, it stays positive, and the
 /// cell stays convex ), the cell then lies in the box its vertices sweep, and its smallest area on `[ 0, 1 ]` times the
 /// image's smallest value over that box passes `eps` with a margin far above the rounding of the exact integral. The
 /// exact computation would then find no crossing: the cell has no limit, without its image integrals.
-__device__ bool stays_above( const Along &al, I64 k, int nb, double eps ) {
+__device__ bool stays_above( const Along &al, I64 k, int nb, double eps, double T = 1.0 ) {
     const double4 p = al.seeds[ k ];
     double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
     double qx = 0, qy = 0, qux = 0, quy = 0, fx = 0, fy = 0, fux = 0, fuy = 0;
     bool kept = true;
     auto edge = [&]( double ax, double ay, double aux, double auy, double bx, double by, double bux, double buy ) {
-        const double e0x = bx - ax, e0y = by - ay, e1x = e0x + ( bux - aux ), e1y = e0y + ( buy - auy );
+        const double e0x = bx - ax, e0y = by - ay, e1x = e0x + T * ( bux - aux ), e1y = e0y + T * ( buy - auy );
         kept = kept && e0x * e1x + e0y * e1y > 0;
     };
     const bool ok = for_each_vertex( al, k, nb, 0.0, [&]( int q, double vx, double vy, double ux, double uy ) {
-        xmin = fmin( xmin, fmin( vx, vx + ux ) ); xmax = fmax( xmax, fmax( vx, vx + ux ) );
-        ymin = fmin( ymin, fmin( vy, vy + uy ) ); ymax = fmax( ymax, fmax( vy, vy + uy ) );
+        xmin = fmin( xmin, fmin( vx, vx + T * ux ) ); xmax = fmax( xmax, fmax( vx, vx + T * ux ) );
+        ymin = fmin( ymin, fmin( vy, vy + T * uy ) ); ymax = fmax( ymax, fmax( vy, vy + T * uy ) );
         if ( q == 0 ) { fx = vx; fy = vy; fux = ux; fuy = uy; }
         else edge( qx, qy, qux, quy, vx, vy, ux, uy );
         qx = vx; qy = vy; qux = ux; quy = uy;
@@ -168,10 +181,10 @@ __device__ bool stays_above( const Along &al, I64 k, int nb, double eps ) {
     if ( ! kept ) return false;
     double a0, a1, a2;
     if ( ! area_polynomial( al, k, nb, 0.0, a0, a1, a2 ) ) return false;
-    double amin = fmin( a0, a0 + a1 + a2 );
+    double amin = fmin( a0, a0 + T * ( a1 + T * a2 ) );
     if ( a2 > 0 ) {
         const double ts = -a1 / ( 2 * a2 );
-        if ( ts > 0 && ts < 1 ) amin = fmin( amin, a0 + ts * ( a1 + ts * a2 ) );
+        if ( ts > 0 && ts < T ) amin = fmin( amin, a0 + ts * ( a1 + ts * a2 ) );
     }
     if ( ! ( amin > 0 ) ) return false;
     // the pixels of the box, a pixel more on each side, and the pyramid's level where it spans two blocks at most
@@ -195,7 +208,11 @@ __device__ bool stays_above( const Along &al, I64 k, int nb, double eps ) {
     return amin * rho > eps * ( 1 + 1e-6 );
 }
 
-__device__ double forward_cell( const Along &al, I64 k, double eps ) {
+/// the forward limit of cell `k` ( 1e300: none ). With `best`, the smallest limit found so far by other lanes, a cell that
+/// cannot go under it returns early ( 1e300, or a value not under it ): a cell certified above `eps` on `[ 0, T ]`,
+/// `T = best / 0.99`, has a limit above `best` ( the bisection's end is at least 0.99 times a `t` of mass under `eps`,
+/// and every such `t` is past `T` ); the smallest limit over the cells is therefore the same
+__device__ double forward_cell( const Along &al, I64 k, double eps, const unsigned long long *best = nullptr ) {
     const int nb = al.nb_edges[ k ];
     if ( nb < 3 ) return 1e300;
     double a0, a1, a2;
@@ -210,7 +227,12 @@ __device__ double forward_cell( const Along &al, I64 k, double eps ) {
     }
     // an image: the frozen mass at `t = 1`, and at the bottom of the area's parabola ( a cell that dips, then grows back ),
     // unless the cell is certified to stay above `eps`
-    if ( al.dn.kind == 1 && al.pyr.nl > 0 && stays_above( al, k, nb, eps ) ) return 1e300;
+    double T = 1;
+    if ( best ) {
+        const double b = best_so_far( best );
+        if ( b < 0.99 ) T = b / 0.99;
+    }
+    if ( al.dn.kind == 1 && al.pyr.nl > 0 && stays_above( al, k, nb, eps, T ) ) return 1e300;
     double m1, hi = -1;
     if ( ! frozen_mass( al, k, nb, 1.0, m1 ) ) return 1e300;
     if ( m1 < eps ) hi = 1;
@@ -220,7 +242,7 @@ __device__ double forward_cell( const Along &al, I64 k, double eps ) {
         if ( ts > 0 && ts < 1 && frozen_mass( al, k, nb, ts, ms ) && ms < eps ) hi = ts;
     }
     if ( hi < 0 ) return 1e300;
-    return mass_bisection( al, k, nb, 0.0, hi, eps );
+    return mass_bisection( al, k, nb, 0.0, hi, eps, best );
 }
 
 __device__ double backward_cell( const Along &al, I64 k, double eps, double t_trial ) {
@@ -251,8 +273,24 @@ __device__ double backward_cell( const Along &al, I64 k, double eps, double t_tr
 struct ForwardF {
     Along al;
     double eps;
-    __device__ void operator()( I64 k, double *v ) const { v[ 0 ] = forward_cell( al, k, eps ); }
+    unsigned long long *best;
+    __device__ void operator()( I64 k, double *v ) const {
+        v[ 0 ] = forward_cell( al, k, eps, best );
+        if ( v[ 0 ] < 1e300 ) publish_best( best, v[ 0 ] );
+    }
 };
+
+/// the forward limits of every `stride`-th cell, published: the bound the full pass prunes with from its first wave on
+__global__ void k_forward_sample( Along al, double eps, unsigned long long *best, int stride ) {
+    const I64 k = ( I64( blockIdx.x ) * blockDim.x + threadIdx.x ) * stride;
+    if ( k >= al.n ) return;
+    const double v = forward_cell( al, k, eps, best );
+    if ( v < 1e300 ) publish_best( best, v );
+}
+
+__global__ void k_best_reset( unsigned long long *best ) {
+    if ( threadIdx.x == 0 ) *best = (unsigned long long) __double_as_longlong( 1e300 );
+}
 
 struct BackwardF {
     Along al;
@@ -299,7 +337,14 @@ __global__ void k_blur_axis( int nx, int ny, int axis, int r, const double *ker,
 } // namespace
 
 void alpha_forward( const Along &al, double eps, double *out, hipStream_t st ) {
-    reduce_wide<1>( al.n, ForwardF{ al, eps }, Ops<1>{ { OP_MIN } }, out, st );
+    unsigned long long *best = arena().get<unsigned long long>( "lim.best", 1 );
+    k_best_reset<<<1, 64, 0, st>>>( best );
+    check_launch( "k_best_reset" );
+    constexpr int stride = 32;
+    const I64 ns = ( al.n + stride - 1 ) / stride;
+    k_forward_sample<<<blocks_for( ns, 256 ), 256, 0, st>>>( al, eps, best, stride );
+    check_launch( "k_forward_sample" );
+    reduce_wide<1>( al.n, ForwardF{ al, eps, best }, Ops<1>{ { OP_MIN } }, out, st );
 }
 
 void alpha_backward( const Along &al, const double *a, double eps, double t_trial, double *out, hipStream_t st ) {
