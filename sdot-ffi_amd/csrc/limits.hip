@@ -280,14 +280,6 @@ struct ForwardF {
     }
 };
 
-/// the forward limits of every `stride`-th cell, published: the bound the full pass prunes with from its first wave on
-__global__ void k_forward_sample( Along al, double eps, unsigned long long *best, int stride ) {
-    const I64 k = ( I64( blockIdx.x ) * blockDim.x + threadIdx.x ) * stride;
-    if ( k >= al.n ) return;
-    const double v = forward_cell( al, k, eps, best );
-    if ( v < 1e300 ) publish_best( best, v );
-}
-
 __global__ void k_best_reset( unsigned long long *best ) {
     if ( threadIdx.x == 0 ) *best = (unsigned long long) __double_as_longlong( 1e300 );
 }
@@ -340,10 +332,8 @@ void alpha_forward( const Along &al, double eps, double *out, hipStream_t st ) {
     unsigned long long *best = arena().get<unsigned long long>( "lim.best", 1 );
     k_best_reset<<<1, 64, 0, st>>>( best );
     check_launch( "k_best_reset" );
-    constexpr int stride = 32;
-    const I64 ns = ( al.n + stride - 1 ) / stride;
-    k_forward_sample<<<blocks_for( ns, 256 ), 256, 0, st>>>( al, eps, best, stride );
-    check_launch( "k_forward_sample" );
+    // ( the lanes publish the limits they complete as they go, and the others read them: a pass over a sample first to
+    //   seed the bound cost more than it saved, 2.4 ms of long bisections on a thin grid )
     reduce_wide<1>( al.n, ForwardF{ al, eps, best }, Ops<1>{ { OP_MIN } }, out, st );
 }
 
