@@ -8,7 +8,12 @@ namespace sdot_amd {
 struct MgOptions {
     int    pack = 8;          ///< seeds per aggregate ( consecutive tree ranks )
     int    nu = 2;            ///< Jacobi sweeps before and after the coarse correction
-    double omega = 0.7;       ///< their damping
+    double omega = 0.7;       ///< their damping when `chebyshev` is off
+    /// the `nu` sweeps as one Chebyshev smoother: sweep `s` damped by `1 / lambda_s`, the Chebyshev nodes of
+    /// `[ 2 / ratio, 2 ]` ( 2 bounds the spectrum of `D^-1 A` for a Laplacian whose diagonal is its row sum, by
+    /// Gershgorin ), so that the `nu` sweeps together damp that interval by the Chebyshev polynomial's bound
+    bool   chebyshev = true;
+    double ratio = 8;
     int    kcycle = 2;        ///< levels 1 .. kcycle take their coarse correction by two flexible CG steps ( a K-cycle ); below, one cycle
     I64    bottom = 1024;     ///< the levels from the first of at most this many rows run as one single-workgroup kernel ( 0: none )
     bool   graphs = true;     ///< the batches of iterations replayed from HIP graphs

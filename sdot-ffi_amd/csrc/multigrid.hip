@@ -140,6 +140,24 @@ __global__ __launch_bounds__( 256 ) void k_dense_inverse( Mat C, double *inv ) {
 
 // ---- the cycle --------------------------------------------------------------------------------------------
 
+/// the damping of each sweep of a smoothing pass ( at most four sweeps; read by selects, never indexed per lane )
+struct Damping {
+    double w0 = 0.7, w1 = 0.7, w2 = 0.7, w3 = 0.7;
+    __host__ __device__ __forceinline__ double operator()( int s ) const { return s == 0 ? w0 : s == 1 ? w1 : s == 2 ? w2 : w3; }
+};
+
+/// the sweeps' dampings for `o`: Chebyshev nodes of `[ 2 / ratio, 2 ]` ( largest first ), or `omega` for each
+Damping damping_of( const MgOptions &o ) {
+    Damping d;
+    if ( ! o.chebyshev ) { d.w0 = d.w1 = d.w2 = d.w3 = o.omega; return d; }
+    const double b = 2.0, a = b / o.ratio, th = ( b + a ) / 2, de = ( b - a ) / 2;
+    const double pi = 3.14159265358979323846;
+    double w[ 4 ] = { 0.7, 0.7, 0.7, 0.7 };
+    for ( int i = 0; i < o.nu && i < 4; ++i ) w[ i ] = 1 / ( th + de * std::cos( ( 2 * i + 1 ) * pi / ( 2 * o.nu ) ) );
+    d.w0 = w[ 0 ]; d.w1 = w[ 1 ]; d.w2 = w[ 2 ]; d.w3 = w[ 3 ];
+    return d;
+}
+
 __global__ void k_jacobi_first( I64 n, const double *r, const double *dia, double omega, double *x, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
     const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
@@ -152,8 +170,8 @@ __global__ void k_jacobi( Mat A, const double *r, const double *x, double *xo, d
     if ( i < A.n ) xo[ i ] = x[ i ] + omega * ( r[ i ] - row_times( A, i, x ) ) / A.dia[ i ];
 }
 
-/// the first two sweeps from zero in one pass: `x1 = omega D^-1 r` read on the fly, `xo = x1 + omega D^-1 ( r - A x1 )`
-__global__ void k_jacobi_two( Mat A, const double *r, double *xo, double omega, const double *state ) {
+/// the first two sweeps from zero in one pass: `x1 = omega D^-1 r` read on the fly, `xo = x1 + omega2 D^-1 ( r - A x1 )`
+__global__ void k_jacobi_two( Mat A, const double *r, double *xo, double omega, double omega2, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
     const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
     if ( i >= A.n ) return;
@@ -166,7 +184,7 @@ __global__ void k_jacobi_two( Mat A, const double *r, double *xo, double omega, 
         const int j = A.col[ e ];
         s -= A.val[ e ] * ( omega * r[ j ] / A.dia[ j ] );
     }
-    xo[ i ] = x1 + omega * ( r[ i ] - s ) / A.dia[ i ];
+    xo[ i ] = x1 + omega2 * ( r[ i ] - s ) / A.dia[ i ];
 }
 
 /// the coarse correction prolonged and the first post-sweep in one pass: `x' = x + P xc` read on the fly,
@@ -236,7 +254,7 @@ struct BottomLevel {
 };
 struct Bottom {
     int nl = 0, pack = 4, nu = 2, nc = 0;
-    double omega = 0.7;
+    Damping omega;
     const double *inv = nullptr;
     int rows = 0;                                        ///< the levels' rows together
     BottomLevel L[ MAX_LEVELS ];
@@ -246,16 +264,17 @@ constexpr int BOTTOM_ROWS = 640, BOTTOM_ENTRIES = 6144;   ///< what the LDS copy
 
 /// one V-cycle over the levels `L[ 0 .. nl - 1 ]` ( `L[ 0 ].r` in, `L[ 0 ].x` out ), the steps separated by barriers
 template<class Lv>
-__device__ void bottom_cycle( const Lv *L, int nl, int pack, int nu, double omega, const double *inv, int nc ) {
+__device__ void bottom_cycle( const Lv *L, int nl, int pack, int nu, const Damping &omega, const double *inv, int nc ) {
     const int tid = threadIdx.x;
     for ( int l = 0; l + 1 < nl; ++l ) {
         const Lv &A = L[ l ];
         // pre-smoothing from zero, `nu` sweeps ping-ponging so that the last lands in `x`
         double *cur = ( nu - 1 ) % 2 ? A.t : A.x, *oth = ( nu - 1 ) % 2 ? A.x : A.t;
-        for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) cur[ i ] = omega * A.r[ i ] / A.dia[ i ];
+        for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) cur[ i ] = omega( 0 ) * A.r[ i ] / A.dia[ i ];
         __syncthreads();
         for ( int s = 1; s < nu; ++s ) {
-            for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) oth[ i ] = cur[ i ] + omega * ( A.r[ i ] - A.row_times( i, cur ) ) / A.dia[ i ];
+            const double w = omega( s );
+            for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) oth[ i ] = cur[ i ] + w * ( A.r[ i ] - A.row_times( i, cur ) ) / A.dia[ i ];
             __syncthreads();
             double *sw = cur; cur = oth; oth = sw;
         }
@@ -286,7 +305,8 @@ __device__ void bottom_cycle( const Lv *L, int nl, int pack, int nu, double omeg
         for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) cur[ i ] = A.x[ i ] + xc[ i / pack ];
         __syncthreads();
         for ( int s = 0; s < nu; ++s ) {
-            for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) oth[ i ] = cur[ i ] + omega * ( A.r[ i ] - A.row_times( i, cur ) ) / A.dia[ i ];
+            const double w = omega( s );
+            for ( I64 i = tid; i < A.n; i += BOTTOM_THREADS ) oth[ i ] = cur[ i ] + w * ( A.r[ i ] - A.row_times( i, cur ) ) / A.dia[ i ];
             __syncthreads();
             double *sw = cur; cur = oth; oth = sw;
         }
@@ -348,17 +368,18 @@ __global__ __launch_bounds__( BOTTOM_THREADS ) void k_bottom( Bottom B, const do
         return s;
     };
     const int pack = B.pack, nu = B.nu;
-    const double omega = B.omega;
+    const Damping omega = B.omega;
     for ( int l = 0; l + 1 < B.nl; ++l ) {
         const int b = s_base[ l ], n = s_base[ l + 1 ] - b;
         // pre-smoothing from zero, `nu` sweeps ping-ponging between `t` ( 1 ) and `x` ( 0 ) so that the last lands in `x`
         int cur = ( nu - 1 ) % 2, oth = 1 - cur;
-        for ( int i = tid; i < n; i += BOTTOM_THREADS ) s_xt[ cur ][ b + i ] = omega * s_r[ b + i ] / s_dia[ b + i ];
+        for ( int i = tid; i < n; i += BOTTOM_THREADS ) s_xt[ cur ][ b + i ] = omega( 0 ) * s_r[ b + i ] / s_dia[ b + i ];
         __syncthreads();
         for ( int s = 1; s < nu; ++s ) {
+            const double w = omega( s );
             for ( int i = tid; i < n; i += BOTTOM_THREADS ) {
                 const int g = b + i;
-                s_xt[ oth ][ g ] = s_xt[ cur ][ g ] + omega * ( s_r[ g ] - row_times( g, cur ) ) / s_dia[ g ];
+                s_xt[ oth ][ g ] = s_xt[ cur ][ g ] + w * ( s_r[ g ] - row_times( g, cur ) ) / s_dia[ g ];
             }
             __syncthreads();
             cur = oth; oth = 1 - cur;
@@ -389,9 +410,10 @@ __global__ __launch_bounds__( BOTTOM_THREADS ) void k_bottom( Bottom B, const do
         for ( int i = tid; i < n; i += BOTTOM_THREADS ) s_xt[ cur ][ b + i ] = s_xt[ 0 ][ b + i ] + s_xt[ 0 ][ bc + i / pack ];
         __syncthreads();
         for ( int s = 0; s < nu; ++s ) {
+            const double w = omega( s );
             for ( int i = tid; i < n; i += BOTTOM_THREADS ) {
                 const int g = b + i;
-                s_xt[ oth ][ g ] = s_xt[ cur ][ g ] + omega * ( s_r[ g ] - row_times( g, cur ) ) / s_dia[ g ];
+                s_xt[ oth ][ g ] = s_xt[ cur ][ g ] + w * ( s_r[ g ] - row_times( g, cur ) ) / s_dia[ g ];
             }
             __syncthreads();
             cur = oth; oth = 1 - cur;
@@ -664,10 +686,11 @@ void cycle( Hierarchy &H, int l, const double *r, double *x, const MgOptions &o,
     // pre-smoothing from zero, `nu` sweeps ( the first two in one pass ) ping-ponging so that the last lands in `x`
     const int nk = o.nu >= 2 ? o.nu - 1 : 1;             // the launches
     double *cur = ( nk - 1 ) % 2 ? H.t[ l ] : x, *tmp = ( nk - 1 ) % 2 ? x : H.t[ l ];
-    if ( o.nu >= 2 ) k_jacobi_two<<<nb, 256, 0, st>>>( A, r, cur, o.omega, state );
-    else             k_jacobi_first<<<nb, 256, 0, st>>>( A.n, r, A.dia, o.omega, cur, state );
+    const Damping w = damping_of( o );
+    if ( o.nu >= 2 ) k_jacobi_two<<<nb, 256, 0, st>>>( A, r, cur, w( 0 ), w( 1 ), state );
+    else             k_jacobi_first<<<nb, 256, 0, st>>>( A.n, r, A.dia, w( 0 ), cur, state );
     for ( int s = 1; s < nk; ++s ) {
-        k_jacobi<<<nb, 256, 0, st>>>( A, r, cur, tmp, o.omega, state );
+        k_jacobi<<<nb, 256, 0, st>>>( A, r, cur, tmp, w( s + 1 ), state );
         std::swap( cur, tmp );
     }
     // the coarse correction: the residual restricted, solved by the next level's K-cycle or cycle, prolonged
@@ -677,17 +700,17 @@ void cycle( Hierarchy &H, int l, const double *r, double *x, const MgOptions &o,
     // post-smoothing: the prolongation fused with the first sweep ( into `t`, since it reads `x` around each row ), or for
     // an odd `nu` prolonged into `t` alone, then the sweeps ping-ponging so that the last lands in `x`
     if ( o.nu % 2 == 0 ) {
-        k_prolong_jacobi<<<nb, 256, 0, st>>>( A, r, x, H.x[ l + 1 ], o.pack, H.t[ l ], o.omega, state );
+        k_prolong_jacobi<<<nb, 256, 0, st>>>( A, r, x, H.x[ l + 1 ], o.pack, H.t[ l ], w( 0 ), state );
         double *c2 = H.t[ l ], *t2 = x;
         for ( int s = 1; s < o.nu; ++s ) {
-            k_jacobi<<<nb, 256, 0, st>>>( A, r, c2, t2, o.omega, state );
+            k_jacobi<<<nb, 256, 0, st>>>( A, r, c2, t2, w( s ), state );
             std::swap( c2, t2 );
         }
     } else {
         k_prolong_to<<<nb, 256, 0, st>>>( A.n, o.pack, x, H.x[ l + 1 ], H.t[ l ], state );
         double *c2 = H.t[ l ], *t2 = x;
         for ( int s = 0; s < o.nu; ++s ) {
-            k_jacobi<<<nb, 256, 0, st>>>( A, r, c2, t2, o.omega, state );
+            k_jacobi<<<nb, 256, 0, st>>>( A, r, c2, t2, w( s ), state );
             std::swap( c2, t2 );
         }
     }
@@ -749,6 +772,7 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
               int &nb_levels ) {
     SDOT_CHECK( o.pack >= 2 && o.pack <= 64 && ( o.pack & ( o.pack - 1 ) ) == 0, "multigrid: the aggregates must be a power of two from 2 to 64" );
     SDOT_CHECK( o.nu >= 1, "multigrid: at least one smoothing sweep" );
+    SDOT_CHECK( ! o.chebyshev || o.nu <= 4, "multigrid: a Chebyshev smoother of at most four sweeps" );
     Arena &A = arena();
     double *state = A.get<double>( "mg.state", S_NB_STATE );
     double *hs = pinned<double>( "mg.hstate", S_NB_STATE );
@@ -795,7 +819,7 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
             if ( H.A[ l ].n <= o.bottom ) { H.lb = l; break; }
     if ( H.lb < H.nl - 1 ) {
         Bottom &B = H.bottom;
-        B.nl = H.nl - H.lb; B.pack = o.pack; B.nu = o.nu; B.omega = o.omega; B.nc = H.nc; B.inv = H.inv; B.rows = 0;
+        B.nl = H.nl - H.lb; B.pack = o.pack; B.nu = o.nu; B.omega = damping_of( o ); B.nc = H.nc; B.inv = H.inv; B.rows = 0;
         for ( int l = H.lb; l < H.nl; ++l ) {
             BottomLevel &L = B.L[ l - H.lb ];
             L.n = H.A[ l ].n; L.cnt = H.A[ l ].row_cnt; L.col = H.A[ l ].col; L.val = H.A[ l ].val; L.dia = H.A[ l ].dia;
@@ -847,7 +871,7 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
             key.val( H.A[ l ] ); key.val( H.x[ l ] ); key.val( H.t[ l ] ); key.val( H.r[ l ] ); key.val( H.c1[ l ] ); key.val( H.c2[ l ] );
             key.val( H.v1[ l ] ); key.val( H.v2[ l ] ); key.val( H.r2[ l ] );
         }
-        key.val( H.bottom ); key.val( o.pack ); key.val( o.nu ); key.val( o.omega ); key.val( o.kcycle ); key.val( o.bottom );
+        key.val( H.bottom ); key.val( o.pack ); key.val( o.nu ); key.val( o.omega ); key.val( o.chebyshev ); key.val( o.ratio ); key.val( o.kcycle ); key.val( o.bottom );
         key.val( p ); key.val( q ); key.val( r ); key.val( z ); key.val( d ); key.val( state ); key.val( batch );
         exec = g_graphs.find( key.h );
         hipStream_t cs = exec ? nullptr : g_capture.get();
