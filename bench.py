@@ -175,7 +175,7 @@ def main():
     target = sdot.Image( img_h, origin = [ 0, 0 ], frame = np.eye( 2 ) / args.res )
     settings = sdot.Iterative( tol = args.tol, continuation = args.continuation, on_failure = "ignore", precision = args.precision,
                                tuning = sdot.Tuning( step = args.step, linear_solver = args.linear, linear_tol = args.linear_tol ) )
-    linear = "jacobi-pcg" if args.linear == "cg" else "multigrid flexible CG ( aggregates of 8 tree ranks, damped-Jacobi smoothing, K-cycle on levels 1-2 )"
+    linear = "jacobi-pcg" if args.linear == "cg" else "multigrid flexible CG ( aggregates of 8 tree ranks, degree-2 Chebyshev smoothing, K-cycle on levels 1-2 )"
     print( f"bench: n = { args.n }, image { args.res }^2, tol { args.tol }", file = sys.stderr, flush = True )
 
     def step():
