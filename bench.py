@@ -30,6 +30,7 @@ sys.path.insert( 0, os.path.join( ROOT, "sdot-ffi_amd" ) )
 
 METRIC = "Diracs/s to 1e-9 mass err (2D/3D 1M) vs plain GPU path; served p50/p99 latency"
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 # algorithmic flops of the cell kernel's events ( 2D ): a plane / vertex side test, one vertex of a pruning test,
 # a new vertex ( intersection )
@@ -203,14 +204,24 @@ def main():
     from sdot.SdotPlanNd import _KERNEL_FLOAT
     cells_float = { _native.FP64: "fp64", _native.FP32: "fp32 ( vertices re-solved in fp64 )",
                     _native.MIXED: "mixed ( fp32, fp64 from the first stagnating step )" }[ _KERNEL_FLOAT[ args.precision ] ]
-    # ---- rooflines: the cell kernel ( FP64 VALU ) and the CG's SpMV ( HBM ), the dominant one first
+    # ---- rooflines: the cell kernel ( FP64 or FP32 VALU ) and the CG's SpMV ( HBM ), the dominant one first
     cells_ms, cells_launches = times[ "cells" ]
     flops = FLOP_SIDE * ev[ "side_tests" ] + FLOP_PRUNE * ev[ "prune_vertex_tests" ] + FLOP_NEWV * ev[ "new_vertices" ]
     cells_avg_ms = cells_ms / max( cells_launches, 1 )
     cells_tf = ( flops / max( cells_launches, 1 ) ) / ( cells_avg_ms * 1e-3 ) / 1e12 if cells_launches else 0.0
-    roof_cells = dict( kernel = "k_cells_pass1", bound = "fp64-valu", achieved = cells_tf, peak = FP64_PEAK_TFLOPS, unit = "TFLOP/s",
-                       frac = cells_tf / FP64_PEAK_TFLOPS, traffic = measured_traffic( "k_cells_pass1", args ), avg_launch_ms = cells_avg_ms, launches = cells_launches,
-                       flops_per_launch = flops / max( cells_launches, 1 ) )
+    # the FP32 kernel ( 'fp32', or 'mixed' that never switched ): its walk and cuts in float, then the FP64 finish, timed together
+    kf = _KERNEL_FLOAT[ args.precision ]
+    k32 = kf == _native.FP32 or ( kf == _native.MIXED and st[ "it_double" ] < 0 )
+    if k32:
+        t_walk, t_fin = measured_traffic( "k_cells_walk32", args ), measured_traffic( "k_cells_finish32", args )
+        roof_cells = dict( kernel = "k_cells_walk32 + k_cells_finish32", bound = "fp32-valu", achieved = cells_tf, peak = FP32_PEAK_TFLOPS,
+                           unit = "TFLOP/s", frac = cells_tf / FP32_PEAK_TFLOPS,
+                           traffic = None if t_walk is None or t_fin is None else t_walk + t_fin, avg_launch_ms = cells_avg_ms,
+                           launches = cells_launches, flops_per_launch = flops / max( cells_launches, 1 ) )
+    else:
+        roof_cells = dict( kernel = "k_cells_pass1", bound = "fp64-valu", achieved = cells_tf, peak = FP64_PEAK_TFLOPS, unit = "TFLOP/s",
+                           frac = cells_tf / FP64_PEAK_TFLOPS, traffic = measured_traffic( "k_cells_pass1", args ), avg_launch_ms = cells_avg_ms,
+                           launches = cells_launches, flops_per_launch = flops / max( cells_launches, 1 ) )
     spmv_ms, spmv_launches = times[ "spmv" ]
     spmv_avg_ms = spmv_ms / max( spmv_launches, 1 )
     spmv_gbs = ( spmv_bytes / max( spmv_launches, 1 ) ) / ( spmv_avg_ms * 1e-3 ) / 1e9 if spmv_launches else 0.0

@@ -21,9 +21,9 @@ _STATUS = { 0: "running", 1: "converged", 2: "max iterations", 3: "stagnation", 
 _START = { 0: "weights0", 1: "voronoi", 2: "similarity" }
 _LIN = { "auto": 0, "cholesky": 1, "amg": 2, "cg": 3, "mg": 4 }
 _STEP = { "trials": 0, "limits": 1 }
-#: the 2D cells' float per `precision` ( `src/sdot/SdotPlanNd.py:772-781`: 'mixed' the FP32 kernel, then the FP64 one from
-#: the first FP32 step that stagnates )
-_KERNEL_FLOAT = { "auto": _native.FP64, "fp64": _native.FP64, "fp32": _native.FP32, "mixed": _native.MIXED }
+#: the 2D cells' float per `precision` ( `src/sdot/SdotPlanNd.py:772-781`: 'mixed', and 'auto' as on the reference's card,
+#: the FP32 kernel, then the FP64 one from the first FP32 step that stagnates )
+_KERNEL_FLOAT = { "auto": _native.MIXED, "fp64": _native.FP64, "fp32": _native.FP32, "mixed": _native.MIXED }
 _RESIDUAL = { "lin": 0, "log": 1, "power": 2 }
 _CONTINUATION = { "never": 0, "auto": 1, "always": 2 }
 #: THE AGGREGATION ( `sdotplan/Aggregation.h` ): a pair is merged when one ulp of its weights moves more than `tol / margin` of

@@ -24,6 +24,8 @@ def solve_batch( problems, settings = None, streams = 8 ):
     "raise", in which case the first one raises `OtNotConverged` after the whole batch ran."""
     from .OtProblem import Iterative, OtNotConverged
     settings = settings if settings is not None else Iterative()
+    if settings.precision in ( "fp32", "mixed" ):
+        raise NotImplementedError( "batched solves run FP64 cells ( precision 'auto' or 'fp64' )" )
     problems = list( problems )
     if not problems:
         return []

@@ -79,6 +79,8 @@ class Server:
         """queues the solve of `problem` ( a 2D `OtProblem` ) with `settings` ( an `Iterative` ); returns its `Ticket`"""
         from .OtProblem import Iterative
         settings = settings if settings is not None else Iterative()
+        if settings.precision in ( "fp32", "mixed" ):
+            raise NotImplementedError( "served solves run FP64 cells ( precision 'auto' or 'fp64' )" )
         if problem.nb_dims != 2:
             raise NotImplementedError( f"Server: 2D problems on the MI355X path ( not { problem.nb_dims }D )" )
         src, dst = problem.source, problem.target

@@ -240,7 +240,7 @@ def test_multigrid_solves_a_large_laplacian( native, oracle ):
 
 def _solve_both( oracle, pos, density_gpu, density_oracle, tol, lin_tol, step = "trials", continuation = "never", maxit = 100 ):
     from sdot import Iterative, OtProblem, SumOfDiracs, Tuning
-    sol = OtProblem( SumOfDiracs( pos ), density_gpu ).solve( Iterative( tol = tol, continuation = continuation, max_iter = maxit,
+    sol = OtProblem( SumOfDiracs( pos ), density_gpu ).solve( Iterative( precision = "fp64", tol = tol, continuation = continuation, max_iter = maxit,
                                                                           tuning = Tuning( step = step, linear_solver = "cg", linear_tol = lin_tol ) ) )
     ro = oracle.solve( pos, tol = tol, lin_tol = lin_tol, density = density_oracle, step = step, continuation = continuation, maxit = maxit )
     return sol, ro
@@ -295,7 +295,7 @@ def test_limits_step_matches_the_oracle( oracle, native ):
     n = len( pos )
     box = UniformBox( [ 0, 0 ], [ 1, 1 ] )
     for factor in ( 0.9, 1.5 ):
-        opts = card_options( Iterative( tol = 1e-10, continuation = "never", tuning = Tuning( step = "limits", linear_solver = "cg", linear_tol = 1e-11 ) ), box )
+        opts = card_options( Iterative( precision = "fp64", tol = 1e-10, continuation = "never", tuning = Tuning( step = "limits", linear_solver = "cg", linear_tol = 1e-11 ) ), box )
         opts[ native.CARD_OPTIONS.index( "factor" ) ] = factor
         say( "case limits clusters factor", factor )
         r = native.solve_2d( pos, np.full( n, 1 / n ), None, opts, [ 0, 0, 1, 1 ], box )
@@ -403,7 +403,7 @@ def test_aggregation_matches_the_oracle( oracle, native, name, step, lin ):
     n = len( pos )
     tol = 1e-11 / n
     say( "case aggregation", name, step, lin )
-    it = Iterative( tol = tol, max_iter = 60, continuation = "never", tuning = Tuning( step = step, linear_solver = lin, linear_tol = 1e-14 ) )
+    it = Iterative( precision = "fp64", tol = tol, max_iter = 60, continuation = "never", tuning = Tuning( step = step, linear_solver = lin, linear_tol = 1e-14 ) )
     sol = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( it )
     st = sol.stats
     w = sol.weights.cpu().numpy()
@@ -433,7 +433,7 @@ def test_without_the_merge_duplicates_do_not_converge( native ):
     from sdot import Iterative, OtProblem, SumOfDiracs, Tuning, UniformBox
     pos, _ = H.degenerate_clouds()[ "exact pair" ]
     say( "case duplicates without the merge" )
-    it = Iterative( tol = 1e-11 / len( pos ), max_iter = 30, aggregate = False, on_failure = "ignore", continuation = "never",
+    it = Iterative( precision = "fp64", tol = 1e-11 / len( pos ), max_iter = 30, aggregate = False, on_failure = "ignore", continuation = "never",
                     tuning = Tuning( linear_solver = "cg" ) )
     off = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( it )
     assert not off.converged and off.stats[ "aggregation" ] == "off" and off.clusters is None
@@ -443,8 +443,8 @@ def test_solve_is_bitwise_reproducible( native ):
     from sdot import Iterative, OtProblem, SumOfDiracs, UniformBox
     pos = np.random.default_rng( 2 ).uniform( 0, 1, ( 30000, 2 ) )
     say( "case determinism" )
-    a = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( tol = 1e-10 ) )
-    b = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( tol = 1e-10 ) )
+    a = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( precision = "fp64", tol = 1e-10 ) )
+    b = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( precision = "fp64", tol = 1e-10 ) )
     assert np.array_equal( a.weights.cpu().numpy(), b.weights.cpu().numpy() )
     assert a.stats[ "nb_diag" ] == b.stats[ "nb_diag" ]
 
@@ -453,7 +453,7 @@ def test_similarity_start_on_the_gpu( native ):
     from sdot import Iterative, OtProblem, SumOfDiracs, UniformBox
     pos = np.random.default_rng( 3 ).uniform( 2, 3, ( 2000, 2 ) )
     say( "case similarity" )
-    sol = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( tol = 1e-10 ) )
+    sol = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( precision = "fp64", tol = 1e-10 ) )
     assert sol.stats[ "start" ] == "similarity" and sol.converged
 
 
@@ -463,9 +463,9 @@ def test_warm_restart_takes_few_steps( native ):
     pos = rng.uniform( 0, 1, ( 20000, 2 ) )
     pb = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) )
     say( "case warm restart" )
-    first = pb.solve( Iterative( tol = 1e-10 ) )
+    first = pb.solve( Iterative( precision = "fp64", tol = 1e-10 ) )
     pb.source = SumOfDiracs( pos + 1e-5 * rng.normal( size = pos.shape ).clip( -3, 3 ) )
-    second = pb.solve( Iterative( tol = 1e-10 ) )
+    second = pb.solve( Iterative( precision = "fp64", tol = 1e-10 ) )
     assert second.stats[ "warm_start" ] == "ot_plan" and second.stats[ "nb_iter" ] <= 3 < first.stats[ "nb_iter" ]
 
 
@@ -474,7 +474,7 @@ def test_full_size_uniform_million( native ):
     from sdot import Iterative, OtProblem, SumOfDiracs, UniformBox
     pos = np.random.default_rng( 1 ).uniform( 0, 1, ( 1_000_000, 2 ) )
     say( "case 1M uniform" )
-    sol = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( tol = 1e-9 ) )
+    sol = OtProblem( SumOfDiracs( pos ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( Iterative( precision = "fp64", tol = 1e-9 ) )
     m = sol.cell_masses.cpu().numpy()
     assert sol.converged and np.abs( m - 1e-6 ).max() <= 1e-9
     assert abs( m.sum() - 1 ) < 1e-10
@@ -499,7 +499,7 @@ def test_batch_results_are_those_of_single_solves( native ):
     # each instance's plan is the one it gets alone, to the bit: nothing is shared between the streams but the device
     from sdot import Iterative, solve_batch
     probs = _batch_problems()
-    it = Iterative( tol = 1e-9, continuation = "never" )
+    it = Iterative( precision = "fp64", tol = 1e-9, continuation = "never" )
     say( "case batch of", len( probs ), "sizes", [ p.nb_diracs for p in probs ] )
     plans = solve_batch( probs, it, streams = 4 )
     assert len( plans ) == len( probs )
@@ -530,7 +530,7 @@ def test_server_results_are_those_of_single_solves( native ):
     import time
     from sdot import Iterative, Server
     probs = _batch_problems( k = 12, seed = 12 )
-    it = Iterative( tol = 1e-9, continuation = "never" )
+    it = Iterative( precision = "fp64", tol = 1e-9, continuation = "never" )
     gaps = np.random.default_rng( 3 ).exponential( 0.01, len( probs ) )
     say( "case server of", len( probs ), "requests" )
     with Server( slots = 4 ) as srv:
@@ -553,7 +553,7 @@ def test_server_admission_keeps_to_the_memory_budget( native ):
     # a budget that holds one request at a time: the requests run one after the other, in arrival order
     from sdot import Iterative, Server
     probs = _batch_problems( k = 4, seed = 13 )
-    it = Iterative( tol = 1e-9, continuation = "never" )
+    it = Iterative( precision = "fp64", tol = 1e-9, continuation = "never" )
     say( "case server with a small budget" )
     with Server( slots = 4, memory = 1.0 ) as srv:
         tickets = [ srv.submit( pb, it ) for pb in probs ]
