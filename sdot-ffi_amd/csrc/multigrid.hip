@@ -158,22 +158,31 @@ Damping damping_of( const MgOptions &o ) {
     return d;
 }
 
+/// the row of the calling lane, the blocks of one XCD taking a contiguous range of rows: blocks are dealt round-robin to
+/// the 8 XCDs, each with its own L2, and a row's neighbours ( near ranks ) are then read from the same L2
+__device__ __forceinline__ I64 xcd_row() {
+    const int nblk = int( gridDim.x ), b = int( blockIdx.x );
+    const int per = nblk / 8, rem = nblk % 8, xcd = b % 8, slot = b / 8;
+    const int lb = xcd < rem ? xcd * ( per + 1 ) + slot : rem * ( per + 1 ) + ( xcd - rem ) * per + slot;
+    return I64( lb ) * blockDim.x + threadIdx.x;
+}
+
 __global__ void k_jacobi_first( I64 n, const double *r, const double *dia, double omega, double *x, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    const I64 i = xcd_row();
     if ( i < n ) x[ i ] = omega * r[ i ] / dia[ i ];
 }
 
 __global__ void k_jacobi( Mat A, const double *r, const double *x, double *xo, double omega, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    const I64 i = xcd_row();
     if ( i < A.n ) xo[ i ] = x[ i ] + omega * ( r[ i ] - row_times( A, i, x ) ) / A.dia[ i ];
 }
 
 /// the first two sweeps from zero in one pass: `x1 = omega D^-1 r` read on the fly, `xo = x1 + omega2 D^-1 ( r - A x1 )`
 __global__ void k_jacobi_two( Mat A, const double *r, double *xo, double omega, double omega2, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    const I64 i = xcd_row();
     if ( i >= A.n ) return;
     const double x1 = omega * r[ i ] / A.dia[ i ];
     double s = A.dia[ i ] * x1;
@@ -192,7 +201,7 @@ __global__ void k_jacobi_two( Mat A, const double *r, double *xo, double omega, 
 __global__ void k_prolong_jacobi( Mat A, const double *r, const double *x, const double *xc, int pack, double *xo, double omega,
                                   const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    const I64 i = xcd_row();
     if ( i >= A.n ) return;
     const double xi = x[ i ] + xc[ i / pack ];
     double s = A.dia[ i ] * xi;
@@ -209,7 +218,7 @@ __global__ void k_prolong_jacobi( Mat A, const double *r, const double *x, const
 /// `xo = x + P xc`
 __global__ void k_prolong_to( I64 n, int pack, const double *x, const double *xc, double *xo, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    const I64 i = xcd_row();
     if ( i < n ) xo[ i ] = x[ i ] + xc[ i / pack ];
 }
 
@@ -217,7 +226,7 @@ __global__ void k_prolong_to( I64 n, int pack, const double *x, const double *xc
 /// summed by shuffles ( `pack` divides 64 )
 __global__ void k_restrict( Mat A, const double *r, const double *x, int pack, double *rc, const double *state ) {
     if ( state[ S_DONE ] != 0 ) return;
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    const I64 i = xcd_row();
     double v = i < A.n ? r[ i ] - row_times( A, i, x ) : 0.0;
     for ( int off = pack >> 1; off > 0; off >>= 1 ) v += __shfl_down( v, off, pack );
     if ( i < A.n && ( i % pack ) == 0 ) rc[ i / pack ] = v;
