@@ -50,6 +50,87 @@ def workload( n = 1_000_000, res = 1024 ):
     return pos, v
 
 
+def bump_image( res, centres, sigmas, floor = 0.1 ):
+    """`floor + sum_k exp( -|x - c_k|^2 / 2 s_k^2 )` at the centres of a `res x res` grid of the unit square, first index along x"""
+    xs = ( np.arange( res ) + 0.5 ) / res
+    X, Y = np.meshgrid( xs, xs, indexing = "ij" )
+    v = np.full( ( res, res ), floor )
+    for c, s in zip( centres, sigmas ):
+        v += np.exp( -( ( X - c[ 0 ] ) ** 2 + ( Y - c[ 1 ] ) ** 2 ) / ( 2 * s ** 2 ) )
+    return v
+
+
+def workload_batch( k = 256, res = 256, n_min = 1_000, n_max = 200_000 ):
+    """THE BATCHED WORKLOAD: `k` independent 2D instances, Dirac counts log-uniform in [ n_min, n_max ] ( default_rng(5) ), each
+    uniform in [0,1]^2 with its own `res^2` image of 4 Gaussian bumps ( sigma uniform in [ 0.03, 0.15 ] ) + a 0.1 floor
+    ( default_rng( [ 5, q ] ) for instance q )"""
+    sizes = np.round( np.exp( np.random.default_rng( 5 ).uniform( np.log( n_min ), np.log( n_max ), k ) ) ).astype( np.int64 )
+    out = []
+    for q, n in enumerate( sizes ):
+        r = np.random.default_rng( [ 5, q ] )
+        pos = r.uniform( 0, 1, ( int( n ), 2 ) )
+        c = r.uniform( 0, 1, ( 4, 2 ) )
+        s = r.uniform( 0.03, 0.15, 4 )
+        out.append( ( pos, bump_image( res, c, s ) ) )
+    return out
+
+
+def bench_batch( args ):
+    """the batched workload: one JSON line, `value` the Diracs solved per second by the batched solver, `vs_baseline` its
+    ratio to the plain FP64 path ( one instance per call, one after the other, on the same instances )"""
+    import torch
+    import sdot
+    dev = torch.device( "cuda", 0 )
+    torch.cuda.set_device( dev )
+    inst = workload_batch( args.batch_k, args.batch_res )
+    problems = [ sdot.OtProblem( sdot.SumOfDiracs( torch.as_tensor( p, device = dev ) ),
+                                 sdot.Image( v, origin = [ 0, 0 ], frame = np.eye( 2 ) / args.batch_res ) ) for p, v in inst ]
+    total = int( sum( len( p ) for p, _ in inst ) )
+    print( f"bench batch: { len( problems ) } instances, { total } Diracs", file = sys.stderr, flush = True )
+    settings = sdot.Iterative( tol = args.tol, continuation = args.continuation, on_failure = "ignore", precision = args.precision,
+                               tuning = sdot.Tuning( step = args.step ) )
+    plain = sdot.Iterative( tol = args.tol, continuation = args.continuation, on_failure = "ignore", precision = "fp64",
+                            tuning = sdot.Tuning( step = args.step ) )
+
+    def batched():
+        return sdot.solve_batch( problems, settings, streams = args.streams )
+
+    for w in range( args.warmup ):
+        batched()
+        print( f"bench batch: warmup { w }", file = sys.stderr, flush = True )
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range( args.steps ):
+        plans = batched()
+        print( f"bench batch: step { k }", file = sys.stderr, flush = True )
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    conv = sum( p.converged for p in plans )
+    worst = max( p.stats[ "residual" ] for p in plans )
+    # the plain path: one solve per call, FP64 cells, one instance after the other
+    t_plain = None
+    if not args.no_plain:
+        problems[ 0 ].solve( plain )
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        pl = [ pb.solve( plain ) for pb in problems ]
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t1
+        print( f"bench batch: plain FP64 path { t_plain:.2f} s, { sum( p.converged for p in pl ) } converged", file = sys.stderr, flush = True )
+    value = total * args.steps / elapsed
+    line = dict( metric = METRIC, value = value, unit = "Diracs/s", n_gpus = 1, steps = args.steps, warmup = args.warmup,
+                 ms_per_step = 1e3 * elapsed / args.steps, higher_is_better = True, scaling = "weak",
+                 vs_baseline = None if t_plain is None else value / ( total / t_plain ), dtype = "f64", data = "synthetic",
+                 config = dict( workload = f"{ len( problems ) } independent 2D instances, Dirac counts log-uniform in 1k-200k ( { total } in all ), "
+                                           f"each with its own { args.batch_res }^2 image of 4 Gaussian bumps + 0.1 floor, max mass error <= { args.tol:g} "
+                                           f"( absolute ) per instance",
+                                instances = len( problems ), diracs = total, tol_abs = args.tol, step = args.step, continuation = args.continuation,
+                                precision = args.precision, streams = args.streams ),
+                 batch = dict( converged = conv, worst_residual = worst,
+                               plain_fp64_s = t_plain, plain_fp64_diracs_per_s = None if t_plain is None else total / t_plain ) )
+    print( json.dumps( line ), flush = True )
+
+
 def workload3d( n = 1_000_000 ):
     pos = np.random.default_rng( 3 ).uniform( 0, 1, ( n, 3 ) )
     rng = np.random.default_rng( 4 )
@@ -129,8 +210,13 @@ def bench_3d( args ):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument( "--gpus", type = int, default = 1 )
-    ap.add_argument( "--config", default = "image2d", choices = [ "image2d", "gauss3d" ],
-                     help = "image2d: the 2D 1024^2-image workload ( the default line ); gauss3d: the 3D four-Gaussian workload" )
+    ap.add_argument( "--config", default = "image2d", choices = [ "image2d", "gauss3d", "batch" ],
+                     help = "image2d: the 2D 1024^2-image workload ( the default line ); gauss3d: the 3D four-Gaussian workload; "
+                            "batch: 256 independent 2D instances" )
+    ap.add_argument( "--batch-k", type = int, default = 256 )
+    ap.add_argument( "--batch-res", type = int, default = 256 )
+    ap.add_argument( "--streams", type = int, default = 8 )
+    ap.add_argument( "--no-plain", action = "store_true", help = "batch: skip the plain FP64 one-instance-per-call leg" )
     ap.add_argument( "--steps", type = int, default = 3 )
     ap.add_argument( "--warmup", type = int, default = 1 )
     ap.add_argument( "--n", type = int, default = 1_000_000 )
@@ -156,6 +242,10 @@ def main():
         import torch
         assert torch.cuda.is_available(), "bench.py needs the GPU"
         return bench_3d( args )
+    if args.config == "batch":
+        import torch
+        assert torch.cuda.is_available(), "bench.py needs the GPU"
+        return bench_batch( args )
     import torch
     import sdot
     from sdot import _native

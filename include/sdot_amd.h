@@ -53,6 +53,11 @@ void        sdot_amd_release( void );
 int sdot_amd_set_timing( int on );
 int sdot_amd_kernel_times( double *ms, long long *launches, double *spmv_bytes, unsigned long long *events, int reset );
 
+/* THE MULTIGRID'S HIP GRAPHS ( no counterpart in the reference, whose card launches every kernel ): the batches of flexible-CG
+ * iterations captured as graphs and the replays of them, since the last `sdot_amd_kernel_times( ..., reset = 1 )`, counted
+ * whether timing is on or not; a solve that replays far more than it captures reuses its graphs. HOST [ 1 ] each. */
+int sdot_amd_graph_counts( long long *captures, long long *replays );
+
 /* ONE DIAGRAM: the masses of the power cells of `( positions, weights )` in the box
  * `box = { lo_x, lo_y, hi_x, hi_y }` ( HOST ) against a density, and optionally the facets
  * `c_ij = int_facet rho / ( 2 |p_i - p_j| )` as triplets ( rows in user order, each row in its cell's
@@ -197,17 +202,31 @@ int sdot_amd_solve_2d_batch( void *stream, int nb_streams, long long k, const lo
 /* THE SERVER: continuous batching of solves ( no counterpart in the reference, which solves one problem per call; for
  * callers that serve a stream of requests ). `nb_slots` slots ( 1 .. 32 ), each a HIP stream driven by its own host
  * thread with its own device buffers, kept between its requests. A free slot takes the oldest waiting request whose
- * estimated footprint fits `memory_budget` bytes beside what the other slots hold; requests that arrive while others
- * run are admitted as slots and memory free up, and their work overlaps on the device. A request's result does not
- * depend on what ran beside it. Returns a handle, or 0 on error ( `sdot_amd_last_error` ). */
+ * footprint ( `sdot_amd_solve_bytes` ) fits `memory_budget` bytes beside what the other slots hold; requests that arrive
+ * while others run are admitted as slots and memory free up, and their work overlaps on the device. A request that fits on
+ * no slot runs once every other slot idles and has freed its buffers. A request's result does not depend on what ran beside
+ * it. Returns a handle, or 0 on error ( `sdot_amd_last_error` ). */
 long long sdot_amd_server_start( int nb_slots, double memory_budget );
 
-/* A request: the arguments of `sdot_amd_solve_2d` minus the stream, the duplicates, the history and the clusters. Every
+/* A request ( 2D, FP64 cells ): the arguments of `sdot_amd_solve_2d` minus the stream, the duplicates, the history and the clusters. Every
  * array must stay alive, and the outputs untouched, until `sdot_amd_server_wait` returns for its ticket. */
 int sdot_amd_server_submit( long long server, long long n, const double *positions, const double *nu, const double *w0,
                             const double *options, const double *box, int density_kind, double rho, const double *img,
                             double *weights, double *cell_masses, double *barycenters, double *stats, double *cost,
                             long long *ticket );
+
+/* A request of either dimension and kernel float: `dim` 2 or 3, `precision` as `sdot_amd_solve_2d_precision` ( 3D: FP64 or
+ * MIXED, both FP64 cells ), then the arguments of `sdot_amd_server_submit` with the density as `sdot_amd_solve_2d`'s ( 2D ) or
+ * `sdot_amd_solve_3d`'s ( 3D: a constant, or `nb_density` Gaussians of 5 doubles in `density` ); a 3D request steps by
+ * trials. Its admission footprint is `sdot_amd_solve_bytes` of the request. */
+int sdot_amd_server_submit_nd( long long server, int dim, int precision, long long n, const double *positions, const double *nu,
+                               const double *w0, const double *options, const double *box, int density_kind, double rho,
+                               const double *density, int nb_density, double *weights, double *cell_masses, double *barycenters,
+                               double *stats, double *cost, long long *ticket );
+
+/* What the server's slots held: `info` HOST [ 2 ]: the largest sum of the slots' device bytes seen at the end of a request,
+ * and the requests done. */
+int sdot_amd_server_info( long long server, double *info );
 
 /* Blocks until the request is done. `error` HOST [ 3 ]: 0, or the kind, code and value that stopped it; `times` HOST [ 3 ]:
  * its arrival, start and end, seconds from the server's start ( may be null ). */
@@ -215,6 +234,18 @@ int sdot_amd_server_wait( long long server, long long ticket, long long *error, 
 
 /* Stops the server once its running requests are done ( waiting requests are dropped ) and frees its slots. */
 int sdot_amd_server_stop( long long server );
+
+/* WHAT A SOLVE TAKES FROM THE DEVICE, part by part, before it starts ( the model the server admits requests by ).
+ * Replaces `card_solve_bytes` ( src/sdot/CardMemory.py:47-103, the reference's model of its card's takes ) for this library's
+ * own buffers. `parts` HOST [ SDOT_AMD_NB_MEMORY_PARTS ] ( output, bytes ): 0 the tree and its build, 1 the two diagram
+ * slots ( masses, Laplacian rows, overflow lists ), 2 the cells' edges for the step by the limits, 3 the FP32 kernel's
+ * polygons ( 2D ) or the polytopes of a 3D sum of Gaussians, 4 the third pass's vertex slots, 5 Newton's vectors, 6 the
+ * moments, 7 the linear solver ( multigrid levels and vectors ), 8 the density's copies ( blurred images, the pyramid of
+ * minima ), 9 reductions and small state. `options` the 41 doubles of `sdot_amd_solve_2d` ( may be null: defaults ),
+ * `precision` an SDOT_AMD_FP64 / FP32 / MIXED. Held to what a solve really takes by the tests ( within 10 % above it on
+ * generic clouds ); a diagram whose rows overflow their first room takes more. */
+#define SDOT_AMD_NB_MEMORY_PARTS 10
+int sdot_amd_solve_bytes( long long n, int dim, int density_kind, const double *options, int precision, double *parts );
 
 /* THE TREE: the BSP tree of a 2D cloud ( median cut on the longest axis, at most `leaf` seeds per leaf,
  * perfect binary tree of depth ceil( log2( n / leaf ) ) + 1 in preorder ).

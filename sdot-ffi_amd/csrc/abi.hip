@@ -7,6 +7,7 @@
 #include "multigrid.h"
 #include "batch.h"
 #include "server.h"
+#include "memory.h"
 
 #include <cmath>
 
@@ -25,6 +26,7 @@ Arena &arena() {
 }
 
 void set_thread_arena( Arena *a ) { tl_arena = a; }
+Arena *thread_arena() { return tl_arena; }
 
 Timing &timing() {
     static Timing t;
@@ -186,6 +188,13 @@ int sdot_amd_kernel_times( double *ms, long long *launches, double *spmv_bytes, 
             SDOT_HIP( hipMemsetAsync( ev, 0, 8 * sizeof( unsigned long long ), side_stream() ) );
             SDOT_HIP( hipStreamSynchronize( side_stream() ) );
         }
+    } );
+}
+
+int sdot_amd_graph_counts( long long *captures, long long *replays ) {
+    return guarded( [&] {
+        if ( captures ) *captures = timing().graph_captures.load();
+        if ( replays ) *replays = timing().graph_replays.load();
     } );
 }
 
@@ -367,31 +376,61 @@ long long sdot_amd_server_start( int nb_slots, double memory_budget ) {
     return rc == 0 ? h : 0;
 }
 
-int sdot_amd_server_submit( long long server, long long n, const double *positions, const double *nu, const double *w0,
-                            const double *options, const double *box, int density_kind, double rho, const double *img,
-                            double *weights, double *cell_masses, double *barycenters, double *stats, double *cost,
-                            long long *ticket ) {
+int sdot_amd_server_submit_nd( long long server, int dim, int precision, long long n, const double *positions, const double *nu,
+                               const double *w0, const double *options, const double *box, int density_kind, double rho,
+                               const double *density, int nb_density, double *weights, double *cell_masses, double *barycenters,
+                               double *stats, double *cost, long long *ticket ) {
     return guarded( [&] {
         SDOT_CHECK( n >= 2 && positions && nu && options && box && weights && cell_masses && barycenters && stats && cost && ticket,
                     "server: a required array is null" );
-        check_box( box );
+        SDOT_CHECK( dim == 2 || dim == 3, "server: a 2D or 3D request" );
+        SDOT_CHECK( precision == SDOT_AMD_FP64 || precision == SDOT_AMD_FP32 || precision == SDOT_AMD_MIXED,
+                    "server: the precision is SDOT_AMD_FP64, SDOT_AMD_FP32 or SDOT_AMD_MIXED" );
+        SDOT_CHECK( dim == 2 || precision != SDOT_AMD_FP32, "server: the 3D cells are FP64 ( SDOT_AMD_FP64 or SDOT_AMD_MIXED )" );
+        if ( dim == 3 ) check_box3( box );
+        else check_box( box );
         SDOT_CHECK( options[ O_MAXIT ] >= 0, "server: negative maxit" );
         Server *S = find_server( server );
         auto r = std::make_unique<Request>();
         r->opts.assign( options, options + NB_OPTS );
         r->opts[ O_AGG_NB_DUPS ] = 0;
-        const double geo[ 4 ] = { options[ O_IMG_X0 ], options[ O_IMG_Y0 ], options[ O_IMG_HX ], options[ O_IMG_HY ] };
-        const int nx = int( options[ O_IMG_NX ] ), ny = int( options[ O_IMG_NY ] );
         BatchJob &J = r->job;
         J.in.n = n; J.in.positions = positions; J.in.nu = nu; J.in.w0 = w0;
-        for ( int c = 0; c < 4; ++c ) J.in.box[ c ] = box[ c ];
-        J.in.dens = make_density( density_kind, rho, img, nx, ny, geo );
+        J.in.dim = dim;
+        J.in.precision = dim == 3 ? SDOT_AMD_FP64 : precision;
+        if ( dim == 3 ) {
+            for ( int c = 0; c < 6; ++c ) J.in.box3[ c ] = box[ c ];
+            J.in.dens3 = make_density3( density_kind, rho, density, nb_density );
+            r->opts[ O_STEP ] = 0;                       // ( 3D steps by trials )
+        } else {
+            for ( int c = 0; c < 4; ++c ) J.in.box[ c ] = box[ c ];
+            const double geo[ 4 ] = { options[ O_IMG_X0 ], options[ O_IMG_Y0 ], options[ O_IMG_HX ], options[ O_IMG_HY ] };
+            J.in.dens = make_density( density_kind, rho, density, int( options[ O_IMG_NX ] ), int( options[ O_IMG_NY ] ), geo );
+        }
         J.out.weights = weights; J.out.cell_masses = cell_masses; J.out.barycenters = barycenters; J.out.stats = stats; J.out.cost = cost;
-        // the footprint estimate: the buffers of a solve per seed ( two diagram slots with their rows, the multigrid's
-        // levels, the step's edges ), the image's blurred copies, and a fixed part
-        const double img_bytes = density_kind == SDOT_AMD_IMAGE ? 3.0 * 8.0 * double( nx ) * double( ny ) : 0.0;
-        r->bytes = 2048.0 * double( n ) + img_bytes + double( 32 << 20 );
+        // the admission's footprint: the byte model of the solve, part by part
+        r->bytes = solve_bytes( n, dim, density_kind, r->opts.data(), J.in.precision );
         *ticket = S->submit( std::move( r ) );
+    } );
+}
+
+int sdot_amd_server_submit( long long server, long long n, const double *positions, const double *nu, const double *w0,
+                            const double *options, const double *box, int density_kind, double rho, const double *img,
+                            double *weights, double *cell_masses, double *barycenters, double *stats, double *cost,
+                            long long *ticket ) {
+    return sdot_amd_server_submit_nd( server, 2, SDOT_AMD_FP64, n, positions, nu, w0, options, box, density_kind, rho, img,
+                                      options ? int( options[ O_IMG_NX ] ) : 0, weights, cell_masses, barycenters, stats, cost, ticket );
+}
+
+int sdot_amd_server_info( long long server, double *info ) {
+    return guarded( [&] {
+        SDOT_CHECK( info, "server_info: info is required" );
+        Server *S = find_server( server );
+        double peak = 0;
+        long long done = 0;
+        S->info( peak, done );
+        info[ 0 ] = peak;
+        info[ 1 ] = double( done );
     } );
 }
 
@@ -415,6 +454,13 @@ int sdot_amd_server_stop( long long server ) {
             g_servers.erase( it );
         }
         s.reset();
+    } );
+}
+
+int sdot_amd_solve_bytes( long long n, int dim, int density_kind, const double *options, int precision, double *parts ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && ( dim == 2 || dim == 3 ) && parts, "solve_bytes: n >= 1, dim 2 or 3 and parts are required" );
+        solve_parts( n, dim, density_kind, options, precision, parts );
     } );
 }
 

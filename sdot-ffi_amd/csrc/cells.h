@@ -60,8 +60,9 @@ enum CellMode { MODE_MASSES = 0, MODE_FACETS = 1, MODE_MOMENTS = 2 };
 /// `lo / hi`. Resets and uses the counters; the overflow pass is launched without a read back.
 /// `fp32`: the walk and the cuts in float, every vertex then re-solved in double from its two planes before the measures
 /// ( the cells past the first pass run the FP64 kernel either way ).
+/// `ovf_slots`: the lanes ( a multiple of 64 ) of the third pass, each with `CAP3` vertex slots in global memory.
 void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st,
-                bool fp32 = false );
+                bool fp32 = false, int ovf_slots = OVF_SLOTS );
 
 /// the device scratch a diagram needs besides its outputs ( overflow list and slots )
 void cells_scratch( DiagramOut &out, I64 n, const std::string &tag, hipStream_t st );

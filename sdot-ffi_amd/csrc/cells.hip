@@ -676,15 +676,15 @@ __global__ __launch_bounds__( 64 ) void k_cells_pass2( Params P ) {
     flush_events( P, ev );
 }
 
-/// the third pass: `CAP3` slots per cell in global memory, `OVF_SLOTS` lanes; past that a cell is a failure
-__global__ __launch_bounds__( 64 ) void k_cells_pass3( Params P, double *gx, double *gy, int *gid ) {
+/// the third pass: `CAP3` slots per cell in global memory, `slots` lanes ( the grid ); past that a cell is a failure
+__global__ __launch_bounds__( 64 ) void k_cells_pass3( Params P, double *gx, double *gy, int *gid, unsigned slots ) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const Store<1> c{ gx + size_t( slot ) * CAP3, gy + size_t( slot ) * CAP3, gid + size_t( slot ) * CAP3 };
     const unsigned count = P.out.counters[ 4 ];
-    const unsigned rounds = ( count + OVF_SLOTS - 1 ) / OVF_SLOTS;
+    const unsigned rounds = ( count + slots - 1 ) / slots;
     Ev ev;
     for ( unsigned r = 0; r < rounds; ++r ) {
-        const unsigned idx = r * OVF_SLOTS + unsigned( slot );
+        const unsigned idx = r * slots + unsigned( slot );
         const bool live = idx < count;
         const int k = live ? P.out.ovf_list[ P.n + int( idx ) ] : 0;
         int nb = 0;
@@ -746,7 +746,9 @@ void cells_scratch( DiagramOut &out, I64 n, const std::string &tag, hipStream_t 
     }
 }
 
-void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st, bool fp32 ) {
+void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st, bool fp32,
+                int ovf_slots ) {
+    SDOT_CHECK( ovf_slots >= 64 && ovf_slots % 64 == 0, "cells: the third pass's lanes are a positive multiple of 64" );
     Params P;
     P.wk.nodes = t.nodes;
     P.wk.seeds = t.seeds;
@@ -779,10 +781,10 @@ void run_cells( const DevTree &t, const double *lo, const double *hi, const Dens
     timed = timing().begin( TK_CELLS_OVF, st, tp );
     k_cells_pass2<<<PASS2_BLOCKS, 64, 0, st>>>( P );
     check_launch( "k_cells_pass2" );
-    double *gx = A.get<double>( "ovf.x", size_t( OVF_SLOTS ) * CAP3 );
-    double *gy = A.get<double>( "ovf.y", size_t( OVF_SLOTS ) * CAP3 );
-    int *gid = A.get<int>( "ovf.id", size_t( OVF_SLOTS ) * CAP3 );
-    k_cells_pass3<<<OVF_SLOTS / 64, 64, 0, st>>>( P, gx, gy, gid );
+    double *gx = A.get<double>( "ovf.x", size_t( ovf_slots ) * CAP3 );
+    double *gy = A.get<double>( "ovf.y", size_t( ovf_slots ) * CAP3 );
+    int *gid = A.get<int>( "ovf.id", size_t( ovf_slots ) * CAP3 );
+    k_cells_pass3<<<unsigned( ovf_slots ) / 64, 64, 0, st>>>( P, gx, gy, gid, unsigned( ovf_slots ) );
     check_launch( "k_cells_pass3" );
     if ( fp32 && mode == MODE_FACETS ) {
         k_sym_rows<<<blocks_for( t.n, 256 ), 256, 0, st>>>( int( t.n ), out.row_off, out.row_cnt, out.col, out.val, out.dia, out.counters,

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -72,6 +73,25 @@ struct Arena {
     struct Buf { void *ptr = nullptr; size_t bytes = 0; };
     std::map<std::string, Buf> bufs, hbufs;          ///< device buffers; pinned host staging
     std::mutex mtx;
+    // A FIXED REGION ( `fixed()` set it ): the buffers are then carved from caller-given memory, device and host, in the order
+    // they are asked for, 256-byte aligned; nothing is allocated, and a request past the region is a capacity failure
+    unsigned char *fix_dev = nullptr, *fix_host = nullptr;
+    size_t fix_dev_bytes = 0, fix_dev_used = 0, fix_host_bytes = 0, fix_host_used = 0;
+
+    void fixed( void *dev, size_t dev_bytes, void *host, size_t host_bytes ) {
+        std::lock_guard<std::mutex> lk( mtx );
+        bufs.clear(); hbufs.clear();
+        fix_dev = static_cast<unsigned char *>( dev ); fix_dev_bytes = dev_bytes; fix_dev_used = 0;
+        fix_host = static_cast<unsigned char *>( host ); fix_host_bytes = host_bytes; fix_host_used = 0;
+    }
+    static void *carve( unsigned char *base, size_t cap, size_t &used, size_t bytes, const std::string &name ) {
+        const size_t at = ( used + 255 ) & ~size_t( 255 );
+        if ( at + bytes > cap )
+            throw Failure( 3, 0, I64( at + bytes ), "the fixed scratch region is too small for buffer '" + name + "' (" + std::to_string( at + bytes ) +
+                                                        " of " + std::to_string( cap ) + " bytes)" );
+        used = at + bytes;
+        return base + at;
+    }
 
     /// pinned host staging, grow-only ( readbacks of the few scalars Newton decides on )
     template<class T>
@@ -79,6 +99,10 @@ struct Arena {
         std::lock_guard<std::mutex> lk( mtx );
         Buf &b = hbufs[ name ];
         const size_t bytes = std::max<size_t>( count * sizeof( T ), 64 );
+        if ( b.bytes < bytes && fix_dev ) {
+            b.ptr = carve( fix_host, fix_host_bytes, fix_host_used, bytes, name );
+            b.bytes = bytes;
+        }
         if ( b.bytes < bytes ) {
             if ( b.ptr ) (void) hipHostFree( b.ptr );
             b.ptr = nullptr;
@@ -94,6 +118,10 @@ struct Arena {
         std::lock_guard<std::mutex> lk( mtx );
         const size_t bytes = std::max<size_t>( count * sizeof( T ), 256 );
         Buf &b = bufs[ name ];
+        if ( b.bytes < bytes && fix_dev ) {
+            b.ptr = carve( fix_dev, fix_dev_bytes, fix_dev_used, bytes, name );
+            b.bytes = bytes;
+        }
         if ( b.bytes < bytes ) {
             if ( b.ptr ) SDOT_HIP( hipFree( b.ptr ) );
             b.ptr = nullptr;
@@ -117,6 +145,7 @@ struct Arena {
 
     void release() {
         std::lock_guard<std::mutex> lk( mtx );
+        if ( fix_dev ) { bufs.clear(); hbufs.clear(); fix_dev_used = fix_host_used = 0; return; }
         for ( auto &kv : bufs )
             if ( kv.second.ptr ) (void) hipFree( kv.second.ptr );
         bufs.clear();
@@ -130,6 +159,8 @@ struct Arena {
 /// current device's
 Arena &arena();
 void set_thread_arena( Arena *a );
+/// the calling thread's own arena ( null: it uses the device's )
+Arena *thread_arena();
 
 /// pinned host staging, grow-only ( readbacks of the few scalars Newton decides on )
 template<class T>
@@ -148,6 +179,8 @@ struct Timing {
     double ms[ TK_NB ] = {};
     long long launches[ TK_NB ] = {};
     double spmv_bytes = 0;          ///< algorithmic bytes of the timed SpMV launches
+    // the multigrid's HIP graphs ( counted whether timing is on or not ): batches of iterations captured, and replayed
+    std::atomic<long long> graph_captures{ 0 }, graph_replays{ 0 };
     std::mutex mtx;
 
     hipEvent_t take() {
@@ -187,6 +220,8 @@ struct Timing {
         std::lock_guard<std::mutex> lk( mtx );
         for ( int k = 0; k < TK_NB; ++k ) { ms[ k ] = 0; launches[ k ] = 0; }
         spmv_bytes = 0;
+        graph_captures = 0;
+        graph_replays = 0;
     }
 };
 

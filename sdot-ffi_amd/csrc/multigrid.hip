@@ -727,21 +727,22 @@ void cycle( Hierarchy &H, int l, const double *r, double *x, const MgOptions &o,
 
 /// THE ITERATIONS AS A GRAPH: a batch of flexible CG iterations is ~100 launches of small kernels, whose issue costs
 /// more than their work on the coarse levels; replayed from a HIP graph it is one launch. A graph is valid as long as
-/// every kernel argument is: the key hashes them all ( the hierarchy's levels, vectors and sizes, the options ). The
-/// arena keeps the buffers of a solve of the same size, so Newton's successive systems ( two slots of rows ) replay
+/// every kernel argument is: its key is the full record of them ( the hierarchy's levels, vectors and sizes, the options ),
+/// written field by field ( no padding bytes ) and compared whole, so that two different argument sets never share a graph.
+/// The arena keeps the buffers of a solve of the same size, so Newton's successive systems ( two slots of rows ) replay
 /// two graphs. A few per thread are kept ( least recently used out ).
 struct GraphCache {
-    struct Entry { uint64_t key; hipGraphExec_t exec; hipGraph_t graph; uint64_t used; };
+    struct Entry { std::vector<unsigned char> key; hipGraphExec_t exec; hipGraph_t graph; uint64_t used; };
     std::vector<Entry> entries;
     uint64_t clock = 0;
     ~GraphCache() {
         for ( auto &e : entries ) { (void) hipGraphExecDestroy( e.exec ); (void) hipGraphDestroy( e.graph ); }
     }
-    hipGraphExec_t find( uint64_t key ) {
+    hipGraphExec_t find( const std::vector<unsigned char> &key ) {
         for ( auto &e : entries ) if ( e.key == key ) { e.used = ++clock; return e.exec; }
         return nullptr;
     }
-    void put( uint64_t key, hipGraph_t g, hipGraphExec_t x ) {
+    void put( const std::vector<unsigned char> &key, hipGraph_t g, hipGraphExec_t x ) {
         if ( entries.size() >= 8 ) {
             auto old = std::min_element( entries.begin(), entries.end(), []( const Entry &a, const Entry &b ) { return a.used < b.used; } );
             (void) hipGraphExecDestroy( old->exec );
@@ -765,14 +766,23 @@ struct CaptureStream {
 };
 thread_local CaptureStream g_capture;
 
-/// FNV-1a over bytes
-struct Hash {
-    uint64_t h = 1469598103934665603ull;
-    void add( const void *p, size_t nb ) {
-        const unsigned char *c = static_cast<const unsigned char *>( p );
-        for ( size_t i = 0; i < nb; ++i ) { h ^= c[ i ]; h *= 1099511628211ull; }
+/// a graph's key: the values of the kernels' arguments appended one by one ( scalars and pointers, never a struct with
+/// padding )
+struct Key {
+    std::vector<unsigned char> b;
+    template<class T> void val( const T &v ) {
+        const unsigned char *c = reinterpret_cast<const unsigned char *>( &v );
+        b.insert( b.end(), c, c + sizeof( T ) );
     }
-    template<class T> void val( const T &v ) { add( &v, sizeof( T ) ); }
+    void mat( const Mat &A ) { val( A.n ); val( A.step ); val( A.row_off ); val( A.row_cnt ); val( A.col ); val( A.val ); val( A.dia ); }
+    void bottom( const Bottom &B ) {
+        val( B.nl ); val( B.pack ); val( B.nu ); val( B.nc ); val( B.omega.w0 ); val( B.omega.w1 ); val( B.omega.w2 ); val( B.omega.w3 );
+        val( B.inv ); val( B.rows );
+        for ( int l = 0; l < B.nl; ++l ) {
+            const BottomLevel &L = B.L[ l ];
+            val( L.n ); val( L.cnt ); val( L.col ); val( L.val ); val( L.dia ); val( L.x ); val( L.t ); val( L.r );
+        }
+    }
 };
 
 } // namespace
@@ -874,21 +884,25 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
     // the graph of a batch, found by the hash of every argument its kernels take
     hipGraphExec_t exec = nullptr;
     if ( o.graphs ) {
-        Hash key;
-        key.val( A0 ); key.val( H.nl ); key.val( H.lb ); key.val( H.nc ); key.val( H.inv ); key.val( H.part ); key.val( H.kpart );
+        Key key;
+        key.mat( A0 ); key.val( H.nl ); key.val( H.lb ); key.val( H.nc ); key.val( H.inv ); key.val( H.part ); key.val( H.kpart );
         for ( int l = 0; l < H.nl; ++l ) {
-            key.val( H.A[ l ] ); key.val( H.x[ l ] ); key.val( H.t[ l ] ); key.val( H.r[ l ] ); key.val( H.c1[ l ] ); key.val( H.c2[ l ] );
+            key.mat( H.A[ l ] ); key.val( H.x[ l ] ); key.val( H.t[ l ] ); key.val( H.r[ l ] ); key.val( H.c1[ l ] ); key.val( H.c2[ l ] );
             key.val( H.v1[ l ] ); key.val( H.v2[ l ] ); key.val( H.r2[ l ] );
         }
-        key.val( H.bottom ); key.val( o.pack ); key.val( o.nu ); key.val( o.omega ); key.val( o.chebyshev ); key.val( o.ratio ); key.val( o.kcycle ); key.val( o.bottom );
+        key.bottom( H.bottom ); key.val( o.pack ); key.val( o.nu ); key.val( o.omega ); key.val( o.chebyshev ); key.val( o.ratio );
+        key.val( o.kcycle ); key.val( o.bottom );
         key.val( p ); key.val( q ); key.val( r ); key.val( z ); key.val( d ); key.val( state ); key.val( batch );
-        exec = g_graphs.find( key.h );
+        exec = g_graphs.find( key.b );
         hipStream_t cs = exec ? nullptr : g_capture.get();
         if ( ! exec && cs && hipStreamBeginCapture( cs, hipStreamCaptureModeThreadLocal ) == hipSuccess ) {
             hipGraph_t g = nullptr;
             iterations( cs, false );
             const hipError_t e = hipStreamEndCapture( cs, &g );
-            if ( e == hipSuccess && g && hipGraphInstantiate( &exec, g, nullptr, nullptr, 0 ) == hipSuccess ) g_graphs.put( key.h, g, exec );
+            if ( e == hipSuccess && g && hipGraphInstantiate( &exec, g, nullptr, nullptr, 0 ) == hipSuccess ) {
+                g_graphs.put( key.b, g, exec );
+                timing().graph_captures += 1;
+            }
             else {                                       // ( no graph: the batches are launched one by one )
                 exec = nullptr;
                 if ( g ) (void) hipGraphDestroy( g );
@@ -900,7 +914,10 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
     for ( ;; ) {
         // the first batch launched one by one ( its SpMV timed live ); the next ones replayed from the graph
         if ( launched == 0 || ! exec ) iterations( st, launched == 0 );
-        else SDOT_HIP( hipGraphLaunch( exec, st ) );
+        else {
+            SDOT_HIP( hipGraphLaunch( exec, st ) );
+            timing().graph_replays += 1;
+        }
         check_launch( "pcg iterations" );
         launched += batch;
         SDOT_HIP( hipMemcpyAsync( hs, state, sizeof( double ) * S_NB_STATE, hipMemcpyDeviceToHost, st ) );

@@ -34,13 +34,19 @@ public:
     /// blocks until the request is done, then hands it back ( and forgets it )
     std::unique_ptr<Request> wait( I64 ticket );
     double now() const;
+    /// the largest sum of the slots' device bytes seen at the start and the end of a request, and the requests done
+    void info( double &peak, I64 &done );
 
 private:
     void run( int slot );
     int dev_ = 0;
     double budget_;
     std::vector<std::thread> threads_;
-    std::vector<double> held_;         ///< per slot: the device bytes its arena holds
+    std::vector<double> held_;         ///< per slot: the device bytes its arena holds ( or reserves )
+    std::vector<Arena *> arenas_;      ///< per slot: its arena ( set by its thread )
+    double peak_ = 0;                  ///< the largest sum of the arenas' bytes seen
+    I64 nb_done_ = 0;
+    void sample();                     ///< updates `peak_` ( the caller holds `mtx_` )
     std::vector<char> busy_, drop_;    ///< per slot: running a request; to free its buffers ( a request fits nowhere )
     std::deque<I64> queue_;            ///< waiting tickets, oldest first
     std::map<I64, std::unique_ptr<Request>> reqs_;

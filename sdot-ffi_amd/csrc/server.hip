@@ -16,6 +16,7 @@ Server::Server( int nb_slots, double budget_bytes ) : budget_( budget_bytes ), t
     held_.assign( size_t( nb_slots ), 0.0 );
     busy_.assign( size_t( nb_slots ), 0 );
     drop_.assign( size_t( nb_slots ), 0 );
+    arenas_.assign( size_t( nb_slots ), nullptr );
     for ( int s = 0; s < nb_slots; ++s ) threads_.emplace_back( [ this, s ] { run( s ); } );
 }
 
@@ -26,6 +27,18 @@ Server::~Server() {
     }
     cv_work_.notify_all();
     for ( auto &t : threads_ ) t.join();
+}
+
+void Server::sample() {
+    double s = 0;
+    for ( Arena *a : arenas_ ) if ( a ) s += double( a->bytes_held() );
+    peak_ = std::max( peak_, s );
+}
+
+void Server::info( double &peak, I64 &done ) {
+    std::lock_guard<std::mutex> lk( mtx_ );
+    peak = peak_;
+    done = nb_done_;
 }
 
 double Server::now() const { return std::chrono::duration<double>( std::chrono::steady_clock::now() - t0_ ).count(); }
@@ -60,18 +73,27 @@ void Server::run( int slot ) {
     auto arena_own = std::make_unique<Arena>();
     bool ok = hipSetDevice( dev_ ) == hipSuccess && hipStreamCreateWithFlags( &st, hipStreamNonBlocking ) == hipSuccess;
     set_thread_arena( arena_own.get() );
+    {
+        std::lock_guard<std::mutex> lk( mtx_ );
+        arenas_[ size_t( slot ) ] = arena_own.get();
+    }
     for ( ;; ) {
         Request *r = nullptr;
         bool release = false;
         {
             std::unique_lock<std::mutex> lk( mtx_ );
-            // the oldest waiting request this slot can take within the budget; if none fits anywhere while every slot
-            // idles, the oldest one, after this slot's arena is released
+            auto others_of = [&]( size_t s ) {
+                double o = 0;
+                for ( size_t u = 0; u < held_.size(); ++u ) if ( u != s ) o += held_[ u ];
+                return o;
+            };
+            // the oldest waiting request this slot can take within the budget. THE FALLBACK, for a request that fits on no
+            // slot: when every other slot idles and no idle slot could take the oldest request within the budget, this slot
+            // takes it, and every other slot that holds memory is told to free it; the request starts once they have
             cv_work_.wait( lk, [ & ] {
                 if ( stop_ || drop_[ size_t( slot ) ] ) return true;
                 if ( queue_.empty() ) return false;
-                double others = 0;
-                for ( size_t s = 0; s < held_.size(); ++s ) if ( int( s ) != slot ) others += held_[ s ];
+                const double others = others_of( size_t( slot ) );
                 for ( auto q = queue_.begin(); q != queue_.end(); ++q ) {
                     Request *c = reqs_[ *q ].get();
                     if ( others + std::max( held_[ size_t( slot ) ], c->bytes ) <= budget_ ) {
@@ -81,27 +103,37 @@ void Server::run( int slot ) {
                         return true;
                     }
                 }
-                bool idle = true;
-                for ( size_t s = 0; s < busy_.size(); ++s ) if ( int( s ) != slot && busy_[ s ] ) idle = false;
-                if ( idle ) {
-                    r = reqs_[ queue_.front() ].get();
-                    queue_.pop_front();
-                    release = true;
-                    held_[ size_t( slot ) ] = r->bytes;
-                    for ( size_t s = 0; s < drop_.size(); ++s ) if ( int( s ) != slot && held_[ s ] > 0 ) drop_[ s ] = 1;
-                    return true;
-                }
-                return false;
+                for ( size_t s = 0; s < busy_.size(); ++s ) if ( int( s ) != slot && busy_[ s ] ) return false;
+                Request *c = reqs_[ queue_.front() ].get();
+                for ( size_t s = 0; s < held_.size(); ++s )
+                    if ( others_of( s ) + std::max( held_[ s ], c->bytes ) <= budget_ ) return false;   // ( that slot takes it )
+                r = c;
+                queue_.pop_front();
+                release = true;
+                held_[ size_t( slot ) ] = c->bytes;
+                for ( size_t s = 0; s < drop_.size(); ++s ) if ( int( s ) != slot && held_[ s ] > 0 ) drop_[ s ] = 1;
+                return true;
             } );
             if ( ! r && drop_[ size_t( slot ) ] && ! stop_ ) {   // another slot needs the memory this idle one holds
-                drop_[ size_t( slot ) ] = 0;
-                held_[ size_t( slot ) ] = 0;
                 lk.unlock();
                 arena_own->release();
+                lk.lock();
+                drop_[ size_t( slot ) ] = 0;
+                held_[ size_t( slot ) ] = 0;                 // ( only once the memory is freed )
+                lk.unlock();
+                cv_work_.notify_all();
                 continue;
             }
             if ( ! r ) break;                           // ( stopping, nothing taken )
             busy_[ size_t( slot ) ] = 1;
+            if ( release ) {                             // the other slots free their memory first
+                cv_work_.notify_all();
+                cv_work_.wait( lk, [ & ] {
+                    if ( stop_ ) return true;
+                    for ( size_t s = 0; s < drop_.size(); ++s ) if ( drop_[ s ] ) return false;
+                    return true;
+                } );
+            }
         }
         BatchJob &J = r->job;
         J.t_start = now();
@@ -124,9 +156,15 @@ void Server::run( int slot ) {
             held_[ size_t( slot ) ] = double( arena_own->bytes_held() );
             busy_[ size_t( slot ) ] = 0;
             r->done = true;
+            ++nb_done_;
+            sample();
         }
         cv_done_.notify_all();
         cv_work_.notify_all();                           // ( the memory this slot holds may have changed )
+    }
+    {
+        std::lock_guard<std::mutex> lk( mtx_ );
+        arenas_[ size_t( slot ) ] = nullptr;
     }
     set_thread_arena( nullptr );
     arena_own->release();

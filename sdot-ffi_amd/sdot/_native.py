@@ -31,7 +31,7 @@ EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_byte
             "sdot_amd_solve_2d", "sdot_amd_tree_nodes", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
             "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_start", "sdot_amd_server_submit",
             "sdot_amd_server_wait", "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d",
-            "sdot_amd_solve_2d_precision" ]
+            "sdot_amd_solve_2d_precision", "sdot_amd_graph_counts", "sdot_amd_solve_bytes", "sdot_amd_server_submit_nd", "sdot_amd_server_info" ]
 
 CONSTANT, IMAGE, GAUSSIANS = 0, 1, 2
 
@@ -78,11 +78,19 @@ def lib():
         L.sdot_amd_server_start.restype = _LL
         L.sdot_amd_server_submit.argtypes = [ _LL, _LL, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_server_wait.argtypes = [ _LL, _LL, _P, _P ]
+        L.sdot_amd_server_submit_nd.argtypes = [ _LL, _I, _I, _LL, _P, _P, _P, _P, _P, _I, _D, _P, _I, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_server_submit_nd.restype = _I
+        L.sdot_amd_server_info.argtypes = [ _LL, _P ]
+        L.sdot_amd_server_info.restype = _I
         L.sdot_amd_server_stop.argtypes = [ _LL ]
         L.sdot_amd_set_timing.argtypes = [ _I ]
         L.sdot_amd_set_timing.restype = _I
         L.sdot_amd_kernel_times.argtypes = [ _P, _P, _P, _P, _I ]
         L.sdot_amd_kernel_times.restype = _I
+        L.sdot_amd_graph_counts.argtypes = [ _P, _P ]
+        L.sdot_amd_graph_counts.restype = _I
+        L.sdot_amd_solve_bytes.argtypes = [ _LL, _I, _I, _P, _I, _P ]
+        L.sdot_amd_solve_bytes.restype = _I
         for name in ( "sdot_amd_measures_2d", "sdot_amd_solve_2d", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
                       "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_submit", "sdot_amd_server_wait",
                       "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d", "sdot_amd_solve_2d_precision" ):
@@ -394,6 +402,32 @@ def kernel_times( reset = False ):
                                          int( bool( reset ) ) ) )
     return ( { k: ( float( ms[ i ] ), int( la[ i ] ) ) for i, k in enumerate( TIMED_KERNELS ) }, float( sb[ 0 ] ),
              dict( zip( ( "side_tests", "prune_vertex_tests", "new_vertices", "node_visits", "cuts" ), [ int( x ) for x in ev ] ) ) )
+
+
+def graph_counts():
+    """the multigrid's graphs since the last reset of the kernel times: `( captured, replayed )`"""
+    c, r = _LL(), _LL()
+    _check( lib().sdot_amd_graph_counts( ctypes.byref( c ), ctypes.byref( r ) ) )
+    return int( c.value ), int( r.value )
+
+
+#: the parts of `solve_bytes`, in the ABI's order
+MEMORY_PARTS = [ "tree", "diagram slots", "step edges", "fp32 polygons / 3D polytopes", "overflow slots", "newton vectors", "moments",
+                 "linear solver", "density copies", "reductions" ]
+
+
+def solve_bytes( n, dim, density_kind, options = None, precision = 0 ):
+    """what a solve of `n` seeds takes from the device, per part ( `{ part: bytes }`, `MEMORY_PARTS` )"""
+    parts = np.zeros( len( MEMORY_PARTS ) )
+    op = None if options is None else np.ascontiguousarray( np.asarray( options, dtype = np.float64 ) )
+    _check( lib().sdot_amd_solve_bytes( int( n ), int( dim ), int( density_kind ), None if op is None else op.ctypes.data_as( _P ), int( precision ),
+                                        parts.ctypes.data_as( _P ) ) )
+    return dict( zip( MEMORY_PARTS, parts.tolist() ) )
+
+
+def release():
+    """frees the device memory the library's arena holds ( for the calling thread's device )"""
+    lib().sdot_amd_release()
 
 
 def device_bytes():

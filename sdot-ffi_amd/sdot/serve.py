@@ -6,7 +6,7 @@
 
 The reference solves one problem per call and has no server. Here each slot is a HIP stream driven by its own host
 thread with its own device buffers ( kept between its requests ); a free slot takes the oldest waiting request whose
-estimated footprint fits the memory budget beside what the other slots hold. Requests that arrive while others run are
+footprint ( the byte model `sdot.memory.solve_bytes`, part by part ) fits the memory budget beside what the other slots hold. Requests that arrive while others run are
 admitted as soon as a slot and the memory allow, and their work overlaps on the device. A request's plan does not
 depend on what ran beside it ( the kernels are deterministic and nothing is shared ). `ticket.times` gives its arrival,
 start and end ( seconds from the server's start ): the latency a caller sees is `end - arrival`.
@@ -76,40 +76,61 @@ class Server:
         self._open = set()
 
     def submit( self, problem, settings = None ):
-        """queues the solve of `problem` ( a 2D `OtProblem` ) with `settings` ( an `Iterative` ); returns its `Ticket`"""
+        """queues the solve of `problem` ( a 2D or 3D `OtProblem` ) with `settings` ( an `Iterative` ); returns its `Ticket`.
+        The cells' float follows `settings.precision` as a single solve's does ( 2D: 'auto' and 'mixed' the FP32 kernel then
+        the FP64 one; 3D: FP64 cells ), so a served plan is the plan `problem.solve( settings )` returns"""
         from .OtProblem import Iterative
+        from .SdotPlanNd import _KERNEL_FLOAT
         settings = settings if settings is not None else Iterative()
-        if settings.precision in ( "fp32", "mixed" ):
-            raise NotImplementedError( "served solves run FP64 cells ( precision 'auto' or 'fp64' )" )
-        if problem.nb_dims != 2:
-            raise NotImplementedError( f"Server: 2D problems on the MI355X path ( not { problem.nb_dims }D )" )
+        d = problem.nb_dims
+        if d not in ( 2, 3 ):
+            raise NotImplementedError( f"Server: 2D or 3D problems on the MI355X path ( not { d }D )" )
         src, dst = problem.source, problem.target
-        if not isinstance( dst, ( Image, UniformBox, SumOfGaussians ) ):
-            raise NotImplementedError( f"Server: an Image, a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
         problem.domain
         T = _native.torch()
         dev = _native.device()
-        pos = _native.on_device( src.positions ).reshape( -1, 2 ).contiguous()
+        pos = _native.on_device( src.positions ).reshape( -1, d ).contiguous()
         n = int( pos.shape[ 0 ] )
         nu = _native.on_device( src.weights ).reshape( -1 ).contiguous()
         w0 = None if settings.weights0 is None else _native.on_device( settings.weights0 ).reshape( -1 ).contiguous()
         lo, hi = dst.box()
-        box = np.asarray( [ lo[ 0 ], lo[ 1 ], hi[ 0 ], hi[ 1 ] ], dtype = np.float64 )
         opts = np.ascontiguousarray( card_options( settings, dst ) )
-        kind, rho, img, nx, ny, geo = dst.native_args( device_tensor = _native.on_device )
-        keep = dict( pos = pos, nu = nu, w0 = w0, img = img, box = box, opts = opts,
+        if d == 3:
+            if settings.precision == "fp32":
+                raise NotImplementedError( "Server: 3D cells are FP64 ( precision 'auto', 'fp64' or 'mixed' )" )
+            if not isinstance( dst, ( UniformBox, SumOfGaussians ) ):
+                raise NotImplementedError( f"Server, 3D: a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
+            box = np.asarray( [ *lo[ :3 ], *hi[ :3 ] ], dtype = np.float64 )
+            opts[ _native.CARD_OPTIONS.index( "step" ) ] = 0.0
+            kind, rho, dens, nd = dst.native_args3( device_tensor = _native.on_device )
+            precision = _native.FP64
+        else:
+            if not isinstance( dst, ( Image, UniformBox, SumOfGaussians ) ):
+                raise NotImplementedError( f"Server: an Image, a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
+            box = np.asarray( [ lo[ 0 ], lo[ 1 ], hi[ 0 ], hi[ 1 ] ], dtype = np.float64 )
+            kind, rho, dens, nx, ny, geo = dst.native_args( device_tensor = _native.on_device )
+            nd = nx
+            precision = _KERNEL_FLOAT[ settings.precision ]
+        keep = dict( pos = pos, nu = nu, w0 = w0, img = dens, box = box, opts = opts,
                      weights = T.empty( n, dtype = T.float64, device = dev ), masses = T.empty( n, dtype = T.float64, device = dev ),
-                     bary = T.empty( ( n, 2 ), dtype = T.float64, device = dev ), stats = np.zeros( 31 ), cost = np.zeros( 1 ) )
+                     bary = T.empty( ( n, d ), dtype = T.float64, device = dev ), stats = np.zeros( 31 ), cost = np.zeros( 1 ) )
         T.cuda.current_stream().synchronize()           # ( the inputs are on the device before a slot reads them )
         ticket = ctypes.c_longlong( 0 )
         P = _native._ptr
-        _native._check( _native.lib().sdot_amd_server_submit( self._h, n, P( pos ), P( nu ), P( w0 ), opts.ctypes.data_as( _native._P ),
-                                                              box.ctypes.data_as( _native._P ), kind, rho, P( img ), P( keep[ "weights" ] ),
-                                                              P( keep[ "masses" ] ), P( keep[ "bary" ] ), keep[ "stats" ].ctypes.data_as( _native._P ),
-                                                              keep[ "cost" ].ctypes.data_as( _native._P ), ctypes.byref( ticket ) ) )
+        _native._check( _native.lib().sdot_amd_server_submit_nd( self._h, d, precision, n, P( pos ), P( nu ), P( w0 ), opts.ctypes.data_as( _native._P ),
+                                                                 box.ctypes.data_as( _native._P ), kind, rho, P( dens ), int( nd ),
+                                                                 P( keep[ "weights" ] ), P( keep[ "masses" ] ), P( keep[ "bary" ] ),
+                                                                 keep[ "stats" ].ctypes.data_as( _native._P ),
+                                                                 keep[ "cost" ].ctypes.data_as( _native._P ), ctypes.byref( ticket ) ) )
         t = Ticket( self, ticket.value, problem, settings, keep )
         self._open.add( t )
         return t
+
+    def info( self ):
+        """`{ "peak_bytes": the largest sum of the slots' device bytes seen at the end of a request, "done": requests done }`"""
+        out = np.zeros( 2 )
+        _native._check( _native.lib().sdot_amd_server_info( self._h, out.ctypes.data_as( _native._P ) ) )
+        return dict( peak_bytes = float( out[ 0 ] ), done = int( out[ 1 ] ) )
 
     def close( self ):
         """waits for every open request, then stops the slots"""

@@ -38,23 +38,57 @@ def clouds( rng, n = 3000 ):
 # ---- the power diagram by lifting -------------------------------------------------------------------------------
 
 
-def lifted_neighbours( pos, w, lo = ( 0, 0 ), hi = ( 1, 1 ) ):
-    """the neighbour pairs `{ ( i, j ) }` of the power diagram restricted to the box, as the lower convex hull of the
-    lifted points `( p, |p|^2 - w )` gives them ( a regular triangulation ), keeping the pairs whose common facet crosses
-    the box. Independent of the oracle: scipy's Qhull."""
+def _clip_length( a, d, t1, lo, hi ):
+    """the length of `{ a + t d : 0 <= t <= t1 } ∩ box` ( `t1` may be `inf`: a ray ), by Liang-Barsky"""
+    t0 = 0.0
+    for k in range( 2 ):
+        if d[ k ] == 0:
+            if a[ k ] < lo[ k ] or a[ k ] > hi[ k ]:
+                return 0.0
+            continue
+        ta, tb = ( lo[ k ] - a[ k ] ) / d[ k ], ( hi[ k ] - a[ k ] ) / d[ k ]
+        t0, t1 = max( t0, min( ta, tb ) ), min( t1, max( ta, tb ) )
+        if t1 <= t0:
+            return 0.0
+    return ( t1 - t0 ) * float( np.hypot( d[ 0 ], d[ 1 ] ) )
+
+
+def lifted_neighbours( pos, w, lo = ( 0, 0 ), hi = ( 1, 1 ), margin = 1e-10, ambiguous = False ):
+    """the neighbour pairs `{ ( i, j ) }` of the power diagram restricted to the box `lo, hi`, from the lower convex hull of
+    the lifted points `( p, |p|^2 - w )` ( a regular triangulation: scipy's Qhull, independent of the oracle ). A hull edge's
+    power facet is the segment between the power centres of its two lower faces ( a face `z = a x + b y + c` has its centre
+    at `( a / 2, b / 2 )` ), or the ray from its one face's centre away from that face's third vertex; a pair is kept when
+    that facet meets the box over a length above `margin`. With `ambiguous`, also returns the pairs whose facet meets it over
+    a length in `( 0, margin ]`, which rounding may keep or drop."""
     from scipy.spatial import ConvexHull
     pos = np.asarray( pos, dtype = np.float64 )
     w = np.zeros( len( pos ) ) if w is None else np.asarray( w, dtype = np.float64 )
     lift = np.column_stack( [ pos, ( pos ** 2 ).sum( 1 ) - w ] )
     hull = ConvexHull( lift )
-    pairs = set()
+    faces = {}                                           # edge -> [ ( centre, third vertex ) ]
     for simplex, eq in zip( hull.simplices, hull.equations ):
         if eq[ 2 ] >= 0:                                 # upper faces are not in the regular triangulation
             continue
+        c = np.array( [ -eq[ 0 ] / eq[ 2 ], -eq[ 1 ] / eq[ 2 ] ] ) / 2
         for a in range( 3 ):
-            i, j = int( simplex[ a ] ), int( simplex[ ( a + 1 ) % 3 ] )
-            pairs.add( ( min( i, j ), max( i, j ) ) )
-    return pairs
+            i, j, k = int( simplex[ a ] ), int( simplex[ ( a + 1 ) % 3 ] ), int( simplex[ ( a + 2 ) % 3 ] )
+            faces.setdefault( ( min( i, j ), max( i, j ) ), [] ).append( ( c, k ) )
+    pairs, amb = set(), set()
+    for ( i, j ), fl in faces.items():
+        if len( fl ) >= 2:
+            length = _clip_length( fl[ 0 ][ 0 ], fl[ 1 ][ 0 ] - fl[ 0 ][ 0 ], 1.0, lo, hi )
+        else:
+            c, k = fl[ 0 ]
+            e = pos[ j ] - pos[ i ]
+            nrm = np.array( [ e[ 1 ], -e[ 0 ] ] )
+            if nrm @ ( pos[ k ] - pos[ i ] ) > 0:
+                nrm = -nrm
+            length = _clip_length( c, nrm, np.inf, lo, hi )
+        if length > margin:
+            pairs.add( ( i, j ) )
+        elif length > 0:
+            amb.add( ( i, j ) )
+    return ( pairs, amb ) if ambiguous else pairs
 
 
 def halfspace_cell( pos, w, i, lo = ( 0, 0 ), hi = ( 1, 1 ) ):

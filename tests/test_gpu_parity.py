@@ -114,15 +114,21 @@ def test_image_facets_match_the_oracle( native, oracle ):
     assert np.abs( fc - oc ).max() <= 1e-12 * np.abs( oc ).max()
 
 
-def test_neighbour_sets_match_the_lifted_hull( native ):
+@pytest.mark.parametrize( "label", [ "uniform 300", "uniform", "lines", "lines weighted" ] )
+def test_neighbour_sets_equal_the_lifted_hull( native, label ):
+    """identical neighbour sets on generic inputs: the pairs whose power facet meets the box ( scipy's lifted hull ), no more
+    and no fewer, up to facets shorter than 1e-10 that rounding may keep or drop"""
     rng = np.random.default_rng( 5 )
-    n = 300
-    pos = rng.uniform( 0.05, 0.95, ( n, 2 ) )
-    w = rng.uniform( -2e-3, 2e-3, n )
-    say( "case neighbours" )
+    if label == "uniform 300":
+        pos = rng.uniform( 0.05, 0.95, ( 300, 2 ) )
+        w = rng.uniform( -2e-3, 2e-3, 300 )
+    else:
+        pos, w = next( ( p, ww ) for lab, p, ww in H.clouds( rng, 3000 ) if lab == label )
+    say( f"case neighbours: { label }" )
     _, ( fi, fj, _ ) = _gpu_measures( native, pos, w, facets = True )
     ours = { ( min( i, j ), max( i, j ) ) for i, j in zip( fi.tolist(), fj.tolist() ) }
-    assert ours <= H.lifted_neighbours( pos, w )
+    hull, amb = H.lifted_neighbours( pos, w, ambiguous = True )
+    assert hull <= ours <= hull | amb, ( len( hull - ours ), len( ours - hull - amb ) )
 
 
 def test_overflowed_cells_go_through_the_second_pass( native, oracle ):
@@ -550,15 +556,45 @@ def test_server_results_are_those_of_single_solves( native ):
 
 
 def test_server_admission_keeps_to_the_memory_budget( native ):
-    # a budget that holds one request at a time: the requests run one after the other, in arrival order
-    from sdot import Iterative, Server
+    # a budget that holds one request at a time: the requests run one after the other, in arrival order, and the slots'
+    # device bytes together ( sampled at every request's end ) never pass the budget
+    from sdot import Iterative, Server, memory
     probs = _batch_problems( k = 4, seed = 13 )
     it = Iterative( precision = "fp64", tol = 1e-9, continuation = "never" )
-    say( "case server with a small budget" )
-    with Server( slots = 4, memory = 1.0 ) as srv:
-        tickets = [ srv.submit( pb, it ) for pb in probs ]
-        plans = [ t.result() for t in tickets ]
-    assert all( p.converged for p in plans )
-    times = sorted( t.times[ 1: ] for t in tickets )
-    for ( s0, e0 ), ( s1, e1 ) in zip( times, times[ 1: ] ):
-        assert e0 <= s1
+    need = [ sum( memory.solve_bytes( pb.nb_diracs, 2, pb.target, it ).values() ) for pb in probs ]
+    budget = 1.2 * max( need )
+    assert 2 * min( need ) > budget
+    say( f"case server with a budget of { budget / 1e6 :.0f} MB for requests of { [ round( b / 1e6 ) for b in need ] } MB" )
+    for mem in ( 1.0, budget ):
+        with Server( slots = 3, memory = mem ) as srv:
+            tickets = [ srv.submit( pb, it ) for pb in probs ]
+            plans = [ t.result() for t in tickets ]
+            info = srv.info()
+        assert all( p.converged for p in plans ) and info[ "done" ] == len( probs )
+        times = sorted( t.times[ 1: ] for t in tickets )
+        for ( s0, e0 ), ( s1, e1 ) in zip( times, times[ 1: ] ):
+            assert e0 <= s1
+        assert info[ "peak_bytes" ] <= max( budget, max( need ) ), ( info, budget )
+
+
+def test_solve_bytes_holds_the_real_footprint( native ):
+    # the byte model against what the library really took for a solve ( its arena, emptied first ): never under it, and
+    # at most 10 % above on these generic clouds
+    import sdot
+    from sdot import memory
+    rng = np.random.default_rng( 21 )
+    img = sdot.Image( H.bumps_image( 128, 128, rng, k = 4 ), origin = [ 0, 0 ], frame = np.eye( 2 ) / 128 )
+    cases = [ ( "2D image, mixed, limits", 2, 60_000, img, sdot.Iterative( tol = 1e-9, continuation = "never" ) ),
+              ( "2D constant, fp64, trials", 2, 30_000, sdot.UniformBox( [ 0, 0 ], [ 1, 1 ] ),
+                sdot.Iterative( tol = 1e-9, precision = "fp64", tuning = sdot.Tuning( step = "trials" ) ) ),
+              ( "3D Gaussians", 3, 4_000, sdot.SumOfGaussians( rng.uniform( 0.3, 0.7, ( 2, 3 ) ), [ 0.15, 0.2 ], support_box = ( [ 0, 0, 0 ], [ 1, 1, 1 ] ) ),
+                sdot.Iterative( tol = 1e-9 ) ) ]
+    for label, d, n, dens, it in cases:
+        say( f"case byte model: { label }, n = { n }" )
+        native.release()
+        pos = rng.uniform( 0, 1, ( n, d ) )
+        sol = sdot.OtProblem( sdot.SumOfDiracs( pos ), dens ).solve( it )
+        assert sol.converged
+        real = native.device_bytes()
+        model = sum( memory.solve_bytes( n, d, dens, it ).values() )
+        assert real <= model <= 1.10 * real, ( label, real, model )

@@ -92,5 +92,7 @@ def test_mixed_switches_to_fp64_where_fp32_stagnates( native ):
         res[ precision ] = sdot.OtProblem( sdot.SumOfDiracs( pos ), sdot.UniformBox( [ 0, 0 ], [ 1, 1 ] ) ).solve( st ).stats
     f, m = res[ "fp32" ], res[ "mixed" ]
     assert f[ "it_double" ] == 0 and f[ "status" ] != "converged"
-    assert 0 < m[ "it_double" ] <= m[ "nb_iter" ]
+    # the reference's accounting ( `gpu/Newton2D.cuh:1701-1721` ): the switch uses up the iteration it happens in, which stays
+    # counted, and `it_double` is that iteration's index ( one stage here, so `nb_iter` counts it and those after it )
+    assert 0 < m[ "it_double" ] and m[ "it_double" ] + 1 <= m[ "nb_iter" ]
     assert m[ "residual" ] < f[ "residual" ]
