@@ -35,6 +35,8 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic flops of the cell kernel's events ( 2D ): a plane / vertex side test, one vertex of a pruning test,
 # a new vertex ( intersection )
 FLOP_SIDE, FLOP_PRUNE, FLOP_NEWV = 4, 11, 10
+# the same in 3D
+FLOP3_SIDE, FLOP3_PRUNE, FLOP3_NEWV = 6, 16, 40
 
 
 def workload( n = 1_000_000, res = 1024 ):
@@ -174,6 +176,14 @@ def bench_3d( args ):
     spmv_avg_ms = spmv_ms / max( spmv_launches, 1 )
     spmv_gbs = ( spmv_bytes / max( spmv_launches, 1 ) ) / ( spmv_avg_ms * 1e-3 ) / 1e9 if spmv_launches else 0.0
     cells_ms, cells_launches = times[ "cells" ]
+    # the cell kernels ( walk and cuts, then the Gaussian measures ) against the FP64 VALU peak: the algorithmic flops of their
+    # events ( 3D: 6 a side test, 16 a pruning vertex test, 40 a new vertex ), over every diagram of the timed solves
+    flops3 = FLOP3_SIDE * ev[ "side_tests" ] + FLOP3_PRUNE * ev[ "prune_vertex_tests" ] + FLOP3_NEWV * ev[ "new_vertices" ]
+    cells_avg_ms = cells_ms / max( cells_launches, 1 )
+    cells_tf = ( flops3 / max( cells_launches, 1 ) ) / ( cells_avg_ms * 1e-3 ) / 1e12 if cells_launches else 0.0
+    roof_cells = dict( kernel = "k_cells3 + k_cells3_measure", bound = "fp64-valu", achieved = cells_tf, peak = FP64_PEAK_TFLOPS, unit = "TFLOP/s",
+                       frac = cells_tf / FP64_PEAK_TFLOPS, traffic = None, avg_launch_ms = cells_avg_ms, launches = cells_launches,
+                       flops_per_launch = flops3 / max( cells_launches, 1 ) )
     line = dict( metric = METRIC, value = args.n * args.steps / elapsed, unit = "Diracs/s", n_gpus = 1, steps = args.steps, warmup = args.warmup,
                  ms_per_step = 1e3 * elapsed / args.steps, higher_is_better = True, scaling = "weak", vs_baseline = None, dtype = "f64",
                  data = "synthetic",
@@ -181,7 +191,8 @@ def bench_3d( args ):
                                            f"max mass error <= { args.tol:g} ( absolute )",
                                 n = args.n, tol_abs = args.tol, start = "cold ( Voronoi ), tree included", step = "trials", continuation = "auto",
                                 linear = "jacobi-pcg" if args.linear == "cg" else "multigrid flexible CG", cells = "fp64" ),
-                 roofline = dict( kernel = "k_cg_spmv_dot" if args.linear == "cg" else "k_fcg_spmv", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s",
+                 roofline = roof_cells,
+                 roofline_secondary = dict( kernel = "k_cg_spmv_dot" if args.linear == "cg" else "k_fcg_spmv", bound = "hbm", achieved = spmv_gbs, peak = HBM_PEAK_GBS, unit = "GB/s",
                                   frac = spmv_gbs / HBM_PEAK_GBS, traffic = None, avg_launch_ms = spmv_avg_ms, launches_timed = spmv_launches,
                                   bytes_per_launch = spmv_bytes / max( spmv_launches, 1 ) ),
                  cells = dict( kernels_ms_per_diagram = cells_ms / max( cells_launches, 1 ), diagrams = cells_launches,
@@ -216,7 +227,8 @@ def main():
     ap.add_argument( "--batch-k", type = int, default = 256 )
     ap.add_argument( "--batch-res", type = int, default = 256 )
     ap.add_argument( "--streams", type = int, default = 8 )
-    ap.add_argument( "--no-plain", action = "store_true", help = "batch: skip the plain FP64 one-instance-per-call leg" )
+    ap.add_argument( "--no-plain", action = "store_true", help = "skip the plain FP64 one-instance-per-call leg ( vs_baseline )" )
+    ap.add_argument( "--no-lebesgue", action = "store_true", help = "skip the 1M uniform Lebesgue figures" )
     ap.add_argument( "--steps", type = int, default = 3 )
     ap.add_argument( "--warmup", type = int, default = 1 )
     ap.add_argument( "--n", type = int, default = 1_000_000 )
@@ -324,7 +336,8 @@ def main():
     dominant, other = ( roof_spmv, roof_cells ) if spmv_total_ms > cells_ms else ( roof_cells, roof_spmv )
 
     line = dict( metric = METRIC, value = args.n * args.steps / elapsed, unit = "Diracs/s", n_gpus = 1, steps = args.steps, warmup = args.warmup,
-                 ms_per_step = ms_step, higher_is_better = True, scaling = "weak", vs_baseline = None, dtype = "f64", data = "synthetic",
+                 ms_per_step = ms_step, higher_is_better = True, scaling = "weak", vs_baseline = None,
+                 dtype = "f32+f64" if kf != _native.FP64 else "f64", data = "synthetic",
                  config = dict( workload = f"2D { args.n } Diracs uniform in [0,1]^2, { args.res }^2 image of 8 Gaussian bumps + 0.1 floor, "
                                            f"max mass error <= { args.tol:g} ( absolute )",
                                 n = args.n, image = [ args.res, args.res ], tol_abs = args.tol, start = "cold ( Voronoi ), tree included",
@@ -336,6 +349,51 @@ def main():
                                t_diag = st[ "t_diag" ], t_majorant = st[ "t_majorant" ], t_lin = st[ "t_lin" ], t_total = st[ "t_total" ],
                                device_bytes = _native.device_bytes() ) )
 
+    # the FP64 cells of the returned weights: the error of the plan against the FP64 kernel ( the oracle's cells: equal to the
+    # bit on a constant density, to 1e-14 relative on an image, by the parity tests )
+    w_fin = sols[ -1 ].weights
+    m64 = _native.measures_2d( pos, w_fin, [ 0, 0, 1, 1 ], target.normalized_version() )
+    err64 = float( ( m64 - torch.as_tensor( sols[ -1 ].target_masses, device = dev ) ).abs().max() )
+    line[ "solve" ][ "fp64_cells_residual_abs" ] = err64
+    line[ "solve" ][ "fp64_cells_residual_rel" ] = err64 / nu
+    line[ "solve" ][ "reported_vs_fp64_cells" ] = float( ( m64 - sols[ -1 ].cell_masses ).abs().max() )
+    del m64
+    # THE PLAIN PATH: the same solve with FP64 cells, one instance per call ( the metric's baseline, `vs_baseline` )
+    if not args.no_plain:
+        plain = sdot.Iterative( tol = args.tol, continuation = args.continuation, on_failure = "ignore", precision = "fp64",
+                                tuning = sdot.Tuning( step = args.step, linear_solver = args.linear, linear_tol = args.linear_tol ) )
+        tp = []
+        for r in range( 2 ):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            sp = sdot.OtProblem( sdot.SumOfDiracs( pos ), target ).solve( plain )
+            torch.cuda.synchronize()
+            tp.append( time.perf_counter() - t1 )
+            print( f"bench: plain FP64 path: { sp.stats[ 'status' ] }, { sp.stats[ 'nb_iter' ] } it, { tp[ -1 ]:.3f} s", file = sys.stderr, flush = True )
+        t_plain = min( tp )
+        line[ "vs_baseline" ] = line[ "value" ] / ( args.n / t_plain )
+        line[ "plain_fp64" ] = dict( seconds = t_plain, diracs_per_s = args.n / t_plain, status = sp.stats[ "status" ], nb_iter = sp.stats[ "nb_iter" ],
+                                     lin_nb_iter = sp.stats[ "lin_nb_iter" ], what = "the same workload, FP64 cells, one solve per call ( min of 2 )" )
+    # the 1M uniform Lebesgue workload ( the reference's published 1M figures are on it ): the same 1M seeds, the unit square
+    if not args.no_lebesgue and args.n == 1_000_000:
+        leb = {}
+        box = sdot.UniformBox( [ 0, 0 ], [ 1, 1 ] )
+        for prec in ( "auto", "fp64" ):
+            st_l = sdot.Iterative( tol = args.tol, continuation = "never", on_failure = "ignore", precision = prec,
+                                   tuning = sdot.Tuning( linear_solver = args.linear ) )
+            ts = []
+            for r in range( 3 ):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                sl = sdot.OtProblem( sdot.SumOfDiracs( pos ), box ).solve( st_l )
+                torch.cuda.synchronize()
+                ts.append( time.perf_counter() - t1 )
+            leb[ "mixed" if prec == "auto" else prec ] = dict( seconds = min( ts[ 1: ] ), status = sl.stats[ "status" ], nb_iter = sl.stats[ "nb_iter" ],
+                                                            lin_nb_iter = sl.stats[ "lin_nb_iter" ], residual_abs = sl.stats[ "residual" ] )
+            print( f"bench: 1M uniform Lebesgue ( { prec } ): { min( ts[ 1: ] ):.3f} s, { sl.stats[ 'nb_iter' ] } it", file = sys.stderr, flush = True )
+        leb[ "what" ] = ( "the same 1M seeds, the Lebesgue measure of the unit square, tol 1e-9 absolute, cold start, min of 2 after a "
+                          "warm-up; the reference publishes 0.383 s ( mixed, A6000 ) and 0.839 s ( FP64, 2080 Ti ) on this workload" )
+        line[ "uniform_lebesgue_1M" ] = leb
     if not args.no_cpu_baseline:
         line[ "cpu_baseline" ] = cpu_baseline( pos_h, img_h, args )
     print( json.dumps( line ), flush = True )

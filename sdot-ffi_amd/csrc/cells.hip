@@ -740,9 +740,10 @@ void cells_scratch( DiagramOut &out, I64 n, const std::string &tag, hipStream_t 
     Arena &A = arena();
     out.counters = A.get<unsigned>( tag + ".counters", 8 );
     out.ovf_list = A.get<int>( tag + ".ovf_list", 2 * size_t( n ) );
-    if ( ! out.events ) {
-        out.events = A.get<unsigned long long>( "events", 8 );
-        SDOT_HIP( hipMemsetAsync( out.events, 0, 8 * sizeof( unsigned long long ), st ) );
+    if ( ! out.events ) {                                // ( the counters run on from solve to solve: zeroed when made, and by the
+        bool fresh = false;                              // timing's reset )
+        out.events = A.get<unsigned long long>( "events", 8, &fresh );
+        if ( fresh ) SDOT_HIP( hipMemsetAsync( out.events, 0, 8 * sizeof( unsigned long long ), st ) );
     }
 }
 

@@ -113,11 +113,13 @@ struct Arena {
         return reinterpret_cast<T *>( b.ptr );
     }
 
+    /// the buffer `name` of at least `count` elements; `fresh`, when given, tells whether it was ( re )made by this call
     template<class T>
-    T *get( const std::string &name, size_t count ) {
+    T *get( const std::string &name, size_t count, bool *fresh = nullptr ) {
         std::lock_guard<std::mutex> lk( mtx );
         const size_t bytes = std::max<size_t>( count * sizeof( T ), 256 );
         Buf &b = bufs[ name ];
+        if ( fresh ) *fresh = b.bytes < bytes;
         if ( b.bytes < bytes && fix_dev ) {
             b.ptr = carve( fix_dev, fix_dev_bytes, fix_dev_used, bytes, name );
             b.bytes = bytes;
