@@ -78,6 +78,19 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
                           double *masses, long long *facet_i, long long *facet_j, double *facet_c, long long facet_cap,
                           long long *nb_facets, unsigned long long *events );
 
+/* ONE DIAGRAM'S MASSES WITHOUT ALLOCATION. No counterpart in the reference: this entry point is for callers that cannot allocate
+ * device memory ( code under HIP graph capture, tight serving loops, harnesses that forbid `hipMalloc` ). It gives the masses of
+ * `sdot_amd_measures_2d` ( the same arguments without the facets; the same values, to the bit ) and allocates nothing, device
+ * or pinned host: its scratch ( the tree, the majorants, the rank-order masses, the overflow lists and slots ) is a fixed device
+ * array of 40 MiB that each GPU the library runs on holds until the process ends. At most
+ * `sdot_amd_measures_2d_noalloc_max_seeds()` seeds ( 131072 ); more is an invalid argument ( kind 1 ). Calls on one GPU share
+ * that GPU's array: the library serializes them ( a lock held for the whole call, which returns once the masses are in place ),
+ * and a caller must not rely on two calls on one GPU overlapping. */
+int       sdot_amd_measures_2d_noalloc( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                                        int density_kind, double rho, const double *img, int nx, int ny, const double *img_geo,
+                                        double *masses, unsigned long long *events );
+long long sdot_amd_measures_2d_noalloc_max_seeds( void );
+
 /* THE SOLVE: weights `w` such that every cell's mass matches its target `nu_i`, by the damped Newton of
  * Kitagawa-Merigot-Thibert, FP64 cells; the step by trials or by the limits of the cells ( option `step` ),
  * directions by Jacobi-PCG or multigrid-PCG ( option `lin` ), the width continuation of an image or of

@@ -8,6 +8,7 @@
 #include "batch.h"
 #include "server.h"
 #include "memory.h"
+#include "noalloc.h"
 
 #include <cmath>
 
@@ -209,6 +210,19 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
         measures_2d( hipStream_t( stream ), n, positions, weights, box, dn, masses, facet_i, facet_j, facet_c, facet_cap, nb_facets, events );
     } );
 }
+
+int sdot_amd_measures_2d_noalloc( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                                  int density_kind, double rho, const double *img, int nx, int ny, const double *img_geo, double *masses,
+                                  unsigned long long *events ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && masses, "measures: positions and masses are required" );
+        check_box( box );
+        const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
+        measures_2d_noalloc( hipStream_t( stream ), n, positions, weights, box, dn, masses, events );
+    } );
+}
+
+long long sdot_amd_measures_2d_noalloc_max_seeds( void ) { return (long long) NOALLOC_MAX_SEEDS; }
 
 int sdot_amd_solve_2d_precision( void *stream, long long n, const double *positions, const double *nu, const double *w0, const long long *dups,
                                  long long nb_dup_values, const double *options, const double *box, int density_kind, double rho,

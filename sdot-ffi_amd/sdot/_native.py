@@ -31,7 +31,8 @@ EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_byte
             "sdot_amd_solve_2d", "sdot_amd_tree_nodes", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
             "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_start", "sdot_amd_server_submit",
             "sdot_amd_server_wait", "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d",
-            "sdot_amd_solve_2d_precision", "sdot_amd_graph_counts", "sdot_amd_solve_bytes", "sdot_amd_server_submit_nd", "sdot_amd_server_info" ]
+            "sdot_amd_solve_2d_precision", "sdot_amd_graph_counts", "sdot_amd_solve_bytes", "sdot_amd_server_submit_nd", "sdot_amd_server_info",
+            "sdot_amd_measures_2d_noalloc", "sdot_amd_measures_2d_noalloc_max_seeds" ]
 
 CONSTANT, IMAGE, GAUSSIANS = 0, 1, 2
 
@@ -66,6 +67,8 @@ def lib():
         L.sdot_amd_tree_nodes.restype = _LL
         L.sdot_amd_tree_nodes.argtypes = [ _LL, _I ]
         L.sdot_amd_measures_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P, _P, _LL, _P, _P ]
+        L.sdot_amd_measures_2d_noalloc.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P ]
+        L.sdot_amd_measures_2d_noalloc_max_seeds.restype = _LL
         L.sdot_amd_solve_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _LL, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_solve_2d_precision.argtypes = [ _P, _LL, _P, _P, _P, _P, _LL, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I ]
         L.sdot_amd_build_tree_2d.argtypes = [ _P, _LL, _P, _I, _P, _P, _P, _P, _P, _P, _P ]
@@ -93,7 +96,8 @@ def lib():
         L.sdot_amd_solve_bytes.restype = _I
         for name in ( "sdot_amd_measures_2d", "sdot_amd_solve_2d", "sdot_amd_build_tree_2d", "sdot_amd_refresh_majorants_2d",
                       "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_submit", "sdot_amd_server_wait",
-                      "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d", "sdot_amd_solve_2d_precision" ):
+                      "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d", "sdot_amd_solve_2d_precision",
+                      "sdot_amd_measures_2d_noalloc" ):
             getattr( L, name ).restype = _I
         if L.sdot_amd_abi_version() != 1:
             raise RuntimeError( "libsdot_amd.so: unexpected ABI version" )
@@ -179,6 +183,23 @@ def measures_2d( positions, weights, box, density, facets = False, events = Fals
     if events:
         res.append( dict( zip( ( "side_tests", "prune_vertex_tests", "new_vertices", "node_visits", "cuts" ), [ int( x ) for x in ev ] ) ) )
     return res[ 0 ] if len( res ) == 1 else tuple( res )
+
+
+def measures_2d_noalloc( positions, weights, box, density, masses = None ):
+    """the masses of `measures_2d` by the entry point that allocates nothing ( its scratch a fixed device array ): `masses`
+    ( a device tensor of `n` float64, given or made here: the caller's allocation, not the library's )"""
+    T = torch()
+    pos = on_device( positions ).reshape( -1, 2 ).contiguous()
+    n = pos.shape[ 0 ]
+    w = None if weights is None else on_device( weights ).reshape( -1 )
+    boxh, boxp = _host( box )
+    kind, rho, img, nx, ny, geo = density.native_args( device_tensor = on_device )
+    geoh, geop = _host( geo )
+    if masses is None:
+        masses = T.empty( n, dtype = T.float64, device = device() )
+    _check( lib().sdot_amd_measures_2d_noalloc( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop,
+                                                _ptr( masses ), None ) )
+    return masses
 
 
 def _density3_args( density ):
