@@ -35,6 +35,7 @@ def lib():
         _lib.oracle_measures.restype = ctypes.c_longlong
         _lib.oracle_cell.restype = ctypes.c_int
         _lib.oracle_cg.restype = ctypes.c_int
+        _lib.oracle_mg.restype = ctypes.c_int
         _lib.oracle_solve.restype = ctypes.c_int
         _lib.oracle_max_depth_for.restype = ctypes.c_int
         _lib.oracle_get_threads.restype = ctypes.c_int
@@ -186,6 +187,22 @@ def cg( n, fi, fj, fc, b, tol = 1e-12, maxit = 20000 ):
     return d, its
 
 
+def mg( n, fi, fj, fc, b, rank_of = None, nu = 3, tol = 1e-12, maxit = 20000 ):
+    """the multigrid-preconditioned CG on the Laplacian of the facets ( first level aggregated by `rank_of >> 3` );
+    returns `( d, iterations, levels )`"""
+    fi = np.ascontiguousarray( fi, dtype = np.int64 )
+    fj = np.ascontiguousarray( fj, dtype = np.int64 )
+    fc = _f64( fc )
+    b = _f64( b )
+    ro = None if rank_of is None else np.ascontiguousarray( rank_of, dtype = np.int64 )
+    d = np.zeros( n )
+    lv = np.zeros( 1, dtype = np.int32 )
+    its = lib().oracle_mg( ctypes.c_longlong( n ), ctypes.c_longlong( len( fi ) ), _l( fi ), _l( fj ), _d( fc ), _d( b ),
+                           None if ro is None else _l( ro ), ctypes.c_int( nu ), ctypes.c_double( tol ), ctypes.c_int( maxit ), _d( d ),
+                           lv.ctypes.data_as( ctypes.POINTER( ctypes.c_int ) ) )
+    return d, its, int( lv[ 0 ] )
+
+
 STATS = [ "status", "residual", "residual0", "nb_iter", "nb_diag", "nb_backtracks", "t_majorant", "t_diag", "t_asm", "t_lin", "t_lim", "eps",
           "domain_mass", "nb_overflowed", "nb_cell_lim", "nb_limit_rounds", "lin_nb_hierarchies", "lin_nb_iter", "lin_worst", "start", "t_total",
           "nb_continuation_steps", "min_start_mass", "it_switch", "it_double", "scratch_bytes",
@@ -196,10 +213,10 @@ HISTORY = [ "step", "t", "residual_l2", "min_measure", "max_abs_residual", "nb_d
 def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = None, tol = 1e-9, tol_rel = 0.0, maxit = 100, max_backtracks = 60,
            t_min = 1e-10, mult_ok = 4.0, residual = 1, power = 0.5, switch = 2.0, lin_tol = 1e-12, lin_maxit = 20000, keep_start = False,
            start_box = None, step = "trials", factor = 0.9, continuation = "never", conv_threshold = 1e-2, conv_s0 = 0.0,
-           conv_ratio = 2 ** 0.5, conv_min = 0.0, dups = None, gap = 0.0, margin = 0.0 ):
+           conv_ratio = 2 ** 0.5, conv_min = 0.0, dups = None, gap = 0.0, margin = 0.0, linear = "cg" ):
     """the damped-Newton solve ( step "trials" or "limits"; continuation "never", "auto" or "always" ); `dups`: the exact
     duplicates, `( dup, rep )` rows of user ids, whose weights are tied at `w_rep - k gap`; `margin`: the aggregation of
-    near-coincident pairs ( 0: off ); returns a dict ( weights, masses, barycenters, cost, stats, history, clusters )"""
+    near-coincident pairs ( 0: off ); `linear`: the Newton systems by Jacobi-PCG ( "cg" ) or the multigrid ( "mg" ); returns a dict ( weights, masses, barycenters, cost, stats, history, clusters )"""
     pos = _f64( pos )
     n = pos.shape[ 0 ]
     nu = np.full( n, 1.0 / n ) if nu is None else _f64( nu )
@@ -207,7 +224,7 @@ def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = No
     kind, value, nx, ny, geo, img, keep = _density_args( density )
     opts = _f64( [ tol, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol, lin_maxit, int( keep_start ),
                    { "trials": 0, "limits": 1 }[ step ], factor, { "never": 0, "auto": 1, "always": 2 }[ continuation ], conv_threshold,
-                   conv_s0, conv_ratio, conv_min, gap, 0 if dups is None else len( dups ), margin ] )
+                   conv_s0, conv_ratio, conv_min, gap, 0 if dups is None else len( dups ), margin, { "cg": 0, "mg": 1 }[ linear ] ] )
     dp = None if dups is None or len( dups ) == 0 else np.ascontiguousarray( np.asarray( dups, dtype = np.int64 ).reshape( -1, 2 ) )
     sb = _f64( [ 0, 0, 0, 0, 0 ] if start_box is None else [ 1, *start_box[ 0 ], *start_box[ 1 ] ] )
     weights = np.zeros( n )
@@ -308,15 +325,17 @@ def cell_3d( pos, i, w = None, lo = ( 0, 0, 0 ), hi = ( 1, 1, 1 ), brute_force =
 
 def solve_3d( pos, nu = None, w0 = None, lo = ( 0, 0, 0 ), hi = ( 1, 1, 1 ), density = None, tol = 1e-9, tol_rel = 0.0, maxit = 100,
               max_backtracks = 60, t_min = 1e-10, mult_ok = 4.0, residual = 1, power = 0.5, switch = 2.0, lin_tol = 1e-12, lin_maxit = 20000,
-              keep_start = False, continuation = "never", conv_threshold = 1e-2, conv_s0 = 0.0, conv_ratio = 2 ** 0.5, conv_min = 0.0 ):
-    """the 3D damped-Newton solve ( step by trials ); returns a dict ( weights, masses, barycenters, cost, stats, history )"""
+              keep_start = False, continuation = "never", conv_threshold = 1e-2, conv_s0 = 0.0, conv_ratio = 2 ** 0.5, conv_min = 0.0,
+              linear = "cg" ):
+    """the 3D damped-Newton solve ( step by trials ); `linear` as `solve`; returns a dict ( weights, masses, barycenters, cost, stats, history )"""
     pos = _f64( pos )
     n = pos.shape[ 0 ]
     nu = np.full( n, 1.0 / n ) if nu is None else _f64( nu )
     w0 = None if w0 is None else _f64( w0 )
     kind, value, ng, g = _density3_args( density )
     opts = _f64( [ tol, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol, lin_maxit, int( keep_start ),
-                   0, 0.9, { "never": 0, "auto": 1, "always": 2 }[ continuation ], conv_threshold, conv_s0, conv_ratio, conv_min, 0, 0, 0 ] )
+                   0, 0.9, { "never": 0, "auto": 1, "always": 2 }[ continuation ], conv_threshold, conv_s0, conv_ratio, conv_min, 0, 0, 0,
+                   { "cg": 0, "mg": 1 }[ linear ] ] )
     weights = np.zeros( n )
     stats = np.zeros( len( STATS ) )
     hist = np.zeros( ( maxit + 1, len( HISTORY ) ) )

@@ -52,7 +52,10 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <condition_variable>
 #include <functional>
+#include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -867,6 +870,8 @@ bool cg_solve( const Laplacian &L, const std::vector<double> &b, std::vector<dou
     return err < 1;
 }
 
+#include "oracle_mg.h"
+
 // ---- the diagram as Newton sees it ( `sdotplan/Sweep.h:142-364` ) ------------------------------
 
 /// the cuts of a cell's edges in polygon order ( user index of the neighbour, or `-1 - f` a side of the box ), at most
@@ -1245,6 +1250,7 @@ struct Options {
     int    keep_start = 0;
     int    step = TRIALS;
     double factor = 0.9;     ///< LIMITS: the trial is `factor` times the limit
+    int    linear = 0;       ///< the Newton systems: 0 Jacobi-PCG, 1 the multigrid ( `Linear.cpp:208-233` )
 };
 
 /// `Report.h:37-57`: two refused trials whose secant slopes agree and extrapolate under `nr / 4`
@@ -1466,9 +1472,21 @@ struct SolverT {
     I64 nb_cell_lim = 0, nb_limit_rounds = 0;
     double residual = 0, residual0 = 0, eps = 0, t_asm = 0, t_lim = 0;
     LinStats lst;
+    std::unique_ptr<MgSolver> mg;                        ///< `o.linear == 1`: kept over the solve ( hierarchy, subspace )
     std::function<void( int, double, int )> after_step;
 
     SolverT( SW &sw, const Options &o ) : sw( sw ), o( o ) {}
+
+    /// the Newton system, by the solver `o.linear` names
+    bool linear_solve( const Laplacian &L, const std::vector<double> &b, std::vector<double> &dd ) {
+        if ( o.linear != 1 ) return cg_solve( L, b, dd, o.lin_tol, o.lin_maxit, lst );
+        if ( ! mg ) {
+            mg = std::make_unique<MgSolver>( nb_threads() );
+            mg->nu = SW::DIM >= 3 ? 1 : 3;
+            mg->set_ranks( sw.t.rank_of );
+        }
+        return mg->solve( L, b, dd, o.lin_tol, o.lin_maxit, lst );
+    }
 
     /// `Newton.h:176-203`: on the targets `nue` when there are duplicates ( a duplicate's `b` is 0 )
     void rhs( const std::vector<double> &A, std::vector<double> &b ) const {
@@ -1629,7 +1647,7 @@ struct SolverT {
             // the pairs whose plane the doubles can no longer place are merged: the merit reads their clusters from here on
             if ( detect( L, w ) ) nr = merit( shares( a, ae ) );
             t_asm += now() - t0;
-            if ( ! cg_solve( L, b, d, o.lin_tol, o.lin_maxit, lst ) ) { status = 4; return false; }
+            if ( ! linear_solve( L, b, d ) ) { status = 4; return false; }
             double t = std::min( 1.0, o.mult_ok * t_last );
             bool taken = false, already = false;
             Hopeless hope;
@@ -1921,10 +1939,30 @@ int oracle_cg( I64 n, I64 nf, const I64 *fi, const I64 *fj, const double *fc, co
     return ok ? st.nb_iter : -1;
 }
 
+/// the multigrid-preconditioned CG alone on the Laplacian of facets `( fi, fj, fc )`, the first level aggregated by
+/// `rank_of[ i ] >> 3` ( null: the index ); `nu` smoothing steps; `L d = b`, `d[ 0 ] = 0`. Returns the CG iterations ( -1:
+/// not converged ), `levels[ 0 ]` the depth of the hierarchy.
+int oracle_mg( I64 n, I64 nf, const I64 *fi, const I64 *fj, const double *fc, const double *b, const I64 *rank_of, int nu,
+               double tol, int maxit, double *d, int *levels ) {
+    std::vector<orc::Facet> fa( sz( nf ) );
+    for ( I64 q = 0; q < nf; ++q ) fa[ size_t( q ) ] = orc::Facet{ fi[ q ], fj[ q ], fc[ q ] };
+    orc::Laplacian L;
+    L.assemble( n, fa );
+    orc::MgSolver mg( orc::nb_threads() );
+    mg.nu = nu;
+    if ( rank_of ) mg.set_ranks( std::vector<I64>( rank_of, rank_of + n ) );
+    std::vector<double> B( b, b + n ), D;
+    orc::LinStats st;
+    const bool ok = mg.solve( L, B, D, tol, maxit, st );
+    std::memcpy( d, D.data(), sizeof( double ) * size_t( n ) );
+    if ( levels ) levels[ 0 ] = mg.nb_levels();
+    return ok ? st.nb_iter : -1;
+}
+
 /// THE SOLVE ( `sdotplan/Solve.h:111-371`, no aggregation ).
-/// `opts` ( 19 doubles ): tol_abs, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol,
+/// `opts` ( 23 doubles ): tol_abs, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol,
 /// lin_maxit, keep_start, step ( 0 trials, 1 limits ), factor, continuation ( 0 never, 1 auto, 2 always ), conv_threshold,
-/// conv_s0, conv_ratio, conv_min. `start_box`: flag, lo[ 2 ], hi[ 2 ].
+/// conv_s0, conv_ratio, conv_min, gap, nb_dups, margin, linear ( 0 Jacobi-PCG, 1 multigrid ). `start_box`: flag, lo[ 2 ], hi[ 2 ].
 /// Outputs: `weights[ n ]`, `stats[ 31 ]`, `hist[ ( maxit + 1 ) * 8 ]` ( returns the row count ),
 /// `masses[ n ]`, `bary[ n * 2 ]`, `cost[ 1 ]`.
 int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w0, const double *lo, const double *hi, int kind,
@@ -1938,6 +1976,7 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
     o.t_min = opts[ 4 ]; o.mult_ok = opts[ 5 ]; o.residual = int( opts[ 6 ] ); o.power = opts[ 7 ]; o.switch_residual = opts[ 8 ];
     o.lin_tol = opts[ 9 ]; o.lin_maxit = int( opts[ 10 ] ); o.keep_start = int( opts[ 11 ] );
     o.step = int( opts[ 12 ] ); o.factor = opts[ 13 ] > 0 ? opts[ 13 ] : 0.9;
+    o.linear = int( opts[ 22 ] );
     const int continuation = int( opts[ 14 ] );
     const double conv_threshold = opts[ 15 ], conv_s0 = opts[ 16 ], conv_ratio = opts[ 17 ], conv_min = opts[ 18 ];
     Tree t;
@@ -2214,6 +2253,7 @@ int oracle_solve_3d( I64 n, const double *pos, const double *nu_in, const double
     o.t_min = opts[ 4 ]; o.mult_ok = opts[ 5 ]; o.residual = int( opts[ 6 ] ); o.power = opts[ 7 ]; o.switch_residual = opts[ 8 ];
     o.lin_tol = opts[ 9 ]; o.lin_maxit = int( opts[ 10 ] ); o.keep_start = int( opts[ 11 ] );
     o.step = TRIALS;
+    o.linear = int( opts[ 22 ] );
     const int continuation = int( opts[ 14 ] );
     const double conv_threshold = opts[ 15 ], conv_s0 = opts[ 16 ], conv_ratio = opts[ 17 ], conv_min = opts[ 18 ];
     Tree3 t;

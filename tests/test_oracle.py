@@ -309,6 +309,72 @@ def test_cg_solves_the_gauged_laplacian( oracle ):
     assert np.abs( d - ref ).max() <= 1e-9 * np.abs( ref ).max()
 
 
+def _sparse_direct( n, fi, fj, fc, b ):
+    """the gauged system `L d = b`, `d[ 0 ] = 0`, by a sparse LU ( SuperLU ) of `L` with row and column 0 struck out"""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sl
+    L = sp.coo_matrix( ( -np.asarray( fc ), ( fi, fj ) ), shape = ( n, n ) ).tocsr()
+    L = L + sp.diags( -np.asarray( L.sum( axis = 1 ) ).ravel() )
+    ref = np.zeros( n )
+    ref[ 1: ] = sl.spsolve( L[ 1:, 1: ].tocsc(), b[ 1: ] )
+    return ref
+
+
+@pytest.mark.parametrize( "by_rank", [ True, False ] )
+def test_multigrid_solves_the_gauged_laplacian( oracle, by_rank ):
+    """the multigrid-preconditioned CG ( the CPU baseline's linear solver ) against a sparse direct solve, on the Laplacian
+    of an 8000-seed diagram ( a hierarchy of 2 levels ). Aggregated by tree rank, its preconditioner must cut the iterations
+    of the Jacobi-PCG by a factor of 10 at least ( a wrong prolongation, Galerkin sign or smoother degrades it to CG or
+    worse ); by index ( a random cloud's index order: aggregates scattered over the square ) it must still converge"""
+    rng = np.random.default_rng( 12 )
+    n = 8000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    _, ( fi, fj, fc ) = oracle.measures( pos, facets = True )
+    b = rng.normal( size = n )
+    b -= b.mean()
+    rank_of = None
+    if by_rank:
+        order = oracle.build_tree( pos )[ "order" ]
+        rank_of = np.empty( n, np.int64 )
+        rank_of[ order ] = np.arange( n )
+    d, its, levels = oracle.mg( n, fi, fj, fc, b, rank_of = rank_of, tol = 1e-12 )
+    assert levels == 2 and its > 0 and d[ 0 ] == 0
+    ref = _sparse_direct( n, fi, fj, fc, b )
+    assert np.abs( d - ref ).max() <= 1e-8 * np.abs( ref ).max()
+    _, its_cg = oracle.cg( n, fi, fj, fc, b, tol = 1e-12 )
+    print( f"multigrid { its } iterations, Jacobi-PCG { its_cg }" )
+    assert not by_rank or 10 * its <= its_cg, ( its, its_cg )
+
+
+def test_multigrid_newton_solve_is_the_cg_one( oracle ):
+    """a Newton solve whose systems go through the multigrid takes the same steps as the Jacobi-PCG one: the same
+    iteration count, and weights equal to 1e-9 of their spread ( both linear solves run to 1e-12 )"""
+    rng = np.random.default_rng( 13 )
+    n = 6000
+    pos = rng.uniform( 0, 1, ( n, 2 ) )
+    img = 1.0 + 0.5 * np.sin( 6 * np.linspace( 0, 1, 64 ) )[ :, None ] * np.cos( 4 * np.linspace( 0, 1, 64 ) )[ None, : ]
+    dens = dict( values = img, x0 = 0.0, y0 = 0.0, hx = 1 / 64, hy = 1 / 64 )
+    a = oracle.solve( pos, density = dens, tol = 1e-12, step = "limits" )
+    b = oracle.solve( pos, density = dens, tol = 1e-12, step = "limits", linear = "mg" )
+    assert a[ "stats" ][ "status" ] == 1 and b[ "stats" ][ "status" ] == 1
+    assert a[ "stats" ][ "nb_iter" ] == b[ "stats" ][ "nb_iter" ]
+    w = a[ "weights" ]
+    assert np.abs( w - b[ "weights" ] ).max() <= 1e-9 * ( w.max() - w.min() )
+    assert b[ "stats" ][ "lin_nb_iter" ] * 5 <= a[ "stats" ][ "lin_nb_iter" ]
+
+
+def test_multigrid_3d_newton_solve_is_the_cg_one( oracle ):
+    rng = np.random.default_rng( 14 )
+    n = 3000
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    a = oracle.solve_3d( pos, tol = 1e-12 )
+    b = oracle.solve_3d( pos, tol = 1e-12, linear = "mg" )
+    assert a[ "stats" ][ "status" ] == 1 and b[ "stats" ][ "status" ] == 1
+    assert a[ "stats" ][ "nb_iter" ] == b[ "stats" ][ "nb_iter" ]
+    w = a[ "weights" ]
+    assert np.abs( w - b[ "weights" ] ).max() <= 1e-9 * ( w.max() - w.min() )
+
+
 # ---- the solve ( reference tests/test_SdotPlanNd.py:39-330 ) -----------------------------------------------------
 
 
