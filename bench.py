@@ -122,7 +122,8 @@ def bench_batch( args ):
     value = total * args.steps / elapsed
     line = dict( metric = METRIC, value = value, unit = "Diracs/s", n_gpus = 1, steps = args.steps, warmup = args.warmup,
                  ms_per_step = 1e3 * elapsed / args.steps, higher_is_better = True, scaling = "weak",
-                 vs_baseline = None if t_plain is None else value / ( total / t_plain ), dtype = "f64", data = "synthetic",
+                 vs_baseline = None if t_plain is None else value / ( total / t_plain ), dtype = "f64" if args.precision == "fp64" else "f32+f64",
+                 data = "synthetic",
                  config = dict( workload = f"{ len( problems ) } independent 2D instances, Dirac counts log-uniform in 1k-200k ( { total } in all ), "
                                            f"each with its own { args.batch_res }^2 image of 4 Gaussian bumps + 0.1 floor, max mass error <= { args.tol:g} "
                                            f"( absolute ) per instance",
@@ -130,6 +131,86 @@ def bench_batch( args ):
                                 precision = args.precision, streams = args.streams ),
                  batch = dict( converged = conv, worst_residual = worst,
                                plain_fp64_s = t_plain, plain_fp64_diracs_per_s = None if t_plain is None else total / t_plain ) )
+    print( json.dumps( line ), flush = True )
+
+
+def workload_serve( k = 40, rate = 1.0, n_min = 1_000, n_max = 500_000, res = 256 ):
+    """THE SERVED WORKLOAD: `k` requests, Poisson arrivals at `rate` per second ( exponential gaps, default_rng(6) ), 70 % 2D /
+    30 % 3D, Dirac counts log-uniform in [ n_min, n_max ]; a 2D request is uniform in [0,1]^2 with a `res^2` image of 4 Gaussian
+    bumps + 0.1 floor, a 3D one uniform in [0,1]^3 with 4 Gaussians ( sigma 0.1, means uniform in [0.2,0.8]^3, masses
+    Dirichlet(1) ) truncated to the unit cube ( default_rng( [ 6, q ] ) for request q ). Returns `[ ( arrival, dim, positions,
+    density parameters ) ]`"""
+    rng = np.random.default_rng( 6 )
+    gaps = rng.exponential( 1.0 / rate, k )
+    arrivals = np.cumsum( gaps ) - gaps[ 0 ]
+    dims = np.where( rng.uniform( 0, 1, k ) < 0.7, 2, 3 )
+    sizes = np.round( np.exp( rng.uniform( np.log( n_min ), np.log( n_max ), k ) ) ).astype( np.int64 )
+    out = []
+    for q in range( k ):
+        r = np.random.default_rng( [ 6, q ] )
+        n, d = int( sizes[ q ] ), int( dims[ q ] )
+        pos = r.uniform( 0, 1, ( n, d ) )
+        if d == 2:
+            dens = bump_image( res, r.uniform( 0, 1, ( 4, 2 ) ), r.uniform( 0.03, 0.15, 4 ) )
+        else:
+            dens = np.column_stack( [ r.uniform( 0.2, 0.8, ( 4, 3 ) ), np.full( 4, 0.1 ), r.dirichlet( np.ones( 4 ) ) ] )
+        out.append( ( float( arrivals[ q ] ), d, pos, dens ) )
+    return out
+
+
+def bench_serve( args ):
+    """the served workload: requests submitted at their arrival times to a `Server`, one JSON line: `value` the Diracs solved
+    per second over the stream ( first arrival to last completion ), the solves per second and the p50 / p99 latency ( completion
+    minus arrival ) beside it"""
+    import torch
+    import sdot
+    dev = torch.device( "cuda", 0 )
+    torch.cuda.set_device( dev )
+    reqs = workload_serve( args.serve_k, args.serve_rate, res = args.batch_res )
+    settings2 = sdot.Iterative( tol = args.tol, continuation = "never", on_failure = "ignore", precision = args.precision )
+    settings3 = sdot.Iterative( tol = args.tol, continuation = "auto", on_failure = "ignore" )
+
+    def problem( d, pos, dens ):
+        p = torch.as_tensor( pos, device = dev )
+        if d == 2:
+            return sdot.OtProblem( sdot.SumOfDiracs( p ), sdot.Image( dens, origin = [ 0, 0 ], frame = np.eye( 2 ) / args.batch_res ) )
+        return sdot.OtProblem( sdot.SumOfDiracs( p ), sdot.SumOfGaussians( dens[ :, :3 ], dens[ :, 3 ], dens[ :, 4 ],
+                                                                          support_box = ( [ 0, 0, 0 ], [ 1, 1, 1 ] ) ) )
+
+    problems = [ problem( d, pos, dens ) for _, d, pos, dens in reqs ]
+    total = int( sum( len( pos ) for _, _, pos, _ in reqs ) )
+    print( f"bench serve: { len( reqs ) } requests, { total } Diracs, rate { args.serve_rate }/s", file = sys.stderr, flush = True )
+    # a warm-up stream of the first few requests ( the slots' buffers and the graphs made once )
+    with sdot.Server( slots = args.streams ) as srv:
+        for pb, ( _, d, _, _ ) in list( zip( problems, reqs ) )[ :4 ]:
+            srv.submit( pb, settings2 if d == 2 else settings3 ).result()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tickets = []
+        for q, ( pb, ( at, d, _, _ ) ) in enumerate( zip( problems, reqs ) ):
+            wait = at - ( time.perf_counter() - t0 )
+            if wait > 0:
+                time.sleep( wait )
+            tickets.append( srv.submit( pb, settings2 if d == 2 else settings3 ) )
+        plans = [ t.result() for t in tickets ]
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        info = srv.info()
+    lat = np.array( [ t.times[ 2 ] - t.times[ 0 ] for t in tickets ] )
+    queue = np.array( [ t.times[ 1 ] - t.times[ 0 ] for t in tickets ] )
+    conv = sum( p.converged for p in plans )
+    worst = max( p.stats[ "residual" ] for p in plans )
+    line = dict( metric = METRIC, value = total / elapsed, unit = "Diracs/s", n_gpus = 1, steps = 1, warmup = 1,
+                 ms_per_step = 1e3 * elapsed, higher_is_better = True, scaling = "weak", vs_baseline = None, dtype = "f32+f64", data = "synthetic",
+                 config = dict( workload = f"{ len( reqs ) } served solves, Poisson arrivals at { args.serve_rate }/s, 70 % 2D ( { args.batch_res }^2 "
+                                           f"image of 4 bumps + 0.1 floor ) / 30 % 3D ( 4 Gaussians sigma 0.1 in place of a 64^3 image ), Dirac "
+                                           f"counts log-uniform in 1k-500k ( { total } in all ), max mass error <= { args.tol:g} ( absolute )",
+                                requests = len( reqs ), diracs = total, rate_per_s = args.serve_rate, slots = args.streams,
+                                precision_2d = args.precision, cells_3d = "fp64" ),
+                 serve = dict( solves_per_s = len( reqs ) / elapsed, p50_latency_s = float( np.percentile( lat, 50 ) ),
+                               p99_latency_s = float( np.percentile( lat, 99 ) ), max_latency_s = float( lat.max() ),
+                               p50_queue_s = float( np.percentile( queue, 50 ) ), converged = conv, worst_residual = worst,
+                               peak_slot_bytes = info[ "peak_bytes" ] ) )
     print( json.dumps( line ), flush = True )
 
 
@@ -221,9 +302,11 @@ def bench_3d( args ):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument( "--gpus", type = int, default = 1 )
-    ap.add_argument( "--config", default = "image2d", choices = [ "image2d", "gauss3d", "batch" ],
+    ap.add_argument( "--config", default = "image2d", choices = [ "image2d", "gauss3d", "batch", "serve" ],
                      help = "image2d: the 2D 1024^2-image workload ( the default line ); gauss3d: the 3D four-Gaussian workload; "
-                            "batch: 256 independent 2D instances" )
+                            "batch: 256 independent 2D instances; serve: a Poisson stream of 2D / 3D requests to a server" )
+    ap.add_argument( "--serve-k", type = int, default = 40 )
+    ap.add_argument( "--serve-rate", type = float, default = 1.0 )
     ap.add_argument( "--batch-k", type = int, default = 256 )
     ap.add_argument( "--batch-res", type = int, default = 256 )
     ap.add_argument( "--streams", type = int, default = 8 )
@@ -258,6 +341,10 @@ def main():
         import torch
         assert torch.cuda.is_available(), "bench.py needs the GPU"
         return bench_batch( args )
+    if args.config == "serve":
+        import torch
+        assert torch.cuda.is_available(), "bench.py needs the GPU"
+        return bench_serve( args )
     import torch
     import sdot
     from sdot import _native
@@ -298,6 +385,8 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     times, spmv_bytes, ev = _native.kernel_times()
+    fp32c = _native.fp32_counts()
+    mg32 = _native.mg32_counts()
     _native.set_timing( False )
 
     st = sols[ -1 ].stats
@@ -347,7 +436,11 @@ def main():
                                nb_iter = st[ "nb_iter" ], nb_diag = st[ "nb_diag" ], lin_nb_iter = st[ "lin_nb_iter" ],
                                nb_continuation_steps = st[ "nb_continuation_steps" ], t_lim = st[ "t_lim" ], it_double = st[ "it_double" ],
                                t_diag = st[ "t_diag" ], t_majorant = st[ "t_majorant" ], t_lin = st[ "t_lin" ], t_total = st[ "t_total" ],
-                               device_bytes = _native.device_bytes() ) )
+                               device_bytes = _native.device_bytes(),
+                               fp32_repairs = dict( diagrams = fp32c[ "diagrams" ], cells_redone_fp64 = fp32c[ "redone" ],
+                                                    facets_dropped = fp32c[ "dropped" ] ),
+                               mg_fp32_refinement = dict( systems = mg32[ "solves" ], fp32_rounds = mg32[ "rounds" ],
+                                                          finished_in_fp64 = mg32[ "fallbacks" ] ) ) )
 
     # the FP64 cells of the returned weights: the error of the plan against the FP64 kernel ( the oracle's cells: equal to the
     # bit on a constant density, to 1e-14 relative on an image, by the parity tests )
@@ -412,8 +505,20 @@ def measured_traffic( kernel, args ):
     return None if k is None else k[ "bytes" ]
 
 
+def cpu_model():
+    try:
+        with open( "/proc/cpuinfo" ) as f:
+            for line in f:
+                if line.startswith( "model name" ):
+                    return line.split( ":", 1 )[ 1 ].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline( pos_h, img_h, args ):
-    """the oracle's solve of the first `cpu_sample` Diracs of the same workload, on the host cores"""
+    """the oracle's solve of the first `cpu_sample` Diracs of the same workload on the host cores, with the linear solver the
+    reference's CPU solve picks by default ( the multigrid ): one untimed run, then the best of three"""
     sys.path.insert( 0, os.path.join( ROOT, "oracle" ) )
     import oracle
     m = min( args.cpu_sample, len( pos_h ) )
@@ -422,13 +527,21 @@ def cpu_baseline( pos_h, img_h, args ):
     v = img_h / ( img_h.mean() )                         # ( the unit square: mean value = mass )
     dens = dict( values = np.ascontiguousarray( v.T ), x0 = 0.0, y0 = 0.0, hx = 1.0 / args.res, hy = 1.0 / args.res )
     tol = args.tol * len( pos_h ) / m                    # the same relative accuracy per Dirac
-    t0 = time.perf_counter()
-    r = oracle.solve( pos_h[ :m ], density = dens, tol = tol, lin_tol = 1e-6, step = args.step, continuation = args.continuation )
-    dt = time.perf_counter() - t0
+    ts = []
+    for k in range( 4 ):
+        t0 = time.perf_counter()
+        r = oracle.solve( pos_h[ :m ], density = dens, tol = tol, lin_tol = 1e-6, step = args.step, continuation = args.continuation,
+                          linear = "mg" )
+        ts.append( time.perf_counter() - t0 )
+        print( f"bench: CPU baseline run { k }: { ts[ -1 ]:.2f} s, status { r[ 'stats' ][ 'status' ]:.0f}", file = sys.stderr, flush = True )
+    dt = min( ts[ 1: ] )
+    s = r[ "stats" ]
     return dict( value = m / dt, unit = "Diracs/s", cores = cores, kind = "port",
-                 sample = f"the oracle ( C++ FP64 restatement, Jacobi-PCG ) solving the first { m } Diracs of the same workload to "
-                          f"max |m - nu| <= { tol:.1e} ( the same error relative to nu ), { r[ 'stats' ][ 'nb_iter' ]:.0f} Newton iterations, "
-                          f"{ dt:.1f} s, status { r[ 'stats' ][ 'status' ]:.0f}" )
+                 sample = f"the oracle ( C++ FP64 restatement built -O3 -march=x86-64-v3, multigrid-preconditioned CG as the reference's "
+                          f"CPU default ) solving the first { m } Diracs of the same workload to max |m - nu| <= { tol:.1e} ( the same error "
+                          f"relative to nu ), { s[ 'nb_iter' ]:.0f} Newton iterations, best of 3 after one warm-up: { dt:.2f} s "
+                          f"( runs { ', '.join( f'{ t:.2f}' for t in ts[ 1: ] ) } s; linear { s[ 't_lin' ]:.2f} s, cells { s[ 't_diag' ]:.2f} s ), "
+                          f"status { s[ 'status' ]:.0f}, on { cores } threads of { cpu_model() }" )
 
 
 if __name__ == "__main__":

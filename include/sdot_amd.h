@@ -58,6 +58,18 @@ int sdot_amd_kernel_times( double *ms, long long *launches, double *spmv_bytes, 
  * whether timing is on or not; a solve that replays far more than it captures reuses its graphs. HOST [ 1 ] each. */
 int sdot_amd_graph_counts( long long *captures, long long *replays );
 
+/* THE FP32 KERNEL'S TOPOLOGY REPAIRS ( no counterpart in the reference, whose float kernel keeps what it found ): over the
+ * diagrams solves made with the FP32 kernel since the last `sdot_amd_kernel_times( ..., reset = 1 )`, the diagrams, the cells
+ * redone by the FP64 kernel because a facet was found by one of its two cells only, and the facets still one-sided after that
+ * ( dropped from the Laplacian ). HOST [ 1 ] each, may be null. */
+int sdot_amd_fp32_counts( long long *diagrams, long long *redone, long long *dropped );
+
+/* THE MULTIGRID'S FP32 REFINEMENT ( the reference's card runs its levels in float, `gpu/Linear2D.cuh:993-1000`, with no FP64
+ * outer loop to count ): since the last `sdot_amd_kernel_times( ..., reset = 1 )`, the Newton systems solved by FP32 multigrid
+ * rounds inside the FP64 iterative refinement, the rounds, and the systems finished by the FP64 multigrid because a round did
+ * not halve the FP64 residual. HOST [ 1 ] each, may be null. */
+int sdot_amd_mg32_counts( long long *solves, long long *rounds, long long *fallbacks );
+
 /* ONE DIAGRAM: the masses of the power cells of `( positions, weights )` in the box
  * `box = { lo_x, lo_y, hi_x, hi_y }` ( HOST ) against a density, and optionally the facets
  * `c_ij = int_facet rho / ( 2 |p_i - p_j| )` as triplets ( rows in user order, each row in its cell's
@@ -77,6 +89,47 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
                           int density_kind, double rho, const double *img, int nx, int ny, const double *img_geo,
                           double *masses, long long *facet_i, long long *facet_j, double *facet_c, long long facet_cap,
                           long long *nb_facets, unsigned long long *events );
+
+/* ONE DIAGRAM with the kernel's float chosen: `sdot_amd_measures_2d`'s arguments, then `precision` SDOT_AMD_FP64 ( that call )
+ * or SDOT_AMD_FP32 ( the walk and the cuts in float on 32-byte float node records, every vertex re-solved in double from its two
+ * planes; a cell that fails the re-solve's conditioning test, and both cells of a facet only one of them found, are redone by
+ * the FP64 kernel; the rows made symmetric ). SDOT_AMD_FP64 .. SDOT_AMD_MIXED are defined below, with the solve.
+ * Replaces `power_diagram_measures_card_2d` with the kernel variant of `kernel_dtype` ( `card_variant_for( 32 | 64, ... )`,
+ * src/sdot/PowerDiagram_Bsp.py:421-443, src/sdot/PowerDiagram.py:375-391 ). */
+int sdot_amd_measures_2d_precision( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                                    int density_kind, double rho, const double *img, int nx, int ny, const double *img_geo,
+                                    double *masses, long long *facet_i, long long *facet_j, double *facet_c, long long facet_cap,
+                                    long long *nb_facets, unsigned long long *events, int precision );
+
+/* THE MOMENTS OF ONE DIAGRAM: for each cell `int rho` ( `masses` ), `int x rho` ( `first`, DEVICE [ n ][ 2 ] ) and `int |x|^2 rho`
+ * ( `second` ), user order: enough for a transport cost ( `second - 2 p . first + |p|^2 mass` ) and the barycentres
+ * ( `first / mass` ). The arguments of `sdot_amd_measures_2d` otherwise ( any of its densities ).
+ * Replaces `power_diagram_moments` ( src/sdot/PowerDiagram.py:433-459, `diagram/Ops.h:187-228` ). */
+int sdot_amd_moments_2d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                         double rho, const double *img, int nx, int ny, const double *img_geo, double *masses, double *first, double *second );
+
+/* The same in 3D ( `first` DEVICE [ n ][ 3 ] ), the density as `sdot_amd_measures_3d`'s. */
+int sdot_amd_moments_3d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                         double rho, const double *gauss, int ng, double *masses, double *first, double *second );
+
+/* THE CELLS THEMSELVES: the vertices of every cell, counter-clockwise, in user order. Cell `i` owns the vertices
+ * `offsets[ i ] .. offsets[ i + 1 ]` ( DEVICE [ n + 1 ] int64 ) of `vertices` ( DEVICE [ vertex_cap ][ 2 ], absolute coordinates ), and
+ * `cuts[ v ]` ( DEVICE [ vertex_cap ] int64 ) names the cut of the edge that leaves vertex `v`: a seed's user index, or `-1 - f` for
+ * side `f` of the box ( 0 bottom, 1 right, 2 top, 3 left ). `nb_vertices` HOST [ 1 ]: the total. With `vertices` null, only the
+ * total is computed ( a first call to size the buffers ); past `vertex_cap`, kind 3 with the total in `value`.
+ * Replaces `power_diagram_cells` ( src/sdot/PowerDiagram.py:490-515, `diagram/Ops.h` `build_cell` ). */
+int sdot_amd_cells_2d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                       double rho, const double *img, int nx, int ny, const double *img_geo, long long *offsets, double *vertices,
+                       long long *cuts, long long vertex_cap, long long *nb_vertices );
+
+/* THE ADJOINT OF THE MASSES: the cotangent `grad_masses` ( DEVICE [ n ], user order ) -> `grad_weights` ( DEVICE [ n ],
+ * `sum_i g_i d m_i / d w` = `L g` ) and `grad_positions` ( DEVICE [ n ][ 2 ], `sum_i g_i d m_i / d p` ); either may be null. A
+ * constant or an image density ( the facets' first moments of a sum of Gaussians are not written: kind 1 ).
+ * Replaces `sdot::gpu2d::measures_vjp` ( src/sdot/include/sdot/gpu/Cell2D.cuh:1816-1839; the CPU's `measures_bwd`,
+ * `diagram/Ops.h:437-459` ). */
+int sdot_amd_measures_vjp_2d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                              double rho, const double *img, int nx, int ny, const double *img_geo, const double *grad_masses,
+                              double *grad_weights, double *grad_positions );
 
 /* ONE DIAGRAM'S MASSES WITHOUT ALLOCATION. No counterpart in the reference: this entry point is for callers that cannot allocate
  * device memory ( code under HIP graph capture, tight serving loops, harnesses that forbid `hipMalloc` ). It gives the masses of
@@ -211,6 +264,14 @@ int sdot_amd_solve_2d_batch( void *stream, int nb_streams, long long k, const lo
                              const double *rho, const double *const *imgs, const double *img_geo,
                              double *weights, double *cell_masses, double *barycenters, double *stats, double *costs,
                              long long *errors, double *times );
+
+/* THE BATCH with the kernel's float chosen: `sdot_amd_solve_2d_batch`'s arguments, then `precision` as `sdot_amd_solve_2d_precision`'s
+ * ( every instance solved as `sdot_amd_solve_2d_precision` would solve it alone ). */
+int sdot_amd_solve_2d_batch_precision( void *stream, int nb_streams, long long k, const long long *offsets, const double *positions,
+                                       const double *nu, const double *w0, const double *options, const double *boxes, int density_kind,
+                                       const double *rho, const double *const *imgs, const double *img_geo,
+                                       double *weights, double *cell_masses, double *barycenters, double *stats, double *costs,
+                                       long long *errors, double *times, int precision );
 
 /* THE SERVER: continuous batching of solves ( no counterpart in the reference, which solves one problem per call; for
  * callers that serve a stream of requests ). `nb_slots` slots ( 1 .. 32 ), each a HIP stream driven by its own host

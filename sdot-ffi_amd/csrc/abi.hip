@@ -192,6 +192,22 @@ int sdot_amd_kernel_times( double *ms, long long *launches, double *spmv_bytes, 
     } );
 }
 
+int sdot_amd_fp32_counts( long long *diagrams, long long *redone, long long *dropped ) {
+    return guarded( [&] {
+        if ( diagrams ) *diagrams = timing().fp32_diagrams.load();
+        if ( redone ) *redone = timing().fp32_redone.load();
+        if ( dropped ) *dropped = timing().fp32_dropped.load();
+    } );
+}
+
+int sdot_amd_mg32_counts( long long *solves, long long *rounds, long long *fallbacks ) {
+    return guarded( [&] {
+        if ( solves ) *solves = timing().mg32_solves.load();
+        if ( rounds ) *rounds = timing().mg32_rounds.load();
+        if ( fallbacks ) *fallbacks = timing().mg32_fallbacks.load();
+    } );
+}
+
 int sdot_amd_graph_counts( long long *captures, long long *replays ) {
     return guarded( [&] {
         if ( captures ) *captures = timing().graph_captures.load();
@@ -208,6 +224,68 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
         SDOT_CHECK( ! facet_i || ( facet_j && facet_c ), "measures: facet_i, facet_j and facet_c go together" );
         const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
         measures_2d( hipStream_t( stream ), n, positions, weights, box, dn, masses, facet_i, facet_j, facet_c, facet_cap, nb_facets, events );
+    } );
+}
+
+int sdot_amd_measures_2d_precision( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                                    int density_kind, double rho, const double *img, int nx, int ny, const double *img_geo, double *masses,
+                                    long long *facet_i, long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets,
+                                    unsigned long long *events, int precision ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && masses, "measures: positions and masses are required" );
+        SDOT_CHECK( precision == SDOT_AMD_FP64 || precision == SDOT_AMD_FP32, "measures: the precision is SDOT_AMD_FP64 or SDOT_AMD_FP32" );
+        check_box( box );
+        SDOT_CHECK( ! facet_i || ( facet_j && facet_c ), "measures: facet_i, facet_j and facet_c go together" );
+        const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
+        measures_2d( hipStream_t( stream ), n, positions, weights, box, dn, masses, facet_i, facet_j, facet_c, facet_cap, nb_facets, events,
+                     precision == SDOT_AMD_FP32 );
+    } );
+}
+
+int sdot_amd_moments_2d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                         double rho, const double *img, int nx, int ny, const double *img_geo, double *masses, double *first, double *second ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && masses && first && second, "moments: positions, masses, first and second are required" );
+        check_box( box );
+        const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
+        moments_2d( hipStream_t( stream ), n, positions, weights, box, dn, masses, first, second );
+    } );
+}
+
+int sdot_amd_moments_3d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                         double rho, const double *gauss, int ng, double *masses, double *first, double *second ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && masses && first && second, "moments: positions, masses, first and second are required" );
+        check_box3( box );
+        const Density3 dn = make_density3( density_kind, rho, gauss, ng );
+        moments_3d( hipStream_t( stream ), n, positions, weights, box, dn, masses, first, second );
+    } );
+}
+
+int sdot_amd_cells_2d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                       double rho, const double *img, int nx, int ny, const double *img_geo, long long *offsets, double *vertices,
+                       long long *cuts, long long vertex_cap, long long *nb_vertices ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && nb_vertices, "cells: positions and nb_vertices are required" );
+        SDOT_CHECK( ! vertices || ( offsets && cuts ), "cells: offsets, vertices and cuts go together" );
+        check_box( box );
+        const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
+        const I64 nv = cells_2d( hipStream_t( stream ), n, positions, weights, box, dn, offsets, vertices, cuts, vertex_cap );
+        *nb_vertices = nv;
+        if ( vertices && nv > vertex_cap )
+            throw Failure( 3, 0, nv, "cells: " + std::to_string( nv ) + " vertices, past vertex_cap ( " + std::to_string( vertex_cap ) + " )" );
+    } );
+}
+
+int sdot_amd_measures_vjp_2d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                              double rho, const double *img, int nx, int ny, const double *img_geo, const double *grad_masses,
+                              double *grad_weights, double *grad_positions ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && grad_masses && ( grad_weights || grad_positions ),
+                    "measures_vjp: positions, grad_masses and a gradient to write are required" );
+        check_box( box );
+        const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
+        measures_vjp_2d( hipStream_t( stream ), n, positions, weights, box, dn, grad_masses, grad_weights, grad_positions );
     } );
 }
 
@@ -310,12 +388,14 @@ int sdot_amd_solve_3d( void *stream, long long n, const double *positions, const
     } );
 }
 
-int sdot_amd_solve_2d_batch( void *stream, int nb_streams, long long k, const long long *offsets, const double *positions,
-                             const double *nu, const double *w0, const double *options, const double *boxes, int density_kind,
-                             const double *rho, const double *const *imgs, const double *img_geo,
-                             double *weights, double *cell_masses, double *barycenters, double *stats, double *costs,
-                             long long *errors, double *times ) {
+namespace {
+int batch_call( void *stream, int nb_streams, long long k, const long long *offsets, const double *positions, const double *nu, const double *w0,
+                const double *options, const double *boxes, int density_kind, const double *rho, const double *const *imgs,
+                const double *img_geo, double *weights, double *cell_masses, double *barycenters, double *stats, double *costs,
+                long long *errors, double *times, int precision ) {
     return guarded( [&] {
+        SDOT_CHECK( precision == SDOT_AMD_FP64 || precision == SDOT_AMD_FP32 || precision == SDOT_AMD_MIXED,
+                    "batch: the precision is SDOT_AMD_FP64, SDOT_AMD_FP32 or SDOT_AMD_MIXED" );
         SDOT_CHECK( k >= 1 && offsets && positions && nu && options && boxes && weights && cell_masses && barycenters && stats && costs && errors,
                     "batch: a required array is null" );
         SDOT_CHECK( offsets[ 0 ] == 0, "batch: offsets[ 0 ] must be 0" );
@@ -350,6 +430,7 @@ int sdot_amd_solve_2d_batch( void *stream, int nb_streams, long long k, const lo
             J.in.nu = nu + b;
             J.in.w0 = w0 ? w0 + b : nullptr;
             J.in.options = o;
+            J.in.precision = precision;
             for ( int c = 0; c < 4; ++c ) J.in.box[ c ] = boxes[ 4 * q + c ];
             J.in.dens = make_density( density_kind, rho ? rho[ q ] : 1.0, imgs ? imgs[ q ] : nullptr, nx, ny, geo );
             J.out.weights = weights + b;
@@ -364,6 +445,25 @@ int sdot_amd_solve_2d_batch( void *stream, int nb_streams, long long k, const lo
             if ( times ) { times[ 2 * q ] = jobs[ size_t( q ) ].t_start; times[ 2 * q + 1 ] = jobs[ size_t( q ) ].t_end; }
         }
     } );
+}
+} // namespace
+
+int sdot_amd_solve_2d_batch( void *stream, int nb_streams, long long k, const long long *offsets, const double *positions,
+                             const double *nu, const double *w0, const double *options, const double *boxes, int density_kind,
+                             const double *rho, const double *const *imgs, const double *img_geo,
+                             double *weights, double *cell_masses, double *barycenters, double *stats, double *costs,
+                             long long *errors, double *times ) {
+    return batch_call( stream, nb_streams, k, offsets, positions, nu, w0, options, boxes, density_kind, rho, imgs, img_geo, weights,
+                       cell_masses, barycenters, stats, costs, errors, times, SDOT_AMD_FP64 );
+}
+
+int sdot_amd_solve_2d_batch_precision( void *stream, int nb_streams, long long k, const long long *offsets, const double *positions,
+                                       const double *nu, const double *w0, const double *options, const double *boxes, int density_kind,
+                                       const double *rho, const double *const *imgs, const double *img_geo,
+                                       double *weights, double *cell_masses, double *barycenters, double *stats, double *costs,
+                                       long long *errors, double *times, int precision ) {
+    return batch_call( stream, nb_streams, k, offsets, positions, nu, w0, options, boxes, density_kind, rho, imgs, img_geo, weights,
+                       cell_masses, barycenters, stats, costs, errors, times, precision );
 }
 
 namespace {
@@ -560,11 +660,12 @@ int sdot_amd_linear_solves_2d( void *stream, long long n, const long long *row, 
         k_csr_from<<<blocks_for( std::max<long long>( n, nnz ), 256 ), 256, 0, st>>>( n, row, col, row_off, row_cnt, col32, nnz );
         check_launch( "k_csr_from" );
         Csr L;
-        L.n = n; L.row_off = row_off; L.row_cnt = row_cnt; L.col = col32; L.val = val; L.dia = dia;
+        L.n = n; L.row_off = row_off; L.row_cnt = row_cnt; L.col = col32; L.val = val; L.dia = dia; L.nnz = nnz; L.cap = std::max<long long>( nnz, 1 );
         // options ( `SdotPlanNd.py:204-222` ): n, k, method ( 0 cg, 1 mg ), tol, smoothed, float, recycle, rebuild, kcycle, nu
         const double tol = options && options[ 3 ] > 0 ? options[ 3 ] : 1e-6;
         const bool mg = options && options[ 2 ] == 1;
         MgOptions mo;
+        mo.fp32 = options && options[ 5 ] == 1;          // ( the multigrid's corrections in FP32 inside an FP64 refinement )
         if ( options && options[ 8 ] >= 0 ) mo.kcycle = int( options[ 8 ] );
         if ( options && options[ 9 ] > 0 ) mo.nu = int( options[ 9 ] );
         for ( long long j = 0; j < k; ++j ) {

@@ -29,8 +29,9 @@ struct DiagramOut {
     I64       facet_cap = 0;
     // moments ( instead of facets ): barycentres ( absolute, rank order ) and per-cell cost
     double   *bary = nullptr, *cost = nullptr;
-    // counters on the device: [ 0 ] facets used, [ 1 ] cells past the first pass, [ 2 ] failures,
-    // [ 3 ] the first failing rank, [ 4 ] cells past the second pass
+    // counters on the device: [ 0 ] facets used, [ 1 ] cells past the first pass ( FP32 kernel: and the cells that failed its
+    // conditioning test ), [ 2 ] failures, [ 3 ] the first failing rank, [ 4 ] cells past the second pass, [ 6 ] FP32 kernel:
+    // cells redone in FP64 for a facet only one side had, [ 7 ] facets still one-sided after that, dropped
     unsigned *counters = nullptr;
     int      *ovf_list = nullptr;
     unsigned long long *events = nullptr;   ///< side tests, pruning vertex tests, new vertices, node visits, cuts
@@ -40,6 +41,17 @@ struct DiagramOut {
     // facets mode: slots left free at the end of every row ( a wave's rows are `64 x ( longest + row_extra )` ), for the
     // links the solver appends after the cells ( the exact duplicates' rows )
     int       row_extra = 0;
+    // cells mode: each cell's vertex count ( rank order ), and, when `cell_off` is set, its vertices from `cell_off[ order[ k ] ]`
+    // on ( user order ): absolute coordinates in `cell_xy`, and in `cell_cut` the cut of the edge that leaves each vertex ( a
+    // seed's user index, or `-1 - f` for side `f` of the box )
+    int       *cell_nb = nullptr;
+    const long long *cell_off = nullptr;
+    double    *cell_xy = nullptr;
+    long long *cell_cut = nullptr;
+    const int *order = nullptr;
+    // vjp mode: the cotangent of the masses in ( rank order ), the gradients of the weights and of the positions out ( rank order )
+    const double *vjp_g = nullptr;
+    double    *grad_w = nullptr, *grad_pos = nullptr;
 };
 
 /// the Laplacian rows of a wave's 64 cells are interleaved: entry `q` of the row of lane `l` at `base + q * 64 + l`
@@ -48,13 +60,20 @@ constexpr int ROW_STEP = 64;
 /// the vertex slots of the passes ( the first in LDS, the second in global slots )
 constexpr int CELL_BLOCK = 128;
 constexpr int CAP1 = 16;             ///< first pass, LDS
+#ifndef SDOT_AMD_CAPW
+#define SDOT_AMD_CAPW 16
+#endif
+/// the FP32 walk's LDS slots ( even, at most `CAP1` ): 12 bytes a slot, so its count sets the walk's waves per CU
+constexpr int CAPW = SDOT_AMD_CAPW;
+static_assert( CAPW % 2 == 0 && CAPW >= 6 && CAPW <= CAP1, "the FP32 walk's slots: even, from 6 to CAP1" );
 constexpr int CAP2 = 96;             ///< second pass, LDS ( one wave per block )
 constexpr int CAP3 = 2048;           ///< third pass, global slots: the vertex limit of a cell
 constexpr int PASS2_BLOCKS = 1024;
+constexpr int REDO_BLOCKS = 256;     ///< the FP32 kernel's FP64 redo of one-sided facets ( one wave each )
 constexpr int OVF_SLOTS = 4096;
 constexpr int EDGE_CAP = 16;         ///< the edges a cell keeps for the step by the limits ( `gpu/Cell2D.cuh:97` )
 
-enum CellMode { MODE_MASSES = 0, MODE_FACETS = 1, MODE_MOMENTS = 2 };
+enum CellMode { MODE_MASSES = 0, MODE_FACETS = 1, MODE_MOMENTS = 2, MODE_CELLS = 3, MODE_VJP = 4 };
 
 /// one diagram: every cell of the tree at its current weights ( `t.seeds[ k ].z` ) in the box
 /// `lo / hi`. Resets and uses the counters; the overflow pass is launched without a read back.

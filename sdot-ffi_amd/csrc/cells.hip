@@ -171,11 +171,82 @@ __device__ __forceinline__ double proximity( const NodeRec &b, double p0x, doubl
     return e0 * e0 + e1 * e1;
 }
 
+/// the seed as the float pruning sees it: each coordinate and the weight split in two floats ( `x = xh + xl` ), so that the
+/// box centred on the seed carries an error relative to its distance from the seed, not to the coordinates
+struct Frame32 {
+    float xh, xl, yh, yl, wh, wl;
+};
+
+__device__ __forceinline__ Frame32 frame32( double x, double y, double w ) {
+    Frame32 f;
+    f.xh = float( x ); f.xl = float( x - double( f.xh ) );
+    f.yh = float( y ); f.yl = float( y - double( f.yh ) );
+    f.wh = float( w ); f.wl = float( w - double( f.wh ) );
+    return f;
+}
+
+/// the test of `can_cut` on a float record: the node's outward-rounded box centred on the seed, its rounded-up majorant,
+/// in float. It answers "may cut" up to a margin of 1e-6 of the magnitude of its terms ( sixteen float epsilons, far above the
+/// roundings of the float vertex, of the centred box and of the majorant's constant ), so that it prunes no node the exact
+/// test would keep on the same polygon
+template<class S>
+__device__ __forceinline__ bool can_cut( const S &c, int nb, const NodeF &r, const Frame32 &f, Ev &ev ) {
+    ev.visits += 1;
+    const float lx = ( r.lo_x - f.xh ) - f.xl, ly = ( r.lo_y - f.yh ) - f.yl;
+    const float hx = ( r.hi_x - f.xh ) - f.xl, hy = ( r.hi_y - f.yh ) - f.yl;
+    const float cc = ( ( f.wh - r.wb ) + f.wl ) - ( r.wa_x * f.xh + r.wa_y * f.yh );
+    const float hax = 0.5f * r.wa_x, hay = 0.5f * r.wa_y, acc = fabsf( cc );
+    auto test = [&]( int i ) {
+        const float vx = float( c.X( i ) ), vy = float( c.Y( i ) );
+        const float yx = fminf( fmaxf( vx + hax, lx ), hx ), yy = fminf( fmaxf( vy + hay, ly ), hy );
+        const float ux = yx - vx, uy = yy - vy;
+        const float p = ux * ux + uy * uy, q = vx * vx + vy * vy;
+        const float ax = r.wa_x * yx, ay = r.wa_y * yy;
+        const float sv = ( p - q ) + ( cc - ( ax + ay ) );
+        const float tv = ( p + q ) + ( acc + ( fabsf( ax ) + fabsf( ay ) ) );
+        return sv <= 1e-6f * tv;
+    };
+    for ( int i = 0; i < nb; i += 2 ) {                  // ( two vertices a step, their reads issued together; `cap` is even )
+        const bool t0 = test( i ), u1 = test( i + 1 ), t1 = i + 1 < nb && u1;
+        if ( t0 ) { ev.prune += i + 1; return true; }
+        if ( t1 ) { ev.prune += i + 2; return true; }
+    }
+    ev.prune += nb;
+    return false;
+}
+
+/// `can_cut` on a double record ( the seed itself in `p0x, p0y, w0` )
+template<class S>
+__device__ __forceinline__ bool can_cut( const S &c, int nb, const NodeRec &r, double p0x, double p0y, double w0, const Frame32 &, Ev &ev ) {
+    return can_cut( c, nb, r, p0x, p0y, w0, ev );
+}
+template<class S>
+__device__ __forceinline__ bool can_cut( const S &c, int nb, const NodeF &r, double, double, double, const Frame32 &f, Ev &ev ) {
+    return can_cut( c, nb, r, f, ev );
+}
+
+__device__ __forceinline__ double proximity( const NodeF &b, double p0x, double p0y ) {
+    double e0 = fmax( double( b.lo_x ) - p0x, p0x - double( b.hi_x ) ), e1 = fmax( double( b.lo_y ) - p0y, p0y - double( b.hi_y ) );
+    e0 = e0 > 0 ? e0 : 0;
+    e1 = e1 > 0 ? e1 : 0;
+    return e0 * e0 + e1 * e1;
+}
+
 struct Walk {
     const NodeRec *nodes;
+    const NodeF   *nodes32;                              ///< the float records ( the FP32 walk )
+    const int     *node_end;                             ///< each node's end ( a float record's leaf )
     const double4 *seeds;
     int depth;
 };
+
+/// the record of node `m` and what the walk needs of it, for either kind of record
+__device__ __forceinline__ NodeRec node_at( const Walk &wk, int m, const NodeRec * ) { return wk.nodes[ m ]; }
+__device__ __forceinline__ NodeF node_at( const Walk &wk, int m, const NodeF * ) { return wk.nodes32[ m ]; }
+__device__ __forceinline__ bool node_live( const NodeRec &r ) { return r.beg < r.end; }
+__device__ __forceinline__ bool node_live( const NodeF &r ) { return r.beg >= 0; }
+__device__ __forceinline__ int node_stop( const Walk &, int, const NodeRec &r ) { return r.end; }
+__device__ __forceinline__ int node_stop( const Walk &wk, int m, const NodeF & ) { return wk.node_end[ m ]; }
 
 /// builds the cell of rank `k` in `c`; returns the vertex count ( 0: empty ), or -1 when the slots
 /// were too few ( the cell is then left for the next pass ). One loop iteration does ONE thing of about
@@ -183,11 +254,13 @@ struct Walk {
 /// so the lanes of a wave, each at its own place in its walk, stay in step. The loads run one step
 /// ahead of their use: a descent loads both children's records ( the nearer is the next node tested,
 /// from registers ), and a leaf's next seed is loaded while the current one cuts.
-template<class S>
+template<class S, class ND = NodeRec>
 __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *lo, const double *hi, const S &c, int cap, Ev &ev ) {
     using T = typename S::T;
+    const ND *kind = nullptr;                            // ( which records the walk reads: `NodeRec` or the float `NodeF` )
     const double4 p = wk.seeds[ k ];
     const double p0x = p.x, p0y = p.y, w0 = p.z;
+    const Frame32 f32 = frame32( p0x, p0y, w0 );
     c.X( 0 ) = T( lo[ 0 ] - p0x ); c.Y( 0 ) = T( lo[ 1 ] - p0y );
     c.X( 1 ) = T( hi[ 0 ] - p0x ); c.Y( 1 ) = T( lo[ 1 ] - p0y );
     c.X( 2 ) = T( hi[ 0 ] - p0x ); c.Y( 2 ) = T( hi[ 1 ] - p0y );
@@ -198,7 +271,7 @@ __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *l
     unsigned long long pend = 0, left = 0;
     int q = 0, qe = 0;                                   // the open leaf's remaining seeds
     bool visit = true;                                   // ( m, h ) is still to be tested
-    NodeRec rec;                                         // the record of ( m, h ) when the descent loaded it
+    ND rec;                                              // the record of ( m, h ) when the descent loaded it
     bool have = false;
     double4 nxt = make_double4( 0, 0, 0, 0 );            // the open leaf's seed `q`, loaded ahead
     for ( ;; ) {
@@ -228,16 +301,16 @@ __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *l
             pend &= ~bit;
         }
         visit = false;
-        const NodeRec r = have ? rec : wk.nodes[ m ];
+        const ND r = have ? rec : node_at( wk, m, kind );
         have = false;
-        if ( r.beg < r.end && can_cut( c, nb, r, p0x, p0y, w0, ev ) ) {
+        if ( node_live( r ) && can_cut( c, nb, r, p0x, p0y, w0, f32, ev ) ) {
             if ( h <= 1 ) {
                 q = r.beg;
-                qe = r.end;
+                qe = node_stop( wk, m, r );
                 nxt = wk.seeds[ q ];
             } else {
                 const int lc = m + 1, rc = m + ( 1 << ( h - 1 ) );
-                const NodeRec rl = wk.nodes[ lc ], rr = wk.nodes[ rc ];
+                const ND rl = node_at( wk, lc, kind ), rr = node_at( wk, rc, kind );
                 const bool go_left = proximity( rl, p0x, p0y ) <= proximity( rr, p0x, p0y );
                 const unsigned long long bit = 1ull << ( h - 1 );
                 pend |= bit;
@@ -432,6 +505,57 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
     if ( ! active ) return;
     Grid g{};
     if ( P.dn.kind == 1 ) g = grid_from( P.dn, p.x, p.y );
+    if ( P.mode == MODE_CELLS ) {
+        P.out.cell_nb[ k ] = nb;
+        if ( P.out.cell_off ) {
+            const long long o = P.out.cell_off[ P.out.order[ k ] ];
+            for ( int e = 0; e < nb; ++e ) {
+                P.out.cell_xy[ 2 * ( o + e ) ] = c.X( e ) + p.x;
+                P.out.cell_xy[ 2 * ( o + e ) + 1 ] = c.Y( e ) + p.y;
+                const int id = c.I( e );
+                P.out.cell_cut[ o + e ] = id >= 0 ? (long long) P.out.order[ id ] : (long long) id;
+            }
+        }
+        return;
+    }
+    if ( P.mode == MODE_VJP ) {
+        // the adjoint of the masses: for each facet ( k, j ) of density integral `F` and first moment `M = int rho ( x - p_k )`,
+        // `d m_k / d w_j = -F / 2 |d|` and `d m_k / d p_k = sum_j M / |d|` ( the facet moves along `d = p_j - p_k` ), so
+        // `grad_w_k = sum_j ( g_k - g_j ) F / 2 |d|` and `grad_p_k = sum_j ( g_k - g_j ) M / |d|`, each from the cell's own facets
+        const double gk = P.out.vjp_g[ k ];
+        double gw = 0, gx = 0, gy = 0;
+        for ( int e = 0; e < nb; ++e ) {
+            const int id = c.I( e );
+            if ( id < 0 ) continue;
+            const int f = e + 1 < nb ? e + 1 : 0;
+            const double ax = c.X( e ), ay = c.Y( e ), bx = c.X( f ), by = c.Y( f );
+            const double ex = bx - ax, ey = by - ay, L = sqrt( ex * ex + ey * ey );
+            double F = 0, Mx = 0, My = 0;
+            if ( P.dn.kind == 1 ) {
+                grid_walk( g, ax, ay, bx, by, [&]( int i, int j, double t0, double t1 ) {
+                    const double v = g.val( i, j ) * L, dt = t1 - t0, h = ( t1 * t1 - t0 * t0 ) / 2;
+                    F += v * dt;
+                    Mx += v * ( ax * dt + ex * h );
+                    My += v * ( ay * dt + ey * h );
+                } );
+            } else {
+                F = P.dn.rho * L;
+                Mx = F * ( ax + bx ) / 2;
+                My = F * ( ay + by ) / 2;
+            }
+            const double4 s = P.wk.seeds[ id ];
+            const double dx = s.x - p.x, dy = s.y - p.y, dd = sqrt( dx * dx + dy * dy );
+            if ( ! ( dd > 0 ) ) continue;
+            const double dg = gk - P.out.vjp_g[ id ];
+            gw += dg * F / ( 2 * dd );
+            gx += dg * Mx / dd;
+            gy += dg * My / dd;
+        }
+        P.out.grad_w[ k ] = gw;
+        P.out.grad_pos[ 2 * k ] = gx;
+        P.out.grad_pos[ 2 * k + 1 ] = gy;
+        return;
+    }
     if ( P.mode == MODE_MOMENTS ) {
         double m = 0, mx = 0, my = 0, m2 = 0;
         if ( nb > 0 ) {
@@ -573,16 +697,16 @@ __global__ __launch_bounds__( CELL_BLOCK ) void k_cells_pass1( Params P ) {
 }
 
 /// the first pass of the FP32 kernel: the walk and the cuts in float ( the seed's frame; each plane computed in double,
-/// then rounded ), `CAP1` float slots in LDS; the polygon ( float vertices, the cut of each edge ) to `po`, slot `i` of
+/// then rounded ), `CAPW` float slots in LDS; the polygon ( float vertices, the cut of each edge ) to `po`, slot `i` of
 /// rank `k` at `i n + k`, and its vertex count ( -1: listed for the second pass, which runs the FP64 kernel )
 __global__ __launch_bounds__( CELL_BLOCK ) void k_cells_walk32( Params P, Poly32 po ) {
-    __shared__ float sx[ CAP1 * CELL_BLOCK ], sy[ CAP1 * CELL_BLOCK ];
-    __shared__ int sid[ CAP1 * CELL_BLOCK ];
+    __shared__ float sx[ CAPW * CELL_BLOCK ], sy[ CAPW * CELL_BLOCK ];
+    __shared__ int sid[ CAPW * CELL_BLOCK ];
     const int k = pass1_rank();
     const Store<CELL_BLOCK, float> c{ sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x };
     Ev ev;
     if ( k < P.n ) {
-        const int nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP1, ev );
+        const int nb = make_cell<Store<CELL_BLOCK, float>, NodeF>( P.wk, k, P.lo, P.hi, c, CAPW, ev );
         if ( nb < 0 ) {
             const unsigned at = atomicAdd( &P.out.counters[ 1 ], 1u );
             P.out.ovf_list[ at ] = k;
@@ -610,12 +734,13 @@ __global__ __launch_bounds__( CELL_BLOCK ) void k_cells_finish32( Params P, Poly
     const Store<CELL_BLOCK> c{ sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x };
     const bool live = k < P.n;
     const int nb = live ? po.nb[ k ] : 0;
-    const bool active = live && nb >= 0;
+    bool active = live && nb >= 0;
     if ( active && nb > 0 ) {
         const double4 p = P.wk.seeds[ k ];
         const double x0 = P.lo[ 0 ] - p.x, x1 = P.hi[ 0 ] - p.x, y0 = P.lo[ 1 ] - p.y, y1 = P.hi[ 1 ] - p.y;
         const double tol2 = 1e-8 * ( fmax( x0 * x0, x1 * x1 ) + fmax( y0 * y0, y1 * y1 ) );
         const size_t n = size_t( P.n );
+        bool bad = false;
         double ax, ay, ao, ea;
         plane_of( P, po.id[ size_t( nb - 1 ) * n + size_t( k ) ], p.x, p.y, p.z, ax, ay, ao, ea );
         for ( int i = 0; i < nb; ++i ) {
@@ -632,18 +757,63 @@ __global__ __launch_bounds__( CELL_BLOCK ) void k_cells_finish32( Params P, Poly
                 vy = ( ax * bo - bx * ao ) / det;
                 ok = ( ( vx - x ) * ( vx - x ) + ( vy - y ) * ( vy - y ) ) * det2 <= tol2 * e12;
             }
-            if ( ! ok ) {
-                const double r = eb > 0 ? ( bo - ( bx * x + by * y ) ) / eb : 0.0;
-                vx = x + r * bx;
-                vy = y + r * by;
-            }
+            bad |= ! ok;
             c.X( i ) = vx;
             c.Y( i ) = vy;
             c.I( i ) = id;
             ax = bx; ay = by; ao = bo; ea = eb;
         }
+        if ( bad ) {                                     // THE CONDITIONING TEST failed: the cell is redone by the FP64 kernel
+            const unsigned at = atomicAdd( &P.out.counters[ 1 ], 1u );
+            P.out.ovf_list[ at ] = k;
+            active = false;
+        }
     }
     finish_cell( P, k, c, nb, active, true );
+}
+
+/// the FP32 rows checked against each other: a facet `( k, j )` that only one of its two cells has ( a topology the floats
+/// decided differently: a vertex of one cell within the float's error of the other's plane ) lists BOTH cells, once, for the
+/// FP64 kernel. `mark` holds the stamp of the diagram that listed a rank last
+__global__ void k_sym_check( int n, const int *row_off, const int *row_cnt, const int *col, unsigned *mark, unsigned stamp, int *list,
+                             unsigned *count, const unsigned *counters, I64 cap ) {
+    const int k = int( blockIdx.x ) * int( blockDim.x ) + int( threadIdx.x );
+    if ( k >= n || I64( counters[ 0 ] ) > cap ) return;
+    const int ok = row_off[ k ], ck = row_cnt[ k ];
+    for ( int q = 0; q < ck; ++q ) {
+        const int j = col[ ok + q * ROW_STEP ];
+        const int oj = row_off[ j ], cj = row_cnt[ j ];
+        bool found = false;
+        for ( int r = 0; r < cj && ! found; ++r ) found = col[ oj + r * ROW_STEP ] == k;
+        if ( found ) continue;
+        if ( atomicExch( &mark[ k ], stamp ) != stamp ) list[ atomicAdd( count, 1u ) ] = k;
+        if ( atomicExch( &mark[ j ], stamp ) != stamp ) list[ atomicAdd( count, 1u ) ] = j;
+    }
+}
+
+/// the cells `k_sym_check` listed, redone by the FP64 kernel ( one lane per cell, `CAP2` slots in LDS ); each writes a new row
+/// and its measures over the FP32 ones. A cell past `CAP2` keeps its FP32 result
+__global__ __launch_bounds__( 64 ) void k_cells_redo( Params P, const int *list, const unsigned *count_p ) {
+    __shared__ double sx[ CAP2 * 64 ], sy[ CAP2 * 64 ];
+    __shared__ int sid[ CAP2 * 64 ];
+    const Store<64> c{ sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x };
+    const unsigned count = *count_p;
+    const unsigned lanes = gridDim.x * 64;
+    const unsigned rounds = ( count + lanes - 1 ) / lanes;
+    Ev ev;
+    for ( unsigned r = 0; r < rounds; ++r ) {
+        const unsigned idx = r * lanes + blockIdx.x * 64 + threadIdx.x;
+        const bool live = idx < count;
+        const int k = live ? list[ idx ] : 0;
+        int nb = 0;
+        bool active = live;
+        if ( live ) {
+            nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP2, ev );
+            active = nb >= 0;
+        }
+        finish_cell( P, k, c, nb, active, false );
+    }
+    flush_events( P, ev );
 }
 
 /// the second pass: the cells that outgrew `CAP1`, one lane per cell with `CAP2` slots in LDS, one wave per
@@ -694,11 +864,13 @@ __global__ __launch_bounds__( 64 ) void k_cells_pass3( Params P, double *gx, dou
             if ( nb < 0 ) {
                 if ( atomicAdd( &P.out.counters[ 2 ], 1u ) == 0 ) P.out.counters[ 3 ] = unsigned( k );
                 active = false;
-                P.out.mass[ k ] = 0;
+                if ( P.out.mass ) P.out.mass[ k ] = 0;
                 if ( P.mode == MODE_FACETS ) {
                     P.out.row_cnt[ k ] = 0; P.out.row_off[ k ] = 0; P.out.dia[ k ] = 1;
                     if ( P.out.edges ) P.out.nb_edges[ k ] = -1;
                 }
+                if ( P.mode == MODE_CELLS ) P.out.cell_nb[ k ] = 0;
+                if ( P.mode == MODE_VJP ) { P.out.grad_w[ k ] = 0; P.out.grad_pos[ 2 * k ] = 0; P.out.grad_pos[ 2 * k + 1 ] = 0; }
             }
         }
         finish_cell( P, k, c, nb, active, false );
@@ -712,7 +884,7 @@ __global__ __launch_bounds__( 64 ) void k_cells_pass3( Params P, double *gx, dou
 /// `dia` is the row's sum again. The value a lane reads in another row is one its owner keeps, so the rows are updated in
 /// place. Nothing is done when the rows did not fit ( the diagram is then redone )
 __global__ void k_sym_rows( int n, const int *row_off, const int *row_cnt, const int *col, double *val, double *dia, const unsigned *counters,
-                            I64 cap ) {
+                            I64 cap, unsigned *dropped ) {
     const int k = int( blockIdx.x ) * int( blockDim.x ) + int( threadIdx.x );
     if ( k >= n || I64( counters[ 0 ] ) > cap ) return;
     const int ok = row_off[ k ], ck = row_cnt[ k ];
@@ -727,6 +899,7 @@ __global__ void k_sym_rows( int n, const int *row_off, const int *row_cnt, const
             if ( col[ oj + r * ROW_STEP ] == k ) { mirror = val[ oj + r * ROW_STEP ]; break; }
         const double c = mirror < 0 ? 0.0 : ( j < k ? mirror : own );
         if ( c != own ) val[ at ] = c;
+        if ( mirror < 0 ) atomicAdd( dropped, 1u );
         sum += c;
     }
     dia[ k ] = sum > 0 ? sum : 1.0;
@@ -734,6 +907,11 @@ __global__ void k_sym_rows( int n, const int *row_off, const int *row_cnt, const
 
 __global__ void k_reset_counters( unsigned *c ) {
     if ( threadIdx.x < 8 ) c[ threadIdx.x ] = 0;
+}
+
+__global__ void k_fill_u32( I64 n, unsigned *v, unsigned x ) {
+    const I64 k = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( k < n ) v[ k ] = x;
 }
 
 void cells_scratch( DiagramOut &out, I64 n, const std::string &tag, hipStream_t st ) {
@@ -750,8 +928,11 @@ void cells_scratch( DiagramOut &out, I64 n, const std::string &tag, hipStream_t 
 void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st, bool fp32,
                 int ovf_slots ) {
     SDOT_CHECK( ovf_slots >= 64 && ovf_slots % 64 == 0, "cells: the third pass's lanes are a positive multiple of 64" );
+    SDOT_CHECK( ! fp32 || t.nodes32, "cells: the FP32 walk needs the tree's float records" );
     Params P;
     P.wk.nodes = t.nodes;
+    P.wk.nodes32 = t.nodes32;
+    P.wk.node_end = t.node_end;
     P.wk.seeds = t.seeds;
     P.wk.depth = t.depth;
     P.n = int( t.n );
@@ -766,9 +947,9 @@ void run_cells( const DevTree &t, const double *lo, const double *hi, const Dens
     Arena &A = arena();
     if ( fp32 ) {
         Poly32 po;
-        po.x = A.get<float>( "p32.x", size_t( CAP1 ) * size_t( t.n ) );
-        po.y = A.get<float>( "p32.y", size_t( CAP1 ) * size_t( t.n ) );
-        po.id = A.get<int>( "p32.id", size_t( CAP1 ) * size_t( t.n ) );
+        po.x = A.get<float>( "p32.x", size_t( CAPW ) * size_t( t.n ) );
+        po.y = A.get<float>( "p32.y", size_t( CAPW ) * size_t( t.n ) );
+        po.id = A.get<int>( "p32.id", size_t( CAPW ) * size_t( t.n ) );
         po.nb = A.get<int>( "p32.nb", size_t( t.n ) );
         k_cells_walk32<<<blocks_for( t.n, CELL_BLOCK ), CELL_BLOCK, 0, st>>>( P, po );
         check_launch( "k_cells_walk32" );
@@ -788,8 +969,25 @@ void run_cells( const DevTree &t, const double *lo, const double *hi, const Dens
     k_cells_pass3<<<unsigned( ovf_slots ) / 64, 64, 0, st>>>( P, gx, gy, gid, unsigned( ovf_slots ) );
     check_launch( "k_cells_pass3" );
     if ( fp32 && mode == MODE_FACETS ) {
+        // the cells whose facets the floats decided differently from their neighbours', redone by the FP64 kernel; then the rows
+        // made symmetric, a facet still one-sided dropped ( counted )
+        static std::atomic<unsigned> stamps{ 0 };
+        unsigned stamp = ++stamps;
+        if ( stamp == 0 ) stamp = ++stamps;
+        bool fresh = false;
+        unsigned *mark = A.get<unsigned>( "cells.mark", size_t( t.n ), &fresh );
+        int *redo = A.get<int>( "cells.redo", size_t( t.n ) );
+        if ( fresh ) {
+            k_fill_u32<<<blocks_for( t.n, 256 ), 256, 0, st>>>( t.n, mark, 0u );
+            check_launch( "k_fill_u32" );
+        }
+        k_sym_check<<<blocks_for( t.n, 256 ), 256, 0, st>>>( int( t.n ), out.row_off, out.row_cnt, out.col, mark, stamp, redo, &out.counters[ 6 ],
+                                                            out.counters, out.facet_cap );
+        check_launch( "k_sym_check" );
+        k_cells_redo<<<REDO_BLOCKS, 64, 0, st>>>( P, redo, &out.counters[ 6 ] );
+        check_launch( "k_cells_redo" );
         k_sym_rows<<<blocks_for( t.n, 256 ), 256, 0, st>>>( int( t.n ), out.row_off, out.row_cnt, out.col, out.val, out.dia, out.counters,
-                                                           out.facet_cap );
+                                                           out.facet_cap, &out.counters[ 7 ] );
         check_launch( "k_sym_rows" );
     }
     if ( timed ) timing().end( tp, st );

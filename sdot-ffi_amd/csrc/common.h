@@ -183,6 +183,13 @@ struct Timing {
     double spmv_bytes = 0;          ///< algorithmic bytes of the timed SpMV launches
     // the multigrid's HIP graphs ( counted whether timing is on or not ): batches of iterations captured, and replayed
     std::atomic<long long> graph_captures{ 0 }, graph_replays{ 0 };
+    // the FP32 kernel's diagrams in solves ( counted whether timing is on or not ): the diagrams, the cells redone by the FP64
+    // kernel for a facet only one side had, the facets still one-sided after that ( dropped )
+    std::atomic<long long> fp32_diagrams{ 0 }, fp32_redone{ 0 }, fp32_dropped{ 0 };
+    void fp32_cells( unsigned redone, unsigned dropped ) { ++fp32_diagrams; fp32_redone += redone; fp32_dropped += dropped; }
+    // the multigrid's FP32 refinement ( counted whether timing is on or not ): the solves, their FP32 rounds, and the solves
+    // finished by the FP64 multigrid because a round made no progress
+    std::atomic<long long> mg32_solves{ 0 }, mg32_rounds{ 0 }, mg32_fallbacks{ 0 };
     std::mutex mtx;
 
     hipEvent_t take() {
@@ -223,6 +230,8 @@ struct Timing {
         for ( int k = 0; k < TK_NB; ++k ) { ms[ k ] = 0; launches[ k ] = 0; }
         spmv_bytes = 0;
         graph_captures = 0;
+        fp32_diagrams = 0; fp32_redone = 0; fp32_dropped = 0;
+        mg32_solves = 0; mg32_rounds = 0; mg32_fallbacks = 0;
         graph_replays = 0;
     }
 };

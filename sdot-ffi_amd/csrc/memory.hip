@@ -24,10 +24,11 @@ void solve_parts( I64 n, int dim, int density_kind, const double *o, int precisi
     for ( int q = 0; q < SDOT_AMD_NB_MEMORY_PARTS; ++q ) parts[ q ] = 0;
     const double N = double( ( I64( 1 ) << max_depth_for( n, 10 ) ) - 1 ), half = ( N + 1 ) / 2, m = double( n );
     const bool d3 = dim == 3;
-    // the tree: records, boxes, slices, the level table, the permutation, the seeds, the majorants' moments; and its build's
-    // sort keys, segments and flags ( 2D on the device; 3D built on the host, uploaded )
-    if ( d3 ) parts[ 0 ] = 96 * N + 48 * N + 12 * N + 8 * m + 32 * m + ( 16 + 3 + 3 ) * 8 * N;
-    else      parts[ 0 ] = 64 * N + 32 * N + 12 * N + 8 * m + 32 * m + ( 11 + 2 + 3 ) * 8 * N +
+    // the tree: records, boxes, slices, the level table, the permutation, the seeds, the majorants' moments; and its device
+    // build's sort keys, sorted lists, segments and flags
+    if ( d3 ) parts[ 0 ] = 96 * N + 48 * N + 12 * N + 8 * m + 32 * m + ( 16 + 3 + 3 ) * 8 * N +
+                           ( 16 + 8 + 24 + 4 + 1 + 8 ) * m + 1024 * std::ceil( m / RS_TILE_B ) + 24 * half + 8 * N + 48 * N;
+    else      parts[ 0 ] = 64 * N + 32 * N + 32 * N + 12 * N + 8 * m + 32 * m + ( 11 + 2 + 3 ) * 8 * N +
                            ( 16 + 24 + 4 + 1 + 8 ) * m + 1024 * std::ceil( m / RS_TILE_B ) + 24 * half + 8 * N + 32 * N;
     // two diagram slots: masses, row offsets and counts, diagonals, the rows ( 12 n + 4096 entries, 3D: 44 per seed ), the
     // overflow lists and counters
@@ -37,7 +38,7 @@ void solve_parts( I64 n, int dim, int density_kind, const double *o, int precisi
     const bool limits = ! d3 && o && int( o[ O_STEP ] ) == 1;
     if ( limits ) parts[ 2 ] = 2 * ( EDGE_CAP * 4 * m + 4 * m );
     // the FP32 kernel's polygons ( 2D, 'fp32' and 'mixed' ); 3D with a sum of Gaussians: each cell's polytope
-    if ( ! d3 && precision != 0 ) parts[ 3 ] = CAP1 * 12 * m + 4 * m;
+    if ( ! d3 && precision != 0 ) parts[ 3 ] = CAPW * 12 * m + 4 * m + 8 * m;   // ( and the FP64 redo's marks and list )
     if ( d3 && density_kind == SDOT_AMD_GAUSSIANS ) parts[ 3 ] = 64 * m * ( 3 * 8 + 2 * 4 + 3 * 8 + 4 ) + 8 * m;
     // the third pass's vertex slots ( a fixed pool )
     parts[ 4 ] = d3 ? 0 : double( OVF_SLOTS ) * CAP3 * 20;
@@ -49,13 +50,23 @@ void solve_parts( I64 n, int dim, int density_kind, const double *o, int precisi
     if ( o && int( o[ O_LIN ] ) == 1 ) {
         const int pack = o[ O_MG_SHIFT ] > 0 ? 1 << int( o[ O_MG_SHIFT ] ) : 8;
         const int kc = o[ O_MG_KCYCLE ] >= 0 ? int( o[ O_MG_KCYCLE ] ) : 2;
-        double lin = 4 * 8 * m + 8 * m;
-        double nl = m;
-        for ( int l = 1; nl > 64 && l < 24; ++l ) {
-            nl = std::ceil( nl / pack );
-            lin += nl * ( 4 + MG_STRIDE_B * 12 + 8 ) + 3 * 8 * nl + ( l <= kc ? 5 * 8 * nl : 0 );
-        }
-        parts[ 7 ] = lin + 24 * 5 * 512 * 8 + 64 * 64 * 8;
+        // the levels and vectors in `s`-byte values: FP32 inside the FP64 refinement ( `mg_float` ), else FP64
+        const bool f32 = o[ O_MG_FLOAT ] != 0;
+        auto levels = [&]( double s ) {
+            double b = 4 * s * m + s * m;
+            double nl = m;
+            for ( int l = 1; nl > 64 && l < 24; ++l ) {
+                nl = std::ceil( nl / pack );
+                b += nl * ( 4 + MG_STRIDE_B * ( 4 + s ) + s ) + 3 * s * nl + ( l <= kc ? 5 * s * nl : 0 );
+            }
+            return b;
+        };
+        double lin = levels( 8 );
+        // the refinement: its FP32 levels, FP32 copy of the rows and diagonal, FP64 residual, scaled right-hand side and
+        // correction, and the FP64 correction of a stalled refinement ( whose FP64 levels are those above )
+        if ( f32 ) lin += levels( 4 ) + 4 * cap + 4 * m + 8 * m + 8 * m + 8 * m;
+        // each precision's partial sums, K-cycle partials and dense inverse of the coarsest level
+        parts[ 7 ] = lin + ( f32 ? 2 : 1 ) * ( 24 * 5 * 512 * 8 + 64 * 64 * 8 + 4 * 512 * 8 );
     } else
         parts[ 7 ] = 3 * 8 * m;
     // the density's copies: an image's blurred copies ( the width continuation ) and its pyramid of minima ( the limits )

@@ -31,6 +31,7 @@ enum Residual { R_LIN = 0, R_LOG = 1, R_POWER = 2 };
 /// the sparse Laplacian of one diagram, rank order ( rows of `DiagramOut` )
 struct Csr {
     I64 n = 0, nnz = 0, step = 1;       ///< entry `q` of row `i` at `row_off[ i ] + q * step`
+    I64 cap = 0;                        ///< the entries `col` / `val` hold ( every `row_off[ i ] + q * step` is below it )
     const int *row_off = nullptr, *row_cnt = nullptr, *col = nullptr;
     const double *val = nullptr, *dia = nullptr;
 };
@@ -41,6 +42,7 @@ struct CgStats {
     // the last solve's end: its residual by the recurrence and the true one ( relative ), a breakdown ( `p . q <= 0` )
     double last_err = 0, last_true = 0;
     bool last_breakdown = false;
+    int nb_fp64_fallbacks = 0;          ///< FP32 refinements that made no progress and were redone in FP64
 };
 
 /// Jacobi-preconditioned CG for `L d = b`, the unknown `gauge` held at 0 ( reference
@@ -77,10 +79,26 @@ void solve_2d( const SolveInputs &in, const SolveOutputs &out, hipStream_t st );
 
 /// the masses ( and facets ) of one diagram, user order ( see `sdot_amd_measures_2d` )
 void measures_2d( hipStream_t st, I64 n, const double *positions, const double *weights, const double *box, const Density &dn,
-                  double *masses, long long *fi, long long *fj, double *fc, I64 facet_cap, I64 *nb_facets, unsigned long long *events );
+                  double *masses, long long *fi, long long *fj, double *fc, I64 facet_cap, I64 *nb_facets, unsigned long long *events,
+                  bool fp32 = false );
 
 /// the masses ( and facets ) of one 3D diagram, user order ( see `sdot_amd_measures_3d` )
 void measures_3d( hipStream_t st, I64 n, const double *positions, const double *weights, const double *box, const Density3 &dn,
                   double *masses, long long *fi, long long *fj, double *fc, I64 facet_cap, I64 *nb_facets, unsigned long long *events );
+
+/// `( masses, first, second )` of each cell ( `int rho`, `int x rho`, `int |x|^2 rho` ), user order ( see `sdot_amd_moments_2d` )
+void moments_2d( hipStream_t st, I64 n, const double *positions, const double *weights, const double *box, const Density &dn, double *masses,
+                 double *first, double *second );
+void moments_3d( hipStream_t st, I64 n, const double *positions, const double *weights, const double *box, const Density3 &dn, double *masses,
+                 double *first, double *second );
+
+/// the cells' vertices ( see `sdot_amd_cells_2d` ); returns the vertex count, and writes nothing past the offsets' count when
+/// `xy` is null or the count passes `cap`
+I64 cells_2d( hipStream_t st, I64 n, const double *positions, const double *weights, const double *box, const Density &dn, long long *offsets,
+              double *xy, long long *cut, I64 cap );
+
+/// the adjoint of the masses ( see `sdot_amd_measures_vjp_2d` )
+void measures_vjp_2d( hipStream_t st, I64 n, const double *positions, const double *weights, const double *box, const Density &dn,
+                      const double *grad_masses, double *grad_weights, double *grad_positions );
 
 } // namespace sdot_amd

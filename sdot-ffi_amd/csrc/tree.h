@@ -18,12 +18,24 @@ struct alignas( 64 ) NodeRec {
 };
 static_assert( sizeof( NodeRec ) == 64, "one node, one 64-byte record" );
 
+/// the same node for the FP32 walk ( the role of the reference card's float `Node`, `gpu/Cell2D.cuh:167-169, 1462-1489` ):
+/// the box rounded OUTWARD to float, the slopes rounded to nearest and `wb` rounded UP after adding what the slopes' rounding
+/// can change on the box, so that `w <= wa . y + wb` still holds for the node's seeds; `beg` the first seed's rank, -1 for a
+/// node with no seed ( a leaf's end is read from `node_end` ). 32 bytes: two 16-byte loads
+struct alignas( 32 ) NodeF {
+    float lo_x, lo_y, hi_x, hi_y;
+    float wa_x, wa_y, wb;
+    int   beg;
+};
+static_assert( sizeof( NodeF ) == 32, "one float node, one 32-byte record" );
+
 /// the tree on the device, plus what its host build keeps
 struct DevTree {
     I64  n = 0, nb_nodes = 0;
     int  depth = 0, leaf = 10;
     // device
     NodeRec *nodes = nullptr;     ///< [ N ] the records the walk reads ( majorants refreshed per diagram )
+    NodeF   *nodes32 = nullptr;   ///< [ N ] the FP32 walk's records ( refreshed with `nodes` when set )
     double  *node_box = nullptr;  ///< [ N ][ 4 ] lo x, lo y, hi x, hi y
     int     *node_beg = nullptr;  ///< [ N ]
     int     *node_end = nullptr;  ///< [ N ]

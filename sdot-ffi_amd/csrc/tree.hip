@@ -165,6 +165,7 @@ void build_tree( DevTree &t, I64 n, const double *pos, int leaf, hipStream_t st,
 
     Arena &A = arena();
     t.nodes = A.get<NodeRec>( tag + ".nodes", size_t( N ) );
+    t.nodes32 = A.get<NodeF>( tag + ".nodes32", size_t( N ) );
     t.node_box = A.get<double>( tag + ".node_box", size_t( N ) * 4 );
     t.node_beg = A.get<int>( tag + ".node_beg", size_t( N ) );
     t.node_end = A.get<int>( tag + ".node_end", size_t( N ) );
@@ -365,7 +366,7 @@ __global__ void k_major_spread( const double4 *seeds, int n, const int *node_end
 
 /// the decision of every node ( `bsp_build_level.h:176-208` ) and its walk record
 __global__ void k_major_final( const double *S, int N, const double *fit, const unsigned long long *rk, const double *box, const int *beg,
-                               const int *end, NodeRec *nodes ) {
+                               const int *end, NodeRec *nodes, NodeF *nodes32 ) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if ( m >= N ) return;
     NodeRec r;
@@ -397,6 +398,18 @@ __global__ void k_major_final( const double *S, int N, const double *fit, const 
         r.wb = bb + 1e-6 * ( fabs( bb ) + spread + amax );
     }
     nodes[ m ] = r;
+    if ( nodes32 ) {
+        NodeF f;
+        f.lo_x = __double2float_rd( r.lo_x ); f.lo_y = __double2float_rd( r.lo_y );
+        f.hi_x = __double2float_ru( r.hi_x ); f.hi_y = __double2float_ru( r.hi_y );
+        f.wa_x = float( r.wa_x ); f.wa_y = float( r.wa_y );
+        // `( wa - wa_f ) . y` on the box, bounded by the farthest corner, goes into `wb` before it is rounded up
+        const double ex = fabs( r.wa_x - double( f.wa_x ) ) * fmax( fabs( r.lo_x ), fabs( r.hi_x ) );
+        const double ey = fabs( r.wa_y - double( f.wa_y ) ) * fmax( fabs( r.lo_y ), fabs( r.hi_y ) );
+        f.wb = __double2float_ru( r.wb + ex + ey );
+        f.beg = r.beg < r.end ? r.beg : -1;
+        nodes32[ m ] = f;
+    }
 }
 
 void refresh_majorants( DevTree &t, hipStream_t st ) {
@@ -418,7 +431,7 @@ void refresh_majorants( DevTree &t, hipStream_t st ) {
     check_launch( "k_major_fit" );
     k_major_spread<<<blocks_for( t.n, 256 ), 256, 0, st>>>( t.seeds, int( t.n ), t.node_end, depth, t.fit, N, t.rkeys );
     check_launch( "k_major_spread" );
-    k_major_final<<<blocks_for( N, 256 ), 256, 0, st>>>( t.stat, N, t.fit, t.rkeys, t.node_box, t.node_beg, t.node_end, t.nodes );
+    k_major_final<<<blocks_for( N, 256 ), 256, 0, st>>>( t.stat, N, t.fit, t.rkeys, t.node_box, t.node_beg, t.node_end, t.nodes, t.nodes32 );
     check_launch( "k_major_final" );
     if ( timed ) timing().end( tp, st );
 }

@@ -33,8 +33,9 @@ struct DevTree3 {
     unsigned long long *rkeys = nullptr;   ///< [ 3 ][ N ]
 };
 
-/// builds the tree of `pos_host[ n ][ 3 ]` ( user order ) on the host cores and uploads it; weights zero
-void build_tree3( DevTree3 &t, I64 n, const double *pos_host, int leaf, hipStream_t st, const std::string &tag );
+/// builds the tree from `pos_dev[ n ][ 3 ]` ( device, user order ): three sorted lists partitioned level by
+/// level, no copy of the cloud to the host
+void build_tree_device3( DevTree3 &t, I64 n, const double *pos_dev, int leaf, hipStream_t st, const std::string &tag );
 
 /// `w_user[ order[ k ] ]` / `w_rank[ k ]` into `seeds[ k ].w`
 void set_weights_user3( DevTree3 &t, const double *w_user_dev, hipStream_t st );

@@ -1,161 +1,13 @@
-// The 3D BSP tree: host build ( median cut on the longest axis, level by level, nodes of a level over the host cores )
-// and the device-side refresh of the affine weight majorants ( centred moments merged up the tree, one root -> leaf pass
-// over the seeds for the residual extremes; as the 2D refresh, with three coordinates ).
+// The 3D BSP tree's weights and the device-side refresh of its affine weight majorants: centred moments merged up the tree,
+// one root -> leaf pass over the seeds for the residual extremes, as the 2D refresh with three coordinates. The tree itself is
+// built on the device by `build_tree_device3` ( tree_gpu.hip ).
 
 #include "tree3.h"
 
 #include <algorithm>
 #include <cmath>
-#include <functional>
-#include <thread>
 
 namespace sdot_amd {
-
-int max_depth_for( I64 n, I64 leaf );
-
-namespace {
-
-void run_par( int nt, const std::function<void( int )> &f ) {
-    if ( nt <= 1 ) { f( 0 ); return; }
-    std::vector<std::thread> ts;
-    for ( int i = 0; i < nt; ++i ) ts.emplace_back( f, i );
-    for ( auto &t : ts ) t.join();
-}
-
-} // namespace
-
-void build_tree3( DevTree3 &t, I64 n, const double *pos, int leaf, hipStream_t st, const std::string &tag ) {
-    SDOT_CHECK( n >= 1, "a tree needs at least one seed" );
-    SDOT_CHECK( n < ( I64( 1 ) << 31 ) - 8, "more seeds than 32-bit ranks can index" );
-    const int depth = max_depth_for( n, leaf );
-    const I64 N = ( I64( 1 ) << depth ) - 1;
-    t.n = n; t.depth = depth; t.leaf = leaf; t.nb_nodes = N;
-
-    std::vector<double> P[ 3 ];
-    for ( auto &v : P ) v.resize( size_t( n ) );
-    std::vector<int> ord( ( size_t( n ) ) );
-    for ( I64 i = 0; i < n; ++i ) {
-        for ( int d = 0; d < 3; ++d ) P[ d ][ size_t( i ) ] = pos[ 3 * i + d ];
-        ord[ size_t( i ) ] = int( i );
-    }
-    std::vector<int> hbeg( ( size_t( N ) ) ), hend( ( size_t( N ) ) );
-    std::vector<double> hbox( size_t( N ) * 6, 0.0 );
-    std::vector<int> beg{ 0 }, end{ int( n ) };
-    const int hw = int( std::max( 1u, std::min( 16u, std::thread::hardware_concurrency() ) ) );
-    std::vector<int> idx( ( size_t( n ) ) );
-    for ( int level = 0; level < depth; ++level ) {
-        const I64 first = ( I64( 1 ) << level ) - 1;
-        const int nb = int( beg.size() );
-        std::vector<int> mid( end );
-        auto do_node = [&]( int j ) {
-            const int b = beg[ size_t( j ) ], e = end[ size_t( j ) ];
-            const I64 node = first + j;
-            hbeg[ size_t( node ) ] = b;
-            hend[ size_t( node ) ] = e;
-            if ( e <= b ) return;
-            double lo[ 3 ], hi[ 3 ];
-            for ( int d = 0; d < 3; ++d ) lo[ d ] = hi[ d ] = P[ d ][ size_t( b ) ];
-            for ( int k = b + 1; k < e; ++k )
-                for ( int d = 0; d < 3; ++d ) { lo[ d ] = std::min( lo[ d ], P[ d ][ size_t( k ) ] ); hi[ d ] = std::max( hi[ d ], P[ d ][ size_t( k ) ] ); }
-            double *bx = &hbox[ size_t( node ) * 6 ];
-            for ( int d = 0; d < 3; ++d ) { bx[ d ] = lo[ d ]; bx[ 3 + d ] = hi[ d ]; }
-            int ax = 0;
-            for ( int d = 1; d < 3; ++d ) if ( hi[ d ] - lo[ d ] > hi[ ax ] - lo[ ax ] ) ax = d;
-            if ( e - b > leaf && hi[ ax ] > lo[ ax ] ) {
-                const int m = b + ( e - b ) / 2;
-                const std::vector<double> &K = P[ ax ];
-                for ( int k = b; k < e; ++k ) idx[ size_t( k ) ] = k;
-                std::nth_element( idx.begin() + b, idx.begin() + m, idx.begin() + e, [&]( int u, int v ) {
-                    return K[ size_t( u ) ] < K[ size_t( v ) ] || ( K[ size_t( u ) ] == K[ size_t( v ) ] && u < v );
-                } );
-                std::vector<double> tp( size_t( e - b ) );
-                std::vector<int> to( size_t( e - b ) );
-                for ( int d = 0; d < 3; ++d ) {
-                    for ( int k = b; k < e; ++k ) tp[ size_t( k - b ) ] = P[ d ][ size_t( idx[ size_t( k ) ] ) ];
-                    std::copy( tp.begin(), tp.end(), P[ d ].begin() + b );
-                }
-                for ( int k = b; k < e; ++k ) to[ size_t( k - b ) ] = ord[ size_t( idx[ size_t( k ) ] ) ];
-                std::copy( to.begin(), to.end(), ord.begin() + b );
-                mid[ size_t( j ) ] = m;
-            }
-        };
-        if ( nb >= 4 * hw ) run_par( hw, [&]( int th ) { for ( int j = th; j < nb; j += hw ) do_node( j ); } );
-        else for ( int j = 0; j < nb; ++j ) do_node( j );
-        if ( level + 1 < depth ) {
-            std::vector<int> nb_( size_t( 2 * nb ) ), ne_( size_t( 2 * nb ) );
-            for ( int j = 0; j < nb; ++j ) {
-                nb_[ size_t( 2 * j ) ] = beg[ size_t( j ) ]; ne_[ size_t( 2 * j ) ] = mid[ size_t( j ) ];
-                nb_[ size_t( 2 * j + 1 ) ] = mid[ size_t( j ) ]; ne_[ size_t( 2 * j + 1 ) ] = end[ size_t( j ) ];
-            }
-            beg.swap( nb_ );
-            end.swap( ne_ );
-        }
-    }
-    // the seeds of each leaf in ascending user index ( a canonical cut order inside a leaf )
-    {
-        const I64 first = ( I64( 1 ) << ( depth - 1 ) ) - 1;
-        run_par( hw, [&]( int th ) {
-            std::vector<std::pair<int, int>> tmp;
-            std::vector<double> tp[ 3 ];
-            for ( I64 h = first + th; h < N; h += hw ) {
-                const int b = hbeg[ size_t( h ) ], e = hend[ size_t( h ) ];
-                if ( e - b < 2 ) continue;
-                tmp.clear();
-                for ( int k = b; k < e; ++k ) tmp.emplace_back( ord[ size_t( k ) ], k );
-                std::sort( tmp.begin(), tmp.end() );
-                for ( int d = 0; d < 3; ++d ) {
-                    tp[ d ].resize( tmp.size() );
-                    for ( size_t q = 0; q < tmp.size(); ++q ) tp[ d ][ q ] = P[ d ][ size_t( tmp[ q ].second ) ];
-                }
-                for ( size_t q = 0; q < tmp.size(); ++q ) {
-                    for ( int d = 0; d < 3; ++d ) P[ d ][ size_t( b ) + q ] = tp[ d ][ q ];
-                    ord[ size_t( b ) + q ] = tmp[ q ].first;
-                }
-            }
-        } );
-    }
-    std::vector<int> pre( ( size_t( N ) ), 0 );
-    for ( int level = 0; level + 1 < depth; ++level ) {
-        const int h = depth - level;
-        for ( I64 i = ( I64( 1 ) << level ) - 1; i < ( I64( 1 ) << ( level + 1 ) ) - 1; ++i ) {
-            pre[ size_t( 2 * i + 1 ) ] = pre[ size_t( i ) ] + 1;
-            pre[ size_t( 2 * i + 2 ) ] = pre[ size_t( i ) ] + int( I64( 1 ) << ( h - 1 ) );
-        }
-    }
-    std::vector<int> pbeg( ( size_t( N ) ) ), pend( ( size_t( N ) ) );
-    std::vector<double> pbox( size_t( N ) * 6 );
-    for ( I64 h = 0; h < N; ++h ) {
-        const int p = pre[ size_t( h ) ];
-        pbeg[ size_t( p ) ] = hbeg[ size_t( h ) ];
-        pend[ size_t( p ) ] = hend[ size_t( h ) ];
-        for ( int c = 0; c < 6; ++c ) pbox[ size_t( p ) * 6 + size_t( c ) ] = hbox[ size_t( h ) * 6 + size_t( c ) ];
-    }
-    std::vector<int> rank_of( ( size_t( n ) ) );
-    for ( I64 k = 0; k < n; ++k ) rank_of[ size_t( ord[ size_t( k ) ] ) ] = int( k );
-    std::vector<double4> seeds( ( size_t( n ) ) );
-    for ( I64 k = 0; k < n; ++k ) seeds[ size_t( k ) ] = make_double4( P[ 0 ][ size_t( k ) ], P[ 1 ][ size_t( k ) ], P[ 2 ][ size_t( k ) ], 0.0 );
-
-    Arena &A = arena();
-    t.nodes = A.get<NodeRec3>( tag + ".nodes3", size_t( N ) );
-    t.node_box = A.get<double>( tag + ".node_box3", size_t( N ) * 6 );
-    t.node_beg = A.get<int>( tag + ".node_beg3", size_t( N ) );
-    t.node_end = A.get<int>( tag + ".node_end3", size_t( N ) );
-    t.lvl_nodes = A.get<int>( tag + ".lvl_nodes3", size_t( N ) );
-    t.order = A.get<int>( tag + ".order3", size_t( n ) );
-    t.rank_of = A.get<int>( tag + ".rank_of3", size_t( n ) );
-    t.seeds = A.get<double4>( tag + ".seeds3", size_t( n ) );
-    t.stat = A.get<double>( tag + ".stat3", size_t( N ) * 16 );
-    t.fit = A.get<double>( tag + ".fit3", size_t( N ) * 3 );
-    t.rkeys = A.get<unsigned long long>( tag + ".rkeys3", size_t( N ) * 3 );
-    SDOT_HIP( hipMemcpyAsync( t.node_box, pbox.data(), sizeof( double ) * pbox.size(), hipMemcpyHostToDevice, st ) );
-    SDOT_HIP( hipMemcpyAsync( t.node_beg, pbeg.data(), sizeof( int ) * size_t( N ), hipMemcpyHostToDevice, st ) );
-    SDOT_HIP( hipMemcpyAsync( t.node_end, pend.data(), sizeof( int ) * size_t( N ), hipMemcpyHostToDevice, st ) );
-    SDOT_HIP( hipMemcpyAsync( t.lvl_nodes, pre.data(), sizeof( int ) * size_t( N ), hipMemcpyHostToDevice, st ) );
-    SDOT_HIP( hipMemcpyAsync( t.order, ord.data(), sizeof( int ) * size_t( n ), hipMemcpyHostToDevice, st ) );
-    SDOT_HIP( hipMemcpyAsync( t.rank_of, rank_of.data(), sizeof( int ) * size_t( n ), hipMemcpyHostToDevice, st ) );
-    SDOT_HIP( hipMemcpyAsync( t.seeds, seeds.data(), sizeof( double4 ) * size_t( n ), hipMemcpyHostToDevice, st ) );
-    SDOT_HIP( hipStreamSynchronize( st ) );              // ( the host vectors die with this scope )
-}
 
 // ---- weights -------------------------------------------------------------------------------------------------------
 

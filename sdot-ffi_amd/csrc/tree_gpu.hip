@@ -15,6 +15,7 @@
 // Everything stays on the device; the host only issues launches ( their count is a function of `n` ).
 
 #include "tree.h"
+#include "tree3.h"
 
 namespace sdot_amd {
 
@@ -29,11 +30,11 @@ __device__ __forceinline__ unsigned long long dkey( double x ) {
     return ( u >> 63 ) ? ~u : ( u | 0x8000000000000000ull );
 }
 
-/// the keys of one axis, and the identity permutation
-__global__ void k_keys( I64 n, const double *pos, int axis, unsigned long long *key, int *val ) {
+/// the keys of one axis ( `dim` coordinates a seed ), and the identity permutation
+__global__ void k_keys( I64 n, const double *pos, int dim, int axis, unsigned long long *key, int *val ) {
     const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
     if ( i >= n ) return;
-    key[ i ] = dkey( pos[ 2 * i + axis ] );
+    key[ i ] = dkey( pos[ dim * i + axis ] );
     val[ i ] = int( i );
 }
 
@@ -266,7 +267,168 @@ __global__ void k_to_preorder( I64 N, const int *pre, const int *hbeg, const int
     for ( int c = 0; c < 4; ++c ) box[ 4 * size_t( p ) + c ] = hbox[ 4 * size_t( h ) + c ];
 }
 
+// ---- 3D: the same scheme with three sorted lists ----------------------------------------------------------------------
+//
+// Every node's slice is a contiguous segment of each of the three lists. The cut list ( the longest axis' ) decides each
+// seed's side; then EVERY list is stably partitioned by those sides ( for the cut list itself the partition is the identity:
+// its first half is the left child ), one exclusive scan per list.
+
+struct Lists3 { const int *id[ 3 ]; };
+
+/// one node of level `L` in 3D: its slice, box ( the ends of the three lists' segments ) and cut
+__global__ void k_level_nodes3( int count, int first, const int *beg, const int *end, const double *pos, Lists3 L, int leaf, double *hbox,
+                                int *axis, int *mid, int *hbeg, int *hend ) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( j >= count ) return;
+    const int b = beg[ j ], e = end[ j ], node = first + j;
+    hbeg[ node ] = b;
+    hend[ node ] = e;
+    double *bx = hbox + 6 * size_t( node );
+    if ( e <= b ) {
+        for ( int c = 0; c < 6; ++c ) bx[ c ] = 0;
+        axis[ j ] = 0;
+        mid[ j ] = e;
+        return;
+    }
+    double lo[ 3 ], hi[ 3 ];
+    for ( int d = 0; d < 3; ++d ) {
+        lo[ d ] = pos[ 3 * size_t( L.id[ d ][ b ] ) + d ];
+        hi[ d ] = pos[ 3 * size_t( L.id[ d ][ e - 1 ] ) + d ];
+        bx[ d ] = lo[ d ];
+        bx[ 3 + d ] = hi[ d ];
+    }
+    int ax = 0;                                          // ( the first longest axis, as the host build picks it )
+    for ( int d = 1; d < 3; ++d ) if ( hi[ d ] - lo[ d ] > hi[ ax ] - lo[ ax ] ) ax = d;
+    axis[ j ] = ax;
+    mid[ j ] = ( e - b > leaf && hi[ ax ] > lo[ ax ] ) ? b + ( e - b ) / 2 : e;
+}
+
+/// the side of every seed from its node's cut list: `flag[ id ] = position < mid`
+__global__ void k_flags3( I64 n, const int *seg, const int *axis, const int *mid, Lists3 L, unsigned char *flag ) {
+    const I64 p = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( p >= n ) return;
+    const int j = seg[ p ];
+    flag[ L.id[ axis[ j ] ][ p ] ] = p < mid[ j ];
+}
+
+/// the left flags of list `ids`, in its order
+__global__ void k_list_flags( I64 n, const int *ids, const unsigned char *flag, unsigned *f ) {
+    const I64 p = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( p < n ) f[ p ] = flag[ ids[ p ] ];
+}
+
+/// the stable partition of list `ids` inside every segment by the flags ( `s` their exclusive scan in the list's order )
+__global__ void k_partition_list( I64 n, const int *seg, const int *beg, const int *mid, const int *ids, const unsigned char *flag,
+                                  const unsigned *s, int *out ) {
+    const I64 p = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( p >= n ) return;
+    const int j = seg[ p ];
+    const int b = beg[ j ], m = mid[ j ], id = ids[ p ];
+    const unsigned left_before = s[ p ] - s[ b ];
+    out[ flag[ id ] ? b + int( left_before ) : m + int( ( p - b ) - left_before ) ] = id;
+}
+
+__global__ void k_rank_tables3( I64 n, const int *ids, const double *pos, int *order, int *rank_of, double4 *seeds ) {
+    const I64 k = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( k >= n ) return;
+    const int u = ids[ k ];
+    order[ k ] = u;
+    rank_of[ u ] = int( k );
+    seeds[ k ] = make_double4( pos[ 3 * size_t( u ) ], pos[ 3 * size_t( u ) + 1 ], pos[ 3 * size_t( u ) + 2 ], 0.0 );
+}
+
+__global__ void k_to_preorder3( I64 N, const int *pre, const int *hbeg, const int *hend, const double *hbox, int *beg, int *end, double *box ) {
+    const I64 h = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( h >= N ) return;
+    const int p = pre[ h ];
+    beg[ p ] = hbeg[ h ];
+    end[ p ] = hend[ h ];
+    for ( int c = 0; c < 6; ++c ) box[ 6 * size_t( p ) + c ] = hbox[ 6 * size_t( h ) + c ];
+}
+
 } // namespace
+
+void build_tree_device3( DevTree3 &t, I64 n, const double *pos, int leaf, hipStream_t st, const std::string &tag ) {
+    SDOT_CHECK( n >= 1, "a tree needs at least one seed" );
+    SDOT_CHECK( n < ( I64( 1 ) << 31 ) - 8, "more seeds than 32-bit ranks can index" );
+    const int depth = max_depth_for( n, leaf );
+    const I64 N = ( I64( 1 ) << depth ) - 1;
+    t.n = n; t.depth = depth; t.leaf = leaf; t.nb_nodes = N;
+    Arena &A = arena();
+    t.nodes = A.get<NodeRec3>( tag + ".nodes3", size_t( N ) );
+    t.node_box = A.get<double>( tag + ".node_box3", size_t( N ) * 6 );
+    t.node_beg = A.get<int>( tag + ".node_beg3", size_t( N ) );
+    t.node_end = A.get<int>( tag + ".node_end3", size_t( N ) );
+    t.lvl_nodes = A.get<int>( tag + ".lvl_nodes3", size_t( N ) );
+    t.order = A.get<int>( tag + ".order3", size_t( n ) );
+    t.rank_of = A.get<int>( tag + ".rank_of3", size_t( n ) );
+    t.seeds = A.get<double4>( tag + ".seeds3", size_t( n ) );
+    t.stat = A.get<double>( tag + ".stat3", size_t( N ) * 16 );
+    t.fit = A.get<double>( tag + ".fit3", size_t( N ) * 3 );
+    t.rkeys = A.get<unsigned long long>( tag + ".rkeys3", size_t( N ) * 3 );
+    // scratch ( shared with the 2D build's names: one build at a time on a thread's arena )
+    unsigned long long *ka = A.get<unsigned long long>( "tb.ka", size_t( n ) ), *kb = A.get<unsigned long long>( "tb.kb", size_t( n ) );
+    int *va = A.get<int>( "tb.va", size_t( n ) ), *vb = A.get<int>( "tb.vb", size_t( n ) );
+    int *lst[ 3 ], *lso[ 3 ];
+    for ( int d = 0; d < 3; ++d ) {
+        lst[ d ] = A.get<int>( "tb3.l" + std::to_string( d ), size_t( n ) );
+        lso[ d ] = A.get<int>( "tb3.o" + std::to_string( d ), size_t( n ) );
+    }
+    const int nt = int( ( n + RS_TILE - 1 ) / RS_TILE );
+    unsigned *hist = A.get<unsigned>( "tb.hist", size_t( 256 ) * nt );
+    const I64 half = I64( 1 ) << ( depth - 1 );
+    int *beg = A.get<int>( "tb.beg", size_t( half ) ), *end = A.get<int>( "tb.end", size_t( half ) );
+    int *nbeg = A.get<int>( "tb.nbeg", size_t( half ) ), *nend = A.get<int>( "tb.nend", size_t( half ) );
+    int *axis = A.get<int>( "tb.axis", size_t( half ) ), *mid = A.get<int>( "tb.mid", size_t( half ) );
+    int *hbeg = A.get<int>( "tb.hbeg", size_t( N ) ), *hend = A.get<int>( "tb.hend", size_t( N ) );
+    double *hbox = A.get<double>( "tb3.hbox", size_t( N ) * 6 );
+    int *seg = A.get<int>( "tb.seg", size_t( n ) );
+    unsigned char *flag = A.get<unsigned char>( "tb.flag", size_t( n ) );
+    unsigned *f = A.get<unsigned>( "tb.f", size_t( n ) ), *s = A.get<unsigned>( "tb.s", size_t( n ) );
+    const I64 nsb = ( n + 1023 ) / 1024;
+    unsigned *bsum = A.get<unsigned>( "tb.bsum", size_t( nsb ) );
+    const unsigned nbk = blocks_for( n, 256 );
+
+    // 1. the three sorted lists
+    for ( int ax = 0; ax < 3; ++ax ) {
+        k_keys<<<nbk, 256, 0, st>>>( n, pos, 3, ax, ka, va );
+        const Sorted r = radix_sort( n, ka, va, kb, vb, hist, st );
+        SDOT_HIP( hipMemcpyAsync( lst[ ax ], r.val, sizeof( int ) * size_t( n ), hipMemcpyDeviceToDevice, st ) );
+    }
+    // 2. the levels
+    const int zero = 0, nn = int( n );
+    SDOT_HIP( hipMemcpyAsync( beg, &zero, sizeof( int ), hipMemcpyHostToDevice, st ) );
+    SDOT_HIP( hipMemcpyAsync( end, &nn, sizeof( int ), hipMemcpyHostToDevice, st ) );
+    for ( int L = 0; L < depth; ++L ) {
+        const int count = 1 << L, first = count - 1;
+        k_level_nodes3<<<blocks_for( count, 256 ), 256, 0, st>>>( count, first, beg, end, pos, Lists3{ { lst[ 0 ], lst[ 1 ], lst[ 2 ] } }, leaf,
+                                                                  hbox, axis, mid, hbeg, hend );
+        if ( L + 1 == depth ) break;
+        k_fill_seg<<<count, 256, 0, st>>>( count, beg, end, seg );
+        k_flags3<<<nbk, 256, 0, st>>>( n, seg, axis, mid, Lists3{ { lst[ 0 ], lst[ 1 ], lst[ 2 ] } }, flag );
+        for ( int d = 0; d < 3; ++d ) {
+            k_list_flags<<<nbk, 256, 0, st>>>( n, lst[ d ], flag, f );
+            k_scan_blocks<<<unsigned( nsb ), 1024, 0, st>>>( n, f, s, bsum );
+            k_scan_single<<<1, 1024, 0, st>>>( bsum, int( nsb ) );
+            k_scan_add<<<nbk, 256, 0, st>>>( n, s, bsum );
+            k_partition_list<<<nbk, 256, 0, st>>>( n, seg, beg, mid, lst[ d ], flag, s, lso[ d ] );
+        }
+        for ( int d = 0; d < 3; ++d ) std::swap( lst[ d ], lso[ d ] );
+        k_children<<<blocks_for( count, 256 ), 256, 0, st>>>( count, beg, end, mid, nbeg, nend );
+        std::swap( beg, nbeg );
+        std::swap( end, nend );
+        check_launch( "3D tree level" );
+    }
+    // 3. leaves, rank tables, preorder
+    const int L = depth - 1;
+    k_leaves<<<blocks_for( I64( 1 ) << L, 256 ), 256, 0, st>>>( 1 << L, ( 1 << L ) - 1, hbeg, hend, lst[ 0 ] );
+    k_rank_tables3<<<nbk, 256, 0, st>>>( n, lst[ 0 ], pos, t.order, t.rank_of, t.seeds );
+    for ( int l = 0; l < depth; ++l ) k_preorder<<<blocks_for( I64( 1 ) << l, 256 ), 256, 0, st>>>( l, depth, t.lvl_nodes );
+    k_to_preorder3<<<blocks_for( N, 256 ), 256, 0, st>>>( N, t.lvl_nodes, hbeg, hend, hbox, t.node_beg, t.node_end, t.node_box );
+    check_launch( "3D tree finish" );
+    // `beg` / `end` came from host constants ( `zero`, `nn` ): the copies must be done before they go
+    SDOT_HIP( hipStreamSynchronize( st ) );
+}
 
 void build_tree_device( DevTree &t, I64 n, const double *pos, int leaf, hipStream_t st, const std::string &tag ) {
     SDOT_CHECK( n >= 1, "a tree needs at least one seed" );
@@ -276,6 +438,7 @@ void build_tree_device( DevTree &t, I64 n, const double *pos, int leaf, hipStrea
     t.n = n; t.depth = depth; t.leaf = leaf; t.nb_nodes = N;
     Arena &A = arena();
     t.nodes = A.get<NodeRec>( tag + ".nodes", size_t( N ) );
+    t.nodes32 = A.get<NodeF>( tag + ".nodes32", size_t( N ) );
     t.node_box = A.get<double>( tag + ".node_box", size_t( N ) * 4 );
     t.node_beg = A.get<int>( tag + ".node_beg", size_t( N ) );
     t.node_end = A.get<int>( tag + ".node_end", size_t( N ) );
@@ -308,7 +471,7 @@ void build_tree_device( DevTree &t, I64 n, const double *pos, int leaf, hipStrea
 
     // 1. the two sorted lists
     for ( int ax = 0; ax < 2; ++ax ) {
-        k_keys<<<nbk, 256, 0, st>>>( n, pos, ax, ka, va );
+        k_keys<<<nbk, 256, 0, st>>>( n, pos, 2, ax, ka, va );
         const Sorted r = radix_sort( n, ka, va, kb, vb, hist, st );
         SDOT_HIP( hipMemcpyAsync( ax ? yid : xid, r.val, sizeof( int ) * size_t( n ), hipMemcpyDeviceToDevice, st ) );
     }

@@ -10,7 +10,8 @@ from .SdotPlanNd import SdotPlanNd
 from ._native import KernelFailure, SdotError
 from .batch import solve_batch
 from .serve import Server
+from .quantization import quantization
 
 __all__ = [ "Image", "SumOfDiracs", "SumOfGaussians", "UniformBox", "Direct", "Iterative", "OtNotConverged", "OtProblem", "Tuning",
             "ot_solve", "PowerDiagram", "axis_aligned_box", "box_half_spaces", "SdotPlanNd", "KernelFailure", "SdotError",
-            "solve_batch", "Server" ]
+            "solve_batch", "Server", "quantization" ]

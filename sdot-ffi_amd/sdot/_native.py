@@ -32,7 +32,9 @@ EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_byte
             "sdot_amd_linear_solves_2d", "sdot_amd_solve_2d_batch", "sdot_amd_server_start", "sdot_amd_server_submit",
             "sdot_amd_server_wait", "sdot_amd_server_stop", "sdot_amd_measures_3d", "sdot_amd_solve_3d",
             "sdot_amd_solve_2d_precision", "sdot_amd_graph_counts", "sdot_amd_solve_bytes", "sdot_amd_server_submit_nd", "sdot_amd_server_info",
-            "sdot_amd_measures_2d_noalloc", "sdot_amd_measures_2d_noalloc_max_seeds" ]
+            "sdot_amd_measures_2d_noalloc", "sdot_amd_measures_2d_noalloc_max_seeds", "sdot_amd_fp32_counts", "sdot_amd_mg32_counts",
+            "sdot_amd_measures_2d_precision", "sdot_amd_moments_2d", "sdot_amd_moments_3d", "sdot_amd_cells_2d",
+            "sdot_amd_measures_vjp_2d", "sdot_amd_solve_2d_batch_precision" ]
 
 CONSTANT, IMAGE, GAUSSIANS = 0, 1, 2
 
@@ -67,6 +69,16 @@ def lib():
         L.sdot_amd_tree_nodes.restype = _LL
         L.sdot_amd_tree_nodes.argtypes = [ _LL, _I ]
         L.sdot_amd_measures_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P, _P, _LL, _P, _P ]
+        L.sdot_amd_measures_2d_precision.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P, _P, _LL, _P, _P, _I ]
+        L.sdot_amd_measures_2d_precision.restype = _I
+        L.sdot_amd_moments_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P ]
+        L.sdot_amd_moments_2d.restype = _I
+        L.sdot_amd_moments_3d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _P, _P, _P ]
+        L.sdot_amd_moments_3d.restype = _I
+        L.sdot_amd_cells_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P, _LL, _P ]
+        L.sdot_amd_cells_2d.restype = _I
+        L.sdot_amd_measures_vjp_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P ]
+        L.sdot_amd_measures_vjp_2d.restype = _I
         L.sdot_amd_measures_2d_noalloc.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P ]
         L.sdot_amd_measures_2d_noalloc_max_seeds.restype = _LL
         L.sdot_amd_solve_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _LL, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P ]
@@ -75,6 +87,8 @@ def lib():
         L.sdot_amd_refresh_majorants_2d.argtypes = [ _P, _LL, _LL, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_linear_solves_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _LL, _P, _P, _P ]
         L.sdot_amd_solve_2d_batch.argtypes = [ _P, _I, _LL, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_solve_2d_batch_precision.argtypes = L.sdot_amd_solve_2d_batch.argtypes + [ _I ]
+        L.sdot_amd_solve_2d_batch_precision.restype = _I
         L.sdot_amd_measures_3d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _P, _P, _P, _P, _LL, _P, _P ]
         L.sdot_amd_solve_3d.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _I, _D, _P, _I, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_server_start.argtypes = [ _I, _D ]
@@ -90,6 +104,10 @@ def lib():
         L.sdot_amd_set_timing.restype = _I
         L.sdot_amd_kernel_times.argtypes = [ _P, _P, _P, _P, _I ]
         L.sdot_amd_kernel_times.restype = _I
+        L.sdot_amd_fp32_counts.argtypes = [ _P, _P, _P ]
+        L.sdot_amd_fp32_counts.restype = _I
+        L.sdot_amd_mg32_counts.argtypes = [ _P, _P, _P ]
+        L.sdot_amd_mg32_counts.restype = _I
         L.sdot_amd_graph_counts.argtypes = [ _P, _P ]
         L.sdot_amd_graph_counts.restype = _I
         L.sdot_amd_solve_bytes.argtypes = [ _LL, _I, _I, _P, _I, _P ]
@@ -157,8 +175,9 @@ def _host( a ):
 # ---- the calls -------------------------------------------------------------------------------------------
 
 
-def measures_2d( positions, weights, box, density, facets = False, events = False ):
-    """masses ( device tensor, user order ); with `facets` also `( fi, fj, fc )` ( device tensors ); with `events` the counts"""
+def measures_2d( positions, weights, box, density, facets = False, events = False, precision = 0 ):
+    """masses ( device tensor, user order ); with `facets` also `( fi, fj, fc )` ( device tensors ); with `events` the counts;
+    `precision` FP64 or FP32 ( the kernel's float )"""
     T = torch()
     pos = on_device( positions ).reshape( -1, 2 ).contiguous()
     n = pos.shape[ 0 ]
@@ -173,9 +192,9 @@ def measures_2d( positions, weights, box, density, facets = False, events = Fals
     fc = T.empty( cap, dtype = T.float64, device = device() ) if facets else None
     nbf = np.zeros( 1, np.int64 )
     ev = np.zeros( 5, np.uint64 )
-    _check( lib().sdot_amd_measures_2d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop, _ptr( masses ),
-                                        _ptr( fi ), _ptr( fj ), _ptr( fc ), cap, nbf.ctypes.data_as( _P ),
-                                        ev.ctypes.data_as( _P ) if events else None ) )
+    _check( lib().sdot_amd_measures_2d_precision( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop,
+                                                  _ptr( masses ), _ptr( fi ), _ptr( fj ), _ptr( fc ), cap, nbf.ctypes.data_as( _P ),
+                                                  ev.ctypes.data_as( _P ) if events else None, int( precision ) ) )
     res = [ masses ]
     if facets:
         k = int( nbf[ 0 ] )
@@ -183,6 +202,67 @@ def measures_2d( positions, weights, box, density, facets = False, events = Fals
     if events:
         res.append( dict( zip( ( "side_tests", "prune_vertex_tests", "new_vertices", "node_visits", "cuts" ), [ int( x ) for x in ev ] ) ) )
     return res[ 0 ] if len( res ) == 1 else tuple( res )
+
+
+def _diagram_args( positions, weights, box, density, dim = 2 ):
+    pos = on_device( positions ).reshape( -1, dim ).contiguous()
+    w = None if weights is None else on_device( weights ).reshape( -1 )
+    return pos, w, _host( box )
+
+
+def moments( positions, weights, box, density ):
+    """`( masses, first, second )` of each cell ( device tensors, user order; `first` `[ n, d ]` )"""
+    T = torch()
+    dim = len( box ) // 2
+    pos, w, ( boxh, boxp ) = _diagram_args( positions, weights, box, density, dim )
+    n = pos.shape[ 0 ]
+    m = T.empty( n, dtype = T.float64, device = device() )
+    f = T.empty( ( n, dim ), dtype = T.float64, device = device() )
+    s2 = T.empty( n, dtype = T.float64, device = device() )
+    if dim == 3:
+        kind, rho, g, ng = _density3_args( density )
+        _check( lib().sdot_amd_moments_3d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( g ), ng, _ptr( m ), _ptr( f ), _ptr( s2 ) ) )
+    else:
+        kind, rho, img, nx, ny, geo = density.native_args( device_tensor = on_device )
+        geoh, geop = _host( geo )
+        _check( lib().sdot_amd_moments_2d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop, _ptr( m ),
+                                           _ptr( f ), _ptr( s2 ) ) )
+    return m, f, s2
+
+
+def cells_2d( positions, weights, box, density ):
+    """every cell's vertices: `( offsets [ n + 1 ], vertices [ V, 2 ], cuts [ V ] )` ( device tensors, user order; a cut is a seed's
+    user index, or `-1 - f` for side `f` of the box )"""
+    T = torch()
+    pos, w, ( boxh, boxp ) = _diagram_args( positions, weights, box, density )
+    n = pos.shape[ 0 ]
+    kind, rho, img, nx, ny, geo = density.native_args( device_tensor = on_device )
+    geoh, geop = _host( geo )
+    nv = np.zeros( 1, np.int64 )
+    _check( lib().sdot_amd_cells_2d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop, None, None, None, 0,
+                                     nv.ctypes.data_as( _P ) ) )
+    cap = int( nv[ 0 ] )
+    off = T.empty( n + 1, dtype = T.int64, device = device() )
+    xy = T.empty( ( max( cap, 1 ), 2 ), dtype = T.float64, device = device() )
+    cut = T.empty( max( cap, 1 ), dtype = T.int64, device = device() )
+    _check( lib().sdot_amd_cells_2d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop, _ptr( off ), _ptr( xy ),
+                                     _ptr( cut ), cap, nv.ctypes.data_as( _P ) ) )
+    return off, xy[ :cap ], cut[ :cap ]
+
+
+def measures_vjp_2d( positions, weights, box, density, grad_masses ):
+    """`( grad_weights [ n ], grad_positions [ n, 2 ] )`: the cotangent of the masses pulled back ( device tensors, user order )"""
+    T = torch()
+    pos, w, ( boxh, boxp ) = _diagram_args( positions, weights, box, density )
+    n = pos.shape[ 0 ]
+    g = on_device( grad_masses ).reshape( -1 )
+    kind, rho, img, nx, ny, geo = density.native_args( device_tensor = on_device )
+    geoh, geop = _host( geo )
+    gw = T.empty( n, dtype = T.float64, device = device() )
+    gp = T.empty( ( n, 2 ), dtype = T.float64, device = device() )
+    _check( lib().sdot_amd_measures_vjp_2d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop, _ptr( g ),
+                                            _ptr( gw ), _ptr( gp ) ) )
+    return gw, gp
 
 
 def measures_2d_noalloc( positions, weights, box, density, masses = None ):
@@ -300,7 +380,7 @@ def solve_2d( positions, nu, w0, options, box, density, dups = None, precision =
                  stats = stats, history = hist[ :int( nb_steps[ 0 ] ) ] )
 
 
-def solve_2d_batch( instances, options, streams = 8 ):
+def solve_2d_batch( instances, options, streams = 8, precision = 0 ):
     """`k` independent solves at once ( see `sdot_amd_solve_2d_batch` ): `instances` is a list of `( positions, nu, w0,
     box, density )`, every density of the same kind; `options` the 41-double vector they share. Returns one dict per
     instance: device tensors ( views of the concatenated outputs ), `stats`, `cost`, `error` ( kind, code, value: 0 when
@@ -345,11 +425,11 @@ def solve_2d_batch( instances, options, streams = 8 ):
     costs = np.zeros( k )
     errors = np.zeros( ( k, 3 ), np.int64 )
     times = np.zeros( ( k, 2 ) )
-    _check( lib().sdot_amd_solve_2d_batch( _stream(), int( streams ), k, offsets.ctypes.data_as( _P ), _ptr( P ), _ptr( nu ), _ptr( w0 ), optp,
+    _check( lib().sdot_amd_solve_2d_batch_precision( _stream(), int( streams ), k, offsets.ctypes.data_as( _P ), _ptr( P ), _ptr( nu ), _ptr( w0 ), optp,
                                            boxes.ctypes.data_as( _P ), kind, rho.ctypes.data_as( _P ), ctypes.cast( img_ptrs, _P ),
                                            geo.ctypes.data_as( _P ), _ptr( weights ), _ptr( masses ), _ptr( bary ),
                                            stats.ctypes.data_as( _P ), costs.ctypes.data_as( _P ), errors.ctypes.data_as( _P ),
-                                           times.ctypes.data_as( _P ) ) )
+                                           times.ctypes.data_as( _P ), int( precision ) ) )
     out = []
     for i in range( k ):
         a, b = int( offsets[ i ] ), int( offsets[ i + 1 ] )
@@ -423,6 +503,22 @@ def kernel_times( reset = False ):
                                          int( bool( reset ) ) ) )
     return ( { k: ( float( ms[ i ] ), int( la[ i ] ) ) for i, k in enumerate( TIMED_KERNELS ) }, float( sb[ 0 ] ),
              dict( zip( ( "side_tests", "prune_vertex_tests", "new_vertices", "node_visits", "cuts" ), [ int( x ) for x in ev ] ) ) )
+
+
+def fp32_counts():
+    """the FP32 kernel's diagrams since the last reset of the kernel times, the cells it had redone in FP64 for a one-sided
+    facet, and the facets dropped as still one-sided: `{ diagrams, redone, dropped }`"""
+    d, r, x = _LL(), _LL(), _LL()
+    _check( lib().sdot_amd_fp32_counts( ctypes.byref( d ), ctypes.byref( r ), ctypes.byref( x ) ) )
+    return dict( diagrams = int( d.value ), redone = int( r.value ), dropped = int( x.value ) )
+
+
+def mg32_counts():
+    """the multigrid's FP32 refinement since the last reset of the kernel times: the systems it solved, its FP32 rounds, and
+    the systems finished in FP64 after a round without progress: `{ solves, rounds, fallbacks }`"""
+    s, r, f = _LL(), _LL(), _LL()
+    _check( lib().sdot_amd_mg32_counts( ctypes.byref( s ), ctypes.byref( r ), ctypes.byref( f ) ) )
+    return dict( solves = int( s.value ), rounds = int( r.value ), fallbacks = int( f.value ) )
 
 
 def graph_counts():

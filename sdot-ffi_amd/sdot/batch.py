@@ -6,8 +6,9 @@ streams ( `streams` ), largest first, each stream driven by its own host thread 
 chip stays busy while each solve waits on its own decisions; an instance's result does not depend on what ran beside
 it ( the kernels are deterministic, nothing is shared ). One call of `sdot_amd_solve_2d_batch`.
 
-What a batched plan does not carry: the per-step `history` ( empty ) and the exact duplicates' merge ( `aggregate` is
-done by single solves ).
+The cells' float follows `settings.precision` as a single solve's does ( 'auto' and 'mixed': the FP32 kernel, then the FP64
+one from the first step it cannot take ), so each plan is the one `problem.solve( settings )` returns. What a batched plan does
+not carry: the per-step `history` ( empty ) and the exact duplicates' merge ( `aggregate` is done by single solves ).
 """
 
 import numpy as np
@@ -24,8 +25,6 @@ def solve_batch( problems, settings = None, streams = 8 ):
     "raise", in which case the first one raises `OtNotConverged` after the whole batch ran."""
     from .OtProblem import Iterative, OtNotConverged
     settings = settings if settings is not None else Iterative()
-    if settings.precision in ( "fp32", "mixed" ):
-        raise NotImplementedError( "batched solves run FP64 cells ( precision 'auto' or 'fp64' )" )
     problems = list( problems )
     if not problems:
         return []
@@ -41,7 +40,8 @@ def solve_batch( problems, settings = None, streams = 8 ):
         w0 = settings.weights0 if settings.weights0 is not None else None
         instances.append( ( src.positions, src.weights, w0, [ lo[ 0 ], lo[ 1 ], hi[ 0 ], hi[ 1 ] ], dst ) )
     opts = card_options( settings, problems[ 0 ].target )
-    res = _native.solve_2d_batch( instances, opts, streams = streams )
+    from .SdotPlanNd import _KERNEL_FLOAT
+    res = _native.solve_2d_batch( instances, opts, streams = streams, precision = _KERNEL_FLOAT[ settings.precision ] )
     plans = []
     for pb, r in zip( problems, res ):
         if r[ "error" ][ 0 ] != 0:

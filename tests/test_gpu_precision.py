@@ -96,3 +96,44 @@ def test_mixed_switches_to_fp64_where_fp32_stagnates( native ):
     # counted, and `it_double` is that iteration's index ( one stage here, so `nb_iter` counts it and those after it )
     assert 0 < m[ "it_double" ] and m[ "it_double" ] + 1 <= m[ "nb_iter" ]
     assert m[ "residual" ] < f[ "residual" ]
+
+
+@pytest.mark.parametrize( "label", [ "uniform", "lines", "lines weighted" ] )
+def test_fp32_neighbour_sets_equal_the_lifted_hull( native, label ):
+    """the FP32 kernel's facets ( float walk, vertices re-solved in double, one-sided facets redone in FP64 ): the same neighbour
+    sets as the lifted hull on generic inputs, up to facets shorter than 1e-10, and its masses within 1e-12 of the FP64 kernel's"""
+    rng = np.random.default_rng( 5 )
+    pos, w = next( ( p, ww ) for lab, p, ww in H.clouds( rng, 3000 ) if lab == label )
+    say( f"case fp32 neighbours: { label }" )
+    native.kernel_times( reset = True )
+    from sdot.distributions import UniformBox
+    m32, ( fi, fj, _ ) = native.measures_2d( pos, w, [ 0, 0, 1, 1 ], UniformBox( [ 0, 0 ], [ 1, 1 ] ), facets = True, precision = native.FP32 )
+    m64 = native.measures_2d( pos, w, [ 0, 0, 1, 1 ], UniformBox( [ 0, 0 ], [ 1, 1 ] ) )
+    counts = native.fp32_counts()
+    say( f"  fp32 repairs: { counts }" )
+    assert counts[ "diagrams" ] == 1 and counts[ "dropped" ] == 0
+    ours = { ( min( i, j ), max( i, j ) ) for i, j in zip( fi.cpu().tolist(), fj.cpu().tolist() ) }
+    hull, amb = H.lifted_neighbours( pos, w, ambiguous = True )
+    assert hull <= ours <= hull | amb, ( len( hull - ours ), len( ours - hull - amb ) )
+    assert float( ( m32 - m64 ).abs().max() ) <= 1e-12 * float( m64.abs().max() )
+
+
+def test_fp32_topology_repairs_at_1m_on_the_bench_image( native ):
+    """a diagram of the bench's size ( 1M seeds, 1024^2 image of 8 bumps ) at weights of a cell's order by the FP32 kernel: no
+    facet left one-sided, masses within 1e-6 of a mean cell of the FP64 kernel's ( 1e-3 of the bench's tolerance )"""
+    import torch
+    from sdot import Image
+    rng = np.random.default_rng( 1 )
+    n = 1_000_000
+    pos = torch.as_tensor( rng.uniform( 0, 1, ( n, 2 ) ), device = "cuda" )
+    img = H.bumps_image( 1024, 1024, np.random.default_rng( 2 ), k = 8 )
+    dens = Image( img, origin = [ 0, 0 ], frame = np.eye( 2 ) / 1024 ).normalized_version()
+    say( "case fp32 repairs: 1M, 1024^2 image" )
+    w = torch.as_tensor( rng.uniform( -0.3, 0.3, n ) / n, device = "cuda" )
+    native.kernel_times( reset = True )
+    m32 = native.measures_2d( pos, w, [ 0, 0, 1, 1 ], dens, precision = native.FP32 )
+    m64 = native.measures_2d( pos, w, [ 0, 0, 1, 1 ], dens )
+    counts = native.fp32_counts()
+    say( f"  fp32 repairs: { counts }" )
+    assert counts[ "dropped" ] == 0
+    assert float( ( m32 - m64 ).abs().max() ) <= 1e-6 / n

@@ -74,16 +74,12 @@ def test_card_options_carry_the_duplicates():
 
 
 def test_precision_maps_to_the_kernels_float():
-    """'auto' and 'mixed' run the FP32 kernel then FP64 ( as the reference's card ), 'fp32' and 'fp64' one kernel; the batched
-    path runs FP64 cells and refuses the others before any device work; 3D refuses 'fp32'"""
-    from sdot import Iterative, OtProblem, SumOfDiracs, UniformBox, _native, solve_batch
+    """'auto' and 'mixed' run the FP32 kernel then FP64 ( as the reference's card ), 'fp32' and 'fp64' one kernel; 3D refuses
+    'fp32' before any device work"""
+    from sdot import Iterative, OtProblem, SumOfDiracs, UniformBox, _native
     from sdot.SdotPlanNd import _KERNEL_FLOAT
     assert _KERNEL_FLOAT == { "auto": _native.MIXED, "mixed": _native.MIXED, "fp32": _native.FP32, "fp64": _native.FP64 }
     rng = np.random.default_rng( 0 )
-    pb = OtProblem( SumOfDiracs( rng.uniform( 0, 1, ( 50, 2 ) ) ), UniformBox( [ 0, 0 ], [ 1, 1 ] ) )
-    for precision in ( "fp32", "mixed" ):
-        with pytest.raises( NotImplementedError ):
-            solve_batch( [ pb ], Iterative( precision = precision ) )
     pb3 = OtProblem( SumOfDiracs( rng.uniform( 0, 1, ( 50, 3 ) ) ), UniformBox( [ 0, 0, 0 ], [ 1, 1, 1 ] ) )
     with pytest.raises( NotImplementedError ):
         pb3.solve( Iterative( precision = "fp32" ) )
