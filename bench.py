@@ -439,8 +439,8 @@ def main():
                                device_bytes = _native.device_bytes(),
                                fp32_repairs = dict( diagrams = fp32c[ "diagrams" ], cells_redone_fp64 = fp32c[ "redone" ],
                                                     facets_dropped = fp32c[ "dropped" ] ),
-                               mg_fp32_refinement = dict( systems = mg32[ "solves" ], fp32_rounds = mg32[ "rounds" ],
-                                                          finished_in_fp64 = mg32[ "fallbacks" ] ) ) )
+                               mg_fp32_preconditioner = dict( systems = mg32[ "solves" ], iterations = mg32[ "iterations" ],
+                                                              redone_in_fp64 = mg32[ "fallbacks" ] ) ) )
 
     # the FP64 cells of the returned weights: the error of the plan against the FP64 kernel ( the oracle's cells: equal to the
     # bit on a constant density, to 1e-14 relative on an image, by the parity tests )

@@ -64,11 +64,11 @@ int sdot_amd_graph_counts( long long *captures, long long *replays );
  * ( dropped from the Laplacian ). HOST [ 1 ] each, may be null. */
 int sdot_amd_fp32_counts( long long *diagrams, long long *redone, long long *dropped );
 
-/* THE MULTIGRID'S FP32 REFINEMENT ( the reference's card runs its levels in float, `gpu/Linear2D.cuh:993-1000`, with no FP64
- * outer loop to count ): since the last `sdot_amd_kernel_times( ..., reset = 1 )`, the Newton systems solved by FP32 multigrid
- * rounds inside the FP64 iterative refinement, the rounds, and the systems finished by the FP64 multigrid because a round did
- * not halve the FP64 residual. HOST [ 1 ] each, may be null. */
-int sdot_amd_mg32_counts( long long *solves, long long *rounds, long long *fallbacks );
+/* THE MULTIGRID'S FP32 PRECONDITIONER ( the reference's card keeps its levels in float, `gpu/Linear2D.cuh:993-1000` ): since
+ * the last `sdot_amd_kernel_times( ..., reset = 1 )`, the Newton systems solved by the FP64 flexible CG with the FP32 multigrid
+ * cycle as its preconditioner, their iterations, and the systems redone with the FP64 cycle because that solve did not converge.
+ * HOST [ 1 ] each, may be null. */
+int sdot_amd_mg32_counts( long long *solves, long long *iterations, long long *fallbacks );
 
 /* ONE DIAGRAM: the masses of the power cells of `( positions, weights )` in the box
  * `box = { lo_x, lo_y, hi_x, hi_y }` ( HOST ) against a density, and optionally the facets

@@ -200,10 +200,10 @@ int sdot_amd_fp32_counts( long long *diagrams, long long *redone, long long *dro
     } );
 }
 
-int sdot_amd_mg32_counts( long long *solves, long long *rounds, long long *fallbacks ) {
+int sdot_amd_mg32_counts( long long *solves, long long *iterations, long long *fallbacks ) {
     return guarded( [&] {
         if ( solves ) *solves = timing().mg32_solves.load();
-        if ( rounds ) *rounds = timing().mg32_rounds.load();
+        if ( iterations ) *iterations = timing().mg32_iterations.load();
         if ( fallbacks ) *fallbacks = timing().mg32_fallbacks.load();
     } );
 }

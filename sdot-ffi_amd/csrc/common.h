@@ -187,9 +187,9 @@ struct Timing {
     // kernel for a facet only one side had, the facets still one-sided after that ( dropped )
     std::atomic<long long> fp32_diagrams{ 0 }, fp32_redone{ 0 }, fp32_dropped{ 0 };
     void fp32_cells( unsigned redone, unsigned dropped ) { ++fp32_diagrams; fp32_redone += redone; fp32_dropped += dropped; }
-    // the multigrid's FP32 refinement ( counted whether timing is on or not ): the solves, their FP32 rounds, and the solves
-    // finished by the FP64 multigrid because a round made no progress
-    std::atomic<long long> mg32_solves{ 0 }, mg32_rounds{ 0 }, mg32_fallbacks{ 0 };
+    // the multigrid's FP32 preconditioner ( counted whether timing is on or not ): the solves, their flexible-CG iterations, and
+    // the solves redone with the FP64 preconditioner because they did not converge
+    std::atomic<long long> mg32_solves{ 0 }, mg32_iterations{ 0 }, mg32_fallbacks{ 0 };
     std::mutex mtx;
 
     hipEvent_t take() {
@@ -231,7 +231,7 @@ struct Timing {
         spmv_bytes = 0;
         graph_captures = 0;
         fp32_diagrams = 0; fp32_redone = 0; fp32_dropped = 0;
-        mg32_solves = 0; mg32_rounds = 0; mg32_fallbacks = 0;
+        mg32_solves = 0; mg32_iterations = 0; mg32_fallbacks = 0;
         graph_replays = 0;
     }
 };

@@ -50,10 +50,12 @@ void solve_parts( I64 n, int dim, int density_kind, const double *o, int precisi
     if ( o && int( o[ O_LIN ] ) == 1 ) {
         const int pack = o[ O_MG_SHIFT ] > 0 ? 1 << int( o[ O_MG_SHIFT ] ) : 8;
         const int kc = o[ O_MG_KCYCLE ] >= 0 ? int( o[ O_MG_KCYCLE ] ) : 2;
-        // the levels and vectors in `s`-byte values: FP32 inside the FP64 refinement ( `mg_float` ), else FP64
+        // the coarse levels in `s`-byte values; the FP64 flexible CG's four vectors and the finest level's work vector; with the
+        // FP32 preconditioner ( `mg_float` ) its own levels, work vector, FP32 copies of the rows, diagonal, `r` and `z`, beside
+        // the FP64 solve it falls back to
         const bool f32 = o[ O_MG_FLOAT ] != 0;
-        auto levels = [&]( double s ) {
-            double b = 4 * s * m + s * m;
+        auto coarse = [&]( double s ) {
+            double b = 0;
             double nl = m;
             for ( int l = 1; nl > 64 && l < 24; ++l ) {
                 nl = std::ceil( nl / pack );
@@ -61,10 +63,8 @@ void solve_parts( I64 n, int dim, int density_kind, const double *o, int precisi
             }
             return b;
         };
-        double lin = levels( 8 );
-        // the refinement: its FP32 levels, FP32 copy of the rows and diagonal, FP64 residual, scaled right-hand side and
-        // correction, and the FP64 correction of a stalled refinement ( whose FP64 levels are those above )
-        if ( f32 ) lin += levels( 4 ) + 4 * cap + 4 * m + 8 * m + 8 * m + 8 * m;
+        double lin = 4 * 8 * m + 8 * m + coarse( 8 );
+        if ( f32 ) lin += 4 * 8 * m + 4 * m + coarse( 4 ) + 4 * cap + 4 * m + 8 * m;
         // each precision's partial sums, K-cycle partials and dense inverse of the coarsest level
         parts[ 7 ] = lin + ( f32 ? 2 : 1 ) * ( 24 * 5 * 512 * 8 + 64 * 64 * 8 + 4 * 512 * 8 );
     } else

@@ -836,19 +836,28 @@ struct FcgResult {
     double err = 0, true_err = 0;        ///< relative residual by the recurrence, and the true one
 };
 
-/// THE FLEXIBLE CG preconditioned by one multigrid cycle, every vector and level in `R`, the scalars in double: `A0 d = b` from
-/// `d = 0` to the relative residual `tol` in at most `maxit` iterations. The arena's buffers are named after `pfx`, so the
-/// FP64 and the FP32 solves of the refinement keep their own.
-template<class R>
-FcgResult fcg_mg( const MatT<R> &A0m, I64 nnz, const R *b, R *d, double tol, int maxit, const MgOptions &o, hipStream_t st, int &nb_levels,
-                  const std::string &pfx ) {
+/// `y = x` in another float
+template<class X, class Y>
+__global__ void k_convert( I64 n, const X *x, Y *y ) {
+    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
+    if ( i < n ) y[ i ] = Y( x[ i ] );
+}
+
+/// THE FLEXIBLE CG preconditioned by one multigrid cycle: `A0 d = b` from `d = 0` to the relative residual `tol` in at most
+/// `maxit` iterations, the outer vectors and `A0` in `R`, the cycle's levels and vectors in `P` ( from `A0p`, `A0` in `P` ),
+/// the scalars in double. With `P = float` under `R = double` the preconditioner is an FP32 operator inside an FP64 Krylov
+/// iteration: its rounding only makes it a slightly different preconditioner, which the flexible CG's explicit `z . q`
+/// orthogonalisation absorbs. The arena's buffers are named after `pfx`, so the mixed and the FP64 solves keep their own.
+template<class R, class P>
+FcgResult fcg_mg( const MatT<R> &A0m, const MatT<P> &A0p, I64 nnz, const R *b, R *d, double tol, int maxit, const MgOptions &o,
+                  hipStream_t st, int &nb_levels, const std::string &pfx ) {
     Arena &A = arena();
     double *state = A.get<double>( pfx + ".state", S_NB_STATE );
     double *hs = pinned<double>( pfx + ".hstate", S_NB_STATE );
-    Hierarchy<R> H;
+    Hierarchy<P> H;
     H.part = A.get<double>( pfx + ".part", 4 * size_t( RED_BLOCKS ) );
     H.kpart = A.get<double>( pfx + ".kpart", size_t( MAX_LEVELS ) * K_NB * RED_BLOCKS );
-    H.A[ 0 ] = A0m;
+    H.A[ 0 ] = A0p;
     H.nl = 1;
     k_zero_state<<<1, 64, 0, st>>>( state );
     // the hierarchy
@@ -857,23 +866,23 @@ FcgResult fcg_mg( const MatT<R> &A0m, I64 nnz, const R *b, R *d, double tol, int
         const std::string tag = pfx + ".l" + std::to_string( l + 1 );
         int *rc = A.get<int>( tag + ".rc", size_t( nc ) );
         int *col = A.get<int>( tag + ".col", size_t( nc ) * MG_STRIDE );
-        R *val = A.get<R>( tag + ".val", size_t( nc ) * MG_STRIDE ), *dia = A.get<R>( tag + ".dia", size_t( nc ) );
-        k_coarse_op<R, R><<<blocks_for( nc, MG_BLOCK ), MG_BLOCK, 0, st>>>( H.A[ l ], o.pack, nc, rc, col, val, dia, state );
+        P *val = A.get<P>( tag + ".val", size_t( nc ) * MG_STRIDE ), *dia = A.get<P>( tag + ".dia", size_t( nc ) );
+        k_coarse_op<P, P><<<blocks_for( nc, MG_BLOCK ), MG_BLOCK, 0, st>>>( H.A[ l ], o.pack, nc, rc, col, val, dia, state );
         check_launch( "k_coarse_op" );
-        MatT<R> &C = H.A[ l + 1 ];
+        MatT<P> &C = H.A[ l + 1 ];
         C.n = nc; C.step = nc; C.row_off = nullptr; C.row_cnt = rc; C.col = col; C.val = val; C.dia = dia;
         H.nl = l + 2;
     }
     for ( int l = 0; l < H.nl; ++l ) {
         const std::string tag = pfx + ".v" + std::to_string( l );
         const size_t m = size_t( H.A[ l ].n );
-        H.t[ l ] = A.get<R>( tag + ".t", m );
+        H.t[ l ] = A.get<P>( tag + ".t", m );
         if ( l ) {
-            H.x[ l ] = A.get<R>( tag + ".x", m ); H.r[ l ] = A.get<R>( tag + ".r", m );
+            H.x[ l ] = A.get<P>( tag + ".x", m ); H.r[ l ] = A.get<P>( tag + ".r", m );
             if ( l <= o.kcycle && l < H.nl - 1 ) {
-                H.c1[ l ] = A.get<R>( tag + ".c1", m ); H.c2[ l ] = A.get<R>( tag + ".c2", m );
-                H.v1[ l ] = A.get<R>( tag + ".v1", m ); H.v2[ l ] = A.get<R>( tag + ".v2", m );
-                H.r2[ l ] = A.get<R>( tag + ".r2", m );
+                H.c1[ l ] = A.get<P>( tag + ".c1", m ); H.c2[ l ] = A.get<P>( tag + ".c2", m );
+                H.v1[ l ] = A.get<P>( tag + ".v1", m ); H.v2[ l ] = A.get<P>( tag + ".v2", m );
+                H.r2[ l ] = A.get<P>( tag + ".r2", m );
             }
         }
     }
@@ -886,17 +895,17 @@ FcgResult fcg_mg( const MatT<R> &A0m, I64 nnz, const R *b, R *d, double tol, int
         for ( int l = 1; l < H.nl; ++l )
             if ( H.A[ l ].n <= o.bottom ) { H.lb = l; break; }
     if ( H.lb < H.nl - 1 ) {
-        Bottom<R> &B = H.bottom;
+        Bottom<P> &B = H.bottom;
         B.nl = H.nl - H.lb; B.pack = o.pack; B.nu = o.nu; B.omega = damping_of( o ); B.nc = H.nc; B.inv = H.inv; B.rows = 0;
         for ( int l = H.lb; l < H.nl; ++l ) {
-            BottomLevel<R> &L = B.L[ l - H.lb ];
+            BottomLevel<P> &L = B.L[ l - H.lb ];
             L.n = H.A[ l ].n; L.cnt = H.A[ l ].row_cnt; L.col = H.A[ l ].col; L.val = H.A[ l ].val; L.dia = H.A[ l ].dia;
             L.x = H.x[ l ]; L.t = H.t[ l ]; L.r = H.r[ l ];
             B.rows += int( L.n );
         }
     } else
         H.lb = H.nl;                                     // ( a bottom of the coarsest alone: the dense apply as before )
-    k_dense_inverse<R><<<1, 256, 0, st>>>( H.A[ H.nl - 1 ], H.inv );
+    k_dense_inverse<P><<<1, 256, 0, st>>>( H.A[ H.nl - 1 ], H.inv );
     check_launch( "coarsest inverse" );
     nb_levels = H.nl;
 
@@ -904,12 +913,24 @@ FcgResult fcg_mg( const MatT<R> &A0m, I64 nnz, const R *b, R *d, double tol, int
     const I64 n = A0m.n;
     R *r = A.get<R>( pfx + ".r", size_t( n ) ), *z = A.get<R>( pfx + ".z", size_t( n ) );
     R *p = A.get<R>( pfx + ".p", size_t( n ) ), *q = A.get<R>( pfx + ".q", size_t( n ) );
-    const MatT<R> &A0 = H.A[ 0 ];
+    constexpr bool same = std::is_same<R, P>::value;
+    P *rp = nullptr, *zp = nullptr;                      // ( the cycle's own copies of `r` and `z` when it runs in another float )
+    if ( ! same ) { rp = A.get<P>( pfx + ".rp", size_t( n ) ); zp = A.get<P>( pfx + ".zp", size_t( n ) ); }
+    const MatT<R> &A0 = A0m;
     double *part = H.part;
+    // z = M( r )
+    auto precondition = [&]( hipStream_t s ) {
+        if constexpr ( same ) cycle( H, 0, r, z, o, state, s );
+        else {
+            k_convert<R, P><<<blocks_for( n, 256 ), 256, 0, s>>>( n, r, rp );
+            cycle( H, 0, rp, zp, o, state, s );
+            k_convert<P, R><<<blocks_for( n, 256 ), 256, 0, s>>>( n, zp, z );
+        }
+    };
     k_pcg_init<R><<<RED_BLOCKS, RED_THREADS, 0, st>>>( A0, b, d, r, part );
     k_pcg_init_final<<<1, RED_THREADS, 0, st>>>( part, state, tol );
     // z = M( r ), p = z
-    cycle( H, 0, r, z, o, state, st );
+    precondition( st );
     k_pcg_first_dir<R><<<blocks_for( n, 256 ), 256, 0, st>>>( n, z, p, state );
     check_launch( "pcg init" );
     const int batch = 4;
@@ -926,7 +947,7 @@ FcgResult fcg_mg( const MatT<R> &A0m, I64 nnz, const R *b, R *d, double tol, int
             }
             k_pcg_update<R><<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, p, q, d, r, part, state );
             k_pcg_check<<<1, RED_THREADS, 0, st>>>( part, state );
-            cycle( H, 0, r, z, o, state, st );
+            precondition( st );
             k_fcg_zq<R><<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, z, q, part, state );
             k_pcg_dir<R><<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, z, p, part, state );
         }
@@ -935,8 +956,8 @@ FcgResult fcg_mg( const MatT<R> &A0m, I64 nnz, const R *b, R *d, double tol, int
     hipGraphExec_t exec = nullptr;
     if ( o.graphs ) {
         Key key;
-        key.val( int( sizeof( R ) ) );
-        key.mat( A0 ); key.val( H.nl ); key.val( H.lb ); key.val( H.nc ); key.val( H.inv ); key.val( H.part ); key.val( H.kpart );
+        key.val( int( sizeof( R ) ) ); key.val( int( sizeof( P ) ) );
+        key.mat( A0 ); key.val( rp ); key.val( zp ); key.val( H.nl ); key.val( H.lb ); key.val( H.nc ); key.val( H.inv ); key.val( H.part ); key.val( H.kpart );
         for ( int l = 0; l < H.nl; ++l ) {
             key.mat( H.A[ l ] ); key.val( H.x[ l ] ); key.val( H.t[ l ] ); key.val( H.r[ l ] ); key.val( H.c1[ l ] ); key.val( H.c2[ l ] );
             key.val( H.v1[ l ] ); key.val( H.v2[ l ] ); key.val( H.r2[ l ] );
@@ -990,7 +1011,7 @@ FcgResult fcg_mg( const MatT<R> &A0m, I64 nnz, const R *b, R *d, double tol, int
     return res;
 }
 
-// ---- the iterative refinement: FP32 solves inside an FP64 residual ----------------------------------------------------
+// ---- the FP32 preconditioner's copy of the rows -----------------------------------------------------------------------
 
 /// the FP32 copy of a Laplacian's values and diagonal ( the same layout: rows at `row_off`, `step` apart )
 __global__ void k_to_float_rows( I64 n, const int *row_off, const int *row_cnt, I64 step, const double *val, const double *dia, float *valf,
@@ -1002,36 +1023,6 @@ __global__ void k_to_float_rows( I64 n, const int *row_off, const int *row_cnt, 
     for ( int q = 0; q < row_cnt[ i ]; ++q ) valf[ b + q * step ] = float( val[ b + q * step ] );
 }
 
-/// `rf = r / |r|`, `|r|` from the partial sums of `|r|^2` ( slot 0 ), summed by every block
-__global__ __launch_bounds__( RED_THREADS ) void k_scaled_to_float( I64 n, const double *r, const double *part, float *rf ) {
-    __shared__ double sh[ RED_THREADS ];
-    const double nr = sqrt( sum_parts( part, RED_BLOCKS, sh ) );
-    const double s = nr > 0 ? 1.0 / nr : 0.0;
-    for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < n; i += I64( gridDim.x ) * RED_THREADS ) rf[ i ] = float( r[ i ] * s );
-}
-
-/// `d += |r| e` ( the correction scaled back )
-__global__ __launch_bounds__( RED_THREADS ) void k_refine_add( I64 n, const float *e, const double *part, double *d ) {
-    __shared__ double sh[ RED_THREADS ];
-    const double nr = sqrt( sum_parts( part, RED_BLOCKS, sh ) );
-    for ( I64 i = I64( blockIdx.x ) * RED_THREADS + threadIdx.x; i < n; i += I64( gridDim.x ) * RED_THREADS ) d[ i ] += nr * double( e[ i ] );
-}
-
-__global__ __launch_bounds__( RED_THREADS ) void k_norm_to( const double *part, double *out ) {
-    __shared__ double sh[ RED_THREADS ];
-    const double v = sum_parts( part, RED_BLOCKS, sh );
-    if ( threadIdx.x == 0 ) out[ 0 ] = sqrt( v );
-}
-
-__global__ void k_add_f64( I64 n, const double *e, double *d ) {
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
-    if ( i < n ) d[ i ] += e[ i ];
-}
-
-__global__ void k_zero_f64( I64 n, double *v ) {
-    const I64 i = I64( blockIdx.x ) * blockDim.x + threadIdx.x;
-    if ( i < n ) v[ i ] = 0;
-}
 
 } // namespace
 
@@ -1048,70 +1039,27 @@ int mg_solve( const Csr &L, int gauge, const double *b, double *d, double tol, i
     bool done = false;
     double err = 1, true_err = 1;
     bool breakdown = false;
-    double *res64 = nullptr, nb = 0, nr = 0;           // ( the refinement's FP64 residual of `d`, | b |, its norm )
     if ( o.fp32 ) {
-        // ITERATIVE REFINEMENT: the residual `r = b - A d` in FP64, the correction by the FP32 flexible CG on the FP32 copy of the
-        // system ( scaled to a unit right-hand side ), `d += |r| e`; a few rounds, each to a relative 1e-4 ( or what is left to
-        // the tolerance ). Falls back to the FP64 solve when a round makes no progress
-        float *valf = A.get<float>( "mgr.valf", size_t( L.cap ) ), *diaf = A.get<float>( "mgr.diaf", size_t( n ) );
+        // THE FP32 PRECONDITIONER: the hierarchy, its smoothing and its coarse solves in FP32 ( from an FP32 copy of the rows ),
+        // the flexible CG, its SpMV and its vectors in FP64; the FP64 preconditioner when that solve does not converge
+        float *valf = A.get<float>( "mgm.valf", size_t( L.cap ) ), *diaf = A.get<float>( "mgm.diaf", size_t( n ) );
         k_to_float_rows<<<blocks_for( n, 256 ), 256, 0, st>>>( n, L.row_off, L.row_cnt, L.step, L.val, L.dia, valf, diaf );
         check_launch( "k_to_float_rows" );
         MatT<float> Af;
         Af.n = n; Af.step = L.step; Af.row_off = L.row_off; Af.row_cnt = L.row_cnt; Af.col = L.col; Af.val = valf; Af.dia = diaf;
-        double *r = res64 = A.get<double>( "mgr.r", size_t( n ) ), *part = A.get<double>( "mgr.part", 2 * size_t( RED_BLOCKS ) );
-        double *nrm = A.get<double>( "mgr.nrm", 2 );
-        double *hn = pinned<double>( "mgr.hn", 2 );
-        float *rf = A.get<float>( "mgr.rf", size_t( n ) ), *ef = A.get<float>( "mgr.ef", size_t( n ) );
-        k_zero_f64<<<blocks_for( n, 256 ), 256, 0, st>>>( n, d );
-        // | b |
-        k_true_res<double><<<RED_BLOCKS, RED_THREADS, 0, st>>>( A0, b, d, part, r );
-        k_norm_to<<<1, RED_THREADS, 0, st>>>( part, nrm );
-        SDOT_HIP( hipMemcpyAsync( hn, nrm, sizeof( double ), hipMemcpyDeviceToHost, st ) );
-        SDOT_HIP( hipStreamSynchronize( st ) );
-        nb = hn[ 0 ];
-        nr = nb;
-        if ( ! ( nb > 0 ) ) done = true;
-        for ( int round = 0; ! done && round < 6 && total < maxit; ++round ) {
-            const double inner = std::max( 0.5 * tol * nb / nr, 1e-4 );
-            k_scaled_to_float<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, r, part, rf );
-            const FcgResult f = fcg_mg<float>( Af, L.nnz, rf, ef, inner, maxit - total, o, st, nb_levels, "mgf" );
-            if ( f.bad ) return -2;
-            ++timing().mg32_rounds;
-            total += f.its;
-            breakdown |= f.failed;
-            if ( f.failed || f.its == 0 ) break;
-            k_refine_add<<<RED_BLOCKS, RED_THREADS, 0, st>>>( n, ef, part, d );
-            k_true_res<double><<<RED_BLOCKS, RED_THREADS, 0, st>>>( A0, b, d, part, r );
-            k_norm_to<<<1, RED_THREADS, 0, st>>>( part, nrm );
-            check_launch( "refinement round" );
-            SDOT_HIP( hipMemcpyAsync( hn, nrm, sizeof( double ), hipMemcpyDeviceToHost, st ) );
-            SDOT_HIP( hipStreamSynchronize( st ) );
-            const double nr2 = hn[ 0 ];
-            if ( ! ( nr2 < 0.5 * nr ) ) { nr = nr2; break; }   // ( no progress: the FP64 solve below )
-            nr = nr2;
-            err = true_err = nr / nb;
-            if ( nr <= tol * nb ) done = true;
-        }
-        ++timing().mg32_solves;
-        if ( ! done && nb > 0 ) { cs.nb_fp64_fallbacks += 1; ++timing().mg32_fallbacks; }
-        if ( done ) { err = true_err = nb > 0 ? nr / nb : 0.0; }
-    }
-    if ( ! done && res64 && nr < nb ) {
-        // the FP32 rounds stalled after some progress: the FP64 multigrid solves for the correction of `d` from its FP64 residual,
-        // to what is left of the tolerance
-        const double scale = nr / nb;
-        double *e = A.get<double>( "mgr.e", size_t( n ) );
-        const FcgResult f = fcg_mg<double>( A0, L.nnz, res64, e, tol / scale, std::max( maxit - total, 4 ), o, st, nb_levels, "mg" );
+        const FcgResult f = fcg_mg<double, float>( A0, Af, L.nnz, b, d, tol, maxit, o, st, nb_levels, "mgm" );
         if ( f.bad ) return -2;
-        k_add_f64<<<blocks_for( n, 256 ), 256, 0, st>>>( n, e, d );
-        check_launch( "k_add_f64" );
         total += f.its;
-        err = f.err * scale;
-        true_err = f.true_err * scale;
+        err = f.err;
+        true_err = f.true_err;
         breakdown = f.failed;
-        done = ! f.failed && err <= tol && true_err <= 10 * tol;
-    } else if ( ! done ) {
-        const FcgResult f = fcg_mg<double>( A0, L.nnz, b, d, tol, std::max( maxit - total, 4 ), o, st, nb_levels, "mg" );
+        done = ! f.failed && f.err <= tol && f.true_err <= 10 * tol;
+        ++timing().mg32_solves;
+        timing().mg32_iterations += f.its;
+        if ( ! done ) { cs.nb_fp64_fallbacks += 1; ++timing().mg32_fallbacks; }
+    }
+    if ( ! done ) {
+        const FcgResult f = fcg_mg<double, double>( A0, A0, L.nnz, b, d, tol, std::max( maxit - total, 4 ), o, st, nb_levels, "mg" );
         if ( f.bad ) return -2;
         total += f.its;
         err = f.err;

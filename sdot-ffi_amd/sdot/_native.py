@@ -514,11 +514,11 @@ def fp32_counts():
 
 
 def mg32_counts():
-    """the multigrid's FP32 refinement since the last reset of the kernel times: the systems it solved, its FP32 rounds, and
-    the systems finished in FP64 after a round without progress: `{ solves, rounds, fallbacks }`"""
+    """the multigrid's FP32 preconditioner since the last reset of the kernel times: the systems solved with it, their
+    flexible-CG iterations, and the systems redone with the FP64 preconditioner: `{ solves, iterations, fallbacks }`"""
     s, r, f = _LL(), _LL(), _LL()
     _check( lib().sdot_amd_mg32_counts( ctypes.byref( s ), ctypes.byref( r ), ctypes.byref( f ) ) )
-    return dict( solves = int( s.value ), rounds = int( r.value ), fallbacks = int( f.value ) )
+    return dict( solves = int( s.value ), iterations = int( r.value ), fallbacks = int( f.value ) )
 
 
 def graph_counts():
