@@ -665,7 +665,7 @@ int sdot_amd_linear_solves_2d( void *stream, long long n, const long long *row, 
         const double tol = options && options[ 3 ] > 0 ? options[ 3 ] : 1e-6;
         const bool mg = options && options[ 2 ] == 1;
         MgOptions mo;
-        mo.fp32 = options && options[ 5 ] == 1;          // ( the multigrid's corrections in FP32 inside an FP64 refinement )
+        mo.fp32 = options && options[ 5 ] == 1;          // ( the multigrid's cycle in FP32, the flexible CG in FP64 )
         if ( options && options[ 8 ] >= 0 ) mo.kcycle = int( options[ 8 ] );
         if ( options && options[ 9 ] > 0 ) mo.nu = int( options[ 9 ] );
         for ( long long j = 0; j < k; ++j ) {

@@ -504,7 +504,10 @@ __device__ void finish_cell( const Params &P, int k, const S &c, int nb, bool ac
     }
     if ( ! active ) return;
     Grid g{};
-    if ( P.dn.kind == 1 ) g = grid_from( P.dn, p.x, p.y );
+    if ( P.dn.kind == 1 ) {
+        g = grid_from( P.dn, p.x, p.y );
+        if ( nb > 0 ) g.anchor( c.X( 0 ) );
+    }
     if ( P.mode == MODE_CELLS ) {
         P.out.cell_nb[ k ] = nb;
         if ( P.out.cell_off ) {

@@ -16,6 +16,13 @@ struct Grid {
     int nx, ny;
     double x0, y0, hx, hy;   ///< x0, y0 in the seed's frame
     int iref;                ///< the column the row integrals start from
+    /// the row integrals start from the column of `x` ( seed frame ): any column that is the same for the whole cell gives
+    /// the same loop integral, and one in the cell keeps `F0`'s loop and terms of the cell's width, where a converged
+    /// transport puts cells tens of cells away from their seeds
+    __device__ void anchor( double x ) {
+        const double c = floor( ( x - x0 ) / hx );
+        iref = c < 0 ? 0 : ( c >= double( nx - 1 ) ? nx - 1 : int( c ) );
+    }
     __device__ double u( int i ) const { return x0 + double( i ) * hx; }
     __device__ double v( int j ) const { return y0 + double( j ) * hy; }
     __device__ double val( int i, int j ) const { return img[ size_t( j ) * size_t( nx ) + size_t( i ) ]; }

@@ -101,10 +101,10 @@ __device__ bool frozen_mass( const Along &al, I64 k, int nb, double t, double &m
         }
         return true;
     }
-    const Grid g = grid_from( al.dn, p.x, p.y );
+    Grid g = grid_from( al.dn, p.x, p.y );
     double acc = 0, fx = 0, fy = 0, qx = 0, qy = 0;
     const bool ok = for_each_vertex( al, k, nb, t, [&]( int q, double vx, double vy, double, double ) {
-        if ( q == 0 ) { fx = vx; fy = vy; }
+        if ( q == 0 ) { fx = vx; fy = vy; g.anchor( vx ); }
         else acc += segment_mass( g, qx, qy, vx, vy );
         qx = vx; qy = vy;
     } );

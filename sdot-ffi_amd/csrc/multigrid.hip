@@ -43,7 +43,7 @@ enum { K_RHO1 = 0, K_ALPHA1, K_BETA2, K_GAMMA, K_ALPHA2, K_NB };
 
 /// a level's matrix: entry `q` of row `i` at `base + q * step`, `base = row_off[ i ]` ( CSR, `step` 1 ) or `i`
 /// ( column-major, `row_off` null, `step` the row count ); the off-diagonal values are the `c_ij > 0`
-/// ( `R` the type the values are stored in: double, or float for the FP32 solves of the iterative refinement )
+/// ( `R` the type the values are stored in: double, or float for the FP32 preconditioner's levels )
 template<class R>
 struct MatT {
     I64 n = 0, step = 1;

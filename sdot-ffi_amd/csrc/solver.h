@@ -42,7 +42,7 @@ struct CgStats {
     // the last solve's end: its residual by the recurrence and the true one ( relative ), a breakdown ( `p . q <= 0` )
     double last_err = 0, last_true = 0;
     bool last_breakdown = false;
-    int nb_fp64_fallbacks = 0;          ///< FP32 refinements that made no progress and were redone in FP64
+    int nb_fp64_fallbacks = 0;          ///< systems the FP32 preconditioner did not bring to the tolerance, redone with the FP64 one
 };
 
 /// Jacobi-preconditioned CG for `L d = b`, the unknown `gauge` held at 0 ( reference

@@ -104,7 +104,7 @@ def card_options( settings, density, verbose = False, nb_dups = 0, gap = 0.0 ):
     if tun.mg_pack:
         put( "mg_shift", int( tun.mg_pack ).bit_length() - 1 )
     put( "mg_nu", tun.mg_nu or 0 ); put( "mg_kcycle", -1 if tun.mg_kcycle is None else tun.mg_kcycle )
-    # the multigrid's corrections in FP32 inside an FP64 refinement ( `mg_precision` "float", the reference card's default,
+    # the multigrid's cycle in FP32 as the FP64 flexible CG's preconditioner ( `mg_precision` "float", the reference card's float levels,
     # `SdotPlanNd.py:241, 809-812` ) or all in FP64 ( "double" )
     mg_precision = tun.mg_precision or "float"
     if mg_precision not in ( "float", "double" ):

@@ -1,6 +1,10 @@
 """3D parity on the MI355X, through the C ABI: the diagram's masses and facets against the oracle ( constant density and a
 sum of Gaussians ), closed forms ( the box's volume, a grid's equal cells ), and solves that reach the tolerance with the
-oracle's plan ( weights, masses, barycenters, cost )."""
+oracle's plan ( weights, masses, barycenters, cost ).
+
+The bar of the masses and facets is 1e-12 relative. With Gaussians of sigma 0.1, the cells far in their tails hold masses
+down to 1e-9 of the mean ( the oracle's rounding even makes a few slightly negative ): those are held to 1e-12 of the mean
+mass, every other cell to 1e-12 of its own."""
 import itertools
 
 import numpy as np
@@ -22,7 +26,7 @@ def rows_of( fi, fj, fc ):
     return { ( int( i ), int( j ) ): float( c ) for i, j, c in zip( fi, fj, fc ) }
 
 
-@pytest.mark.parametrize( "kind,n,seed", [ ( "lebesgue", 3000, 0 ), ( "gaussians", 3000, 1 ), ( "lebesgue", 20000, 2 ) ] )
+@pytest.mark.parametrize( "kind,n,seed", [ ( "lebesgue", 3000, 0 ), ( "gaussians", 3000, 1 ), ( "lebesgue", 20000, 2 ), ( "gaussians", 20000, 3 ) ] )
 def test_3d_masses_and_facets_match_the_oracle( native, oracle, kind, n, seed ):
     print( f"3d measures: { kind } n={ n } seed={ seed }", flush = True )
     rng = np.random.default_rng( seed )
@@ -32,12 +36,11 @@ def test_3d_masses_and_facets_match_the_oracle( native, oracle, kind, n, seed ):
     m, ( fi, fj, fc ), ev = native.measures_3d( pos, w, BOX3, g, facets = True, events = True )
     m = m.cpu().numpy()
     mr, ( ri, rj, rc ) = oracle.measures_3d( pos, w, density = None if g is None else dict( gaussians = g ), facets = True )
-    scale = mr.mean()
-    assert np.abs( m - mr ).max() <= 1e-11 * scale, np.abs( m - mr ).max() / scale
+    bar = 1e-12 * np.maximum( np.abs( mr ), mr.mean() )
+    assert np.all( np.abs( m - mr ) <= bar ), ( np.abs( m - mr ) / bar ).max()
     got, ref = rows_of( fi.cpu().numpy(), fj.cpu().numpy(), fc.cpu().numpy() ), rows_of( ri, rj, rc )
     assert set( got ) == set( ref )
-    cmax = max( abs( v ) for v in ref.values() )
-    assert max( abs( got[ k ] - ref[ k ] ) for k in ref ) <= 1e-11 * cmax
+    assert max( abs( got[ k ] - ref[ k ] ) / abs( ref[ k ] ) for k in ref if ref[ k ] != 0 ) <= 1e-12
     if kind == "lebesgue":
         assert abs( m.sum() - 1 ) < 1e-12
     assert ev[ "cuts" ] > 0 and ev[ "side_tests" ] > ev[ "cuts" ]
@@ -116,3 +119,37 @@ def test_3d_through_the_reference_api( native ):
     box = sdot.UniformBox( [ 0, 0, 0 ], [ 2, 1, 1 ] )
     sol = sdot.OtProblem( sdot.SumOfDiracs( pos * [ 2, 1, 1 ] ), box ).solve( sdot.Iterative( tol = 1e-9 * 1e6 / n ) )
     assert sol.converged
+
+
+def test_3d_four_gaussians_of_sigma_0_1_at_20k( native, oracle ):
+    """the 3D solve's own recipe ( four Gaussians of sigma 0.1, means in [0.2, 0.8]^3, Dirichlet masses, the width continuation )
+    at 20k Diracs: the plan converges, and the oracle's FP64 cells of its weights pass the tolerance"""
+    rng = np.random.default_rng( 31 )
+    n = 20_000
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    g = gaussians( rng, sigma = 0.1 )
+    tol = 1e-9 * 1e6 / n / 1e2
+    print( f"3d solve: four Gaussians sigma 0.1, n = { n }", flush = True )
+    r = native.solve_3d( pos, np.full( n, 1.0 / n ), None, _opts( tol, "mg", continuation = "auto" ), BOX3, g )
+    st = r[ "stats" ]
+    assert st[ 0 ] == 1, st[ :6 ]
+    w = r[ "weights" ].cpu().numpy()
+    target = np.full( n, 1.0 / n ) * st[ 12 ]
+    assert np.abs( r[ "cell_masses" ].cpu().numpy() - target ).max() <= tol
+    mo = oracle.measures_3d( pos, w, density = dict( gaussians = g ) )
+    assert np.abs( mo - target ).max() <= 1.01 * tol
+
+
+def test_3d_four_gaussians_of_sigma_0_1_match_the_oracles_solve( native, oracle ):
+    """the same recipe at 5k Diracs, solved by the oracle too ( its multigrid ): the same weights"""
+    rng = np.random.default_rng( 32 )
+    n = 5_000
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    g = gaussians( rng, sigma = 0.1 )
+    tol = 1e-9 * 1e6 / n / 1e2
+    print( f"3d solve against the oracle's: four Gaussians sigma 0.1, n = { n }", flush = True )
+    r = native.solve_3d( pos, np.full( n, 1.0 / n ), None, _opts( tol, "mg", continuation = "auto" ), BOX3, g )
+    ref = oracle.solve_3d( pos, density = dict( gaussians = g ), tol = tol, lin_tol = 1e-12, continuation = "auto", linear = "mg" )
+    assert r[ "stats" ][ 0 ] == 1 and ref[ "stats" ][ "status" ] == 1
+    w, wr = r[ "weights" ].cpu().numpy(), ref[ "weights" ]
+    assert np.abs( w - wr ).max() <= 1e-5 * ( wr.max() - wr.min() )
