@@ -101,6 +101,16 @@ int sdot_amd_measures_2d_precision( void *stream, long long n, const double *pos
                                     double *masses, long long *facet_i, long long *facet_j, double *facet_c, long long facet_cap,
                                     long long *nb_facets, unsigned long long *events, int precision );
 
+/* ONE DIAGRAM IN A POLYGON DOMAIN: `sdot_amd_measures_2d_precision`'s arguments, then the domain, a convex polygon of
+ * `nb_domain` ( 3 to 12 ) vertices `domain_xy` HOST [ nb_domain ][ 2 ], counterclockwise. Each cell starts from the polygon
+ * instead of the box ( `box` stays the polygon's bounding box ); edge `i` ( vertex `i` to `i + 1` ) is the cut `-5 - i`. The
+ * density is zero outside the polygon. Replaces the card's start cell ( `Cuts`, `start_vertices`: src/sdot/include/sdot/gpu/
+ * Cell2D.cuh:295-333, 465-478, laid down from src/sdot/StartCell.py and src/sdot/PowerDiagram.py:166-195 ). */
+int sdot_amd_measures_2d_domain( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                                 int density_kind, double rho, const double *img, int nx, int ny, const double *img_geo,
+                                 double *masses, long long *facet_i, long long *facet_j, double *facet_c, long long facet_cap,
+                                 long long *nb_facets, unsigned long long *events, int precision, int nb_domain, const double *domain_xy );
+
 /* THE MOMENTS OF ONE DIAGRAM: for each cell `int rho` ( `masses` ), `int x rho` ( `first`, DEVICE [ n ][ 2 ] ) and `int |x|^2 rho`
  * ( `second` ), user order: enough for a transport cost ( `second - 2 p . first + |p|^2 mass` ) and the barycentres
  * ( `first / mass` ). The arguments of `sdot_amd_measures_2d` otherwise ( any of its densities ).
@@ -203,6 +213,17 @@ int sdot_amd_solve_2d_precision( void *stream, long long n, const double *positi
                                  int density_kind, double rho, const double *img,
                                  double *weights, double *history, long long *nb_steps, double *stats,
                                  double *cell_masses, double *barycenters, double *cost, long long *clusters, int precision );
+
+/* THE SOLVE IN A POLYGON DOMAIN: `sdot_amd_solve_2d_precision`'s arguments, then the domain as in
+ * `sdot_amd_measures_2d_domain` ( a convex counterclockwise polygon of 3 to 12 vertices; `box` its bounding box ). The targets
+ * are rescaled to the polygon's mass of the density. Replaces the card's solve with `start_vertices` / `bnd_*` set
+ * ( src/sdot/SdotPlanNd.py:864-890 with a `Polytope` target, src/sdot/distributions/Polytope.py ). */
+int sdot_amd_solve_2d_domain( void *stream, long long n, const double *positions, const double *nu, const double *w0,
+                              const long long *dups, long long nb_dup_values, const double *options, const double *box,
+                              int density_kind, double rho, const double *img,
+                              double *weights, double *history, long long *nb_steps, double *stats,
+                              double *cell_masses, double *barycenters, double *cost, long long *clusters, int precision,
+                              int nb_domain, const double *domain_xy );
 
 /* ONE 3D DIAGRAM: the masses of the power cells of `( positions, weights )` in the box
  * `box = { lo_x, lo_y, lo_z, hi_x, hi_y, hi_z }` ( HOST ) against a constant density or a sum of Gaussians, and optionally

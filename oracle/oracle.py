@@ -6,6 +6,7 @@ reference's user order ( `positions[ n, 2 ]`, `weights[ n ]` ), and image values
 `img[ ny, nx ]` ( row `j` = pixel row along y ).
 """
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -110,8 +111,29 @@ def _density_args( density ):
     return 1, 0.0, img.shape[ 1 ], img.shape[ 0 ], geo, img, ( img, geo )
 
 
-def measures( pos, w = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = None, facets = False, brute_force = False, events = False ):
-    """masses `[ n ]` ( and if asked the facets `( fi, fj, fc )` in row order, and the event counts )"""
+@contextlib.contextmanager
+def polygon_domain( xy ):
+    """the calls inside take the convex polygon `xy` ( `[ nb, 2 ]`, counterclockwise ) as their domain: each cell starts from
+    it, edge `i` the cut `-5 - i` ( `None`: the box )"""
+    if xy is None:
+        yield
+        return
+    v = np.ascontiguousarray( _f64( xy ).reshape( -1, 2 ) )
+    lib().oracle_set_domain( ctypes.c_int( len( v ) ), _d( v ) )
+    try:
+        yield
+    finally:
+        lib().oracle_set_domain( ctypes.c_int( 0 ), None )
+
+
+def measures( pos, w = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = None, facets = False, brute_force = False, events = False, domain = None ):
+    """masses `[ n ]` ( and if asked the facets `( fi, fj, fc )` in row order, and the event counts ); `domain`: a convex polygon
+    ( counterclockwise vertices ) in place of the box"""
+    with polygon_domain( domain ):
+        return _measures( pos, w, lo, hi, density, facets, brute_force, events )
+
+
+def _measures( pos, w, lo, hi, density, facets, brute_force, events ):
     pos = _f64( pos )
     n = pos.shape[ 0 ]
     ww = None if w is None else _f64( w )
@@ -137,8 +159,14 @@ def measures( pos, w = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = None, facet
     return res[ 0 ] if len( res ) == 1 else tuple( res )
 
 
-def cell( pos, i, w = None, lo = ( 0, 0 ), hi = ( 1, 1 ), brute_force = False, cap = 4096 ):
-    """the vertices `[ nv, 2 ]` of the cell of seed `i` and the ids of its edges ( -1: the domain )"""
+def cell( pos, i, w = None, lo = ( 0, 0 ), hi = ( 1, 1 ), brute_force = False, cap = 4096, domain = None ):
+    """the vertices `[ nv, 2 ]` of the cell of seed `i` and the ids of its edges ( -1 - f: side f of the box, -5 - e: edge e
+    of the polygon `domain` )"""
+    with polygon_domain( domain ):
+        return _cell( pos, i, w, lo, hi, brute_force, cap )
+
+
+def _cell( pos, i, w, lo, hi, brute_force, cap ):
     pos = _f64( pos )
     n = pos.shape[ 0 ]
     ww = None if w is None else _f64( w )
@@ -213,10 +241,11 @@ HISTORY = [ "step", "t", "residual_l2", "min_measure", "max_abs_residual", "nb_d
 def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = None, tol = 1e-9, tol_rel = 0.0, maxit = 100, max_backtracks = 60,
            t_min = 1e-10, mult_ok = 4.0, residual = 1, power = 0.5, switch = 2.0, lin_tol = 1e-12, lin_maxit = 20000, keep_start = False,
            start_box = None, step = "trials", factor = 0.9, continuation = "never", conv_threshold = 1e-2, conv_s0 = 0.0,
-           conv_ratio = 2 ** 0.5, conv_min = 0.0, dups = None, gap = 0.0, margin = 0.0, linear = "cg" ):
+           conv_ratio = 2 ** 0.5, conv_min = 0.0, dups = None, gap = 0.0, margin = 0.0, linear = "cg", domain = None ):
     """the damped-Newton solve ( step "trials" or "limits"; continuation "never", "auto" or "always" ); `dups`: the exact
     duplicates, `( dup, rep )` rows of user ids, whose weights are tied at `w_rep - k gap`; `margin`: the aggregation of
-    near-coincident pairs ( 0: off ); `linear`: the Newton systems by Jacobi-PCG ( "cg" ) or the multigrid ( "mg" ); returns a dict ( weights, masses, barycenters, cost, stats, history, clusters )"""
+    near-coincident pairs ( 0: off ); `linear`: the Newton systems by Jacobi-PCG ( "cg" ) or the multigrid ( "mg" ); `domain`: a convex
+    polygon ( counterclockwise vertices; `lo`, `hi` its bounding box ) in place of the box; returns a dict ( weights, masses, barycenters, cost, stats, history, clusters )"""
     pos = _f64( pos )
     n = pos.shape[ 0 ]
     nu = np.full( n, 1.0 / n ) if nu is None else _f64( nu )
@@ -234,10 +263,11 @@ def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = No
     bary = np.zeros( ( n, 2 ) )
     cost = np.zeros( 1 )
     clusters = np.zeros( n, dtype = np.int64 )
-    nb = lib().oracle_solve( ctypes.c_longlong( n ), _d( pos ), _d( nu ), _d( w0 ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
-                             ctypes.c_double( value ), ctypes.c_int( nx ), ctypes.c_int( ny ), _d( geo ), _d( img ), _d( opts ), _d( sb ),
-                             None if dp is None else dp.ctypes.data_as( ctypes.POINTER( ctypes.c_longlong ) ), _d( weights ), _d( stats ), _d( hist ), _d( masses ), _d( bary ), _d( cost ),
-                             clusters.ctypes.data_as( ctypes.POINTER( ctypes.c_longlong ) ) )
+    with polygon_domain( domain ):
+        nb = lib().oracle_solve( ctypes.c_longlong( n ), _d( pos ), _d( nu ), _d( w0 ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
+                                 ctypes.c_double( value ), ctypes.c_int( nx ), ctypes.c_int( ny ), _d( geo ), _d( img ), _d( opts ), _d( sb ),
+                                 None if dp is None else dp.ctypes.data_as( ctypes.POINTER( ctypes.c_longlong ) ), _d( weights ), _d( stats ), _d( hist ), _d( masses ), _d( bary ), _d( cost ),
+                                 clusters.ctypes.data_as( ctypes.POINTER( ctypes.c_longlong ) ) )
     return dict( weights = weights, masses = masses, barycenters = bary, cost = float( cost[ 0 ] ),
                  stats = dict( zip( STATS, stats.tolist() ) ), history = hist[ :nb ].copy(), clusters = clusters )
 

@@ -76,7 +76,27 @@ static double now() {
 /// `( i, j )`, pixel `( i, j )` covering `[ x0 + i hx, x0 + ( i + 1 ) hx ] x [ y0 + j hy, ... ]`;
 /// kind 2: a sum of `ng` isotropic Gaussians, `gauss[ 4 q .. ]` = centre x, y, sigma, mass, each widened to
 /// `sqrt( sigma^2 + conv_s^2 )` ( the width continuation, `sdotplan/convolved/SumOfGaussians.h:34` )
+/// a convex polygon domain, the cell the cuts start from ( `StartCell.py`, `gpu/Cell2D.cuh:295-333, 465-478` ): vertices
+/// counterclockwise, edge `i` ( vertex `i` to `i + 1` ) the cut `-5 - i`, its line `nx x + ny y <= off` ( unit outward normal );
+/// empty: the box
+struct Domain {
+    std::vector<double> x, y, nx, ny, off;
+    bool empty() const { return x.empty(); }
+    void set( int nb, const double *xy ) {
+        x.assign( size_t( nb ), 0.0 ); y = x; nx = x; ny = x; off = x;
+        for ( int i = 0; i < nb; ++i ) { x[ size_t( i ) ] = xy[ 2 * i ]; y[ size_t( i ) ] = xy[ 2 * i + 1 ]; }
+        for ( int i = 0; i < nb; ++i ) {
+            const int j = ( i + 1 ) % nb;
+            const double ex = x[ size_t( j ) ] - x[ size_t( i ) ], ey = y[ size_t( j ) ] - y[ size_t( i ) ], len = std::sqrt( ex * ex + ey * ey );
+            nx[ size_t( i ) ] = ey / len;
+            ny[ size_t( i ) ] = -ex / len;
+            off[ size_t( i ) ] = nx[ size_t( i ) ] * x[ size_t( i ) ] + ny[ size_t( i ) ] * y[ size_t( i ) ];
+        }
+    }
+};
+
 struct Density {
+    Domain        dom;             ///< the polygon domain ( empty: the box )
     int           kind  = 0;
     double        value = 1;
     int           nx = 0, ny = 0;
@@ -377,8 +397,15 @@ struct Poly {
     void reserve( int c ) { if ( int( x.size() ) < c ) { x.resize( size_t( c ) ); y.resize( size_t( c ) ); s.resize( size_t( c ) ); id.resize( size_t( c ) ); } }
 };
 
-/// the box `[ lo, hi ]` seen from `p0`, counter-clockwise
-static void init_box( Poly &c, const double *lo, const double *hi, double p0x, double p0y ) {
+/// the box `[ lo, hi ]` seen from `p0`, counter-clockwise; or the polygon domain `dom` when there is one
+static void init_box( Poly &c, const double *lo, const double *hi, double p0x, double p0y, const Domain *dom = nullptr ) {
+    if ( dom && ! dom->empty() ) {
+        const int nb = int( dom->x.size() );
+        c.reserve( std::max( 16, 2 * nb ) );
+        c.nb = nb;
+        for ( int i = 0; i < nb; ++i ) { c.x[ size_t( i ) ] = dom->x[ size_t( i ) ] - p0x; c.y[ size_t( i ) ] = dom->y[ size_t( i ) ] - p0y; c.id[ size_t( i ) ] = -5 - i; }
+        return;
+    }
     c.reserve( 16 );
     c.nb = 4;
     c.x[ 0 ] = lo[ 0 ] - p0x; c.y[ 0 ] = lo[ 1 ] - p0y;
@@ -496,9 +523,9 @@ static double proximity( const Tree &t, I64 m, double p0x, double p0y ) {
 
 /// the cell of rank `k0` in `c`, from the domain box, cut by the planes the tree proposes
 /// ( `diagram/Ops.h:82`, `cell/Providers.h:167-236` ). Returns false if the cell is empty.
-bool make_cell( const Tree &t, I64 k0, const double *lo, const double *hi, Poly &c, Events *ev ) {
+bool make_cell( const Tree &t, I64 k0, const double *lo, const double *hi, Poly &c, Events *ev, const Domain *dom = nullptr ) {
     const double p0x = t.px[ size_t( k0 ) ], p0y = t.py[ size_t( k0 ) ], w0 = t.w[ size_t( k0 ) ];
-    init_box( c, lo, hi, p0x, p0y );
+    init_box( c, lo, hi, p0x, p0y, dom );
     I64 stack[ 128 ];
     int top = 0;
     stack[ top++ ] = ( I64( 0 ) << 7 ) | t.depth;
@@ -532,9 +559,9 @@ bool make_cell( const Tree &t, I64 k0, const double *lo, const double *hi, Poly 
 }
 
 /// the same cell, cut by EVERY other seed in rank order: no tree at all ( `cell/Providers.h:36-64` )
-bool make_cell_all( const Tree &t, I64 k0, const double *lo, const double *hi, Poly &c ) {
+bool make_cell_all( const Tree &t, I64 k0, const double *lo, const double *hi, Poly &c, const Domain *dom = nullptr ) {
     const double p0x = t.px[ size_t( k0 ) ], p0y = t.py[ size_t( k0 ) ], w0 = t.w[ size_t( k0 ) ];
-    init_box( c, lo, hi, p0x, p0y );
+    init_box( c, lo, hi, p0x, p0y, dom );
     for ( I64 k = 0; k < t.n; ++k ) {
         if ( k == k0 ) continue;
         const double dx = t.px[ size_t( k ) ] - p0x, dy = t.py[ size_t( k ) ] - p0y;
@@ -915,7 +942,7 @@ struct Sweep {
             piece.reserve( 64 );
             for ( I64 k = b; k < e; ++k ) {
                 const I64 i = t.order[ size_t( k ) ];
-                if ( ! make_cell( t, k, lo, hi, c, &eth[ size_t( th ) ] ) ) { a[ size_t( i ) ] = 0; if ( edges ) edges->nb[ size_t( i ) ] = 0; continue; }
+                if ( ! make_cell( t, k, lo, hi, c, &eth[ size_t( th ) ], &dn.dom ) ) { a[ size_t( i ) ] = 0; if ( edges ) edges->nb[ size_t( i ) ] = 0; continue; }
                 if ( edges ) {
                     int ne = c.nb < 3 ? 0 : c.nb;
                     if ( ne > EDGE_CAP ) ne = -1;
@@ -956,7 +983,7 @@ struct Sweep {
                 const I64 i = t.order[ size_t( k ) ];
                 const double p0x = t.px[ size_t( k ) ], p0y = t.py[ size_t( k ) ];
                 double m = 0, mx[ 2 ] = { 0, 0 }, m2 = 0;
-                if ( make_cell( t, k, lo, hi, c, nullptr ) && c.nb >= 3 ) {
+                if ( make_cell( t, k, lo, hi, c, nullptr, &dn.dom ) && c.nb >= 3 ) {
                     if ( dn.kind == 2 ) gauss_polygon_moments( dn, c, p0x, p0y, m, mx, m2 );
                     else for_each_piece( dn, c, p0x, p0y, piece, [&]( const Poly &pc, double value ) { piece_moments( pc, value, m, mx, m2 ); } );
                 }
@@ -1033,6 +1060,15 @@ struct Along {
             ny = pos[ 2 * c + 1 ] - py;
             off = 0.5 * ( nx * nx + ny * ny ) + 0.5 * ( ( w[ k ] + t0 * d[ k ] ) - ( w[ c ] + t0 * d[ c ] ) );
             delta = 0.5 * ( d[ k ] - d[ c ] );
+            return;
+        }
+        if ( c <= -5 ) {                                 // an edge of the polygon domain
+            const Domain &D = dn->dom;
+            const size_t j = size_t( -5 - c );
+            nx = D.nx[ j ];
+            ny = D.ny[ j ];
+            off = D.off[ j ] - nx * px - ny * py;
+            delta = 0;
             return;
         }
         const int f = -1 - c;
@@ -1760,8 +1796,17 @@ static void similarity( const Tree &t, const double *pos, const double *lo, cons
 using orc::I64;
 using orc::sz;
 
+/// the polygon domain of the next calls ( `oracle_set_domain`; empty: the box ), set by the caller around them
+static orc::Domain g_domain;
+
+extern "C" void oracle_set_domain( int nb, const double *xy ) {
+    if ( nb >= 3 && xy ) g_domain.set( nb, xy );
+    else g_domain = orc::Domain();
+}
+
 static orc::Density make_density( int kind, double value, int nx, int ny, const double *geo, const double *img ) {
     orc::Density d;
+    d.dom = g_domain;
     d.kind = kind;
     d.value = value;
     if ( kind == 1 ) {
@@ -1828,7 +1873,7 @@ I64 oracle_measures( I64 n, const double *pos, const double *w, const double *lo
         orc::Poly c, piece;
         for ( I64 k = 0; k < n; ++k ) {
             const I64 i = t.order[ size_t( k ) ];
-            if ( ! orc::make_cell_all( t, k, lo, hi, c ) ) continue;
+            if ( ! orc::make_cell_all( t, k, lo, hi, c, &g_domain ) ) continue;
             a[ size_t( i ) ] = orc::mass_and_facets( t, sw.dn, k, c, piece, fi != nullptr, [&]( I64 j, double cij ) {
                 fa.push_back( orc::Facet{ i, t.order[ size_t( j ) ], cij } );
             } );
@@ -1860,7 +1905,7 @@ void oracle_cell_events( I64 n, const double *pos, const double *w, const double
         c.reserve( 64 );
         for ( I64 k = b; k < e; ++k ) {
             orc::Events E;
-            orc::make_cell( t, k, lo, hi, c, &E );
+            orc::make_cell( t, k, lo, hi, c, &E, &g_domain );
             I64 *o = ev + 5 * k;
             o[ 0 ] = E.side_tests; o[ 1 ] = E.prune_vertex_tests; o[ 2 ] = E.new_vertices; o[ 3 ] = E.node_visits; o[ 4 ] = E.cuts;
         }
@@ -1879,7 +1924,7 @@ int oracle_cell( I64 n, const double *pos, const double *w, const double *lo, co
     orc::set_weights( t, W.data() );
     orc::Poly c;
     const I64 k = t.rank_of[ size_t( i ) ];
-    const bool ok = brute_force ? orc::make_cell_all( t, k, lo, hi, c ) : orc::make_cell( t, k, lo, hi, c, nullptr );
+    const bool ok = brute_force ? orc::make_cell_all( t, k, lo, hi, c, &g_domain ) : orc::make_cell( t, k, lo, hi, c, nullptr, &g_domain );
     if ( ! ok ) return 0;
     if ( c.nb > cap ) return -c.nb;
     for ( int q = 0; q < c.nb; ++q ) {

@@ -227,19 +227,37 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
     } );
 }
 
-int sdot_amd_measures_2d_precision( void *stream, long long n, const double *positions, const double *weights, const double *box,
-                                    int density_kind, double rho, const double *img, int nx, int ny, const double *img_geo, double *masses,
-                                    long long *facet_i, long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets,
-                                    unsigned long long *events, int precision ) {
+static int measures_entry( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
+                           double rho, const double *img, int nx, int ny, const double *img_geo, double *masses, long long *facet_i,
+                           long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets, unsigned long long *events,
+                           int precision, int nb_domain, const double *domain_xy ) {
     return guarded( [&] {
         SDOT_CHECK( n >= 1 && positions && masses, "measures: positions and masses are required" );
         SDOT_CHECK( precision == SDOT_AMD_FP64 || precision == SDOT_AMD_FP32, "measures: the precision is SDOT_AMD_FP64 or SDOT_AMD_FP32" );
         check_box( box );
         SDOT_CHECK( ! facet_i || ( facet_j && facet_c ), "measures: facet_i, facet_j and facet_c go together" );
-        const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
+        Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
+        if ( nb_domain ) set_domain( dn, nb_domain, domain_xy );
         measures_2d( hipStream_t( stream ), n, positions, weights, box, dn, masses, facet_i, facet_j, facet_c, facet_cap, nb_facets, events,
                      precision == SDOT_AMD_FP32 );
     } );
+}
+
+int sdot_amd_measures_2d_precision( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                                    int density_kind, double rho, const double *img, int nx, int ny, const double *img_geo, double *masses,
+                                    long long *facet_i, long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets,
+                                    unsigned long long *events, int precision ) {
+    return measures_entry( stream, n, positions, weights, box, density_kind, rho, img, nx, ny, img_geo, masses, facet_i, facet_j, facet_c,
+                           facet_cap, nb_facets, events, precision, 0, nullptr );
+}
+
+int sdot_amd_measures_2d_domain( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                                 int density_kind, double rho, const double *img, int nx, int ny, const double *img_geo, double *masses,
+                                 long long *facet_i, long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets,
+                                 unsigned long long *events, int precision, int nb_domain, const double *domain_xy ) {
+    if ( nb_domain < 3 || ! domain_xy ) return guarded( [&] { SDOT_CHECK( false, "measures: a polygon domain of at least 3 vertices" ); } );
+    return measures_entry( stream, n, positions, weights, box, density_kind, rho, img, nx, ny, img_geo, masses, facet_i, facet_j, facet_c,
+                           facet_cap, nb_facets, events, precision, nb_domain, domain_xy );
 }
 
 int sdot_amd_moments_2d( void *stream, long long n, const double *positions, const double *weights, const double *box, int density_kind,
@@ -302,10 +320,10 @@ int sdot_amd_measures_2d_noalloc( void *stream, long long n, const double *posit
 
 long long sdot_amd_measures_2d_noalloc_max_seeds( void ) { return (long long) NOALLOC_MAX_SEEDS; }
 
-int sdot_amd_solve_2d_precision( void *stream, long long n, const double *positions, const double *nu, const double *w0, const long long *dups,
-                                 long long nb_dup_values, const double *options, const double *box, int density_kind, double rho,
-                                 const double *img, double *weights, double *history, long long *nb_steps, double *stats, double *cell_masses,
-                                 double *barycenters, double *cost, long long *clusters, int precision ) {
+static int solve_entry( void *stream, long long n, const double *positions, const double *nu, const double *w0, const long long *dups,
+                        long long nb_dup_values, const double *options, const double *box, int density_kind, double rho, const double *img,
+                        double *weights, double *history, long long *nb_steps, double *stats, double *cell_masses, double *barycenters,
+                        double *cost, long long *clusters, int precision, int nb_domain, const double *domain_xy ) {
     return guarded( [&] {
         SDOT_CHECK( n >= 2 && positions && nu && options && weights && cell_masses && barycenters, "solve: a required array is null" );
         SDOT_CHECK( precision == SDOT_AMD_FP64 || precision == SDOT_AMD_FP32 || precision == SDOT_AMD_MIXED,
@@ -327,6 +345,7 @@ int sdot_amd_solve_2d_precision( void *stream, long long n, const double *positi
         for ( int c = 0; c < 4; ++c ) in.box[ c ] = box[ c ];
         const double geo[ 4 ] = { options[ O_IMG_X0 ], options[ O_IMG_Y0 ], options[ O_IMG_HX ], options[ O_IMG_HY ] };
         in.dens = make_density( density_kind, rho, img, int( options[ O_IMG_NX ] ), int( options[ O_IMG_NY ] ), geo );
+        if ( nb_domain ) set_domain( in.dens, nb_domain, domain_xy );
         SolveOutputs out;
         out.weights = weights;
         out.cell_masses = cell_masses;
@@ -338,6 +357,23 @@ int sdot_amd_solve_2d_precision( void *stream, long long n, const double *positi
         out.cost = cost;
         solve_2d( in, out, hipStream_t( stream ) );
     } );
+}
+
+int sdot_amd_solve_2d_precision( void *stream, long long n, const double *positions, const double *nu, const double *w0, const long long *dups,
+                                 long long nb_dup_values, const double *options, const double *box, int density_kind, double rho,
+                                 const double *img, double *weights, double *history, long long *nb_steps, double *stats, double *cell_masses,
+                                 double *barycenters, double *cost, long long *clusters, int precision ) {
+    return solve_entry( stream, n, positions, nu, w0, dups, nb_dup_values, options, box, density_kind, rho, img, weights, history, nb_steps,
+                        stats, cell_masses, barycenters, cost, clusters, precision, 0, nullptr );
+}
+
+int sdot_amd_solve_2d_domain( void *stream, long long n, const double *positions, const double *nu, const double *w0, const long long *dups,
+                              long long nb_dup_values, const double *options, const double *box, int density_kind, double rho,
+                              const double *img, double *weights, double *history, long long *nb_steps, double *stats, double *cell_masses,
+                              double *barycenters, double *cost, long long *clusters, int precision, int nb_domain, const double *domain_xy ) {
+    if ( nb_domain < 3 || ! domain_xy ) return guarded( [&] { SDOT_CHECK( false, "solve: a polygon domain of at least 3 vertices" ); } );
+    return solve_entry( stream, n, positions, nu, w0, dups, nb_dup_values, options, box, density_kind, rho, img, weights, history, nb_steps,
+                        stats, cell_masses, barycenters, cost, clusters, precision, nb_domain, domain_xy );
 }
 
 int sdot_amd_solve_2d( void *stream, long long n, const double *positions, const double *nu, const double *w0, const long long *dups,

@@ -7,6 +7,16 @@ namespace sdot_amd {
 
 /// the target density: a constant on the domain box, a piecewise-constant image `img[ j * nx + i ]` on pixels
 /// `[ x0 + i hx, x0 + ( i + 1 ) hx ] x [ y0 + j hy, ... ]`, or a sum of isotropic Gaussians truncated to the box
+/// THE CELL THE CUTS START FROM when the domain is a convex polygon and not the box ( the reference's start cell,
+/// `StartCell.py`, `gpu/Cell2D.cuh:295-333, 465-478` ): its vertices counterclockwise, and the line of each edge ( edge `i` from
+/// vertex `i` to `i + 1`, the cut `-5 - i`: `nx x + ny y <= off`, `( nx, ny )` the unit outward normal ). `nb = 0`: the box
+constexpr int DOMAIN_MAX = 12;             ///< at most this many vertices ( the first passes' 16 slots keep room for the cuts )
+struct Polygon2 {
+    int    nb = 0;
+    double x[ DOMAIN_MAX ] = {}, y[ DOMAIN_MAX ] = {};
+    double nx[ DOMAIN_MAX ] = {}, ny[ DOMAIN_MAX ] = {}, off[ DOMAIN_MAX ] = {};
+};
+
 struct Density {
     int           kind = 0;        ///< 0 constant, 1 image, 2 a sum of isotropic Gaussians
     double        rho = 1;
@@ -15,7 +25,12 @@ struct Density {
     double        x0 = 0, y0 = 0, hx = 1, hy = 1;
     const double *gauss = nullptr; ///< device, `[ ng ][ 4 ]`: centre x, y, width, mass
     int           ng = 0;
+    Polygon2      dom;             ///< the domain when it is a polygon ( the density is zero outside it )
 };
+
+/// the polygon domain of `nb` counterclockwise vertices `xy[ nb ][ 2 ]` ( HOST ): its edges' lines; throws when it is not a
+/// convex counterclockwise polygon of 3 to `DOMAIN_MAX` vertices
+void set_domain( Density &dn, int nb, const double *xy );
 
 /// what one diagram writes, in RANK order: the masses, the rows of the Laplacian ( each cell's
 /// own view of its facets: `col[ row_off[ k ] + q * ROW_STEP ]`, `val[ ... ]`, `q < row_cnt[ k ]`, and

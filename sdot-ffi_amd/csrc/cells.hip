@@ -255,18 +255,25 @@ __device__ __forceinline__ int node_stop( const Walk &wk, int m, const NodeF & )
 /// ahead of their use: a descent loads both children's records ( the nearer is the next node tested,
 /// from registers ), and a leaf's next seed is loaded while the current one cuts.
 template<class S, class ND = NodeRec>
-__device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *lo, const double *hi, const S &c, int cap, Ev &ev ) {
+__device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *lo, const double *hi, const Polygon2 &dom, const S &c, int cap,
+                                          Ev &ev ) {
     using T = typename S::T;
     const ND *kind = nullptr;                            // ( which records the walk reads: `NodeRec` or the float `NodeF` )
     const double4 p = wk.seeds[ k ];
     const double p0x = p.x, p0y = p.y, w0 = p.z;
     const Frame32 f32 = frame32( p0x, p0y, w0 );
-    c.X( 0 ) = T( lo[ 0 ] - p0x ); c.Y( 0 ) = T( lo[ 1 ] - p0y );
-    c.X( 1 ) = T( hi[ 0 ] - p0x ); c.Y( 1 ) = T( lo[ 1 ] - p0y );
-    c.X( 2 ) = T( hi[ 0 ] - p0x ); c.Y( 2 ) = T( hi[ 1 ] - p0y );
-    c.X( 3 ) = T( lo[ 0 ] - p0x ); c.Y( 3 ) = T( hi[ 1 ] - p0y );
-    for ( int i = 0; i < 4; ++i ) c.I( i ) = -1 - i;     // the box's sides: bottom, right, top, left
     int nb = 4;
+    if ( dom.nb > 0 ) {                                  // the polygon domain, laid down as it is ( its edges: `-5 - i` )
+        if ( dom.nb > cap ) return -1;
+        for ( int i = 0; i < dom.nb; ++i ) { c.X( i ) = T( dom.x[ i ] - p0x ); c.Y( i ) = T( dom.y[ i ] - p0y ); c.I( i ) = -5 - i; }
+        nb = dom.nb;
+    } else {
+        c.X( 0 ) = T( lo[ 0 ] - p0x ); c.Y( 0 ) = T( lo[ 1 ] - p0y );
+        c.X( 1 ) = T( hi[ 0 ] - p0x ); c.Y( 1 ) = T( lo[ 1 ] - p0y );
+        c.X( 2 ) = T( hi[ 0 ] - p0x ); c.Y( 2 ) = T( hi[ 1 ] - p0y );
+        c.X( 3 ) = T( lo[ 0 ] - p0x ); c.Y( 3 ) = T( hi[ 1 ] - p0y );
+        for ( int i = 0; i < 4; ++i ) c.I( i ) = -1 - i; // the box's sides: bottom, right, top, left
+    }
     int m = 0, h = wk.depth;
     unsigned long long pend = 0, left = 0;
     int q = 0, qe = 0;                                   // the open leaf's remaining seeds
@@ -656,8 +663,8 @@ __device__ __forceinline__ int pass1_rank() {
 }
 
 /// the plane of a cut in double, from its identifier ( `gpu/Cell2D.cuh:946-971` ): the bisector with seed `id`,
-/// `n . v = ( ( w0 - w ) + |n|^2 ) / 2` in the seed's frame ( `e = |n|^2` ), or side `-1 - id` of the box ( 0 bottom,
-/// 1 right, 2 top, 3 left; `e = 1` )
+/// `n . v = ( ( w0 - w ) + |n|^2 ) / 2` in the seed's frame ( `e = |n|^2` ), side `-1 - id` of the box ( 0 bottom,
+/// 1 right, 2 top, 3 left; `e = 1` ), or edge `-5 - id` of the polygon domain ( `e = 1` )
 __device__ __forceinline__ void plane_of( const Params &P, int id, double p0x, double p0y, double w0, double &nx, double &ny, double &off,
                                           double &e ) {
     if ( id >= 0 ) {
@@ -666,6 +673,15 @@ __device__ __forceinline__ void plane_of( const Params &P, int id, double p0x, d
         ny = q.y - p0y;
         e = nx * nx + ny * ny;
         off = ( ( w0 - q.z ) + e ) / 2;
+        return;
+    }
+    if ( id <= -5 ) {                                    // an edge of the polygon domain
+        const Polygon2 &D = P.dn.dom;
+        const int j = -5 - id;
+        nx = D.nx[ j ];
+        ny = D.ny[ j ];
+        e = 1.0;
+        off = D.off[ j ] - nx * p0x - ny * p0y;
         return;
     }
     const int f = -1 - id;
@@ -688,7 +704,7 @@ __global__ __launch_bounds__( CELL_BLOCK ) void k_cells_pass1( Params P ) {
     int nb = 0;
     bool active = k < P.n;
     if ( active ) {
-        nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP1, ev );
+        nb = make_cell( P.wk, k, P.lo, P.hi, P.dn.dom, c, CAP1, ev );
         if ( nb < 0 ) {
             const unsigned at = atomicAdd( &P.out.counters[ 1 ], 1u );
             P.out.ovf_list[ at ] = k;
@@ -709,7 +725,7 @@ __global__ __launch_bounds__( CELL_BLOCK ) void k_cells_walk32( Params P, Poly32
     const Store<CELL_BLOCK, float> c{ sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x };
     Ev ev;
     if ( k < P.n ) {
-        const int nb = make_cell<Store<CELL_BLOCK, float>, NodeF>( P.wk, k, P.lo, P.hi, c, CAPW, ev );
+        const int nb = make_cell<Store<CELL_BLOCK, float>, NodeF>( P.wk, k, P.lo, P.hi, P.dn.dom, c, CAPW, ev );
         if ( nb < 0 ) {
             const unsigned at = atomicAdd( &P.out.counters[ 1 ], 1u );
             P.out.ovf_list[ at ] = k;
@@ -811,7 +827,7 @@ __global__ __launch_bounds__( 64 ) void k_cells_redo( Params P, const int *list,
         int nb = 0;
         bool active = live;
         if ( live ) {
-            nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP2, ev );
+            nb = make_cell( P.wk, k, P.lo, P.hi, P.dn.dom, c, CAP2, ev );
             active = nb >= 0;
         }
         finish_cell( P, k, c, nb, active, false );
@@ -837,7 +853,7 @@ __global__ __launch_bounds__( 64 ) void k_cells_pass2( Params P ) {
         int nb = 0;
         bool active = live;
         if ( live ) {
-            nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP2, ev );
+            nb = make_cell( P.wk, k, P.lo, P.hi, P.dn.dom, c, CAP2, ev );
             if ( nb < 0 ) {
                 const unsigned at = atomicAdd( &P.out.counters[ 4 ], 1u );
                 P.out.ovf_list[ P.n + int( at ) ] = k;              // ( the second half: the first holds this pass's input )
@@ -863,7 +879,7 @@ __global__ __launch_bounds__( 64 ) void k_cells_pass3( Params P, double *gx, dou
         int nb = 0;
         bool active = live;
         if ( live ) {
-            nb = make_cell( P.wk, k, P.lo, P.hi, c, CAP3, ev );
+            nb = make_cell( P.wk, k, P.lo, P.hi, P.dn.dom, c, CAP3, ev );
             if ( nb < 0 ) {
                 if ( atomicAdd( &P.out.counters[ 2 ], 1u ) == 0 ) P.out.counters[ 3 ] = unsigned( k );
                 active = false;
@@ -926,6 +942,31 @@ void cells_scratch( DiagramOut &out, I64 n, const std::string &tag, hipStream_t 
         out.events = A.get<unsigned long long>( "events", 8, &fresh );
         if ( fresh ) SDOT_HIP( hipMemsetAsync( out.events, 0, 8 * sizeof( unsigned long long ), st ) );
     }
+}
+
+void set_domain( Density &dn, int nb, const double *xy ) {
+    SDOT_CHECK( nb >= 3 && nb <= DOMAIN_MAX && xy, "domain: a convex polygon of 3 to 12 vertices" );
+    Polygon2 &D = dn.dom;
+    D.nb = nb;
+    double area = 0;
+    for ( int i = 0; i < nb; ++i ) {
+        D.x[ i ] = xy[ 2 * i ];
+        D.y[ i ] = xy[ 2 * i + 1 ];
+        SDOT_CHECK( std::isfinite( D.x[ i ] ) && std::isfinite( D.y[ i ] ), "domain: a vertex is not finite" );
+    }
+    for ( int i = 0; i < nb; ++i ) {
+        const int j = ( i + 1 ) % nb, l = ( i + 2 ) % nb;
+        const double ex = D.x[ j ] - D.x[ i ], ey = D.y[ j ] - D.y[ i ];
+        const double fx = D.x[ l ] - D.x[ j ], fy = D.y[ l ] - D.y[ j ];
+        const double len = std::sqrt( ex * ex + ey * ey );
+        SDOT_CHECK( len > 0, "domain: two consecutive vertices coincide" );
+        SDOT_CHECK( ex * fy - ey * fx > 0, "domain: the vertices are not those of a convex polygon, counterclockwise" );
+        D.nx[ i ] = ey / len;                            // ( the outward normal of a counterclockwise edge )
+        D.ny[ i ] = -ex / len;
+        D.off[ i ] = D.nx[ i ] * D.x[ i ] + D.ny[ i ] * D.y[ i ];
+        area += D.x[ i ] * D.y[ j ] - D.x[ j ] * D.y[ i ];
+    }
+    SDOT_CHECK( area > 0, "domain: the polygon is not counterclockwise" );
 }
 
 void run_cells( const DevTree &t, const double *lo, const double *hi, const Density &dn, int mode, DiagramOut &out, hipStream_t st, bool fp32,

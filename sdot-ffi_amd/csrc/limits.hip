@@ -29,6 +29,15 @@ __device__ __forceinline__ void line_along( const Along &al, I64 k, int c, doubl
         delta = 0.5 * ( al.d[ k ] - al.d[ c ] );
         return;
     }
+    if ( c <= -5 ) {                                     // an edge of the polygon domain: it does not move either
+        const Polygon2 &D = al.dn.dom;
+        const int j = -5 - c;
+        nx = D.nx[ j ];
+        ny = D.ny[ j ];
+        off = D.off[ j ] - nx * px - ny * py;
+        delta = 0;
+        return;
+    }
     const int f = -1 - c;                               // 0 bottom, 1 right, 2 top, 3 left: they do not move
     nx = ( f & 1 ) ? 1.0 : 0.0;
     ny = ( f & 1 ) ? 0.0 : 1.0;

@@ -48,17 +48,30 @@ class PowerDiagram:
     def __init__( self, positions, weights = None, boundaries = None, accelerator = None, distribution = None, kernel_dtype = None, **kwargs ):
         self.positions = positions
         self.weights = weights
-        if distribution is not None:
-            dist = distribution.normalized_version()
-            lo, hi = dist.box()
-        elif boundaries is not None:
-            box = axis_aligned_box( *boundaries )
-            if box is None:
-                raise NotImplementedError( "PowerDiagram on the MI355X path: a box domain" )
-            lo, hi = box
-            dist = UniformBox( lo, hi )
-        else:
+        # THE DOMAIN ( `PowerDiagram.py:166-195` ): the half-planes given, intersected with the support the distribution declares;
+        # a box, or a convex polygon the cells start from ( `StartCell.start_cell` )
+        self.domain = None
+        dist = distribution.normalized_version() if distribution is not None else None
+        if dist is None and boundaries is None:
             raise ValueError( "PowerDiagram: a domain is needed ( `boundaries` or `distribution` )" )
+        halves = [ b for b in ( boundaries, dist.bounding_half_spaces() if dist is not None else None ) if b is not None ]
+        dirs = np.concatenate( [ np.asarray( h[ 0 ], dtype = np.float64 ) for h in halves ] )
+        offs = np.concatenate( [ np.asarray( h[ 1 ], dtype = np.float64 ).reshape( -1 ) for h in halves ] )
+        box = axis_aligned_box( dirs, offs )
+        if box is not None:
+            lo, hi = box
+        else:
+            from .StartCell import MAX_VERTICES, start_cell
+            cell = start_cell( dirs, offs ) if dirs.shape[ 1 ] == 2 else None
+            if cell is None:
+                raise NotImplementedError( "PowerDiagram on the MI355X path: a box domain, or a bounded 2D convex polygon" )
+            if len( cell[ "vertices" ] ) > MAX_VERTICES:
+                raise NotImplementedError( f"PowerDiagram on the MI355X path: a polygon domain of at most { MAX_VERTICES } vertices" )
+            self.domain = cell[ "vertices" ]
+            lo, hi = cell[ "bounding_box" ]
+        if dist is None:                                 # ( no distribution: the volumes of the cells, `rho = 1` )
+            vol = float( np.prod( np.asarray( hi ) - np.asarray( lo ) ) )
+            dist = UniformBox( lo, hi, target_mass = vol )
         if len( lo ) not in ( 2, 3 ):
             raise NotImplementedError( "PowerDiagram on the MI355X path: 2D or 3D" )
         self.distribution = dist
@@ -84,7 +97,12 @@ class PowerDiagram:
     def _measures( self, facets ):
         if self.nb_dims == 3:
             return _native.measures_3d( self.positions, self.weights, self._box(), self.distribution, facets = facets )
-        return _native.measures_2d( self.positions, self.weights, self._box(), self.distribution, facets = facets, precision = self._precision )
+        return _native.measures_2d( self.positions, self.weights, self._box(), self.distribution, facets = facets, precision = self._precision,
+                                    domain = self.domain )
+
+    def _box_only( self, what ):
+        if self.domain is not None:
+            raise NotImplementedError( f"PowerDiagram on the MI355X path: { what } in a box domain ( the masses and facets take a polygon )" )
 
     @property
     def measures( self ):
@@ -107,6 +125,7 @@ class PowerDiagram:
     def moments( self ):
         """`( masses, first, second )`: for each cell `int rho`, `int x rho` ( `[ n, d ]` ) and `int |x|^2 rho` ( device tensors,
         user order ): a transport cost is `second - 2 p . first + |p|^2 masses`, the barycentres `first / masses`"""
+        self._box_only( "moments" )
         return _native.moments( self.positions, self.weights, self._box(), self.distribution )
 
     def hessian_rows( self ):
@@ -132,6 +151,7 @@ class PowerDiagram:
         ( a seed's index, or `-1 - f` for side `f` of the box: 0 bottom, 1 right, 2 top, 3 left )"""
         if self.nb_dims != 2:
             raise NotImplementedError( "PowerDiagram.cells on the MI355X path: 2D" )
+        self._box_only( "cells" )
         return _native.cells_2d( self.positions, self.weights, self._box(), self.distribution )
 
     def laplacian_solve( self, rhs, linear_solver = "auto" ):
@@ -175,6 +195,7 @@ def _make_measures_fn():
         @staticmethod
         def backward( ctx, g ):
             pd = ctx.pd
+            pd._box_only( "the masses' adjoint" )
             gw, gp = _native.measures_vjp_2d( pd.positions.detach() if hasattr( pd.positions, "detach" ) else pd.positions,
                                               None if pd.weights is None else ( pd.weights.detach() if hasattr( pd.weights, "detach" ) else pd.weights ),
                                               pd._box(), pd.distribution, g.contiguous() )

@@ -9,7 +9,7 @@ reference's layouts ( `_STATS`, `_HISTORY` ) and `OtNotConverged` is raised the 
 import numpy as np
 
 from . import _native
-from .distributions import Image, SumOfGaussians, UniformBox, _host
+from .distributions import Image, Polytope, SumOfGaussians, UniformBox, _host
 
 _STATS = [ "status", "residual", "residual0", "nb_iter", "nb_diag", "nb_backtracks", "t_majorant", "t_diag", "t_asm", "t_lin", "t_lim", "eps",
            "domain_mass", "nb_overflowed", "nb_cell_lim", "nb_limit_rounds", "lin_nb_hierarchies", "lin_nb_iter", "lin_worst", "start", "t_total",
@@ -182,8 +182,11 @@ class SdotPlanNd:
             self._set_result( res, src, settings )
             return
         box = [ lo[ 0 ], lo[ 1 ], hi[ 0 ], hi[ 1 ] ]
-        if not isinstance( dst, ( Image, UniformBox, SumOfGaussians ) ):
-            raise NotImplementedError( f"SdotPlanNd on the MI355X path: an Image, a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
+        if not isinstance( dst, ( Image, UniformBox, SumOfGaussians, Polytope ) ):
+            raise NotImplementedError( f"SdotPlanNd on the MI355X path: an Image, a UniformBox, a SumOfGaussians or a Polytope target "
+                                       f"( not { type( dst ).__name__ } )" )
+        # a Polytope target IS the domain: the cells start from its polygon ( `StartCell.py` )
+        domain = dst.domain_polygon() if isinstance( dst, Polytope ) else None
         # THE EXACT DUPLICATES ( `SdotPlanNd.py:650-685` ): found here on the host, merged by the solve
         dup = rep = np.zeros( 0, dtype = np.int64 )
         gap = 0.0
@@ -195,7 +198,8 @@ class SdotPlanNd:
                 gap = duplicate_gap( src.positions )
         opts = card_options( settings, dst, verbose, len( dup ), gap )
         pairs = np.stack( [ dup, rep ], axis = 1 ) if len( dup ) else None
-        res = _native.solve_2d( src.positions, src.weights, w0, opts, box, dst, dups = pairs, precision = _KERNEL_FLOAT[ settings.precision ] )
+        res = _native.solve_2d( src.positions, src.weights, w0, opts, box, dst, dups = pairs, precision = _KERNEL_FLOAT[ settings.precision ],
+                                domain = domain )
         self._set_result( res, src, settings )
 
     def _set_result( self, res, src, settings ):

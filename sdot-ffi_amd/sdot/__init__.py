@@ -3,7 +3,7 @@
 power diagrams, on hand-written HIP kernels behind the C ABI of `include/sdot_amd.h`.
 """
 
-from .distributions import Image, SumOfDiracs, SumOfGaussians, UniformBox
+from .distributions import Image, Polytope, SumOfDiracs, SumOfGaussians, UniformBox
 from .OtProblem import Direct, Iterative, OtNotConverged, OtProblem, Tuning, ot_solve
 from .PowerDiagram import PowerDiagram, axis_aligned_box, box_half_spaces
 from .SdotPlanNd import SdotPlanNd
@@ -11,7 +11,8 @@ from ._native import KernelFailure, SdotError
 from .batch import solve_batch
 from .serve import Server
 from .quantization import quantization
+from .StartCell import start_cell
 
-__all__ = [ "Image", "SumOfDiracs", "SumOfGaussians", "UniformBox", "Direct", "Iterative", "OtNotConverged", "OtProblem", "Tuning",
+__all__ = [ "Image", "Polytope", "start_cell", "SumOfDiracs", "SumOfGaussians", "UniformBox", "Direct", "Iterative", "OtNotConverged", "OtProblem", "Tuning",
             "ot_solve", "PowerDiagram", "axis_aligned_box", "box_half_spaces", "SdotPlanNd", "KernelFailure", "SdotError",
             "solve_batch", "Server", "quantization" ]

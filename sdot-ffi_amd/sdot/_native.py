@@ -34,7 +34,7 @@ EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_byte
             "sdot_amd_solve_2d_precision", "sdot_amd_graph_counts", "sdot_amd_solve_bytes", "sdot_amd_server_submit_nd", "sdot_amd_server_info",
             "sdot_amd_measures_2d_noalloc", "sdot_amd_measures_2d_noalloc_max_seeds", "sdot_amd_fp32_counts", "sdot_amd_mg32_counts",
             "sdot_amd_measures_2d_precision", "sdot_amd_moments_2d", "sdot_amd_moments_3d", "sdot_amd_cells_2d",
-            "sdot_amd_measures_vjp_2d", "sdot_amd_solve_2d_batch_precision" ]
+            "sdot_amd_measures_vjp_2d", "sdot_amd_solve_2d_batch_precision", "sdot_amd_measures_2d_domain", "sdot_amd_solve_2d_domain" ]
 
 CONSTANT, IMAGE, GAUSSIANS = 0, 1, 2
 
@@ -71,6 +71,8 @@ def lib():
         L.sdot_amd_measures_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P, _P, _LL, _P, _P ]
         L.sdot_amd_measures_2d_precision.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P, _P, _LL, _P, _P, _I ]
         L.sdot_amd_measures_2d_precision.restype = _I
+        L.sdot_amd_measures_2d_domain.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P, _P, _LL, _P, _P, _I, _I, _P ]
+        L.sdot_amd_measures_2d_domain.restype = _I
         L.sdot_amd_moments_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P ]
         L.sdot_amd_moments_2d.restype = _I
         L.sdot_amd_moments_3d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _P, _P, _P ]
@@ -83,6 +85,8 @@ def lib():
         L.sdot_amd_measures_2d_noalloc_max_seeds.restype = _LL
         L.sdot_amd_solve_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _LL, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_solve_2d_precision.argtypes = [ _P, _LL, _P, _P, _P, _P, _LL, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I ]
+        L.sdot_amd_solve_2d_domain.argtypes = [ _P, _LL, _P, _P, _P, _P, _LL, _P, _P, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P ]
+        L.sdot_amd_solve_2d_domain.restype = _I
         L.sdot_amd_build_tree_2d.argtypes = [ _P, _LL, _P, _I, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_refresh_majorants_2d.argtypes = [ _P, _LL, _LL, _P, _P, _P, _P, _P, _P, _P ]
         L.sdot_amd_linear_solves_2d.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _LL, _P, _P, _P ]
@@ -175,9 +179,18 @@ def _host( a ):
 # ---- the calls -------------------------------------------------------------------------------------------
 
 
-def measures_2d( positions, weights, box, density, facets = False, events = False, precision = 0 ):
+def _domain_args( domain ):
+    """`( nb, pointer, keep )` of a polygon domain ( counterclockwise vertices ), or `( 0, None, None )`"""
+    if domain is None:
+        return 0, None, None
+    v = np.ascontiguousarray( np.asarray( domain, dtype = np.float64 ).reshape( -1, 2 ) )
+    return len( v ), v.ctypes.data_as( _P ), v
+
+
+def measures_2d( positions, weights, box, density, facets = False, events = False, precision = 0, domain = None ):
     """masses ( device tensor, user order ); with `facets` also `( fi, fj, fc )` ( device tensors ); with `events` the counts;
-    `precision` FP64 or FP32 ( the kernel's float )"""
+    `precision` FP64 or FP32 ( the kernel's float ); `domain`: a convex polygon ( counterclockwise vertices, `box` its bounding
+    box ) the cells start from instead of the box"""
     T = torch()
     pos = on_device( positions ).reshape( -1, 2 ).contiguous()
     n = pos.shape[ 0 ]
@@ -192,9 +205,16 @@ def measures_2d( positions, weights, box, density, facets = False, events = Fals
     fc = T.empty( cap, dtype = T.float64, device = device() ) if facets else None
     nbf = np.zeros( 1, np.int64 )
     ev = np.zeros( 5, np.uint64 )
-    _check( lib().sdot_amd_measures_2d_precision( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop,
-                                                  _ptr( masses ), _ptr( fi ), _ptr( fj ), _ptr( fc ), cap, nbf.ctypes.data_as( _P ),
-                                                  ev.ctypes.data_as( _P ) if events else None, int( precision ) ) )
+    evp = ev.ctypes.data_as( _P ) if events else None
+    if domain is None:
+        _check( lib().sdot_amd_measures_2d_precision( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop,
+                                                      _ptr( masses ), _ptr( fi ), _ptr( fj ), _ptr( fc ), cap, nbf.ctypes.data_as( _P ), evp,
+                                                      int( precision ) ) )
+    else:
+        nd, dp, keep = _domain_args( domain )
+        _check( lib().sdot_amd_measures_2d_domain( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( img ), nx, ny, geop,
+                                                   _ptr( masses ), _ptr( fi ), _ptr( fj ), _ptr( fc ), cap, nbf.ctypes.data_as( _P ), evp,
+                                                   int( precision ), nd, dp ) )
     res = [ masses ]
     if facets:
         k = int( nbf[ 0 ] )
@@ -350,9 +370,10 @@ def solve_3d( positions, nu, w0, options, box, density = None ):
 FP64, FP32, MIXED = 0, 1, 2
 
 
-def solve_2d( positions, nu, w0, options, box, density, dups = None, precision = FP64 ):
+def solve_2d( positions, nu, w0, options, box, density, dups = None, precision = FP64, domain = None ):
     """one solve; returns a dict of device tensors and host arrays ( see `sdot_amd_solve_2d` ); `dups`: the exact
-    duplicates, `[ k, 2 ]` rows `( dup, rep )` of user ids; `precision`: FP64, FP32 or MIXED cells"""
+    duplicates, `[ k, 2 ]` rows `( dup, rep )` of user ids; `precision`: FP64, FP32 or MIXED cells; `domain`: a convex polygon
+    ( counterclockwise vertices, `box` its bounding box ) the cells start from"""
     T = torch()
     pos = on_device( positions ).reshape( -1, 2 ).contiguous()
     n = pos.shape[ 0 ]
@@ -372,10 +393,14 @@ def solve_2d( positions, nu, w0, options, box, density, dups = None, precision =
     stats = np.zeros( 31 )
     cost = np.zeros( 1 )
     dp = None if dups is None or len( dups ) == 0 else on_device( np.asarray( dups, dtype = np.int64 ).reshape( -1 ), dtype = T.int64 )
-    _check( lib().sdot_amd_solve_2d_precision( _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), _ptr( dp ), 0 if dp is None else dp.numel(),
-                                               optp, boxp, kind, rho, _ptr( img ),
-                                               _ptr( weights ), hist.ctypes.data_as( _P ), nb_steps.ctypes.data_as( _P ), stats.ctypes.data_as( _P ),
-                                               _ptr( masses ), _ptr( bary ), cost.ctypes.data_as( _P ), _ptr( clusters ), int( precision ) ) )
+    args = [ _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), _ptr( dp ), 0 if dp is None else dp.numel(), optp, boxp, kind, rho, _ptr( img ),
+             _ptr( weights ), hist.ctypes.data_as( _P ), nb_steps.ctypes.data_as( _P ), stats.ctypes.data_as( _P ), _ptr( masses ), _ptr( bary ),
+             cost.ctypes.data_as( _P ), _ptr( clusters ), int( precision ) ]
+    if domain is None:
+        _check( lib().sdot_amd_solve_2d_precision( *args ) )
+    else:
+        nd, ddp, keep = _domain_args( domain )
+        _check( lib().sdot_amd_solve_2d_domain( *args, nd, ddp ) )
     return dict( weights = weights, cell_masses = masses, barycenters = bary, clusters = clusters, cost = float( cost[ 0 ] ),
                  stats = stats, history = hist[ :int( nb_steps[ 0 ] ) ] )
 

@@ -135,6 +135,47 @@ class UniformBox:
         return 0, self.rho, None, 0
 
 
+class Polytope:
+    """a CONSTANT density on a convex polygon, zero elsewhere ( reference `distributions/Polytope.py:36-80` ): the half-planes
+    `directions . x <= offsets`, normalized to the total mass `target_mass`. Its support is the polygon, which makes it a
+    domain: the cells of a power diagram start from it ( `StartCell.start_cell` ). 2D on this path, at most 12 vertices."""
+
+    def __init__( self, directions, offsets, density = 1.0, target_mass = 1.0 ):
+        from .StartCell import MAX_VERTICES, start_cell
+        cell = start_cell( directions, offsets )
+        if cell is None:
+            raise ValueError( "Polytope: the half-planes do not bound a 2D polygon ( empty, flat, unbounded, or not 2D )" )
+        if len( cell[ "vertices" ] ) > MAX_VERTICES:
+            raise NotImplementedError( f"Polytope on the MI355X path: at most { MAX_VERTICES } vertices ( got { len( cell[ 'vertices' ] ) } )" )
+        self.directions = np.asarray( directions, dtype = np.float64 )
+        self.offsets = np.asarray( offsets, dtype = np.float64 ).reshape( -1 )
+        self.vertices = cell[ "vertices" ]
+        self.area = cell[ "area" ]
+        self.density = float( density )
+        self.nb_dims = 2
+        self.target_mass = target_mass
+
+    def normalized_version( self, nb_dims = None ):
+        return self
+
+    def box( self ):
+        return self.vertices.min( axis = 0 ), self.vertices.max( axis = 0 )
+
+    def bounding_half_spaces( self ):
+        return self.directions.copy(), self.offsets.copy()
+
+    def domain_polygon( self ):
+        """the counterclockwise vertices of the support"""
+        return self.vertices.copy()
+
+    @property
+    def rho( self ):
+        return self.target_mass / self.area
+
+    def native_args( self, device_tensor ):
+        return 0, self.rho, None, 0, 0, [ 0.0, 0.0, 1.0, 1.0 ]
+
+
 class SumOfGaussians:
     """isotropic Gaussians `weights_k N( positions_k, sigmas_k^2 I )` truncated to `support_box`
     ( reference `distributions/SumOfGaussians.py:11-101` ): `weights_k` is the MASS of Gaussian `k` ( all 1 by

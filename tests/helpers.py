@@ -91,8 +91,9 @@ def lifted_neighbours( pos, w, lo = ( 0, 0 ), hi = ( 1, 1 ), margin = 1e-10, amb
     return ( pairs, amb ) if ambiguous else pairs
 
 
-def halfspace_cell( pos, w, i, lo = ( 0, 0 ), hi = ( 1, 1 ) ):
-    """the vertices of cell `i` ( counter-clockwise ) by half-space intersection ( scipy ), or `None` if empty"""
+def halfspace_cell( pos, w, i, lo = ( 0, 0 ), hi = ( 1, 1 ), extra = None ):
+    """the vertices of cell `i` ( counter-clockwise ) by half-space intersection ( scipy ), or `None` if empty; `extra`: more
+    half-planes `( directions, offsets )` ( `direction . x <= offset`, absolute coordinates ) the domain has"""
     from scipy.spatial import HalfspaceIntersection, ConvexHull
     pos = np.asarray( pos, dtype = np.float64 )
     w = np.zeros( len( pos ) ) if w is None else np.asarray( w, dtype = np.float64 )
@@ -105,6 +106,9 @@ def halfspace_cell( pos, w, i, lo = ( 0, 0 ), hi = ( 1, 1 ) ):
         off = ( d @ d + w[ i ] - w[ j ] ) / 2
         hs.append( [ d[ 0 ], d[ 1 ], -off ] )
     hs += [ [ 1, 0, -( hi[ 0 ] - p0[ 0 ] ) ], [ -1, 0, lo[ 0 ] - p0[ 0 ] ], [ 0, 1, -( hi[ 1 ] - p0[ 1 ] ) ], [ 0, -1, lo[ 1 ] - p0[ 1 ] ] ]
+    if extra is not None:
+        for n, c in zip( np.asarray( extra[ 0 ], dtype = np.float64 ), np.asarray( extra[ 1 ], dtype = np.float64 ) ):
+            hs.append( [ n[ 0 ], n[ 1 ], -( c - n @ p0 ) ] )
     hs = np.asarray( hs )
     # an interior point: the seed's cell may not contain the seed; use a linear program
     from scipy.optimize import linprog
@@ -117,6 +121,25 @@ def halfspace_cell( pos, w, i, lo = ( 0, 0 ), hi = ( 1, 1 ) ):
     v = hsi.intersections
     hull = ConvexHull( v )
     return v[ hull.vertices ] + p0
+
+
+def regular_polygon( m, centre = ( 0.5, 0.5 ), radius = 0.45, phase = 0.1 ):
+    """the half-planes `( directions, offsets )` and the counterclockwise vertices of a regular `m`-gon"""
+    a = phase + 2 * np.pi * np.arange( m ) / m
+    v = np.column_stack( [ centre[ 0 ] + radius * np.cos( a ), centre[ 1 ] + radius * np.sin( a ) ] )
+    e = np.roll( v, -1, axis = 0 ) - v
+    n = np.column_stack( [ e[ :, 1 ], -e[ :, 0 ] ] )
+    n /= np.linalg.norm( n, axis = 1 )[ :, None ]
+    return ( n, ( n * v ).sum( axis = 1 ) ), v
+
+
+def points_in_polygon( n, dirs, offs, rng, lo = ( 0, 0 ), hi = ( 1, 1 ) ):
+    """`n` points uniform in the polygon `dirs . x <= offs` ( rejection from the box `[ lo, hi ]` )"""
+    out = np.zeros( ( 0, 2 ) )
+    while len( out ) < n:
+        p = rng.uniform( lo, hi, ( 2 * n, 2 ) )
+        out = np.concatenate( [ out, p[ np.all( p @ np.asarray( dirs ).T <= np.asarray( offs ), axis = 1 ) ] ] )
+    return out[ :n ]
 
 
 def polygon_area( xy ):
