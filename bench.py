@@ -90,9 +90,9 @@ def bench_batch( args ):
     total = int( sum( len( p ) for p, _ in inst ) )
     print( f"bench batch: { len( problems ) } instances, { total } Diracs", file = sys.stderr, flush = True )
     settings = sdot.Iterative( tol = args.tol, continuation = args.continuation, on_failure = "ignore", precision = args.precision,
-                               tuning = sdot.Tuning( step = args.step ) )
+                               tuning = sdot.Tuning( step = args.step, linear_solver = args.linear ) )
     plain = sdot.Iterative( tol = args.tol, continuation = args.continuation, on_failure = "ignore", precision = "fp64",
-                            tuning = sdot.Tuning( step = args.step ) )
+                            tuning = sdot.Tuning( step = args.step, linear_solver = args.linear ) )
 
     def batched():
         return sdot.solve_batch( problems, settings, streams = args.streams )

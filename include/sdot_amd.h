@@ -243,6 +243,26 @@ int sdot_amd_measures_3d( void *stream, long long n, const double *positions, co
                           int density_kind, double rho, const double *gauss, int ng, double *masses, long long *facet_i,
                           long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets, unsigned long long *events );
 
+/* ONE 3D DIAGRAM AGAINST A VOXEL IMAGE: `sdot_amd_measures_3d`'s arguments with the density a piecewise-constant image:
+ * `img` DEVICE [ nz ][ ny ][ nx ] ( voxel `( i, j, k )` at `( k ny + j ) nx + i` ), `img_geo` HOST [ 6 ]: the origin x, y, z,
+ * then the voxel's sides hx, hy, hz; the voxel `( i, j, k )` covers `[ x0 + i hx, x0 + ( i + 1 ) hx ] x ...`. Replaces the
+ * reference's N-D `Image` against 3D cells ( src/sdot/distributions/Image.py, `Image.cxx:106-232`, run on the host there ). */
+int sdot_amd_measures_3d_image( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                                const double *img, int nx, int ny, int nz, const double *img_geo, double *masses, long long *facet_i,
+                                long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets );
+
+/* THE MOMENTS OF 3D CELLS AGAINST A VOXEL IMAGE: `sdot_amd_moments_3d` with the image of `sdot_amd_measures_3d_image` as the
+ * density ( the reference's `PowerDiagram.moments` with an `Image`, src/sdot/PowerDiagram.py:433-460, host code there ). */
+int sdot_amd_moments_3d_image( void *stream, long long n, const double *positions, const double *weights, const double *box, const double *img,
+                               int nx, int ny, int nz, const double *img_geo, double *masses, double *first, double *second );
+
+/* THE 3D SOLVE AGAINST A VOXEL IMAGE: `sdot_amd_solve_3d`'s arguments with the image of `sdot_amd_measures_3d_image` as the
+ * density ( no width continuation: it is the Gaussians' in 3D ). Replaces the reference's 3D solve with an `Image` target
+ * ( src/sdot/SdotPlanNd.py:442-641, CPU there ). */
+int sdot_amd_solve_3d_image( void *stream, long long n, const double *positions, const double *nu, const double *w0, const double *options,
+                             const double *box, const double *img, int nx, int ny, int nz, const double *img_geo, double *weights,
+                             double *history, long long *nb_steps, double *stats, double *cell_masses, double *barycenters, double *cost );
+
 /* THE 3D SOLVE: the same damped Newton as `sdot_amd_solve_2d` on 3D cells, the step by trials ( the limits are a 2D step ),
  * a constant density or a sum of Gaussians ( the width continuation widens them ), no duplicates.
  * Replaces the 3D card solve `sdot::gpu2d::solve` with `SolveCard<3,...>` ( src/sdot/include/sdot/gpu/Newton2D.cuh:819,

@@ -111,6 +111,26 @@ def _density_args( density ):
     return 1, 0.0, img.shape[ 1 ], img.shape[ 0 ], geo, img, ( img, geo )
 
 
+def _support( lo, hi, density ):
+    """the domain of the calls: the box, cut for an image density by the block of its grid, as the reference cuts the box by
+    the distribution's bounding half-spaces ( src/sdot/PowerDiagram.py:158 ). The image is zero off its grid ( its cells
+    are clipped by the pixels ): what the cut changes is where the limits' frozen cells start from"""
+    lo, hi = _f64( lo ).copy(), _f64( hi ).copy()
+    if isinstance( density, dict ) and ( "values" in density or "voxels" in density ):
+        v = _f64( density[ "values" ] if "values" in density else density[ "voxels" ] )
+        if "values" in density:                          # ( `values[ ny, nx ]` )
+            o, h, nb = [ density[ "x0" ], density[ "y0" ] ], [ density[ "hx" ], density[ "hy" ] ], [ v.shape[ 1 ], v.shape[ 0 ] ]
+        else:                                            # ( `voxels[ nx, ny, nz ]` )
+            o = [ density[ "x0" ], density[ "y0" ], density[ "z0" ] ]
+            h, nb = [ density[ "hx" ], density[ "hy" ], density[ "hz" ] ], list( v.shape )
+        for a in range( len( lo ) ):
+            lo[ a ] = max( lo[ a ], o[ a ] )
+            hi[ a ] = min( hi[ a ], o[ a ] + nb[ a ] * h[ a ] )
+        if np.any( hi <= lo ):
+            raise ValueError( "oracle: the image's grid and the box do not meet" )
+    return lo, hi
+
+
 @contextlib.contextmanager
 def polygon_domain( xy ):
     """the calls inside take the convex polygon `xy` ( `[ nb, 2 ]`, counterclockwise ) as their domain: each cell starts from
@@ -137,8 +157,7 @@ def _measures( pos, w, lo, hi, density, facets, brute_force, events ):
     pos = _f64( pos )
     n = pos.shape[ 0 ]
     ww = None if w is None else _f64( w )
-    lo = _f64( lo )
-    hi = _f64( hi )
+    lo, hi = _support( lo, hi, density )
     kind, value, nx, ny, geo, img, keep = _density_args( density )
     masses = np.zeros( n )
     cap = 8 * n + 64 if facets else 0
@@ -184,6 +203,7 @@ def along( pos, w, d, t = 0.0, eps = 0.0, lo = ( 0, 0 ), hi = ( 1, 1 ), density 
     forward[ n ] at floor eps, backward[ n ] from the trial t )"""
     pos = _f64( pos )
     n = pos.shape[ 0 ]
+    lo, hi = _support( lo, hi, density )
     kind, value, nx, ny, geo, img, keep = _density_args( density )
     poly = np.zeros( ( n, 3 ) )
     ok = np.zeros( n, np.int32 )
@@ -250,6 +270,7 @@ def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = No
     n = pos.shape[ 0 ]
     nu = np.full( n, 1.0 / n ) if nu is None else _f64( nu )
     w0 = None if w0 is None else _f64( w0 )
+    lo, hi = _support( lo, hi, density )
     kind, value, nx, ny, geo, img, keep = _density_args( density )
     opts = _f64( [ tol, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol, lin_maxit, int( keep_start ),
                    { "trials": 0, "limits": 1 }[ step ], factor, { "never": 0, "auto": 1, "always": 2 }[ continuation ], conv_threshold,
@@ -275,11 +296,30 @@ def solve( pos, nu = None, w0 = None, lo = ( 0, 0 ), hi = ( 1, 1 ), density = No
 # ---- 3D ------------------------------------------------------------------------------------------------------------
 
 def _density3_args( density ):
-    """`density`: `None` / a float ( constant ), or `dict( gaussians = [ ng, 5 ] )` ( centre x, y, z, sigma, mass per row )"""
+    """`density`: `None` / a float ( constant ), `dict( gaussians = [ ng, 5 ] )` ( centre x, y, z, sigma, mass per row ), or
+    `dict( voxels = v[ nx, ny, nz ], x0, y0, z0, hx, hy, hz )` ( a voxel image, first index along x )"""
     if density is None or np.isscalar( density ):
         return 0, 1.0 if density is None else float( density ), 0, None
+    if "voxels" in density:
+        return 1, 0.0, 0, None
     g = _f64( density[ "gaussians" ] ).reshape( -1, 5 )
     return 2, 0.0, g.shape[ 0 ], g
+
+
+@contextlib.contextmanager
+def _image3( density ):
+    """the voxel image of `density` ( if it is one ) set for the calls inside"""
+    if density is None or np.isscalar( density ) or "voxels" not in density:
+        yield
+        return
+    v = _f64( density[ "voxels" ] )
+    flat = np.ascontiguousarray( v.transpose( 2, 1, 0 ) ).reshape( -1 )        # ( `( k ny + j ) nx + i` )
+    geo = _f64( [ density.get( "x0", 0.0 ), density.get( "y0", 0.0 ), density.get( "z0", 0.0 ), density[ "hx" ], density[ "hy" ], density[ "hz" ] ] )
+    lib().oracle_set_image3( ctypes.c_int( v.shape[ 0 ] ), ctypes.c_int( v.shape[ 1 ] ), ctypes.c_int( v.shape[ 2 ] ), _d( geo ), _d( flat ) )
+    try:
+        yield
+    finally:
+        lib().oracle_set_image3( ctypes.c_int( 0 ), ctypes.c_int( 0 ), ctypes.c_int( 0 ), None, None )
 
 
 def rules():
@@ -314,6 +354,7 @@ def measures_3d( pos, w = None, lo = ( 0, 0, 0 ), hi = ( 1, 1, 1 ), density = No
     pos = _f64( pos )
     n = pos.shape[ 0 ]
     ww = np.zeros( n ) if w is None else _f64( w )
+    lo, hi = _support( lo, hi, density )
     kind, value, ng, g = _density3_args( density )
     masses = np.zeros( n )
     cap = 24 * n + 64 if facets else 0
@@ -321,9 +362,10 @@ def measures_3d( pos, w = None, lo = ( 0, 0, 0 ), hi = ( 1, 1, 1 ), density = No
     fj = np.zeros( cap, np.int64 )
     fc = np.zeros( cap )
     info = np.zeros( 6, np.int64 )
-    nf = lib().oracle_measures_3d( ctypes.c_longlong( n ), _d( pos ), _d( ww ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
-                                   ctypes.c_double( value ), ctypes.c_int( ng ), _d( g ), _d( masses ), _l( fi ), _l( fj ), _d( fc ),
-                                   ctypes.c_longlong( cap ), _l( info ) )
+    with _image3( density ):
+        nf = lib().oracle_measures_3d( ctypes.c_longlong( n ), _d( pos ), _d( ww ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
+                                       ctypes.c_double( value ), ctypes.c_int( ng ), _d( g ), _d( masses ), _l( fi ), _l( fj ), _d( fc ),
+                                       ctypes.c_longlong( cap ), _l( info ) )
     res = [ masses ]
     if facets:
         if nf > cap:
@@ -362,6 +404,7 @@ def solve_3d( pos, nu = None, w0 = None, lo = ( 0, 0, 0 ), hi = ( 1, 1, 1 ), den
     n = pos.shape[ 0 ]
     nu = np.full( n, 1.0 / n ) if nu is None else _f64( nu )
     w0 = None if w0 is None else _f64( w0 )
+    lo, hi = _support( lo, hi, density )
     kind, value, ng, g = _density3_args( density )
     opts = _f64( [ tol, tol_rel, maxit, max_backtracks, t_min, mult_ok, residual, power, switch, lin_tol, lin_maxit, int( keep_start ),
                    0, 0.9, { "never": 0, "auto": 1, "always": 2 }[ continuation ], conv_threshold, conv_s0, conv_ratio, conv_min, 0, 0, 0,
@@ -372,8 +415,9 @@ def solve_3d( pos, nu = None, w0 = None, lo = ( 0, 0, 0 ), hi = ( 1, 1, 1 ), den
     masses = np.zeros( n )
     bary = np.zeros( ( n, 3 ) )
     cost = np.zeros( 1 )
-    nb = lib().oracle_solve_3d( ctypes.c_longlong( n ), _d( pos ), _d( nu ), _d( w0 ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
-                                ctypes.c_double( value ), ctypes.c_int( ng ), _d( g ), _d( opts ), _d( weights ), _d( stats ), _d( hist ),
-                                _d( masses ), _d( bary ), _d( cost ) )
+    with _image3( density ):
+        nb = lib().oracle_solve_3d( ctypes.c_longlong( n ), _d( pos ), _d( nu ), _d( w0 ), _d( _f64( lo ) ), _d( _f64( hi ) ), ctypes.c_int( kind ),
+                                    ctypes.c_double( value ), ctypes.c_int( ng ), _d( g ), _d( opts ), _d( weights ), _d( stats ), _d( hist ),
+                                    _d( masses ), _d( bary ), _d( cost ) )
     return dict( weights = weights, masses = masses, barycenters = bary, cost = float( cost[ 0 ] ),
                  stats = dict( zip( STATS, stats.tolist() ) ), history = hist[ :nb ].copy() )

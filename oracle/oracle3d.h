@@ -15,8 +15,13 @@
 // tetrahedra ( apex at the mean of its vertices, bases those triangles ) integrated by the 14-point degree-5 rule. Both rules are checked on every monomial of degree <= 5 by the tests.
 
 struct Density3 {
-    int           kind  = 0;             ///< 0 a constant, 2 a sum of Gaussians
+    int           kind  = 0;             ///< 0 a constant, 1 a voxel image, 2 a sum of Gaussians
     double        value = 1;
+    /// a voxel image ( `Image.cxx:106-232` in 3D ): voxel `( i, j, k )` covers `[ x0 + i hx, x0 + ( i + 1 ) hx ] x ...`, its value at
+    /// `img[ ( k ny + j ) nx + i ]`; the cells are cut by every voxel they meet
+    const double *img = nullptr;
+    int           nx = 0, ny = 0, nz = 0;
+    double        x0 = 0, y0 = 0, z0 = 0, hx = 1, hy = 1, hz = 1;
     int           ng = 0;
     const double *gauss = nullptr;       ///< [ ng ][ 5 ]: x, y, z, sigma, mass
     double        conv_s = 0;            ///< the continuation's width: sigma' = sqrt( sigma^2 + s^2 )
@@ -562,6 +567,45 @@ static double face_integral( const Density3 &dn, const double *p0, const Poly3 &
     return r;
 }
 
+// ---- a voxel image: the cell cut by every voxel it meets ------------------------------------------------------------
+
+/// the voxels of the cell ( seed frame, the seed at `p0` ), each cut by its six planes: `f( value, piece )` per non-empty piece.
+/// The cells outside the grid take the value of its edge voxels, as the card's kernels do
+template<class F>
+static void for_each_voxel_piece( const Density3 &dn, const double *p0, const Poly3 &c, F &&f ) {
+    double lo[ 3 ] = { 1e300, 1e300, 1e300 }, hi[ 3 ] = { -1e300, -1e300, -1e300 };
+    for ( int v = 0; v < c.nv; ++v ) {
+        const double q[ 3 ] = { c.x[ sz( v ) ], c.y[ sz( v ) ], c.z[ sz( v ) ] };
+        for ( int d = 0; d < 3; ++d ) { lo[ d ] = std::min( lo[ d ], q[ d ] ); hi[ d ] = std::max( hi[ d ], q[ d ] ); }
+    }
+    const double o[ 3 ] = { dn.x0 - p0[ 0 ], dn.y0 - p0[ 1 ], dn.z0 - p0[ 2 ] }, h[ 3 ] = { dn.hx, dn.hy, dn.hz };
+    const int nn[ 3 ] = { dn.nx, dn.ny, dn.nz };
+    int a[ 3 ], b[ 3 ];
+    for ( int d = 0; d < 3; ++d ) {
+        if ( hi[ d ] < o[ d ] || lo[ d ] > o[ d ] + nn[ d ] * h[ d ] ) return;      // ( the image is zero outside its grid )
+        auto vox = [&]( double x ) { const double t = std::floor( ( x - o[ d ] ) / h[ d ] ); return t < 0 ? 0 : ( t >= nn[ d ] - 1 ? nn[ d ] - 1 : int( t ) ); };
+        a[ d ] = vox( lo[ d ] ); b[ d ] = vox( hi[ d ] );
+    }
+    Poly3 p;
+    for ( int k = a[ 2 ]; k <= b[ 2 ]; ++k )
+        for ( int j = a[ 1 ]; j <= b[ 1 ]; ++j )
+            for ( int i = a[ 0 ]; i <= b[ 0 ]; ++i ) {
+                p = c;
+                const int idx[ 3 ] = { i, j, k };
+                bool empty = false;
+                for ( int d = 0; d < 3 && ! empty; ++d ) {            // ( the voxel's two planes along each axis )
+                    double n[ 3 ] = { 0, 0, 0 };
+                    n[ d ] = -1;
+                    empty = cut3( p, n[ 0 ], n[ 1 ], n[ 2 ], -( o[ d ] + idx[ d ] * h[ d ] ), -100 - 2 * d, nullptr ) == EMPTY;
+                    if ( ! empty ) {
+                        n[ d ] = 1;
+                        empty = cut3( p, n[ 0 ], n[ 1 ], n[ 2 ], o[ d ] + ( idx[ d ] + 1 ) * h[ d ], -101 - 2 * d, nullptr ) == EMPTY;
+                    }
+                }
+                if ( ! empty && p.nv >= 4 ) f( dn.img[ ( size_t( k ) * size_t( dn.ny ) + size_t( j ) ) * size_t( dn.nx ) + size_t( i ) ], p );
+            }
+}
+
 // ---- the walk -----------------------------------------------------------------------------------------------------
 
 static bool can_cut3( const Tree3 &t, I64 m, const Poly3 &c, const double *p0, double w0, Events *ev ) {
@@ -652,6 +696,32 @@ template<class F>
 static double mass_and_facets3( const Tree3 &t, const Density3 &dn, I64 k, const Poly3 &c, bool with_facets, F &&facet ) {
     const double *p0 = &t.p[ sz( 3 * k ) ];
     double m = 0;
+    if ( dn.kind == 1 ) {                                // a voxel image: the volumes and face areas of the voxels' pieces
+        std::vector<std::pair<int, double>> parts;       // ( the facets' `int rho`, by neighbour rank )
+        Density3 unit;
+        unit.kind = 0; unit.value = 1;
+        for_each_voxel_piece( dn, p0, c, [&]( double rho, const Poly3 &piece ) {
+            for_each_face_and_tet( piece, [&]( int f, const std::vector<int> &loop ) {
+                const Face3 &fc = piece.f[ sz( f ) ];
+                if ( ! with_facets || fc.id < 0 ) return;
+                parts.emplace_back( fc.id, rho * face_integral( unit, p0, piece, loop ) );
+            }, [&]( const double *, const double *, const double *, const double *, double vol ) { m += rho * vol; } );
+        } );
+        if ( with_facets ) {
+            std::stable_sort( parts.begin(), parts.end(), []( const auto &x, const auto &y ) { return x.first < y.first; } );
+            for ( size_t q = 0; q < parts.size(); ) {
+                size_t e = q;
+                double s = 0;
+                while ( e < parts.size() && parts[ e ].first == parts[ q ].first ) s += parts[ e++ ].second;
+                const I64 j = parts[ q ].first;
+                double d = 0;
+                for ( const Face3 &fc : c.f ) if ( fc.id == j ) d = std::sqrt( fc.nx * fc.nx + fc.ny * fc.ny + fc.nz * fc.nz );
+                if ( d > 0 ) facet( j, s / ( 2 * d ) );
+                q = e;
+            }
+        }
+        return m;
+    }
     for_each_face_and_tet( c, [&]( int f, const std::vector<int> &loop ) {
         const Face3 &fc = c.f[ sz( f ) ];
         const double d = std::sqrt( fc.nx * fc.nx + fc.ny * fc.ny + fc.nz * fc.nz );
@@ -730,11 +800,22 @@ struct Sweep3 {
                 const I64 i = t.order[ sz( k ) ];
                 const double *p0 = &t.p[ sz( 3 * k ) ];
                 double m = 0, mx[ 3 ] = { 0, 0, 0 }, m2 = 0;
-                if ( make_cell3( t, k, lo, hi, c, nullptr ) )
-                    for_each_face_and_tet( c, []( int, const std::vector<int> & ) {},
-                                           [&]( const double *A, const double *B, const double *C, const double *D, double vol ) {
-                                               tet_moments( dn, p0, A, B, C, D, vol, m, mx, m2 );
-                                           } );
+                if ( make_cell3( t, k, lo, hi, c, nullptr ) ) {
+                    if ( dn.kind == 1 ) {                // a voxel image: the pieces' moments, each at its voxel's value
+                        for_each_voxel_piece( dn, p0, c, [&]( double rho, const Poly3 &piece ) {
+                            Density3 cst;
+                            cst.kind = 0; cst.value = rho;
+                            for_each_face_and_tet( piece, []( int, const std::vector<int> & ) {},
+                                                   [&]( const double *A, const double *B, const double *C, const double *D, double vol ) {
+                                                       tet_moments( cst, p0, A, B, C, D, vol, m, mx, m2 );
+                                                   } );
+                        } );
+                    } else
+                        for_each_face_and_tet( c, []( int, const std::vector<int> & ) {},
+                                               [&]( const double *A, const double *B, const double *C, const double *D, double vol ) {
+                                                   tet_moments( dn, p0, A, B, C, D, vol, m, mx, m2 );
+                                               } );
+                }
                 cth[ sz( th ) ] += m2;
                 for ( int d = 0; d < 3; ++d ) bary[ sz( 3 * i + d ) ] = m > 0 ? p0[ d ] + mx[ d ] / m : p0[ d ];
             }

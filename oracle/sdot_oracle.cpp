@@ -1125,7 +1125,9 @@ struct Along {
 
 /// `int F0 dy` along the segment `a -> b` of the seed frame, `F0( x, y ) = int rho( s, y ) ds` from the seed's column to `x`
 /// along the row of `y`: exact per pixel piece ( `F0` linear there: its middle times the piece's `dy` ). The pieces come from
-/// the sorted crossings of the segment with the grid lines ( `gpu/Density2D.cuh:95` walks them instead ).
+/// the sorted crossings of the segment with the grid lines ( `gpu/Density2D.cuh:95` walks them instead ). A frozen cell may
+/// leave the domain: its points off the grid take the nearest pixel, as `image_col` / `image_row` of `gpu/Density2D.cuh:82-83`
+/// clamp them.
 static double image_edge_mass( const Density &dn, double px, double py, double ax, double ay, double bx, double by ) {
     const double ox = dn.x0 - px, oy = dn.y0 - py;
     int iref = int( std::floor( -ox / dn.hx ) );
@@ -2202,7 +2204,19 @@ int oracle_solve( I64 n, const double *pos, const double *nu_in, const double *w
 
 // ---- 3D ---------------------------------------------------------------------------------------------------------------
 
+/// the voxel image of the next 3D calls with density kind 1 ( `oracle_set_image3` ), kept by the caller around them
+static orc::Density3 g_image3;
+
+extern "C" void oracle_set_image3( int nx, int ny, int nz, const double *geo, const double *img ) {
+    g_image3 = orc::Density3();
+    if ( ! img ) return;
+    g_image3.kind = 1;
+    g_image3.img = img; g_image3.nx = nx; g_image3.ny = ny; g_image3.nz = nz;
+    g_image3.x0 = geo[ 0 ]; g_image3.y0 = geo[ 1 ]; g_image3.z0 = geo[ 2 ]; g_image3.hx = geo[ 3 ]; g_image3.hy = geo[ 4 ]; g_image3.hz = geo[ 5 ];
+}
+
 static orc::Density3 make_density3( int kind, double value, int ng, const double *gauss ) {
+    if ( kind == 1 ) return g_image3;
     orc::Density3 d;
     d.kind = kind;
     d.value = value;

@@ -10,6 +10,7 @@
 #include "memory.h"
 #include "noalloc.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace sdot_amd {
@@ -92,6 +93,18 @@ Density3 make_density3( int kind, double rho, const double *gauss, int ng ) {
     return d;
 }
 
+/// a voxel image `img[ ( k ny + j ) nx + i ]` ( device ), `geo` HOST [ 6 ]: origin x, y, z, then the voxel's sides
+Density3 make_image3( const double *img, int nx, int ny, int nz, const double *geo ) {
+    SDOT_CHECK( img && nx >= 1 && ny >= 1 && nz >= 1 && geo, "a 3D image needs its voxels, their counts and its geometry" );
+    SDOT_CHECK( geo[ 3 ] > 0 && geo[ 4 ] > 0 && geo[ 5 ] > 0, "a 3D image's voxels have positive sides" );
+    Density3 d;
+    d.kind = 1;
+    d.rho = 0;
+    d.img = img; d.nx = nx; d.ny = ny; d.nz = nz;
+    d.x0 = geo[ 0 ]; d.y0 = geo[ 1 ]; d.z0 = geo[ 2 ]; d.hx = geo[ 3 ]; d.hy = geo[ 4 ]; d.hz = geo[ 5 ];
+    return d;
+}
+
 void check_box3( const double *box ) {
     SDOT_CHECK( box != nullptr, "the domain box is required" );
     SDOT_CHECK( box[ 3 ] > box[ 0 ] && box[ 4 ] > box[ 1 ] && box[ 5 ] > box[ 2 ], "the domain box is empty" );
@@ -100,6 +113,38 @@ void check_box3( const double *box ) {
 void check_box( const double *box ) {
     SDOT_CHECK( box != nullptr, "the domain box is required" );
     SDOT_CHECK( box[ 2 ] > box[ 0 ] && box[ 3 ] > box[ 1 ], "the domain box is empty" );
+}
+
+/// An image is zero outside its grid ( the reference's `Image` has the grid as its support, `Image.cxx:106-160` ): the
+/// integrals over the cells in `box` are those over the cells in the part of `box` that the grid covers, which the kernels then
+/// take as their domain ( seeds may lie outside it ). `out` is that part for an image, `box` for the other densities.
+const double *support_box( const Density &dn, const double *box, double *out ) {
+    if ( dn.kind != SDOT_AMD_IMAGE ) return box;
+    out[ 0 ] = std::max( box[ 0 ], dn.x0 );
+    out[ 1 ] = std::max( box[ 1 ], dn.y0 );
+    out[ 2 ] = std::min( box[ 2 ], dn.x0 + double( dn.nx ) * dn.hx );
+    out[ 3 ] = std::min( box[ 3 ], dn.y0 + double( dn.ny ) * dn.hy );
+    SDOT_CHECK( out[ 2 ] > out[ 0 ] && out[ 3 ] > out[ 1 ], "the image's grid and the domain box do not meet" );
+    if ( dn.dom.nb ) {                                   // ( a polygon domain starts the cells: it has to lie on the grid )
+        const double tx = 1e-12 * ( double( dn.nx ) * dn.hx ), ty = 1e-12 * ( double( dn.ny ) * dn.hy );
+        for ( int i = 0; i < dn.dom.nb; ++i )
+            SDOT_CHECK( dn.dom.x[ i ] >= dn.x0 - tx && dn.dom.x[ i ] <= dn.x0 + double( dn.nx ) * dn.hx + tx &&
+                        dn.dom.y[ i ] >= dn.y0 - ty && dn.dom.y[ i ] <= dn.y0 + double( dn.ny ) * dn.hy + ty,
+                        "a polygon domain with an image density lies on the image's grid" );
+    }
+    return out;
+}
+
+/// `support_box` of a 3D voxel image
+const double *support_box3( const Density3 &dn, const double *box, double *out ) {
+    if ( dn.kind != 1 ) return box;
+    const double o[ 3 ] = { dn.x0, dn.y0, dn.z0 }, e[ 3 ] = { dn.x0 + dn.nx * dn.hx, dn.y0 + dn.ny * dn.hy, dn.z0 + dn.nz * dn.hz };
+    for ( int d = 0; d < 3; ++d ) {
+        out[ d ] = std::max( box[ d ], o[ d ] );
+        out[ 3 + d ] = std::min( box[ 3 + d ], e[ d ] );
+        SDOT_CHECK( out[ 3 + d ] > out[ d ], "the image's grid and the domain box do not meet" );
+    }
+    return out;
 }
 
 /// heap rank -> preorder node, and each preorder node's height
@@ -223,7 +268,9 @@ int sdot_amd_measures_2d( void *stream, long long n, const double *positions, co
         check_box( box );
         SDOT_CHECK( ! facet_i || ( facet_j && facet_c ), "measures: facet_i, facet_j and facet_c go together" );
         const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
-        measures_2d( hipStream_t( stream ), n, positions, weights, box, dn, masses, facet_i, facet_j, facet_c, facet_cap, nb_facets, events );
+        double sb[ 4 ];
+        measures_2d( hipStream_t( stream ), n, positions, weights, support_box( dn, box, sb ), dn, masses, facet_i, facet_j, facet_c, facet_cap,
+                     nb_facets, events );
     } );
 }
 
@@ -238,8 +285,9 @@ static int measures_entry( void *stream, long long n, const double *positions, c
         SDOT_CHECK( ! facet_i || ( facet_j && facet_c ), "measures: facet_i, facet_j and facet_c go together" );
         Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
         if ( nb_domain ) set_domain( dn, nb_domain, domain_xy );
-        measures_2d( hipStream_t( stream ), n, positions, weights, box, dn, masses, facet_i, facet_j, facet_c, facet_cap, nb_facets, events,
-                     precision == SDOT_AMD_FP32 );
+        double sb[ 4 ];
+        measures_2d( hipStream_t( stream ), n, positions, weights, support_box( dn, box, sb ), dn, masses, facet_i, facet_j, facet_c, facet_cap,
+                     nb_facets, events, precision == SDOT_AMD_FP32 );
     } );
 }
 
@@ -266,7 +314,8 @@ int sdot_amd_moments_2d( void *stream, long long n, const double *positions, con
         SDOT_CHECK( n >= 1 && positions && masses && first && second, "moments: positions, masses, first and second are required" );
         check_box( box );
         const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
-        moments_2d( hipStream_t( stream ), n, positions, weights, box, dn, masses, first, second );
+        double sb[ 4 ];
+        moments_2d( hipStream_t( stream ), n, positions, weights, support_box( dn, box, sb ), dn, masses, first, second );
     } );
 }
 
@@ -277,6 +326,17 @@ int sdot_amd_moments_3d( void *stream, long long n, const double *positions, con
         check_box3( box );
         const Density3 dn = make_density3( density_kind, rho, gauss, ng );
         moments_3d( hipStream_t( stream ), n, positions, weights, box, dn, masses, first, second );
+    } );
+}
+
+int sdot_amd_moments_3d_image( void *stream, long long n, const double *positions, const double *weights, const double *box, const double *img,
+                               int nx, int ny, int nz, const double *img_geo, double *masses, double *first, double *second ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && masses && first && second, "moments: positions, masses, first and second are required" );
+        check_box3( box );
+        const Density3 dn = make_image3( img, nx, ny, nz, img_geo );
+        double sb[ 6 ];
+        moments_3d( hipStream_t( stream ), n, positions, weights, support_box3( dn, box, sb ), dn, masses, first, second );
     } );
 }
 
@@ -303,7 +363,8 @@ int sdot_amd_measures_vjp_2d( void *stream, long long n, const double *positions
                     "measures_vjp: positions, grad_masses and a gradient to write are required" );
         check_box( box );
         const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
-        measures_vjp_2d( hipStream_t( stream ), n, positions, weights, box, dn, grad_masses, grad_weights, grad_positions );
+        double sb[ 4 ];
+        measures_vjp_2d( hipStream_t( stream ), n, positions, weights, support_box( dn, box, sb ), dn, grad_masses, grad_weights, grad_positions );
     } );
 }
 
@@ -314,7 +375,8 @@ int sdot_amd_measures_2d_noalloc( void *stream, long long n, const double *posit
         SDOT_CHECK( n >= 1 && positions && masses, "measures: positions and masses are required" );
         check_box( box );
         const Density dn = make_density( density_kind, rho, img, nx, ny, img_geo );
-        measures_2d_noalloc( hipStream_t( stream ), n, positions, weights, box, dn, masses, events );
+        double sb[ 4 ];
+        measures_2d_noalloc( hipStream_t( stream ), n, positions, weights, support_box( dn, box, sb ), dn, masses, events );
     } );
 }
 
@@ -346,6 +408,9 @@ static int solve_entry( void *stream, long long n, const double *positions, cons
         const double geo[ 4 ] = { options[ O_IMG_X0 ], options[ O_IMG_Y0 ], options[ O_IMG_HX ], options[ O_IMG_HY ] };
         in.dens = make_density( density_kind, rho, img, int( options[ O_IMG_NX ] ), int( options[ O_IMG_NY ] ), geo );
         if ( nb_domain ) set_domain( in.dens, nb_domain, domain_xy );
+        double sb[ 4 ];
+        const double *b = support_box( in.dens, box, sb );
+        for ( int c = 0; c < 4; ++c ) in.box[ c ] = b[ c ];
         SolveOutputs out;
         out.weights = weights;
         out.cell_masses = cell_masses;
@@ -394,6 +459,59 @@ int sdot_amd_measures_3d( void *stream, long long n, const double *positions, co
         measures_3d( hipStream_t( stream ), n, positions, weights, box, make_density3( density_kind, rho, gauss, ng ), masses, facet_i, facet_j,
                      facet_c, facet_cap, nb_facets, events );
     } );
+}
+
+int sdot_amd_measures_3d_image( void *stream, long long n, const double *positions, const double *weights, const double *box,
+                                const double *img, int nx, int ny, int nz, const double *img_geo, double *masses, long long *facet_i,
+                                long long *facet_j, double *facet_c, long long facet_cap, long long *nb_facets ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 1 && positions && masses, "measures: positions and masses are required" );
+        check_box3( box );
+        SDOT_CHECK( ! facet_i || ( facet_j && facet_c ), "measures: facet_i, facet_j and facet_c go together" );
+        const Density3 dn = make_image3( img, nx, ny, nz, img_geo );
+        double sb[ 6 ];
+        measures_3d( hipStream_t( stream ), n, positions, weights, support_box3( dn, box, sb ), dn, masses, facet_i, facet_j, facet_c, facet_cap,
+                     nb_facets, nullptr );
+    } );
+}
+
+static int solve3_entry( void *stream, long long n, const double *positions, const double *nu, const double *w0, const double *options,
+                         const double *box, const Density3 &dn, double *weights, double *history, long long *nb_steps, double *stats,
+                         double *cell_masses, double *barycenters, double *cost ) {
+    return guarded( [&] {
+        SDOT_CHECK( n >= 2 && positions && nu && options && weights && cell_masses && barycenters, "solve: a required array is null" );
+        check_box3( box );
+        SDOT_CHECK( options[ O_MAXIT ] >= 0, "solve: negative maxit" );
+        SolveInputs in;
+        in.dim = 3;
+        in.n = n;
+        in.positions = positions;
+        in.nu = nu;
+        in.w0 = w0;
+        in.options = options;
+        double sb[ 6 ];
+        const double *b = support_box3( dn, box, sb );
+        for ( int c = 0; c < 6; ++c ) in.box3[ c ] = b[ c ];
+        in.dens3 = dn;
+        SolveOutputs out;
+        out.weights = weights;
+        out.cell_masses = cell_masses;
+        out.barycenters = barycenters;
+        out.history = history;
+        out.nb_steps = nb_steps;
+        out.stats = stats;
+        out.cost = cost;
+        solve_2d( in, out, hipStream_t( stream ) );
+    } );
+}
+
+int sdot_amd_solve_3d_image( void *stream, long long n, const double *positions, const double *nu, const double *w0, const double *options,
+                             const double *box, const double *img, int nx, int ny, int nz, const double *img_geo, double *weights,
+                             double *history, long long *nb_steps, double *stats, double *cell_masses, double *barycenters, double *cost ) {
+    Density3 dn;
+    const int e = guarded( [&] { dn = make_image3( img, nx, ny, nz, img_geo ); } );
+    if ( e ) return e;
+    return solve3_entry( stream, n, positions, nu, w0, options, box, dn, weights, history, nb_steps, stats, cell_masses, barycenters, cost );
 }
 
 int sdot_amd_solve_3d( void *stream, long long n, const double *positions, const double *nu, const double *w0, const double *options,
@@ -469,6 +587,9 @@ int batch_call( void *stream, int nb_streams, long long k, const long long *offs
             J.in.precision = precision;
             for ( int c = 0; c < 4; ++c ) J.in.box[ c ] = boxes[ 4 * q + c ];
             J.in.dens = make_density( density_kind, rho ? rho[ q ] : 1.0, imgs ? imgs[ q ] : nullptr, nx, ny, geo );
+            double sb[ 4 ];
+            const double *bq = support_box( J.in.dens, J.in.box, sb );
+            for ( int c = 0; c < 4; ++c ) J.in.box[ c ] = bq[ c ];
             J.out.weights = weights + b;
             J.out.cell_masses = cell_masses + b;
             J.out.barycenters = barycenters + 2 * b;
@@ -556,6 +677,9 @@ int sdot_amd_server_submit_nd( long long server, int dim, int precision, long lo
             for ( int c = 0; c < 4; ++c ) J.in.box[ c ] = box[ c ];
             const double geo[ 4 ] = { options[ O_IMG_X0 ], options[ O_IMG_Y0 ], options[ O_IMG_HX ], options[ O_IMG_HY ] };
             J.in.dens = make_density( density_kind, rho, density, int( options[ O_IMG_NX ] ), int( options[ O_IMG_NY ] ), geo );
+            double sb[ 4 ];
+            const double *b = support_box( J.in.dens, box, sb );
+            for ( int c = 0; c < 4; ++c ) J.in.box[ c ] = b[ c ];
         }
         J.out.weights = weights; J.out.cell_masses = cell_masses; J.out.barycenters = barycenters; J.out.stats = stats; J.out.cost = cost;
         // the admission's footprint: the byte model of the solve, part by part

@@ -8,10 +8,14 @@ namespace sdot_amd {
 
 /// the 3D target density: a constant on the domain box, or a sum of isotropic Gaussians truncated to the box
 struct Density3 {
-    int           kind = 0;        ///< 0 constant, 2 a sum of Gaussians
+    int           kind = 0;        ///< 0 constant, 1 a voxel image, 2 a sum of Gaussians
     double        rho = 1;
     const double *gauss = nullptr; ///< device, `[ ng ][ 5 ]`: centre x, y, z, width, mass
     int           ng = 0;
+    /// a voxel image: voxel `( i, j, k )` covers `[ x0 + i hx, x0 + ( i + 1 ) hx ] x ...`, its value at `img[ ( k ny + j ) nx + i ]`
+    const double *img = nullptr;   ///< device
+    int           nx = 0, ny = 0, nz = 0;
+    double        x0 = 0, y0 = 0, z0 = 0, hx = 1, hy = 1, hz = 1;
 };
 
 /// the vertex and face slots of a cell in the first pass ( one lane per vertex ); a cell past them goes to the second

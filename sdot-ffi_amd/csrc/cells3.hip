@@ -137,6 +137,148 @@ struct CellSums {
     double mass = 0, mx = 0, my = 0, mz = 0, m2 = 0;
 };
 
+// ---- a voxel image: fluxes through the faces, cut by the voxel grid ------------------------------------------------------
+//
+// The divergence theorem with fields along x: `int_cell rho = flux of ( G, 0, 0 )`, `G( x, y, z ) = int_a^x rho( s, y, z ) ds`
+// along the voxel row of `( y, z )` from a column `a` the same for the whole cell ( its mean vertex's: the terms stay of the
+// cell's size ). In a voxel `G` is affine in `x`, so a face cut by the voxel grid integrates it exactly piece by piece. The
+// moments are fluxes too: `( H, 0, 0 )`, `( y G, 0, 0 )`, `( z G, 0, 0 )`, `( K + ( y^2 + z^2 ) G, 0, 0 )` with
+// `H = int_a^x s rho`, `K = int_a^x s^2 rho` ( degree 3 at most in a voxel: the 7-point degree-5 rule on each piece's fan ).
+// The reference clips the cell by every voxel instead ( `Image.cxx:106-232`, in N-D ); the oracle does that.
+
+constexpr int VPMAX = 12;                     ///< the vertices of a triangle's piece after six cuts ( 3 + 6 )
+struct VPoly { int n = 0; double x[ VPMAX ], y[ VPMAX ], z[ VPMAX ]; };
+
+/// the part of convex polygon `a` where coordinate `axis` is `< c` ( `below` ) or `>= c`: the slabs `[ c_i, c_i+1 )` share a
+/// polygon out without a gap, and one lying on a plane goes to one slab only ( a box face on a grid plane, its flux counted once )
+__device__ __forceinline__ void clip_axis( const VPoly &a, int axis, double c, bool below, VPoly &o ) {
+    o.n = 0;
+    for ( int q = 0; q < a.n; ++q ) {
+        const int r = q + 1 < a.n ? q + 1 : 0;
+        const double sq = ( axis == 0 ? a.x[ q ] : axis == 1 ? a.y[ q ] : a.z[ q ] ) - c;
+        const double sr = ( axis == 0 ? a.x[ r ] : axis == 1 ? a.y[ r ] : a.z[ r ] ) - c;
+        const bool iq = below ? sq < 0 : sq >= 0, ir = below ? sr < 0 : sr >= 0;
+        if ( iq && o.n < VPMAX ) { o.x[ o.n ] = a.x[ q ]; o.y[ o.n ] = a.y[ q ]; o.z[ o.n ] = a.z[ q ]; ++o.n; }
+        if ( iq != ir && o.n < VPMAX ) {
+            const double t = sq / ( sq - sr );
+            o.x[ o.n ] = a.x[ q ] + t * ( a.x[ r ] - a.x[ q ] );
+            o.y[ o.n ] = a.y[ q ] + t * ( a.y[ r ] - a.y[ q ] );
+            o.z[ o.n ] = a.z[ q ] + t * ( a.z[ r ] - a.z[ q ] );
+            if ( axis == 0 ) o.x[ o.n ] = c; else if ( axis == 1 ) o.y[ o.n ] = c; else o.z[ o.n ] = c;
+            ++o.n;
+        }
+    }
+}
+
+__device__ __forceinline__ int voxel_of( double v, double v0, double h, int n ) {
+    const double c = floor( ( v - v0 ) / h );
+    return c < 0 ? 0 : ( c >= double( n - 1 ) ? n - 1 : int( c ) );
+}
+
+/// the slabs `q0 .. q1` that a polygon spanning `[ lo, hi ]` along an axis may meet, decided by the comparisons `clip_axis`
+/// makes ( `voxel_of`'s division rounds otherwise: a polygon on or next to a plane would miss the slab that takes it )
+__device__ __forceinline__ void slabs_of( double lo, double hi, double v0, double h, int n, int &q0, int &q1 ) {
+    q0 = voxel_of( lo, v0, h, n );
+    q1 = voxel_of( hi, v0, h, n );
+    while ( q0 > 0 && lo - ( v0 + q0 * h ) < 0 ) --q0;
+    while ( q1 < n - 1 && hi - ( v0 + ( q1 + 1 ) * h ) >= 0 ) ++q1;
+}
+
+/// the triangle `( g, v, o )` of a face of outward unit normal `u` ( seed frame, the seed at `p0` ) against a voxel image:
+/// the facet's `int rho`, the flux of `( G, 0, 0 )` ( the mass ), and with `moments` the moments' fluxes, `G` taken from the
+/// column of `a` ( seed frame )
+__device__ CellSums image_terms( const Density3 &dn, const double4 &p0, bool facets, bool moments, double a, V3 g, V3 v, V3 o, V3 u ) {
+    CellSums S;
+    const double X0 = dn.x0 - p0.x, Y0 = dn.y0 - p0.y, Z0 = dn.z0 - p0.z;   // ( the grid in the seed's frame )
+    const int iref = voxel_of( a, X0, dn.hx, dn.nx );
+    VPoly T;
+    T.n = 3;
+    T.x[ 0 ] = g.x; T.y[ 0 ] = g.y; T.z[ 0 ] = g.z;
+    T.x[ 1 ] = v.x; T.y[ 1 ] = v.y; T.z[ 1 ] = v.z;
+    T.x[ 2 ] = o.x; T.y[ 2 ] = o.y; T.z[ 2 ] = o.z;
+    const double zmin = fmin( g.z, fmin( v.z, o.z ) ), zmax = fmax( g.z, fmax( v.z, o.z ) );
+    int k0, k1;
+    slabs_of( zmin, zmax, Z0, dn.hz, dn.nz, k0, k1 );
+    const double s15 = sqrt( 15.0 );
+    const double ta1 = ( 6 - s15 ) / 21, tw1 = ( 155 - s15 ) / 1200, ta2 = ( 6 + s15 ) / 21, tw2 = ( 155 + s15 ) / 1200;
+    VPoly A, B, C, D, E;
+    for ( int k = k0; k <= k1; ++k ) {
+        // the slab of row `k` ( the first and last slabs unbounded outside the grid: the edge voxels continue the image )
+        if ( k > 0 ) clip_axis( T, 2, Z0 + k * dn.hz, false, A ); else A = T;
+        if ( k < dn.nz - 1 ) clip_axis( A, 2, Z0 + ( k + 1 ) * dn.hz, true, B ); else B = A;
+        if ( B.n < 3 ) continue;
+        double ymin = 1e300, ymax = -1e300;
+        for ( int q = 0; q < B.n; ++q ) { ymin = fmin( ymin, B.y[ q ] ); ymax = fmax( ymax, B.y[ q ] ); }
+        int j0, j1;
+        slabs_of( ymin, ymax, Y0, dn.hy, dn.ny, j0, j1 );
+        for ( int j = j0; j <= j1; ++j ) {
+            if ( j > 0 ) clip_axis( B, 1, Y0 + j * dn.hy, false, C ); else C = B;
+            if ( j < dn.ny - 1 ) clip_axis( C, 1, Y0 + ( j + 1 ) * dn.hy, true, D ); else D = C;
+            if ( D.n < 3 ) continue;
+            double xmin = 1e300, xmax = -1e300;
+            for ( int q = 0; q < D.n; ++q ) { xmin = fmin( xmin, D.x[ q ] ); xmax = fmax( xmax, D.x[ q ] ); }
+            int i0, i1;
+            slabs_of( xmin, xmax, X0, dn.hx, dn.nx, i0, i1 );
+            const double *row = dn.img + ( size_t( k ) * size_t( dn.ny ) + size_t( j ) ) * size_t( dn.nx );
+            // the row integrals from column `iref` to the left side of column `i0`
+            double s0 = 0, s1 = 0, s2 = 0;
+            auto side = [&]( int q ) { return X0 + q * dn.hx; };
+            if ( i0 >= iref )
+                for ( int q = iref; q < i0; ++q ) {
+                    const double lo = side( q ), hi = side( q + 1 ), r = row[ q ];
+                    s0 += r * ( hi - lo ); s1 += r * ( hi * hi - lo * lo ) / 2; s2 += r * ( hi * hi * hi - lo * lo * lo ) / 3;
+                }
+            else
+                for ( int q = i0; q < iref; ++q ) {
+                    const double lo = side( q ), hi = side( q + 1 ), r = row[ q ];
+                    s0 -= r * ( hi - lo ); s1 -= r * ( hi * hi - lo * lo ) / 2; s2 -= r * ( hi * hi * hi - lo * lo * lo ) / 3;
+                }
+            for ( int i = i0; i <= i1; ++i ) {
+                if ( i > 0 ) clip_axis( D, 0, side( i ), false, E ); else E = D;
+                VPoly F;
+                if ( i < dn.nx - 1 ) clip_axis( E, 0, side( i + 1 ), true, F ); else F = E;
+                const double r = row[ i ], xl = side( i );
+                if ( F.n >= 3 ) {
+                    // the piece's fan from its first vertex
+                    for ( int q = 1; q + 1 < F.n; ++q ) {
+                        const double ax = F.x[ 0 ], ay = F.y[ 0 ], az = F.z[ 0 ];
+                        const double bx = F.x[ q ] - ax, by = F.y[ q ] - ay, bz = F.z[ q ] - az;
+                        const double cx = F.x[ q + 1 ] - ax, cy = F.y[ q + 1 ] - ay, cz = F.z[ q + 1 ] - az;
+                        const double nx = by * cz - bz * cy, ny = bz * cx - bx * cz, nz = bx * cy - by * cx;
+                        const double ar = 0.5 * sqrt( nx * nx + ny * ny + nz * nz );
+                        if ( facets ) S.part += r * ar;
+                        if ( ! moments ) {       // `G` affine in the voxel: the centroid
+                            const double xc = ax + ( bx + cx ) / 3;
+                            S.mass += ar * u.x * ( s0 + r * ( xc - xl ) );
+                            continue;
+                        }
+                        for ( int qq = 0; qq < 7; ++qq ) {
+                            double b1, b2, wq;
+                            if ( qq == 0 ) { b1 = b2 = 1.0 / 3; wq = 9.0 / 40; }
+                            else {
+                                const double t = qq <= 3 ? ta1 : ta2;
+                                wq = qq <= 3 ? tw1 : tw2;
+                                const int jj = ( qq - 1 ) % 3;
+                                b1 = jj == 1 ? 1 - 2 * t : t; b2 = jj == 2 ? 1 - 2 * t : t;
+                            }
+                            const double x = ax + b1 * bx + b2 * cx, y = ay + b1 * by + b2 * cy, z = az + b1 * bz + b2 * cz;
+                            const double G = s0 + r * ( x - xl ), H = s1 + r * ( x * x - xl * xl ) / 2, K = s2 + r * ( x * x * x - xl * xl * xl ) / 3;
+                            const double w = wq * ar * u.x;
+                            S.mass += w * G;
+                            S.mx += w * H; S.my += w * y * G; S.mz += w * z * G;
+                            S.m2 += w * ( K + ( y * y + z * z ) * G );
+                        }
+                    }
+                }
+                const double hi = side( i + 1 );
+                s0 += r * ( hi - xl ); s1 += r * ( hi * hi - xl * xl ) / 2; s2 += r * ( hi * hi * hi - xl * xl * xl ) / 3;
+            }
+        }
+    }
+    return S;
+}
+
+
 /// the triangle `( g, v, o )` of a face of outward unit normal `u`, and the tetrahedron it makes with the cell's mean
 /// vertex `A` ( seed frame; the seed at `p0` ): the facet's `int rho`, the mass ( a constant: the tetrahedron's volume; a
 /// sum of Gaussians: the flux through the triangle ), the moments ( the tetrahedron's )
@@ -615,6 +757,7 @@ struct MeasLds {
 /// the second kernel of a sum of Gaussians: the cells the first pass kept, one wave each. The triangles of the cell ( a
 /// face's mean vertex and one of its edges, `3 nv` of them ) are spread over the lanes, each face's sums taken in a fixed
 /// order: the same terms as the face-by-face loop of the first pass, with every lane at work
+template<bool IMAGE>
 __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3_measure( Params3 P ) {
     __shared__ MeasLds lds[ WAVES3 ];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -679,7 +822,10 @@ __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3_measure( Params3 P ) {
     CellSums S;
     for ( int t = lane; t < T; t += 64 ) {
         const int f = M.tf[ t ], v = M.tv[ t ], w = M.tw[ t ];
-        const CellSums d = edge_terms<true>( P.dn, p0, facets, moments, M.fid[ f ] < 0, Av, V3{ M.gx[ f ], M.gy[ f ], M.gz[ f ] },
+        const CellSums d = IMAGE ? image_terms( P.dn, p0, facets, moments, Av.x, V3{ M.gx[ f ], M.gy[ f ], M.gz[ f ] },
+                                                V3{ M.px[ v ], M.py[ v ], M.pz[ v ] }, V3{ M.px[ w ], M.py[ w ], M.pz[ w ] },
+                                                V3{ M.ux[ f ], M.uy[ f ], M.uz[ f ] } )
+                                 : edge_terms<true>( P.dn, p0, facets, moments, M.fid[ f ] < 0, Av, V3{ M.gx[ f ], M.gy[ f ], M.gz[ f ] },
                                              V3{ M.px[ v ], M.py[ v ], M.pz[ v ] }, V3{ M.px[ w ], M.py[ w ], M.pz[ w ] },
                                              V3{ M.ux[ f ], M.uy[ f ], M.uz[ f ] } );
         M.tpart[ t ] = d.part;
@@ -951,8 +1097,9 @@ __global__ __launch_bounds__( 64 ) void k_cells3_pass2( Params3 P ) {
         CellSums S;
         for ( int t = lane; t < T; t += 64 ) {
             const int f = C.tf[ t ], v = C.tv[ t ], w = C.tw[ t ];
-            const CellSums d = edge_terms<true>( P.dn, p0, facets, moments, C.fid[ f ] < 0, Av, V3{ C.X[ f ], C.Y[ f ], C.Z[ f ] },
-                                                 V3{ C.x[ v ], C.y[ v ], C.z[ v ] }, V3{ C.x[ w ], C.y[ w ], C.z[ w ] }, V3{ C.fx[ f ], C.fy[ f ], C.fz[ f ] } );
+            const V3 gf{ C.X[ f ], C.Y[ f ], C.Z[ f ] }, vv{ C.x[ v ], C.y[ v ], C.z[ v ] }, ww{ C.x[ w ], C.y[ w ], C.z[ w ] }, uf{ C.fx[ f ], C.fy[ f ], C.fz[ f ] };
+            const CellSums d = P.dn.kind == 1 ? image_terms( P.dn, p0, facets, moments, Av.x, gf, vv, ww, uf )
+                                              : edge_terms<true>( P.dn, p0, facets, moments, C.fid[ f ] < 0, Av, gf, vv, ww, uf );
             S.mass += d.mass; S.mx += d.mx; S.my += d.my; S.mz += d.mz; S.m2 += d.m2;
             C.tp[ t ] = d.part;
         }
@@ -1010,7 +1157,7 @@ void run_cells_3d( const DevTree3 &t, const double *lo, const double *hi, const 
     check_launch( "k_reset_counters3" );
     Timing::Pending tp;
     const bool timed = timing().begin( TK_CELLS, st, tp );
-    const bool split = dn.kind == 2;                     // ( a sum of Gaussians: the measures in a kernel of their own )
+    const bool split = dn.kind == 2 || dn.kind == 1;     // ( Gaussians or an image: the measures in a kernel of their own )
     if ( split ) {
         Arena &A = arena();
         const size_t m = size_t( t.n ) * 64;
@@ -1021,7 +1168,8 @@ void run_cells_3d( const DevTree3 &t, const double *lo, const double *hi, const 
         P.pb.nv = A.get<int>( "cells3.nv", size_t( t.n ) ); P.pb.nf = A.get<int>( "cells3.nf", size_t( t.n ) );
         k_cells3<true><<<blocks_for( t.n, WAVES3 ), 64 * WAVES3, 0, st>>>( P );
         check_launch( "k_cells3" );
-        k_cells3_measure<<<blocks_for( t.n, WAVES3 ), 64 * WAVES3, 0, st>>>( P );
+        if ( dn.kind == 1 ) k_cells3_measure<true><<<blocks_for( t.n, WAVES3 ), 64 * WAVES3, 0, st>>>( P );
+        else                k_cells3_measure<false><<<blocks_for( t.n, WAVES3 ), 64 * WAVES3, 0, st>>>( P );
         check_launch( "k_cells3_measure" );
     } else {
         k_cells3<false><<<blocks_for( t.n, WAVES3 ), 64 * WAVES3, 0, st>>>( P );

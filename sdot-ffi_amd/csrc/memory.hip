@@ -39,7 +39,7 @@ void solve_parts( I64 n, int dim, int density_kind, const double *o, int precisi
     if ( limits ) parts[ 2 ] = 2 * ( EDGE_CAP * 4 * m + 4 * m );
     // the FP32 kernel's polygons ( 2D, 'fp32' and 'mixed' ); 3D with a sum of Gaussians: each cell's polytope
     if ( ! d3 && precision != 0 ) parts[ 3 ] = CAPW * 12 * m + 4 * m + 8 * m;   // ( and the FP64 redo's marks and list )
-    if ( d3 && density_kind == SDOT_AMD_GAUSSIANS ) parts[ 3 ] = 64 * m * ( 3 * 8 + 2 * 4 + 3 * 8 + 4 ) + 8 * m;
+    if ( d3 && ( density_kind == SDOT_AMD_GAUSSIANS || density_kind == SDOT_AMD_IMAGE ) ) parts[ 3 ] = 64 * m * ( 3 * 8 + 2 * 4 + 3 * 8 + 4 ) + 8 * m;
     // the third pass's vertex slots ( a fixed pool )
     parts[ 4 ] = d3 ? 0 : double( OVF_SLOTS ) * CAP3 * 20;
     // Newton's vectors ( targets, weights and trial weights, direction, right-hand side, staging ), the moments' diagram

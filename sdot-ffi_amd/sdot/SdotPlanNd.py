@@ -118,7 +118,7 @@ def card_options( settings, density, verbose = False, nb_dups = 0, gap = 0.0 ):
     put( "agg_gap", gap ); put( "agg_nb_dups", nb_dups )
     if settings.aggregate:
         put( "agg_margin", _AGGREGATE_MARGIN if tun.aggregation_margin is None else tun.aggregation_margin )
-    if isinstance( density, Image ):
+    if isinstance( density, Image ) and density.nb_dims == 2:     # ( a 3D image goes by its own arguments, no continuation )
         lo, hi = density.box()
         h = np.diag( density.frame )
         nx, ny = density.shape
@@ -165,9 +165,10 @@ class SdotPlanNd:
             w0, self._warm_start = None, "none"
         problem.domain
         lo, hi = dst.box()
-        if d == 3:                                       # ( a constant or a sum of Gaussians; the step by trials, no duplicates )
-            if not isinstance( dst, ( UniformBox, SumOfGaussians ) ):
-                raise NotImplementedError( f"SdotPlanNd on the MI355X path, 3D: a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
+        if d == 3:                                       # ( a constant, a sum of Gaussians or a voxel image; the step by trials, no duplicates )
+            if not isinstance( dst, ( UniformBox, SumOfGaussians, Image ) ):
+                raise NotImplementedError( f"SdotPlanNd on the MI355X path, 3D: a UniformBox, a SumOfGaussians or an Image target "
+                                           f"( not { type( dst ).__name__ } )" )
             if settings.aggregate:
                 dup, rep = exact_duplicates( src.positions )
                 if len( dup ):

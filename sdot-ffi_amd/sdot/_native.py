@@ -34,7 +34,8 @@ EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_byte
             "sdot_amd_solve_2d_precision", "sdot_amd_graph_counts", "sdot_amd_solve_bytes", "sdot_amd_server_submit_nd", "sdot_amd_server_info",
             "sdot_amd_measures_2d_noalloc", "sdot_amd_measures_2d_noalloc_max_seeds", "sdot_amd_fp32_counts", "sdot_amd_mg32_counts",
             "sdot_amd_measures_2d_precision", "sdot_amd_moments_2d", "sdot_amd_moments_3d", "sdot_amd_cells_2d",
-            "sdot_amd_measures_vjp_2d", "sdot_amd_solve_2d_batch_precision", "sdot_amd_measures_2d_domain", "sdot_amd_solve_2d_domain" ]
+            "sdot_amd_measures_vjp_2d", "sdot_amd_solve_2d_batch_precision", "sdot_amd_measures_2d_domain", "sdot_amd_solve_2d_domain",
+            "sdot_amd_measures_3d_image", "sdot_amd_solve_3d_image", "sdot_amd_moments_3d_image" ]
 
 CONSTANT, IMAGE, GAUSSIANS = 0, 1, 2
 
@@ -73,6 +74,12 @@ def lib():
         L.sdot_amd_measures_2d_precision.restype = _I
         L.sdot_amd_measures_2d_domain.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P, _P, _LL, _P, _P, _I, _I, _P ]
         L.sdot_amd_measures_2d_domain.restype = _I
+        L.sdot_amd_measures_3d_image.argtypes = [ _P, _LL, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _LL, _P ]
+        L.sdot_amd_measures_3d_image.restype = _I
+        L.sdot_amd_solve_3d_image.argtypes = [ _P, _LL, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_solve_3d_image.restype = _I
+        L.sdot_amd_moments_3d_image.argtypes = [ _P, _LL, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P ]
+        L.sdot_amd_moments_3d_image.restype = _I
         L.sdot_amd_moments_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P ]
         L.sdot_amd_moments_2d.restype = _I
         L.sdot_amd_moments_3d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _P, _P, _P ]
@@ -241,7 +248,14 @@ def moments( positions, weights, box, density ):
     s2 = T.empty( n, dtype = T.float64, device = device() )
     if dim == 3:
         kind, rho, g, ng = _density3_args( density )
-        _check( lib().sdot_amd_moments_3d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( g ), ng, _ptr( m ), _ptr( f ), _ptr( s2 ) ) )
+        if kind == IMAGE:
+            nx, ny, nz, geo = ng
+            geoh, geop = _host( geo )
+            _check( lib().sdot_amd_moments_3d_image( _stream(), n, _ptr( pos ), _ptr( w ), boxp, _ptr( g ), nx, ny, nz, geop, _ptr( m ),
+                                                     _ptr( f ), _ptr( s2 ) ) )
+        else:
+            _check( lib().sdot_amd_moments_3d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( g ), ng, _ptr( m ), _ptr( f ),
+                                               _ptr( s2 ) ) )
     else:
         kind, rho, img, nx, ny, geo = density.native_args( device_tensor = on_device )
         geoh, geop = _host( geo )
@@ -304,7 +318,8 @@ def measures_2d_noalloc( positions, weights, box, density, masses = None ):
 
 def _density3_args( density ):
     """`( kind, rho, gaussians device tensor [ ng, 5 ] or None, ng )` of a 3D density: `None` / a float ( a constant ), an
-    object with `native_args3`, or `[ ng, 5 ]` rows ( centre x, y, z, width, mass )"""
+    object with `native_args3`, or `[ ng, 5 ]` rows ( centre x, y, z, width, mass ). A voxel image gives `( IMAGE, 0, voxels,
+    ( nx, ny, nz, geo ) )` ( `geo`: origin x, y, z and the voxel's sides )"""
     if density is None or np.isscalar( density ):
         return CONSTANT, 1.0 if density is None else float( density ), None, 0
     if hasattr( density, "native_args3" ):
@@ -328,9 +343,15 @@ def measures_3d( positions, weights, box, density = None, facets = False, events
     fc = T.empty( cap, dtype = T.float64, device = device() ) if facets else None
     nbf = np.zeros( 1, np.int64 )
     ev = np.zeros( 5, np.uint64 )
-    _check( lib().sdot_amd_measures_3d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( g ), ng, _ptr( masses ),
-                                        _ptr( fi ), _ptr( fj ), _ptr( fc ), cap, nbf.ctypes.data_as( _P ),
-                                        ev.ctypes.data_as( _P ) if events else None ) )
+    if kind == IMAGE:
+        nx, ny, nz, geo = ng
+        geoh, geop = _host( geo )
+        _check( lib().sdot_amd_measures_3d_image( _stream(), n, _ptr( pos ), _ptr( w ), boxp, _ptr( g ), nx, ny, nz, geop, _ptr( masses ),
+                                                  _ptr( fi ), _ptr( fj ), _ptr( fc ), cap, nbf.ctypes.data_as( _P ) ) )
+    else:
+        _check( lib().sdot_amd_measures_3d( _stream(), n, _ptr( pos ), _ptr( w ), boxp, kind, rho, _ptr( g ), ng, _ptr( masses ),
+                                            _ptr( fi ), _ptr( fj ), _ptr( fc ), cap, nbf.ctypes.data_as( _P ),
+                                            ev.ctypes.data_as( _P ) if events else None ) )
     res = [ masses ]
     if facets:
         k = int( nbf[ 0 ] )
@@ -359,9 +380,16 @@ def solve_3d( positions, nu, w0, options, box, density = None ):
     nb_steps = np.zeros( 1, np.int64 )
     stats = np.zeros( 31 )
     cost = np.zeros( 1 )
-    _check( lib().sdot_amd_solve_3d( _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), optp, boxp, kind, rho, _ptr( g ), ng,
-                                     _ptr( weights ), hist.ctypes.data_as( _P ), nb_steps.ctypes.data_as( _P ), stats.ctypes.data_as( _P ),
-                                     _ptr( masses ), _ptr( bary ), cost.ctypes.data_as( _P ) ) )
+    if kind == IMAGE:
+        nx, ny, nz, geo = ng
+        geoh, geop = _host( geo )
+        _check( lib().sdot_amd_solve_3d_image( _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), optp, boxp, _ptr( g ), nx, ny, nz, geop,
+                                               _ptr( weights ), hist.ctypes.data_as( _P ), nb_steps.ctypes.data_as( _P ), stats.ctypes.data_as( _P ),
+                                               _ptr( masses ), _ptr( bary ), cost.ctypes.data_as( _P ) ) )
+    else:
+        _check( lib().sdot_amd_solve_3d( _stream(), n, _ptr( pos ), _ptr( nu ), _ptr( w0 ), optp, boxp, kind, rho, _ptr( g ), ng,
+                                         _ptr( weights ), hist.ctypes.data_as( _P ), nb_steps.ctypes.data_as( _P ), stats.ctypes.data_as( _P ),
+                                         _ptr( masses ), _ptr( bary ), cost.ctypes.data_as( _P ) ) )
     return dict( weights = weights, cell_masses = masses, barycenters = bary, clusters = None, cost = float( cost[ 0 ] ),
                  stats = stats, history = hist[ :int( nb_steps[ 0 ] ) ] )
 

@@ -92,6 +92,17 @@ class Image:
         lo, hi = self.box()
         return np.concatenate( [ np.eye( d ), -np.eye( d ) ] ), np.concatenate( [ hi, -lo ] )
 
+    def native_args3( self, device_tensor ):
+        """`( kind, rho, voxels, ( nx, ny, nz, geo ) )` for the 3D calls: voxel `( i, j, k )` at `( k ny + j ) nx + i`, `geo` the
+        origin and the voxel's sides"""
+        v = _host( self.values ).astype( np.float64 )
+        if v.ndim != 3:
+            raise NotImplementedError( "Image: a 3D diagram takes a 3D image" )
+        nx, ny, nz = v.shape
+        h = np.diag( self.frame )
+        vox = device_tensor( np.ascontiguousarray( v.transpose( 2, 1, 0 ) ).reshape( -1 ) )
+        return 1, 0.0, vox, ( nx, ny, nz, [ *self.origin[ :3 ], *h[ :3 ] ] )
+
     def native_args( self, device_tensor ):
         """`( kind, rho, img, nx, ny, geo )` for the C ABI: `img[ j * nx + i ]` ( `PowerDiagram_Bsp.py:266-268` )"""
         v = _host( self.values ).astype( np.float64 )

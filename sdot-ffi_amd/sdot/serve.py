@@ -98,7 +98,7 @@ class Server:
         if d == 3:
             if settings.precision == "fp32":
                 raise NotImplementedError( "Server: 3D cells are FP64 ( precision 'auto', 'fp64' or 'mixed' )" )
-            if not isinstance( dst, ( UniformBox, SumOfGaussians ) ):
+            if not isinstance( dst, ( UniformBox, SumOfGaussians ) ):     # ( a voxel image: `SdotPlanNd`, one solve per call )
                 raise NotImplementedError( f"Server, 3D: a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
             box = np.asarray( [ *lo[ :3 ], *hi[ :3 ] ], dtype = np.float64 )
             opts[ _native.CARD_OPTIONS.index( "step" ) ] = 0.0

@@ -469,6 +469,29 @@ def test_frozen_image_mass_is_the_clipped_mass( oracle ):
         assert np.abs( fm - ref ).max() <= 1e-13 * ref.max(), t
 
 
+def test_an_image_is_zero_outside_its_grid( oracle ):
+    # a grid on part of the box ( overhanging it on the left, short of it on the right and at the top ): the pixel clipping
+    # of the measures counts nothing off the grid, the Green walk of the frozen cells ( which start from the box's part
+    # the grid covers ) agrees, and a one-value image is the constant density on that part
+    pos, w, d = _weighted_cloud( 24 )
+    rng = np.random.default_rng( 25 )
+    dens = _img( rng.uniform( 0.5, 2, ( 20, 30 ) ), lo = ( -0.2, 0.1 ), hi = ( 0.7, 0.8 ) )
+    m = oracle.measures( pos, w, density = dens )
+    v, hx, hy = dens[ "values" ], dens[ "hx" ], dens[ "hy" ]
+    fx = np.clip( ( np.minimum( -0.2 + hx * ( np.arange( 30 ) + 1 ), 1 ) - np.maximum( -0.2 + hx * np.arange( 30 ), 0 ) ) / hx, 0, 1 )
+    assert abs( m.sum() - ( v * fx[ None, : ] ).sum() * hx * hy ) < 1e-13
+    for t in ( 0.0, 1e-3 ):
+        fm = oracle.along( pos, w, d, t = t, density = dens )[ "frozen_mass" ]
+        ref = oracle.measures( pos, w + t * d, density = dens )
+        ok = np.isfinite( fm )                           # ( NaN: no frozen cell, the seed's cell being empty )
+        assert np.all( ok | ( ref == 0 ) ) and ok.sum() > 100
+        assert np.abs( fm[ ok ] - ref[ ok ] ).max() <= 1e-13 * ref.max(), t
+    one = _img( np.full( ( 20, 30 ), 1.5 ), lo = ( -0.2, 0.1 ), hi = ( 0.7, 0.8 ) )
+    a = oracle.measures( pos, w, density = one )
+    b = oracle.measures( pos, w, lo = ( 0, 0.1 ), hi = ( 0.7, 0.8 ), density = 1.5 )
+    assert np.abs( a - b ).max() < 1e-15
+
+
 def test_forward_limit_is_the_first_root( oracle ):
     pos, w, d = _weighted_cloud( 23, spread = 2e-5 )
     a = oracle.measures( pos, w )

@@ -278,3 +278,132 @@ def test_a_grid_is_solved_at_the_start():
     assert r[ "stats" ][ "status" ] == 1 and r[ "stats" ][ "nb_iter" ] == 0
     assert np.allclose( r[ "barycenters" ], pos, atol = 1e-14 )
     assert abs( r[ "cost" ] - 3 * ( 0.2 ** 2 / 12 ) ) < 1e-15      # each cube of side h: int |x - c|^2 = h^5 / 4, summed
+
+
+# ---- voxel images --------------------------------------------------------------------------------------------------
+
+def voxel_image( rng, shape = ( 4, 4, 5 ), origin = ( -0.1, 0.05, 0.0 ), h = ( 0.3, 0.25, 0.2 ) ):
+    """random voxels on a grid that overhangs the unit box along x, leaves a strip of it empty along y and fits it along z"""
+    return dict( voxels = rng.uniform( 0.5, 2.0, shape ), x0 = origin[ 0 ], y0 = origin[ 1 ], z0 = origin[ 2 ], hx = h[ 0 ], hy = h[ 1 ],
+                 hz = h[ 2 ] )
+
+
+def polytope_piece( hs, lo, hi ):
+    """`( volume, centroid )` of the scipy half-spaces `hs` cut by the block `[ lo, hi ]`, or None when it is ( nearly ) empty:
+    the interior point is the Chebyshev centre ( a linear program ), the volume a fan of tetrahedra from it"""
+    from scipy.optimize import linprog
+    A = np.concatenate( [ hs ] + [ np.array( [ [ *( -np.eye( 3 )[ ax ] ), lo[ ax ] ], [ *np.eye( 3 )[ ax ], -hi[ ax ] ] ] ) for ax in range( 3 ) ] )
+    nrm = np.linalg.norm( A[ :, :3 ], axis = 1 )
+    lp = linprog( [ 0, 0, 0, -1 ], A_ub = np.column_stack( [ A[ :, :3 ], nrm ] ), b_ub = -A[ :, 3 ], bounds = [ ( None, None ) ] * 3 + [ ( 0, None ) ] )
+    if lp.status != 0 or lp.x[ 3 ] < 1e-9:
+        return None
+    pts = HalfspaceIntersection( A, lp.x[ :3 ] ).intersections
+    hull = ConvexHull( pts )
+    c = pts.mean( 0 )
+    vol, mom = 0.0, np.zeros( 3 )
+    for s in hull.simplices:
+        a, b, d = pts[ s ]
+        v = abs( np.linalg.det( np.array( [ a - c, b - c, d - c ] ) ) ) / 6
+        vol += v
+        mom += v * ( a + b + d + c ) / 4
+    return vol, mom / vol
+
+
+def image_integrals( pos, w, i, img ):
+    """`( mass, first moment )` of cell `i` against the voxel image, voxel by voxel with scipy"""
+    hs, _ = halfspaces_of( pos, w, i )
+    xyz, _, _ = O.cell_3d( pos, i, w )
+    v = img[ "voxels" ]
+    o = np.array( [ img[ "x0" ], img[ "y0" ], img[ "z0" ] ] )
+    h = np.array( [ img[ "hx" ], img[ "hy" ], img[ "hz" ] ] )
+    a = np.clip( np.floor( ( xyz.min( 0 ) - o ) / h ).astype( int ), 0, np.array( v.shape ) - 1 )
+    b = np.clip( np.floor( ( xyz.max( 0 ) - o ) / h ).astype( int ), 0, np.array( v.shape ) - 1 )
+    mass, first = 0.0, np.zeros( 3 )
+    for ijk in itertools.product( *[ range( a[ d ], b[ d ] + 1 ) for d in range( 3 ) ] ):
+        lo = o + h * np.array( ijk )
+        p = polytope_piece( hs, lo, lo + h )
+        if p is not None:
+            mass += v[ ijk ] * p[ 0 ]
+            first += v[ ijk ] * p[ 0 ] * p[ 1 ]
+    return mass, first
+
+
+def test_a_constant_image_is_the_constant_density():
+    rng = np.random.default_rng( 21 )
+    n = 80
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    w = rng.normal( 0, 1e-3, n )
+    img = dict( voxels = np.full( ( 3, 5, 4 ), 1.7 ), x0 = 0.0, y0 = 0.0, z0 = 0.0, hx = 1 / 3, hy = 0.2, hz = 0.25 )
+    m, ( fi, fj, fc ) = O.measures_3d( pos, w, density = img, facets = True )
+    mc, ( gi, gj, gc ) = O.measures_3d( pos, w, density = 1.7, facets = True )
+    assert np.allclose( m, mc, rtol = 1e-13, atol = 0 )
+    A = { ( i, j ): c for i, j, c in zip( fi, fj, fc ) }
+    B = { ( i, j ): c for i, j, c in zip( gi, gj, gc ) }
+    assert set( A ) == set( B )
+    assert max( abs( A[ k ] - B[ k ] ) / abs( B[ k ] ) for k in B ) < 1e-12
+
+
+def test_image_masses_match_scipy_voxel_by_voxel():
+    rng = np.random.default_rng( 22 )
+    n = 24
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    w = rng.uniform( 0, 2e-3, n )
+    img = voxel_image( rng )
+    m = O.measures_3d( pos, w, density = img )
+    # the image's mass inside the box: x from 0 to 1 of the grid's -0.1 to 1.1, y from 0.05 to 1 ( zero below: the image is
+    # zero outside its grid ), z all of it
+    v = img[ "voxels" ]
+    fx = np.array( [ 0.2, 0.3, 0.3, 0.2 ] ) / 0.3          # ( the share of each column and row inside the box )
+    fy = np.array( [ 0.25, 0.25, 0.25, 0.2 ] ) / 0.25
+    inside = np.einsum( "ijk,i,j->", v, fx, fy ) * 0.3 * 0.25 * 0.2
+    assert abs( m.sum() - inside ) < 1e-13 * inside
+    for i in range( n ):
+        mass, _ = image_integrals( pos, w, i, img )
+        assert abs( m[ i ] - mass ) < 1e-11 * max( mass, 1e-3 ), ( i, m[ i ], mass )
+
+
+def test_image_masses_are_linear_in_the_image():
+    rng = np.random.default_rng( 23 )
+    pos = rng.uniform( 0, 1, ( 150, 3 ) )
+    a, b = voxel_image( rng ), voxel_image( rng )
+    ab = dict( a, voxels = 2.0 * a[ "voxels" ] - 0.5 * b[ "voxels" ] )
+    ma, mb, mab = ( O.measures_3d( pos, density = d ) for d in ( a, b, ab ) )
+    assert np.allclose( mab, 2.0 * ma - 0.5 * mb, rtol = 0, atol = 1e-15 * np.abs( ma ).max() * 10 )
+
+
+def test_the_image_laplacian_is_the_derivative_of_the_masses():
+    rng = np.random.default_rng( 24 )
+    n = 100
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    w = rng.normal( 0, 1e-3, n )
+    img = voxel_image( rng )
+    m, ( fi, fj, fc ) = O.measures_3d( pos, w, density = img, facets = True )
+    L = np.zeros( ( n, n ) )
+    for i, j, c in zip( fi, fj, fc ):
+        L[ i, j ] += c
+    assert np.allclose( L, L.T, rtol = 1e-9, atol = 1e-13 )
+    h = 1e-6
+    for j in range( 0, n, 9 ):
+        wp, wm = w.copy(), w.copy()
+        wp[ j ] += h
+        wm[ j ] -= h
+        fd = ( O.measures_3d( pos, wp, density = img ) - O.measures_3d( pos, wm, density = img ) ) / ( 2 * h )
+        expect = -L[ :, j ]
+        expect[ j ] = L[ j ].sum()
+        assert np.allclose( fd, expect, rtol = 0, atol = 2e-5 * np.abs( expect ).max() ), j
+
+
+def test_the_3d_image_solve_reaches_the_tolerance_and_its_barycentres_are_the_voxel_integrals():
+    rng = np.random.default_rng( 25 )
+    n = 300
+    pos = rng.uniform( 0, 1, ( n, 3 ) )
+    pos[ :, 1 ] = 0.05 + 0.95 * pos[ :, 1 ]               # ( no seed in the empty strip )
+    img = voxel_image( rng )
+    r = O.solve_3d( pos, density = img, tol = 1e-12 )
+    st = r[ "stats" ]
+    assert st[ "status" ] == 1, st
+    assert np.abs( r[ "masses" ] - st[ "domain_mass" ] / n ).max() <= 1e-12
+    for i in range( 0, n, 23 ):
+        mass, first = image_integrals( pos, r[ "weights" ], i, img )
+        assert abs( r[ "masses" ][ i ] - mass ) < 1e-10 * mass
+        assert np.allclose( r[ "barycenters" ][ i ], first / mass, rtol = 0, atol = 1e-10 ), i
