@@ -62,6 +62,16 @@ def bump_image( res, centres, sigmas, floor = 0.1 ):
     return v
 
 
+def bump_volume( res, centres, sigmas, floor = 0.1 ):
+    """`bump_image` in 3D: a `res^3` voxel image of the unit cube, first index along x"""
+    xs = ( np.arange( res ) + 0.5 ) / res
+    X, Y, Z = np.meshgrid( xs, xs, xs, indexing = "ij" )
+    v = np.full( ( res, res, res ), floor )
+    for c, s in zip( centres, sigmas ):
+        v += np.exp( -( ( X - c[ 0 ] ) ** 2 + ( Y - c[ 1 ] ) ** 2 + ( Z - c[ 2 ] ) ** 2 ) / ( 2 * s ** 2 ) )
+    return v
+
+
 def workload_batch( k = 256, res = 256, n_min = 1_000, n_max = 200_000 ):
     """THE BATCHED WORKLOAD: `k` independent 2D instances, Dirac counts log-uniform in [ n_min, n_max ] ( default_rng(5) ), each
     uniform in [0,1]^2 with its own `res^2` image of 4 Gaussian bumps ( sigma uniform in [ 0.03, 0.15 ] ) + a 0.1 floor
@@ -134,12 +144,12 @@ def bench_batch( args ):
     print( json.dumps( line ), flush = True )
 
 
-def workload_serve( k = 40, rate = 1.0, n_min = 1_000, n_max = 500_000, res = 256 ):
+def workload_serve( k = 40, rate = 1.0, n_min = 1_000, n_max = 500_000, res = 256, res3 = 64 ):
     """THE SERVED WORKLOAD: `k` requests, Poisson arrivals at `rate` per second ( exponential gaps, default_rng(6) ), 70 % 2D /
     30 % 3D, Dirac counts log-uniform in [ n_min, n_max ]; a 2D request is uniform in [0,1]^2 with a `res^2` image of 4 Gaussian
-    bumps + 0.1 floor, a 3D one uniform in [0,1]^3 with 4 Gaussians ( sigma 0.1, means uniform in [0.2,0.8]^3, masses
-    Dirichlet(1) ) truncated to the unit cube ( default_rng( [ 6, q ] ) for request q ). Returns `[ ( arrival, dim, positions,
-    density parameters ) ]`"""
+    bumps + 0.1 floor, a 3D one uniform in [0,1]^3 with a `res3^3` voxel image of 4 bumps ( sigma uniform in [ 0.05, 0.15 ],
+    centres uniform in [0.2,0.8]^3 ) + 0.1 floor ( default_rng( [ 6, q ] ) for request q ). Returns `[ ( arrival, dim,
+    positions, image ) ]`"""
     rng = np.random.default_rng( 6 )
     gaps = rng.exponential( 1.0 / rate, k )
     arrivals = np.cumsum( gaps ) - gaps[ 0 ]
@@ -153,7 +163,7 @@ def workload_serve( k = 40, rate = 1.0, n_min = 1_000, n_max = 500_000, res = 25
         if d == 2:
             dens = bump_image( res, r.uniform( 0, 1, ( 4, 2 ) ), r.uniform( 0.03, 0.15, 4 ) )
         else:
-            dens = np.column_stack( [ r.uniform( 0.2, 0.8, ( 4, 3 ) ), np.full( 4, 0.1 ), r.dirichlet( np.ones( 4 ) ) ] )
+            dens = bump_volume( res3, r.uniform( 0.2, 0.8, ( 4, 3 ) ), r.uniform( 0.05, 0.15, 4 ) )
         out.append( ( float( arrivals[ q ] ), d, pos, dens ) )
     return out
 
@@ -174,8 +184,7 @@ def bench_serve( args ):
         p = torch.as_tensor( pos, device = dev )
         if d == 2:
             return sdot.OtProblem( sdot.SumOfDiracs( p ), sdot.Image( dens, origin = [ 0, 0 ], frame = np.eye( 2 ) / args.batch_res ) )
-        return sdot.OtProblem( sdot.SumOfDiracs( p ), sdot.SumOfGaussians( dens[ :, :3 ], dens[ :, 3 ], dens[ :, 4 ],
-                                                                          support_box = ( [ 0, 0, 0 ], [ 1, 1, 1 ] ) ) )
+        return sdot.OtProblem( sdot.SumOfDiracs( p ), sdot.Image( dens, origin = [ 0, 0, 0 ], frame = np.eye( 3 ) / dens.shape[ 0 ] ) )
 
     problems = [ problem( d, pos, dens ) for _, d, pos, dens in reqs ]
     total = int( sum( len( pos ) for _, _, pos, _ in reqs ) )
@@ -203,7 +212,7 @@ def bench_serve( args ):
     line = dict( metric = METRIC, value = total / elapsed, unit = "Diracs/s", n_gpus = 1, steps = 1, warmup = 1,
                  ms_per_step = 1e3 * elapsed, higher_is_better = True, scaling = "weak", vs_baseline = None, dtype = "f32+f64", data = "synthetic",
                  config = dict( workload = f"{ len( reqs ) } served solves, Poisson arrivals at { args.serve_rate }/s, 70 % 2D ( { args.batch_res }^2 "
-                                           f"image of 4 bumps + 0.1 floor ) / 30 % 3D ( 4 Gaussians sigma 0.1 in place of a 64^3 image ), Dirac "
+                                           f"image of 4 bumps + 0.1 floor ) / 30 % 3D ( 64^3 voxel image of 4 bumps + 0.1 floor ), Dirac "
                                            f"counts log-uniform in 1k-500k ( { total } in all ), max mass error <= { args.tol:g} ( absolute )",
                                 requests = len( reqs ), diracs = total, rate_per_s = args.serve_rate, slots = args.streams,
                                 precision_2d = args.precision, cells_3d = "fp64" ),

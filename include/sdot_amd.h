@@ -339,6 +339,14 @@ int sdot_amd_server_submit_nd( long long server, int dim, int precision, long lo
                                const double *density, int nb_density, double *weights, double *cell_masses, double *barycenters,
                                double *stats, double *cost, long long *ticket );
 
+/* A 3D REQUEST AGAINST A VOXEL IMAGE: `sdot_amd_server_submit_nd`'s 3D request with the image of `sdot_amd_measures_3d_image`
+ * ( `img` DEVICE [ nz ][ ny ][ nx ], `img_geo` HOST [ 6 ] ) as the density; FP64 cells. No counterpart in the reference ( it has
+ * no server ): the served 3D share's density. */
+int sdot_amd_server_submit_3d_image( long long server, long long n, const double *positions, const double *nu, const double *w0,
+                                     const double *options, const double *box, const double *img, int nx, int ny, int nz,
+                                     const double *img_geo, double *weights, double *cell_masses, double *barycenters, double *stats,
+                                     double *cost, long long *ticket );
+
 /* What the server's slots held: `info` HOST [ 2 ]: the largest sum of the slots' device bytes seen at the end of a request,
  * and the requests done. */
 int sdot_amd_server_info( long long server, double *info );

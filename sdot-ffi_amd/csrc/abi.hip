@@ -647,10 +647,12 @@ long long sdot_amd_server_start( int nb_slots, double memory_budget ) {
     return rc == 0 ? h : 0;
 }
 
-int sdot_amd_server_submit_nd( long long server, int dim, int precision, long long n, const double *positions, const double *nu,
-                               const double *w0, const double *options, const double *box, int density_kind, double rho,
-                               const double *density, int nb_density, double *weights, double *cell_masses, double *barycenters,
-                               double *stats, double *cost, long long *ticket ) {
+/// a request to the server; `image3`: a 3D voxel image ( `density` its voxels, `image3[ 0 .. 2 ]` the counts, `image3_geo` the
+/// origin and the sides ) in place of `density_kind`
+static int server_submit( long long server, int dim, int precision, long long n, const double *positions, const double *nu, const double *w0,
+                          const double *options, const double *box, int density_kind, double rho, const double *density, int nb_density,
+                          const int *image3, const double *image3_geo, double *weights, double *cell_masses, double *barycenters,
+                          double *stats, double *cost, long long *ticket ) {
     return guarded( [&] {
         SDOT_CHECK( n >= 2 && positions && nu && options && box && weights && cell_masses && barycenters && stats && cost && ticket,
                     "server: a required array is null" );
@@ -670,8 +672,11 @@ int sdot_amd_server_submit_nd( long long server, int dim, int precision, long lo
         J.in.dim = dim;
         J.in.precision = dim == 3 ? SDOT_AMD_FP64 : precision;
         if ( dim == 3 ) {
-            for ( int c = 0; c < 6; ++c ) J.in.box3[ c ] = box[ c ];
-            J.in.dens3 = make_density3( density_kind, rho, density, nb_density );
+            J.in.dens3 = image3 ? make_image3( density, image3[ 0 ], image3[ 1 ], image3[ 2 ], image3_geo )
+                                : make_density3( density_kind, rho, density, nb_density );
+            double sb[ 6 ];
+            const double *b = support_box3( J.in.dens3, box, sb );
+            for ( int c = 0; c < 6; ++c ) J.in.box3[ c ] = b[ c ];
             r->opts[ O_STEP ] = 0;                       // ( 3D steps by trials )
         } else {
             for ( int c = 0; c < 4; ++c ) J.in.box[ c ] = box[ c ];
@@ -683,9 +688,26 @@ int sdot_amd_server_submit_nd( long long server, int dim, int precision, long lo
         }
         J.out.weights = weights; J.out.cell_masses = cell_masses; J.out.barycenters = barycenters; J.out.stats = stats; J.out.cost = cost;
         // the admission's footprint: the byte model of the solve, part by part
-        r->bytes = solve_bytes( n, dim, density_kind, r->opts.data(), J.in.precision );
+        r->bytes = solve_bytes( n, dim, image3 ? SDOT_AMD_IMAGE : density_kind, r->opts.data(), J.in.precision );
         *ticket = S->submit( std::move( r ) );
     } );
+}
+
+int sdot_amd_server_submit_nd( long long server, int dim, int precision, long long n, const double *positions, const double *nu,
+                               const double *w0, const double *options, const double *box, int density_kind, double rho,
+                               const double *density, int nb_density, double *weights, double *cell_masses, double *barycenters,
+                               double *stats, double *cost, long long *ticket ) {
+    return server_submit( server, dim, precision, n, positions, nu, w0, options, box, density_kind, rho, density, nb_density, nullptr, nullptr,
+                          weights, cell_masses, barycenters, stats, cost, ticket );
+}
+
+int sdot_amd_server_submit_3d_image( long long server, long long n, const double *positions, const double *nu, const double *w0,
+                                     const double *options, const double *box, const double *img, int nx, int ny, int nz,
+                                     const double *img_geo, double *weights, double *cell_masses, double *barycenters, double *stats,
+                                     double *cost, long long *ticket ) {
+    const int counts[ 3 ] = { nx, ny, nz };
+    return server_submit( server, 3, SDOT_AMD_FP64, n, positions, nu, w0, options, box, SDOT_AMD_IMAGE, 0.0, img, 0, counts, img_geo, weights,
+                          cell_masses, barycenters, stats, cost, ticket );
 }
 
 int sdot_amd_server_submit( long long server, long long n, const double *positions, const double *nu, const double *w0,

@@ -17,6 +17,15 @@ struct Polygon2 {
     double nx[ DOMAIN_MAX ] = {}, ny[ DOMAIN_MAX ] = {}, off[ DOMAIN_MAX ] = {};
 };
 
+/// entry `j` of one of a `Polygon2`'s arrays, read with constant indices only: the polygon travels in the kernels'
+/// arguments, and an index from data into them makes every lane copy the arguments to scratch memory first
+__device__ __forceinline__ double dom_at( const double ( &a )[ DOMAIN_MAX ], int j ) {
+    double v = 0;
+#pragma unroll
+    for ( int q = 0; q < DOMAIN_MAX; ++q ) v = q == j ? a[ q ] : v;
+    return v;
+}
+
 struct Density {
     int           kind = 0;        ///< 0 constant, 1 image, 2 a sum of isotropic Gaussians
     double        rho = 1;

@@ -678,10 +678,10 @@ __device__ __forceinline__ void plane_of( const Params &P, int id, double p0x, d
     if ( id <= -5 ) {                                    // an edge of the polygon domain
         const Polygon2 &D = P.dn.dom;
         const int j = -5 - id;
-        nx = D.nx[ j ];
-        ny = D.ny[ j ];
+        nx = dom_at( D.nx, j );
+        ny = dom_at( D.ny, j );
         e = 1.0;
-        off = D.off[ j ] - nx * p0x - ny * p0y;
+        off = dom_at( D.off, j ) - nx * p0x - ny * p0y;
         return;
     }
     const int f = -1 - id;

@@ -32,9 +32,9 @@ __device__ __forceinline__ void line_along( const Along &al, I64 k, int c, doubl
     if ( c <= -5 ) {                                     // an edge of the polygon domain: it does not move either
         const Polygon2 &D = al.dn.dom;
         const int j = -5 - c;
-        nx = D.nx[ j ];
-        ny = D.ny[ j ];
-        off = D.off[ j ] - nx * px - ny * py;
+        nx = dom_at( D.nx, j );
+        ny = dom_at( D.ny, j );
+        off = dom_at( D.off, j ) - nx * px - ny * py;
         delta = 0;
         return;
     }

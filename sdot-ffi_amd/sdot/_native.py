@@ -35,7 +35,7 @@ EXPORTS = [ "sdot_amd_abi_version", "sdot_amd_last_error", "sdot_amd_device_byte
             "sdot_amd_measures_2d_noalloc", "sdot_amd_measures_2d_noalloc_max_seeds", "sdot_amd_fp32_counts", "sdot_amd_mg32_counts",
             "sdot_amd_measures_2d_precision", "sdot_amd_moments_2d", "sdot_amd_moments_3d", "sdot_amd_cells_2d",
             "sdot_amd_measures_vjp_2d", "sdot_amd_solve_2d_batch_precision", "sdot_amd_measures_2d_domain", "sdot_amd_solve_2d_domain",
-            "sdot_amd_measures_3d_image", "sdot_amd_solve_3d_image", "sdot_amd_moments_3d_image" ]
+            "sdot_amd_measures_3d_image", "sdot_amd_solve_3d_image", "sdot_amd_moments_3d_image", "sdot_amd_server_submit_3d_image" ]
 
 CONSTANT, IMAGE, GAUSSIANS = 0, 1, 2
 
@@ -80,6 +80,8 @@ def lib():
         L.sdot_amd_solve_3d_image.restype = _I
         L.sdot_amd_moments_3d_image.argtypes = [ _P, _LL, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P ]
         L.sdot_amd_moments_3d_image.restype = _I
+        L.sdot_amd_server_submit_3d_image.argtypes = [ _LL, _LL, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P ]
+        L.sdot_amd_server_submit_3d_image.restype = _I
         L.sdot_amd_moments_2d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _I, _P, _P, _P, _P ]
         L.sdot_amd_moments_2d.restype = _I
         L.sdot_amd_moments_3d.argtypes = [ _P, _LL, _P, _P, _P, _I, _D, _P, _I, _P, _P, _P ]

@@ -98,8 +98,8 @@ class Server:
         if d == 3:
             if settings.precision == "fp32":
                 raise NotImplementedError( "Server: 3D cells are FP64 ( precision 'auto', 'fp64' or 'mixed' )" )
-            if not isinstance( dst, ( UniformBox, SumOfGaussians ) ):     # ( a voxel image: `SdotPlanNd`, one solve per call )
-                raise NotImplementedError( f"Server, 3D: a UniformBox or a SumOfGaussians target ( not { type( dst ).__name__ } )" )
+            if not isinstance( dst, ( UniformBox, SumOfGaussians, Image ) ):
+                raise NotImplementedError( f"Server, 3D: a UniformBox, a SumOfGaussians or an Image target ( not { type( dst ).__name__ } )" )
             box = np.asarray( [ *lo[ :3 ], *hi[ :3 ] ], dtype = np.float64 )
             opts[ _native.CARD_OPTIONS.index( "step" ) ] = 0.0
             kind, rho, dens, nd = dst.native_args3( device_tensor = _native.on_device )
@@ -117,11 +117,17 @@ class Server:
         T.cuda.current_stream().synchronize()           # ( the inputs are on the device before a slot reads them )
         ticket = ctypes.c_longlong( 0 )
         P = _native._ptr
-        _native._check( _native.lib().sdot_amd_server_submit_nd( self._h, d, precision, n, P( pos ), P( nu ), P( w0 ), opts.ctypes.data_as( _native._P ),
-                                                                 box.ctypes.data_as( _native._P ), kind, rho, P( dens ), int( nd ),
-                                                                 P( keep[ "weights" ] ), P( keep[ "masses" ] ), P( keep[ "bary" ] ),
-                                                                 keep[ "stats" ].ctypes.data_as( _native._P ),
-                                                                 keep[ "cost" ].ctypes.data_as( _native._P ), ctypes.byref( ticket ) ) )
+        outs = ( P( keep[ "weights" ] ), P( keep[ "masses" ] ), P( keep[ "bary" ] ), keep[ "stats" ].ctypes.data_as( _native._P ),
+                 keep[ "cost" ].ctypes.data_as( _native._P ), ctypes.byref( ticket ) )
+        if d == 3 and kind == _native.IMAGE:             # ( a voxel image: its counts and geometry beside the voxels )
+            nx, ny, nz, geo = nd
+            keep[ "geo" ] = np.ascontiguousarray( geo, dtype = np.float64 )
+            _native._check( _native.lib().sdot_amd_server_submit_3d_image( self._h, n, P( pos ), P( nu ), P( w0 ), opts.ctypes.data_as( _native._P ),
+                                                                           box.ctypes.data_as( _native._P ), P( dens ), nx, ny, nz,
+                                                                           keep[ "geo" ].ctypes.data_as( _native._P ), *outs ) )
+        else:
+            _native._check( _native.lib().sdot_amd_server_submit_nd( self._h, d, precision, n, P( pos ), P( nu ), P( w0 ), opts.ctypes.data_as( _native._P ),
+                                                                     box.ctypes.data_as( _native._P ), kind, rho, P( dens ), int( nd ), *outs ) )
         t = Ticket( self, ticket.value, problem, settings, keep )
         self._open.add( t )
         return t

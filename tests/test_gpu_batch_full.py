@@ -4,8 +4,8 @@
   image of 4 Gaussian bumps + 0.1 floor, mixed precision ( the FP32 kernel, FP64 from the first step it cannot take ): every
   plan converges; the oracle's FP64 cells of the returned weights pass the tolerance on a sample of instances; the oracle's own
   solve of the three smallest gives the same weights; an instance solved alone gives the same plan to the bit;
-* the server: a stream of 2D and 3D requests, each plan the one its solve gets alone, and the 2D ones passing the tolerance on
-  the oracle's FP64 cells.
+* the server: a stream of 2D requests ( 128^2 images ) and 3D ones ( 32^3 voxel images ), each plan the one its solve gets
+  alone, and a sample of both passing the tolerance on the oracle's FP64 cells.
 """
 
 import os
@@ -79,7 +79,7 @@ def test_served_requests_match_their_solo_runs_and_the_oracle( native, oracle ):
     import torch
     import sdot
     bench = _bench()
-    reqs = bench.workload_serve( 12, 50.0, n_min = 1_000, n_max = 30_000, res = 128 )
+    reqs = bench.workload_serve( 12, 50.0, n_min = 1_000, n_max = 30_000, res = 128, res3 = 32 )
     dev = torch.device( "cuda", 0 )
     tol = 1e-9
     s2 = sdot.Iterative( tol = tol, continuation = "never", on_failure = "ignore", precision = "mixed" )
@@ -90,8 +90,7 @@ def test_served_requests_match_their_solo_runs_and_the_oracle( native, oracle ):
         if d == 2:
             problems.append( sdot.OtProblem( sdot.SumOfDiracs( p ), sdot.Image( dens, origin = [ 0, 0 ], frame = np.eye( 2 ) / 128 ) ) )
         else:
-            problems.append( sdot.OtProblem( sdot.SumOfDiracs( p ), sdot.SumOfGaussians( dens[ :, :3 ], dens[ :, 3 ], dens[ :, 4 ],
-                                                                                      support_box = ( [ 0, 0, 0 ], [ 1, 1, 1 ] ) ) ) )
+            problems.append( sdot.OtProblem( sdot.SumOfDiracs( p ), sdot.Image( dens, origin = [ 0, 0, 0 ], frame = np.eye( 3 ) / 32 ) ) )
     say( f"case server: { len( reqs ) } requests ( { sum( d == 3 for _, d, _, _ in reqs ) } 3D )" )
     with sdot.Server( slots = 4 ) as srv:
         tickets = [ srv.submit( pb, s2 if r[ 1 ] == 2 else s3 ) for pb, r in zip( problems, reqs ) ]
@@ -104,3 +103,8 @@ def test_served_requests_match_their_solo_runs_and_the_oracle( native, oracle ):
             pos = r[ 2 ]
             mo = oracle.measures( pos, pl.weights.cpu().numpy(), density = _oracle_image( np.asarray( pb.target.values ), 128 ) )
             assert np.abs( mo - mo.sum() / len( pos ) ).max() <= 1.01 * tol, q
+        if r[ 1 ] == 3 and len( r[ 2 ] ) <= 12_000:     # ( the oracle's 3D cells of the served weights )
+            v = np.asarray( pb.target.values )
+            d = dict( voxels = v, x0 = 0.0, y0 = 0.0, z0 = 0.0, hx = 1 / 32, hy = 1 / 32, hz = 1 / 32 )
+            mo = oracle.measures_3d( r[ 2 ], pl.weights.cpu().numpy(), density = d )
+            assert np.abs( mo - mo.sum() / len( r[ 2 ] ) ).max() <= 1.01 * tol, q

@@ -167,3 +167,34 @@ def test_2d_image_is_zero_outside_its_grid( native, oracle ):
     assert r[ "stats" ][ 3 ] == ro[ "stats" ][ "nb_iter" ]
     wg = r[ "weights" ].cpu().numpy()
     assert np.abs( wg - ro[ "weights" ] ).max() < 1e-9 * np.abs( ro[ "weights" ] ).max()
+
+
+def test_served_3d_image_requests_are_single_solves( native, oracle ):
+    """3D voxel-image requests mixed with 2D ones on a server: each plan is the one it gets alone, to the bit, and a sampled
+    3D plan's masses are the oracle's at its weights"""
+    import torch
+    import sdot
+    rng = np.random.default_rng( 6 )
+    probs = []
+    for q in range( 6 ):
+        if q % 2:
+            v = rng.uniform( 0.3, 1.5, ( 24, 24, 24 ) )
+            pos = torch.as_tensor( rng.uniform( 0, 1, ( int( rng.integers( 2000, 9000 ) ), 3 ) ), device = "cuda" )
+            probs.append( sdot.OtProblem( sdot.SumOfDiracs( pos ), sdot.Image( v, origin = [ 0, 0, 0 ], frame = np.eye( 3 ) / 24 ) ) )
+        else:
+            v = rng.uniform( 0.3, 1.5, ( 64, 64 ) )
+            pos = torch.as_tensor( rng.uniform( 0, 1, ( int( rng.integers( 2000, 20000 ) ), 2 ) ), device = "cuda" )
+            probs.append( sdot.OtProblem( sdot.SumOfDiracs( pos ), sdot.Image( v, origin = [ 0, 0 ], frame = np.eye( 2 ) / 64 ) ) )
+    it = sdot.Iterative( precision = "fp64", tol = 1e-10, continuation = "never" )
+    say( "served 3d image requests" )
+    with sdot.Server( slots = 3 ) as srv:
+        plans = [ t.result() for t in [ srv.submit( pb, it ) for pb in probs ] ]
+    for pb, pl in zip( probs, plans ):
+        alone = pb.solve( it )
+        assert pl.converged and alone.converged
+        assert torch.equal( pl.weights, alone.weights ) and torch.equal( pl.cell_masses, alone.cell_masses ) and pl.cost == alone.cost
+    pb, pl = probs[ 1 ], plans[ 1 ]
+    v = pb.target.values
+    d = dict( voxels = v / ( v.sum() / 24 ** 3 ), x0 = 0.0, y0 = 0.0, z0 = 0.0, hx = 1 / 24, hy = 1 / 24, hz = 1 / 24 )
+    mo = oracle.measures_3d( pb.source.positions.cpu().numpy(), pl.weights.cpu().numpy(), density = d )
+    assert np.abs( mo - pl.cell_masses.cpu().numpy() ).max() <= 1e-12 * mo.max()
