@@ -66,7 +66,7 @@ __device__ void main_run( const S &c, int nb, T dx, T dy, T off, int &i1, int &n
     for ( int q = a ? a - 1 : nb - 1; side( c, q, dx, dy, off ) > 0; q = q ? q - 1 : nb - 1 ) i1 = q;
     nb_out = 1;
     for ( int q = a + 1 < nb ? a + 1 : 0; side( c, q, dx, dy, off ) > 0; q = q + 1 < nb ? q + 1 : 0 ) ++nb_out;
-    nb_out += ( a - i1 + nb ) % nb;
+    nb_out += a >= i1 ? a - i1 : a - i1 + nb;           // ( `( a - i1 ) mod nb` without a division )
 }
 
 /// the cut by `dx . v <= off`, in place; `CUT_NO_ROOM` leaves the cell intact
@@ -108,7 +108,8 @@ __device__ int cut( const S &c, int &nb, int cap, T dx, T dy, T off, int pid, Ev
     }
     const int nb_in = nb - nb_out, new_nb = nb_in + 2;
     if ( new_nb > cap ) return CUT_NO_ROOM;
-    const int j0 = ( i1 + nb - 1 ) % nb, j2 = ( i1 + nb_out - 1 ) % nb, j3 = ( j2 + 1 ) % nb;
+    // ( the indices mod `nb` by comparisons: `i1 < nb` and `nb_out <= nb`, so one subtraction at most )
+    const int j0 = i1 ? i1 - 1 : nb - 1, j2 = i1 + nb_out - 1 >= nb ? i1 + nb_out - 1 - nb : i1 + nb_out - 1, j3 = j2 + 1 == nb ? 0 : j2 + 1;
     const T s0 = side( c, j0, dx, dy, off ), s1 = side( c, i1, dx, dy, off );
     const T s2 = side( c, j2, dx, dy, off ), s3 = side( c, j3, dx, dy, off );
     const T ta = s0 / ( s0 - s1 );
@@ -281,8 +282,11 @@ __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *l
     ND rec;                                              // the record of ( m, h ) when the descent loaded it
     bool have = false;
     double4 nxt = make_double4( 0, 0, 0, 0 );            // the open leaf's seed `q`, loaded ahead
+    // two inner loops, "while-while": the open leaf's seeds, then the walk down to the next leaf that may cut. A wave's
+    // lanes run one kind of step at a time ( one loop with both kinds of step in it runs both bodies whenever its lanes
+    // disagree ); each lane takes the same steps in the same order either way
     for ( ;; ) {
-        if ( q < qe ) {
+        while ( q < qe ) {
             const int s = q++;
             const double4 sd = nxt;
             if ( q < qe ) nxt = wk.seeds[ q ];
@@ -292,45 +296,46 @@ __device__ __forceinline__ int make_cell( const Walk &wk, int k, const double *l
             const int res = cut( c, nb, cap, T( dx ), T( dy ), T( off ), s, ev );
             if ( res == CUT_EMPTY ) return 0;
             if ( res == CUT_NO_ROOM ) return -1;
-            continue;
         }
-        if ( ! visit ) {                                  // pop: climb to the lowest pending height, take the sibling
-            if ( ! pend ) break;
-            const int hp = __ffsll( (long long) pend ) - 1;
-            if ( h < hp ) {                              // up from `h` to `hp`: a left child's parent is 1 before it, a right
-                const unsigned long long R = ( ( 1ull << hp ) - 1 ) & ~( ( 1ull << h ) - 1 );   // child's `2^height` before
-                m -= __popcll( left & R ) + int( R & ~left );
-                h = hp;
+        for ( bool open = false; ! open; ) {
+            if ( ! visit ) {                                  // pop: climb to the lowest pending height, take the sibling
+                if ( ! pend ) return nb;
+                const int hp = __ffsll( (long long) pend ) - 1;
+                if ( h < hp ) {                              // up from `h` to `hp`: a left child's parent is 1 before it, a right
+                    const unsigned long long R = ( ( 1ull << hp ) - 1 ) & ~( ( 1ull << h ) - 1 );   // child's `2^height` before
+                    m -= __popcll( left & R ) + int( R & ~left );
+                    h = hp;
+                }
+                const unsigned long long bit = 1ull << hp;
+                if ( left & bit ) { m = m - 1 + ( 1 << hp ); left &= ~bit; }
+                else              { m = m + 1 - ( 1 << hp ); left |= bit; }
+                pend &= ~bit;
             }
-            const unsigned long long bit = 1ull << hp;
-            if ( left & bit ) { m = m - 1 + ( 1 << hp ); left &= ~bit; }
-            else              { m = m + 1 - ( 1 << hp ); left |= bit; }
-            pend &= ~bit;
-        }
-        visit = false;
-        const ND r = have ? rec : node_at( wk, m, kind );
-        have = false;
-        if ( node_live( r ) && can_cut( c, nb, r, p0x, p0y, w0, f32, ev ) ) {
-            if ( h <= 1 ) {
-                q = r.beg;
-                qe = node_stop( wk, m, r );
-                nxt = wk.seeds[ q ];
-            } else {
-                const int lc = m + 1, rc = m + ( 1 << ( h - 1 ) );
-                const ND rl = node_at( wk, lc, kind ), rr = node_at( wk, rc, kind );
-                const bool go_left = proximity( rl, p0x, p0y ) <= proximity( rr, p0x, p0y );
-                const unsigned long long bit = 1ull << ( h - 1 );
-                pend |= bit;
-                left = go_left ? ( left | bit ) : ( left & ~bit );
-                m = go_left ? lc : rc;
-                rec = go_left ? rl : rr;
-                have = true;
-                h -= 1;
-                visit = true;
+            visit = false;
+            const ND r = have ? rec : node_at( wk, m, kind );
+            have = false;
+            if ( node_live( r ) && can_cut( c, nb, r, p0x, p0y, w0, f32, ev ) ) {
+                if ( h <= 1 ) {
+                    q = r.beg;
+                    qe = node_stop( wk, m, r );
+                    nxt = wk.seeds[ q ];
+                    open = true;
+                } else {
+                    const int lc = m + 1, rc = m + ( 1 << ( h - 1 ) );
+                    const ND rl = node_at( wk, lc, kind ), rr = node_at( wk, rc, kind );
+                    const bool go_left = proximity( rl, p0x, p0y ) <= proximity( rr, p0x, p0y );
+                    const unsigned long long bit = 1ull << ( h - 1 );
+                    pend |= bit;
+                    left = go_left ? ( left | bit ) : ( left & ~bit );
+                    m = go_left ? lc : rc;
+                    rec = go_left ? rl : rr;
+                    have = true;
+                    h -= 1;
+                    visit = true;
+                }
             }
         }
     }
-    return nb;
 }
 
 // ---- integrals -------------------------------------------------------------------------------------
