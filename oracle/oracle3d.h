@@ -11,7 +11,7 @@
 // Integrals against a sum of Gaussians use fixed rules instead of the reference's adaptive `PointwiseDensity`
 // ( `PointwiseDensity.h:31-134`, relative tolerance 1e-5 ): a cell's MASS is the flux through its faces of a field whose
 // divergence is the density ( `flux_field` ), so that the masses of a diagram sum to the box's whatever the weights; a
-// face integral uses the triangles from the face's mean vertex and the 7-point degree-5 rule. The moments cut the cell in
+// face integral uses the triangles from the face's area centroid and the 7-point degree-5 rule. The moments cut the cell in
 // tetrahedra ( apex at the mean of its vertices, bases those triangles ) integrated by the 14-point degree-5 rule. Both rules are checked on every monomial of degree <= 5 by the tests.
 
 struct Density3 {
@@ -461,8 +461,31 @@ static void face_loop( const Poly3 &c, int f, std::vector<int> &loop ) {
     loop.clear();
 }
 
+/// the area centroid of a face polygon ( its loop of vertices ), from the triangles of its mean vertex: the apex of the
+/// face's triangles for the rules. The mean of the vertices would jump when a vertex appears or vanishes with an edge of
+/// zero length, and the rules' error with it ( ~`( h / sigma )^5` of a cell's mass for a Gaussian, `h` the cell's size ):
+/// a jump of the masses in the weights, which stalls a Newton solve asked for less than that error; the centroid moves with
+/// the vertices
+static void face_centroid( const Poly3 &c, const std::vector<int> &loop, double *G ) {
+    const size_t nl = loop.size();
+    double gx = 0, gy = 0, gz = 0;
+    for ( int v : loop ) { gx += c.x[ sz( v ) ]; gy += c.y[ sz( v ) ]; gz += c.z[ sz( v ) ]; }
+    gx /= double( nl ); gy /= double( nl ); gz /= double( nl );
+    double sa = 0, sx = 0, sy = 0, sw = 0;
+    for ( size_t i = 0; i < nl; ++i ) {
+        const int v1 = loop[ i ], v2 = loop[ ( i + 1 ) % nl ];
+        const double ux = c.x[ sz( v1 ) ] - gx, uy = c.y[ sz( v1 ) ] - gy, uz = c.z[ sz( v1 ) ] - gz;
+        const double vx = c.x[ sz( v2 ) ] - gx, vy = c.y[ sz( v2 ) ] - gy, vz = c.z[ sz( v2 ) ] - gz;
+        const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+        const double a = 0.5 * std::sqrt( cx * cx + cy * cy + cz * cz );
+        sa += a; sx += a * ( ux + vx ); sy += a * ( uy + vy ); sw += a * ( uz + vz );
+    }
+    if ( sa > 0 ) { gx += sx / ( 3 * sa ); gy += sy / ( 3 * sa ); gz += sw / ( 3 * sa ); }
+    G[ 0 ] = gx; G[ 1 ] = gy; G[ 2 ] = gz;
+}
+
 /// for every face: `f( face, loop )`; then its tetrahedra ( apex `A` the mean of the cell's vertices, base the triangle
-/// of the face's mean vertex and one of its edges ) with their volumes: `t( A, B, C, D, vol )`. Both in the seed's frame.
+/// of the face's area centroid and one of its edges ) with their volumes: `t( A, B, C, D, vol )`. Both in the seed's frame.
 template<class FF, class TF>
 static void for_each_face_and_tet( const Poly3 &c, FF &&ff, TF &&tf ) {
     double ax = 0, ay = 0, az = 0;
@@ -475,9 +498,8 @@ static void for_each_face_and_tet( const Poly3 &c, FF &&ff, TF &&tf ) {
         if ( loop.size() < 3 ) continue;
         ff( f, loop );
         const size_t nl = loop.size();
-        double B[ 3 ] = { 0, 0, 0 };
-        for ( int v : loop ) { B[ 0 ] += c.x[ sz( v ) ]; B[ 1 ] += c.y[ sz( v ) ]; B[ 2 ] += c.z[ sz( v ) ]; }
-        for ( double &b : B ) b /= double( nl );
+        double B[ 3 ];
+        face_centroid( c, loop, B );
         for ( size_t i = 0; i < nl; ++i ) {
             const int v1 = loop[ i ], v2 = loop[ ( i + 1 ) % nl ];
             const double C[ 3 ] = { c.x[ sz( v1 ) ], c.y[ sz( v1 ) ], c.z[ sz( v1 ) ] };
@@ -523,14 +545,14 @@ static void tet_moments( const Density3 &dn, const double *p0, const double *A, 
 constexpr int BOX_SUB = 4;
 
 /// `int_face rho` over a face polygon ( seed frame ), or with `flux` the flux `int_face F . n` of the field above through
-/// it ( `n` the face's outward unit normal ): triangles from the polygon's mean vertex, so that the two cells of a face
-/// integrate it alike whatever vertex their loops start from
+/// it ( `n` the face's outward unit normal ): triangles from the polygon's area centroid ( `face_centroid` ), so that the two
+/// cells of a face integrate it alike whatever vertex their loops start from
 static double face_integral( const Density3 &dn, const double *p0, const Poly3 &c, const std::vector<int> &loop, const double *flux = nullptr,
                              int sub = 1 ) {
     const size_t nl = loop.size();
-    double gx = 0, gy = 0, gz = 0;
-    for ( int v : loop ) { gx += c.x[ sz( v ) ]; gy += c.y[ sz( v ) ]; gz += c.z[ sz( v ) ]; }
-    gx /= double( nl ); gy /= double( nl ); gz /= double( nl );
+    double G[ 3 ];
+    face_centroid( c, loop, G );
+    const double gx = G[ 0 ], gy = G[ 1 ], gz = G[ 2 ];
     double r = 0;
     for ( size_t i = 0; i < nl; ++i ) {
         const int v1 = loop[ i ], v2 = loop[ ( i + 1 ) % nl ];
@@ -570,7 +592,7 @@ static double face_integral( const Density3 &dn, const double *p0, const Poly3 &
 // ---- a voxel image: the cell cut by every voxel it meets ------------------------------------------------------------
 
 /// the voxels of the cell ( seed frame, the seed at `p0` ), each cut by its six planes: `f( value, piece )` per non-empty piece.
-/// The cells outside the grid take the value of its edge voxels, as the card's kernels do
+/// The image is zero off its grid: what lies outside it is in no piece
 template<class F>
 static void for_each_voxel_piece( const Density3 &dn, const double *p0, const Poly3 &c, F &&f ) {
     double lo[ 3 ] = { 1e300, 1e300, 1e300 }, hi[ 3 ] = { -1e300, -1e300, -1e300 };

@@ -137,6 +137,27 @@ struct CellSums {
     double mass = 0, mx = 0, my = 0, mz = 0, m2 = 0;
 };
 
+/// the area centroid of face `f` from its triangles `( f, v, w )` of the list ( `tf`, `tv`, `tw`, `T` of them ) and the mean
+/// of its vertices `( gx, gy, gz )`: the apex of the face's triangles for the rules. The mean would jump when a vertex
+/// appears or vanishes with an edge of zero length, and the rules' error with it ( ~`( h / sigma )^5` of a cell's mass for
+/// a Gaussian, `h` the cell's size ): a jump of the masses in the weights, which stalls a Newton solve asked for less than
+/// that error; the centroid moves with the vertices
+__device__ __forceinline__ void face_centroid( int f, int T, const int *tf, const int *tv, const int *tw, const double *px, const double *py,
+                                              const double *pz, double gx, double gy, double gz, double *G ) {
+    double sa = 0, sx = 0, sy = 0, sz = 0;
+    for ( int t = 0; t < T; ++t ) {
+        if ( tf[ t ] != f ) continue;
+        const int v = tv[ t ], w = tw[ t ];
+        const double ux = px[ v ] - gx, uy = py[ v ] - gy, uz = pz[ v ] - gz;
+        const double vx = px[ w ] - gx, vy = py[ w ] - gy, vz = pz[ w ] - gz;
+        const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+        const double a = 0.5 * sqrt( cx * cx + cy * cy + cz * cz );
+        sa += a; sx += a * ( ux + vx ); sy += a * ( uy + vy ); sz += a * ( uz + vz );
+    }
+    G[ 0 ] = gx; G[ 1 ] = gy; G[ 2 ] = gz;
+    if ( sa > 0 ) { G[ 0 ] += sx / ( 3 * sa ); G[ 1 ] += sy / ( 3 * sa ); G[ 2 ] += sz / ( 3 * sa ); }
+}
+
 // ---- a voxel image: fluxes through the faces, cut by the voxel grid ------------------------------------------------------
 //
 // The divergence theorem with fields along x: `int_cell rho = flux of ( G, 0, 0 )`, `G( x, y, z ) = int_a^x rho( s, y, z ) ds`
@@ -818,6 +839,13 @@ __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3_measure( Params3 P ) {
         }
     }
     wave_sync();
+    // each face's apex moved to its area centroid ( `face_centroid` )
+    if ( lane < nf && M.fok[ lane ] ) {
+        double G[ 3 ];
+        face_centroid( lane, T, M.tf, M.tv, M.tw, M.px, M.py, M.pz, M.gx[ lane ], M.gy[ lane ], M.gz[ lane ], G );
+        M.gx[ lane ] = G[ 0 ]; M.gy[ lane ] = G[ 1 ]; M.gz[ lane ] = G[ 2 ];
+    }
+    wave_sync();
     const bool facets = P.mode == MODE_FACETS, moments = P.mode == MODE_MOMENTS;
     CellSums S;
     for ( int t = lane; t < T; t += 64 ) {
@@ -1092,6 +1120,14 @@ __global__ __launch_bounds__( 64 ) void k_cells3_pass2( Params3 P ) {
             }
             T += __shfl( inc, 63, 64 );
         }
+        __syncthreads();
+        // each face's apex moved to its area centroid ( `face_centroid` )
+        for ( int f = lane; f < nf; f += 64 )
+            if ( C.idx[ f ] ) {
+                double G[ 3 ];
+                face_centroid( f, T, C.tf, C.tv, C.tw, C.x, C.y, C.z, C.X[ f ], C.Y[ f ], C.Z[ f ], G );
+                C.X[ f ] = G[ 0 ]; C.Y[ f ] = G[ 1 ]; C.Z[ f ] = G[ 2 ];
+            }
         __syncthreads();
         const bool facets = P.mode == MODE_FACETS, moments = P.mode == MODE_MOMENTS;
         CellSums S;
