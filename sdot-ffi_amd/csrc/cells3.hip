@@ -9,7 +9,7 @@
 // `cell/Pruning.h:46-76` in three coordinates, facets `c_kj = int_face rho / ( 2 |p_j - p_k| )` ( `Laplacian.h:40-72` ).
 // Sums of Gaussians: a cell's mass is the flux through its faces of a field whose divergence is the density, so the masses
 // of a diagram sum to the box's whatever the weights; face integrals by the 7-point degree-5 rule on the triangles of a
-// face's mean vertex, moments by the 14-point degree-5 rule on the tetrahedra of those triangles and the cell's mean vertex.
+// face's area centroid, moments by the 14-point degree-5 rule on the tetrahedra of those triangles and the cell's mean vertex.
 
 #include "cells3.h"
 
@@ -137,25 +137,37 @@ struct CellSums {
     double mass = 0, mx = 0, my = 0, mz = 0, m2 = 0;
 };
 
-/// the area centroid of face `f` from its triangles `( f, v, w )` of the list ( `tf`, `tv`, `tw`, `T` of them ) and the mean
-/// of its vertices `( gx, gy, gz )`: the apex of the face's triangles for the rules. The mean would jump when a vertex
-/// appears or vanishes with an edge of zero length, and the rules' error with it ( ~`( h / sigma )^5` of a cell's mass for
-/// a Gaussian, `h` the cell's size ): a jump of the masses in the weights, which stalls a Newton solve asked for less than
-/// that error; the centroid moves with the vertices
-__device__ __forceinline__ void face_centroid( int f, int T, const int *tf, const int *tv, const int *tw, const double *px, const double *py,
-                                              const double *pz, double gx, double gy, double gz, double *G ) {
-    double sa = 0, sx = 0, sy = 0, sz = 0;
-    for ( int t = 0; t < T; ++t ) {
-        if ( tf[ t ] != f ) continue;
-        const int v = tv[ t ], w = tw[ t ];
-        const double ux = px[ v ] - gx, uy = py[ v ] - gy, uz = pz[ v ] - gz;
-        const double vx = px[ w ] - gx, vy = py[ w ] - gy, vz = pz[ w ] - gz;
-        const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
-        const double a = 0.5 * sqrt( cx * cx + cy * cy + cz * cz );
-        sa += a; sx += a * ( ux + vx ); sy += a * ( uy + vy ); sz += a * ( uz + vz );
-    }
-    G[ 0 ] = gx; G[ 1 ] = gy; G[ 2 ] = gz;
+/// the area centroid of face `f` of a cell and the count of its vertices, in one walk over the vertices: the edges of `f` at
+/// vertex `v`, `f` its `j`-th face ( `face_of( v, j )` ), lead to its neighbours `nb_of( v, j + 1 )` and `nb_of( v, j + 2 )`
+/// ( mod 3 ), each edge taken from its lower vertex, and the triangles of those edges with the face's first vertex make the
+/// convex face ( weighted by `| ( u x v ) . n |`, `n` the face's normal of any length: the areas up to a common factor, without
+/// a square root ). It is the apex of the face's triangles for the rules: the mean of the vertices would jump when a vertex
+/// appears or vanishes with an edge of zero length, and the rules' error with it ( ~`( h / sigma )^5` of a cell's mass for a
+/// Gaussian, `h` the cell's size ), a jump of the masses in the weights that stalls a Newton solve asked for less than that
+/// error; the centroid moves with the vertices
+template<class FaceOf, class NbOf>
+__device__ __forceinline__ int face_apex( int f, int nv, FaceOf face_of, NbOf nb_of, const double *px, const double *py, const double *pz,
+                                          double nx, double ny, double nz, double *G ) {
+    int cnt = 0, v0 = -1;
+    double ox = 0, oy = 0, oz = 0, sa = 0, sx = 0, sy = 0, sz = 0;
+    for ( int v = 0; v < nv; ++v )
+        for ( int j = 0; j < 3; ++j ) {
+            if ( face_of( v, j ) != f ) continue;
+            ++cnt;
+            if ( v0 < 0 ) { v0 = v; ox = px[ v ]; oy = py[ v ]; oz = pz[ v ]; }
+            for ( int e = 1; e <= 2; ++e ) {
+                const int w = nb_of( v, ( j + e ) % 3 );
+                if ( ! ( v < w ) ) continue;
+                const double ux = px[ v ] - ox, uy = py[ v ] - oy, uz = pz[ v ] - oz;
+                const double vx = px[ w ] - ox, vy = py[ w ] - oy, vz = pz[ w ] - oz;
+                const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+                const double a = fabs( cx * nx + cy * ny + cz * nz );
+                sa += a; sx += a * ( ux + vx ); sy += a * ( uy + vy ); sz += a * ( uz + vz );
+            }
+        }
+    G[ 0 ] = ox; G[ 1 ] = oy; G[ 2 ] = oz;
     if ( sa > 0 ) { G[ 0 ] += sx / ( 3 * sa ); G[ 1 ] += sy / ( 3 * sa ); G[ 2 ] += sz / ( 3 * sa ); }
+    return cnt;
 }
 
 // ---- a voxel image: fluxes through the faces, cut by the voxel grid ------------------------------------------------------
@@ -763,12 +775,12 @@ __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3( Params3 P ) {
     measure_cell<false>( P, LdsFaces{ L }, c, lane, k0, p0, row0 );
 }
 
-/// what a wave of the measure kernel keeps in LDS: the cell's vertices, its faces ( mean vertex, unit normal, the norm of
+/// what a wave of the measure kernel keeps in LDS: the cell's vertices, its faces ( area centroid, unit normal, the norm of
 /// the plane's normal, id, validity ), its triangles ( face, two vertices ) with their facet parts
 constexpr int MTRI = 6 * CELL3_VERTICES / 2 + 64;
 struct MeasLds {
     double px[ CELL3_VERTICES ], py[ CELL3_VERTICES ], pz[ CELL3_VERTICES ];
-    int    pl[ CELL3_VERTICES ];
+    int    pl[ CELL3_VERTICES ], nb[ CELL3_VERTICES ];
     double gx[ CELL3_FACES ], gy[ CELL3_FACES ], gz[ CELL3_FACES ], ux[ CELL3_FACES ], uy[ CELL3_FACES ], uz[ CELL3_FACES ], dn[ CELL3_FACES ];
     int    fid[ CELL3_FACES ], fok[ CELL3_FACES ];
     int    tf[ MTRI ], tv[ MTRI ], tw[ MTRI ];
@@ -776,7 +788,7 @@ struct MeasLds {
 };
 
 /// the second kernel of a sum of Gaussians: the cells the first pass kept, one wave each. The triangles of the cell ( a
-/// face's mean vertex and one of its edges, `3 nv` of them ) are spread over the lanes, each face's sums taken in a fixed
+/// face's area centroid and one of its edges, `3 nv` of them ) are spread over the lanes, each face's sums taken in a fixed
 /// order: the same terms as the face-by-face loop of the first pass, with every lane at work
 template<bool IMAGE>
 __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3_measure( Params3 P ) {
@@ -793,21 +805,18 @@ __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3_measure( Params3 P ) {
     const bool act = lane < nv;
     double x = 0, y = 0, z = 0;
     int pl = 0, nb = 0;
-    if ( act ) { x = P.pb.vx[ at ]; y = P.pb.vy[ at ]; z = P.pb.vz[ at ]; pl = P.pb.vpl[ at ]; nb = P.pb.vnb[ at ]; M.px[ lane ] = x; M.py[ lane ] = y; M.pz[ lane ] = z; M.pl[ lane ] = pl; }
+    if ( act ) { x = P.pb.vx[ at ]; y = P.pb.vy[ at ]; z = P.pb.vz[ at ]; pl = P.pb.vpl[ at ]; nb = P.pb.vnb[ at ]; M.px[ lane ] = x; M.py[ lane ] = y; M.pz[ lane ] = z; M.pl[ lane ] = pl; M.nb[ lane ] = nb; }
     const double inv_nv = nv > 0 ? 1.0 / nv : 0.0;
     const V3 Av{ wave_total( x ) * inv_nv, wave_total( y ) * inv_nv, wave_total( z ) * inv_nv };
     wave_sync();
-    // the faces, a lane each: the mean of their vertices ( in vertex order ), the unit normal
+    // the faces, a lane each: the area centroid ( `face_apex` ), the unit normal
     if ( lane < nf ) {
-        double sx = 0, sy = 0, sz = 0;
-        int cnt = 0;
-        for ( int v = 0; v < nv; ++v ) {
-            const int q = M.pl[ v ];
-            if ( byte_of( q, 0 ) == lane || byte_of( q, 1 ) == lane || byte_of( q, 2 ) == lane ) { sx += M.px[ v ]; sy += M.py[ v ]; sz += M.pz[ v ]; ++cnt; }
-        }
         const double fx = P.pb.fx[ at ], fy = P.pb.fy[ at ], fz = P.pb.fz[ at ];
         const double d = sqrt( fx * fx + fy * fy + fz * fz );
-        M.gx[ lane ] = cnt ? sx / cnt : 0; M.gy[ lane ] = cnt ? sy / cnt : 0; M.gz[ lane ] = cnt ? sz / cnt : 0;
+        double G[ 3 ];
+        const int cnt = face_apex( lane, nv, [&]( int v, int j ) { return byte_of( M.pl[ v ], j ); },
+                                   [&]( int v, int j ) { return byte_of( M.nb[ v ], j ); }, M.px, M.py, M.pz, fx, fy, fz, G );
+        M.gx[ lane ] = G[ 0 ]; M.gy[ lane ] = G[ 1 ]; M.gz[ lane ] = G[ 2 ];
         M.ux[ lane ] = fx / d; M.uy[ lane ] = fy / d; M.uz[ lane ] = fz / d; M.dn[ lane ] = d;
         M.fid[ lane ] = P.pb.fid[ at ];
         M.fok[ lane ] = cnt >= 3;
@@ -837,13 +846,6 @@ __global__ __launch_bounds__( 64 * WAVES3 ) void k_cells3_measure( Params3 P ) {
                 if ( lane < w ) { M.tf[ t ] = f; M.tv[ t ] = lane; M.tw[ t ] = w; ++t; }
             }
         }
-    }
-    wave_sync();
-    // each face's apex moved to its area centroid ( `face_centroid` )
-    if ( lane < nf && M.fok[ lane ] ) {
-        double G[ 3 ];
-        face_centroid( lane, T, M.tf, M.tv, M.tw, M.px, M.py, M.pz, M.gx[ lane ], M.gy[ lane ], M.gz[ lane ], G );
-        M.gx[ lane ] = G[ 0 ]; M.gy[ lane ] = G[ 1 ]; M.gz[ lane ] = G[ 2 ];
     }
     wave_sync();
     const bool facets = P.mode == MODE_FACETS, moments = P.mode == MODE_MOMENTS;
@@ -1081,11 +1083,10 @@ __global__ __launch_bounds__( 64 ) void k_cells3_pass2( Params3 P ) {
         const V3 Av{ wave_total( ax ) * inv_nv, wave_total( ay ) * inv_nv, wave_total( az ) * inv_nv };
         __syncthreads();
         for ( int f = lane; f < nf; f += 64 ) {
-            double sx = 0, sy = 0, sz = 0;
-            int cnt = 0;
-            for ( int v = 0; v < nv; ++v )
-                if ( C.pl[ 3 * v ] == f || C.pl[ 3 * v + 1 ] == f || C.pl[ 3 * v + 2 ] == f ) { sx += C.x[ v ]; sy += C.y[ v ]; sz += C.z[ v ]; ++cnt; }
-            C.X[ f ] = cnt ? sx / cnt : 0; C.Y[ f ] = cnt ? sy / cnt : 0; C.Z[ f ] = cnt ? sz / cnt : 0;
+            double G[ 3 ];                               // ( the area centroid, `face_apex` )
+            const int cnt = face_apex( f, nv, [&]( int v, int j ) { return C.pl[ 3 * v + j ]; }, [&]( int v, int j ) { return C.nb[ 3 * v + j ]; },
+                                       C.x, C.y, C.z, C.fx[ f ], C.fy[ f ], C.fz[ f ], G );
+            C.X[ f ] = G[ 0 ]; C.Y[ f ] = G[ 1 ]; C.Z[ f ] = G[ 2 ];
             const double d = sqrt( C.fx[ f ] * C.fx[ f ] + C.fy[ f ] * C.fy[ f ] + C.fz[ f ] * C.fz[ f ] );
             C.s[ f ] = d;
             C.idx[ f ] = cnt >= 3;
@@ -1120,14 +1121,6 @@ __global__ __launch_bounds__( 64 ) void k_cells3_pass2( Params3 P ) {
             }
             T += __shfl( inc, 63, 64 );
         }
-        __syncthreads();
-        // each face's apex moved to its area centroid ( `face_centroid` )
-        for ( int f = lane; f < nf; f += 64 )
-            if ( C.idx[ f ] ) {
-                double G[ 3 ];
-                face_centroid( f, T, C.tf, C.tv, C.tw, C.x, C.y, C.z, C.X[ f ], C.Y[ f ], C.Z[ f ], G );
-                C.X[ f ] = G[ 0 ]; C.Y[ f ] = G[ 1 ]; C.Z[ f ] = G[ 2 ];
-            }
         __syncthreads();
         const bool facets = P.mode == MODE_FACETS, moments = P.mode == MODE_MOMENTS;
         CellSums S;
