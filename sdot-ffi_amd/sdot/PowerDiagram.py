@@ -9,7 +9,7 @@ Mirrors `src/sdot/PowerDiagram.py` ( the constructor `:120`, `measures` `:325-39
 import numpy as np
 
 from . import _native
-from .distributions import UniformBox
+from .distributions import Image, UniformBox
 
 
 def box_half_spaces( lo, hi ):
@@ -54,6 +54,17 @@ class PowerDiagram:
         dist = distribution.normalized_version() if distribution is not None else None
         if dist is None and boundaries is None:
             raise ValueError( "PowerDiagram: a domain is needed ( `boundaries` or `distribution` )" )
+        # a turned grid ( `Image.rotation` ): the diagram built in the grid's frame, where the grid is axis-aligned and the power
+        # distances are the same; the positions ( differentiably ) and the half-planes taken there, the coordinates that come out
+        # ( moments, cells ) brought back
+        self._turn = None
+        if isinstance( dist, Image ) and dist.rotation is not None:
+            self._turn = dist
+            self.positions = dist.to_local( positions )
+            if boundaries is not None:
+                bd = np.asarray( boundaries[ 0 ], dtype = np.float64 )
+                boundaries = ( bd @ dist.rotation.T, np.asarray( boundaries[ 1 ], dtype = np.float64 ).reshape( -1 ) - bd @ dist.origin )
+            dist = dist.local_version()
         halves = [ b for b in ( boundaries, dist.bounding_half_spaces() if dist is not None else None ) if b is not None ]
         dirs = np.concatenate( [ np.asarray( h[ 0 ], dtype = np.float64 ) for h in halves ] )
         offs = np.concatenate( [ np.asarray( h[ 1 ], dtype = np.float64 ).reshape( -1 ) for h in halves ] )
@@ -126,7 +137,15 @@ class PowerDiagram:
         """`( masses, first, second )`: for each cell `int rho`, `int x rho` ( `[ n, d ]` ) and `int |x|^2 rho` ( device tensors,
         user order ): a transport cost is `second - 2 p . first + |p|^2 masses`, the barycentres `first / masses`"""
         self._box_only( "moments" )
-        return _native.moments( self.positions, self.weights, self._box(), self.distribution )
+        m, f, s = _native.moments( self.positions, self.weights, self._box(), self.distribution )
+        if self._turn is not None:                       # ( `x = o + R^T y`: `int x rho = o m + R^T int y rho`, `|x|^2 = |o|^2 + 2 R o . y + |y|^2` )
+            R, o = self._turn.rotation, self._turn.origin
+            T = _native.torch()
+            Rt = T.as_tensor( R, dtype = f.dtype, device = f.device )
+            ot = T.as_tensor( o, dtype = f.dtype, device = f.device )
+            s = s + 2 * ( f @ ( Rt @ ot ) ) + float( o @ o ) * m
+            f = f @ Rt + m[ :, None ] * ot
+        return m, f, s
 
     def hessian_rows( self ):
         """`( nb_nbrs, ids, vals )` ( host arrays ): each cell's neighbours `ids[ i, :nb_nbrs[ i ] ]` ( user indices, in the cell's
@@ -152,7 +171,10 @@ class PowerDiagram:
         if self.nb_dims != 2:
             raise NotImplementedError( "PowerDiagram.cells on the MI355X path: 2D" )
         self._box_only( "cells" )
-        return _native.cells_2d( self.positions, self.weights, self._box(), self.distribution )
+        off, xy, cuts = _native.cells_2d( self.positions, self.weights, self._box(), self.distribution )
+        if self._turn is not None:
+            xy = self._turn.from_local( xy )
+        return off, xy, cuts
 
     def laplacian_solve( self, rhs, linear_solver = "auto" ):
         """`x = L^+ rhs` with `x[ 0 ] == 0`: the Laplacian of the diagram's facets ( `d m_i / d w_j` ), `rhs` summing to zero.

@@ -9,7 +9,7 @@ reference's layouts ( `_STATS`, `_HISTORY` ) and `OtNotConverged` is raised the 
 import numpy as np
 
 from . import _native
-from .distributions import Image, Polytope, SumOfGaussians, UniformBox, _host
+from .distributions import Image, Polytope, SumOfDiracs, SumOfGaussians, UniformBox, _host
 
 _STATS = [ "status", "residual", "residual0", "nb_iter", "nb_diag", "nb_backtracks", "t_majorant", "t_diag", "t_asm", "t_lin", "t_lim", "eps",
            "domain_mass", "nb_overflowed", "nb_cell_lim", "nb_limit_rounds", "lin_nb_hierarchies", "lin_nb_iter", "lin_worst", "start", "t_total",
@@ -145,6 +145,13 @@ class SdotPlanNd:
     def _build( self, problem, settings, verbose, warm = None ):
         self.problem = problem
         src, dst = problem.source, problem.target
+        self._turn = None
+        if isinstance( dst, Image ) and dst.rotation is not None:
+            # a turned grid: the problem solved in the grid's frame ( a move and a turn: the same power distances, so the same
+            # weights and masses ), its barycentres brought back ( `_set_result` )
+            self._turn = dst
+            src = SumOfDiracs( dst.to_local( src.positions ), src.weights, src.target_mass )
+            dst = dst.local_version()
         d = problem.nb_dims
         if d not in ( 2, 3 ):
             raise NotImplementedError( f"SdotPlanNd on the MI355X path: 2D or 3D ( not { d }D )" )
@@ -211,6 +218,8 @@ class SdotPlanNd:
         self.cell_masses = res[ "cell_masses" ]
         #: the barycenter of each cell ( its seed if empty ), `[ n, d ]`
         self.barycenters = res[ "barycenters" ]
+        if getattr( self, "_turn", None ) is not None:
+            self.barycenters = self._turn.from_local( self.barycenters )
         #: the transport cost `sum_i int_{cell_i} |x - p_i|^2 rho`
         self.cost = res[ "cost" ]
         self._clusters = res[ "clusters" ]

@@ -51,10 +51,27 @@ class SumOfDiracs:
         return res
 
 
+def _affine( x, A, b ):
+    """`x A^T + b` for rows `x`, numpy or torch ( a torch tensor stays one, on its device, differentiable )"""
+    try:
+        import torch
+        if isinstance( x, torch.Tensor ):
+            At = torch.as_tensor( np.ascontiguousarray( A.T ), dtype = x.dtype, device = x.device )
+            return x @ At + torch.as_tensor( b, dtype = x.dtype, device = x.device )
+    except ImportError:
+        pass
+    return np.asarray( x, dtype = np.float64 ) @ A.T + b
+
+
 class Image:
     """piecewise-constant density on a grid: pixel `( i, j )` ( first index along x ) covers
     `origin + frame^T [ i, i + 1 ] x [ j, j + 1 ]` ( reference `distributions/Image.py:12`, `Image.cxx:97` ).
-    Regular grids with a diagonal frame."""
+    Regular grids: a diagonal frame, or a TURNED one whose rows are orthogonal ( a rotated or reflected grid of rectangular
+    pixels ). A turned grid is solved in its own frame: `y = R ( x - origin )` ( `R` the frame's rows made unit ) moves and
+    turns the plane without changing a distance, so power diagrams, masses and weights are the same there, and the grid is
+    axis-aligned ( `local_version` ); positions go there ( `to_local` ) and barycentres come back ( `from_local` ). A frame
+    whose rows are not orthogonal ( sheared pixels ) is refused: the reference's card refuses every frame that is not
+    diagonal ( `PowerDiagram_Bsp.py:247-248` )."""
 
     def __init__( self, values, origin = None, frame = None, target_mass = 1.0 ):
         self.values = values
@@ -63,8 +80,28 @@ class Image:
         self.origin = np.zeros( v.ndim ) if origin is None else np.asarray( _host( origin ), dtype = np.float64 ).reshape( -1 )
         self.frame = np.eye( v.ndim ) if frame is None else np.asarray( _host( frame ), dtype = np.float64 ).reshape( v.ndim, v.ndim )
         self.target_mass = target_mass
+        #: the turn of a turned grid ( `None`: a diagonal frame ), and its pixels' sides
+        self.rotation = None
         if np.any( self.frame != np.diag( np.diag( self.frame ) ) ) or np.any( np.diag( self.frame ) <= 0 ):
-            raise NotImplementedError( "Image: this implementation takes a diagonal frame with positive steps ( a regular grid )" )
+            steps = np.sqrt( ( self.frame ** 2 ).sum( axis = 1 ) )
+            gram = self.frame @ self.frame.T
+            if not np.all( steps > 0 ) or np.abs( gram - np.diag( np.diag( gram ) ) ).max() > 1e-12 * float( steps.max() ) ** 2:
+                raise NotImplementedError( "Image: a diagonal frame, or one whose rows are orthogonal ( a turned grid ); sheared pixels "
+                                           "are not taken" )
+            self.rotation = self.frame / steps[ :, None ]
+            self.steps = steps
+
+    def to_local( self, x ):
+        """positions `[ n, d ]` in the frame of a turned grid: `R ( x - origin )`"""
+        return _affine( x, self.rotation, -self.rotation @ self.origin )
+
+    def from_local( self, y ):
+        """positions `[ n, d ]` back from the frame of a turned grid: `origin + R^T y`"""
+        return _affine( y, self.rotation.T, self.origin )
+
+    def local_version( self ):
+        """the same image in the frame of a turned grid: origin 0, the diagonal frame of its pixels' sides"""
+        return Image( self.values, np.zeros( self.nb_dims ), np.diag( self.steps ), self.target_mass )
 
     @property
     def shape( self ):
@@ -82,15 +119,24 @@ class Image:
         return Image( v * ( self.target_mass / m ), self.origin, self.frame, self.target_mass )
 
     def box( self ):
-        """`( lo, hi )` of the grid"""
-        h = np.diag( self.frame )
-        return self.origin.copy(), self.origin + h * np.asarray( self.shape, dtype = np.float64 )
+        """`( lo, hi )` of the grid ( of its corners, for a turned grid )"""
+        n = np.asarray( self.shape, dtype = np.float64 )
+        if self.rotation is None:
+            h = np.diag( self.frame )
+            return self.origin.copy(), self.origin + h * n
+        import itertools
+        corners = np.array( [ self.origin + self.frame.T @ ( n * np.array( c ) ) for c in itertools.product( ( 0, 1 ), repeat = self.nb_dims ) ] )
+        return corners.min( axis = 0 ), corners.max( axis = 0 )
 
     def bounding_half_spaces( self ):
-        """the block of the grid as half-spaces `dir . x <= off` ( `Image.py:66-80` )"""
+        """the block of the grid as half-spaces `dir . x <= off` ( `Image.py:66-80` ); a turned grid's along its own axes"""
         d = self.nb_dims
-        lo, hi = self.box()
-        return np.concatenate( [ np.eye( d ), -np.eye( d ) ] ), np.concatenate( [ hi, -lo ] )
+        if self.rotation is None:
+            lo, hi = self.box()
+            return np.concatenate( [ np.eye( d ), -np.eye( d ) ] ), np.concatenate( [ hi, -lo ] )
+        R = self.rotation
+        ext = self.steps * np.asarray( self.shape, dtype = np.float64 )
+        return np.concatenate( [ R, -R ] ), np.concatenate( [ R @ self.origin + ext, -( R @ self.origin ) ] )
 
     def native_args3( self, device_tensor ):
         """`( kind, rho, voxels, ( nx, ny, nz, geo ) )` for the 3D calls: voxel `( i, j, k )` at `( k ny + j ) nx + i`, `geo` the
@@ -98,6 +144,8 @@ class Image:
         v = _host( self.values ).astype( np.float64 )
         if v.ndim != 3:
             raise NotImplementedError( "Image: a 3D diagram takes a 3D image" )
+        if self.rotation is not None:
+            raise ValueError( "Image: a turned grid is integrated in its own frame ( `local_version`, positions by `to_local` )" )
         nx, ny, nz = v.shape
         h = np.diag( self.frame )
         vox = device_tensor( np.ascontiguousarray( v.transpose( 2, 1, 0 ) ).reshape( -1 ) )
@@ -108,6 +156,8 @@ class Image:
         v = _host( self.values ).astype( np.float64 )
         if v.ndim != 2:
             raise NotImplementedError( "Image: 2D images on this path" )
+        if self.rotation is not None:
+            raise ValueError( "Image: a turned grid is integrated in its own frame ( `local_version`, positions by `to_local` )" )
         if v.size and np.all( v == v.flat[ 0 ] ):        # equal values: the constant density ( `PowerDiagram_Bsp.py:223-231` )
             return 0, float( v.flat[ 0 ] ), None, 0, 0, [ 0.0, 0.0, 1.0, 1.0 ]
         nx, ny = v.shape

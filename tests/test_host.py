@@ -83,3 +83,40 @@ def test_precision_maps_to_the_kernels_float():
     pb3 = OtProblem( SumOfDiracs( rng.uniform( 0, 1, ( 50, 3 ) ) ), UniformBox( [ 0, 0, 0 ], [ 1, 1, 1 ] ) )
     with pytest.raises( NotImplementedError ):
         pb3.solve( Iterative( precision = "fp32" ) )
+
+
+# ---- turned grids ----------------------------------------------------------------------------------------------------
+
+def _turn( th ):
+    return np.array( [ [ np.cos( th ), np.sin( th ) ], [ -np.sin( th ), np.cos( th ) ] ] )
+
+
+def test_a_turned_grid_is_an_axis_aligned_one_in_its_frame():
+    from sdot.distributions import Image
+    rng = np.random.default_rng( 1 )
+    F = np.diag( [ 0.5, 0.25 ] ) @ _turn( 0.3 )
+    im = Image( rng.uniform( 0.5, 1.5, ( 4, 3 ) ), origin = [ 1.0, 2.0 ], frame = F )
+    assert im.rotation is not None and np.allclose( im.steps, [ 0.5, 0.25 ] )
+    x = rng.uniform( 0, 3, ( 50, 2 ) )
+    assert np.abs( im.from_local( im.to_local( x ) ) - x ).max() < 1e-15
+    # a pixel's centre goes to the axis-aligned one's, distances are kept
+    assert np.allclose( im.to_local( np.array( [ [ 1.0, 2.0 ] ] ) + ( F.T @ [ 2.5, 1.5 ] )[ None ] ), [ [ 1.25, 0.375 ] ], atol = 1e-15 )
+    y = im.to_local( x )
+    assert np.allclose( np.linalg.norm( x[ :, None ] - x[ None ], axis = 2 ), np.linalg.norm( y[ :, None ] - y[ None ], axis = 2 ), atol = 1e-14 )
+    # the half-planes hold exactly the grid's corners, the mass is the values times the pixels' area
+    d, o = im.bounding_half_spaces()
+    for ij in ( ( 0, 0 ), ( 4, 0 ), ( 0, 3 ), ( 4, 3 ) ):
+        p = np.array( [ 1.0, 2.0 ] ) + F.T @ np.array( ij, dtype = float )
+        assert np.all( d @ p <= o + 1e-12 ) and np.isclose( ( d @ p - o ).max(), 0, atol = 1e-12 )
+    assert np.isclose( im.mass, im.values.sum() * 0.125 )
+    loc = im.local_version()
+    assert loc.rotation is None and np.allclose( loc.box()[ 1 ], [ 2.0, 0.75 ] )
+
+
+def test_sheared_pixels_are_refused_and_a_turned_grid_crosses_in_its_frame_only():
+    from sdot.distributions import Image
+    with pytest.raises( NotImplementedError ):
+        Image( np.ones( ( 3, 3 ) ), frame = [ [ 1.0, 0.2 ], [ 0.0, 1.0 ] ] )
+    im = Image( np.ones( ( 3, 3 ) ), frame = _turn( 0.5 ) )
+    with pytest.raises( ValueError ):
+        im.native_args( device_tensor = lambda a: a )
