@@ -80,6 +80,47 @@ def duplicate_gap( positions ):
     return _DUPLICATE_GAP * ( s2 if s2 > 0 else 1.0 )
 
 
+def _solve_3d_merged( src, dup, rep, w0, opts, box, dst ):
+    """THE EXACT DUPLICATES OF A 3D CLOUD ( `dup[ q ]` at the place of `rep[ q ]` ): the solve of the cloud without them, each
+    representative carrying the targets of its duplicates, brought back to the whole cloud: a duplicate's weight is its
+    representative's less `k gap` ( the `k`-th duplicate of that representative, `gap = duplicate_gap`, as the 2D solve ties
+    them, `SdotPlanNd.py:650-685` ), its cell empty ( mass 0, its seed as barycentre ), its cluster its representative. This
+    is the plan the 2D solve's merge reaches on the device: the duplicates' empty cells take no part in the others'"""
+    T = _native.torch()
+    n = int( src.nb_diracs )
+    keep = np.ones( n, dtype = bool )
+    keep[ dup ] = False
+    kept = np.flatnonzero( keep )
+    nu = np.asarray( _host( src.weights ), dtype = np.float64 ).reshape( -1 ).copy()
+    np.add.at( nu, rep, nu[ dup ] )
+    pos = _native.on_device( src.positions ).reshape( n, 3 )
+    ki = T.as_tensor( kept, device = pos.device )
+    w0r = None if w0 is None else _native.on_device( w0 ).reshape( -1 )[ ki ].contiguous()
+    res = _native.solve_3d( pos[ ki ].contiguous(), nu[ kept ], w0r, opts, box, dst )
+    k = np.zeros( len( dup ) )
+    count = {}
+    for q, r in enumerate( rep.tolist() ):
+        count[ r ] = count.get( r, 0 ) + 1
+        k[ q ] = count[ r ]
+    di, ri = T.as_tensor( dup, device = pos.device ), T.as_tensor( rep, device = pos.device )
+    w = T.zeros( n, dtype = T.float64, device = pos.device )
+    w[ ki ] = res[ "weights" ]
+    w[ di ] = w[ ri ] - T.as_tensor( k * duplicate_gap( src.positions ), dtype = T.float64, device = pos.device )
+    masses = T.zeros( n, dtype = T.float64, device = pos.device )
+    masses[ ki ] = res[ "cell_masses" ]
+    bary = pos.clone()
+    bary[ ki ] = res[ "barycenters" ]
+    clusters = T.arange( n, dtype = T.int64, device = pos.device )
+    clusters[ di ] = ri
+    stats = np.asarray( res[ "stats" ], dtype = np.float64 ).copy()
+    nb_reps = len( np.unique( rep ) )
+    stats[ _STATS.index( "nb_duplicates" ) ] = len( dup )
+    stats[ _STATS.index( "nb_clusters" ) ] = nb_reps
+    stats[ _STATS.index( "nb_aggregated" ) ] = len( dup ) + nb_reps
+    return dict( weights = w, cell_masses = masses, barycenters = bary, clusters = clusters, cost = res[ "cost" ], stats = stats,
+                 history = res[ "history" ] )
+
+
 def card_options( settings, density, verbose = False, nb_dups = 0, gap = 0.0 ):
     """the 41-double option vector ( `SdotPlanNd.py:806-833` of the reference, same names and meanings )"""
     tun = settings.tuning
@@ -176,17 +217,20 @@ class SdotPlanNd:
             if not isinstance( dst, ( UniformBox, SumOfGaussians, Image ) ):
                 raise NotImplementedError( f"SdotPlanNd on the MI355X path, 3D: a UniformBox, a SumOfGaussians or an Image target "
                                            f"( not { type( dst ).__name__ } )" )
+            dup = rep = np.zeros( 0, dtype = np.int64 )
             if settings.aggregate:
                 dup, rep = exact_duplicates( src.positions )
-                if len( dup ):
-                    raise NotImplementedError( f"SdotPlanNd on the MI355X path, 3D: { len( dup ) } exact duplicates ( merge them first )" )
             self._dup_note = "checked" if settings.aggregate else "not checked ( aggregate = False )"
             step = settings.tuning.step
             if step == "limits":
                 raise NotImplementedError( "SdotPlanNd on the MI355X path, 3D: the step is by trials ( the limits are a 2D step )" )
             opts = card_options( settings, dst, verbose, 0, 0.0 )
             opts[ _native.CARD_OPTIONS.index( "step" ) ] = 0.0   # ( 'auto': trials in 3D )
-            res = _native.solve_3d( src.positions, src.weights, w0, opts, [ *lo[ :3 ], *hi[ :3 ] ], dst )
+            box3 = [ *lo[ :3 ], *hi[ :3 ] ]
+            if len( dup ):
+                res = _solve_3d_merged( src, dup, rep, w0, opts, box3, dst )
+            else:
+                res = _native.solve_3d( src.positions, src.weights, w0, opts, box3, dst )
             self._set_result( res, src, settings )
             return
         box = [ lo[ 0 ], lo[ 1 ], hi[ 0 ], hi[ 1 ] ]

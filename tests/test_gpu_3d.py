@@ -169,3 +169,42 @@ def test_3d_gaussians_at_200k_reach_1e_9( native ):
     sol = sdot.OtProblem( sdot.SumOfDiracs( pos ), target ).solve( sdot.Iterative( tol = 1e-9, on_failure = "ignore" ) )
     assert sol.converged, sol.stats[ "status" ]
     assert np.abs( sol.cell_masses.cpu().numpy() - sol.target_masses ).max() <= 1e-9
+
+
+def test_3d_exact_duplicates_are_merged( native, oracle ):
+    """a 3D cloud with exact pairs and triples: the solve of the cloud without its duplicates, their targets on their
+    representatives; the duplicates' weights tied below their representatives' ( `k gap` ), their cells empty, and the oracle's
+    FP64 cells of the whole cloud at the returned weights are the returned masses"""
+    import torch
+    import sdot
+    from sdot.SdotPlanNd import duplicate_gap
+    rng = np.random.default_rng( 41 )
+    base = rng.uniform( 0, 1, ( 4000, 3 ) )
+    picks = rng.choice( 4000, 30, replace = False )
+    pos = np.concatenate( [ base, base[ picks ], base[ picks[ :10 ] ] ] )      # 30 pairs, 10 of them triples
+    n = len( pos )
+    nu = rng.uniform( 0.5, 1.5, n )
+    nu /= nu.sum()
+    g = gaussians( rng, sigma = 0.2 )
+    target = sdot.SumOfGaussians( g[ :, :3 ], g[ :, 3 ], g[ :, 4 ], support_box = ( [ 0, 0, 0 ], [ 1, 1, 1 ] ) )
+    tol = 1e-9 * 1e6 / n / 1e2
+    print( "3d exact duplicates", flush = True )
+    sol = sdot.OtProblem( sdot.SumOfDiracs( torch.as_tensor( pos, device = "cuda" ), nu ), target ).solve( sdot.Iterative( tol = tol ) )
+    assert sol.converged and sol.stats[ "nb_duplicates" ] == 40 and sol.stats[ "nb_clusters" ] == 30
+    w, m = sol.weights.cpu().numpy(), sol.cell_masses.cpu().numpy()
+    dup = np.arange( 4000, n )
+    rep = np.concatenate( [ picks, picks[ :10 ] ] )
+    k = np.concatenate( [ np.ones( 30 ), 2 * np.ones( 10 ) ] )
+    gap = duplicate_gap( pos )
+    assert np.allclose( w[ dup ], w[ rep ] - k * gap, rtol = 0, atol = 1e-15 )
+    assert ( m[ dup ] == 0 ).all()
+    assert np.array_equal( sol.clusters.cpu().numpy()[ dup ], rep )
+    # each cluster's mass is its targets' sum
+    tgt = sol.target_masses
+    tgt = tgt.cpu().numpy() if hasattr( tgt, "cpu" ) else np.asarray( tgt, dtype = np.float64 )
+    cl = tgt.copy()
+    np.add.at( cl, rep, tgt[ dup ] )
+    keep = np.arange( 4000 )
+    assert np.abs( m[ keep ] - cl[ keep ] ).max() <= tol
+    mo = oracle.measures_3d( pos, w, density = dict( gaussians = g ) )
+    assert np.abs( mo - m ).max() <= 1e-12 * m.max()
