@@ -816,7 +816,9 @@ __global__ void k_sym_check( int n, const int *row_off, const int *row_cnt, cons
 }
 
 /// the cells `k_sym_check` listed, redone by the FP64 kernel ( one lane per cell, `CAP2` slots in LDS ); each writes a new row
-/// and its measures over the FP32 ones. A cell past `CAP2` keeps its FP32 result
+/// and its measures over the FP32 ones. A cell past `CAP2` keeps its FP32 result. The list is short ( hundreds of cells ) and
+/// dealt one cell per wave before any wave takes a second: a wave's time is then one walk, not its lanes' divergent walks
+/// one after the other
 __global__ __launch_bounds__( 64 ) void k_cells_redo( Params P, const int *list, const unsigned *count_p ) {
     __shared__ double sx[ CAP2 * 64 ], sy[ CAP2 * 64 ];
     __shared__ int sid[ CAP2 * 64 ];
@@ -826,7 +828,7 @@ __global__ __launch_bounds__( 64 ) void k_cells_redo( Params P, const int *list,
     const unsigned rounds = ( count + lanes - 1 ) / lanes;
     Ev ev;
     for ( unsigned r = 0; r < rounds; ++r ) {
-        const unsigned idx = r * lanes + blockIdx.x * 64 + threadIdx.x;
+        const unsigned idx = r * lanes + threadIdx.x * gridDim.x + blockIdx.x;   // consecutive cells to different waves
         const bool live = idx < count;
         const int k = live ? list[ idx ] : 0;
         int nb = 0;
@@ -841,7 +843,7 @@ __global__ __launch_bounds__( 64 ) void k_cells_redo( Params P, const int *list,
 }
 
 /// the second pass: the cells that outgrew `CAP1`, one lane per cell with `CAP2` slots in LDS, one wave per
-/// block and a fixed grid striding over the list ( its length only the device knows ); the cells that
+/// block and a fixed grid striding over the list ( its length only the device knows ), dealt one cell per wave first; the cells that
 /// outgrow `CAP2` too are listed again for the third pass
 __global__ __launch_bounds__( 64 ) void k_cells_pass2( Params P ) {
     __shared__ double sx[ CAP2 * 64 ], sy[ CAP2 * 64 ];
@@ -852,7 +854,7 @@ __global__ __launch_bounds__( 64 ) void k_cells_pass2( Params P ) {
     const unsigned rounds = ( count + lanes - 1 ) / lanes;
     Ev ev;
     for ( unsigned r = 0; r < rounds; ++r ) {
-        const unsigned idx = r * lanes + blockIdx.x * 64 + threadIdx.x;
+        const unsigned idx = r * lanes + threadIdx.x * gridDim.x + blockIdx.x;   // consecutive cells to different waves
         const bool live = idx < count;
         const int k = live ? P.out.ovf_list[ idx ] : 0;
         int nb = 0;
