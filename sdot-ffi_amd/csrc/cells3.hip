@@ -97,32 +97,22 @@ __device__ double rho_at( const Density3 &dn, double x, double y, double z ) {
     return r;
 }
 
-/// `int_0^1 exp( -u t^2 ) t^2 dt`: the series below 1, else `( sqrt( pi ) erf( sqrt u ) / 4 - sqrt( u ) exp( -u ) / 2 ) / u^1.5`
-__device__ double radial_h( double u ) {
+/// `int_0^1 exp( -u t^2 ) t^2 dt`, `e = exp( -u )`: below 1 the series `sum_k ( -u )^k / ( k! ( 2 k + 3 ) )` to `k = 19` by
+/// Horner ( the next term is below 3e-20 of the sum ), else `( sqrt( pi ) erf( sqrt u ) / 4 - sqrt( u ) e / 2 ) / u^1.5`
+__device__ __forceinline__ double radial_h( double u, double e ) {
     if ( u < 1 ) {
-        double term = 1, r = 1.0 / 3;
-        for ( int k = 1; k < 40; ++k ) {
-            term *= -u / k;
-            const double c = term / ( 2 * k + 3 );
-            r += c;
-            if ( fabs( c ) < 1e-18 * r ) break;
-        }
+        const double c[ 20 ] = { 0.3333333333333333, -0.2, 0.07142857142857142, -0.018518518518518517, 0.003787878787878788,
+                                 -0.000641025641025641, 9.259259259259259e-05, -1.1671335200746965e-05, 1.3053467000835422e-06,
+                                 -1.3122532963802806e-07, 1.1981443140863431e-08, -1.0020843354176688e-09, 7.732132217728926e-11,
+                                 -5.537601323041936e-12, 3.7002405153966856e-13, -2.317322342975702e-14, 1.3655649521106815e-15,
+                                 -7.598533119852758e-17, 4.0049248637400584e-18, -2.005032986981544e-19 };
+        double r = c[ 19 ];
+#pragma unroll
+        for ( int k = 18; k >= 0; --k ) r = r * u + c[ k ];
         return r;
     }
     const double su = sqrt( u );
-    return ( 0.44311346272637900 * erf( su ) - 0.5 * su * exp( -u ) ) / ( u * su );
-}
-
-/// the field of divergence `rho`: per Gaussian `peak ( x - mu ) h( |x - mu|^2 / ( 2 s^2 ) )`, dotted with `n`
-__device__ double flux_dot( const Density3 &dn, double x, double y, double z, double nx, double ny, double nz ) {
-    double fx = 0, fy = 0, fz = 0;
-    for ( int q = 0; q < dn.ng; ++q ) {
-        const double *g = dn.gauss + 5 * q;
-        const double s = g[ 3 ], dx = x - g[ 0 ], dy = y - g[ 1 ], dz = z - g[ 2 ];
-        const double f = gauss_peak( g[ 4 ], s ) * radial_h( ( dx * dx + dy * dy + dz * dz ) / ( 2 * s * s ) );
-        fx += f * dx; fy += f * dy; fz += f * dz;
-    }
-    return fx * nx + fy * ny + fz * nz;
+    return ( 0.44311346272637900 * erf( su ) - 0.5 * su * e ) / ( u * su );
 }
 
 // ---- what a face's edge contributes ----------------------------------------------------------------------------------
@@ -130,6 +120,40 @@ __device__ double flux_dot( const Density3 &dn, double x, double y, double z, do
 struct V3 { double x, y, z; };
 
 constexpr int BOX_SUB = 4;                    ///< a box face's triangles: on `BOX_SUB x BOX_SUB` sub-triangles
+
+/// the 7-point rule ( degree 5 ) on the triangle `( P0, P1, P2 )` ( seed frame, the seed at `p0` ) of area `ar` and unit
+/// normal `n`, against a sum of Gaussians: adds `int rho` to `part` and the flux of `F` to `flux`, `F` the field of divergence
+/// `rho` ( per Gaussian `peak ( x - mu ) h( |x - mu|^2 / ( 2 s^2 ) )` ). The Gaussians outside, so that each one's peak and
+/// width divide once per triangle, and `exp( -u )` serves the density and the field
+__device__ __forceinline__ void gauss_rule7( const Density3 &dn, const double4 &p0, V3 P0, V3 P1, V3 P2, double ar, V3 n, double &part,
+                                             double &flux ) {
+    const double s15 = sqrt( 15.0 );
+    const double ta1 = ( 6 - s15 ) / 21, tw1 = ( 155 - s15 ) / 1200, ta2 = ( 6 + s15 ) / 21, tw2 = ( 155 + s15 ) / 1200;
+    for ( int q = 0; q < dn.ng; ++q ) {
+        const double *g = dn.gauss + 5 * q;
+        const double s = g[ 3 ], peak = gauss_peak( g[ 4 ], s ), inv = 1 / ( 2 * s * s );
+        const double mx = g[ 0 ] - p0.x, my = g[ 1 ] - p0.y, mz = g[ 2 ] - p0.z;
+        double pr = 0, fl = 0;
+#pragma unroll 1
+        for ( int qq = 0; qq < 7; ++qq ) {
+            double b0, b1, b2, wq;
+            if ( qq == 0 ) { b0 = b1 = b2 = 1.0 / 3; wq = 9.0 / 40; }
+            else {
+                const double a = qq <= 3 ? ta1 : ta2;
+                wq = qq <= 3 ? tw1 : tw2;
+                const int jj = ( qq - 1 ) % 3;
+                b0 = jj == 0 ? 1 - 2 * a : a; b1 = jj == 1 ? 1 - 2 * a : a; b2 = jj == 2 ? 1 - 2 * a : a;
+            }
+            const double dx = b0 * P0.x + b1 * P1.x + b2 * P2.x - mx, dy = b0 * P0.y + b1 * P1.y + b2 * P2.y - my;
+            const double dz = b0 * P0.z + b1 * P1.z + b2 * P2.z - mz;
+            const double u = ( dx * dx + dy * dy + dz * dz ) * inv, e = exp( -u );
+            pr += wq * e;
+            fl += wq * radial_h( u, e ) * ( dx * n.x + dy * n.y + dz * n.z );
+        }
+        part += ar * peak * pr;
+        flux += ar * peak * fl;
+    }
+}
 
 /// the sums a cell accumulates over the triangles ( face mean `g`, edge `v o` ) of its faces
 struct CellSums {
@@ -339,8 +363,6 @@ __device__ __forceinline__ CellSums edge_terms( const Density3 &dn, const double
         return S;
     }
     if ( ! moments ) {                                   // the 7-point rule: the facet's density and the mass's flux
-        const double s15 = sqrt( 15.0 );
-        const double ta1 = ( 6 - s15 ) / 21, tw1 = ( 155 - s15 ) / 1200, ta2 = ( 6 + s15 ) / 21, tw2 = ( 155 + s15 ) / 1200;
         // a box face: its flux on a `BOX_SUB x BOX_SUB` grid of sub-triangles, so that the masses' total ( the box's flux )
         // hardly depends on the weights; an inner face: the triangle itself
         const int sub = box_face ? BOX_SUB : 1;
@@ -358,20 +380,10 @@ __device__ __forceinline__ CellSums edge_terms( const Density3 &dn, const double
                     at( i0, j0, P0.x, P0.y, P0.z );
                     at( i1, j1, P1.x, P1.y, P1.z );
                     at( i2, j2, P2.x, P2.y, P2.z );
-                    for ( int qq = 0; qq < 7; ++qq ) {
-                        double b0, b1, b2, wq;
-                        if ( qq == 0 ) { b0 = b1 = b2 = 1.0 / 3; wq = 9.0 / 40; }
-                        else {
-                            const double a = qq <= 3 ? ta1 : ta2;
-                            wq = qq <= 3 ? tw1 : tw2;
-                            const int jj = ( qq - 1 ) % 3;
-                            b0 = jj == 0 ? 1 - 2 * a : a; b1 = jj == 1 ? 1 - 2 * a : a; b2 = jj == 2 ? 1 - 2 * a : a;
-                        }
-                        const double X = p0.x + ( b0 * P0.x + b1 * P1.x + b2 * P2.x ), Y = p0.y + ( b0 * P0.y + b1 * P1.y + b2 * P2.y );
-                        const double Z = p0.z + ( b0 * P0.z + b1 * P1.z + b2 * P2.z );
-                        if ( facets ) S.part += wq * sar * rho_at( dn, X, Y, Z );
-                        S.mass += wq * sar * flux_dot( dn, X, Y, Z, u.x, u.y, u.z );
-                    }
+                    double part = 0, flux = 0;
+                    gauss_rule7( dn, p0, P0, P1, P2, sar, u, part, flux );
+                    if ( facets ) S.part += part;
+                    S.mass += flux;
                 }
         return S;
     }
