@@ -120,3 +120,46 @@ def test_sheared_pixels_are_refused_and_a_turned_grid_crosses_in_its_frame_only(
     im = Image( np.ones( ( 3, 3 ) ), frame = _turn( 0.5 ) )
     with pytest.raises( ValueError ):
         im.native_args( device_tensor = lambda a: a )
+
+
+def test_3d_duplicates_merge_on_the_host( monkeypatch ):
+    """the host side of a 3D cloud's exact duplicates, the device solve replaced by a stand-in that records what it is given:
+    the call sees the cloud without its duplicates, each representative carrying its duplicates' targets; the duplicates come
+    back `k gap` under their representative, with empty cells, their seed as barycentre and their representative as cluster"""
+    import torch
+    from sdot import SumOfDiracs, _native
+    from sdot.SdotPlanNd import _STATS, _solve_3d_merged, duplicate_gap, exact_duplicates
+    rng = np.random.default_rng( 5 )
+    base = rng.uniform( 0, 1, ( 50, 3 ) )
+    pos = np.concatenate( [ base, base[ [ 7, 3, 7 ] ] ] )              # 7 twice ( k = 1, 2 ), 3 once
+    nu = rng.uniform( 1, 2, len( pos ) )
+    seen = {}
+
+    def fake_solve_3d( positions, nu_r, w0, opts, box, density ):
+        seen.update( positions = positions.numpy().copy(), nu = np.asarray( nu_r ).copy() )
+        n = len( nu_r )
+        return dict( weights = torch.arange( n, dtype = torch.float64 ), cell_masses = torch.full( ( n, ), 2.0, dtype = torch.float64 ),
+                     barycenters = torch.zeros( ( n, 3 ), dtype = torch.float64 ), cost = 1.5,
+                     stats = np.zeros( len( _STATS ) ), history = None )
+
+    monkeypatch.setattr( _native, "on_device", lambda a, dtype = None: torch.as_tensor( np.asarray( a ), dtype = dtype or torch.float64 ) )
+    monkeypatch.setattr( _native, "solve_3d", fake_solve_3d )
+    dup, rep = exact_duplicates( pos )
+    assert dup.tolist() == [ 50, 51, 52 ] and rep.tolist() == [ 7, 3, 7 ]
+    res = _solve_3d_merged( SumOfDiracs( pos, nu ), dup, rep, None, None, [ 0, 0, 0, 1, 1, 1 ], None )
+    assert np.array_equal( seen[ "positions" ], base )
+    want = nu[ :50 ].copy()
+    want[ 7 ] += nu[ 50 ] + nu[ 52 ]
+    want[ 3 ] += nu[ 51 ]
+    assert np.array_equal( seen[ "nu" ], want )
+    w = res[ "weights" ].numpy()
+    gap = duplicate_gap( pos )
+    assert np.array_equal( w[ :50 ], np.arange( 50.0 ) )
+    assert w[ 50 ] == 7 - gap and w[ 51 ] == 3 - gap and w[ 52 ] == 7 - 2 * gap
+    m = res[ "cell_masses" ].numpy()
+    assert ( m[ :50 ] == 2 ).all() and ( m[ 50: ] == 0 ).all()
+    assert np.array_equal( res[ "barycenters" ].numpy()[ 50: ], pos[ 50: ] )
+    assert np.array_equal( res[ "clusters" ].numpy(), np.concatenate( [ np.arange( 50 ), [ 7, 3, 7 ] ] ) )
+    st = res[ "stats" ]
+    assert st[ _STATS.index( "nb_duplicates" ) ] == 3 and st[ _STATS.index( "nb_clusters" ) ] == 2
+    assert st[ _STATS.index( "nb_aggregated" ) ] == 5 and res[ "cost" ] == 1.5
