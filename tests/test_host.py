@@ -149,7 +149,7 @@ def test_3d_duplicates_merge_on_the_host( monkeypatch ):
     res = _solve_3d_merged( SumOfDiracs( pos, nu ), dup, rep, None, None, [ 0, 0, 0, 1, 1, 1 ], None )
     assert np.array_equal( seen[ "positions" ], base )
     want = nu[ :50 ].copy()
-    want[ 7 ] += nu[ 50 ] + nu[ 52 ]
+    want[ 7 ] = ( want[ 7 ] + nu[ 50 ] ) + nu[ 52 ]                    # ( in the duplicates' order )
     want[ 3 ] += nu[ 51 ]
     assert np.array_equal( seen[ "nu" ], want )
     w = res[ "weights" ].numpy()
